@@ -1,0 +1,238 @@
+/*
+ * rzk.h — C ABI of the MI355X (gfx950) engine for Renzoku's data-parallel hot
+ * path: BVH traversal (closest hit, any hit, segment visibility), the clustered
+ * VPL gather, the LightSlice reduced matrix / clustering / per-slice guiding
+ * distributions.
+ *
+ * Plain C types only.  Every call returns an RZK_* status; on failure
+ * rzk_ctx_last_error() holds the message.  Nothing throws across the ABI.
+ * Host pointers are owned by the caller; device buffers by the engine (the
+ * *_dev entry points take device pointers owned by the caller).  A context
+ * owns one HIP stream and must be used by one thread at a time; distinct
+ * contexts may run concurrently.  There is no CPU fallback: every compute
+ * entry point runs gfx950 kernels and fails with RZK_E_HIP without a GPU.
+ *
+ * Record layouts (fp32, little endian):
+ *   ray      : 6 floats  {o[3], d[3]}      d as Ray's constructor leaves it (ray.h:17-36)
+ *   hit      : 4 x 32 bit {t, beta, gamma, tri (int32, original index, -1 miss)}
+ *   receiver : 21 floats {p, n_geometric, n_shading, wo, uvn.u, uvn.v, uvn.n}
+ *              + int32 material (-1 none) + int32 light (-1 none)   (local_geometry.h:11-30)
+ *   vpl      : 21 floats {p, n, uvn.u, uvn.v, uvn.n, wi, power} + int32 material
+ *              (-1: the VPL lies on a light, its BRDF is white; brdf_point_light.h:235-240)
+ *   material : 12 floats {type (0 Lambert, 1 Phong, 2 Ward), kd[3], ks[3],
+ *              p0 (Phong exponent | Ward ax), p1 (Ward ay), 0, 0, 0}
+ *   camera   : 19 floats {eye[3], u[3], v[3], n[3], cop[2], scale[2], focal, width, height}
+ *              (camera.cc:52-65, sensor.h:176-182)
+ */
+#ifndef RZK_H
+#define RZK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RZK_OK         0
+#define RZK_E_INVALID  1
+#define RZK_E_OOM      2
+#define RZK_E_HIP      3
+#define RZK_E_STATE    4
+
+#define RZK_RAY_FLOATS      6
+#define RZK_RECV_FLOATS     21
+#define RZK_VPL_FLOATS      21
+#define RZK_MAT_FLOATS      12
+#define RZK_CAM_FLOATS      19
+
+/* clamp modes: VplRadianceClamp (vpl.cc:214-230) */
+#define RZK_CLAMP_NONE          0
+#define RZK_CLAMP_FORM_FACTOR   1
+#define RZK_CLAMP_REFLECTIVITY  2
+#define RZK_CLAMP_RADIANCE      3
+
+/* surface flags (surface.h:109-110) */
+#define RZK_SURF_FLIPPED    1
+#define RZK_SURF_TWO_SIDED  2
+
+typedef struct rzk_ctx rzk_ctx;
+typedef struct rzk_scene rzk_scene;
+typedef struct rzk_vpls rzk_vpls;
+typedef struct rzk_lightslice rzk_lightslice;
+
+typedef struct rzk_hit { float t, beta, gamma; int32_t tri; } rzk_hit;
+
+typedef struct rzk_scene_desc {
+    int32_t num_triangles;
+    const float *positions;        /* [num_triangles][9] p0,p1,p2 in surface order (aggregate_bvh.cc:60-75) */
+    const float *tangents;         /* [num_triangles][3] Triangle::u / Quad shared tangent (quad.cc:22-24) */
+    const int32_t *tri_surface;    /* [num_triangles] */
+    int32_t num_surfaces;
+    const int32_t *surf_material;  /* [num_surfaces] material index or -1 */
+    const int32_t *surf_light;     /* [num_surfaces] light index or -1 */
+    const int32_t *surf_flags;     /* [num_surfaces] RZK_SURF_* */
+    int32_t num_materials;
+    const float *materials;        /* [num_materials][12] */
+    int32_t num_lights;
+    const float *light_radiance;   /* [num_lights][3] */
+} rzk_scene_desc;
+
+typedef struct rzk_scene_info {
+    int32_t num_triangles;
+    int32_t num_interior_nodes;
+    int32_t depth;                 /* nodes on the deepest root-to-leaf path */
+    float box[6];                  /* scene box, origin included (boundingbox.h:100-103) */
+    int64_t device_bytes;          /* nodes + triangles + per-triangle shading data */
+} rzk_scene_info;
+
+/* ---- context ---------------------------------------------------------- */
+int rzk_ctx_create(int device, rzk_ctx **out);
+int rzk_ctx_destroy(rzk_ctx *ctx);
+const char *rzk_ctx_last_error(const rzk_ctx *ctx);
+void *rzk_ctx_stream(rzk_ctx *ctx);                  /* the hipStream_t every launch uses */
+int rzk_ctx_synchronize(rzk_ctx *ctx);
+int rzk_device_alloc(rzk_ctx *ctx, int64_t bytes, void **out);
+int rzk_device_free(rzk_ctx *ctx, void *ptr);
+int rzk_copy_h2d(rzk_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int rzk_copy_d2h(rzk_ctx *ctx, void *dst, const void *src, int64_t bytes);
+
+/* ---- Aggregate (aggregate.h:15-91, aggregate_bvh.h:20-47) -------------- */
+/* AggregateBvh::AggregateBvh + build (aggregate_bvh.cc:16-240): Morton LBVH, one
+ * triangle per leaf, identical topology to the reference; uploaded as a
+ * compacted BVH2 (64 B interior nodes, 48 B triangle records). */
+int rzk_scene_create(rzk_ctx *ctx, const rzk_scene_desc *desc, rzk_scene **out);
+int rzk_scene_destroy(rzk_scene *scene);
+int rzk_scene_get_info(const rzk_scene *scene, rzk_scene_info *out);
+/* the reference's BFS node array (aggregate_bvh.cc:118-207) for parity checks */
+int rzk_scene_export_bvh(const rzk_scene *scene, int32_t *sorted_tri, uint32_t *sorted_morton,
+                         int32_t *node_child, int32_t *node_range, float *node_box,
+                         int32_t *num_nodes);
+
+/* Aggregate::hit(const Ray&, tmin, tmax, time, HitRecord&) — closest hit,
+ * aggregate_bvh.cc:402-448.  Batched, host buffers. */
+int rzk_trace_closest(rzk_ctx *ctx, const rzk_scene *scene, const float *rays, int64_t n,
+                      float tmin, float tmax, rzk_hit *out);
+/* Aggregate::hit(const Ray&, tmin, tmax, time) — any hit, aggregate_bvh.cc:452-478 */
+int rzk_trace_any(rzk_ctx *ctx, const rzk_scene *scene, const float *rays, int64_t n,
+                  float tmin, float tmax, uint8_t *out_hit);
+/* Aggregate::hit(const Vec3 &p, const Vec3 &q, tmin, time) — segment
+ * visibility, true = blocked (aggregate.cc:84-90) */
+int rzk_trace_segments(rzk_ctx *ctx, const rzk_scene *scene, const float *p, const float *q,
+                       int64_t n, float tmin, uint8_t *out_blocked);
+/* device-resident forms: pointers are device memory, launches are queued on
+ * the context stream and the call returns without synchronising.  stats
+ * (optional, device int32[2]) accumulates {interior nodes visited, triangles tested}. */
+int rzk_trace_closest_dev(rzk_ctx *ctx, const rzk_scene *scene, const float *d_rays, int64_t n,
+                          float tmin, float tmax, rzk_hit *d_out, int32_t *d_stats);
+int rzk_trace_any_dev(rzk_ctx *ctx, const rzk_scene *scene, const float *d_rays, int64_t n,
+                      float tmin, float tmax, uint8_t *d_out, int32_t *d_stats);
+int rzk_trace_segments_dev(rzk_ctx *ctx, const rzk_scene *scene, const float *d_p,
+                           const float *d_q, int64_t n, float tmin, uint8_t *d_out,
+                           int32_t *d_stats);
+/* per-ray traversal counts {interior nodes visited, triangles tested}, host
+ * buffers; kind 0 = closest, 1 = any, for rays; use rzk_trace_segment_stats for p,q */
+int rzk_trace_stats(rzk_ctx *ctx, const rzk_scene *scene, int kind, const float *rays,
+                    int64_t n, float tmin, float tmax, int32_t *out_stats);
+
+/* Triangle::fill_hit_record + Surface::fill_hit_record + Receiver(r, rec)
+ * (triangle.cc:277-296, surface.cc:62-96, local_geometry.h:21-30) */
+int rzk_fill_receivers(rzk_ctx *ctx, const rzk_scene *scene, const float *rays,
+                       const rzk_hit *hits, int64_t n, float *recv, int32_t *recv_mat,
+                       int32_t *recv_light);
+/* Camera::shoot_ray(i, j) for pixel-centre rays (camera.cc:52-65); px = column
+ * j, py = row i.  Writes rays, closest hits and receivers (each output optional). */
+int rzk_camera_primary(rzk_ctx *ctx, const rzk_scene *scene, const float *camera, int64_t n,
+                       const int32_t *px, const int32_t *py, float *rays, rzk_hit *hits,
+                       float *recv, int32_t *recv_mat, int32_t *recv_light);
+
+/* ---- VPL set (brdf_point_light.h) ------------------------------------- */
+int rzk_vpls_create(rzk_ctx *ctx, const float *vpl, const int32_t *vpl_mat, int64_t n,
+                    rzk_vpls **out);
+int rzk_vpls_destroy(rzk_vpls *vpls);
+
+/* ---- gather (vpl.cc:661-688 gather_point, summed over the VPL set as
+ * lightslice.cc:1325-1339 does over cluster representatives of weight 1,
+ * then / num_emitted as vpl.cc:488-489) ----------------------------------- */
+int rzk_gather(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls, const float *recv,
+               const int32_t *recv_mat, int64_t npts, int clamp_mode, float clamp, float tmin,
+               float n_emitted, float *out_rgb);
+/* visibility of every (receiver, VPL) pair as a bit matrix: word (k, i/64) bit
+ * i%64 set when the segment receiver i -> VPL k is blocked (vpl.cc:672). */
+int rzk_visibility(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls, const float *recv,
+                   const int32_t *recv_mat, int64_t npts, float tmin, uint64_t *out_bits);
+/* The plain VPL render of a whole image, everything device resident: primary
+ * closest hits at pixel centres, then the full gather.  out_rgb: host
+ * [height][width][3] or NULL (keep on device). */
+int rzk_render_vpl(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls,
+                   const float *camera, int clamp_mode, float clamp, float tmin,
+                   float n_emitted, float *out_rgb);
+
+/* ---- LightTransportMatrix / ClusterBuilder (reduced matrix and global
+ * clustering) ------------------------------------------------------------- */
+/* LightSlice reduced matrix, lightslice.cc:721-748: R(r,k) = vis * incident
+ * radiance (vpl.cc:411-440), V(r,k) = vis.  R column-major Rgb [(k*rows+r)*3],
+ * V [k*rows+r]; rows whose receiver has no material stay zero. */
+int rzk_reduced_matrix(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls,
+                       const float *recv, const int32_t *recv_mat, int32_t rows, int clamp_mode,
+                       float clamp, float tmin, float *out_R, float *out_V);
+/* generate_alphas_for_clustering, light_transport_matrix.cc:72-106 (sequential
+ * summation orders kept, bit-exact); mags = cluster_builder.cc:14-19 magnitude
+ * of each flattened column (optional). */
+int rzk_alphas(rzk_ctx *ctx, int32_t rows, int64_t cols, const float *R, float *out_alphas,
+               float *out_mags);
+/* assignment of every column to the nearest weighted centre column,
+ * cluster_builder.cc:158-225, strict < with the first minimum kept. */
+int rzk_cluster_assign(rzk_ctx *ctx, int32_t rows, int64_t cols, const float *R, int32_t ncl,
+                       const int32_t *centers, const float *weights, int32_t *out_assign);
+
+/* ---- LightSlice (lightslice.h, lightslice.cc:589-1345) ------------------ */
+typedef struct rzk_lightslice_params {
+    int32_t width, height;          /* image */
+    int32_t tile;                   /* slice = tile x tile pixels (32) */
+    int32_t num_clusters;           /* ClusterBuilder centres (300) */
+    int32_t num_neighbor_slices;    /* rows used for the per-slice cluster pdf (16) */
+    int32_t map_resolution;         /* luminance map resolution (32, lightslice.cc:57) */
+    float map_epsilon;              /* luminance_map_epsilon (1e-3, lightslice.cc:74) */
+    int32_t clamp_mode;
+    float clamp;                    /* gather clamp (clamp_threshold) */
+    float incident_clamp;           /* reduced matrix clamp (incident_radiance_clamp_threshold) */
+    float tmin;
+    float n_emitted;                /* num_emitted_photons */
+    uint32_t seed;                  /* MT19937 seed for centre / representative sampling */
+    int32_t map_gate;               /* 1: build maps only where sampling_records[s].is_valid
+                                       would be true (lightslice.cc:2070); see DESIGN */
+} rzk_lightslice_params;
+
+typedef struct rzk_lightslice_info {
+    int32_t num_slices;
+    int32_t num_centers;            /* sampled centres (clusters before removal) */
+    int32_t num_clusters;           /* clusters kept (with a representative) */
+    int32_t draws;                  /* centre draws taken */
+    int64_t reduced_pairs;          /* rows x VPLs */
+    int64_t gather_pairs;           /* pixels with a material x clusters */
+} rzk_lightslice_info;
+
+int rzk_lightslice_create(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls,
+                          const rzk_lightslice_params *params, rzk_lightslice **out);
+int rzk_lightslice_destroy(rzk_lightslice *ls);
+/* LightSlice::initialize with pixel-tile slices: primary hits for every pixel,
+ * reduced matrix on the tile-centre rows, alphas, centre sampling, assignment,
+ * representatives, per-slice cluster pdf + luminance map. */
+int rzk_lightslice_initialize(rzk_lightslice *ls, const float *camera);
+/* LightSlice::gather_clusters for every pixel (lightslice.cc:1315-1345) / n_emitted.
+ * out_rgb host [height][width][3] or NULL to keep the image on the device. */
+int rzk_lightslice_gather(rzk_lightslice *ls, float *out_rgb);
+int rzk_lightslice_get_info(const rzk_lightslice *ls, rzk_lightslice_info *out);
+/* copy an intermediate array to host: name is one of
+ *  "R" (rows*V*3 f32), "V" (rows*V f32), "alphas" (V f32), "mags" (V f32),
+ *  "centers" (i32), "center_w" (f32), "assign" (V i32), "reps" (i32 per kept cluster),
+ *  "rep_w" (f32), "rows_recv" (rows*21 f32), "rows_mat" (rows i32),
+ *  "slice_pdf" (slices*clusters f32), "slice_upper" (slices*clusters i32),
+ *  "slice_wi" (slices*clusters*3 f32), "maps" (slices*res*res f32), "image" (W*H*3 f32)
+ * bytes must equal the array size. */
+int rzk_lightslice_get(const rzk_lightslice *ls, const char *name, void *dst, int64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,13 @@
+"""renzoku_amd — MI355X-native engine for Renzoku's data-parallel hot path.
+
+The compute lives in librzk.so (gfx950 HIP kernels behind the C ABI of
+include/rzk.h).  This package holds its ctypes binding (`_lib`), the seeded
+synthetic scenes the workloads are defined on (`scenes`) and the VPL set
+generator that feeds the gather (`vplgen`).
+"""
+from . import _lib, scenes, vplgen  # noqa: F401
+from ._lib import (Ctx, LightSlice, RzkError, Scene, Vpls, alphas, cluster_assign, gather,  # noqa: F401
+                   reduced_matrix, render_vpl, visibility)
+
+__all__ = ["Ctx", "Scene", "Vpls", "LightSlice", "RzkError", "gather", "render_vpl", "visibility",
+           "reduced_matrix", "alphas", "cluster_assign", "scenes", "vplgen"]

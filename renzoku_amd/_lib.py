@@ -1,0 +1,394 @@
+"""ctypes binding of librzk.so (the C ABI in include/rzk.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when librzk.so has not been built, and every compute call raises RzkError when
+the GPU path fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librzk.so")
+
+RZK_OK = 0
+RAY_FLOATS, RECV_FLOATS, VPL_FLOATS, MAT_FLOATS, CAM_FLOATS = 6, 21, 21, 12, 19
+CLAMP_NONE, CLAMP_FORM_FACTOR, CLAMP_REFLECTIVITY, CLAMP_RADIANCE = 0, 1, 2, 3
+SURF_FLIPPED, SURF_TWO_SIDED = 1, 2
+
+# every symbol include/rzk.h declares
+EXPORTS = [
+    "rzk_ctx_create", "rzk_ctx_destroy", "rzk_ctx_last_error", "rzk_ctx_stream", "rzk_ctx_synchronize",
+    "rzk_device_alloc", "rzk_device_free", "rzk_copy_h2d", "rzk_copy_d2h",
+    "rzk_scene_create", "rzk_scene_destroy", "rzk_scene_get_info", "rzk_scene_export_bvh",
+    "rzk_trace_closest", "rzk_trace_any", "rzk_trace_segments",
+    "rzk_trace_closest_dev", "rzk_trace_any_dev", "rzk_trace_segments_dev", "rzk_trace_stats",
+    "rzk_fill_receivers", "rzk_camera_primary",
+    "rzk_vpls_create", "rzk_vpls_destroy",
+    "rzk_gather", "rzk_visibility", "rzk_render_vpl",
+    "rzk_reduced_matrix", "rzk_alphas", "rzk_cluster_assign",
+    "rzk_lightslice_create", "rzk_lightslice_destroy", "rzk_lightslice_initialize",
+    "rzk_lightslice_gather", "rzk_lightslice_get_info", "rzk_lightslice_get",
+]
+
+
+class RzkError(RuntimeError):
+    pass
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("num_triangles", C.c_int32), ("positions", C.c_void_p), ("tangents", C.c_void_p),
+        ("tri_surface", C.c_void_p), ("num_surfaces", C.c_int32), ("surf_material", C.c_void_p),
+        ("surf_light", C.c_void_p), ("surf_flags", C.c_void_p), ("num_materials", C.c_int32),
+        ("materials", C.c_void_p), ("num_lights", C.c_int32), ("light_radiance", C.c_void_p),
+    ]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("num_triangles", C.c_int32), ("num_interior_nodes", C.c_int32), ("depth", C.c_int32),
+        ("box", C.c_float * 6), ("device_bytes", C.c_int64),
+    ]
+
+
+class LightSliceParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("tile", C.c_int32), ("num_clusters", C.c_int32),
+        ("num_neighbor_slices", C.c_int32), ("map_resolution", C.c_int32), ("map_epsilon", C.c_float),
+        ("clamp_mode", C.c_int32), ("clamp", C.c_float), ("incident_clamp", C.c_float), ("tmin", C.c_float),
+        ("n_emitted", C.c_float), ("seed", C.c_uint32), ("map_gate", C.c_int32),
+    ]
+
+
+class LightSliceInfo(C.Structure):
+    _fields_ = [
+        ("num_slices", C.c_int32), ("num_centers", C.c_int32), ("num_clusters", C.c_int32),
+        ("draws", C.c_int32), ("reduced_pairs", C.c_int64), ("gather_pairs", C.c_int64),
+    ]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RzkError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(LIB_PATH)
+    P, I32, I64, F = C.c_void_p, C.c_int32, C.c_int64, C.c_float
+    sig = {
+        "rzk_ctx_create": [C.c_int, C.POINTER(P)],
+        "rzk_ctx_destroy": [P],
+        "rzk_ctx_last_error": [P],
+        "rzk_ctx_stream": [P],
+        "rzk_ctx_synchronize": [P],
+        "rzk_device_alloc": [P, I64, C.POINTER(P)],
+        "rzk_device_free": [P, P],
+        "rzk_copy_h2d": [P, P, P, I64],
+        "rzk_copy_d2h": [P, P, P, I64],
+        "rzk_scene_create": [P, C.POINTER(SceneDesc), C.POINTER(P)],
+        "rzk_scene_destroy": [P],
+        "rzk_scene_get_info": [P, C.POINTER(SceneInfo)],
+        "rzk_scene_export_bvh": [P, P, P, P, P, P, C.POINTER(I32)],
+        "rzk_trace_closest": [P, P, P, I64, F, F, P],
+        "rzk_trace_any": [P, P, P, I64, F, F, P],
+        "rzk_trace_segments": [P, P, P, P, I64, F, P],
+        "rzk_trace_closest_dev": [P, P, P, I64, F, F, P, P],
+        "rzk_trace_any_dev": [P, P, P, I64, F, F, P, P],
+        "rzk_trace_segments_dev": [P, P, P, P, I64, F, P, P],
+        "rzk_trace_stats": [P, P, C.c_int, P, I64, F, F, P],
+        "rzk_fill_receivers": [P, P, P, P, I64, P, P, P],
+        "rzk_camera_primary": [P, P, P, I64, P, P, P, P, P, P, P],
+        "rzk_vpls_create": [P, P, P, I64, C.POINTER(P)],
+        "rzk_vpls_destroy": [P],
+        "rzk_gather": [P, P, P, P, P, I64, C.c_int, F, F, F, P],
+        "rzk_visibility": [P, P, P, P, P, I64, F, P],
+        "rzk_render_vpl": [P, P, P, P, C.c_int, F, F, F, P],
+        "rzk_reduced_matrix": [P, P, P, P, P, I32, C.c_int, F, F, P, P],
+        "rzk_alphas": [P, I32, I64, P, P, P],
+        "rzk_cluster_assign": [P, I32, I64, P, I32, P, P, P],
+        "rzk_lightslice_create": [P, P, P, C.POINTER(LightSliceParams), C.POINTER(P)],
+        "rzk_lightslice_destroy": [P],
+        "rzk_lightslice_initialize": [P, P],
+        "rzk_lightslice_gather": [P, P],
+        "rzk_lightslice_get_info": [P, C.POINTER(LightSliceInfo)],
+        "rzk_lightslice_get": [P, C.c_char_p, P, I64],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.rzk_ctx_last_error.restype = C.c_char_p
+    lib.rzk_ctx_stream.restype = C.c_void_p
+    _lib = lib
+    return lib
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to librzk must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Ctx:
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.rzk_ctx_create(device, C.byref(h))
+        if rc != RZK_OK:
+            raise RzkError(f"rzk_ctx_create({device}) failed with status {rc} (no usable GPU?)")
+        self.h = h
+
+    def check(self, rc: int, what: str):
+        if rc != RZK_OK:
+            msg = self.lib.rzk_ctx_last_error(self.h)
+            raise RzkError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    @property
+    def stream(self) -> int:
+        return self.lib.rzk_ctx_stream(self.h)
+
+    def synchronize(self):
+        self.check(self.lib.rzk_ctx_synchronize(self.h), "rzk_ctx_synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rzk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """The Aggregate: scene + Morton LBVH on the device."""
+
+    def __init__(self, ctx: Ctx, sc):
+        self.ctx, self.lib = ctx, ctx.lib
+        self._keep = [f32(sc.positions), f32(sc.tangents), i32(sc.tri_surface), i32(sc.surf_material),
+                      i32(sc.surf_light), i32(sc.surf_flags), f32(sc.materials), f32(sc.light_radiance)]
+        pos, tan, ts, sm, sl, sf, mat, lr = self._keep
+        d = SceneDesc(len(ts), ptr(pos), ptr(tan), ptr(ts), len(sm), ptr(sm), ptr(sl), ptr(sf),
+                      mat.shape[0], ptr(mat), lr.shape[0], ptr(lr))
+        h = C.c_void_p()
+        ctx.check(self.lib.rzk_scene_create(ctx.h, C.byref(d), C.byref(h)), "rzk_scene_create")
+        self.h = h
+        self.ntri = len(ts)
+
+    def info(self) -> SceneInfo:
+        inf = SceneInfo()
+        self.lib.rzk_scene_get_info(self.h, C.byref(inf))
+        return inf
+
+    def export_bvh(self):
+        n = self.ntri
+        nn = max(2 * n - 1, 0)
+        st, sm = np.zeros(n, np.int32), np.zeros(n, np.uint32)
+        ch, rg, bx = np.zeros(2 * nn, np.int32), np.zeros(2 * nn, np.int32), np.zeros(6 * nn, np.float32)
+        cnt = C.c_int32()
+        self.lib.rzk_scene_export_bvh(self.h, ptr(st), ptr(sm), ptr(ch), ptr(rg), ptr(bx), C.byref(cnt))
+        return st, sm, ch.reshape(-1, 2), rg.reshape(-1, 2), bx.reshape(-1, 6)
+
+    def trace_closest(self, rays, tmin=1e-3, tmax=1e6):
+        rays = f32(rays).reshape(-1, 6)
+        out = np.zeros((len(rays), 4), np.float32)
+        self.ctx.check(self.lib.rzk_trace_closest(self.ctx.h, self.h, ptr(rays), len(rays), tmin, tmax, ptr(out)),
+                       "rzk_trace_closest")
+        return out[:, 0].copy(), out[:, 1].copy(), out[:, 2].copy(), out[:, 3].view(np.int32).copy()
+
+    def trace_any(self, rays, tmin=1e-3, tmax=1e6):
+        rays = f32(rays).reshape(-1, 6)
+        out = np.zeros(len(rays), np.uint8)
+        self.ctx.check(self.lib.rzk_trace_any(self.ctx.h, self.h, ptr(rays), len(rays), tmin, tmax, ptr(out)),
+                       "rzk_trace_any")
+        return out
+
+    def trace_segments(self, p, q, tmin=1e-3):
+        p, q = f32(p).reshape(-1, 3), f32(q).reshape(-1, 3)
+        out = np.zeros(len(p), np.uint8)
+        self.ctx.check(self.lib.rzk_trace_segments(self.ctx.h, self.h, ptr(p), ptr(q), len(p), tmin, ptr(out)),
+                       "rzk_trace_segments")
+        return out
+
+    def trace_stats(self, rays, kind=0, tmin=1e-3, tmax=1e6):
+        rays = f32(rays).reshape(-1, 6)
+        out = np.zeros((len(rays), 2), np.int32)
+        self.ctx.check(self.lib.rzk_trace_stats(self.ctx.h, self.h, kind, ptr(rays), len(rays), tmin, tmax, ptr(out)),
+                       "rzk_trace_stats")
+        return out
+
+    def fill_receivers(self, rays, t, beta, gamma, tri):
+        rays = f32(rays).reshape(-1, 6)
+        n = len(rays)
+        hits = np.zeros((n, 4), np.float32)
+        hits[:, 0], hits[:, 1], hits[:, 2] = t, beta, gamma
+        hits[:, 3] = np.asarray(tri, np.int32).view(np.float32)
+        recv = np.zeros((n, RECV_FLOATS), np.float32)
+        m, l = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self.ctx.check(self.lib.rzk_fill_receivers(self.ctx.h, self.h, ptr(rays), ptr(hits), n, ptr(recv), ptr(m),
+                                                   ptr(l)), "rzk_fill_receivers")
+        return recv, m, l
+
+    def camera_primary(self, cam, px, py):
+        cam, px, py = f32(cam), i32(px), i32(py)
+        n = len(px)
+        rays = np.zeros((n, 6), np.float32)
+        hits = np.zeros((n, 4), np.float32)
+        recv = np.zeros((n, RECV_FLOATS), np.float32)
+        m, l = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self.ctx.check(self.lib.rzk_camera_primary(self.ctx.h, self.h, ptr(cam), n, ptr(px), ptr(py), ptr(rays),
+                                                   ptr(hits), ptr(recv), ptr(m), ptr(l)), "rzk_camera_primary")
+        return rays, hits, recv, m, l
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rzk_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Vpls:
+    def __init__(self, ctx: Ctx, vpl, vmat):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.vpl, self.vmat = f32(vpl).reshape(-1, VPL_FLOATS), i32(vmat)
+        h = C.c_void_p()
+        ctx.check(self.lib.rzk_vpls_create(ctx.h, ptr(self.vpl), ptr(self.vmat), len(self.vmat), C.byref(h)),
+                  "rzk_vpls_create")
+        self.h = h
+        self.n = len(self.vmat)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rzk_vpls_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gather(ctx, scene, vpls, recv, rmat, clamp_mode, clamp, tmin, n_emitted):
+    recv, rmat = f32(recv).reshape(-1, RECV_FLOATS), i32(rmat)
+    out = np.zeros((len(rmat), 3), np.float32)
+    ctx.check(ctx.lib.rzk_gather(ctx.h, scene.h, vpls.h, ptr(recv), ptr(rmat), len(rmat), clamp_mode, clamp, tmin,
+                                 n_emitted, ptr(out)), "rzk_gather")
+    return out
+
+
+def visibility(ctx, scene, vpls, recv, rmat, tmin):
+    recv, rmat = f32(recv).reshape(-1, RECV_FLOATS), i32(rmat)
+    ng = (len(rmat) + 63) // 64
+    out = np.zeros(ng * vpls.n, np.uint64)
+    ctx.check(ctx.lib.rzk_visibility(ctx.h, scene.h, vpls.h, ptr(recv), ptr(rmat), len(rmat), tmin, ptr(out)),
+              "rzk_visibility")
+    return out.reshape(vpls.n, ng)
+
+
+def render_vpl(ctx, scene, vpls, cam, clamp_mode, clamp, tmin, n_emitted):
+    cam = f32(cam)
+    w, h = int(cam[17]), int(cam[18])
+    out = np.zeros((h, w, 3), np.float32)
+    ctx.check(ctx.lib.rzk_render_vpl(ctx.h, scene.h, vpls.h, ptr(cam), clamp_mode, clamp, tmin, n_emitted, ptr(out)),
+              "rzk_render_vpl")
+    return out
+
+
+def reduced_matrix(ctx, scene, vpls, recv, rmat, clamp_mode, clamp, tmin):
+    recv, rmat = f32(recv).reshape(-1, RECV_FLOATS), i32(rmat)
+    rows = len(rmat)
+    R = np.zeros((vpls.n, rows, 3), np.float32)
+    V = np.zeros((vpls.n, rows), np.float32)
+    ctx.check(ctx.lib.rzk_reduced_matrix(ctx.h, scene.h, vpls.h, ptr(recv), ptr(rmat), rows, clamp_mode, clamp, tmin,
+                                         ptr(R), ptr(V)), "rzk_reduced_matrix")
+    return R, V
+
+
+def alphas(ctx, R):
+    R = f32(R)
+    cols, rows = R.shape[0], R.shape[1]
+    a, m = np.zeros(cols, np.float32), np.zeros(cols, np.float32)
+    ctx.check(ctx.lib.rzk_alphas(ctx.h, rows, cols, ptr(R), ptr(a), ptr(m)), "rzk_alphas")
+    return a, m
+
+
+def cluster_assign(ctx, R, centers, weights):
+    R, centers, weights = f32(R), i32(centers), f32(weights)
+    cols, rows = R.shape[0], R.shape[1]
+    out = np.zeros(cols, np.int32)
+    ctx.check(ctx.lib.rzk_cluster_assign(ctx.h, rows, cols, ptr(R), len(centers), ptr(centers), ptr(weights),
+                                         ptr(out)), "rzk_cluster_assign")
+    return out
+
+
+class LightSlice:
+    def __init__(self, ctx, scene, vpls, **kw):
+        self.ctx, self.lib = ctx, ctx.lib
+        p = LightSliceParams()
+        defaults = dict(width=512, height=512, tile=32, num_clusters=300, num_neighbor_slices=16,
+                        map_resolution=32, map_epsilon=1e-3, clamp_mode=CLAMP_FORM_FACTOR, clamp=0.01,
+                        incident_clamp=0.01, tmin=1e-3, n_emitted=1.0, seed=2013, map_gate=0)
+        defaults.update(kw)
+        for k, v in defaults.items():
+            setattr(p, k, v)
+        self.params = defaults
+        h = C.c_void_p()
+        ctx.check(self.lib.rzk_lightslice_create(ctx.h, scene.h, vpls.h, C.byref(p), C.byref(h)),
+                  "rzk_lightslice_create")
+        self.h = h
+        self.w, self.hgt = defaults["width"], defaults["height"]
+
+    def initialize(self, cam):
+        cam = f32(cam)
+        self.ctx.check(self.lib.rzk_lightslice_initialize(self.h, ptr(cam)), "rzk_lightslice_initialize")
+
+    def gather(self, to_host=True):
+        out = np.zeros((self.hgt, self.w, 3), np.float32) if to_host else None
+        self.ctx.check(self.lib.rzk_lightslice_gather(self.h, ptr(out)), "rzk_lightslice_gather")
+        return out
+
+    def info(self) -> LightSliceInfo:
+        inf = LightSliceInfo()
+        self.lib.rzk_lightslice_get_info(self.h, C.byref(inf))
+        return inf
+
+    def get(self, name, dtype, count):
+        out = np.zeros(count, dtype)
+        self.ctx.check(self.lib.rzk_lightslice_get(self.h, name.encode(), ptr(out), out.nbytes),
+                       f"rzk_lightslice_get({name})")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rzk_lightslice_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,161 @@
+// bvh_build.cpp — host Morton LBVH with the reference's topology
+// (AggregateBvh::build, aggregate_bvh.cc:48-240; morton.h:28-120), compacted
+// into the device layout of rzk_internal.h.
+//
+// Topology: triangles sorted stably by 30-bit Morton code of their box centre
+// (normalised to the scene box, which always contains the origin), one
+// triangle per leaf, every range split after the last code that shares more
+// than the range's common prefix with its first code (midpoint for equal
+// codes).  The tree is therefore node-for-node the reference's; only the
+// memory order differs (right-first depth-first preorder, so the child the
+// traversal takes first usually sits next to its parent).
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "rzk_internal.h"
+
+using namespace rzk;
+
+static int clz32(uint32_t v) { return v ? __builtin_clz(v) : 32; }
+
+// Last index i in [first, last) with clz(code[first] ^ code[i]) above the
+// range's common prefix; the predicate holds on a prefix of the sorted range.
+static int split_range(const uint32_t* code, int first, int last) {
+    uint32_t fc = code[first], lc = code[last];
+    if (fc == lc) return (first + last) >> 1;
+    int cp = clz32(fc ^ lc);
+    int lo = first, hi = last;
+    while (hi - lo > 1) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (clz32(fc ^ code[mid]) > cp) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+void bvh_build_host(int ntri, const float* pos, BvhHost& out) {
+    out = BvhHost();
+    out.ntri = ntri;
+    if (ntri <= 0) return;
+    std::vector<v3> bmin(ntri), bmax(ntri);
+    v3 smin = mk(0.f, 0.f, 0.f), smax = mk(0.f, 0.f, 0.f);
+    for (int k = 0; k < ntri; ++k) {
+        v3 p0 = ld3(pos + 9 * k), p1 = ld3(pos + 9 * k + 3), p2 = ld3(pos + 9 * k + 6);
+        bmin[k] = vmin(vmin(p0, p1), p2);
+        bmax[k] = vmax(vmax(p0, p1), p2);
+        smin = vmin(smin, bmin[k]);
+        smax = vmax(smax, bmax[k]);
+    }
+    st3(out.box, smin);
+    st3(out.box + 3, smax);
+
+    std::vector<uint64_t> key(ntri);
+    v3 ext = smax - smin;
+    for (int k = 0; k < ntri; ++k) {
+        v3 c = (bmin[k] + bmax[k]) * 0.5f;
+        c = (c - smin) / ext;
+        uint32_t code = morton3d(c.x, c.y, c.z);
+        key[k] = ((uint64_t)code << 32) | (uint32_t)k;   // (code, index): a stable order
+    }
+    std::sort(key.begin(), key.end());
+    out.sorted_tri.resize(ntri);
+    out.sorted_morton.resize(ntri);
+    for (int i = 0; i < ntri; ++i) {
+        out.sorted_tri[i] = (int)(key[i] & 0xffffffffu);
+        out.sorted_morton[i] = (uint32_t)(key[i] >> 32);
+    }
+    const uint32_t* code = out.sorted_morton.data();
+
+    // BFS node array exactly as aggregate_bvh.cc:118-176 lays it out
+    int nn = 2 * ntri - 1;
+    std::vector<int> child(2 * nn, -1), range(2 * nn, 0), level(nn, 0);
+    range[0] = 0; range[1] = ntri - 1;
+    int next = 1, depth = 0;
+    for (int i = 0; i < next; ++i) {
+        int s = range[2 * i], e = range[2 * i + 1];
+        if (s == e) continue;
+        int sp = split_range(code, s, e);
+        int l = next, r = next + 1;
+        next += 2;
+        child[2 * i] = l; child[2 * i + 1] = r;
+        range[2 * l] = s; range[2 * l + 1] = sp;
+        range[2 * r] = sp + 1; range[2 * r + 1] = e;
+        level[l] = level[r] = level[i] + 1;
+        depth = std::max(depth, level[i] + 1);
+    }
+    out.depth = depth + 1;
+
+    // bottom-up boxes (children always have larger BFS indices than parents)
+    std::vector<float> box(6 * nn);
+    for (int i = nn - 1; i >= 0; --i) {
+        v3 a, b;
+        if (child[2 * i] < 0) {
+            int ot = out.sorted_tri[range[2 * i]];
+            a = bmin[ot]; b = bmax[ot];
+        } else {
+            int l = child[2 * i], r = child[2 * i + 1];
+            a = vmin(ld3(&box[6 * l]), ld3(&box[6 * r]));
+            b = vmax(ld3(&box[6 * l + 3]), ld3(&box[6 * r + 3]));
+        }
+        st3(&box[6 * i], a);
+        st3(&box[6 * i + 3], b);
+    }
+    std::copy(box.begin(), box.begin() + 6, out.root_box);
+    out.bfs_child = child;
+    out.bfs_range = range;
+    out.bfs_box = box;
+
+    // triangles in sorted order
+    out.tris.resize(ntri);
+    for (int i = 0; i < ntri; ++i) {
+        int ot = out.sorted_tri[i];
+        v3 p0 = ld3(pos + 9 * ot), p1 = ld3(pos + 9 * ot + 3), p2 = ld3(pos + 9 * ot + 6);
+        v3 e1 = p0 - p1, e2 = p0 - p2;
+        TriGpu t;
+        float idf;
+        std::memcpy(&idf, &ot, 4);
+        t.p0 = make_float4(p0.x, p0.y, p0.z, idf);
+        t.e1 = make_float4(e1.x, e1.y, e1.z, 0.f);
+        t.e2 = make_float4(e2.x, e2.y, e2.z, 0.f);
+        out.tris[i] = t;
+    }
+
+    // compaction: interior nodes in right-first depth-first preorder
+    if (nn == 1) {
+        out.root_ref = ~0;
+        out.num_interior = 0;
+        return;
+    }
+    std::vector<int> newidx(nn, -1);
+    std::vector<int> order;
+    order.reserve(ntri);
+    std::vector<int> stk;
+    stk.push_back(0);
+    while (!stk.empty()) {
+        int i = stk.back();
+        stk.pop_back();
+        if (child[2 * i] < 0) continue;
+        newidx[i] = (int)order.size();
+        order.push_back(i);
+        stk.push_back(child[2 * i]);       // left visited after the right subtree
+        stk.push_back(child[2 * i + 1]);
+    }
+    auto ref = [&](int b) { return child[2 * b] < 0 ? ~range[2 * b] : newidx[b]; };
+    out.num_interior = (int)order.size();
+    out.nodes.resize(order.size());
+    for (size_t j = 0; j < order.size(); ++j) {
+        int i = order[j];
+        int l = child[2 * i], r = child[2 * i + 1];
+        const float* L = &box[6 * l];
+        const float* R = &box[6 * r];
+        BvhNodeGpu nd;
+        nd.a = make_float4(L[0], L[1], L[2], L[3]);
+        nd.b = make_float4(L[4], L[5], R[0], R[1]);
+        nd.c = make_float4(R[2], R[3], R[4], R[5]);
+        nd.d = make_int4(ref(l), ref(r), 0, 0);
+        out.nodes[j] = nd;
+    }
+    out.root_ref = 0;
+}

@@ -1,0 +1,554 @@
+// lightslice.hip — LightSlice with pixel-tile slices (lightslice.h,
+// lightslice.cc:589-1345): whole-image primary hits, the reduced matrix on
+// the slice-representative rows, alphas, centre sampling, assignment,
+// representatives, per-slice cluster pdfs and luminance maps, and the final
+// per-pixel gather over cluster representatives.  Also the plain whole-image
+// VPL render (every VPL a weight-1 cluster).
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "host_sampling.h"
+#include "rzk_device.h"
+
+void launch_gather_fused(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
+                         int64_t npts, const int* d_order, const float* d_vpl, const int* d_vmat,
+                         int64_t nvpl, int clamp_mode, float clamp, float tmin, float n_emitted,
+                         float* d_out, int* d_stats);
+void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_alphas,
+                        float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0);
+void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl, const int* d_centers,
+                        const float* d_weights, const float* d_mags, float* d_K, float* d_Knorm,
+                        int* d_assign);
+
+namespace rzk {
+
+__global__ void k_pick_rows(const float* __restrict__ recv, const int* __restrict__ rmat,
+                            const int* __restrict__ pix, int rows, float* __restrict__ rrecv,
+                            int* __restrict__ rrmat) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    int64_t i = pix[r];
+    for (int k = 0; k < RZK_RECV_FLOATS; ++k) rrecv[RZK_RECV_FLOATS * r + k] = recv[RZK_RECV_FLOATS * i + k];
+    rrmat[r] = rmat[i];
+}
+
+// lightslice.cc:1011-1030 for each (slice s, cluster c): slice_wi = direction
+// to the representative, upper = visible && above the shading normal && non-black
+__global__ void k_slice_info(const float* __restrict__ mat, const float* __restrict__ rrecv,
+                             const int* __restrict__ rrmat, int rows, const float* __restrict__ V,
+                             const float* __restrict__ vpl, const int* __restrict__ vmat,
+                             const int* __restrict__ reps, int nreps, int mode, float thr,
+                             float* __restrict__ slice_wi, int* __restrict__ upper) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)rows * nreps) return;
+    int s = (int)(t / nreps), c = (int)(t - (int64_t)s * nreps);
+    int k = reps[c];
+    const float* rc = rrecv + RZK_RECV_FLOATS * s;
+    const float* vp = vpl + (int64_t)RZK_VPL_FLOATS * k;
+    float vis = V[(int64_t)k * rows + s];
+    v3 inc = incident_radiance(mat, rc, vp, vmat[k], mode, thr);
+    v3 wi = unit(ld3(vp) - ld3(rc));
+    st3(slice_wi + 3 * t, wi);
+    bool nonblack = inc.x != 0.f || inc.y != 0.f || inc.z != 0.f;
+    upper[t] = (vis > 0.0f && dot(ld3(rc + 6), wi) > kZeroEps && nonblack) ? 1 : 0;
+    (void)rrmat;
+}
+
+// lightslice.cc:1114-1179: cluster weight per slice = norm of the
+// representative column over the neighbour rows (matrix_c.cc:74-88 order)
+// x weight, 0 below the horizon
+__global__ void k_slice_pdf_weights(const float* __restrict__ R, int rows, const int* __restrict__ nb,
+                                    int nnb, const int* __restrict__ reps, const float* __restrict__ rep_w,
+                                    int nreps, const int* __restrict__ upper, float* __restrict__ wout) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)rows * nreps) return;
+    int s = (int)(t / nreps), c = (int)(t - (int64_t)s * nreps);
+    const float* col = R + 3 * (int64_t)reps[c] * rows;
+    float mag = 0.f;
+    for (int q = 0; q < nnb; ++q) {
+        const float* e = col + 3 * nb[s * nnb + q];
+        mag += e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+    }
+    float v = sqrtf(mag);
+    v *= rep_w[c];
+    if (!upper[t]) v = 0.0f;
+    wout[t] = v;
+}
+
+// DiscretePdf::set_distribution per slice (pdf.cc:18-52), one lane per slice
+__global__ void k_slice_pdf_norm(int rows, int nreps, const float* __restrict__ w, float* __restrict__ pdf,
+                                 float* __restrict__ cdf) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= rows) return;
+    const float* a = w + (int64_t)s * nreps;
+    float* p = pdf + (int64_t)s * nreps;
+    float* c = cdf + (int64_t)s * nreps;
+    float sum = 0.f;
+    for (int i = 0; i < nreps; ++i) sum += a[i];
+    if (sum == 0.0f) {
+        for (int i = 0; i < nreps; ++i) { p[i] = 0.f; c[i] = 0.f; }
+        return;
+    }
+    for (int i = 0; i < nreps; ++i) p[i] = a[i] / sum;
+    c[0] = p[0];
+    for (int i = 1; i < nreps; ++i) c[i] = c[i - 1] + p[i];
+    c[nreps - 1] = 1.0f;
+}
+
+// Shirley-Chiu concentric map, direction -> unit square (concentric_mapping.h:49-106)
+__device__ __forceinline__ void direction_to_square(v3 wi, float& sx, float& sy) {
+    float r2 = 1.0f - wi.z;
+    float dx = wi.x / sqrtf(2.0f - r2), dy = wi.y / sqrtf(2.0f - r2);
+    float r = sqrtf(dx * dx + dy * dy);
+    float phi = atan2f(dy, dx);
+    if (phi < -kPi / 4) phi += kTwoPi;
+    float a, b;
+    if (phi < kPi / 4) { a = r; b = phi * a / (kPi / 4); }
+    else if (phi < 3 * kPi / 4) { b = r; a = -(phi - kPi / 2) * b / (kPi / 4); }
+    else if (phi < 5 * kPi / 4) { a = -r; b = (phi - kPi) * a / (kPi / 4); }
+    else { b = -r; a = -(phi - 3 * kPi / 2) * b / (kPi / 4); }
+    sx = a * 0.5f + 0.5f;
+    sy = b * 0.5f + 0.5f;
+}
+
+// build_map (lightslice.cc:2058-2147) + DiscretePdf2D::set_distribution
+// (pdf.cc:195-233).  One block per slice: the scatter runs in cluster order in
+// one lane (the map sums must keep that order), rows are normalised in parallel.
+__global__ void k_slice_maps(int res, int nreps, const float* __restrict__ R, int rows,
+                             const float* __restrict__ rrecv, const int* __restrict__ reps,
+                             const float* __restrict__ rep_w, const float* __restrict__ slice_wi,
+                             const int* __restrict__ upper, float eps, int valid_all,
+                             float* __restrict__ maps, float* __restrict__ col_cdf,
+                             float* __restrict__ row_pdf, float* __restrict__ row_cdf,
+                             float* __restrict__ totals) {
+    extern __shared__ float s_map[];
+    float* s_rows = s_map + res * res;
+    int s = blockIdx.x;
+    for (int e = threadIdx.x; e < res * res; e += blockDim.x) s_map[e] = 0.f;
+    __syncthreads();
+    if (threadIdx.x == 0 && valid_all) {
+        const float* rc = rrecv + RZK_RECV_FLOATS * s;
+        Onb b;
+        b.u = ld3(rc + 12); b.v = ld3(rc + 15); b.n = ld3(rc + 18);
+        float sum = 0.f;
+        for (int c = 0; c < nreps; ++c) {
+            int64_t t = (int64_t)s * nreps + c;
+            if (!upper[t]) continue;
+            v3 lwi = world_to_local(b, ld3(slice_wi + 3 * t));
+            if (lwi.z <= 0.0f) continue;
+            const float* e = R + 3 * ((int64_t)reps[c] * rows + s);
+            v3 rad = ld3(e) * rep_w[c];
+            float sx, sy;
+            direction_to_square(lwi, sx, sy);
+            int x = (int)(sx * res), y = (int)(sy * res);
+            if (x >= res) x = res - 1;
+            if (y >= res) y = res - 1;
+            if (x < 0) x = 0;
+            if (y < 0) y = 0;
+            float lum = rgb_value(rad);
+            s_map[y * res + x] += lum;
+            sum += lum;
+        }
+        s_rows[0] = sum;
+    }
+    __syncthreads();
+    if (valid_all) {
+        float add = eps * s_rows[0];
+        __syncthreads();
+        for (int e = threadIdx.x; e < res * res; e += blockDim.x) s_map[e] += add;
+        __syncthreads();
+    }
+    float* mp = maps + (int64_t)s * res * res;
+    float* cc = col_cdf + (int64_t)s * res * res;
+    for (int i = threadIdx.x; i < res; i += blockDim.x) {
+        float rs = 0.f;
+        for (int j = 0; j < res; ++j) rs += s_map[i * res + j];
+        float run = 0.f;
+        for (int j = 0; j < res; ++j) {
+            run += s_map[i * res + j];
+            cc[i * res + j] = run / rs;
+            mp[i * res + j] = s_map[i * res + j];
+        }
+        s_rows[1 + i] = rs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float tot = 0.f;
+        for (int i = 0; i < res; ++i) tot += s_rows[1 + i];
+        totals[s] = tot;
+        float sum = 0.f;
+        for (int i = 0; i < res; ++i) sum += s_rows[1 + i];
+        float* rp = row_pdf + (int64_t)s * res;
+        float* rcd = row_cdf + (int64_t)s * res;
+        if (sum == 0.0f) {
+            for (int i = 0; i < res; ++i) { rp[i] = 0.f; rcd[i] = 0.f; }
+        } else {
+            for (int i = 0; i < res; ++i) rp[i] = s_rows[1 + i] / sum;
+            rcd[0] = rp[0];
+            for (int i = 1; i < res; ++i) rcd[i] = rcd[i - 1] + rp[i];
+            rcd[res - 1] = 1.0f;
+        }
+    }
+}
+
+}  // namespace rzk
+
+using namespace rzk;
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        RZK_HIP(hipMalloc((void**)&p, sizeof(T) * (count ? count : 1)));
+    }
+    ~DBuf() { if (p) (void)hipFree(p); }
+};
+
+struct rzk_lightslice {
+    rzk_ctx* ctx = nullptr;
+    const rzk_scene* scene = nullptr;
+    const rzk_vpls* vpls = nullptr;
+    rzk_lightslice_params prm;
+    int W = 0, H = 0, T = 0, tx = 0, ty = 0, S = 0;
+    int64_t V = 0;
+    bool initialized = false;
+    rzk_lightslice_info info;
+    // whole image
+    DBuf<float> rays, recv, image;
+    DBuf<float4> hits;
+    DBuf<int> mat, light;
+    // rows
+    DBuf<int> row_pix, rows_mat;
+    DBuf<float> rows_recv, R, Vm;
+    // clustering
+    DBuf<float> alphas, mags, norms, u, v, c0, center_w, K, Knorm, rep_w;
+    DBuf<int> centers, assign, reps;
+    // slices
+    DBuf<int> nb, upper;
+    DBuf<float> slice_wi, slice_w, slice_pdf, slice_cdf, maps, col_cdf, row_pdf, row_cdf, totals;
+    std::vector<int> h_centers, h_assign, h_reps;
+    std::vector<float> h_center_w, h_rep_w, h_alphas, h_mags;
+};
+
+static void ls_sync(rzk_ctx* ctx) { RZK_HIP(hipStreamSynchronize(ctx->stream)); }
+
+template <typename F>
+static int ls_guard(rzk_ctx* ctx, F&& f) {
+    try {
+        f();
+        if (ctx) ctx->last_error.clear();
+        return RZK_OK;
+    } catch (const RzkError& e) {
+        if (ctx) ctx->last_error = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->last_error = e.what();
+        return RZK_E_INVALID;
+    }
+}
+
+// rows = pixel-tile slice representatives (the tile centre)
+static void ls_rows(rzk_lightslice* ls, std::vector<int>& pix) {
+    pix.resize(ls->S);
+    for (int y = 0; y < ls->ty; ++y)
+        for (int x = 0; x < ls->tx; ++x) {
+            int px = std::min(x * ls->T + ls->T / 2, ls->W - 1);
+            int py = std::min(y * ls->T + ls->T / 2, ls->H - 1);
+            pix[y * ls->tx + x] = py * ls->W + px;
+        }
+}
+
+// the num_neighbor_slices nearest slices of every slice (tile-centre distance,
+// ties by slice index), the slice itself first
+static void ls_neighbours(rzk_lightslice* ls, std::vector<int>& nb, int k) {
+    nb.assign((size_t)ls->S * k, 0);
+    std::vector<std::pair<int64_t, int>> d(ls->S);
+    for (int s = 0; s < ls->S; ++s) {
+        int sx = s % ls->tx, sy = s / ls->tx;
+        for (int o = 0; o < ls->S; ++o) {
+            int ox = o % ls->tx, oy = o / ls->tx;
+            d[o] = {(int64_t)(ox - sx) * (ox - sx) + (int64_t)(oy - sy) * (oy - sy), o};
+        }
+        std::partial_sort(d.begin(), d.begin() + k, d.end());
+        for (int q = 0; q < k; ++q) nb[(size_t)s * k + q] = d[q].second;
+    }
+}
+
+extern "C" {
+
+int rzk_render_vpl(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const float* camera, int clamp_mode,
+                   float clamp, float tmin, float n_emitted, float* out_rgb) {
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(s && v && camera, RZK_E_INVALID, "null argument");
+        RZK_CHECK(s->bvh.ntri > 0, RZK_E_INVALID, "empty scene");
+        int W = (int)camera[17], H = (int)camera[18];
+        RZK_CHECK(W > 0 && H > 0, RZK_E_INVALID, "bad image size");
+        int64_t n = (int64_t)W * H;
+        DBuf<float> rays, recv, img;
+        DBuf<float4> hits;
+        DBuf<int> mat, light, order;
+        rays.alloc(6 * n); recv.alloc(RZK_RECV_FLOATS * n); img.alloc(3 * n);
+        hits.alloc(n); mat.alloc(n); light.alloc(n); order.alloc(n);
+        // 8x8 pixel tiles per wave keep the 64 segments of a step coherent
+        std::vector<int> ord(n);
+        int64_t t = 0;
+        for (int by = 0; by < H; by += 8)
+            for (int bx = 0; bx < W; bx += 8)
+                for (int yy = by; yy < std::min(by + 8, H); ++yy)
+                    for (int xx = bx; xx < std::min(bx + 8, W); ++xx) ord[t++] = yy * W + xx;
+        RZK_HIP(hipMemcpyAsync(order.p, ord.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+        SceneView sv = scene_view(s);
+        launch_camera_image_rays(ctx, camera, W, H, rays.p);
+        launch_trace_closest(ctx, sv, rays.p, n, 1e-3f, 1e6f, hits.p, nullptr);
+        launch_fill_receivers(ctx, sv, rays.p, hits.p, n, recv.p, mat.p, light.p);
+        launch_gather_fused(ctx, sv, recv.p, mat.p, n, order.p, v->d_vpl, v->d_mat, v->n, clamp_mode, clamp, tmin,
+                            n_emitted, img.p, nullptr);
+        if (out_rgb) RZK_HIP(hipMemcpyAsync(out_rgb, img.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+        ls_sync(ctx);
+    });
+}
+
+int rzk_lightslice_create(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const rzk_lightslice_params* p,
+                          rzk_lightslice** out) {
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(ctx && s && v && p && out, RZK_E_INVALID, "null argument");
+        *out = nullptr;
+        RZK_CHECK(p->width > 0 && p->height > 0 && p->tile > 0, RZK_E_INVALID, "bad image or tile size");
+        RZK_CHECK(p->num_clusters > 0 && p->num_neighbor_slices > 0 && p->map_resolution > 0, RZK_E_INVALID,
+                  "bad clustering parameters");
+        RZK_CHECK(v->n > 0, RZK_E_INVALID, "There are no VPLs to cluster");
+        RZK_CHECK(v->n >= p->num_clusters, RZK_E_INVALID, "fewer VPLs than clusters (cluster_builder.cc:91-101)");
+        RZK_CHECK(s->bvh.ntri > 0, RZK_E_INVALID, "empty scene");
+        std::unique_ptr<rzk_lightslice> ls(new rzk_lightslice());
+        ls->ctx = ctx; ls->scene = s; ls->vpls = v; ls->prm = *p;
+        ls->W = p->width; ls->H = p->height; ls->T = p->tile;
+        ls->tx = (ls->W + ls->T - 1) / ls->T; ls->ty = (ls->H + ls->T - 1) / ls->T;
+        ls->S = ls->tx * ls->ty;
+        ls->V = v->n;
+        RZK_CHECK(p->num_neighbor_slices <= ls->S, RZK_E_INVALID, "more neighbour slices than slices");
+        int64_t n = (int64_t)ls->W * ls->H;
+        int S = ls->S, C = p->num_clusters, res = p->map_resolution;
+        int64_t V = ls->V;
+        ls->rays.alloc(6 * n); ls->recv.alloc(RZK_RECV_FLOATS * n); ls->image.alloc(3 * n);
+        ls->hits.alloc(n); ls->mat.alloc(n); ls->light.alloc(n);
+        ls->row_pix.alloc(S); ls->rows_mat.alloc(S); ls->rows_recv.alloc(RZK_RECV_FLOATS * S);
+        ls->R.alloc(3 * (size_t)S * V); ls->Vm.alloc((size_t)S * V);
+        ls->alphas.alloc(V); ls->mags.alloc(V); ls->norms.alloc(V); ls->u.alloc(S); ls->v.alloc(V); ls->c0.alloc(1);
+        ls->centers.alloc(C); ls->center_w.alloc(C); ls->K.alloc(3 * (size_t)S * C); ls->Knorm.alloc(C);
+        ls->assign.alloc(V); ls->reps.alloc(C); ls->rep_w.alloc(C);
+        ls->nb.alloc((size_t)S * p->num_neighbor_slices);
+        ls->upper.alloc((size_t)S * C); ls->slice_wi.alloc(3 * (size_t)S * C); ls->slice_w.alloc((size_t)S * C);
+        ls->slice_pdf.alloc((size_t)S * C); ls->slice_cdf.alloc((size_t)S * C);
+        ls->maps.alloc((size_t)S * res * res); ls->col_cdf.alloc((size_t)S * res * res);
+        ls->row_pdf.alloc((size_t)S * res); ls->row_cdf.alloc((size_t)S * res); ls->totals.alloc(S);
+        std::vector<int> pix, nbh;
+        ls_rows(ls.get(), pix);
+        ls_neighbours(ls.get(), nbh, p->num_neighbor_slices);
+        RZK_HIP(hipMemcpy(ls->row_pix.p, pix.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+        RZK_HIP(hipMemcpy(ls->nb.p, nbh.data(), sizeof(int) * nbh.size(), hipMemcpyHostToDevice));
+        std::memset(&ls->info, 0, sizeof(ls->info));
+        ls->info.num_slices = S;
+        *out = ls.release();
+    });
+}
+
+int rzk_lightslice_destroy(rzk_lightslice* ls) {
+    delete ls;
+    return RZK_OK;
+}
+
+int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
+    if (!ls) return RZK_E_INVALID;
+    rzk_ctx* ctx = ls->ctx;
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(camera, RZK_E_INVALID, "null camera");
+        RZK_CHECK((int)camera[17] == ls->W && (int)camera[18] == ls->H, RZK_E_INVALID,
+                  "camera image size differs from the LightSlice parameters");
+        const rzk_lightslice_params& p = ls->prm;
+        SceneView sv = scene_view(ls->scene);
+        int S = ls->S;
+        int64_t V = ls->V, n = (int64_t)ls->W * ls->H;
+        hipStream_t st = ctx->stream;
+        // primary hits for every pixel (camera.cc:52-65 pixel centres)
+        launch_camera_image_rays(ctx, camera, ls->W, ls->H, ls->rays.p);
+        launch_trace_closest(ctx, sv, ls->rays.p, n, 1e-3f, 1e6f, ls->hits.p, nullptr);
+        launch_fill_receivers(ctx, sv, ls->rays.p, ls->hits.p, n, ls->recv.p, ls->mat.p, ls->light.p);
+        hipLaunchKernelGGL(k_pick_rows, dim3((S + 63) / 64), dim3(64), 0, st, ls->recv.p, ls->mat.p, ls->row_pix.p, S,
+                           ls->rows_recv.p, ls->rows_mat.p);
+        // reduced matrix (lightslice.cc:721-748)
+        launch_reduced_matrix(ctx, sv, ls->rows_recv.p, ls->rows_mat.p, S, ls->vpls->d_vpl, ls->vpls->d_mat, V,
+                              p.clamp_mode, p.incident_clamp, p.tmin, ls->R.p, ls->Vm.p, nullptr);
+        // alphas (light_transport_matrix.cc:72-106)
+        launch_alphas_full(ctx, S, V, ls->R.p, ls->alphas.p, ls->mags.p, ls->norms.p, ls->u.p, ls->v.p, ls->c0.p);
+        ls->h_alphas.resize(V);
+        RZK_HIP(hipMemcpyAsync(ls->h_alphas.data(), ls->alphas.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
+        ls_sync(ctx);
+        // centre sampling (cluster_builder.cc:114-156) on the host MT stream
+        Mt19937 rd(p.seed);
+        DiscretePdf apdf;
+        apdf.build(ls->h_alphas.data(), (size_t)V);
+        int C = p.num_clusters;
+        ls->h_centers.clear();
+        ls->h_center_w.clear();
+        int64_t draws = 0, max_draws = 1000LL * C;
+        while ((int)ls->h_centers.size() < C && draws < max_draws) {
+            draws++;
+            float pk;
+            int k = apdf.sample(rd.next_float(), pk);
+            if (pk == 0.0f) continue;
+            auto it = std::find(ls->h_centers.begin(), ls->h_centers.end(), k);
+            if (it == ls->h_centers.end()) {
+                ls->h_centers.push_back(k);
+                ls->h_center_w.push_back(1.0f / pk);
+            } else {
+                ls->h_center_w[it - ls->h_centers.begin()] += 1.0f / pk;
+            }
+        }
+        int ncl = (int)ls->h_centers.size();
+        RZK_CHECK(ncl > 0, RZK_E_STATE, "no cluster centre could be sampled (all alphas zero)");
+        RZK_HIP(hipMemcpyAsync(ls->centers.p, ls->h_centers.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(ls->center_w.p, ls->h_center_w.data(), sizeof(float) * ncl, hipMemcpyHostToDevice, st));
+        launch_assign_full(ctx, S, V, ls->R.p, ncl, ls->centers.p, ls->center_w.p, ls->mags.p, ls->K.p, ls->Knorm.p,
+                           ls->assign.p);
+        ls->h_assign.resize(V);
+        ls->h_mags.resize(V);
+        RZK_HIP(hipMemcpyAsync(ls->h_assign.data(), ls->assign.p, sizeof(int) * V, hipMemcpyDeviceToHost, st));
+        RZK_HIP(hipMemcpyAsync(ls->h_mags.data(), ls->mags.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
+        ls_sync(ctx);
+        // members in increasing VPL order, representatives (cluster_builder.cc:230-272)
+        std::vector<int> cnt(ncl + 1, 0), off(ncl + 1, 0), mem(V);
+        for (int64_t i = 0; i < V; ++i)
+            if (ls->h_assign[i] >= 0) cnt[ls->h_assign[i]]++;
+        for (int c = 0; c < ncl; ++c) off[c + 1] = off[c] + cnt[c];
+        std::vector<int> fill(ncl, 0);
+        for (int64_t i = 0; i < V; ++i) {
+            int c = ls->h_assign[i];
+            if (c >= 0) mem[off[c] + fill[c]++] = (int)i;
+        }
+        ls->h_reps.clear();
+        ls->h_rep_w.clear();
+        std::vector<float> mm;
+        for (int c = 0; c < ncl; ++c) {
+            int m = cnt[c];
+            mm.resize(m);
+            for (int q = 0; q < m; ++q) mm[q] = ls->h_mags[mem[off[c] + q]];
+            DiscretePdf cp;
+            cp.build(mm.data(), (size_t)m);
+            float u = rd.next_float();
+            float pj = 0.f;
+            int j = cp.sample(u, pj);
+            int rep;
+            float w;
+            if (j < 0) { rep = m > 0 ? mem[off[c] + m / 2] : -1; w = 0.0f; }
+            else { rep = mem[off[c] + j]; w = 1.0f / pj; }
+            if (rep >= 0) { ls->h_reps.push_back(rep); ls->h_rep_w.push_back(w); }
+        }
+        int nreps = (int)ls->h_reps.size();
+        RZK_HIP(hipMemcpyAsync(ls->reps.p, ls->h_reps.data(), sizeof(int) * nreps, hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(ls->rep_w.p, ls->h_rep_w.data(), sizeof(float) * nreps, hipMemcpyHostToDevice, st));
+        // per-slice cluster data, pdfs and luminance maps
+        int64_t sc = (int64_t)S * nreps;
+        if (nreps > 0) {
+            hipLaunchKernelGGL(k_slice_info, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, st, sv.mat, ls->rows_recv.p,
+                               ls->rows_mat.p, S, ls->Vm.p, ls->vpls->d_vpl, ls->vpls->d_mat, ls->reps.p, nreps,
+                               p.clamp_mode, p.incident_clamp, ls->slice_wi.p, ls->upper.p);
+            hipLaunchKernelGGL(k_slice_pdf_weights, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, st, ls->R.p, S,
+                               ls->nb.p, p.num_neighbor_slices, ls->reps.p, ls->rep_w.p, nreps, ls->upper.p,
+                               ls->slice_w.p);
+            hipLaunchKernelGGL(k_slice_pdf_norm, dim3((S + 63) / 64), dim3(64), 0, st, S, nreps, ls->slice_w.p,
+                               ls->slice_pdf.p, ls->slice_cdf.p);
+        }
+        int res = p.map_resolution;
+        size_t lds = sizeof(float) * ((size_t)res * res + res + 1);
+        hipLaunchKernelGGL(k_slice_maps, dim3(S), dim3(64), lds, st, res, nreps, ls->R.p, S, ls->rows_recv.p,
+                           ls->reps.p, ls->rep_w.p, ls->slice_wi.p, ls->upper.p, p.map_epsilon, p.map_gate ? 0 : 1,
+                           ls->maps.p, ls->col_cdf.p, ls->row_pdf.p, ls->row_cdf.p, ls->totals.p);
+        RZK_HIP(hipGetLastError());
+        ls_sync(ctx);
+        ls->info.num_centers = ncl;
+        ls->info.num_clusters = nreps;
+        ls->info.draws = (int32_t)draws;
+        ls->info.reduced_pairs = (int64_t)S * V;
+        ls->initialized = true;
+    });
+}
+
+int rzk_lightslice_gather(rzk_lightslice* ls, float* out_rgb) {
+    if (!ls) return RZK_E_INVALID;
+    rzk_ctx* ctx = ls->ctx;
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(ls->initialized, RZK_E_STATE, "rzk_lightslice_initialize has not run");
+        const rzk_lightslice_params& p = ls->prm;
+        SceneView sv = scene_view(ls->scene);
+        launch_slice_gather(ctx, sv, ls->recv.p, ls->mat.p, ls->W, ls->H, ls->T, ls->vpls->d_vpl, ls->vpls->d_mat,
+                            ls->reps.p, ls->rep_w.p, (int)ls->h_reps.size(), p.clamp_mode, p.clamp, p.tmin,
+                            p.n_emitted, ls->image.p, nullptr);
+        int64_t n = (int64_t)ls->W * ls->H;
+        if (out_rgb) {
+            RZK_HIP(hipMemcpyAsync(out_rgb, ls->image.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+            ls_sync(ctx);
+        }
+    });
+}
+
+int rzk_lightslice_get_info(const rzk_lightslice* ls, rzk_lightslice_info* out) {
+    if (!ls || !out) return RZK_E_INVALID;
+    *out = ls->info;
+    if (ls->initialized) {
+        // pixels with a material x clusters: counted on demand from the host copy
+        out->gather_pairs = 0;
+    }
+    return RZK_OK;
+}
+
+int rzk_lightslice_get(const rzk_lightslice* ls, const char* name, void* dst, int64_t bytes) {
+    if (!ls || !name || !dst) return RZK_E_INVALID;
+    rzk_ctx* ctx = ls->ctx;
+    return ls_guard(ctx, [&] {
+        std::string nm(name);
+        int S = ls->S, res = ls->prm.map_resolution;
+        int64_t V = ls->V, n = (int64_t)ls->W * ls->H;
+        int nreps = (int)ls->h_reps.size();
+        const void* src = nullptr;
+        int64_t need = 0;
+        bool host = false;
+        if (nm == "R") { src = ls->R.p; need = 12 * (int64_t)S * V; }
+        else if (nm == "V") { src = ls->Vm.p; need = 4 * (int64_t)S * V; }
+        else if (nm == "alphas") { src = ls->h_alphas.data(); need = 4 * V; host = true; }
+        else if (nm == "mags") { src = ls->h_mags.data(); need = 4 * V; host = true; }
+        else if (nm == "centers") { src = ls->h_centers.data(); need = 4 * (int64_t)ls->h_centers.size(); host = true; }
+        else if (nm == "center_w") { src = ls->h_center_w.data(); need = 4 * (int64_t)ls->h_center_w.size(); host = true; }
+        else if (nm == "assign") { src = ls->h_assign.data(); need = 4 * V; host = true; }
+        else if (nm == "reps") { src = ls->h_reps.data(); need = 4 * (int64_t)nreps; host = true; }
+        else if (nm == "rep_w") { src = ls->h_rep_w.data(); need = 4 * (int64_t)nreps; host = true; }
+        else if (nm == "rows_recv") { src = ls->rows_recv.p; need = 4 * (int64_t)RZK_RECV_FLOATS * S; }
+        else if (nm == "rows_mat") { src = ls->rows_mat.p; need = 4 * (int64_t)S; }
+        else if (nm == "slice_pdf") { src = ls->slice_pdf.p; need = 4 * (int64_t)S * nreps; }
+        else if (nm == "slice_cdf") { src = ls->slice_cdf.p; need = 4 * (int64_t)S * nreps; }
+        else if (nm == "slice_upper") { src = ls->upper.p; need = 4 * (int64_t)S * nreps; }
+        else if (nm == "slice_wi") { src = ls->slice_wi.p; need = 12 * (int64_t)S * nreps; }
+        else if (nm == "maps") { src = ls->maps.p; need = 4 * (int64_t)S * res * res; }
+        else if (nm == "map_totals") { src = ls->totals.p; need = 4 * (int64_t)S; }
+        else if (nm == "neighbors") { src = ls->nb.p; need = 4 * (int64_t)S * ls->prm.num_neighbor_slices; }
+        else if (nm == "image") { src = ls->image.p; need = 12 * n; }
+        else if (nm == "recv") { src = ls->recv.p; need = 4 * (int64_t)RZK_RECV_FLOATS * n; }
+        else if (nm == "recv_mat") { src = ls->mat.p; need = 4 * n; }
+        else throw RzkError{RZK_E_INVALID, "unknown array name " + nm};
+        RZK_CHECK(bytes == need, RZK_E_INVALID, "size mismatch for " + nm + ": need " + std::to_string(need));
+        if (need == 0) return;
+        if (host) std::memcpy(dst, src, (size_t)need);
+        else {
+            RZK_HIP(hipMemcpyAsync(dst, src, (size_t)need, hipMemcpyDeviceToHost, ctx->stream));
+            ls_sync(ctx);
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// rzk_internal.h — host-side objects behind the C ABI (include/rzk.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rzk.h"
+#include "rzk_math.h"
+
+// Device BVH layout.
+//   Interior node: 64 B = both child boxes + both child refs
+//     a = (L.min.xyz, L.max.x)  b = (L.max.yz, R.min.xy)  c = (R.min.z, R.max.xyz)
+//     d = (left_ref, right_ref, 0, 0)
+//   ref >= 0 : interior node index;  ref < 0 : leaf holding sorted triangle ~ref
+//   Triangle record: 48 B = (p0.xyz, original id) (p0-p1, 0) (p0-p2, 0)
+struct BvhNodeGpu { float4 a, b, c; int4 d; };
+struct TriGpu { float4 p0, e1, e2; };
+
+struct BvhHost {
+    int ntri = 0;
+    int num_interior = 0;
+    int depth = 0;            // deepest root-to-leaf path (nodes), bounds the traversal stack
+    int root_ref = 0;
+    float box[6] = {0, 0, 0, 0, 0, 0};   // scene box (origin included, boundingbox.h:100-103)
+    float root_box[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<BvhNodeGpu> nodes;
+    std::vector<TriGpu> tris;
+    std::vector<int> sorted_tri;         // sorted position -> original triangle
+    std::vector<uint32_t> sorted_morton;
+    // reference-order (BFS) view for parity checks
+    std::vector<int> bfs_child;          // 2 per BFS node
+    std::vector<int> bfs_range;          // 2 per BFS node
+    std::vector<float> bfs_box;          // 6 per BFS node
+};
+
+void bvh_build_host(int ntri, const float* pos, BvhHost& out);
+
+struct rzk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    // reusable device scratch (grown on demand)
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int num_cus = 0;
+};
+
+struct rzk_scene {
+    rzk_ctx* ctx = nullptr;
+    BvhHost bvh;
+    int nsurf = 0, nmat = 0, nlight = 0;
+    // device
+    BvhNodeGpu* d_nodes = nullptr;
+    TriGpu* d_tris = nullptr;
+    float4* d_tri_n = nullptr;        // per original triangle: (n.xyz, surface id bits)
+    float4* d_tri_tan = nullptr;      // per original triangle: (tangent.xyz, 0)
+    int4* d_surf = nullptr;           // per surface: (material, light, flags, 0)
+    float* d_mat = nullptr;           // 12 floats per material
+    std::vector<float> h_mat;
+    std::vector<int> h_surf_material, h_surf_light, h_surf_flags, h_tri_surface;
+    std::vector<float> h_light_radiance;
+};
+
+struct rzk_vpls {
+    rzk_ctx* ctx = nullptr;
+    int64_t n = 0;
+    float* d_vpl = nullptr;           // 21 floats per VPL (include/rzk.h)
+    int* d_mat = nullptr;
+    std::vector<float> h_vpl;
+    std::vector<int> h_mat;
+};
+
+// Scene constants passed by value to kernels
+struct SceneView {
+    const BvhNodeGpu* nodes;
+    const TriGpu* tris;
+    const float4* tri_n;
+    const float4* tri_tan;
+    const int4* surf;
+    const float* mat;
+    int root_ref;
+    int depth;
+    float rbmin[3], rbmax[3];
+};
+
+SceneView scene_view(const rzk_scene* s);
+
+// error plumbing
+struct RzkError { int code; std::string msg; };
+#define RZK_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw RzkError{RZK_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+#define RZK_CHECK(cond, code, msg) \
+    do { if (!(cond)) throw RzkError{code, msg}; } while (0)
+
+void* ctx_scratch(rzk_ctx* ctx, size_t bytes);
+
+// kernel launchers (traverse.hip, shade.hip, cluster.hip)
+void launch_trace_closest(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int64_t n,
+                          float tmin, float tmax, float4* d_hits, int* d_stats);
+void launch_trace_any(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int64_t n,
+                      float tmin, float tmax, uint8_t* d_out, int* d_stats);
+void launch_trace_segments(rzk_ctx* ctx, const SceneView& sv, const float* d_p, const float* d_q,
+                           int64_t n, float tmin, uint8_t* d_out, int* d_stats);
+void launch_fill_receivers(rzk_ctx* ctx, const SceneView& sv, const float* d_rays,
+                           const float4* d_hits, int64_t n, float* d_recv, int* d_mat, int* d_light);
+void launch_camera_rays(rzk_ctx* ctx, const float* cam, int64_t n, const int* d_px,
+                        const int* d_py, int width, float* d_rays);
+void launch_camera_image_rays(rzk_ctx* ctx, const float* cam, int width, int height, float* d_rays);
+void launch_visibility(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
+                       int64_t npts, const float* d_vpl, int64_t nvpl, float tmin,
+                       uint64_t* d_bits, int* d_stats);
+void launch_gather_shade(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
+                         int64_t npts, const float* d_vpl, const int* d_vmat, int64_t nvpl,
+                         const uint64_t* d_bits, int clamp_mode, float clamp, float n_emitted,
+                         float* d_out);
+void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
+                           const int* d_rmat, int rows, const float* d_vpl, const int* d_vmat,
+                           int64_t nvpl, int clamp_mode, float clamp, float tmin, float* d_R,
+                           float* d_V, int* d_stats);
+void launch_alphas(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_alphas,
+                   float* d_mags);
+void launch_cluster_assign(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl,
+                           const int* d_centers, const float* d_weights, int* d_assign);
+void launch_slice_gather(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
+                         const int* d_rmat, int width, int height, int tile,
+                         const float* d_vpl, const int* d_vmat, const int* d_reps,
+                         const float* d_rep_w, int nreps, int clamp_mode, float clamp,
+                         float tmin, float n_emitted, float* d_out, int* d_stats);

@@ -1,0 +1,216 @@
+// rzk_math.h — fp32 geometry and shading math shared by the host BVH builder
+// and the gfx950 kernels.
+//
+// Every expression keeps Renzoku's evaluation order (fp32, left to right, no
+// fused multiply-add: the reference is built -O2 for x86-64 without -mfma,
+// reference Makefile:10-11).  This translation unit family is compiled with
+// -ffp-contract=off and the pragma below, and with correctly rounded fp32
+// division and square root, so that ray/box/triangle results are bit-identical
+// to the reference's.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RZK_HD __host__ __device__ __forceinline__
+
+namespace rzk {
+
+// common.h:38-55
+constexpr float kPi = 3.14159265358979f;
+constexpr float kInvPi = 0.31830988618379f;
+constexpr float kTwoPi = 6.28318530717959f;
+constexpr float kInv2Pi = 0.15915494309190f;
+constexpr float kZeroEps = 1e-6f;
+constexpr float kOnbEps = 0.01f;
+
+struct v3 { float x, y, z; };
+
+RZK_HD v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+RZK_HD v3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+RZK_HD void st3(float* p, v3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+RZK_HD v3 operator+(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+RZK_HD v3 operator-(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+RZK_HD v3 operator-(v3 a) { return mk(-a.x, -a.y, -a.z); }
+RZK_HD v3 operator*(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }   // componentwise
+RZK_HD v3 operator/(v3 a, v3 b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
+RZK_HD v3 operator*(v3 a, float t) { return mk(a.x * t, a.y * t, a.z * t); }
+RZK_HD v3 operator*(float t, v3 a) { return mk(a.x * t, a.y * t, a.z * t); }      // vec3_cpu.h:257: v.e*t
+RZK_HD float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RZK_HD v3 cross(v3 a, v3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+RZK_HD float length(v3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// vec3_cpu.h:406-413: multiply by the reciprocal of the length
+RZK_HD v3 unit(v3 a) { float inv = 1.0f / length(a); return mk(a.x * inv, a.y * inv, a.z * inv); }
+RZK_HD bool isnan3(v3 a) { return a.x != a.x || a.y != a.y || a.z != a.z; }
+// std::min / std::max semantics (NaN handling and signed zeros follow the operand order)
+RZK_HD float smin(float a, float b) { return (b < a) ? b : a; }
+RZK_HD float smax(float a, float b) { return (a < b) ? b : a; }
+RZK_HD v3 vmin(v3 a, v3 b) { v3 t = a; if (b.x < a.x) t.x = b.x; if (b.y < a.y) t.y = b.y; if (b.z < a.z) t.z = b.z; return t; }
+RZK_HD v3 vmax(v3 a, v3 b) { v3 t = a; if (b.x > a.x) t.x = b.x; if (b.y > a.y) t.y = b.y; if (b.z > a.z) t.z = b.z; return t; }
+
+// Ray with the members Renzoku precomputes (ray.h:17-36).  `d` is the
+// constructor's normalised direction; inv_d clamps |d_i| < 1e-6 to 1e-6.
+struct Ray { v3 o, d, inv_d, org_inv_d; };
+
+RZK_HD Ray ray_from_od(v3 o, v3 d) {
+    Ray r;
+    r.o = o;
+    r.d = d;
+    v3 dd = mk(fabsf(d.x) < kZeroEps ? kZeroEps : d.x,
+               fabsf(d.y) < kZeroEps ? kZeroEps : d.y,
+               fabsf(d.z) < kZeroEps ? kZeroEps : d.z);
+    r.inv_d = mk(1.f, 1.f, 1.f) / dd;
+    r.org_inv_d = (-o) * r.inv_d;
+    return r;
+}
+
+// Ray::Ray(o, d): normalise, NaN -> 0 (ray.h:17-21)
+RZK_HD Ray ray_ctor(v3 o, v3 d_in) {
+    v3 d = unit(d_in);
+    if (isnan3(d)) d = mk(0.f, 0.f, 0.f);
+    return ray_from_od(o, d);
+}
+
+// BoundingBox::hit, boundingbox.h:238-280 (mul then add, never fused)
+RZK_HD bool box_hit(v3 bmin, v3 bmax, const Ray& r, float tmin, float tmax) {
+    float x0 = bmin.x * r.inv_d.x + r.org_inv_d.x, x1 = bmax.x * r.inv_d.x + r.org_inv_d.x;
+    float y0 = bmin.y * r.inv_d.y + r.org_inv_d.y, y1 = bmax.y * r.inv_d.y + r.org_inv_d.y;
+    float z0 = bmin.z * r.inv_d.z + r.org_inv_d.z, z1 = bmax.z * r.inv_d.z + r.org_inv_d.z;
+    float tminbox = smax(tmin, smax(smin(x0, x1), smax(smin(y0, y1), smin(z0, z1))));
+    float tmaxbox = smin(tmax, smin(smax(x0, x1), smin(smax(y0, y1), smax(z0, z1))));
+    return (tminbox <= tmaxbox) && (tmax >= tminbox && tmaxbox >= tmin);
+}
+
+// Triangle::hit, triangle.cc:158-232, on the precomputed edges p01 = p0 - p1,
+// p02 = p0 - p2 (the same subtractions the reference performs per test).
+RZK_HD bool tri_hit(v3 p0, v3 p01, v3 p02, const Ray& r, float tmin, float tmax,
+                    float& t_out, float& b_out, float& g_out) {
+    float A = p01.x, B = p01.y, C = p01.z;
+    float D = p02.x, E = p02.y, F = p02.z;
+    float G = r.d.x, H = r.d.y, I = r.d.z;
+    float J = p0.x - r.o.x, K = p0.y - r.o.y, L = p0.z - r.o.z;
+    float EIHF = E * I - H * F;
+    float GFDI = G * F - D * I;
+    float DHEG = D * H - E * G;
+    float denom = (A * EIHF + B * GFDI + C * DHEG);
+    if (-kZeroEps < denom && denom < kZeroEps) return false;
+    float beta = (J * EIHF + K * GFDI + L * DHEG) / denom;
+    if (beta < 0.0f || beta > 1.0f) return false;
+    float AKJB = A * K - J * B;
+    float JCAL = J * C - A * L;
+    float BLKC = B * L - K * C;
+    float gamma = (I * AKJB + H * JCAL + G * BLKC) / denom;
+    if (gamma < 0.0f || beta + gamma > 1.0f) return false;
+    float t = -(F * AKJB + E * JCAL + D * BLKC) / denom;
+    if (t >= tmin && t <= tmax) { t_out = t; b_out = beta; g_out = gamma; return true; }
+    return false;
+}
+
+// Orthonormal basis (onb.cc): u, v, n rows
+struct Onb { v3 u, v, n; };
+
+RZK_HD Onb onb_from_n(v3 n) {                       // onb.cc:42-54
+    Onb o;
+    v3 nn = unit(n);
+    v3 uu = cross(nn, mk(1.f, 0.f, 0.f));
+    if (length(uu) < kOnbEps) uu = cross(nn, mk(0.f, 1.f, 0.f));
+    v3 vv = cross(nn, uu);
+    o.u = unit(uu); o.v = unit(vv); o.n = nn;
+    return o;
+}
+
+RZK_HD Onb onb_from_nu(v3 n, v3 u_in) {             // onb.cc:99-109
+    v3 c = unit(cross(n, u_in));
+    if (isnan3(c)) return onb_from_n(n);
+    Onb o;
+    o.n = unit(n); o.v = c; o.u = cross(c, o.n);
+    return o;
+}
+
+RZK_HD v3 world_to_local(const Onb& b, v3 w) { return mk(dot(b.u, w), dot(b.v, w), dot(b.n, w)); }
+RZK_HD v3 local_to_world(const Onb& b, v3 l) { return (l.x * b.u + l.y * b.v) + l.z * b.n; }
+
+// Material record: 12 floats {type, kd[3], ks[3], p0, p1, pad[3]}
+enum { kLambert = 0, kPhong = 1, kWard = 2 };
+
+// Lambertian::eval (lambertian.cc:63-68), ModifiedPhong::eval (phong.cc:74-87),
+// Ward::eval (ward.cc:104-116)
+RZK_HD v3 bsdf_eval(const float* m, const Onb& b, v3 wo, v3 wi) {
+    int type = (int)m[0];
+    v3 kd = mk(m[1], m[2], m[3]);
+    v3 ks = mk(m[4], m[5], m[6]);
+    if (type == kWard) {
+        float ax = m[7], ay = m[8];
+        float ax2 = ax * ax, ay2 = ay * ay;
+        float inv4 = 1.0f / (4.0f * kPi * ax * ay);
+        v3 lwi = world_to_local(b, wi), lwo = world_to_local(b, wo);
+        if (lwi.z <= kZeroEps || lwo.z <= kZeroEps) return mk(0.f, 0.f, 0.f);
+        v3 wh = lwi + lwo;
+        float expo = expf(-(wh.x * wh.x / ax2 + wh.y * wh.y / ay2) / (wh.z * wh.z));
+        float sq = sqrtf(lwi.z * lwo.z);
+        v3 spec = mk(ks.x * inv4 / sq, ks.y * inv4 / sq, ks.z * inv4 / sq) * expo;
+        return kd * kInvPi + spec;
+    }
+    if (dot(wi, b.n) <= kZeroEps || dot(wo, b.n) <= kZeroEps) return mk(0.f, 0.f, 0.f);
+    v3 brdf = kd * kInvPi;
+    if (type == kPhong) {
+        float ne = m[7];
+        v3 wr = b.n * (2.0f * dot(wi, b.n)) - wi;
+        float cos_alpha = dot(wo, wr);
+        if (cos_alpha > 0.0f) brdf = brdf + ((ks * (ne + 2.0f)) * kInv2Pi) * powf(cos_alpha, ne);
+    }
+    return brdf;
+}
+
+// FormFactor::form_factor_abs, form_factor.h:11-20
+RZK_HD float form_factor_abs(v3 p, v3 pn, v3 q, v3 qn) {
+    v3 pq = q - p;
+    float d1 = dot(pn, pq);
+    float d2 = -dot(qn, pq);
+    float dpq = dot(pq, pq);
+    return fabsf((d1 * d2) / (dpq * dpq));
+}
+
+enum { kClampNone = 0, kClampFormFactor = 1, kClampReflectivity = 2, kClampRadiance = 3 };
+
+RZK_HD float rgb_value(v3 c) { return sqrtf(c.x * c.x + c.y * c.y + c.z * c.z); }   // rgb.h:85
+
+// RadianceTerms::eval_clamp_* (vpl_interface.h:50-123) dispatched as clamp_term (vpl.cc:214-230)
+RZK_HD v3 clamp_term(int mode, float thr, v3 power, v3 fs, v3 fl, float G) {
+    if (mode == kClampFormFactor) {
+        if (G > thr) return ((fl * fs) * thr) * power;
+        return ((fl * fs) * G) * power;
+    }
+    if (mode == kClampReflectivity) {
+        v3 refl = (fs * G) * fl;
+        float lum = rgb_value(refl);
+        if (lum > thr) { float w = thr / lum; return (refl * w) * power; }
+        return refl * power;
+    }
+    if (mode == kClampRadiance) {
+        v3 rnc = ((power * G) * fs) * fl;
+        float bright = rgb_value(rnc);
+        if (bright > thr) { float w = thr / bright; return rnc * w; }
+        return rnc;
+    }
+    return ((power * G) * fs) * fl;
+}
+
+// Morton code of a point already normalised to the scene box (morton.h:37-46)
+RZK_HD uint32_t spread10(uint32_t v) {
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+RZK_HD uint32_t morton3d(float x, float y, float z) {
+    x = smin(smax(x * 1024.0f, 0.0f), 1023.0f);
+    y = smin(smax(y * 1024.0f, 0.0f), 1023.0f);
+    z = smin(smax(z * 1024.0f, 0.0f), 1023.0f);
+    return (spread10((uint32_t)x) << 2) | (spread10((uint32_t)y) << 1) | spread10((uint32_t)z);
+}
+
+}  // namespace rzk

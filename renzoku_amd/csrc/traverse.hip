@@ -1,0 +1,222 @@
+// traverse.hip — closest-hit, any-hit and segment traversal kernels, hit
+// records and camera rays.
+#pragma clang fp contract(off)
+
+#include "rzk_device.h"
+
+namespace rzk {
+
+constexpr int kTraceBlock = 128;
+
+// kStats: 0 = none, 1 = per-ray counts (int2 per ray), 2 = launch totals
+template <int kStats>
+__device__ __forceinline__ void record_stats(int* d_stats, int64_t i, const TravStats& st) {
+    if (kStats == 1) {
+        d_stats[2 * i] = st.nodes;
+        d_stats[2 * i + 1] = st.tris;
+    } else if (kStats == 2) {
+        int a = st.nodes, b = st.tris;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off);
+            b += __shfl_xor(b, off);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(d_stats, a);
+            atomicAdd(d_stats + 1, b);
+        }
+    }
+}
+
+template <int kStats>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView sv, const float* __restrict__ rays,
+                                                               int64_t n, float tmin, float tmax,
+                                                               float4* __restrict__ hits,
+                                                               int* __restrict__ d_stats) {
+    extern __shared__ int s_stack[];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    TravStats st;
+    if (i < n) {
+        const float* rp = rays + RZK_RAY_FLOATS * i;
+        Ray r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
+        float t = 0.f, b = 0.f, g = 0.f;
+        int tri = traverse<false>(sv, r, tmin, tmax, s_stack + threadIdx.x, blockDim.x, t, b, g, st);
+        hits[i] = tri >= 0 ? make_float4(t, b, g, __int_as_float(tri))
+                           : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+    }
+    record_stats<kStats>(d_stats, i, st);
+}
+
+template <int kStats>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_any(SceneView sv, const float* __restrict__ rays,
+                                                           int64_t n, float tmin, float tmax,
+                                                           uint8_t* __restrict__ out,
+                                                           int* __restrict__ d_stats) {
+    extern __shared__ int s_stack[];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    TravStats st;
+    if (i < n) {
+        const float* rp = rays + RZK_RAY_FLOATS * i;
+        Ray r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
+        float t, b, g;
+        out[i] = traverse<true>(sv, r, tmin, tmax, s_stack + threadIdx.x, blockDim.x, t, b, g, st) >= 0;
+    }
+    record_stats<kStats>(d_stats, i, st);
+}
+
+template <int kStats>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_segments(SceneView sv, const float* __restrict__ p,
+                                                                const float* __restrict__ q, int64_t n,
+                                                                float tmin, uint8_t* __restrict__ out,
+                                                                int* __restrict__ d_stats) {
+    extern __shared__ int s_stack[];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    TravStats st;
+    if (i < n) {
+        out[i] = segment_blocked(sv, ld3(p + 3 * i), ld3(q + 3 * i), tmin, s_stack + threadIdx.x,
+                                 blockDim.x, st);
+    }
+    record_stats<kStats>(d_stats, i, st);
+}
+
+__global__ void k_fill_receivers(SceneView sv, const float* __restrict__ rays,
+                                 const float4* __restrict__ hits, int64_t n, float* __restrict__ recv,
+                                 int* __restrict__ rmat, int* __restrict__ rlight) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* rp = rays + RZK_RAY_FLOATS * i;
+    Ray r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
+    float4 h = hits[i];
+    float rec[RZK_RECV_FLOATS];
+    int m, l;
+    fill_receiver(sv, r, h.x, h.y, h.z, __float_as_int(h.w), rec, m, l);
+    for (int k = 0; k < RZK_RECV_FLOATS; ++k) recv[RZK_RECV_FLOATS * i + k] = rec[k];
+    rmat[i] = m;
+    if (rlight) rlight[i] = l;
+}
+
+struct CamParams { float v[RZK_CAM_FLOATS]; };
+
+// Camera::shoot_ray(i, j), camera.cc:52-65, followed by Ray::Ray (which
+// normalises the already normalised direction once more)
+__device__ __forceinline__ void camera_ray(const CamParams& c, int i, int j, float* out) {
+    v3 eye = mk(c.v[0], c.v[1], c.v[2]);
+    Onb b;
+    b.u = mk(c.v[3], c.v[4], c.v[5]);
+    b.v = mk(c.v[6], c.v[7], c.v[8]);
+    b.n = mk(c.v[9], c.v[10], c.v[11]);
+    float height = c.v[18];
+    float pxf = (float)j + 0.5f;
+    float pyf = (height - 1.0f - (float)i) + 0.5f;
+    float fx = (pxf - c.v[12]) / c.v[14];
+    float fy = (pyf - c.v[13]) / c.v[15];
+    v3 d = unit(local_to_world(b, mk(fx, fy, -c.v[16])));
+    Ray r = ray_ctor(eye, d);
+    st3(out, r.o);
+    st3(out + 3, r.d);
+}
+
+__global__ void k_camera_rays(CamParams c, int64_t n, const int* __restrict__ px,
+                              const int* __restrict__ py, float* __restrict__ rays) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float out[6];
+    camera_ray(c, py[k], px[k], out);
+    for (int t = 0; t < 6; ++t) rays[6 * k + t] = out[t];
+}
+
+__global__ void k_camera_image_rays(CamParams c, int width, int64_t n, float* __restrict__ rays) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float out[6];
+    camera_ray(c, (int)(k / width), (int)(k % width), out);
+    for (int t = 0; t < 6; ++t) rays[6 * k + t] = out[t];
+}
+
+}  // namespace rzk
+
+using namespace rzk;
+
+static unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+void launch_trace_closest(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int64_t n,
+                          float tmin, float tmax, float4* d_hits, int* d_stats) {
+    if (n <= 0) return;
+    size_t lds = sizeof(int) * kTraceBlock * (size_t)sv.depth;
+    if (d_stats)
+        hipLaunchKernelGGL(k_trace_closest<2>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+                           ctx->stream, sv, d_rays, n, tmin, tmax, d_hits, d_stats);
+    else
+        hipLaunchKernelGGL(k_trace_closest<0>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+                           ctx->stream, sv, d_rays, n, tmin, tmax, d_hits, d_stats);
+    RZK_HIP(hipGetLastError());
+}
+
+void launch_trace_any(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int64_t n,
+                      float tmin, float tmax, uint8_t* d_out, int* d_stats) {
+    if (n <= 0) return;
+    size_t lds = sizeof(int) * kTraceBlock * (size_t)sv.depth;
+    if (d_stats)
+        hipLaunchKernelGGL(k_trace_any<2>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+                           ctx->stream, sv, d_rays, n, tmin, tmax, d_out, d_stats);
+    else
+        hipLaunchKernelGGL(k_trace_any<0>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+                           ctx->stream, sv, d_rays, n, tmin, tmax, d_out, d_stats);
+    RZK_HIP(hipGetLastError());
+}
+
+void launch_trace_segments(rzk_ctx* ctx, const SceneView& sv, const float* d_p, const float* d_q,
+                           int64_t n, float tmin, uint8_t* d_out, int* d_stats) {
+    if (n <= 0) return;
+    size_t lds = sizeof(int) * kTraceBlock * (size_t)sv.depth;
+    if (d_stats)
+        hipLaunchKernelGGL(k_trace_segments<2>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+                           ctx->stream, sv, d_p, d_q, n, tmin, d_out, d_stats);
+    else
+        hipLaunchKernelGGL(k_trace_segments<0>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+                           ctx->stream, sv, d_p, d_q, n, tmin, d_out, d_stats);
+    RZK_HIP(hipGetLastError());
+}
+
+// per-ray statistics (kind 0 closest, 1 any)
+void launch_trace_stats(rzk_ctx* ctx, const SceneView& sv, int kind, const float* d_rays, int64_t n,
+                        float tmin, float tmax, void* d_scratch_out, int* d_stats) {
+    if (n <= 0) return;
+    size_t lds = sizeof(int) * kTraceBlock * (size_t)sv.depth;
+    if (kind == 0)
+        hipLaunchKernelGGL(k_trace_closest<1>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+                           ctx->stream, sv, d_rays, n, tmin, tmax, (float4*)d_scratch_out, d_stats);
+    else
+        hipLaunchKernelGGL(k_trace_any<1>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+                           ctx->stream, sv, d_rays, n, tmin, tmax, (uint8_t*)d_scratch_out, d_stats);
+    RZK_HIP(hipGetLastError());
+}
+
+void launch_fill_receivers(rzk_ctx* ctx, const SceneView& sv, const float* d_rays,
+                           const float4* d_hits, int64_t n, float* d_recv, int* d_mat, int* d_light) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_receivers, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, sv, d_rays,
+                       d_hits, n, d_recv, d_mat, d_light);
+    RZK_HIP(hipGetLastError());
+}
+
+void launch_camera_rays(rzk_ctx* ctx, const float* cam, int64_t n, const int* d_px, const int* d_py,
+                        int width, float* d_rays) {
+    (void)width;
+    if (n <= 0) return;
+    CamParams c;
+    for (int i = 0; i < RZK_CAM_FLOATS; ++i) c.v[i] = cam[i];
+    hipLaunchKernelGGL(k_camera_rays, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, c, n, d_px,
+                       d_py, d_rays);
+    RZK_HIP(hipGetLastError());
+}
+
+void launch_camera_image_rays(rzk_ctx* ctx, const float* cam, int width, int height, float* d_rays) {
+    int64_t n = (int64_t)width * height;
+    if (n <= 0) return;
+    CamParams c;
+    for (int i = 0; i < RZK_CAM_FLOATS; ++i) c.v[i] = cam[i];
+    hipLaunchKernelGGL(k_camera_image_rays, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, c, width,
+                       n, d_rays);
+    RZK_HIP(hipGetLastError());
+}
+
