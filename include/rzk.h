@@ -120,15 +120,16 @@ int rzk_trace_any(rzk_ctx *ctx, const rzk_scene *scene, const float *rays, int64
 int rzk_trace_segments(rzk_ctx *ctx, const rzk_scene *scene, const float *p, const float *q,
                        int64_t n, float tmin, uint8_t *out_blocked);
 /* device-resident forms: pointers are device memory, launches are queued on
- * the context stream and the call returns without synchronising.  stats
- * (optional, device int32[2]) accumulates {interior nodes visited, triangles tested}. */
+ * the context stream and the call returns without synchronising.  d_stats
+ * (optional, device uint64[2]) has {interior nodes visited, triangles tested}
+ * added to it. */
 int rzk_trace_closest_dev(rzk_ctx *ctx, const rzk_scene *scene, const float *d_rays, int64_t n,
-                          float tmin, float tmax, rzk_hit *d_out, int32_t *d_stats);
+                          float tmin, float tmax, rzk_hit *d_out, uint64_t *d_stats);
 int rzk_trace_any_dev(rzk_ctx *ctx, const rzk_scene *scene, const float *d_rays, int64_t n,
-                      float tmin, float tmax, uint8_t *d_out, int32_t *d_stats);
+                      float tmin, float tmax, uint8_t *d_out, uint64_t *d_stats);
 int rzk_trace_segments_dev(rzk_ctx *ctx, const rzk_scene *scene, const float *d_p,
                            const float *d_q, int64_t n, float tmin, uint8_t *d_out,
-                           int32_t *d_stats);
+                           uint64_t *d_stats);
 /* per-ray traversal counts {interior nodes visited, triangles tested}, host
  * buffers; kind 0 = closest, 1 = any, for rays; use rzk_trace_segment_stats for p,q */
 int rzk_trace_stats(rzk_ctx *ctx, const rzk_scene *scene, int kind, const float *rays,
@@ -223,6 +224,11 @@ int rzk_lightslice_initialize(rzk_lightslice *ls, const float *camera);
  * out_rgb host [height][width][3] or NULL to keep the image on the device. */
 int rzk_lightslice_gather(rzk_lightslice *ls, float *out_rgb);
 int rzk_lightslice_get_info(const rzk_lightslice *ls, rzk_lightslice_info *out);
+/* Algorithmic work of one step, measured by re-running the reduced matrix and
+ * the gather with traversal counters (results unchanged): out[8] = {primary
+ * rays, reduced pairs, reduced nodes visited, reduced triangles tested, gather
+ * pairs, gather nodes, gather triangles, clusters}. */
+int rzk_lightslice_count_work(rzk_lightslice *ls, int64_t *out);
 /* copy an intermediate array to host: name is one of
  *  "R" (rows*V*3 f32), "V" (rows*V f32), "alphas" (V f32), "mags" (V f32),
  *  "centers" (i32), "center_w" (f32), "assign" (V i32), "reps" (i32 per kept cluster),

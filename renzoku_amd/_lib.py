@@ -31,7 +31,7 @@ EXPORTS = [
     "rzk_gather", "rzk_visibility", "rzk_render_vpl",
     "rzk_reduced_matrix", "rzk_alphas", "rzk_cluster_assign",
     "rzk_lightslice_create", "rzk_lightslice_destroy", "rzk_lightslice_initialize",
-    "rzk_lightslice_gather", "rzk_lightslice_get_info", "rzk_lightslice_get",
+    "rzk_lightslice_gather", "rzk_lightslice_get_info", "rzk_lightslice_count_work", "rzk_lightslice_get",
 ]
 
 
@@ -118,6 +118,7 @@ def load() -> C.CDLL:
         "rzk_lightslice_initialize": [P, P],
         "rzk_lightslice_gather": [P, P],
         "rzk_lightslice_get_info": [P, C.POINTER(LightSliceInfo)],
+        "rzk_lightslice_count_work": [P, P],
         "rzk_lightslice_get": [P, C.c_char_p, P, I64],
     }
     for name, args in sig.items():
@@ -375,6 +376,13 @@ class LightSlice:
         inf = LightSliceInfo()
         self.lib.rzk_lightslice_get_info(self.h, C.byref(inf))
         return inf
+
+    def count_work(self):
+        out = np.zeros(8, np.int64)
+        self.ctx.check(self.lib.rzk_lightslice_count_work(self.h, ptr(out)), "rzk_lightslice_count_work")
+        keys = ["primary_rays", "reduced_pairs", "reduced_nodes", "reduced_tris", "gather_pairs", "gather_nodes",
+                "gather_tris", "clusters"]
+        return dict(zip(keys, (int(x) for x in out)))
 
     def get(self, name, dtype, count):
         out = np.zeros(count, dtype)

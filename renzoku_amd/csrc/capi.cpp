@@ -260,29 +260,29 @@ int rzk_scene_export_bvh(const rzk_scene* s, int32_t* sorted_tri, uint32_t* sort
 
 // ---------------------------------------------------------------------------
 int rzk_trace_closest_dev(rzk_ctx* ctx, const rzk_scene* s, const float* d_rays, int64_t n, float tmin,
-                          float tmax, rzk_hit* d_out, int32_t* d_stats) {
+                          float tmax, rzk_hit* d_out, uint64_t* d_stats) {
     return guarded(ctx, [&] {
         RZK_CHECK(s && n >= 0 && (n == 0 || (d_rays && d_out)), RZK_E_INVALID, "bad trace arguments");
         RZK_CHECK(s->bvh.ntri > 0 || n == 0, RZK_E_INVALID, "empty scene");
-        launch_trace_closest(ctx, scene_view(s), d_rays, n, tmin, tmax, reinterpret_cast<float4*>(d_out), d_stats);
+        launch_trace_closest(ctx, scene_view(s), d_rays, n, tmin, tmax, reinterpret_cast<float4*>(d_out), (int*)d_stats);
     });
 }
 
 int rzk_trace_any_dev(rzk_ctx* ctx, const rzk_scene* s, const float* d_rays, int64_t n, float tmin, float tmax,
-                      uint8_t* d_out, int32_t* d_stats) {
+                      uint8_t* d_out, uint64_t* d_stats) {
     return guarded(ctx, [&] {
         RZK_CHECK(s && n >= 0 && (n == 0 || (d_rays && d_out)), RZK_E_INVALID, "bad trace arguments");
         RZK_CHECK(s->bvh.ntri > 0 || n == 0, RZK_E_INVALID, "empty scene");
-        launch_trace_any(ctx, scene_view(s), d_rays, n, tmin, tmax, d_out, d_stats);
+        launch_trace_any(ctx, scene_view(s), d_rays, n, tmin, tmax, d_out, (int*)d_stats);
     });
 }
 
 int rzk_trace_segments_dev(rzk_ctx* ctx, const rzk_scene* s, const float* d_p, const float* d_q, int64_t n,
-                           float tmin, uint8_t* d_out, int32_t* d_stats) {
+                           float tmin, uint8_t* d_out, uint64_t* d_stats) {
     return guarded(ctx, [&] {
         RZK_CHECK(s && n >= 0 && (n == 0 || (d_p && d_q && d_out)), RZK_E_INVALID, "bad trace arguments");
         RZK_CHECK(s->bvh.ntri > 0 || n == 0, RZK_E_INVALID, "empty scene");
-        launch_trace_segments(ctx, scene_view(s), d_p, d_q, n, tmin, d_out, d_stats);
+        launch_trace_segments(ctx, scene_view(s), d_p, d_q, n, tmin, d_out, (int*)d_stats);
     });
 }
 

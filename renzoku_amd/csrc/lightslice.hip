@@ -498,13 +498,48 @@ int rzk_lightslice_gather(rzk_lightslice* ls, float* out_rgb) {
     });
 }
 
+int rzk_lightslice_count_work(rzk_lightslice* ls, int64_t* out) {
+    if (!ls || !out) return RZK_E_INVALID;
+    rzk_ctx* ctx = ls->ctx;
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(ls->initialized, RZK_E_STATE, "rzk_lightslice_initialize has not run");
+        const rzk_lightslice_params& p = ls->prm;
+        SceneView sv = scene_view(ls->scene);
+        int S = ls->S;
+        int64_t V = ls->V, n = (int64_t)ls->W * ls->H;
+        int nreps = (int)ls->h_reps.size();
+        DBuf<unsigned long long> tot;
+        tot.alloc(4);
+        RZK_HIP(hipMemsetAsync(tot.p, 0, sizeof(unsigned long long) * 4, ctx->stream));
+        launch_reduced_matrix(ctx, sv, ls->rows_recv.p, ls->rows_mat.p, S, ls->vpls->d_vpl, ls->vpls->d_mat, V,
+                              p.clamp_mode, p.incident_clamp, p.tmin, ls->R.p, ls->Vm.p, (int*)tot.p);
+        launch_slice_gather(ctx, sv, ls->recv.p, ls->mat.p, ls->W, ls->H, ls->T, ls->vpls->d_vpl, ls->vpls->d_mat,
+                            ls->reps.p, ls->rep_w.p, nreps, p.clamp_mode, p.clamp, p.tmin, p.n_emitted, ls->image.p,
+                            (int*)(tot.p + 2));
+        unsigned long long h[4];
+        std::vector<int> rm(S), pm(n);
+        RZK_HIP(hipMemcpyAsync(h, tot.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        RZK_HIP(hipMemcpyAsync(rm.data(), ls->rows_mat.p, sizeof(int) * S, hipMemcpyDeviceToHost, ctx->stream));
+        RZK_HIP(hipMemcpyAsync(pm.data(), ls->mat.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+        ls_sync(ctx);
+        int64_t rows_ok = 0, pix_ok = 0;
+        for (int v : rm) rows_ok += v >= 0;
+        for (int v : pm) pix_ok += v >= 0;
+        out[0] = n;
+        out[1] = rows_ok * V;
+        out[2] = (int64_t)h[0];
+        out[3] = (int64_t)h[1];
+        out[4] = pix_ok * nreps;
+        out[5] = (int64_t)h[2];
+        out[6] = (int64_t)h[3];
+        out[7] = nreps;
+        ls->info.gather_pairs = out[4];
+    });
+}
+
 int rzk_lightslice_get_info(const rzk_lightslice* ls, rzk_lightslice_info* out) {
     if (!ls || !out) return RZK_E_INVALID;
     *out = ls->info;
-    if (ls->initialized) {
-        // pixels with a material x clusters: counted on demand from the host copy
-        out->gather_pairs = 0;
-    }
     return RZK_OK;
 }
 

@@ -62,6 +62,20 @@ __device__ __forceinline__ int traverse(const SceneView& sv, const Ray& r, float
     return best;
 }
 
+// Wave-reduced traversal work, added to 64-bit launch totals {nodes, tris}.
+// Every lane of the wave must reach this call.
+__device__ __forceinline__ void add_work_totals(unsigned long long* tot, const TravStats& st) {
+    unsigned long long a = (unsigned long long)st.nodes, b = (unsigned long long)st.tris;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(tot, a);
+        atomicAdd(tot + 1, b);
+    }
+}
+
 // Segment visibility p -> q (aggregate.cc:84-90): true when blocked.
 __device__ __forceinline__ bool segment_blocked(const SceneView& sv, v3 p, v3 q, float tmin,
                                                 int* stk, int pitch, TravStats& st) {

@@ -53,11 +53,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_visibility(SceneView sv, const 
     }
     uint64_t b = __ballot(blocked);
     if (live && lane == 0) bits[k * ngroups + g] = b;
-    if (d_stats) {
-        int a = st.nodes, c = st.tris;
-        for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off); c += __shfl_xor(c, off); }
-        if (lane == 0) { atomicAdd(d_stats, a); atomicAdd(d_stats + 1, c); }
-    }
+    if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
 // ---- gather shading over all VPLs with precomputed visibility -------------
@@ -151,11 +147,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_gather_fused(SceneView sv, cons
         out[3 * i + 1] = acc.y / n_emitted;
         out[3 * i + 2] = acc.z / n_emitted;
     }
-    if (d_stats) {
-        int a = st.nodes, c = st.tris;
-        for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off); c += __shfl_xor(c, off); }
-        if ((threadIdx.x & 63) == 0) { atomicAdd(d_stats, a); atomicAdd(d_stats + 1, c); }
-    }
+    if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
 // ---- LightSlice reduced matrix (lightslice.cc:721-748): pair (row r, VPL k),
@@ -186,11 +178,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_reduced_matrix(SceneView sv, co
         R[3 * t] = rv; R[3 * t + 1] = g; R[3 * t + 2] = b;
         V[t] = vis;
     }
-    if (d_stats) {
-        int a = st.nodes, c = st.tris;
-        for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off); c += __shfl_xor(c, off); }
-        if ((threadIdx.x & 63) == 0) { atomicAdd(d_stats, a); atomicAdd(d_stats + 1, c); }
-    }
+    if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
 // ---- LightSlice::gather_clusters for every pixel (lightslice.cc:1315-1345):
@@ -253,11 +241,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_gather(SceneView sv, cons
         out[3 * i + 1] = acc.y / n_emitted;
         out[3 * i + 2] = acc.z / n_emitted;
     }
-    if (d_stats) {
-        int a = st.nodes, c = st.tris;
-        for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off); c += __shfl_xor(c, off); }
-        if ((threadIdx.x & 63) == 0) { atomicAdd(d_stats, a); atomicAdd(d_stats + 1, c); }
-    }
+    if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
 }  // namespace rzk

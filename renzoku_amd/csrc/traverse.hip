@@ -15,15 +15,7 @@ __device__ __forceinline__ void record_stats(int* d_stats, int64_t i, const Trav
         d_stats[2 * i] = st.nodes;
         d_stats[2 * i + 1] = st.tris;
     } else if (kStats == 2) {
-        int a = st.nodes, b = st.tris;
-        for (int off = 32; off > 0; off >>= 1) {
-            a += __shfl_xor(a, off);
-            b += __shfl_xor(b, off);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(d_stats, a);
-            atomicAdd(d_stats + 1, b);
-        }
+        add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
     }
 }
 

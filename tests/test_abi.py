@@ -1,0 +1,55 @@
+"""CPU checks of the drop-in boundary: librzk.so loads, exports every symbol
+include/rzk.h declares, and rejects bad arguments without touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "rzk.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*|const char\s*\*)\s*(rzk_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from renzoku_amd import _lib
+    lib = _lib.load()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(_lib.EXPORTS) == syms
+
+
+def test_null_arguments_are_rejected_on_cpu():
+    from renzoku_amd import _lib
+    lib = _lib.load()
+    assert lib.rzk_ctx_create(0, None) == _lib.RZK_OK + 1        # RZK_E_INVALID
+    assert lib.rzk_scene_get_info(None, None) == 1
+    assert lib.rzk_vpls_destroy(None) == 0
+    assert lib.rzk_lightslice_get(None, b"R", None, 0) == 1
+
+
+def test_no_gpu_means_error_not_fallback():
+    """Without a GPU the context cannot be created: there is no CPU path."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from renzoku_amd import _lib
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.rzk_ctx_create(0, C.byref(h)) != _lib.RZK_OK
+
+
+def test_scene_generators():
+    from renzoku_amd import scenes
+    c = scenes.cornell_box()
+    assert c.ntri == 36 and len(c.light_tris) == 2
+    assert c.materials.shape[1] == 12 and (c.materials[:, 1:4] >= 0.2).all() and (c.materials[:, 1:4] <= 0.8).all()
+    r = scenes.room(64, 64, target_tris=20_000)
+    assert 18_000 < r.ntri <= 20_000
+    kinds = set(r.materials[:, 0].astype(int).tolist())
+    assert kinds == {0, 1}

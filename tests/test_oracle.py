@@ -1,0 +1,464 @@
+"""Pins the CPU oracle: reference golden vectors where they exist (the
+reference's own mtwist.c), otherwise closed forms, invariants and brute force
+chosen so that a dropped term, a swapped operand or a wrong index fails."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------- RNG --------
+def test_mt_stream_matches_reference_golden():
+    g = np.load(os.path.join(GOLDEN, "mt_ref.npz"))
+    for key in g.files:
+        seed = int(key.split("_")[1])
+        assert np.array_equal(oracle.mt_stream(seed, 2000), g[key]), key
+
+
+def test_mt_stream_matches_reference_build_when_present():
+    ref = oracle.ref_mt_stream(2013, 3000)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    assert np.array_equal(oracle.mt_stream(2013, 3000), ref)
+
+
+def test_mt19937_published_kat():
+    # Matsumoto & Nishimura 1998 mt19937.c, sgenrand(4357)
+    assert list(oracle.mt_stream(4357, 5)) == [3510405877, 4290933890, 2191955339, 564929546, 152112058]
+
+
+def test_random_float_is_24bit():
+    L = oracle.lib()
+    L.orc_mt_seed32(2013)
+    vals = np.array([L.orc_mt_rand() for _ in range(1000)], np.float64)
+    raw = oracle.mt_stream(2013, 1000)
+    assert np.array_equal(vals, (raw & 0xFFFFFF).astype(np.float64) / 2 ** 24)
+    assert vals.max() < 1.0 and vals.min() >= 0.0
+
+
+# ---------------------------------------------------- ray / box / triangle ----
+def test_ray_constructor_normalises_and_clamps():
+    r = oracle.ray_make([1, 2, 3], [0, 0, 5])
+    assert np.array_equal(r[:3], [1, 2, 3]) and np.array_equal(r[3:], [0, 0, 1])
+    r = oracle.ray_make([0, 0, 0], [0, 0, 0])          # NaN direction becomes 0
+    assert np.array_equal(r[3:], [0, 0, 0])
+
+
+def test_box_hit_closed_forms():
+    ray = oracle.ray_make([-5, 0.5, 0.5], [1, 0, 0])
+    assert oracle.box_hit([0, 0, 0], [1, 1, 1], ray, 1e-3, 1e6)
+    assert not oracle.box_hit([0, 0, 0], [1, 1, 1], ray, 1e-3, 4.0)     # box starts at t = 5
+    assert oracle.box_hit([0, 0, 0], [1, 1, 1], ray, 1e-3, 5.0)         # touching counts
+    assert not oracle.box_hit([0, 2, 0], [1, 3, 1], ray, 1e-3, 1e6)     # off to the side
+    inside = oracle.ray_make([0.5, 0.5, 0.5], [0, 1, 0])
+    assert oracle.box_hit([0, 0, 0], [1, 1, 1], inside, 1e-3, 1e6)
+    behind = oracle.ray_make([5, 0.5, 0.5], [1, 0, 0])
+    assert not oracle.box_hit([0, 0, 0], [1, 1, 1], behind, 1e-3, 1e6)
+
+
+def test_triangle_closed_forms():
+    p0, p1, p2 = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    h, t, b, g = oracle.tri_hit(p0, p1, p2, oracle.ray_make([0.25, 0.5, 2], [0, 0, -1]), 1e-3, 1e6)
+    assert h and t == 2.0 and b == 0.25 and g == 0.5           # beta toward p1, gamma toward p2
+    h, t, b, g = oracle.tri_hit(p0, p1, p2, oracle.ray_make([0.5, 0.0, 1], [0, 0, -1]), 1e-3, 1e6)
+    assert h and g == 0.0                                     # edge: inclusive
+    h, *_ = oracle.tri_hit(p0, p1, p2, oracle.ray_make([0, 0, 1], [0, 0, -1]), 1e-3, 1e6)
+    assert h                                                  # vertex
+    h, *_ = oracle.tri_hit(p0, p1, p2, oracle.ray_make([0.6, 0.6, 1], [0, 0, -1]), 1e-3, 1e6)
+    assert not h                                              # beta + gamma > 1
+    h, *_ = oracle.tri_hit(p0, p1, p2, oracle.ray_make([0.2, 0.2, 1], [0, 0, -1]), 1e-3, 0.5)
+    assert not h                                              # beyond tmax
+    # |det| < 1e-6 is rejected in absolute terms: a geometrically hit tiny triangle
+    s = 5e-4
+    h, *_ = oracle.tri_hit([0, 0, 0], [s, 0, 0], [0, s, 0], oracle.ray_make([s / 4, s / 4, 1], [0, 0, -1]), 1e-3, 1e6)
+    assert not h
+    s = 2e-3
+    h, *_ = oracle.tri_hit([0, 0, 0], [s, 0, 0], [0, s, 0], oracle.ray_make([s / 4, s / 4, 1], [0, 0, -1]), 1e-3, 1e6)
+    assert h
+    # parallel ray
+    h, *_ = oracle.tri_hit(p0, p1, p2, oracle.ray_make([-1, 0.2, 0], [1, 0, 0]), 1e-3, 1e6)
+    assert not h
+
+
+def test_triangle_random_against_float64_solve():
+    rng = np.random.default_rng(0)
+    agree = 0
+    for _ in range(400):
+        P = rng.uniform(-10, 10, (3, 3)).astype(np.float32)
+        o = rng.uniform(-10, 10, 3).astype(np.float32)
+        ray = oracle.ray_make(o, (P.mean(0) - o) + rng.normal(0, 2, 3))
+        h, t, b, g = oracle.tri_hit(P[0], P[1], P[2], ray, 1e-3, 1e6)
+        d = ray[3:].astype(np.float64)
+        M = np.stack([P[1] - P[0], P[2] - P[0], -d], 1).astype(np.float64)
+        bb, gg, tt = np.linalg.solve(M, o.astype(np.float64) - P[0])
+        inside = bb >= 0 and gg >= 0 and bb + gg <= 1 and tt >= 1e-3
+        margin = min(abs(bb), abs(gg), abs(1 - bb - gg))
+        if margin > 1e-4:
+            assert h == inside
+            agree += 1
+            if h:
+                assert abs(t - tt) <= 1e-4 * max(1, abs(tt)) and abs(b - bb) < 1e-4 and abs(g - gg) < 1e-4
+    assert agree > 300
+
+
+# ----------------------------------------------------------- Morton / split --
+def test_morton_kats():
+    assert oracle.morton3d(0, 0, 0) == 0
+    assert oracle.morton3d(1, 1, 1) == 0x3FFFFFFF                 # clamped to 1023 on every axis
+    assert oracle.morton3d(1, 0, 0) == 0x24924924                 # x owns bit 3k+2
+    assert oracle.morton3d(0, 0, 1) == 0x09249249
+    assert oracle.morton3d(-3, 0.5, 2) == oracle.morton3d(0, 0.5, 1)
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        x, y, z = rng.random(3).astype(np.float32)
+        xi, yi, zi = (min(max(int(np.float32(v) * np.float32(1024.0)), 0), 1023) for v in (x, y, z))
+        code = 0
+        for k in range(10):
+            code |= ((xi >> k) & 1) << (3 * k + 2) | ((yi >> k) & 1) << (3 * k + 1) | ((zi >> k) & 1) << (3 * k)
+        assert oracle.morton3d(float(x), float(y), float(z)) == code
+
+
+def _split_bruteforce(codes, first, last):
+    fc, lc = int(codes[first]), int(codes[last])
+    if fc == lc:
+        return (first + last) >> 1
+    clz = lambda v: 32 - int(v).bit_length()
+    cp = clz(fc ^ lc)
+    best = first
+    for i in range(first, last):
+        if clz(fc ^ int(codes[i])) > cp:
+            best = i
+    return best
+
+
+def test_find_split_against_bruteforce():
+    rng = np.random.default_rng(2)
+    for trial in range(200):
+        n = int(rng.integers(2, 60))
+        codes = np.sort(rng.integers(0, 1 << (4 if trial % 3 == 0 else 30), n)).astype(np.uint32)
+        first = int(rng.integers(0, n - 1))
+        last = int(rng.integers(first + 1, n))
+        assert oracle.find_split(codes, first, last) == _split_bruteforce(codes, first, last)
+
+
+# --------------------------------------------------------------- BVH -----------
+def _random_rays(sc, n, seed):
+    rng = np.random.default_rng(seed)
+    lo, hi = sc.positions.reshape(-1, 3).min(0), sc.positions.reshape(-1, 3).max(0)
+    o = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    return np.stack([oracle.ray_make(o[i], d[i]) for i in range(n)])
+
+
+@pytest.mark.parametrize("which", ["cornell", "small_room"])
+def test_bvh_closest_matches_bruteforce(which, request):
+    sc = request.getfixturevalue(which)
+    os_ = oracle.OrcScene(sc)
+    rays = _random_rays(sc, 3000, 5)
+    t, b, g, tri = os_.trace_closest(rays)
+    tb, trib = os_.trace_brute(rays)
+    assert np.array_equal(tri >= 0, trib >= 0)
+    # the same triangle and t except on grazing ties: shared edges/vertices hit at
+    # the same t (visit order decides) and coplanar overlaps, where the BVH's slab
+    # test against the shrunken tmax may prune a triangle 1 ulp nearer (the
+    # reference's BVH prunes identically; only brute force has no boxes)
+    diff = np.nonzero((tri != trib) | (t.view(np.uint32) != tb.view(np.uint32)))[0]
+    assert len(diff) <= 0.005 * len(rays)
+    for i in diff:
+        h1, t1, _, _ = oracle.tri_hit(*sc.positions[tri[i]].reshape(3, 3), rays[i], 1e-3, 1e6)
+        h2, t2, _, _ = oracle.tri_hit(*sc.positions[trib[i]].reshape(3, 3), rays[i], 1e-3, 1e6)
+        assert h1 and h2 and abs(t1 - t2) <= 1e-5 * t2
+
+
+@pytest.mark.parametrize("which", ["cornell", "small_room"])
+def test_any_hit_and_segment_consistent_with_closest(which, request):
+    sc = request.getfixturevalue(which)
+    os_ = oracle.OrcScene(sc)
+    rays = _random_rays(sc, 2000, 6)
+    t, b, g, tri = os_.trace_closest(rays)
+    for tmax in (1e6, 100.0, 10.0):
+        a = os_.trace_any(rays, 1e-3, tmax)
+        assert np.array_equal(a.astype(bool), (tri >= 0) & (t <= tmax))
+    # segments p -> q: blocked iff the closest hit along the normalised p->q is before |q-p| - tmin
+    rng = np.random.default_rng(7)
+    lo, hi = sc.positions.reshape(-1, 3).min(0), sc.positions.reshape(-1, 3).max(0)
+    p = (lo + (hi - lo) * rng.random((1500, 3))).astype(np.float32)
+    q = (lo + (hi - lo) * rng.random((1500, 3))).astype(np.float32)
+    blocked = os_.trace_segments(p, q)
+    rays = np.stack([oracle.ray_make(p[i], q[i] - p[i]) for i in range(len(p))])
+    for i in range(len(p)):
+        L = np.float32(np.sqrt(np.float32((q[i] - p[i]) @ (q[i] - p[i])))) - np.float32(1e-3)
+        tt, _, _, tr = os_.trace_closest(rays[i:i + 1], 1e-3, float(L))
+        assert bool(blocked[i]) == bool(tr[0] >= 0)
+
+
+def test_bvh_structure_invariants(small_room):
+    os_ = oracle.OrcScene(small_room)
+    st, sm, ch, rg, bx = os_.export_bvh()
+    n = small_room.ntri
+    assert sorted(st.tolist()) == list(range(n))
+    assert np.all(np.diff(sm.astype(np.int64)) >= 0)
+    # stable: equal codes keep the original triangle order
+    eq = np.nonzero(np.diff(sm.astype(np.int64)) == 0)[0]
+    assert np.all(st[eq] < st[eq + 1])
+    assert len(ch) == 2 * n - 1
+    leaves = ch[:, 0] < 0
+    assert leaves.sum() == n and np.all(rg[leaves, 0] == rg[leaves, 1])
+    inner = np.nonzero(~leaves)[0]
+    for i in inner[:2000]:
+        l, r = ch[i]
+        assert rg[l, 0] == rg[i, 0] and rg[r, 1] == rg[i, 1] and rg[l, 1] + 1 == rg[r, 0]
+        assert np.all(bx[i, :3] <= np.minimum(bx[l, :3], bx[r, :3])) and np.all(bx[i, 3:] >= np.maximum(bx[l, 3:], bx[r, 3:]))
+    assert os_.depth() <= 30 + math.ceil(math.log2(n)) + 1
+
+
+# ------------------------------------------------------------- receivers ---------
+def test_receivers_orthonormal_and_facing(cornell):
+    from renzoku_amd import scenes
+    os_ = oracle.OrcScene(cornell)
+    px, py = scenes.pixel_grid(256, 256)
+    rays = oracle.camera_rays(cornell.camera, px, py)
+    t, b, g, tri = os_.trace_closest(rays)
+    recv, m, l = os_.fill_receivers(rays, t, b, g, tri)
+    ok = m >= 0
+    U, V, N = recv[ok, 12:15], recv[ok, 15:18], recv[ok, 18:21]
+    for A in (U, V, N):
+        assert np.allclose((A * A).sum(1), 1, atol=1e-5)
+    assert np.abs((U * V).sum(1)).max() < 1e-5 and np.abs((U * N).sum(1)).max() < 1e-5
+    # two-sided surfaces face the viewer: shading normal . wo >= 0
+    assert ((recv[ok, 6:9] * recv[ok, 9:12]).sum(1) >= -1e-6).all()
+    assert np.allclose(recv[ok, 18:21], recv[ok, 6:9], atol=1e-6)
+    # p = o + t d
+    assert np.allclose(recv[ok, :3], rays[ok, :3] + rays[ok, 3:] * t[ok, None], atol=1e-3)
+
+
+# ---------------------------------------------------------------- shading -----------
+def _onb(n):
+    n = np.asarray(n, np.float32) / np.linalg.norm(n)
+    a = np.array([1, 0, 0], np.float32) if abs(n[0]) < 0.9 else np.array([0, 1, 0], np.float32)
+    u = np.cross(a, n); u /= np.linalg.norm(u)
+    v = np.cross(n, u)
+    return np.concatenate([u, v, n]).astype(np.float32)
+
+
+def test_lambert_closed_form():
+    mat = np.array([0, 0.5, 0.25, 0.125] + [0] * 8, np.float32)
+    uvn = _onb([0, 0, 1])
+    f = oracle.bsdf_eval(mat, uvn, [0, 0.6, 0.8], [0.8, 0, 0.6])
+    assert np.allclose(f, np.array([0.5, 0.25, 0.125]) / math.pi, rtol=1e-6)
+    assert np.all(oracle.bsdf_eval(mat, uvn, [0, 0.6, 0.8], [0.8, 0, -0.6]) == 0)
+    assert np.all(oracle.bsdf_eval(mat, uvn, [0, 0.6, -0.8], [0.8, 0, 0.6]) == 0)
+
+
+def test_phong_peak_and_symmetry():
+    n_exp = 37.0
+    mat = np.array([1, 0.2, 0.2, 0.2, 0.3, 0.3, 0.3, n_exp] + [0] * 4, np.float32)
+    uvn = _onb([0, 0, 1])
+    wi = np.array([0.6, 0, 0.8], np.float32)
+    wo = np.array([-0.6, 0, 0.8], np.float32)         # mirror of wi: cos_alpha = 1
+    f = oracle.bsdf_eval(mat, uvn, wo, wi)
+    assert np.allclose(f, 0.2 / math.pi + 0.3 * (n_exp + 2) / (2 * math.pi), rtol=1e-5)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        a, b = rng.normal(size=3), rng.normal(size=3)
+        a[2], b[2] = abs(a[2]) + 0.1, abs(b[2]) + 0.1
+        a, b = (a / np.linalg.norm(a)).astype(np.float32), (b / np.linalg.norm(b)).astype(np.float32)
+        ca = 2 * a[2] * b[2] - a @ b
+        expect = 0.2 / math.pi + (0.3 * (n_exp + 2) / (2 * math.pi) * ca ** n_exp if ca > 0 else 0)
+        assert np.allclose(oracle.bsdf_eval(mat, uvn, a, b), expect, rtol=2e-4, atol=1e-7)
+
+
+def test_ward_normal_incidence_and_reciprocity():
+    ax = ay = 0.2
+    mat = np.array([2, 0.1, 0.2, 0.3, 0.4, 0.4, 0.4, ax, ay] + [0] * 3, np.float32)
+    uvn = _onb([0, 0, 1])
+    f = oracle.bsdf_eval(mat, uvn, [0, 0, 1], [0, 0, 1])
+    expect = np.array([0.1, 0.2, 0.3]) / math.pi + 0.4 / (4 * math.pi * ax * ay)
+    assert np.allclose(f, expect, rtol=1e-5)
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        a, b = rng.normal(size=3), rng.normal(size=3)
+        a[2], b[2] = abs(a[2]) + 0.1, abs(b[2]) + 0.1
+        a, b = (a / np.linalg.norm(a)).astype(np.float32), (b / np.linalg.norm(b)).astype(np.float32)
+        assert np.array_equal(oracle.bsdf_eval(mat, uvn, a, b), oracle.bsdf_eval(mat, uvn, b, a))
+        h = a + b
+        ex = math.exp(-(h[0] ** 2 / ax ** 2 + h[1] ** 2 / ay ** 2) / h[2] ** 2)
+        expect = np.array([0.1, 0.2, 0.3]) / math.pi + 0.4 / (4 * math.pi * ax * ay) / math.sqrt(a[2] * b[2]) * ex
+        assert np.allclose(oracle.bsdf_eval(mat, uvn, a, b), expect, rtol=3e-4, atol=1e-8)
+
+
+def test_form_factor_closed_form():
+    for d in (0.5, 3.0, 100.0):
+        g = oracle.form_factor_abs([0, 0, 0], [0, 0, 1], [0, 0, d], [0, 0, -1])
+        assert math.isclose(g, 1.0 / d ** 2, rel_tol=1e-6)
+    # 60 degrees off both normals: cos^2 / d^2
+    g = oracle.form_factor_abs([0, 0, 0], [0, math.sin(math.pi / 3), 0.5], [0, 0, 2], [0, -math.sin(math.pi / 3), -0.5])
+    assert math.isclose(g, 0.25 / 4, rel_tol=1e-5)
+
+
+def test_clamp_term_branches():
+    P, fs, fl = np.array([2, 4, 8], np.float32), np.array([0.5, 0.5, 0.5], np.float32), np.array([0.1, 0.2, 0.4], np.float32)
+    assert np.allclose(oracle.clamp_term(1, 0.01, P, fs, fl, 0.005), fl * fs * 0.005 * P)
+    assert np.allclose(oracle.clamp_term(1, 0.01, P, fs, fl, 0.5), fl * fs * 0.01 * P)       # clamped to tau
+    assert np.allclose(oracle.clamp_term(0, 0.01, P, fs, fl, 0.5), P * 0.5 * fs * fl)
+    r = oracle.clamp_term(3, 0.1, P, fs, fl, 0.5)
+    assert math.isclose(float(np.linalg.norm(r)), 0.1, rel_tol=1e-6)
+    refl = 0.5 * fs * fl
+    r = oracle.clamp_term(2, 0.01, P, fs, fl, 0.5)
+    assert np.allclose(r, 0.01 / np.linalg.norm(refl) * refl * P, rtol=1e-6)
+
+
+# -------------------------------------------------------------- clustering ----------
+def _random_R(rows, cols, seed):
+    rng = np.random.default_rng(seed)
+    R = rng.gamma(0.5, 1.0, (cols, rows, 3)).astype(np.float32)
+    R[rng.random((cols, rows)) < 0.3] = 0
+    return R
+
+
+def test_alphas_closed_form():
+    R = _random_R(24, 300, 1)
+    m = np.sqrt((R.astype(np.float64) ** 2).sum(2))          # (cols, rows) magnitudes
+    norms = np.sqrt((m ** 2).sum(1))
+    expect = norms * norms.sum() - m @ m.sum(0)
+    a = oracle.alphas(R)
+    assert np.allclose(a, expect, rtol=1e-4, atol=1e-3 * np.abs(expect).max() * 1e-3)
+    # alpha_i = sum_j (||m_i|| ||m_j|| - m_i . m_j) >= 0 by Cauchy-Schwarz
+    assert (a >= -1e-3 * np.abs(a).max()).all()
+    mags = oracle.column_magnitudes(R)
+    assert np.allclose(mags, np.sqrt((R.astype(np.float64) ** 2).reshape(300, -1).sum(1)), rtol=1e-5)
+
+
+def test_cluster_assign_bruteforce():
+    R = _random_R(16, 400, 2)
+    rng = np.random.default_rng(3)
+    centers = rng.choice(400, 25, replace=False)
+    w = rng.uniform(0.5, 5, 25).astype(np.float32)
+    got = oracle.cluster_assign(R, centers, w)
+    M = R.reshape(400, -1).astype(np.float64)
+    K = M[centers] * w[:, None]
+    dist = np.abs(np.linalg.norm(K, axis=1)[None, :] * np.linalg.norm(M, axis=1)[:, None] - M @ K.T)
+    order = np.sort(dist, 1)
+    clear = (order[:, 1] - order[:, 0]) > 1e-3 * (order[:, 1] + 1e-9)
+    assert clear.mean() > 0.8
+    assert np.array_equal(got[clear], dist.argmin(1)[clear])
+
+
+def test_discrete_pdf_semantics():
+    ok, pdf, cdf = oracle.pdf_build([1, 0, 3, 4])
+    assert ok and cdf[-1] == 1.0 and np.all(np.diff(cdf) >= 0)
+    assert oracle.pdf_sample(pdf, cdf, 0.0) == (0, pytest.approx(0.125))
+    assert oracle.pdf_sample(pdf, cdf, 0.125)[0] == 2          # strict <: equal to cdf[0] moves on, skips the zero
+    assert oracle.pdf_sample(pdf, cdf, 0.9999)[0] == 3
+    ok, pdf, cdf = oracle.pdf_build([0, 0])
+    assert not ok and oracle.pdf_sample(pdf, cdf, 0.0)[0] == -1
+
+
+def test_cluster_by_sampling_invariants():
+    R = _random_R(16, 600, 4)
+    out = oracle.cluster_by_sampling(R, 40, 2013)
+    c = out["centers"]
+    assert len(c) == 40 and len(set(c.tolist())) == 40
+    assert (out["center_w"] > 0).all()
+    assert out["assign"].min() >= 0 and out["assign"].max() < 40
+    for rep in out["reps"]:
+        assert out["assign"][rep] >= 0
+    # same seed, same result; another seed differs
+    again = oracle.cluster_by_sampling(R, 40, 2013)
+    assert np.array_equal(again["centers"], c) and np.array_equal(again["reps"], out["reps"])
+    assert not np.array_equal(oracle.cluster_by_sampling(R, 40, 7)["centers"], c)
+
+
+# ------------------------------------------------------------ guiding maps ---------
+def test_concentric_round_trip_and_uniformity():
+    rng = np.random.default_rng(5)
+    zs = []
+    for _ in range(500):
+        s = rng.random(2).astype(np.float32) * 0.98 + 0.01
+        w = oracle.square_to_direction(s)
+        assert abs(np.linalg.norm(w) - 1) < 1e-5 and w[2] >= 0
+        s2 = oracle.direction_to_square(w)
+        assert np.allclose(s, s2, atol=2e-5)
+        zs.append(w[2])
+    # uniform on the square -> uniform on the hemisphere -> z ~ U[0, 1]
+    zs = np.sort(zs)
+    assert np.abs(zs - np.linspace(0, 1, len(zs))).max() < 0.08
+
+
+def test_build_map_invariants():
+    rng = np.random.default_rng(6)
+    n, res, eps = 50, 32, 1e-3
+    wi = rng.normal(size=(n, 3)).astype(np.float32)
+    wi[:, 2] = np.abs(wi[:, 2])
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    upper = (rng.random(n) < 0.8).astype(np.int32)
+    w = rng.uniform(1, 3, n).astype(np.float32)
+    rgb = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    uvn = _onb([0, 0, 1])
+    out = oracle.build_map(res, wi, upper, w, rgb, uvn, eps, 1)
+    lum = np.linalg.norm(w[:, None] * rgb, axis=1)[upper > 0].sum()
+    assert math.isclose(out["total"], lum * (1 + eps * res * res), rel_tol=1e-4)
+    assert out["row_cdf"][-1] == 1.0 and np.allclose(out["col_cdf"][:, -1], 1.0, atol=1e-6)
+    # every contribution lands in the cell of its concentric-square coordinate
+    k = int(np.nonzero(upper)[0][0])
+    local = np.array([uvn[0:3] @ wi[k], uvn[3:6] @ wi[k], uvn[6:9] @ wi[k]], np.float32)
+    s = oracle.direction_to_square(local)
+    x, y = min(int(s[0] * res), res - 1), min(int(s[1] * res), res - 1)
+    assert out["map"][y, x] >= np.linalg.norm(w[k] * rgb[k]) * 0.999
+    gated = oracle.build_map(res, wi, upper, w, rgb, uvn, eps, 0)
+    assert gated["total"] == 0 and not gated["ok"]
+
+
+# --------------------------------------------------------------- gather --------------
+def test_gather_point_matches_composition(cornell):
+    from renzoku_amd import scenes, vplgen
+    os_ = oracle.OrcScene(cornell)
+    px, py = scenes.pixel_grid(256, 256)
+    sel = np.arange(0, 65536, 997)
+    rays = oracle.camera_rays(cornell.camera, px[sel], py[sel])
+    t, b, g, tri = os_.trace_closest(rays)
+    recv, m, _ = os_.fill_receivers(rays, t, b, g, tri)
+    vpl, vmat, ne = vplgen.generate_vpls(cornell, os_.trace_closest, os_.fill_receivers, 64, seed=9, batch=256)
+    for i in range(len(sel)):
+        if m[i] < 0:
+            continue
+        for k in range(0, 64, 9):
+            out, vis = os_.gather_point(recv[i], m[i], vpl[k], vmat[k], 1, 0.01, 1e-3)
+            blocked = os_.trace_segments(recv[i:i + 1, :3], vpl[k:k + 1, :3])[0]
+            assert vis == (not blocked)
+            if not vis:
+                assert np.all(out == 0)
+                continue
+            p, pn, q, qn = recv[i, :3], recv[i, 6:9], vpl[k, :3], vpl[k, 3:6]
+            G = oracle.form_factor_abs(p, pn, q, qn)
+            wi = (q - p) / np.linalg.norm(q - p)
+            fs = oracle.bsdf_eval(cornell.materials[m[i]], recv[i, 12:21], recv[i, 9:12], wi)
+            fl = oracle.bsdf_eval(cornell.materials[vmat[k]], vpl[k, 6:15], vpl[k, 15:18], -wi)
+            expect = fl * fs * min(G, 0.01) * vpl[k, 18:21]
+            assert np.allclose(out, expect, rtol=1e-4, atol=1e-12)
+    img = os_.gather(recv, m, vpl, vmat, 1, 0.01, 1e-3, ne)
+    acc = np.zeros((len(sel), 3))
+    for i in range(len(sel)):
+        for k in range(64):
+            acc[i] += os_.gather_point(recv[i], m[i], vpl[k], vmat[k], 1, 0.01, 1e-3)[0]
+    assert rel_l2(img, acc / ne) < 1e-5
+
+
+def test_gather_threads_deterministic(cornell):
+    from renzoku_amd import scenes, vplgen
+    os_ = oracle.OrcScene(cornell)
+    px, py = scenes.pixel_grid(256, 256)
+    sel = np.arange(0, 65536, 61)
+    rays = oracle.camera_rays(cornell.camera, px[sel], py[sel])
+    t, b, g, tri = os_.trace_closest(rays)
+    recv, m, _ = os_.fill_receivers(rays, t, b, g, tri)
+    vpl, vmat, ne = vplgen.generate_vpls(cornell, os_.trace_closest, os_.fill_receivers, 128, seed=2, batch=512)
+    a = os_.gather(recv, m, vpl, vmat, 1, 0.01, 1e-3, ne, nthreads=1)
+    b2 = os_.gather(recv, m, vpl, vmat, 1, 0.01, 1e-3, ne, nthreads=6)
+    assert np.array_equal(a, b2)
