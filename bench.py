@@ -32,7 +32,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-RAY_IN_BYTES, SHADOW_OUT_BYTES, NODE_BYTES, TRI_BYTES = 32, 4, 64, 48
+# algorithmic bytes per shadow segment: ray in, result out, every 4-wide node
+# fetched (128 B: four child boxes + refs) and every triangle record tested
+RAY_IN_BYTES, SHADOW_OUT_BYTES, NODE4_BYTES, TRI_BYTES = 32, 4, 128, 48
 
 
 class HipEvents:
@@ -102,6 +104,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tris", type=int, default=200_000)
     ap.add_argument("--vpls", type=int, default=65_536)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device once (a known-size transfer for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design "
@@ -158,7 +163,7 @@ def main():
 
     shadow_rays = work["reduced_pairs"] + work["gather_pairs"]
     value = shadow_rays / (ms_per_step / 1e3) / 1e6
-    gbytes = (work["gather_pairs"] * (RAY_IN_BYTES + SHADOW_OUT_BYTES) + work["gather_nodes"] * NODE_BYTES +
+    gbytes = (work["gather_pairs"] * (RAY_IN_BYTES + SHADOW_OUT_BYTES) + work["gather_nodes"] * NODE4_BYTES +
               work["gather_tris"] * TRI_BYTES)
     achieved = gbytes / (g_ms / 1e3) / 1e9
     out = {
@@ -194,6 +199,19 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sc, ls, vpl, vmat, reps, rep_w, params)
         log(f"[bench] CPU baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        lib = ctx.lib
+        ctx.check(lib.rzk_device_alloc(ctx.h, nb, C.byref(src)), "alloc")
+        ctx.check(lib.rzk_device_alloc(ctx.h, nb, C.byref(dst)), "alloc")
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMemcpyDtoD.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        assert hip.hipMemcpyDtoD(dst, src, nb) == 0
+        ctx.synchronize()
+        log(f"[bench] copy calibration: {args.copy_calibration} MiB device-to-device")
+        lib.rzk_device_free(ctx.h, src)
+        lib.rzk_device_free(ctx.h, dst)
 
 
 if __name__ == "__main__":

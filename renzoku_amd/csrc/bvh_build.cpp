@@ -126,6 +126,9 @@ void bvh_build_host(int ntri, const float* pos, BvhHost& out) {
     if (nn == 1) {
         out.root_ref = ~0;
         out.num_interior = 0;
+        out.root4_ref = ~0;
+        out.depth4 = 0;
+        out.stack4 = 1;
         return;
     }
     std::vector<int> newidx(nn, -1);
@@ -158,4 +161,79 @@ void bvh_build_host(int ntri, const float* pos, BvhHost& out) {
         out.nodes[j] = nd;
     }
     out.root_ref = 0;
+
+    // 4-wide collapse for any-hit traversal: a 4-wide node holds the up-to-4
+    // descendants reached by repeatedly opening the interior child with the
+    // largest surface area.  Every triangle keeps its own box as the box of
+    // the slot it sits in, so the set of triangles an any-hit query tests is
+    // exactly the binary tree's (a slab test passing for a box implies it
+    // passes for every enclosing box: the rounded slab bounds are monotone).
+    auto area = [&](int b) {
+        const float* B = &box[6 * b];
+        float dx = B[3] - B[0], dy = B[4] - B[1], dz = B[5] - B[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    auto open4 = [&](int b, int c[4]) {
+        int n = 2;
+        c[0] = child[2 * b];
+        c[1] = child[2 * b + 1];
+        while (n < 4) {
+            int best = -1;
+            float ba = -1.f;
+            for (int i = 0; i < n; ++i)
+                if (child[2 * c[i]] >= 0 && area(c[i]) > ba) { ba = area(c[i]); best = i; }
+            if (best < 0) break;
+            int x = c[best];
+            for (int j = n; j > best + 1; --j) c[j] = c[j - 1];
+            c[best] = child[2 * x];
+            c[best + 1] = child[2 * x + 1];
+            ++n;
+        }
+        return n;
+    };
+    std::vector<int> idx4(nn, -1), order4;
+    std::vector<std::pair<int, int>> st4;
+    st4.push_back({0, 1});
+    int depth4 = 0;
+    while (!st4.empty()) {
+        auto [b, d] = st4.back();
+        st4.pop_back();
+        idx4[b] = (int)order4.size();
+        order4.push_back(b);
+        depth4 = std::max(depth4, d);
+        int c[4];
+        int n = open4(b, c);
+        for (int i = n - 1; i >= 0; --i)
+            if (child[2 * c[i]] >= 0) st4.push_back({c[i], d + 1});
+    }
+    out.nodes4.resize(order4.size());
+    for (size_t j = 0; j < order4.size(); ++j) {
+        int c[4];
+        int n = open4(order4[j], c);
+        float lo[3][4], hi[3][4];
+        int rf[4];
+        for (int i = 0; i < 4; ++i) {
+            if (i < n) {
+                const float* B = &box[6 * c[i]];
+                for (int a = 0; a < 3; ++a) { lo[a][i] = B[a]; hi[a][i] = B[3 + a]; }
+                rf[i] = child[2 * c[i]] < 0 ? ~range[2 * c[i]] : idx4[c[i]];
+            } else {
+                for (int a = 0; a < 3; ++a) { lo[a][i] = 0.f; hi[a][i] = 0.f; }
+                rf[i] = kEmptyRef;
+            }
+        }
+        Bvh4NodeGpu nd;
+        nd.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        nd.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        nd.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        nd.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        nd.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        nd.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        nd.ref = make_int4(rf[0], rf[1], rf[2], rf[3]);
+        nd.pad = make_int4(0, 0, 0, 0);
+        out.nodes4[j] = nd;
+    }
+    out.root4_ref = 0;
+    out.depth4 = depth4;
+    out.stack4 = 3 * depth4 + 1;
 }

@@ -3,6 +3,7 @@
 // codes and records the message on the context.
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 
@@ -19,7 +20,7 @@ void launch_gather_fused(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
                          int64_t nvpl, int clamp_mode, float clamp, float tmin, float n_emitted,
                          float* d_out, int* d_stats);
 void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_alphas,
-                        float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0);
+                        float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0, float* d_mT);
 void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl, const int* d_centers,
                         const float* d_weights, const float* d_mags, float* d_K, float* d_Knorm,
                         int* d_assign);
@@ -76,6 +77,7 @@ static int guarded(rzk_ctx* ctx, F&& f) {
 SceneView scene_view(const rzk_scene* s) {
     SceneView v;
     v.nodes = s->d_nodes;
+    v.nodes4 = s->d_nodes4;
     v.tris = s->d_tris;
     v.tri_n = s->d_tri_n;
     v.tri_tan = s->d_tri_tan;
@@ -83,6 +85,10 @@ SceneView scene_view(const rzk_scene* s) {
     v.mat = s->d_mat;
     v.root_ref = s->bvh.root_ref;
     v.depth = s->bvh.depth > 0 ? s->bvh.depth : 1;
+    v.root4_ref = s->bvh.root4_ref;
+    v.stack4 = s->bvh.stack4;
+    v.spill = nullptr;
+    v.spill_pitch = 0;
     for (int i = 0; i < 3; ++i) { v.rbmin[i] = s->bvh.root_box[i]; v.rbmax[i] = s->bvh.root_box[3 + i]; }
     return v;
 }
@@ -96,6 +102,13 @@ void* ctx_scratch(rzk_ctx* ctx, size_t bytes) {
         ctx->scratch_bytes = bytes;
     }
     return ctx->scratch;
+}
+
+void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads) {
+    int64_t extra = sv.stack4 - kLdsStack;
+    if (extra <= 0 || threads <= 0) { sv.spill = nullptr; sv.spill_pitch = 0; return; }
+    sv.spill = static_cast<int*>(ctx_scratch(ctx, sizeof(int) * (size_t)extra * (size_t)threads));
+    sv.spill_pitch = threads;
 }
 
 extern "C" {
@@ -211,6 +224,7 @@ int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
             if (src) RZK_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
         };
         up((void**)&s->d_nodes, s->bvh.nodes.data(), sizeof(BvhNodeGpu) * s->bvh.nodes.size());
+        up((void**)&s->d_nodes4, s->bvh.nodes4.data(), sizeof(Bvh4NodeGpu) * s->bvh.nodes4.size());
         up((void**)&s->d_tris, s->bvh.tris.data(), sizeof(TriGpu) * s->bvh.tris.size());
         up((void**)&s->d_tri_n, tn.data(), sizeof(float4) * n);
         up((void**)&s->d_tri_tan, tt.data(), sizeof(float4) * n);
@@ -223,7 +237,7 @@ int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
 int rzk_scene_destroy(rzk_scene* s) {
     if (!s) return RZK_OK;
     (void)hipSetDevice(s->ctx->device);
-    void* ptrs[] = {s->d_nodes, s->d_tris, s->d_tri_n, s->d_tri_tan, s->d_surf, s->d_mat};
+    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_tris, s->d_tri_n, s->d_tri_tan, s->d_surf, s->d_mat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -236,8 +250,9 @@ int rzk_scene_get_info(const rzk_scene* s, rzk_scene_info* out) {
     out->num_interior_nodes = s->bvh.num_interior;
     out->depth = s->bvh.depth;
     for (int i = 0; i < 6; ++i) out->box[i] = s->bvh.box[i];
-    out->device_bytes = (int64_t)sizeof(BvhNodeGpu) * s->bvh.num_interior + (int64_t)sizeof(TriGpu) * s->bvh.ntri +
-                        (int64_t)2 * sizeof(float4) * s->bvh.ntri;
+    out->device_bytes = (int64_t)sizeof(BvhNodeGpu) * s->bvh.num_interior +
+                        (int64_t)sizeof(Bvh4NodeGpu) * (int64_t)s->bvh.nodes4.size() +
+                        (int64_t)sizeof(TriGpu) * s->bvh.ntri + (int64_t)2 * sizeof(float4) * s->bvh.ntri;
     return RZK_OK;
 }
 
@@ -412,9 +427,28 @@ int rzk_vpls_create(rzk_ctx* ctx, const float* vpl, const int32_t* vpl_mat, int6
         size_t nb = (size_t)(n > 0 ? n : 1);
         RZK_HIP(hipMalloc((void**)&v->d_vpl, sizeof(float) * RZK_VPL_FLOATS * nb));
         RZK_HIP(hipMalloc((void**)&v->d_mat, sizeof(int) * nb));
+        RZK_HIP(hipMalloc((void**)&v->d_order, sizeof(int) * nb));
         if (n) {
+            // Morton order of the VPL positions: kernels that send a wave of
+            // segments to 64 VPLs at once take them in this order, so the
+            // segments of a wave fan out into a narrow cone
+            v3 lo = ld3(vpl), hi = ld3(vpl);
+            for (int64_t i = 0; i < n; ++i) { lo = vmin(lo, ld3(vpl + RZK_VPL_FLOATS * i)); hi = vmax(hi, ld3(vpl + RZK_VPL_FLOATS * i)); }
+            v3 ext = hi - lo;
+            if (ext.x <= 0.f) ext.x = 1.f;
+            if (ext.y <= 0.f) ext.y = 1.f;
+            if (ext.z <= 0.f) ext.z = 1.f;
+            std::vector<uint64_t> key(n);
+            for (int64_t i = 0; i < n; ++i) {
+                v3 c = (ld3(vpl + RZK_VPL_FLOATS * i) - lo) / ext;
+                key[i] = ((uint64_t)morton3d(c.x, c.y, c.z) << 32) | (uint64_t)i;
+            }
+            std::sort(key.begin(), key.end());
+            std::vector<int> order(n);
+            for (int64_t i = 0; i < n; ++i) order[i] = (int)(key[i] & 0xffffffffu);
             RZK_HIP(hipMemcpy(v->d_vpl, vpl, sizeof(float) * RZK_VPL_FLOATS * n, hipMemcpyHostToDevice));
             RZK_HIP(hipMemcpy(v->d_mat, vpl_mat, sizeof(int) * n, hipMemcpyHostToDevice));
+            RZK_HIP(hipMemcpy(v->d_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
         }
         *out = v.release();
     });
@@ -424,6 +458,7 @@ int rzk_vpls_destroy(rzk_vpls* v) {
     if (!v) return RZK_OK;
     if (v->d_vpl) (void)hipFree(v->d_vpl);
     if (v->d_mat) (void)hipFree(v->d_mat);
+    if (v->d_order) (void)hipFree(v->d_order);
     delete v;
     return RZK_OK;
 }
@@ -484,7 +519,7 @@ int rzk_reduced_matrix(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, cons
         DevBuf<int> dm(rows);
         dv.upload(ctx, recv, RZK_RECV_FLOATS * rows);
         dm.upload(ctx, recv_mat, rows);
-        launch_reduced_matrix(ctx, scene_view(s), dv.p, dm.p, rows, v->d_vpl, v->d_mat, v->n, clamp_mode, clamp, tmin,
+        launch_reduced_matrix(ctx, scene_view(s), dv.p, dm.p, rows, v->d_order, v->d_vpl, v->d_mat, v->n, clamp_mode, clamp, tmin,
                               dR.p, dV.p, nullptr);
         dR.download(ctx, out_R, 3 * n);
         dV.download(ctx, out_V, n);
@@ -496,9 +531,10 @@ int rzk_alphas(rzk_ctx* ctx, int32_t rows, int64_t cols, const float* R, float* 
     return guarded(ctx, [&] {
         RZK_CHECK(rows > 0 && cols >= 0 && (cols == 0 || (R && out_alphas)), RZK_E_INVALID, "bad alpha arguments");
         if (cols == 0) return;
-        DevBuf<float> dR(3 * (size_t)rows * cols), da(cols), dm(cols), dn(cols), du(rows), dvv(cols), dc(1);
+        DevBuf<float> dR(3 * (size_t)rows * cols), da(cols), dm(cols), dn(cols), du(rows), dvv(cols), dc(1),
+            dmT((size_t)rows * cols);
         dR.upload(ctx, R, 3 * (size_t)rows * cols);
-        launch_alphas_full(ctx, rows, cols, dR.p, da.p, dm.p, dn.p, du.p, dvv.p, dc.p);
+        launch_alphas_full(ctx, rows, cols, dR.p, da.p, dm.p, dn.p, du.p, dvv.p, dc.p, dmT.p);
         da.download(ctx, out_alphas, cols);
         if (out_mags) dm.download(ctx, out_mags, cols);
         sync(ctx);
@@ -515,12 +551,12 @@ int rzk_cluster_assign(rzk_ctx* ctx, int32_t rows, int64_t cols, const float* R,
         if (cols == 0) return;
         size_t n3 = 3 * (size_t)rows;
         DevBuf<float> dR(n3 * cols), dw(ncl), dK(n3 * ncl), dKn(ncl), dn(cols), dm(cols), da(cols), du(rows),
-            dvv(cols), dc(1);
+            dvv(cols), dc(1), dmT((size_t)rows * cols);
         DevBuf<int> dc_(ncl), das(cols);
         dR.upload(ctx, R, n3 * cols);
         dw.upload(ctx, weights, ncl);
         dc_.upload(ctx, centers, ncl);
-        launch_alphas_full(ctx, rows, cols, dR.p, da.p, dm.p, dn.p, du.p, dvv.p, dc.p);
+        launch_alphas_full(ctx, rows, cols, dR.p, da.p, dm.p, dn.p, du.p, dvv.p, dc.p, dmT.p);
         launch_assign_full(ctx, rows, cols, dR.p, ncl, dc_.p, dw.p, dm.p, dK.p, dKn.p, das.p);
         das.download(ctx, out_assign, cols);
         sync(ctx);

@@ -38,22 +38,76 @@ __global__ void k_col_norms(int rows, int64_t cols, const float* __restrict__ R,
     mags[j] = sqrtf(s2);
 }
 
-// per row: u_i = Σ_j m_ij, j sequential (matrix_c.h:236-245).  One lane per
-// row; loads are issued eight columns ahead of the dependent adds.
-__global__ void k_row_sums(int rows, int64_t cols, const float* __restrict__ R, float* __restrict__ u) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows) return;
-    float acc = 0.f;
-    int64_t j = 0;
-    for (; j + 8 <= cols; j += 8) {
-        float m[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) m[q] = entry_mag(R + 3 * ((j + q) * rows + i));
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc += m[q];
+// m^T[i][j] = |R(i, j)|: 64 x 64 tiles through LDS so that both the column
+// reads of R and the row writes of m^T are coalesced
+__global__ __launch_bounds__(256) void k_mag_transpose(int rows, int64_t cols, const float* __restrict__ R,
+                                                       float* __restrict__ mT) {
+    __shared__ float tile[64][65];
+    int64_t j0 = (int64_t)blockIdx.x * 64;
+    int i0 = blockIdx.y * 64;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        int jj = e / 64, ii = e % 64;
+        int64_t j = j0 + jj;
+        int i = i0 + ii;
+        tile[jj][ii] = (j < cols && i < rows) ? entry_mag(R + 3 * (j * rows + i)) : 0.f;
     }
-    for (; j < cols; ++j) acc += entry_mag(R + 3 * (j * rows + i));
-    u[i] = acc;
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        int ii = e / 64, jj = e % 64;
+        int64_t j = j0 + jj;
+        int i = i0 + ii;
+        if (j < cols && i < rows) mT[(int64_t)i * cols + j] = tile[jj][ii];
+    }
+}
+
+// out[i] = Σ_j x[i][j] in order j = 0, 1, ... (one float add after another,
+// matrix_c.h:236-245 / light_transport_matrix.cc:99-100).  One wave per row:
+// the whole wave streams 1024-element chunks into LDS, one lane performs the
+// ordered additions.
+__global__ __launch_bounds__(64) void k_ordered_row_sums(int rows, int64_t cols, const float* __restrict__ x,
+                                                         float* __restrict__ out) {
+    __shared__ float4 buf[2][256];
+    int i = blockIdx.x;
+    if (i >= rows) return;
+    const float* row = x + (int64_t)i * cols;
+    int lane = threadIdx.x;
+    float acc = 0.f;
+    int64_t nchunk = (cols + 1023) / 1024;
+    auto stage = [&](int64_t c, int slot) {
+        for (int q = 0; q < 4; ++q) {
+            int64_t e = c * 1024 + (int64_t)(q * 64 + lane) * 4;
+            float4 v;
+            v.x = e < cols ? row[e] : 0.f;
+            v.y = e + 1 < cols ? row[e + 1] : 0.f;
+            v.z = e + 2 < cols ? row[e + 2] : 0.f;
+            v.w = e + 3 < cols ? row[e + 3] : 0.f;
+            buf[slot][q * 64 + lane] = v;
+        }
+    };
+    stage(0, 0);
+    __syncthreads();
+    for (int64_t c = 0; c < nchunk; ++c) {
+        int slot = (int)(c & 1);
+        if (c + 1 < nchunk) stage(c + 1, slot ^ 1);
+        if (lane == 0) {
+            int64_t n = cols - c * 1024 < 1024 ? cols - c * 1024 : 1024;
+            int64_t q = 0;
+            for (; q + 4 <= n; q += 4) {
+                float4 v = buf[slot][q >> 2];
+                acc += v.x;
+                acc += v.y;
+                acc += v.z;
+                acc += v.w;
+            }
+            for (; q < n; ++q) {
+                float4 v = buf[slot][q >> 2];
+                float w = (q & 3) == 0 ? v.x : (q & 3) == 1 ? v.y : (q & 3) == 2 ? v.z : v.w;
+                acc += w;
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) out[i] = acc;
 }
 
 // per column: v_j = Σ_i m_ij * u_i, i sequential (matrix_c.h:224-233 on Rt)
@@ -67,22 +121,7 @@ __global__ void k_col_dot_u(int rows, int64_t cols, const float* __restrict__ R,
     v[j] = acc;
 }
 
-// c0 = Σ_j norm_j (sequential, one lane), then alpha_j = norm_j * c0 - v_j
-__global__ void k_sum_norms(int64_t cols, const float* __restrict__ norms, float* __restrict__ c0) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    float acc = 0.f;
-    int64_t j = 0;
-    for (; j + 8 <= cols; j += 8) {
-        float m[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) m[q] = norms[j + q];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc += m[q];
-    }
-    for (; j < cols; ++j) acc += norms[j];
-    *c0 = acc;
-}
-
+// alpha_j = norm_j * c0 - v_j, c0 = Σ_j norm_j in order (k_ordered_row_sums, one row)
 __global__ void k_alphas(int64_t cols, const float* __restrict__ norms, const float* __restrict__ c0,
                          const float* __restrict__ v, float* __restrict__ alphas) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -204,15 +243,18 @@ using namespace rzk;
 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// alphas and magnitudes of all columns; scratch: u (rows), v (cols), norms (cols), c0 (1)
+// alphas and magnitudes of all columns; scratch: u (rows), v (cols), norms
+// (cols), c0 (1), mT (rows * cols)
 void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_alphas,
-                        float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0) {
+                        float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0, float* d_mT) {
     if (cols <= 0 || rows <= 0) return;
     hipLaunchKernelGGL(k_col_norms, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, rows, cols, d_R, d_norms,
                        d_mags);
-    hipLaunchKernelGGL(k_row_sums, dim3(nblk(rows, 64)), dim3(64), 0, ctx->stream, rows, cols, d_R, d_u);
+    hipLaunchKernelGGL(k_mag_transpose, dim3(nblk(cols, 64), nblk(rows, 64)), dim3(256), 0, ctx->stream, rows, cols,
+                       d_R, d_mT);
+    hipLaunchKernelGGL(k_ordered_row_sums, dim3(rows), dim3(64), 0, ctx->stream, rows, cols, d_mT, d_u);
     hipLaunchKernelGGL(k_col_dot_u, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, rows, cols, d_R, d_u, d_v);
-    hipLaunchKernelGGL(k_sum_norms, dim3(1), dim3(64), 0, ctx->stream, cols, d_norms, d_c0);
+    hipLaunchKernelGGL(k_ordered_row_sums, dim3(1), dim3(64), 0, ctx->stream, 1, cols, d_norms, d_c0);
     hipLaunchKernelGGL(k_alphas, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, cols, d_norms, d_c0, d_v,
                        d_alphas);
     RZK_HIP(hipGetLastError());

@@ -18,7 +18,7 @@ void launch_gather_fused(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
                          int64_t nvpl, int clamp_mode, float clamp, float tmin, float n_emitted,
                          float* d_out, int* d_stats);
 void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_alphas,
-                        float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0);
+                        float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0, float* d_mT);
 void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl, const int* d_centers,
                         const float* d_weights, const float* d_mags, float* d_K, float* d_Knorm,
                         int* d_assign);
@@ -226,9 +226,10 @@ struct rzk_lightslice {
     DBuf<int> mat, light;
     // rows
     DBuf<int> row_pix, rows_mat;
+    DBuf<uint64_t> vis_bits;          // per (representative, 8x8 pixel tile) blocked masks
     DBuf<float> rows_recv, R, Vm;
     // clustering
-    DBuf<float> alphas, mags, norms, u, v, c0, center_w, K, Knorm, rep_w;
+    DBuf<float> alphas, mags, norms, u, v, c0, center_w, K, Knorm, rep_w, mT;
     DBuf<int> centers, assign, reps;
     // slices
     DBuf<int> nb, upper;
@@ -338,9 +339,11 @@ int rzk_lightslice_create(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, c
         int64_t V = ls->V;
         ls->rays.alloc(6 * n); ls->recv.alloc(RZK_RECV_FLOATS * n); ls->image.alloc(3 * n);
         ls->hits.alloc(n); ls->mat.alloc(n); ls->light.alloc(n);
-        ls->row_pix.alloc(S); ls->rows_mat.alloc(S); ls->rows_recv.alloc(RZK_RECV_FLOATS * S);
+        ls->row_pix.alloc(S); ls->rows_mat.alloc(S);
+        ls->vis_bits.alloc((size_t)C * ((ls->W + 7) / 8) * ((ls->H + 7) / 8)); ls->rows_recv.alloc(RZK_RECV_FLOATS * S);
         ls->R.alloc(3 * (size_t)S * V); ls->Vm.alloc((size_t)S * V);
         ls->alphas.alloc(V); ls->mags.alloc(V); ls->norms.alloc(V); ls->u.alloc(S); ls->v.alloc(V); ls->c0.alloc(1);
+        ls->mT.alloc((size_t)S * V);
         ls->centers.alloc(C); ls->center_w.alloc(C); ls->K.alloc(3 * (size_t)S * C); ls->Knorm.alloc(C);
         ls->assign.alloc(V); ls->reps.alloc(C); ls->rep_w.alloc(C);
         ls->nb.alloc((size_t)S * p->num_neighbor_slices);
@@ -383,10 +386,11 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         hipLaunchKernelGGL(k_pick_rows, dim3((S + 63) / 64), dim3(64), 0, st, ls->recv.p, ls->mat.p, ls->row_pix.p, S,
                            ls->rows_recv.p, ls->rows_mat.p);
         // reduced matrix (lightslice.cc:721-748)
-        launch_reduced_matrix(ctx, sv, ls->rows_recv.p, ls->rows_mat.p, S, ls->vpls->d_vpl, ls->vpls->d_mat, V,
+        launch_reduced_matrix(ctx, sv, ls->rows_recv.p, ls->rows_mat.p, S, ls->vpls->d_order, ls->vpls->d_vpl, ls->vpls->d_mat, V,
                               p.clamp_mode, p.incident_clamp, p.tmin, ls->R.p, ls->Vm.p, nullptr);
         // alphas (light_transport_matrix.cc:72-106)
-        launch_alphas_full(ctx, S, V, ls->R.p, ls->alphas.p, ls->mags.p, ls->norms.p, ls->u.p, ls->v.p, ls->c0.p);
+        launch_alphas_full(ctx, S, V, ls->R.p, ls->alphas.p, ls->mags.p, ls->norms.p, ls->u.p, ls->v.p, ls->c0.p,
+                           ls->mT.p);
         ls->h_alphas.resize(V);
         RZK_HIP(hipMemcpyAsync(ls->h_alphas.data(), ls->alphas.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
         ls_sync(ctx);
@@ -489,7 +493,7 @@ int rzk_lightslice_gather(rzk_lightslice* ls, float* out_rgb) {
         SceneView sv = scene_view(ls->scene);
         launch_slice_gather(ctx, sv, ls->recv.p, ls->mat.p, ls->W, ls->H, ls->T, ls->vpls->d_vpl, ls->vpls->d_mat,
                             ls->reps.p, ls->rep_w.p, (int)ls->h_reps.size(), p.clamp_mode, p.clamp, p.tmin,
-                            p.n_emitted, ls->image.p, nullptr);
+                            p.n_emitted, ls->image.p, ls->vis_bits.p, nullptr);
         int64_t n = (int64_t)ls->W * ls->H;
         if (out_rgb) {
             RZK_HIP(hipMemcpyAsync(out_rgb, ls->image.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
@@ -511,11 +515,11 @@ int rzk_lightslice_count_work(rzk_lightslice* ls, int64_t* out) {
         DBuf<unsigned long long> tot;
         tot.alloc(4);
         RZK_HIP(hipMemsetAsync(tot.p, 0, sizeof(unsigned long long) * 4, ctx->stream));
-        launch_reduced_matrix(ctx, sv, ls->rows_recv.p, ls->rows_mat.p, S, ls->vpls->d_vpl, ls->vpls->d_mat, V,
+        launch_reduced_matrix(ctx, sv, ls->rows_recv.p, ls->rows_mat.p, S, ls->vpls->d_order, ls->vpls->d_vpl, ls->vpls->d_mat, V,
                               p.clamp_mode, p.incident_clamp, p.tmin, ls->R.p, ls->Vm.p, (int*)tot.p);
         launch_slice_gather(ctx, sv, ls->recv.p, ls->mat.p, ls->W, ls->H, ls->T, ls->vpls->d_vpl, ls->vpls->d_mat,
                             ls->reps.p, ls->rep_w.p, nreps, p.clamp_mode, p.clamp, p.tmin, p.n_emitted, ls->image.p,
-                            (int*)(tot.p + 2));
+                            ls->vis_bits.p, (int*)(tot.p + 2));
         unsigned long long h[4];
         std::vector<int> rm(S), pm(n);
         RZK_HIP(hipMemcpyAsync(h, tot.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));

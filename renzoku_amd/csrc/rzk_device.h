@@ -6,6 +6,15 @@
 
 namespace rzk {
 
+// Pointers into read-only scene data for wave-uniform loads (AMDGPU constant
+// address space: the compiler may fetch through the scalar cache).  Only the
+// device pass sees the address space; the host pass never runs these paths.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define kconst __attribute__((address_space(4)))
+#else
+#define kconst
+#endif
+
 // Traversal statistics: interior nodes visited, triangles tested.
 struct TravStats { int nodes = 0, tris = 0; };
 
@@ -76,13 +85,175 @@ __device__ __forceinline__ void add_work_totals(unsigned long long* tot, const T
     }
 }
 
+// Any-hit traversal stack: the top kLdsStack entries of a lane live in LDS
+// (one column per lane, `pitch` = block size), deeper ones in HBM at
+// spill[(entry - kLdsStack) * spill_pitch + gid], so a launch's LDS footprint
+// stays small whatever the tree depth.
+struct AnyStack {
+    int* lds;
+    int pitch;
+    int* spill;
+    int64_t spitch;
+    int64_t gid;
+    __device__ __forceinline__ void push(int& sp, int v) {
+        if (sp < kLdsStack) lds[sp * pitch] = v;
+        else spill[(int64_t)(sp - kLdsStack) * spitch + gid] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ int pop(int& sp) {
+        --sp;
+        return sp < kLdsStack ? lds[sp * pitch] : spill[(int64_t)(sp - kLdsStack) * spitch + gid];
+    }
+};
+
+__device__ __forceinline__ AnyStack any_stack(const SceneView& sv, int* lds_base) {
+    AnyStack s;
+    s.lds = lds_base + threadIdx.x;
+    s.pitch = blockDim.x;
+    s.spill = sv.spill;
+    s.spitch = sv.spill_pitch;
+    s.gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    return s;
+}
+
+// Any hit over the 4-wide collapse of the reference tree.  The boolean equals
+// the binary traversal's (aggregate_bvh.cc:452-478) for every ray: a triangle
+// is tested iff its own box and all enclosing boxes pass the slab test, and
+// the enclosing ones pass whenever the inner one does.  Visit order is free.
+__device__ __forceinline__ bool any_hit4(const SceneView& sv, const Ray& r, float tmin, float tmax,
+                                         AnyStack& stk, TravStats& st) {
+    if (!box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]), mk(sv.rbmax[0], sv.rbmax[1], sv.rbmax[2]),
+                 r, tmin, tmax))
+        return false;
+    int node = sv.root4_ref;
+    float t, b, g;
+    if (node < 0) {
+        const TriGpu* tp = sv.tris + (~node);
+        float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
+        st.tris++;
+        return tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g);
+    }
+    int sp = 0;
+    for (;;) {
+        const Bvh4NodeGpu* np = sv.nodes4 + node;
+        float4 lx = np->lox, ly = np->loy, lz = np->loz, hx = np->hix, hy = np->hiy, hz = np->hiz;
+        int4 rf = np->ref;
+        st.nodes++;
+        const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w};
+        const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+        const int rfa[4] = {rf.x, rf.y, rf.z, rf.w};
+        int next = -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            int ref = rfa[c];
+            if (ref == kEmptyRef) break;
+            if (!box_hit(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax)) continue;
+            if (ref < 0) {
+                const TriGpu* tp = sv.tris + (~ref);
+                float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
+                st.tris++;
+                if (tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g))
+                    return true;
+            } else if (next < 0) {
+                next = ref;
+            } else {
+                stk.push(sp, ref);
+            }
+        }
+        if (next >= 0) {
+            node = next;
+        } else {
+            if (sp == 0) return false;
+            node = stk.pop(sp);
+        }
+    }
+}
+
+// Wave-coherent any hit: the 64 lanes walk the 4-wide tree together.  The
+// node index is wave-uniform (readfirstlane), so a node is fetched once per
+// wave through the scalar cache; each lane tests the four child boxes against
+// its own ray, a ballot decides which children the wave descends into, and a
+// lane leaves the walk once it has found an occluder.  Each lane still tests
+// exactly the triangles whose box chain its own ray passes (the result is the
+// per-lane any_hit4 result); only the schedule is shared.  Every lane of the
+// wave must call this together; `active` false lanes ride along idle.
+// `wstack` is this wave's LDS stack of sv.stack4 ints.
+__device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r, float tmin, float tmax,
+                                              bool active, int* wstack, TravStats& st) {
+    bool live = active && box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]),
+                                  mk(sv.rbmax[0], sv.rbmax[1], sv.rbmax[2]), r, tmin, tmax);
+    bool hit = false;
+    float t, b, g;
+    if (__ballot(live) == 0ull) return false;
+    int node = sv.root4_ref;
+    if (node < 0) {
+        if (live) {
+            const TriGpu* tp = sv.tris + (~node);
+            float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
+            st.tris++;
+            hit = tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g);
+        }
+        return hit;
+    }
+    int sp = 0;
+    for (;;) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        // wave-uniform address in the constant address space: scalar loads
+        const kconst float* nf = (const kconst float*)(sv.nodes4 + node);
+        const kconst int* ni = (const kconst int*)(sv.nodes4 + node);
+        const float lxa[4] = {nf[0], nf[1], nf[2], nf[3]}, lya[4] = {nf[4], nf[5], nf[6], nf[7]};
+        const float lza[4] = {nf[8], nf[9], nf[10], nf[11]}, hxa[4] = {nf[12], nf[13], nf[14], nf[15]};
+        const float hya[4] = {nf[16], nf[17], nf[18], nf[19]}, hza[4] = {nf[20], nf[21], nf[22], nf[23]};
+        const int rfa[4] = {ni[24], ni[25], ni[26], ni[27]};
+        if (live) st.nodes++;
+        int next = -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            int ref = rfa[c];
+            if (ref == kEmptyRef) break;
+            bool hc = live && box_hit(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax);
+            if (__ballot(hc) == 0ull) continue;
+            if (ref < 0) {
+                const kconst float* tf = (const kconst float*)(sv.tris + (~ref));
+                v3 p0 = mk(tf[0], tf[1], tf[2]), e1 = mk(tf[4], tf[5], tf[6]), e2 = mk(tf[8], tf[9], tf[10]);
+                if (hc) {
+                    st.tris++;
+                    if (tri_hit(p0, e1, e2, r, tmin, tmax, t, b, g)) {
+                        hit = true;
+                        live = false;
+                    }
+                }
+            } else if (next < 0) {
+                next = ref;
+            } else {
+                wstack[sp++] = ref;
+            }
+        }
+        if (__ballot(live) == 0ull) return hit;
+        if (next >= 0) {
+            node = next;
+        } else {
+            if (sp == 0) return hit;
+            node = wstack[--sp];
+        }
+    }
+}
+
 // Segment visibility p -> q (aggregate.cc:84-90): true when blocked.
 __device__ __forceinline__ bool segment_blocked(const SceneView& sv, v3 p, v3 q, float tmin,
-                                                int* stk, int pitch, TravStats& st) {
+                                                AnyStack& stk, TravStats& st) {
     Ray r = ray_ctor(p, q - p);
     float tmax = length(q - p) - tmin;
-    float t, b, g;
-    return traverse<true>(sv, r, tmin, tmax, stk, pitch, t, b, g, st) >= 0;
+    return any_hit4(sv, r, tmin, tmax, stk, st);
+}
+
+// Segment visibility for a whole wave of segments (see any_hit4_wave)
+__device__ __forceinline__ bool segment_blocked_wave(const SceneView& sv, v3 p, v3 q, float tmin, bool active,
+                                                     int* wstack, TravStats& st) {
+    Ray r = ray_ctor(p, q - p);
+    float tmax = length(q - p) - tmin;
+    return any_hit4_wave(sv, r, tmin, tmax, active, wstack, st);
 }
 
 // Receiver record from a closest hit (triangle.cc:277-296, surface.cc:62-96,

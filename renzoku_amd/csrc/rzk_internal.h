@@ -19,6 +19,13 @@
 struct BvhNodeGpu { float4 a, b, c; int4 d; };
 struct TriGpu { float4 p0, e1, e2; };
 
+// 4-wide node of the same tree collapsed two levels at a time, for any-hit
+// (shadow) traversal: 128 B = one L2 line.  Child boxes as SoA float4 lanes,
+// refs as above, kEmptyRef for unused slots (always the trailing ones).
+struct Bvh4NodeGpu { float4 lox, loy, loz, hix, hiy, hiz; int4 ref; int4 pad; };
+constexpr int kEmptyRef = (int)0x80000000;
+constexpr int kLdsStack = 16;   // stack entries per lane kept in LDS; deeper ones spill to HBM
+
 struct BvhHost {
     int ntri = 0;
     int num_interior = 0;
@@ -27,6 +34,10 @@ struct BvhHost {
     float box[6] = {0, 0, 0, 0, 0, 0};   // scene box (origin included, boundingbox.h:100-103)
     float root_box[6] = {0, 0, 0, 0, 0, 0};
     std::vector<BvhNodeGpu> nodes;
+    std::vector<Bvh4NodeGpu> nodes4;
+    int root4_ref = 0;
+    int depth4 = 0;           // 4-wide nodes on the deepest path
+    int stack4 = 0;           // bound on any-hit stack entries (3 per 4-wide level)
     std::vector<TriGpu> tris;
     std::vector<int> sorted_tri;         // sorted position -> original triangle
     std::vector<uint32_t> sorted_morton;
@@ -54,6 +65,7 @@ struct rzk_scene {
     int nsurf = 0, nmat = 0, nlight = 0;
     // device
     BvhNodeGpu* d_nodes = nullptr;
+    Bvh4NodeGpu* d_nodes4 = nullptr;
     TriGpu* d_tris = nullptr;
     float4* d_tri_n = nullptr;        // per original triangle: (n.xyz, surface id bits)
     float4* d_tri_tan = nullptr;      // per original triangle: (tangent.xyz, 0)
@@ -69,6 +81,7 @@ struct rzk_vpls {
     int64_t n = 0;
     float* d_vpl = nullptr;           // 21 floats per VPL (include/rzk.h)
     int* d_mat = nullptr;
+    int* d_order = nullptr;           // VPL indices in Morton order of their positions
     std::vector<float> h_vpl;
     std::vector<int> h_mat;
 };
@@ -76,6 +89,7 @@ struct rzk_vpls {
 // Scene constants passed by value to kernels
 struct SceneView {
     const BvhNodeGpu* nodes;
+    const Bvh4NodeGpu* nodes4;
     const TriGpu* tris;
     const float4* tri_n;
     const float4* tri_tan;
@@ -83,7 +97,11 @@ struct SceneView {
     const float* mat;
     int root_ref;
     int depth;
+    int root4_ref;
+    int stack4;
     float rbmin[3], rbmax[3];
+    int* spill;               // any-hit stack overflow, [entry - kLdsStack][spill_pitch]
+    int64_t spill_pitch;      // threads of the launch using `spill`
 };
 
 SceneView scene_view(const rzk_scene* s);
@@ -100,6 +118,8 @@ struct RzkError { int code; std::string msg; };
     do { if (!(cond)) throw RzkError{code, msg}; } while (0)
 
 void* ctx_scratch(rzk_ctx* ctx, size_t bytes);
+// size sv.spill for a launch of `threads` lanes doing any-hit traversal
+void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads);
 
 // kernel launchers (traverse.hip, shade.hip, cluster.hip)
 void launch_trace_closest(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int64_t n,
@@ -121,7 +141,8 @@ void launch_gather_shade(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
                          const uint64_t* d_bits, int clamp_mode, float clamp, float n_emitted,
                          float* d_out);
 void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
-                           const int* d_rmat, int rows, const float* d_vpl, const int* d_vmat,
+                           const int* d_rmat, int rows, const int* d_vorder, const float* d_vpl,
+                           const int* d_vmat,
                            int64_t nvpl, int clamp_mode, float clamp, float tmin, float* d_R,
                            float* d_V, int* d_stats);
 void launch_alphas(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_alphas,
@@ -132,4 +153,4 @@ void launch_slice_gather(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
                          const int* d_rmat, int width, int height, int tile,
                          const float* d_vpl, const int* d_vmat, const int* d_reps,
                          const float* d_rep_w, int nreps, int clamp_mode, float clamp,
-                         float tmin, float n_emitted, float* d_out, int* d_stats);
+                         float tmin, float n_emitted, float* d_out, uint64_t* d_bits, int* d_stats);

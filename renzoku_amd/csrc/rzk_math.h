@@ -72,13 +72,17 @@ RZK_HD Ray ray_ctor(v3 o, v3 d_in) {
     return ray_from_od(o, d);
 }
 
-// BoundingBox::hit, boundingbox.h:238-280 (mul then add, never fused)
+// BoundingBox::hit, boundingbox.h:238-280 (mul then add, never fused).  The
+// slab values are never NaN (inv_d is finite and non-zero, boxes and origins
+// are finite), so fminf/fmaxf (v_min3/v_max3 on gfx950) pick the same values
+// as the reference's std::min/std::max up to the sign of a zero, which no
+// comparison below can see: the test's outcome is identical.
 RZK_HD bool box_hit(v3 bmin, v3 bmax, const Ray& r, float tmin, float tmax) {
     float x0 = bmin.x * r.inv_d.x + r.org_inv_d.x, x1 = bmax.x * r.inv_d.x + r.org_inv_d.x;
     float y0 = bmin.y * r.inv_d.y + r.org_inv_d.y, y1 = bmax.y * r.inv_d.y + r.org_inv_d.y;
     float z0 = bmin.z * r.inv_d.z + r.org_inv_d.z, z1 = bmax.z * r.inv_d.z + r.org_inv_d.z;
-    float tminbox = smax(tmin, smax(smin(x0, x1), smax(smin(y0, y1), smin(z0, z1))));
-    float tmaxbox = smin(tmax, smin(smax(x0, x1), smin(smax(y0, y1), smax(z0, z1))));
+    float tminbox = fmaxf(fmaxf(tmin, fminf(x0, x1)), fmaxf(fminf(y0, y1), fminf(z0, z1)));
+    float tmaxbox = fminf(fminf(tmax, fmaxf(x0, x1)), fminf(fmaxf(y0, y1), fmaxf(z0, z1)));
     return (tminbox <= tmaxbox) && (tmax >= tminbox && tmaxbox >= tmin);
 }
 

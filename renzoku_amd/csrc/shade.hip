@@ -15,6 +15,11 @@ __device__ __forceinline__ void load_record(float* dst, const float* src, int nf
     for (int i = 0; i < nfl; ++i) dst[i] = src[i];
 }
 
+// This wave's any-hit stack in LDS (sv.stack4 entries per wave)
+__device__ __forceinline__ int* wave_stack(int* base, const SceneView& sv) {
+    return base + (threadIdx.x >> 6) * sv.stack4;
+}
+
 // Cooperative load of VPL records [k0, k0 + cnt) into LDS.
 __device__ __forceinline__ void stage_vpls(float* s_vpl, const float* __restrict__ vpl,
                                            const int* __restrict__ vmat, int64_t k0, int cnt) {
@@ -33,24 +38,26 @@ __global__ __launch_bounds__(kShadeBlock) void k_visibility(SceneView sv, const 
                                                            float tmin, uint64_t* __restrict__ bits,
                                                            int* __restrict__ d_stats) {
     extern __shared__ int s_stack[];
+    int* wstk = wave_stack(s_stack, sv);
     int64_t ngroups = (npts + 63) / 64;
     int64_t wave = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     int lane = threadIdx.x & 63;
     TravStats st;
-    bool blocked = true;
     bool live = wave < ngroups * nvpl;
     int64_t k = 0, g = 0;
+    bool active = false;
+    v3 p = mk(0.f, 0.f, 0.f), q = mk(0.f, 0.f, 0.f);
     if (live) {
         k = wave / ngroups;
         g = wave - k * ngroups;
         int64_t i = g * 64 + lane;
         if (i < npts && rmat[i] >= 0) {
-            const float* rc = recv + RZK_RECV_FLOATS * i;
-            const float* vp = vpl + RZK_VPL_FLOATS * k;
-            blocked = segment_blocked(sv, mk(rc[0], rc[1], rc[2]), mk(vp[0], vp[1], vp[2]), tmin,
-                                      s_stack + threadIdx.x, blockDim.x, st);
+            active = true;
+            p = ld3(recv + RZK_RECV_FLOATS * i);
+            q = ld3(vpl + RZK_VPL_FLOATS * k);
         }
     }
+    bool blocked = segment_blocked_wave(sv, p, q, tmin, active, wstk, st) || !active;
     uint64_t b = __ballot(blocked);
     if (live && lane == 0) bits[k * ngroups + g] = b;
     if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
@@ -114,15 +121,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_gather_fused(SceneView sv, cons
                                                              int* __restrict__ d_stats) {
     extern __shared__ int s_dyn[];
     float* s_vpl = reinterpret_cast<float*>(s_dyn);
-    int* s_stack = s_dyn + kVplTile * kVplRec;
+    int* wstk = wave_stack(s_dyn + kVplTile * kVplRec, sv);
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t i = t < npts ? (order ? order[t] : t) : -1;
-    float rc[RZK_RECV_FLOATS];
-    int m = -1;
-    if (i >= 0) {
-        load_record(rc, recv + RZK_RECV_FLOATS * i, RZK_RECV_FLOATS);
-        m = rmat[i];
-    }
+    const float* rcm = recv + RZK_RECV_FLOATS * (i >= 0 ? i : 0);
+    int m = i >= 0 ? rmat[i] : -1;
+    v3 p = m >= 0 ? ld3(rcm) : mk(0.f, 0.f, 0.f);
     TravStats st;
     v3 acc = mk(0.f, 0.f, 0.f);
     for (int64_t k0 = 0; k0 < nvpl; k0 += kVplTile) {
@@ -130,16 +134,16 @@ __global__ __launch_bounds__(kShadeBlock) void k_gather_fused(SceneView sv, cons
         __syncthreads();
         stage_vpls(s_vpl, vpl, vmat, k0, cnt);
         __syncthreads();
-        if (m >= 0) {
-            for (int kk = 0; kk < cnt; ++kk) {
-                float vp[kVplRec];
-                load_record(vp, s_vpl + kk * kVplRec, kVplRec);
-                if (segment_blocked(sv, mk(rc[0], rc[1], rc[2]), mk(vp[0], vp[1], vp[2]), tmin,
-                                    s_stack + threadIdx.x, blockDim.x, st))
-                    continue;
-                v3 lo = gather_shade(sv.mat, rc, m, vp, __float_as_int(vp[21]), mode, thr);
-                acc = acc + lo * 1.0f;
-            }
+        for (int kk = 0; kk < cnt; ++kk) {
+            const float* sv_rec = s_vpl + kk * kVplRec;
+            bool blocked = segment_blocked_wave(sv, p, mk(sv_rec[0], sv_rec[1], sv_rec[2]), tmin, m >= 0, wstk, st);
+            if (m < 0 || blocked) continue;
+            float rc[RZK_RECV_FLOATS], vp[kVplRec];
+            __asm__ volatile("" ::: "memory");   // re-read the receiver here, do not keep it live
+            load_record(rc, rcm, RZK_RECV_FLOATS);
+            load_record(vp, sv_rec, kVplRec);
+            v3 lo = gather_shade(sv.mat, rc, m, vp, __float_as_int(vp[21]), mode, thr);
+            acc = acc + lo * 1.0f;
         }
     }
     if (i >= 0) {
@@ -150,66 +154,124 @@ __global__ __launch_bounds__(kShadeBlock) void k_gather_fused(SceneView sv, cons
     if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
-// ---- LightSlice reduced matrix (lightslice.cc:721-748): pair (row r, VPL k),
-// thread index = k * rows + r so the column-major outputs are written in order
+// ---- LightSlice reduced matrix (lightslice.cc:721-748): pair (row r, VPL k).
+// A wave holds one row and 64 VPLs that are consecutive in Morton order
+// (k = vorder[kk]), so its 64 segments leave one receiver in a narrow cone.
+// Outputs go to the column-major position of the original VPL index.
 __global__ __launch_bounds__(kShadeBlock) void k_reduced_matrix(SceneView sv, const float* __restrict__ recv,
                                                                const int* __restrict__ rmat, int rows,
+                                                               const int* __restrict__ vorder,
                                                                const float* __restrict__ vpl,
                                                                const int* __restrict__ vmat, int64_t nvpl,
                                                                int mode, float thr, float tmin,
                                                                float* __restrict__ R, float* __restrict__ V,
                                                                int* __restrict__ d_stats) {
     extern __shared__ int s_stack[];
+    int* wstk = wave_stack(s_stack, sv);
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     TravStats st;
-    if (t < (int64_t)rows * nvpl) {
-        int64_t k = t / rows;
-        int r = (int)(t - k * rows);
+    bool inside = t < (int64_t)rows * nvpl;
+    int r = 0;
+    int64_t k = 0;
+    bool active = false;
+    v3 p = mk(0.f, 0.f, 0.f), q = mk(0.f, 0.f, 0.f);
+    if (inside) {
+        r = (int)(t / nvpl);
+        int64_t kk = t - (int64_t)r * nvpl;
+        k = vorder ? vorder[kk] : kk;
+        active = rmat[r] >= 0;
+        if (active) {
+            p = ld3(recv + RZK_RECV_FLOATS * r);
+            q = ld3(vpl + RZK_VPL_FLOATS * k);
+        }
+    }
+    bool blocked = segment_blocked_wave(sv, p, q, tmin, active, wstk, st);
+    if (inside) {
+        int64_t o = k * rows + r;
         float rv = 0.f, g = 0.f, b = 0.f, vis = 0.f;
-        if (rmat[r] >= 0) {
+        if (active) {
             float rc[RZK_RECV_FLOATS], vp[RZK_VPL_FLOATS];
             load_record(rc, recv + RZK_RECV_FLOATS * r, RZK_RECV_FLOATS);
             load_record(vp, vpl + RZK_VPL_FLOATS * k, RZK_VPL_FLOATS);
-            bool visible = !segment_blocked(sv, ld3(rc), ld3(vp), tmin, s_stack + threadIdx.x, blockDim.x, st);
             v3 inc = incident_radiance(sv.mat, rc, vp, vmat[k], mode, thr);
-            vis = (float)visible;
+            vis = (float)(!blocked);
             rv = inc.x * vis; g = inc.y * vis; b = inc.z * vis;
         }
-        R[3 * t] = rv; R[3 * t + 1] = g; R[3 * t + 2] = b;
-        V[t] = vis;
+        R[3 * o] = rv; R[3 * o + 1] = g; R[3 * o + 2] = b;
+        V[o] = vis;
     }
     if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
-// ---- LightSlice::gather_clusters for every pixel (lightslice.cc:1315-1345):
-// Σ_c weight_c * gather_point(rep_c) over visible representatives, in cluster
-// order, then / num_emitted.  Pixels are walked in 8x8 tiles per wave.
-__global__ __launch_bounds__(kShadeBlock) void k_slice_gather(SceneView sv, const float* __restrict__ recv,
-                                                             const int* __restrict__ rmat, int width,
-                                                             int height, const float* __restrict__ vpl,
-                                                             const int* __restrict__ vmat,
-                                                             const int* __restrict__ reps,
-                                                             const float* __restrict__ rep_w, int nreps,
-                                                             int mode, float thr, float tmin,
-                                                             float n_emitted, float* __restrict__ out,
-                                                             int* __restrict__ d_stats) {
-    extern __shared__ int s_dyn[];
-    float* s_vpl = reinterpret_cast<float*>(s_dyn);           // kVplTile records + weight
-    int* s_stack = s_dyn + kVplTile * (kVplRec + 1);
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Pixel of lane `lane` of 8x8 tile `tile` (-1 outside the image)
+__device__ __forceinline__ int64_t tile_pixel(int64_t tile, int lane, int width, int height) {
     int tiles_x = (width + 7) / 8;
-    int64_t tile = t / 64;
-    int lane = (int)(t & 63);
     int px = (int)(tile % tiles_x) * 8 + (lane & 7);
     int py = (int)(tile / tiles_x) * 8 + (lane >> 3);
-    int64_t i = (px < width && py < height) ? (int64_t)py * width + px : -1;
-    float rc[RZK_RECV_FLOATS];
-    int m = -1;
-    if (i >= 0) {
-        load_record(rc, recv + RZK_RECV_FLOATS * i, RZK_RECV_FLOATS);
-        m = rmat[i];
-    }
+    return (px < width && py < height) ? (int64_t)py * width + px : -1;
+}
+
+// ---- LightSlice::gather_clusters, phase 1: visibility of every (pixel,
+// representative) pair.  One wave = one 8x8 pixel tile; it walks the
+// representatives in order and records a 64-bit blocked mask per
+// representative: bits[c * ntiles + tile].  Only the segment endpoints are
+// live here, which keeps the traversal kernel small (high occupancy).
+__global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, const float* __restrict__ recv,
+                                                                 const int* __restrict__ rmat, int width,
+                                                                 int height, const float* __restrict__ vpl,
+                                                                 const int* __restrict__ reps, int nreps,
+                                                                 float tmin, uint64_t* __restrict__ bits,
+                                                                 int* __restrict__ d_stats) {
+    extern __shared__ int s_dyn[];
+    float* s_q = reinterpret_cast<float*>(s_dyn);            // kVplTile positions
+    int* wstk = wave_stack(s_dyn + 3 * kVplTile, sv);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ntiles = (int64_t)((width + 7) / 8) * ((height + 7) / 8);
+    int64_t tile = t / 64;
+    int lane = (int)(t & 63);
+    int64_t i = tile < ntiles ? tile_pixel(tile, lane, width, height) : -1;
+    int m = i >= 0 ? rmat[i] : -1;
+    v3 p = m >= 0 ? ld3(recv + RZK_RECV_FLOATS * i) : mk(0.f, 0.f, 0.f);
     TravStats st;
+    for (int c0 = 0; c0 < nreps; c0 += kVplTile) {
+        int cnt = (nreps - c0) < kVplTile ? (nreps - c0) : kVplTile;
+        __syncthreads();
+        for (int e = threadIdx.x; e < cnt * 3; e += blockDim.x) {
+            int c = e / 3, f = e - c * 3;
+            s_q[e] = vpl[(int64_t)reps[c0 + c] * RZK_VPL_FLOATS + f];
+        }
+        __syncthreads();
+        for (int cc = 0; cc < cnt; ++cc) {
+            bool blocked = segment_blocked_wave(sv, p, mk(s_q[3 * cc], s_q[3 * cc + 1], s_q[3 * cc + 2]), tmin,
+                                                m >= 0, wstk, st) || m < 0;
+            uint64_t b = __ballot(blocked);
+            if (lane == 0 && tile < ntiles) bits[(int64_t)(c0 + cc) * ntiles + tile] = b;
+        }
+    }
+    if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
+}
+
+// ---- phase 2: Σ_c weight_c * gather_point(rep_c) over the visible
+// representatives, in cluster order, then / num_emitted (lightslice.cc:1325-1339,
+// vpl.cc:488-489).  Same pixel-to-lane mapping as phase 1.
+__global__ __launch_bounds__(kShadeBlock) void k_slice_shade(const float* __restrict__ mat,
+                                                            const float* __restrict__ recv,
+                                                            const int* __restrict__ rmat, int width, int height,
+                                                            const float* __restrict__ vpl,
+                                                            const int* __restrict__ vmat,
+                                                            const int* __restrict__ reps,
+                                                            const float* __restrict__ rep_w, int nreps,
+                                                            const uint64_t* __restrict__ bits, int mode,
+                                                            float thr, float n_emitted, float* __restrict__ out) {
+    __shared__ float s_vpl[kVplTile * (kVplRec + 1)];
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ntiles = (int64_t)((width + 7) / 8) * ((height + 7) / 8);
+    int64_t tile = t / 64;
+    int lane = (int)(t & 63);
+    int64_t i = tile < ntiles ? tile_pixel(tile, lane, width, height) : -1;
+    int m = i >= 0 ? rmat[i] : -1;
+    float rc[RZK_RECV_FLOATS];
+    if (m >= 0) load_record(rc, recv + RZK_RECV_FLOATS * i, RZK_RECV_FLOATS);
     v3 acc = mk(0.f, 0.f, 0.f);
     for (int c0 = 0; c0 < nreps; c0 += kVplTile) {
         int cnt = (nreps - c0) < kVplTile ? (nreps - c0) : kVplTile;
@@ -226,12 +288,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_gather(SceneView sv, cons
         __syncthreads();
         if (m >= 0) {
             for (int cc = 0; cc < cnt; ++cc) {
+                uint64_t w = bits[(int64_t)(c0 + cc) * ntiles + tile];
+                if ((w >> lane) & 1ull) continue;
                 float vp[kVplRec + 1];
                 load_record(vp, s_vpl + cc * (kVplRec + 1), kVplRec + 1);
-                if (segment_blocked(sv, mk(rc[0], rc[1], rc[2]), mk(vp[0], vp[1], vp[2]), tmin,
-                                    s_stack + threadIdx.x, blockDim.x, st))
-                    continue;
-                v3 lo = gather_shade(sv.mat, rc, m, vp, __float_as_int(vp[21]), mode, thr);
+                v3 lo = gather_shade(mat, rc, m, vp, __float_as_int(vp[21]), mode, thr);
                 acc = acc + lo * vp[22];
             }
         }
@@ -241,7 +302,6 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_gather(SceneView sv, cons
         out[3 * i + 1] = acc.y / n_emitted;
         out[3 * i + 2] = acc.z / n_emitted;
     }
-    if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
 }  // namespace rzk
@@ -250,13 +310,15 @@ using namespace rzk;
 
 static unsigned blocks_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-void launch_visibility(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
+void launch_visibility(rzk_ctx* ctx, const SceneView& sv_in, const float* d_recv, const int* d_rmat,
                        int64_t npts, const float* d_vpl, int64_t nvpl, float tmin, uint64_t* d_bits,
                        int* d_stats) {
     int64_t waves = ((npts + 63) / 64) * nvpl;
     if (waves <= 0) return;
-    size_t lds = sizeof(int) * kShadeBlock * (size_t)sv.depth;
-    hipLaunchKernelGGL(k_visibility, dim3(blocks_for(waves, kShadeBlock / 64)), dim3(kShadeBlock), lds,
+    SceneView sv = sv_in;
+    unsigned grid = blocks_for(waves, kShadeBlock / 64);
+    size_t lds = sizeof(int) * (kShadeBlock / 64) * (size_t)sv.stack4;
+    hipLaunchKernelGGL(k_visibility, dim3(grid), dim3(kShadeBlock), lds,
                        ctx->stream, sv, d_recv, d_rmat, npts, d_vpl, nvpl, tmin, d_bits, d_stats);
     RZK_HIP(hipGetLastError());
 }
@@ -272,42 +334,51 @@ void launch_gather_shade(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
     RZK_HIP(hipGetLastError());
 }
 
-void launch_gather_fused(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
+void launch_gather_fused(rzk_ctx* ctx, const SceneView& sv_in, const float* d_recv, const int* d_rmat,
                          int64_t npts, const int* d_order, const float* d_vpl, const int* d_vmat,
                          int64_t nvpl, int clamp_mode, float clamp, float tmin, float n_emitted,
                          float* d_out, int* d_stats) {
     if (npts <= 0) return;
-    size_t lds = sizeof(float) * kVplTile * kVplRec + sizeof(int) * kShadeBlock * (size_t)sv.depth;
-    hipLaunchKernelGGL(k_gather_fused, dim3(blocks_for(npts, kShadeBlock)), dim3(kShadeBlock), lds,
+    SceneView sv = sv_in;
+    unsigned grid = blocks_for(npts, kShadeBlock);
+    size_t lds = sizeof(float) * kVplTile * kVplRec + sizeof(int) * (kShadeBlock / 64) * (size_t)sv.stack4;
+    hipLaunchKernelGGL(k_gather_fused, dim3(grid), dim3(kShadeBlock), lds,
                        ctx->stream, sv, d_recv, d_rmat, npts, d_order, d_vpl, d_vmat, nvpl, clamp_mode,
                        clamp, tmin, n_emitted, d_out, d_stats);
     RZK_HIP(hipGetLastError());
 }
 
-void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
-                           int rows, const float* d_vpl, const int* d_vmat, int64_t nvpl,
-                           int clamp_mode, float clamp, float tmin, float* d_R, float* d_V,
+void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv_in, const float* d_recv, const int* d_rmat,
+                           int rows, const int* d_vorder, const float* d_vpl, const int* d_vmat,
+                           int64_t nvpl, int clamp_mode, float clamp, float tmin, float* d_R, float* d_V,
                            int* d_stats) {
     int64_t n = (int64_t)rows * nvpl;
     if (n <= 0) return;
-    size_t lds = sizeof(int) * kShadeBlock * (size_t)sv.depth;
-    hipLaunchKernelGGL(k_reduced_matrix, dim3(blocks_for(n, kShadeBlock)), dim3(kShadeBlock), lds,
-                       ctx->stream, sv, d_recv, d_rmat, rows, d_vpl, d_vmat, nvpl, clamp_mode, clamp,
+    SceneView sv = sv_in;
+    unsigned grid = blocks_for(n, kShadeBlock);
+    size_t lds = sizeof(int) * (kShadeBlock / 64) * (size_t)sv.stack4;
+    hipLaunchKernelGGL(k_reduced_matrix, dim3(grid), dim3(kShadeBlock), lds,
+                       ctx->stream, sv, d_recv, d_rmat, rows, d_vorder, d_vpl, d_vmat, nvpl, clamp_mode, clamp,
                        tmin, d_R, d_V, d_stats);
     RZK_HIP(hipGetLastError());
 }
 
-void launch_slice_gather(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
+void launch_slice_gather(rzk_ctx* ctx, const SceneView& sv_in, const float* d_recv, const int* d_rmat,
                          int width, int height, int tile, const float* d_vpl, const int* d_vmat,
                          const int* d_reps, const float* d_rep_w, int nreps, int clamp_mode,
-                         float clamp, float tmin, float n_emitted, float* d_out, int* d_stats) {
+                         float clamp, float tmin, float n_emitted, float* d_out, uint64_t* d_bits,
+                         int* d_stats) {
     (void)tile;
     int64_t tiles = (int64_t)((width + 7) / 8) * ((height + 7) / 8);
     int64_t threads = tiles * 64;
     if (threads <= 0) return;
-    size_t lds = sizeof(float) * kVplTile * (kVplRec + 1) + sizeof(int) * kShadeBlock * (size_t)sv.depth;
-    hipLaunchKernelGGL(k_slice_gather, dim3(blocks_for(threads, kShadeBlock)), dim3(kShadeBlock), lds,
-                       ctx->stream, sv, d_recv, d_rmat, width, height, d_vpl, d_vmat, d_reps, d_rep_w,
-                       nreps, clamp_mode, clamp, tmin, n_emitted, d_out, d_stats);
+    SceneView sv = sv_in;
+    unsigned grid = blocks_for(threads, kShadeBlock);
+    size_t lds = sizeof(float) * 3 * kVplTile + sizeof(int) * (kShadeBlock / 64) * (size_t)sv.stack4;
+    hipLaunchKernelGGL(k_slice_visibility, dim3(grid), dim3(kShadeBlock), lds, ctx->stream, sv, d_recv, d_rmat,
+                       width, height, d_vpl, d_reps, nreps, tmin, d_bits, d_stats);
+    RZK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_slice_shade, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, sv.mat, d_recv, d_rmat, width,
+                       height, d_vpl, d_vmat, d_reps, d_rep_w, nreps, d_bits, clamp_mode, clamp, n_emitted, d_out);
     RZK_HIP(hipGetLastError());
 }

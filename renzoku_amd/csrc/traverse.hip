@@ -2,6 +2,8 @@
 // records and camera rays.
 #pragma clang fp contract(off)
 
+#include <algorithm>
+
 #include "rzk_device.h"
 
 namespace rzk {
@@ -49,8 +51,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_any(SceneView sv, const f
     if (i < n) {
         const float* rp = rays + RZK_RAY_FLOATS * i;
         Ray r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
-        float t, b, g;
-        out[i] = traverse<true>(sv, r, tmin, tmax, s_stack + threadIdx.x, blockDim.x, t, b, g, st) >= 0;
+        if (kStats == 1) {
+            // per-ray counts follow the reference's binary visit order
+            float t, b, g;
+            out[i] = traverse<true>(sv, r, tmin, tmax, s_stack + threadIdx.x, blockDim.x, t, b, g, st) >= 0;
+        } else {
+            AnyStack stk = any_stack(sv, s_stack);
+            out[i] = any_hit4(sv, r, tmin, tmax, stk, st);
+        }
     }
     record_stats<kStats>(d_stats, i, st);
 }
@@ -64,8 +72,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_segments(SceneView sv, co
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     TravStats st;
     if (i < n) {
-        out[i] = segment_blocked(sv, ld3(p + 3 * i), ld3(q + 3 * i), tmin, s_stack + threadIdx.x,
-                                 blockDim.x, st);
+        AnyStack stk = any_stack(sv, s_stack);
+        out[i] = segment_blocked(sv, ld3(p + 3 * i), ld3(q + 3 * i), tmin, stk, st);
     }
     record_stats<kStats>(d_stats, i, st);
 }
@@ -143,28 +151,34 @@ void launch_trace_closest(rzk_ctx* ctx, const SceneView& sv, const float* d_rays
     RZK_HIP(hipGetLastError());
 }
 
-void launch_trace_any(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int64_t n,
+void launch_trace_any(rzk_ctx* ctx, const SceneView& sv_in, const float* d_rays, int64_t n,
                       float tmin, float tmax, uint8_t* d_out, int* d_stats) {
     if (n <= 0) return;
-    size_t lds = sizeof(int) * kTraceBlock * (size_t)sv.depth;
+    SceneView sv = sv_in;
+    unsigned grid = grid_for(n, kTraceBlock);
+    bind_spill(ctx, sv, (int64_t)grid * kTraceBlock);
+    size_t lds = sizeof(int) * kTraceBlock * (size_t)kLdsStack;
     if (d_stats)
-        hipLaunchKernelGGL(k_trace_any<2>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+        hipLaunchKernelGGL(k_trace_any<2>, dim3(grid), dim3(kTraceBlock), lds,
                            ctx->stream, sv, d_rays, n, tmin, tmax, d_out, d_stats);
     else
-        hipLaunchKernelGGL(k_trace_any<0>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+        hipLaunchKernelGGL(k_trace_any<0>, dim3(grid), dim3(kTraceBlock), lds,
                            ctx->stream, sv, d_rays, n, tmin, tmax, d_out, d_stats);
     RZK_HIP(hipGetLastError());
 }
 
-void launch_trace_segments(rzk_ctx* ctx, const SceneView& sv, const float* d_p, const float* d_q,
+void launch_trace_segments(rzk_ctx* ctx, const SceneView& sv_in, const float* d_p, const float* d_q,
                            int64_t n, float tmin, uint8_t* d_out, int* d_stats) {
     if (n <= 0) return;
-    size_t lds = sizeof(int) * kTraceBlock * (size_t)sv.depth;
+    SceneView sv = sv_in;
+    unsigned grid = grid_for(n, kTraceBlock);
+    bind_spill(ctx, sv, (int64_t)grid * kTraceBlock);
+    size_t lds = sizeof(int) * kTraceBlock * (size_t)kLdsStack;
     if (d_stats)
-        hipLaunchKernelGGL(k_trace_segments<2>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+        hipLaunchKernelGGL(k_trace_segments<2>, dim3(grid), dim3(kTraceBlock), lds,
                            ctx->stream, sv, d_p, d_q, n, tmin, d_out, d_stats);
     else
-        hipLaunchKernelGGL(k_trace_segments<0>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
+        hipLaunchKernelGGL(k_trace_segments<0>, dim3(grid), dim3(kTraceBlock), lds,
                            ctx->stream, sv, d_p, d_q, n, tmin, d_out, d_stats);
     RZK_HIP(hipGetLastError());
 }
@@ -173,7 +187,7 @@ void launch_trace_segments(rzk_ctx* ctx, const SceneView& sv, const float* d_p, 
 void launch_trace_stats(rzk_ctx* ctx, const SceneView& sv, int kind, const float* d_rays, int64_t n,
                         float tmin, float tmax, void* d_scratch_out, int* d_stats) {
     if (n <= 0) return;
-    size_t lds = sizeof(int) * kTraceBlock * (size_t)sv.depth;
+    size_t lds = sizeof(int) * kTraceBlock * (size_t)std::max(sv.depth, kLdsStack);
     if (kind == 0)
         hipLaunchKernelGGL(k_trace_closest<1>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
                            ctx->stream, sv, d_rays, n, tmin, tmax, (float4*)d_scratch_out, d_stats);
