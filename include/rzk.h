@@ -201,7 +201,8 @@ typedef struct rzk_lightslice_params {
     float n_emitted;                /* num_emitted_photons */
     uint32_t seed;                  /* MT19937 seed for centre / representative sampling */
     int32_t map_gate;               /* 1: build maps only where sampling_records[s].is_valid
-                                       would be true (lightslice.cc:2070); see DESIGN */
+                                       would be true (lightslice.cc:2070); 0: every
+                                       slice with a non-empty map */
 } rzk_lightslice_params;
 
 typedef struct rzk_lightslice_info {
@@ -237,6 +238,29 @@ int rzk_lightslice_count_work(rzk_lightslice *ls, int64_t *out);
  *  "slice_wi" (slices*clusters*3 f32), "maps" (slices*res*res f32), "image" (W*H*3 f32)
  * bytes must equal the array size. */
 int rzk_lightslice_get(const rzk_lightslice *ls, const char *name, void *dst, int64_t bytes);
+
+/* ---- guided path tracing (VirtualPointLight::trace_wellington, vpl.cc:934-1041,
+ * METRO_RANDOM; next-event estimation FirstBounce/AreaLight::first_bounce,
+ * area_light.cc:48-142) ---------------------------------------------------- */
+typedef struct rzk_pt_params {
+    int32_t spp;                    /* samples per pixel (a square count is stratified) */
+    int32_t max_bounce;             /* path depth (trace_wellington max_bounce) */
+    int32_t direct_lighting;        /* Integrator::is_direct_lighting */
+    int32_t guide_mode;             /* 0: uniform-hemisphere direction sampler everywhere (what the
+                                       reference's gated sampler does, lightslice.cc:1654-1665);
+                                       1: the slice's luminance map at the first bounce */
+    uint32_t seed;                  /* Philox key: results depend only on (seed, pixel, sample) */
+    float tmin;
+    int32_t x0, y0, w, h;           /* window to render; w = h = 0 for the whole image */
+} rzk_pt_params;
+
+/* Guided path tracing of the LightSlice's image with its per-slice guiding
+ * distributions (rzk_lightslice_initialize must have run).  out_rgb: host
+ * [height][width][3] (pixels outside the window are 0) or NULL to keep the
+ * image on the device; out_rays (optional, 2 x int64): closest-hit rays and
+ * shadow rays traced. */
+int rzk_lightslice_render_guided(rzk_lightslice *ls, const float *camera, const rzk_pt_params *params,
+                                 float *out_rgb, int64_t *out_rays);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,9 @@ def lib():
             "orc_slice_cluster_info": ([P, P, I, P, P, P, P, I, F, P, P], None),
             "orc_slice_cluster_pdf": ([I, P, I, P, I, P, P, P, P], None),
             "orc_build_map": ([I, I, P, P, P, P, P, F, I, P, P, P, P, P], I),
+            "orc_scene_set_lights": ([P, I, P], None), "orc_philox": ([P, C.c_uint32, C.c_uint32], None),
+            "orc_bsdf_pdf": ([P, P, P, P, P], None), "orc_bsdf_sample": ([P, P, P, P, P, P, P], None),
+            "orc_guided_pt": ([P, P, I, I, I, I, C.c_uint32, F, I, I, P, P, P, P, I, I, I, I, P, I], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -96,6 +99,10 @@ class OrcScene:
                                          mat.shape[0], _p(mat))
         self.ntri = len(ts)
         self.materials = mat
+        rad = _f(sc.light_radiance).reshape(-1, 3)
+        self._keep.append(rad)
+        if len(rad) > 0:
+            self.L.orc_scene_set_lights(self.h, len(rad), _p(rad))
 
     def __del__(self):
         try:
@@ -186,6 +193,39 @@ class OrcScene:
         self.L.orc_slice_cluster_info(self.h, _p(_f(recv_row)), len(reps), _p(reps), _p(_f(vpl)), _p(_i(vmat)),
                                       _p(_f(Vrow)), clamp_mode, clamp, _p(wi), _p(up))
         return wi, up
+
+
+def guided_pt(osc, cam, spp, max_bounce, direct, guide_mode, seed, tmin, tile, res=0, maps=None, row_cdf=None,
+              col_cdf=None, totals=None, window=None, nthreads=8):
+    """guided path tracing of a pixel window; returns the full [H][W][3] image
+    (pixels outside the window stay 0)"""
+    cam = _f(cam)
+    W, H = int(cam[17]), int(cam[18])
+    x0, y0, w, h = window if window is not None else (0, 0, W, H)
+    out = np.zeros((H, W, 3), np.float32)
+    arrs = [None if a is None else _f(a) for a in (maps, row_cdf, col_cdf, totals)]
+    lib().orc_guided_pt(osc.h, _p(cam), spp, max_bounce, direct, guide_mode, seed, tmin, tile, res,
+                        *[_p(a) for a in arrs], x0, y0, w, h, _p(out), nthreads)
+    return out
+
+
+def philox(counter, k0, k1):
+    c = np.ascontiguousarray(counter, np.uint32).copy()
+    lib().orc_philox(_p(c), k0, k1)
+    return c
+
+
+def bsdf_pdf(mat, uvn, wo, wi):
+    out = np.zeros(1, np.float32)
+    lib().orc_bsdf_pdf(_p(_f(mat)), _p(_f(uvn)), _p(_f(wo)), _p(_f(wi)), _p(out))
+    return float(out[0])
+
+
+def bsdf_sample(mat, uvn, wo, u):
+    wi, pdf, f = np.zeros(3, np.float32), np.zeros(1, np.float32), np.zeros(3, np.float32)
+    lib().orc_bsdf_sample(_p(_f(mat)), _p(_f(uvn)), _p(_f(wo)), _p(np.ascontiguousarray(u, np.uint32)), _p(wi),
+                          _p(pdf), _p(f))
+    return wi, float(pdf[0]), f
 
 
 def camera_rays(cam, px, py):

@@ -241,7 +241,13 @@ int orc_find_split(const uint32_t *codes, int first, int last) {
 
 typedef struct { float bmin[3], bmax[3]; int start, end, left, right, level; } node_t;
 
+typedef struct { v3 p0, p1, p2, n; float area, cum, ratio; int light, tri; } ltri_t;
+typedef struct { v3 le; int first, count; float sel_cdf, sel_pdf; } light_t;
+
 struct orc_scene {
+    int nlights, nltri;
+    light_t *lights;
+    ltri_t *ltris;
     int ntri;
     v3 *p0, *p1, *p2, *n, *tan;     /* original order */
     int *tri_surface;
@@ -376,6 +382,7 @@ void orc_scene_destroy(orc_scene *s) {
     free(s->p0); free(s->p1); free(s->p2); free(s->n); free(s->tan); free(s->tri_surface);
     free(s->surf_material); free(s->surf_light); free(s->surf_flags); free(s->mat);
     free(s->nodes); free(s->sorted_tri); free(s->sorted_morton);
+    free(s->lights); free(s->ltris);
     free(s);
 }
 
@@ -1088,4 +1095,410 @@ int orc_build_map(int res, int nreps, const float *slice_wi, const int *upper,
     free(rs);
     *total = tot;
     return tot > 0.0f;
+}
+
+
+/* ======================================================================= */
+/* Guided path tracing                                                      */
+/* ======================================================================= */
+/* light tables: emitting triangles per light in surface order, triangle
+ * choice by cumulative area ratio (quad.cc:75-87), light choice by a
+ * DiscretePdf over |Le| x area */
+void orc_scene_set_lights(orc_scene *s, int nlights, const float *radiance) {
+    free(s->lights); free(s->ltris);
+    s->nlights = nlights;
+    s->lights = calloc(nlights + 1, sizeof(light_t));
+    s->ltris = calloc(s->ntri + 1, sizeof(ltri_t));
+    s->nltri = 0;
+    for (int li = 0; li < nlights; ++li) {
+        light_t *L = &s->lights[li];
+        L->le = ld(radiance + 3 * li);
+        L->first = s->nltri;
+        float total = 0.f;
+        for (int k = 0; k < s->ntri; ++k) {
+            int sid = s->tri_surface[k];
+            if (s->surf_light[sid] != li) continue;
+            v3 cr = cross(sub(s->p0[k], s->p1[k]), sub(s->p0[k], s->p2[k]));
+            ltri_t *t = &s->ltris[s->nltri++];
+            t->p0 = s->p0[k]; t->p1 = s->p1[k]; t->p2 = s->p2[k];
+            t->area = len(cr) * 0.5f;                       /* triangle.cc:15 */
+            t->n = unit(cr);
+            if (s->surf_flags[sid] & ORC_SURF_FLIPPED) t->n = neg(t->n);
+            t->light = li; t->tri = k;
+            total += t->area;
+        }
+        L->count = s->nltri - L->first;
+        float cum = 0.f;
+        for (int q = 0; q < L->count; ++q) {
+            ltri_t *t = &s->ltris[L->first + q];
+            t->ratio = t->area / total;
+            cum += t->ratio;
+            t->cum = q == L->count - 1 ? 1.0f : cum;
+        }
+        L->sel_cdf = total * rgb_value(L->le);
+    }
+    float sum = 0.f;
+    for (int i = 0; i < nlights; ++i) sum += s->lights[i].sel_cdf;
+    float cum = 0.f;
+    for (int i = 0; i < nlights; ++i) {
+        light_t *L = &s->lights[i];
+        L->sel_pdf = sum > 0.f ? L->sel_cdf / sum : 0.f;
+        cum += L->sel_pdf;
+        L->sel_cdf = i + 1 == nlights ? 1.0f : cum;
+    }
+}
+
+/* Philox4x32-10 (Salmon et al., SC'11) */
+void orc_philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+static inline float u24(uint32_t r) { return (float)(r & 0x00ffffffu) * (1.0f / 16777216.0f); }
+static inline float u24o(uint32_t r) { return (float)(r & 0x00ffffffu) * (1.0f / 16777216.0f) + (1.0f / 33554432.0f); }
+
+static void path_rng(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t u[12]) {
+    for (uint32_t b = 0; b < 3; ++b) {
+        uint32_t c[4] = {sample, bounce, b, 0x52a7u};
+        orc_philox(c, seed ^ 0x9a3c5e17u, pixel);
+        u[4 * b] = c[0]; u[4 * b + 1] = c[1]; u[4 * b + 2] = c[2]; u[4 * b + 3] = c[3];
+    }
+}
+
+static float rgb_max(v3 c) { float t = c.x; if (c.y > t) t = c.y; if (c.z > t) t = c.z; return t; }
+
+static void lobe_weights(const float *m, float *rd, float *rs) {   /* phong.cc:6-16, ward.cc:6-22 */
+    *rd = rgb_max(ld(m + 1));
+    *rs = rgb_max(ld(m + 4));
+    float total = *rd + *rs;
+    if (total > 0.0f) { *rd /= total; *rs /= total; }
+}
+
+static v3 l2w(v3 u, v3 v, v3 w, v3 l) { return add(add(scl(u, l.x), scl(v, l.y)), scl(w, l.z)); }   /* onb.cc:117-119 */
+static v3 w2l(v3 u, v3 v, v3 w, v3 x) { return mk(dot(u, x), dot(v, x), dot(w, x)); }
+
+static v3 sph2w(v3 u, v3 v, v3 w, float theta, float phi) {       /* onb.cc:155-163 */
+    float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);
+    return l2w(u, v, w, mk(st * cp, st * sp, ct));
+}
+
+/* lambertian.cc:70-75, phong.cc:89-102, ward.cc:118-134 */
+static float bsdf_pdf(const float *m, v3 u, v3 v, v3 w, v3 wo, v3 wi) {
+    if (dot(wi, w) <= ZERO_EPS || dot(wo, w) <= ZERO_EPS) return 0.0f;
+    int type = (int)m[0];
+    if (type == ORC_LAMBERT) return dot(w, wi) * INV_PI;
+    float rd, rs;
+    lobe_weights(m, &rd, &rs);
+    if (type == ORC_PHONG) {
+        float ne = m[7];
+        float pdf = rd * (dot(w, wi) * INV_PI);
+        v3 wr = sub(scl(w, 2.0f * dot(wi, w)), wi);
+        float ca = dot(wo, wr);
+        if (ca > 0.0f) pdf += rs * (ne + 1.0f) * INV_2PI * powf(ca, ne);
+        return pdf;
+    }
+    float ax = m[7], ay = m[8];
+    float ax2 = ax * ax, ay2 = ay * ay;
+    float inv4 = 1.0f / (4.0f * A_PI * ax * ay);
+    float z = dot(w, wi);
+    v3 wh = unit(w2l(u, v, w, add(wi, wo)));
+    float expo = expf(-(wh.x * wh.x / ax2 + wh.y * wh.y / ay2) / (wh.z * wh.z));
+    float ct = wh.z;
+    v3 lwo = unit(w2l(u, v, w, wo));
+    float ct3 = ct * ct * ct;
+    return rd * (z * INV_PI) + rs * (inv4 / (dot(wh, lwo) * ct3) * expo);
+}
+
+void orc_bsdf_pdf(const float mat[ORC_MAT_STRIDE], const float uvn[9], const float wo[3], const float wi[3],
+                  float *pdf) {
+    *pdf = bsdf_pdf(mat, ld(uvn), ld(uvn + 3), ld(uvn + 6), ld(wo), ld(wi));
+}
+
+static v3 eval3(const float *m, v3 u, v3 v, v3 w, v3 wo, v3 wi) {
+    float o[3];
+    bsdf_eval(m, u, v, w, wo, wi, o);
+    return ld(o);
+}
+
+/* lambertian.cc:5-27, phong.cc:21-72, ward.cc:24-101 (u[0] lobe, u[1..2] direction) */
+static v3 bsdf_sample(const float *m, v3 u, v3 v, v3 w, v3 wo, const uint32_t *r, v3 *wi, float *pdf) {
+    *wi = mk(0.f, 0.f, 0.f);
+    *pdf = 0.0f;
+    if (dot(wo, w) <= ZERO_EPS) return mk(0.f, 0.f, 0.f);
+    int type = (int)m[0];
+    if (type == ORC_LAMBERT) {
+        float phi = TWO_PI * u24(r[1]);
+        float st = sqrtf(u24(r[2]));
+        float x = st * cosf(phi), y = st * sinf(phi);
+        float z = sqrtf(1.0f - st * st);
+        *wi = unit(l2w(u, v, w, mk(x, y, z)));
+        *pdf = z * INV_PI;
+        return scl(ld(m + 1), INV_PI);
+    }
+    float rd, rs;
+    lobe_weights(m, &rd, &rs);
+    if (u24(r[0]) < rd) {
+        float phi = TWO_PI * u24(r[1]);
+        float st = sqrtf(u24(r[2]));
+        float x = st * cosf(phi), y = st * sinf(phi);
+        float z = sqrtf(1.0f - st * st);
+        *wi = unit(l2w(u, v, w, mk(x, y, z)));
+    } else if (type == ORC_PHONG) {
+        float ne = m[7];
+        v3 ou, ov, ow;
+        onb_from_nu(wo, w, &ou, &ov, &ow);
+        float phi = TWO_PI * u24(r[1]);
+        float alpha = acosf(powf(u24(r[2]), 1.0f / (ne + 1.0f)));
+        v3 wr = sph2w(ou, ov, ow, alpha, phi);
+        float d = dot(wr, w);
+        if (d < 0.0f) return mk(0.f, 0.f, 0.f);
+        *wi = sub(scl(w, 2.0f * d), wr);
+    } else {
+        float ax = m[7], ay = m[8];
+        float ax2 = ax * ax, ay2 = ay * ay, ay_ax = ay / ax;
+        float s0 = u24o(r[1]), s1 = u24o(r[2]);
+        v3 lwo = unit(w2l(u, v, w, wo));
+        float angle = TWO_PI * s0;
+        float phi = atanf(ay_ax * tanf(angle));
+        if (sinf(phi) * sinf(angle) < 0 || cosf(phi) * cosf(angle) < 0) phi += A_PI;
+        float cp = cosf(phi), sp = sinf(phi);
+        float theta = atanf(sqrtf(-logf(s1) / (cp * cp / ax2 + sp * sp / ay2)));
+        float st = sinf(theta), ct = cosf(theta);
+        v3 wh = mk(st * cp, st * sp, ct);
+        v3 lwi = sub(scl(wh, 2.0f * dot(lwo, wh)), lwo);
+        *wi = l2w(u, v, w, lwi);
+    }
+    *pdf = bsdf_pdf(m, u, v, w, wo, *wi);
+    return eval3(m, u, v, w, wo, *wi);
+}
+
+void orc_bsdf_sample(const float mat[ORC_MAT_STRIDE], const float uvn[9], const float wo[3], const uint32_t u[3],
+                     float wi[3], float *pdf, float brdf[3]) {
+    v3 w_;
+    v3 f = bsdf_sample(mat, ld(uvn), ld(uvn + 3), ld(uvn + 6), ld(wo), u, &w_, pdf);
+    st(wi, w_);
+    st(brdf, f);
+}
+
+static v3 light_query(const light_t *L, v3 n, v3 wo) {          /* area_light.cc:19-25 */
+    if (dot(wo, n) < ZERO_EPS) return mk(0.f, 0.f, 0.f);
+    return L->le;
+}
+
+static float balance(float a, float b) { return a / (a + b); } /* mis.h:11-13 */
+
+/* FirstBounce::radiance -> AreaLight::first_bounce (area_light.cc:48-142) */
+static v3 next_event(const orc_scene *s, const float *rc, int m, const uint32_t *u, float tmin) {
+    float ul = u24(u[1]);
+    int k = -1;
+    for (int i = 0; i < s->nlights; ++i) if (ul < s->lights[i].sel_cdf) { k = i; break; }
+    if (k < 0 || s->lights[k].count <= 0) return mk(0.f, 0.f, 0.f);
+    const light_t *L = &s->lights[k];
+    const float *mat = s->mat + ORC_MAT_STRIDE * m;
+    v3 p = ld(rc), pn = ld(rc + 6), wo = ld(rc + 9), bu = ld(rc + 12), bv = ld(rc + 15), bw = ld(rc + 18);
+    v3 Lo = mk(0.f, 0.f, 0.f);
+    float ut = u24(u[2]);
+    int ti = L->first + L->count - 1;
+    for (int q = 0; q < L->count; ++q) if (ut <= s->ltris[L->first + q].cum) { ti = L->first + q; break; }
+    const ltri_t *T = &s->ltris[ti];
+    float tmp = sqrtf(1.0f - u24(u[3]));
+    float bb = 1.0f - tmp, cc = u24(u[4]) * tmp;
+    float aa = 1.0f - bb - cc;
+    v3 q = add(add(scl(T->p0, aa), scl(T->p1, bb)), scl(T->p2, cc));   /* triangle.cc:298-308 */
+    v3 qn = T->n;
+    float pdf = T->ratio * (1.0f / T->area);
+    if (pdf > 0.0f && !segment_blocked(s, p, q, tmin, NULL)) {
+        v3 wi = unit(sub(q, p));
+        v3 Le = light_query(L, qn, neg(wi));
+        float G = ff_abs(p, pn, q, qn);
+        if (G > 0.0f) {
+            v3 brdf = eval3(mat, bu, bv, bw, wo, wi);
+            float dpdf = bsdf_pdf(mat, bu, bv, bw, wo, wi);
+            v3 pq = sub(q, p);
+            float w1 = balance(pdf * dot(pq, pq) / fabsf(dot(unit(sub(p, q)), qn)), dpdf);
+            Lo = add(Lo, mulv(scl(Le, (w1 * G) / pdf), brdf));
+        }
+    }
+    v3 wi;
+    float dpdf;
+    bsdf_sample(mat, bu, bv, bw, wo, u + 5, &wi, &dpdf);
+    if (dpdf > 0.0f) {
+        ray_t r2 = ray_ctor(p, wi);
+        float t = 0, tb = 0, tg = 0;
+        int tri = -1;
+        if (bvh_closest(s, &r2, tmin, 1e6f, &t, &tb, &tg, &tri, NULL)) {
+            int sid = s->tri_surface[tri];
+            if (s->surf_light[sid] == k) {
+                v3 qh = add(p, scl(wi, t));
+                v3 n = s->n[tri];
+                v3 qhn = unit(add(add(n, scl(sub(n, n), tb)), scl(sub(n, n), tg)));
+                if (s->surf_flags[sid] & ORC_SURF_FLIPPED) qhn = neg(qhn);
+                v3 Le = light_query(L, qhn, neg(wi));
+                float pa = 0.f;
+                for (int qq = 0; qq < L->count; ++qq)
+                    if (s->ltris[L->first + qq].tri == tri) { pa = s->ltris[L->first + qq].ratio * (1.0f / s->ltris[L->first + qq].area); break; }
+                v3 pq = sub(qh, p);
+                float lpdf = pa * dot(pq, pq) / fabsf(dot(unit(sub(p, qh)), qhn));
+                float w2 = balance(dpdf, lpdf);
+                v3 brdf = eval3(mat, bu, bv, bw, wo, wi);
+                float cosv = fabsf(dot(wi, pn));
+                Lo = add(Lo, mulv(scl(Le, (w2 * cosv) / dpdf), brdf));
+            }
+        }
+    }
+    return mk(Lo.x / L->sel_pdf, Lo.y / L->sel_pdf, Lo.z / L->sel_pdf);
+}
+
+typedef struct {
+    const orc_scene *s; const float *cam; int spp, max_bounce, direct, guide_mode; uint32_t seed; float tmin;
+    int tile, res, tiles_x, tiles_y, W, x0, y0, w, h;
+    const float *maps, *row_cdf, *col_cdf, *totals; float *out;
+} pt_args;
+
+static float map_pdf(const pt_args *a, int slice, v3 local) {      /* lightslice.cc:2352-2377 */
+    if (local.z <= 0.0f) return 0.0f;
+    float sq[2], l3[3] = {local.x, local.y, local.z};
+    orc_direction_to_square(l3, sq);
+    int res = a->res;
+    int col = (int)(sq[0] * res), row = (int)(sq[1] * res);
+    if (col >= res) col = res - 1;
+    if (row >= res) row = res - 1;
+    float pdf_k = a->maps[((size_t)slice * res + row) * res + col] / a->totals[slice];
+    float pus = (float)(res * res) * pdf_k;
+    float pud = pus * INV_PI;
+    return pud * 0.5f;
+}
+
+static float guide_pdf(const pt_args *a, int slice, v3 u, v3 v, v3 w, v3 sn, v3 wi) {
+    if (dot(sn, wi) <= 0.0f) return 0.0f;
+    if (slice < 0) return INV_2PI;
+    return map_pdf(a, slice, w2l(u, v, w, wi));
+}
+
+static v3 guide_sample(const pt_args *a, int slice, const float *mat, v3 u, v3 v, v3 w, v3 sn, v3 wo,
+                       const uint32_t *r, v3 *wi, float *pdf) {
+    if (slice < 0) {
+        float phi = TWO_PI * u24(r[0]);
+        float theta = acosf(u24(r[1]));
+        *wi = unit(sph2w(u, v, w, theta, phi));
+    } else {
+        int res = a->res;
+        float ur = u24(r[0]), uc = u24(r[1]);
+        const float *rcdf = a->row_cdf + (size_t)slice * res;
+        int row = -1, col = -1;
+        for (int i = 0; i < res; ++i) if (ur < rcdf[i]) { row = i; break; }
+        if (row >= 0) {
+            const float *ccdf = a->col_cdf + ((size_t)slice * res + row) * res;
+            for (int j = 0; j < res; ++j) if (uc < ccdf[j]) { col = j; break; }
+        }
+        if (row < 0 || col < 0) { *wi = mk(0.f, 0.f, 0.f); *pdf = 0.f; return mk(0.f, 0.f, 0.f); }
+        float sq[2] = {((float)col + u24(r[2])) / (float)res, ((float)row + u24(r[3])) / (float)res};
+        float l3[3];
+        orc_square_to_direction(sq, l3);
+        *wi = l2w(u, v, w, ld(l3));
+    }
+    *pdf = guide_pdf(a, slice, u, v, w, sn, *wi);
+    return eval3(mat, u, v, w, wo, *wi);
+}
+
+/* Camera::shoot_ray(const SensorSample&), camera.cc:67-80 */
+static ray_t camera_sample_ray(const float *c, float x, float y) {
+    v3 u = ld(c + 3), v = ld(c + 6), w = ld(c + 9);
+    float height = c[18];
+    float fx = (x - c[12]) / c[14];
+    float fy = ((height - y) - c[13]) / c[15];
+    v3 d = unit(l2w(u, v, w, mk(fx, fy, -c[16])));
+    return ray_ctor(ld(c), d);
+}
+
+static void pt_range(void *a_, int lo, int hi) {
+    pt_args *a = a_;
+    const orc_scene *s = a->s;
+    for (int t = lo; t < hi; ++t) {
+        int x = a->x0 + t % a->w, y = a->y0 + t / a->w;
+        uint32_t pixel = (uint32_t)(y * a->W + x);
+        int slice = -1;
+        if (a->guide_mode == 1) {
+            int sx = x / a->tile, sy = y / a->tile;
+            if (sx > a->tiles_x - 1) sx = a->tiles_x - 1;
+            if (sy > a->tiles_y - 1) sy = a->tiles_y - 1;
+            int sl = sy * a->tiles_x + sx;
+            if (a->totals[sl] > 0.0f) slice = sl;
+        }
+        int strata = 1;
+        while ((strata + 1) * (strata + 1) <= a->spp) ++strata;
+        int stratified = strata * strata == a->spp;
+        v3 acc = mk(0.f, 0.f, 0.f);
+        for (int smp = 0; smp < a->spp; ++smp) {
+            uint32_t r0[12];
+            path_rng(a->seed, pixel, (uint32_t)smp, 0u, r0);
+            float jx = u24(r0[0]), jy = u24(r0[1]);
+            if (stratified) {
+                jx = ((float)(smp % strata) + jx) / (float)strata;
+                jy = ((float)(smp / strata) + jy) / (float)strata;
+            }
+            ray_t ray = camera_sample_ray(a->cam, (float)x + jx, (float)y + jy);
+            v3 L = mk(0.f, 0.f, 0.f), T = mk(1.f, 1.f, 1.f);
+            float th = 0, tb = 0, tg = 0;
+            int tri = -1;
+            bvh_closest(s, &ray, a->tmin, 1e6f, &th, &tb, &tg, &tri, NULL);
+            for (int bounce = 1; tri >= 0 && bounce <= a->max_bounce; ++bounce) {
+                float rc[ORC_RECV_STRIDE];
+                int m, lidx;
+                fill_receiver(s, &ray, th, tb, tg, tri, rc, &m, &lidx);
+                if (lidx >= 0) {
+                    if (bounce == 1 && a->direct) L = add(L, mulv(T, light_query(&s->lights[lidx], ld(rc + 6), ld(rc + 9))));
+                    break;
+                }
+                if (m < 0) break;
+                uint32_t u[12];
+                path_rng(a->seed, pixel, (uint32_t)smp, (uint32_t)bounce, u);
+                int brdf_sampling = u24(u[0]) < 0.5f;
+                if (bounce > 1 || a->direct) L = add(L, mulv(T, next_event(s, rc, m, u, a->tmin)));
+                if (bounce >= a->max_bounce) break;
+                const float *mat = s->mat + ORC_MAT_STRIDE * m;
+                v3 bu = ld(rc + 12), bv = ld(rc + 15), bw = ld(rc + 18);
+                v3 sn = ld(rc + 6), wo = ld(rc + 9), p = ld(rc);
+                int gslice = bounce == 1 ? slice : -1;
+                v3 wi, brdf;
+                float pdf;
+                if (brdf_sampling) brdf = bsdf_sample(mat, bu, bv, bw, wo, u + 8, &wi, &pdf);
+                else brdf = guide_sample(a, gslice, mat, bu, bv, bw, sn, wo, u + 8, &wi, &pdf);
+                if (!(pdf > 0.0f)) break;
+                ray = ray_ctor(p, wi);
+                tri = -1;
+                bvh_closest(s, &ray, a->tmin, 1e6f, &th, &tb, &tg, &tri, NULL);
+                if (tri < 0) break;
+                if (s->surf_light[s->tri_surface[tri]] >= 0) break;
+                float po = brdf_sampling ? guide_pdf(a, gslice, bu, bv, bw, sn, wi) : bsdf_pdf(mat, bu, bv, bw, wo, wi);
+                float weight = pdf / (pdf + po);
+                float cosv = fabsf(dot(wi, sn));
+                float sc = ((weight * cosv) / pdf) * 2.0f;
+                T = scl(mulv(T, brdf), sc);
+            }
+            acc = add(acc, L);
+        }
+        size_t o = (size_t)y * a->W + x;
+        a->out[3 * o] = acc.x / (float)a->spp;
+        a->out[3 * o + 1] = acc.y / (float)a->spp;
+        a->out[3 * o + 2] = acc.z / (float)a->spp;
+    }
+}
+
+void orc_guided_pt(const orc_scene *s, const float cam[ORC_CAM_STRIDE], int spp, int max_bounce, int direct,
+                   int guide_mode, uint32_t seed, float tmin, int tile, int res, const float *maps,
+                   const float *row_cdf, const float *col_cdf, const float *totals, int x0, int y0, int w,
+                   int h, float *out, int nthreads) {
+    int W = (int)cam[17], H = (int)cam[18];
+    pt_args a = {s, cam, spp, max_bounce, direct, guide_mode, seed, tmin, tile, res,
+                 (W + tile - 1) / tile, (H + tile - 1) / tile, W, x0, y0, w, h, maps, row_cdf, col_cdf, totals, out};
+    parallel_for(w * h, nthreads, pt_range, &a);
 }

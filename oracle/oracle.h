@@ -123,6 +123,18 @@ int  orc_build_map(int res, int nreps, const float *slice_wi, const int *upper,
                    int valid, float *map, float *row_cdf, float *row_pdf, float *col_cdf,
                    float *total);
 
+/* ---- guided path tracing (vpl.cc:934-1041, area_light.cc:48-142) ---- */
+void orc_scene_set_lights(orc_scene *s, int nlights, const float *radiance);
+void orc_philox(uint32_t c[4], uint32_t k0, uint32_t k1);
+void orc_bsdf_pdf(const float mat[ORC_MAT_STRIDE], const float uvn[9], const float wo[3], const float wi[3],
+                  float *pdf);
+void orc_bsdf_sample(const float mat[ORC_MAT_STRIDE], const float uvn[9], const float wo[3], const uint32_t u[3],
+                     float wi[3], float *pdf, float brdf[3]);
+void orc_guided_pt(const orc_scene *s, const float cam[ORC_CAM_STRIDE], int spp, int max_bounce, int direct,
+                   int guide_mode, uint32_t seed, float tmin, int tile, int res, const float *maps,
+                   const float *row_cdf, const float *col_cdf, const float *totals, int x0, int y0, int w,
+                   int h, float *out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,7 @@ EXPORTS = [
     "rzk_reduced_matrix", "rzk_alphas", "rzk_cluster_assign",
     "rzk_lightslice_create", "rzk_lightslice_destroy", "rzk_lightslice_initialize",
     "rzk_lightslice_gather", "rzk_lightslice_get_info", "rzk_lightslice_count_work", "rzk_lightslice_get",
+    "rzk_lightslice_render_guided",
 ]
 
 
@@ -61,6 +62,14 @@ class LightSliceParams(C.Structure):
         ("num_neighbor_slices", C.c_int32), ("map_resolution", C.c_int32), ("map_epsilon", C.c_float),
         ("clamp_mode", C.c_int32), ("clamp", C.c_float), ("incident_clamp", C.c_float), ("tmin", C.c_float),
         ("n_emitted", C.c_float), ("seed", C.c_uint32), ("map_gate", C.c_int32),
+    ]
+
+
+class PtParams(C.Structure):
+    _fields_ = [
+        ("spp", C.c_int32), ("max_bounce", C.c_int32), ("direct_lighting", C.c_int32), ("guide_mode", C.c_int32),
+        ("seed", C.c_uint32), ("tmin", C.c_float), ("x0", C.c_int32), ("y0", C.c_int32), ("w", C.c_int32),
+        ("h", C.c_int32),
     ]
 
 
@@ -119,6 +128,7 @@ def load() -> C.CDLL:
         "rzk_lightslice_gather": [P, P],
         "rzk_lightslice_get_info": [P, C.POINTER(LightSliceInfo)],
         "rzk_lightslice_count_work": [P, P],
+        "rzk_lightslice_render_guided": [P, P, C.POINTER(PtParams), P, P],
         "rzk_lightslice_get": [P, C.c_char_p, P, I64],
     }
     for name, args in sig.items():
@@ -362,6 +372,7 @@ class LightSlice:
                   "rzk_lightslice_create")
         self.h = h
         self.w, self.hgt = defaults["width"], defaults["height"]
+        self.res = defaults["map_resolution"]
 
     def initialize(self, cam):
         cam = f32(cam)
@@ -376,6 +387,18 @@ class LightSlice:
         inf = LightSliceInfo()
         self.lib.rzk_lightslice_get_info(self.h, C.byref(inf))
         return inf
+
+    def render_guided(self, cam, spp=16, max_bounce=5, direct_lighting=0, guide_mode=1, seed=2013, tmin=1e-3,
+                      window=None, to_host=True):
+        p = PtParams(spp, max_bounce, direct_lighting, guide_mode, seed, tmin, 0, 0, 0, 0)
+        if window is not None:
+            p.x0, p.y0, p.w, p.h = window
+        cam = f32(cam)
+        out = np.zeros((self.hgt, self.w, 3), np.float32) if to_host else None
+        rays = np.zeros(2, np.int64)
+        self.ctx.check(self.lib.rzk_lightslice_render_guided(self.h, ptr(cam), C.byref(p), ptr(out), ptr(rays)),
+                       "rzk_lightslice_render_guided")
+        return out, {"closest_rays": int(rays[0]), "shadow_rays": int(rays[1])}
 
     def count_work(self):
         out = np.zeros(8, np.int64)

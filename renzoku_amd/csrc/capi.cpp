@@ -86,6 +86,9 @@ SceneView scene_view(const rzk_scene* s) {
     v.root_ref = s->bvh.root_ref;
     v.depth = s->bvh.depth > 0 ? s->bvh.depth : 1;
     v.root4_ref = s->bvh.root4_ref;
+    v.ltris = s->d_ltris;
+    v.lights = s->d_lights;
+    v.nlights = (int)s->h_lights.size();
     v.stack4 = s->bvh.stack4;
     v.spill = nullptr;
     v.spill_pitch = 0;
@@ -230,6 +233,62 @@ int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
         up((void**)&s->d_tri_tan, tt.data(), sizeof(float4) * n);
         up((void**)&s->d_surf, surf.data(), sizeof(int4) * s->nsurf);
         up((void**)&s->d_mat, s->h_mat.empty() ? nullptr : s->h_mat.data(), sizeof(float) * s->h_mat.size());
+        // light tables: emitting triangles grouped per light in surface order;
+        // triangle choice by cumulative area ratio (quad.cc:75-87), light choice
+        // by a DiscretePdf over radiance magnitude x area (Scene::sample_light)
+        for (int li = 0; li < s->nlight; ++li) {
+            LightGpu L;
+            L.le[0] = s->h_light_radiance.empty() ? 0.f : s->h_light_radiance[3 * li];
+            L.le[1] = s->h_light_radiance.empty() ? 0.f : s->h_light_radiance[3 * li + 1];
+            L.le[2] = s->h_light_radiance.empty() ? 0.f : s->h_light_radiance[3 * li + 2];
+            L.first = (int)s->h_ltris.size();
+            float total = 0.f;
+            std::vector<float> areas;
+            for (int k = 0; k < n; ++k) {
+                int sid = d->tri_surface[k];
+                if (d->surf_light[sid] != li) continue;
+                const float* p = d->positions + 9 * k;
+                v3 p0 = ld3(p), p1 = ld3(p + 3), p2 = ld3(p + 6);
+                v3 cr = cross(p0 - p1, p0 - p2);
+                float area = length(cr) * 0.5f;                // triangle.cc:15
+                v3 nn = unit(cr);
+                if (d->surf_flags[sid] & RZK_SURF_FLIPPED) nn = -nn;   // surface.cc:98-102
+                LightTri t;
+                t.p0 = make_float4(p0.x, p0.y, p0.z, 0.f);
+                t.p1 = make_float4(p1.x, p1.y, p1.z, 0.f);
+                t.p2 = make_float4(p2.x, p2.y, p2.z, 0.f);
+                t.n = make_float4(nn.x, nn.y, nn.z, area);
+                t.light = li;
+                t.tri = k;
+                s->h_ltris.push_back(t);
+                areas.push_back(area);
+                total += area;
+            }
+            L.count = (int)s->h_ltris.size() - L.first;
+            float cum = 0.f;
+            for (int q = 0; q < L.count; ++q) {
+                LightTri& t = s->h_ltris[L.first + q];
+                t.ratio = areas[q] / total;
+                cum += t.ratio;
+                t.cum = q == L.count - 1 ? 1.0f : cum;
+            }
+            L.sel_cdf = total * rgb_value(mk(L.le[0], L.le[1], L.le[2]));   // weight, normalised below
+            s->h_lights.push_back(L);
+        }
+        {
+            float sum = 0.f;
+            for (auto& L : s->h_lights) sum += L.sel_cdf;
+            float cum = 0.f;
+            for (size_t i = 0; i < s->h_lights.size(); ++i) {
+                LightGpu& L = s->h_lights[i];
+                L.sel_pdf = sum > 0.f ? L.sel_cdf / sum : 0.f;
+                cum += L.sel_pdf;
+                L.sel_cdf = i + 1 == s->h_lights.size() ? 1.0f : cum;
+            }
+        }
+        s->nltri = (int)s->h_ltris.size();
+        up((void**)&s->d_ltris, s->h_ltris.empty() ? nullptr : s->h_ltris.data(), sizeof(LightTri) * s->h_ltris.size());
+        up((void**)&s->d_lights, s->h_lights.empty() ? nullptr : s->h_lights.data(), sizeof(LightGpu) * s->h_lights.size());
         *out = s.release();
     });
 }
@@ -237,7 +296,8 @@ int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
 int rzk_scene_destroy(rzk_scene* s) {
     if (!s) return RZK_OK;
     (void)hipSetDevice(s->ctx->device);
-    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_tris, s->d_tri_n, s->d_tri_tan, s->d_surf, s->d_mat};
+    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_tris, s->d_tri_n, s->d_tri_tan, s->d_surf, s->d_mat,
+                    s->d_ltris, s->d_lights};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;

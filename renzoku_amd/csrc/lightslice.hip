@@ -11,6 +11,7 @@
 #include <memory>
 
 #include "host_sampling.h"
+#include "pathtrace.h"
 #include "rzk_device.h"
 
 void launch_gather_fused(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
@@ -96,22 +97,6 @@ __global__ void k_slice_pdf_norm(int rows, int nreps, const float* __restrict__ 
     c[0] = p[0];
     for (int i = 1; i < nreps; ++i) c[i] = c[i - 1] + p[i];
     c[nreps - 1] = 1.0f;
-}
-
-// Shirley-Chiu concentric map, direction -> unit square (concentric_mapping.h:49-106)
-__device__ __forceinline__ void direction_to_square(v3 wi, float& sx, float& sy) {
-    float r2 = 1.0f - wi.z;
-    float dx = wi.x / sqrtf(2.0f - r2), dy = wi.y / sqrtf(2.0f - r2);
-    float r = sqrtf(dx * dx + dy * dy);
-    float phi = atan2f(dy, dx);
-    if (phi < -kPi / 4) phi += kTwoPi;
-    float a, b;
-    if (phi < kPi / 4) { a = r; b = phi * a / (kPi / 4); }
-    else if (phi < 3 * kPi / 4) { b = r; a = -(phi - kPi / 2) * b / (kPi / 4); }
-    else if (phi < 5 * kPi / 4) { a = -r; b = (phi - kPi) * a / (kPi / 4); }
-    else { b = -r; a = -(phi - 3 * kPi / 2) * b / (kPi / 4); }
-    sx = a * 0.5f + 0.5f;
-    sy = b * 0.5f + 0.5f;
 }
 
 // build_map (lightslice.cc:2058-2147) + DiscretePdf2D::set_distribution
@@ -234,6 +219,8 @@ struct rzk_lightslice {
     // slices
     DBuf<int> nb, upper;
     DBuf<float> slice_wi, slice_w, slice_pdf, slice_cdf, maps, col_cdf, row_pdf, row_cdf, totals;
+    DBuf<float> pt_image;
+    DBuf<unsigned long long> pt_work;
     std::vector<int> h_centers, h_assign, h_reps;
     std::vector<float> h_center_w, h_rep_w, h_alphas, h_mags;
 };
@@ -502,6 +489,46 @@ int rzk_lightslice_gather(rzk_lightslice* ls, float* out_rgb) {
     });
 }
 
+int rzk_lightslice_render_guided(rzk_lightslice* ls, const float* camera, const rzk_pt_params* p,
+                                 float* out_rgb, int64_t* out_rays) {
+    if (!ls || !camera || !p) return RZK_E_INVALID;
+    rzk_ctx* ctx = ls->ctx;
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(ls->initialized, RZK_E_STATE, "rzk_lightslice_initialize has not run");
+        RZK_CHECK((int)camera[17] == ls->W && (int)camera[18] == ls->H, RZK_E_INVALID, "camera size differs");
+        RZK_CHECK(p->spp > 0 && p->max_bounce >= 0 && (p->guide_mode == 0 || p->guide_mode == 1), RZK_E_INVALID,
+                  "bad path tracing parameters");
+        PtParams pp;
+        pp.width = ls->W; pp.height = ls->H; pp.spp = p->spp; pp.max_bounce = p->max_bounce;
+        pp.direct = p->direct_lighting ? 1 : 0; pp.guide_mode = p->guide_mode; pp.tile = ls->T;
+        pp.res = ls->prm.map_resolution; pp.tiles_x = ls->tx; pp.tiles_y = ls->ty;
+        pp.x0 = p->w > 0 ? p->x0 : 0; pp.y0 = p->h > 0 ? p->y0 : 0;
+        pp.w = p->w > 0 ? p->w : ls->W; pp.h = p->h > 0 ? p->h : ls->H;
+        RZK_CHECK(pp.x0 >= 0 && pp.y0 >= 0 && pp.x0 + pp.w <= ls->W && pp.y0 + pp.h <= ls->H, RZK_E_INVALID,
+                  "window outside the image");
+        pp.seed = p->seed; pp.tmin = p->tmin;
+        GuideView gv;
+        gv.maps = ls->maps.p; gv.row_cdf = ls->row_cdf.p; gv.col_cdf = ls->col_cdf.p; gv.totals = ls->totals.p;
+        int64_t n = (int64_t)ls->W * ls->H;
+        if (ls->pt_image.n != (size_t)(3 * n)) ls->pt_image.alloc(3 * n);
+        if (ls->pt_work.n == 0) ls->pt_work.alloc(2);
+        if ((int64_t)pp.w * pp.h != n)   // pixels outside the window read back as 0
+            RZK_HIP(hipMemsetAsync(ls->pt_image.p, 0, sizeof(float) * 3 * n, ctx->stream));
+        if (out_rays) RZK_HIP(hipMemsetAsync(ls->pt_work.p, 0, 2 * sizeof(unsigned long long), ctx->stream));
+        launch_guided_pt(ctx, scene_view(ls->scene), camera, pp, gv, ls->pt_image.p, out_rays ? ls->pt_work.p : nullptr);
+        if (out_rgb) RZK_HIP(hipMemcpyAsync(out_rgb, ls->pt_image.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost,
+                                            ctx->stream));
+        if (out_rays) {
+            unsigned long long h[2];
+            RZK_HIP(hipMemcpyAsync(h, ls->pt_work.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+            ls_sync(ctx);
+            out_rays[0] = (int64_t)h[0];
+            out_rays[1] = (int64_t)h[1];
+        }
+        if (out_rgb) ls_sync(ctx);
+    });
+}
+
 int rzk_lightslice_count_work(rzk_lightslice* ls, int64_t* out) {
     if (!ls || !out) return RZK_E_INVALID;
     rzk_ctx* ctx = ls->ctx;
@@ -575,6 +602,8 @@ int rzk_lightslice_get(const rzk_lightslice* ls, const char* name, void* dst, in
         else if (nm == "slice_wi") { src = ls->slice_wi.p; need = 12 * (int64_t)S * nreps; }
         else if (nm == "maps") { src = ls->maps.p; need = 4 * (int64_t)S * res * res; }
         else if (nm == "map_totals") { src = ls->totals.p; need = 4 * (int64_t)S; }
+        else if (nm == "row_cdf") { src = ls->row_cdf.p; need = 4 * (int64_t)S * res; }
+        else if (nm == "col_cdf") { src = ls->col_cdf.p; need = 4 * (int64_t)S * res * res; }
         else if (nm == "neighbors") { src = ls->nb.p; need = 4 * (int64_t)S * ls->prm.num_neighbor_slices; }
         else if (nm == "image") { src = ls->image.p; need = 12 * n; }
         else if (nm == "recv") { src = ls->recv.p; need = 4 * (int64_t)RZK_RECV_FLOATS * n; }

@@ -1,0 +1,273 @@
+// pathtrace.hip — guided path tracing (VirtualPointLight::trace_wellington,
+// vpl.cc:934-1041, METRO_RANDOM branch) with next-event estimation
+// (FirstBounce::radiance, first_bounce.h:54-71 -> AreaLight::first_bounce,
+// area_light.cc:48-142) and the LightSlice direction sampler at the first
+// bounce (multi_slice_sample / sample_map / pdf_map, lightslice.cc:1569-2055,
+// 2313-2377).
+//
+// One lane per pixel walks its samples in order; the recursion of
+// trace_wellington is unrolled into a throughput product.  Every random
+// number is Philox(pixel, sample, bounce, slot), so a pixel's value does not
+// depend on which other pixels or requests share its launch.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+
+#include "pathtrace.h"
+#include "rzk_device.h"
+
+namespace rzk {
+
+// AreaLight::query (area_light.cc:19-25 with AreaLight::pdf, area_light.h:120-131)
+__device__ __forceinline__ v3 light_query(const LightGpu& L, v3 n, v3 wo) {
+    if (dot(wo, n) < kZeroEps) return mk(0.f, 0.f, 0.f);
+    return mk(L.le[0], L.le[1], L.le[2]);
+}
+
+__device__ __forceinline__ float balance(float a, float b) { return a / (a + b); }   // mis.h:11-13
+
+// FirstBounce::radiance with no direction sampler: one light chosen by
+// Scene::sample_light (DiscretePdf, strict <), then AreaLight::first_bounce:
+// light sampling + BSDF sampling combined with the balance heuristic.
+__device__ v3 next_event(const SceneView& sv, const float* rc, int m, const PathRng& rng, float tmin, int* stk,
+                         int pitch, AnyStack& astk, TravStats& st, unsigned long long& nclosest,
+                         unsigned long long& nshadow) {
+    float ul = u24(rng.u[1]);
+    int k = -1;
+    for (int i = 0; i < sv.nlights; ++i)
+        if (ul < sv.lights[i].sel_cdf) { k = i; break; }
+    if (k < 0 || sv.lights[k].count <= 0) return mk(0.f, 0.f, 0.f);
+    LightGpu L = sv.lights[k];
+    const float* mat = sv.mat + RZK_MAT_FLOATS * m;
+    v3 p = ld3(rc), pn = ld3(rc + 6), wo = ld3(rc + 9);
+    Onb b; b.u = ld3(rc + 12); b.v = ld3(rc + 15); b.n = ld3(rc + 18);
+    v3 Lo = mk(0.f, 0.f, 0.f);
+    // light sample: triangle by cumulative area ratio (quad.cc:75-87: t <= ratio),
+    // uniform point on it (triangle.cc:298-308)
+    float ut = u24(rng.u[2]);
+    int ti = L.first + L.count - 1;
+    for (int q = 0; q < L.count; ++q)
+        if (ut <= sv.ltris[L.first + q].cum) { ti = L.first + q; break; }
+    LightTri T = sv.ltris[ti];
+    float tmp = sqrtf(1.0f - u24(rng.u[3]));
+    float bb = 1.0f - tmp;
+    float cc = u24(rng.u[4]) * tmp;
+    float aa = 1.0f - bb - cc;
+    v3 q = (aa * mk(T.p0.x, T.p0.y, T.p0.z) + bb * mk(T.p1.x, T.p1.y, T.p1.z)) + cc * mk(T.p2.x, T.p2.y, T.p2.z);
+    v3 qn = mk(T.n.x, T.n.y, T.n.z);
+    float pdf = T.ratio * (1.0f / T.n.w);
+    if (pdf > 0.0f) {
+        nshadow++;
+        if (!segment_blocked(sv, p, q, tmin, astk, st)) {
+            v3 wi = unit(q - p);
+            v3 Le = light_query(L, qn, -wi);
+            float G = form_factor_abs(p, pn, q, qn);
+            if (G > 0.0f) {
+                v3 brdf = bsdf_eval(mat, b, wo, wi);
+                float dpdf = bsdf_pdf(mat, b, wo, wi);
+                v3 pq = q - p;
+                float w1 = balance(pdf * dot(pq, pq) / fabsf(dot(unit(p - q), qn)), dpdf);
+                Lo = Lo + (((w1 * G) / pdf) * Le) * brdf;
+            }
+        }
+    }
+    // BSDF sample that may reach the same light
+    v3 wi;
+    float dpdf;
+    bsdf_sample(mat, b, wo, &rng.u[5], wi, dpdf);
+    if (dpdf > 0.0f) {
+        Ray r2 = ray_ctor(p, wi);
+        float t = 0.f, tb = 0.f, tg = 0.f;
+        nclosest++;
+        int tri = traverse<false>(sv, r2, tmin, 1e6f, stk, pitch, t, tb, tg, st);
+        if (tri >= 0) {
+            float4 nf = sv.tri_n[tri];
+            int4 sf = sv.surf[__float_as_int(nf.w)];
+            if (sf.y == k) {
+                v3 qh = p + t * wi;
+                v3 n = mk(nf.x, nf.y, nf.z);
+                v3 qhn = unit((n + (n - n) * tb) + (n - n) * tg);
+                if (sf.z & RZK_SURF_FLIPPED) qhn = -qhn;
+                v3 Le = light_query(L, qhn, -wi);
+                float pa = 0.f;                           // Quad::pdf at a point of the hit triangle
+                for (int qq = 0; qq < L.count; ++qq) {
+                    const LightTri& lt = sv.ltris[L.first + qq];
+                    if (lt.tri == tri) { pa = lt.ratio * (1.0f / lt.n.w); break; }
+                }
+                v3 pq = qh - p;
+                float lpdf = pa * dot(pq, pq) / fabsf(dot(unit(p - qh), qhn));
+                float w2 = balance(dpdf, lpdf);
+                v3 brdf = bsdf_eval(mat, b, wo, wi);
+                float cosv = fabsf(dot(wi, pn));
+                Lo = Lo + (((w2 * cosv) / dpdf) * Le) * brdf;
+            }
+        }
+    }
+    return mk(Lo.x / L.sel_pdf, Lo.y / L.sel_pdf, Lo.z / L.sel_pdf);
+}
+
+// pdf_map (lightslice.cc:2352-2377) for a local direction of the receiver frame
+__device__ __forceinline__ float map_pdf(const GuideView& gv, int slice, int res, v3 local) {
+    if (local.z <= 0.0f) return 0.0f;
+    float sx, sy;
+    direction_to_square(local, sx, sy);
+    int col = (int)(sx * res), row = (int)(sy * res);
+    if (col >= res) col = res - 1;
+    if (row >= res) row = res - 1;
+    float pdf_k = gv.maps[((int64_t)slice * res + row) * res + col] / gv.totals[slice];
+    float pdf_unit_square = (float)(res * res) * pdf_k;
+    float pdf_unit_disk = pdf_unit_square * kInvPi;
+    return pdf_unit_disk * 0.5f;
+}
+
+// multi_slice_pdf (lightslice.cc:1930-2055) with the receiver-frame map of
+// `slice` when it is valid, else the uniform-hemisphere fallback
+__device__ __forceinline__ float guide_pdf(const GuideView& gv, int slice, int res, const Onb& b, v3 sn, v3 wi) {
+    if (dot(sn, wi) <= 0.0f) return 0.0f;
+    if (slice < 0) return kInv2Pi;
+    return map_pdf(gv, slice, res, world_to_local(b, wi));
+}
+
+// LightSlice::sample -> multi_slice_sample: sample_map of the slice (receiver
+// frame, same_local_wi) or the uniform-hemisphere fallback; pdf re-evaluated
+// by multi_slice_pdf; returns the receiver BRDF for wi
+__device__ v3 guide_sample(const GuideView& gv, int slice, int res, const float* mat, const Onb& b, v3 sn, v3 wo,
+                           const uint32_t* u, v3& wi, float& pdf) {
+    if (slice < 0) {
+        float phi = kTwoPi * u24(u[0]);
+        float theta = acosf(u24(u[1]));
+        wi = unit(spherical_to_world(b, theta, phi));
+    } else {
+        // DiscretePdf2D::sample (pdf.cc:171-186): row by the row cdf, column by that row's cdf
+        float ur = u24(u[0]), uc = u24(u[1]);
+        const float* rcdf = gv.row_cdf + (int64_t)slice * res;
+        int row = -1, col = -1;
+        for (int i = 0; i < res; ++i)
+            if (ur < rcdf[i]) { row = i; break; }
+        if (row >= 0) {
+            const float* ccdf = gv.col_cdf + ((int64_t)slice * res + row) * res;
+            for (int j = 0; j < res; ++j)
+                if (uc < ccdf[j]) { col = j; break; }
+        }
+        if (row < 0 || col < 0) { wi = mk(0.f, 0.f, 0.f); pdf = 0.f; return mk(0.f, 0.f, 0.f); }
+        float sx = ((float)col + u24(u[2])) / (float)res;
+        float sy = ((float)row + u24(u[3])) / (float)res;
+        wi = local_to_world(b, square_to_direction(sx, sy));
+    }
+    pdf = guide_pdf(gv, slice, res, b, sn, wi);
+    return bsdf_eval(mat, b, wo, wi);
+}
+
+// Camera::shoot_ray(const SensorSample&), camera.cc:67-80
+__device__ __forceinline__ Ray camera_sample_ray(const float* c, float x, float y) {
+    Onb b;
+    b.u = mk(c[3], c[4], c[5]); b.v = mk(c[6], c[7], c[8]); b.n = mk(c[9], c[10], c[11]);
+    float height = c[18];
+    float fx = (x - c[12]) / c[14];
+    float fy = ((height - y) - c[13]) / c[15];
+    v3 d = unit(local_to_world(b, mk(fx, fy, -c[16])));
+    return ray_ctor(mk(c[0], c[1], c[2]), d);
+}
+
+__global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, PtParams pp, GuideView gv,
+                                                   float* __restrict__ out, unsigned long long* __restrict__ work) {
+    extern __shared__ int s_stack[];
+    int pitch = blockDim.x;
+    int* stk = s_stack + threadIdx.x;
+    AnyStack astk = any_stack(sv, s_stack);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    TravStats st;
+    unsigned long long nclosest = 0, nshadow = 0;
+    int64_t npx = (int64_t)pp.w * pp.h;
+    if (t < npx) {
+        int x = pp.x0 + (int)(t % pp.w), y = pp.y0 + (int)(t / pp.w);
+        uint32_t pixel = (uint32_t)(y * pp.width + x);
+        int slice = -1;
+        if (pp.guide_mode == 1) {
+            int sx = min(x / pp.tile, pp.tiles_x - 1), sy = min(y / pp.tile, pp.tiles_y - 1);
+            int s = sy * pp.tiles_x + sx;
+            if (gv.totals[s] > 0.0f) slice = s;
+        }
+        int strata = 1;
+        while ((strata + 1) * (strata + 1) <= pp.spp) ++strata;
+        bool stratified = strata * strata == pp.spp;
+        v3 acc = mk(0.f, 0.f, 0.f);
+        for (int smp = 0; smp < pp.spp; ++smp) {
+            PathRng r0 = path_rng(pp.seed, pixel, (uint32_t)smp, 0u);
+            float jx = u24(r0.u[0]), jy = u24(r0.u[1]);
+            if (stratified) {
+                jx = ((float)(smp % strata) + jx) / (float)strata;
+                jy = ((float)(smp / strata) + jy) / (float)strata;
+            }
+            Ray ray = camera_sample_ray(cam.v, (float)x + jx, (float)y + jy);
+            v3 L = mk(0.f, 0.f, 0.f), T = mk(1.f, 1.f, 1.f);
+            float th = 0.f, tb = 0.f, tg = 0.f;
+            nclosest++;
+            int tri = traverse<false>(sv, ray, pp.tmin, 1e6f, stk, pitch, th, tb, tg, st);
+            for (int bounce = 1; tri >= 0 && bounce <= pp.max_bounce; ++bounce) {
+                float rc[RZK_RECV_FLOATS];
+                int m, lidx;
+                fill_receiver(sv, ray, th, tb, tg, tri, rc, m, lidx);
+                if (lidx >= 0) {
+                    if (bounce == 1 && pp.direct) L = L + T * light_query(sv.lights[lidx], ld3(rc + 6), ld3(rc + 9));
+                    break;
+                }
+                if (m < 0) break;
+                PathRng rng = path_rng(pp.seed, pixel, (uint32_t)smp, (uint32_t)bounce);
+                bool brdf_sampling = u24(rng.u[0]) < 0.5f;
+                if (bounce > 1 || pp.direct)
+                    L = L + T * next_event(sv, rc, m, rng, pp.tmin, stk, pitch, astk, st, nclosest, nshadow);
+                if (bounce >= pp.max_bounce) break;
+                const float* mat = sv.mat + RZK_MAT_FLOATS * m;
+                Onb b; b.u = ld3(rc + 12); b.v = ld3(rc + 15); b.n = ld3(rc + 18);
+                v3 sn = ld3(rc + 6), wo = ld3(rc + 9), p = ld3(rc);
+                int gslice = bounce == 1 ? slice : -1;
+                v3 wi, brdf;
+                float pdf;
+                if (brdf_sampling) brdf = bsdf_sample(mat, b, wo, &rng.u[8], wi, pdf);
+                else brdf = guide_sample(gv, gslice, pp.res, mat, b, sn, wo, &rng.u[8], wi, pdf);
+                if (!(pdf > 0.0f)) break;
+                ray = ray_ctor(p, wi);
+                nclosest++;
+                tri = traverse<false>(sv, ray, pp.tmin, 1e6f, stk, pitch, th, tb, tg, st);
+                if (tri < 0) break;
+                int4 sf = sv.surf[__float_as_int(sv.tri_n[tri].w)];
+                if (sf.y >= 0) break;                               // light: counted by the light sample
+                float pdf_other = brdf_sampling ? guide_pdf(gv, gslice, pp.res, b, sn, wi) : bsdf_pdf(mat, b, wo, wi);
+                float weight = pdf / (pdf + pdf_other);
+                float cosv = fabsf(dot(wi, sn));
+                float s = ((weight * cosv) / pdf) * 2.0f;
+                T = (T * brdf) * s;
+            }
+            acc = acc + L;
+        }
+        int64_t o = (int64_t)y * pp.width + x;
+        out[3 * o] = acc.x / (float)pp.spp;
+        out[3 * o + 1] = acc.y / (float)pp.spp;
+        out[3 * o + 2] = acc.z / (float)pp.spp;
+    }
+    if (work) {
+        unsigned long long a = nclosest, c = nshadow;
+        for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off); c += __shfl_xor(c, off); }
+        if ((threadIdx.x & 63) == 0) { atomicAdd(work, a); atomicAdd(work + 1, c); }
+    }
+}
+
+}  // namespace rzk
+
+using namespace rzk;
+
+void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv_in, const float* cam, const PtParams& pp, const GuideView& gv,
+                      float* d_out, unsigned long long* d_work) {
+    int64_t n = (int64_t)pp.w * pp.h;
+    if (n <= 0) return;
+    SceneView sv = sv_in;
+    const int block = 128;
+    unsigned grid = (unsigned)((n + block - 1) / block);
+    bind_spill(ctx, sv, (int64_t)grid * block);
+    CamView cv;
+    for (int i = 0; i < RZK_CAM_FLOATS; ++i) cv.v[i] = cam[i];
+    size_t lds = sizeof(int) * block * (size_t)std::max(sv.depth, kLdsStack);
+    hipLaunchKernelGGL(k_guided_pt, dim3(grid), dim3(block), lds, ctx->stream, sv, cv, pp, gv, d_out, d_work);
+    RZK_HIP(hipGetLastError());
+}

@@ -24,6 +24,12 @@ struct TriGpu { float4 p0, e1, e2; };
 // refs as above, kEmptyRef for unused slots (always the trailing ones).
 struct Bvh4NodeGpu { float4 lox, loy, loz, hix, hiy, hiz; int4 ref; int4 pad; };
 constexpr int kEmptyRef = (int)0x80000000;
+
+// Emitting triangle for light sampling: vertices, normal (n.w = area) and the
+// cumulative / own area ratio within its light (Quad::sample, quad.cc:75-87)
+struct LightTri { float4 p0, p1, p2, n; float cum, ratio; int light, tri; };   // tri: original triangle id
+// Area light: radiance, triangle range, selection probability (Scene::sample_light)
+struct LightGpu { float le[3]; int first, count; float sel_cdf, sel_pdf, pad; };
 constexpr int kLdsStack = 16;   // stack entries per lane kept in LDS; deeper ones spill to HBM
 
 struct BvhHost {
@@ -71,6 +77,11 @@ struct rzk_scene {
     float4* d_tri_tan = nullptr;      // per original triangle: (tangent.xyz, 0)
     int4* d_surf = nullptr;           // per surface: (material, light, flags, 0)
     float* d_mat = nullptr;           // 12 floats per material
+    LightTri* d_ltris = nullptr;
+    LightGpu* d_lights = nullptr;
+    int nltri = 0;
+    std::vector<LightTri> h_ltris;
+    std::vector<LightGpu> h_lights;
     std::vector<float> h_mat;
     std::vector<int> h_surf_material, h_surf_light, h_surf_flags, h_tri_surface;
     std::vector<float> h_light_radiance;
@@ -100,6 +111,9 @@ struct SceneView {
     int root4_ref;
     int stack4;
     float rbmin[3], rbmax[3];
+    const LightTri* ltris;
+    const LightGpu* lights;
+    int nlights;
     int* spill;               // any-hit stack overflow, [entry - kLdsStack][spill_pitch]
     int64_t spill_pitch;      // threads of the launch using `spill`
 };

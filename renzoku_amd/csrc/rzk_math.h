@@ -217,4 +217,173 @@ RZK_HD uint32_t morton3d(float x, float y, float z) {
     return (spread10((uint32_t)x) << 2) | (spread10((uint32_t)y) << 1) | spread10((uint32_t)z);
 }
 
+// ---------------------------------------------------------------------------
+// Counter-based random numbers for the path tracer: Philox4x32-10 (Salmon,
+// Moraes, Dror, Shaw, SC'11).  Every (pixel, sample, bounce, slot) has its own
+// numbers, so a path is the same whichever batch, wave or order runs it.
+RZK_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Random::operator() (random.h:60-63): 24-bit float in [0, 1)
+RZK_HD float u24(uint32_t r) { return (float)(r & 0x00ffffffu) * (1.0f / 16777216.0f); }
+// Random::rand_exclude_01 (random.h:65-71): in (0, 1)
+RZK_HD float u24_open(uint32_t r) { return (float)(r & 0x00ffffffu) * (1.0f / 16777216.0f) + (1.0f / 33554432.0f); }
+
+// 12 numbers for one bounce of one path sample
+struct PathRng { uint32_t u[12]; };
+RZK_HD PathRng path_rng(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce) {
+    PathRng o;
+    for (uint32_t b = 0; b < 3; ++b) {
+        uint32_t c[4] = {sample, bounce, b, 0x52a7u};
+        philox4x32_10(c, seed ^ 0x9a3c5e17u, pixel);
+        o.u[4 * b] = c[0]; o.u[4 * b + 1] = c[1]; o.u[4 * b + 2] = c[2]; o.u[4 * b + 3] = c[3];
+    }
+    return o;
+}
+
+// Rgb::max (rgb.h:42-47)
+RZK_HD float rgb_max(v3 c) { float t = c.x; if (c.y > t) t = c.y; if (c.z > t) t = c.z; return t; }
+
+// rho_d / rho_s of Phong and Ward (phong.cc:6-16, ward.cc:6-22)
+RZK_HD void lobe_weights(const float* m, float& rho_d, float& rho_s) {
+    rho_d = rgb_max(mk(m[1], m[2], m[3]));
+    rho_s = rgb_max(mk(m[4], m[5], m[6]));
+    float total = rho_d + rho_s;
+    if (total > 0.0f) { rho_d /= total; rho_s /= total; }
+}
+
+// Onb::spherical_to_world (onb.cc:155-163)
+RZK_HD v3 spherical_to_world(const Onb& b, float theta, float phi) {
+    float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);
+    return local_to_world(b, mk(st * cp, st * sp, ct));
+}
+
+// Lambertian::pdf (lambertian.cc:70-75), ModifiedPhong::pdf (phong.cc:89-102),
+// Ward::pdf (ward.cc:118-134)
+RZK_HD float bsdf_pdf(const float* m, const Onb& b, v3 wo, v3 wi) {
+    if (dot(wi, b.n) <= kZeroEps || dot(wo, b.n) <= kZeroEps) return 0.0f;
+    int type = (int)m[0];
+    if (type == kLambert) return dot(b.n, wi) * kInvPi;
+    float rho_d, rho_s;
+    lobe_weights(m, rho_d, rho_s);
+    if (type == kPhong) {
+        float ne = m[7];
+        float pdf = rho_d * (dot(b.n, wi) * kInvPi);
+        v3 wr = b.n * (2.0f * dot(wi, b.n)) - wi;
+        float cos_alpha = dot(wo, wr);
+        if (cos_alpha > 0.0f) pdf += rho_s * (ne + 1.0f) * kInv2Pi * powf(cos_alpha, ne);
+        return pdf;
+    }
+    float ax = m[7], ay = m[8];
+    float ax2 = ax * ax, ay2 = ay * ay;
+    float inv4 = 1.0f / (4.0f * kPi * ax * ay);
+    float z = dot(b.n, wi);
+    v3 wh = unit(world_to_local(b, wi + wo));
+    float expo = expf(-(wh.x * wh.x / ax2 + wh.y * wh.y / ay2) / (wh.z * wh.z));
+    float ct = wh.z;
+    v3 lwo = unit(world_to_local(b, wo));
+    float ct3 = ct * ct * ct;
+    return rho_d * (z * kInvPi) + rho_s * (inv4 / (dot(wh, lwo) * ct3) * expo);
+}
+
+// cosine-weighted direction about b.n from two uniforms (lambertian.cc:13-24)
+RZK_HD v3 cosine_dir(const Onb& b, float u0, float u1) {
+    float phi = kTwoPi * u0;
+    float sin_theta = sqrtf(u1);
+    float x = sin_theta * cosf(phi);
+    float y = sin_theta * sinf(phi);
+    float z = sqrtf(1.0f - sin_theta * sin_theta);
+    return unit(local_to_world(b, mk(x, y, z)));
+}
+
+// BSDF sampling (lambertian.cc:5-27, phong.cc:21-72, ward.cc:24-101): u[0] picks the
+// lobe, u[1], u[2] the direction.  Returns the BRDF value; pdf = 0 on rejection.
+RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3& wi, float& pdf) {
+    wi = mk(0.f, 0.f, 0.f);
+    pdf = 0.0f;
+    if (dot(wo, b.n) <= kZeroEps) return mk(0.f, 0.f, 0.f);
+    int type = (int)m[0];
+    if (type == kLambert) {
+        float phi = kTwoPi * u24(u[1]);
+        float sin_theta = sqrtf(u24(u[2]));
+        float x = sin_theta * cosf(phi), y = sin_theta * sinf(phi);
+        float z = sqrtf(1.0f - sin_theta * sin_theta);
+        wi = unit(local_to_world(b, mk(x, y, z)));
+        pdf = z * kInvPi;
+        return mk(m[1], m[2], m[3]) * kInvPi;
+    }
+    float rho_d, rho_s;
+    lobe_weights(m, rho_d, rho_s);
+    if (u24(u[0]) < rho_d) {
+        wi = cosine_dir(b, u24(u[1]), u24(u[2]));
+    } else if (type == kPhong) {
+        float ne = m[7];
+        Onb o = onb_from_nu(wo, b.n);
+        float phi = kTwoPi * u24(u[1]);
+        float alpha = acosf(powf(u24(u[2]), 1.0f / (ne + 1.0f)));
+        v3 wr = spherical_to_world(o, alpha, phi);
+        float d = dot(wr, b.n);
+        if (d < 0.0f) return mk(0.f, 0.f, 0.f);
+        wi = b.n * (2.0f * d) - wr;
+    } else {
+        float ax = m[7], ay = m[8];
+        float ax2 = ax * ax, ay2 = ay * ay;
+        float ay_ax = ay / ax;
+        float s0 = u24_open(u[1]), s1 = u24_open(u[2]);
+        v3 lwo = unit(world_to_local(b, wo));
+        float angle = kTwoPi * s0;
+        float phi = atanf(ay_ax * tanf(angle));
+        if (sinf(phi) * sinf(angle) < 0 || cosf(phi) * cosf(angle) < 0) phi += kPi;
+        float cp = cosf(phi), sp = sinf(phi);
+        float theta = atanf(sqrtf(-logf(s1) / (cp * cp / ax2 + sp * sp / ay2)));
+        float st = sinf(theta), ct = cosf(theta);
+        v3 wh = mk(st * cp, st * sp, ct);
+        v3 lwi = wh * (2.0f * dot(lwo, wh)) - lwo;
+        wi = local_to_world(b, lwi);
+    }
+    pdf = bsdf_pdf(m, b, wo, wi);
+    return bsdf_eval(m, b, wo, wi);
+}
+
+// Shirley-Chiu concentric maps (concentric_mapping.h:16-114)
+RZK_HD void direction_to_square(v3 wi, float& sx, float& sy) {
+    float r2 = 1.0f - wi.z;
+    float dx = wi.x / sqrtf(2.0f - r2), dy = wi.y / sqrtf(2.0f - r2);
+    float r = sqrtf(dx * dx + dy * dy);
+    float phi = atan2f(dy, dx);
+    if (phi < -kPi / 4) phi += kTwoPi;
+    float a, bb;
+    if (phi < kPi / 4) { a = r; bb = phi * a / (kPi / 4); }
+    else if (phi < 3 * kPi / 4) { bb = r; a = -(phi - kPi / 2) * bb / (kPi / 4); }
+    else if (phi < 5 * kPi / 4) { a = -r; bb = (phi - kPi) * a / (kPi / 4); }
+    else { bb = -r; a = -(phi - 3 * kPi / 2) * bb / (kPi / 4); }
+    sx = a * 0.5f + 0.5f;
+    sy = bb * 0.5f + 0.5f;
+}
+
+RZK_HD v3 square_to_direction(float sx, float sy) {
+    float phi, r;
+    float a = 2.0f * sx - 1.0f, bb = 2.0f * sy - 1.0f;
+    if (a > -bb) {
+        if (a > bb) { r = a; phi = (kPi / 4) * (bb / a); }
+        else { r = bb; phi = (kPi / 4) * (2 - (a / bb)); }
+    } else {
+        if (a < bb) { r = -a; phi = (kPi / 4) * (4 + (bb / a)); }
+        else { r = -bb; if (bb != 0.0f) phi = (kPi / 4) * (6 - (a / bb)); else phi = 0.0f; }
+    }
+    float dx = r * cosf(phi), dy = r * sinf(phi);
+    float r2 = dx * dx + dy * dy;
+    return mk(dx * sqrtf(2.0f - r2), dy * sqrtf(2.0f - r2), 1.0f - r2);
+}
+
 }  // namespace rzk
