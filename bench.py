@@ -96,14 +96,75 @@ def cpu_baseline(sc, ls, vpl, vmat, reps, rep_w, params, target_s=8.0):
                       f"({pairs} shadow segments + shadings) on {cores} threads, {dt:.1f} s"}
 
 
+def run_guided(args):
+    """Guided path tracing of the 1M-triangle instanced scene (Ward BRDFs):
+    1280x720, 16 spp, depth 5, 100k VPLs clustered into 1,000 clusters with
+    16x16-pixel slices driving the first-bounce direction sampler.  A step is
+    one full-frame render_guided with the guide resident; the guide build
+    (LightSlice initialize) is timed once beside it."""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    W, H = 1280, 720
+    t0 = time.perf_counter()
+    sc = scenes.instanced_meshes(W, H, seed=2013, target_tris=args.tris or 1_000_000)
+    ctx = rz.Ctx(0)
+    scene = rz.Scene(ctx, sc)
+    log(f"[bench] scene: {sc.ntri} triangles, BVH depth {scene.info().depth} ({time.perf_counter() - t0:.1f} s)")
+    tracer, filler = vplgen.gpu_tracer(scene)
+    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 100_000, seed=2013, batch=65536)
+    vpls = rz.Vpls(ctx, vpl, vmat)
+    log(f"[bench] VPLs: {len(vmat)} from {n_emitted:.0f} light samples")
+    ls = rz.LightSlice(ctx, scene, vpls, width=W, height=H, tile=16, num_clusters=1000, num_neighbor_slices=16,
+                       map_resolution=32, n_emitted=n_emitted, seed=2013)
+    t1 = time.perf_counter()
+    ls.initialize(sc.camera)
+    ctx.synchronize()
+    init_ms = (time.perf_counter() - t1) * 1e3
+    inf = ls.info()
+    log(f"[bench] guide built: {inf.num_slices} slices, {inf.num_clusters} clusters, {init_ms:.0f} ms")
+    kw = dict(spp=16, max_bounce=5, direct_lighting=0, guide_mode=1, seed=2013, tmin=1e-3, to_host=False)
+    _, rays = ls.render_guided(sc.camera, count_rays=True, **kw)
+    ev = HipEvents()
+    e0, e1 = ev.create(), ev.create()
+    for i in range(args.warmup):
+        ls.render_guided(sc.camera, count_rays=False, **kw)
+    ctx.synchronize()
+    ms = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        ev.record(e0, ctx.stream)
+        ls.render_guided(sc.camera, count_rays=False, **kw)
+        ev.record(e1, ctx.stream)
+        ms.append(ev.elapsed_ms(e0, e1))
+        log(f"[bench] step {i + 1}/{args.steps}: {ms[-1]:.2f} ms")
+    ctx.synchronize()
+    ms_per_step = (time.perf_counter() - t_start) * 1e3 / args.steps
+    nrays = rays["closest_rays"] + rays["shadow_rays"]
+    out = {"metric": "guided path tracing Mrays/s (closest + shadow)", "value": round(nrays / (ms_per_step / 1e3) / 1e6, 3),
+           "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms_per_step, 3), "kernel_ms": round(float(np.mean(ms)), 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded instanced-mesh scene, seeded light-traced VPLs)",
+           "config": {"workload": "guided path tracing 1M tris 1280x720 16 spp depth 5 100k VPLs 16x16 slices "
+                                  "1000 clusters", "triangles": int(sc.ntri), "width": W, "height": H,
+                      "spp": 16, "max_bounce": 5, "vpls": int(len(vmat)), "slices": int(inf.num_slices),
+                      "clusters": int(inf.num_clusters), "parallelism": "single GPU"},
+           "rays": rays, "paths_per_s": round(W * H * 16 / (ms_per_step / 1e3), 1),
+           "guide_build_ms": round(init_ms, 1)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided"],
+                    help="lightslice: the 200k-triangle LightSlice gather step (the headline line); "
+                         "guided: guided path tracing of the 1M-triangle scene")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--tris", type=int, default=200_000)
-    ap.add_argument("--vpls", type=int, default=65_536)
+    ap.add_argument("--tris", type=int, default=0, help="triangle count (default: the workload's)")
+    ap.add_argument("--vpls", type=int, default=0, help="VPL count (default: the workload's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device once (a known-size transfer for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -112,6 +173,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design "
               f"described in DESIGN.md, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "guided":
+        return run_guided(args)
 
     import renzoku_amd as rz
     from renzoku_amd import scenes, vplgen
@@ -121,14 +184,14 @@ def main():
                   map_epsilon=1e-3, clamp_mode=rz._lib.CLAMP_FORM_FACTOR, clamp=0.01, incident_clamp=0.01,
                   tmin=1e-3, seed=2013, map_gate=0)
     t0 = time.perf_counter()
-    sc = scenes.room(W, H, seed=2013, target_tris=args.tris)
+    sc = scenes.room(W, H, seed=2013, target_tris=args.tris or 200_000)
     log(f"[bench] scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
     ctx = rz.Ctx(0)
     scene = rz.Scene(ctx, sc)
     inf = scene.info()
     log(f"[bench] BVH: {inf.num_interior_nodes} interior nodes, depth {inf.depth}")
     tracer, filler = vplgen.gpu_tracer(scene)
-    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls, seed=2013, batch=65536)
+    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 65_536, seed=2013, batch=65536)
     params["n_emitted"] = n_emitted
     vpls = rz.Vpls(ctx, vpl, vmat)
     log(f"[bench] VPLs: {len(vmat)} from {n_emitted:.0f} light samples")

@@ -131,7 +131,8 @@ int rzk_trace_segments_dev(rzk_ctx *ctx, const rzk_scene *scene, const float *d_
                            const float *d_q, int64_t n, float tmin, uint8_t *d_out,
                            uint64_t *d_stats);
 /* per-ray traversal counts {interior nodes visited, triangles tested}, host
- * buffers; kind 0 = closest, 1 = any, for rays; use rzk_trace_segment_stats for p,q */
+ * buffers; kind 0 = closest, 1 = any (counted on the binary tree in the
+ * reference's visit order, so they equal aggregate_bvh.cc's loop counts) */
 int rzk_trace_stats(rzk_ctx *ctx, const rzk_scene *scene, int kind, const float *rays,
                     int64_t n, float tmin, float tmax, int32_t *out_stats);
 

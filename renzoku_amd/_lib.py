@@ -389,15 +389,17 @@ class LightSlice:
         return inf
 
     def render_guided(self, cam, spp=16, max_bounce=5, direct_lighting=0, guide_mode=1, seed=2013, tmin=1e-3,
-                      window=None, to_host=True):
+                      window=None, to_host=True, count_rays=True):
         p = PtParams(spp, max_bounce, direct_lighting, guide_mode, seed, tmin, 0, 0, 0, 0)
         if window is not None:
             p.x0, p.y0, p.w, p.h = window
         cam = f32(cam)
         out = np.zeros((self.hgt, self.w, 3), np.float32) if to_host else None
-        rays = np.zeros(2, np.int64)
+        rays = np.zeros(2, np.int64) if count_rays else None
         self.ctx.check(self.lib.rzk_lightslice_render_guided(self.h, ptr(cam), C.byref(p), ptr(out), ptr(rays)),
                        "rzk_lightslice_render_guided")
+        if rays is None:
+            return out, None
         return out, {"closest_rays": int(rays[0]), "shadow_rays": int(rays[1])}
 
     def count_work(self):
