@@ -263,6 +263,58 @@ typedef struct rzk_pt_params {
 int rzk_lightslice_render_guided(rzk_lightslice *ls, const float *camera, const rzk_pt_params *params,
                                  float *out_rgb, int64_t *out_rays);
 
+/* ---- guides and batched rendering -------------------------------------
+ * A guide is the per-slice guiding data of one camera (the direction sampler
+ * LightSlice implements for DirectionSampler, local_geometry.h:73-79).  A
+ * batch renders many jobs (windows of guided path tracing, each with its own
+ * guide, spp, depth and seed) with one persistent kernel fed from a work
+ * queue, launched as one captured HIP graph.  Every job's image is bit-equal
+ * to rzk_lightslice_render_guided of the same window: samples are summed per
+ * pixel in sample order whatever the batch holds. */
+typedef struct rzk_guide rzk_guide;
+typedef struct rzk_batch rzk_batch;
+
+/* borrow the guiding data of an initialised LightSlice for `camera` (the
+ * camera it was initialised with); valid until the LightSlice is destroyed or
+ * initialised again */
+int rzk_lightslice_guide(rzk_lightslice *ls, const float *camera, rzk_guide **out);
+int rzk_guide_destroy(rzk_guide *guide);
+
+typedef struct rzk_render_job {
+    const rzk_guide *guide;         /* camera, image size and slices of the job */
+    int32_t x0, y0, w, h;           /* window of the guide's image */
+    int32_t spp, max_bounce, direct_lighting, guide_mode;
+    uint32_t seed;
+    float tmin;
+} rzk_render_job;
+
+typedef struct rzk_batch_config {
+    int32_t block;                  /* workgroup size: 64, 128, 192 or 256 lanes */
+    int32_t blocks_per_cu;          /* persistent grid = compute units x blocks_per_cu */
+    int32_t items_per_dequeue;      /* queue depth: work items a wave takes per atomic */
+    int32_t samples_per_item;       /* samples per work item (a work item is 64 pixels) */
+} rzk_batch_config;
+
+/* cfg NULL: defaults {128, 4, 1, 4} */
+int rzk_batch_create(rzk_ctx *ctx, const rzk_scene *scene, const rzk_batch_config *cfg, rzk_batch **out);
+int rzk_batch_destroy(rzk_batch *batch);
+int rzk_batch_set_config(rzk_batch *batch, const rzk_batch_config *cfg);
+int rzk_batch_get_config(const rzk_batch *batch, rzk_batch_config *out);
+/* render n jobs in one graph launch.  out_rgb: host, the jobs' [h][w][3]
+ * images concatenated in job order, or NULL to keep them on the device;
+ * out_rays (optional, 2 x int64): closest-hit and shadow rays traced;
+ * out_ms (optional): device time of the launch.  Synchronous. */
+int rzk_batch_render(rzk_batch *batch, const rzk_render_job *jobs, int32_t n, float *out_rgb,
+                     int64_t *out_rays, float *out_ms);
+/* time candidate configurations on `jobs` (coordinate descent over
+ * samples_per_item, block, blocks_per_cu, items_per_dequeue), keep the
+ * fastest; tried (optional, max_tried entries) receives each candidate with
+ * its time in ms in rzk_batch_trial records; returns the number tried in
+ * *n_tried. */
+typedef struct rzk_batch_trial { rzk_batch_config cfg; float ms; } rzk_batch_trial;
+int rzk_batch_autotune(rzk_batch *batch, const rzk_render_job *jobs, int32_t n, rzk_batch_config *best,
+                       rzk_batch_trial *tried, int32_t max_tried, int32_t *n_tried);
+
 #ifdef __cplusplus
 }
 #endif

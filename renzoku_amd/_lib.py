@@ -33,6 +33,9 @@ EXPORTS = [
     "rzk_lightslice_create", "rzk_lightslice_destroy", "rzk_lightslice_initialize",
     "rzk_lightslice_gather", "rzk_lightslice_get_info", "rzk_lightslice_count_work", "rzk_lightslice_get",
     "rzk_lightslice_render_guided",
+    "rzk_lightslice_guide", "rzk_guide_destroy",
+    "rzk_batch_create", "rzk_batch_destroy", "rzk_batch_set_config", "rzk_batch_get_config", "rzk_batch_render",
+    "rzk_batch_autotune",
 ]
 
 
@@ -71,6 +74,26 @@ class PtParams(C.Structure):
         ("seed", C.c_uint32), ("tmin", C.c_float), ("x0", C.c_int32), ("y0", C.c_int32), ("w", C.c_int32),
         ("h", C.c_int32),
     ]
+
+
+class RenderJob(C.Structure):
+    _fields_ = [
+        ("guide", C.c_void_p), ("x0", C.c_int32), ("y0", C.c_int32), ("w", C.c_int32), ("h", C.c_int32),
+        ("spp", C.c_int32), ("max_bounce", C.c_int32), ("direct_lighting", C.c_int32), ("guide_mode", C.c_int32),
+        ("seed", C.c_uint32), ("tmin", C.c_float),
+    ]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("items_per_dequeue", C.c_int32),
+                ("samples_per_item", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchTrial(C.Structure):
+    _fields_ = [("cfg", BatchConfig), ("ms", C.c_float)]
 
 
 class LightSliceInfo(C.Structure):
@@ -130,6 +153,15 @@ def load() -> C.CDLL:
         "rzk_lightslice_count_work": [P, P],
         "rzk_lightslice_render_guided": [P, P, C.POINTER(PtParams), P, P],
         "rzk_lightslice_get": [P, C.c_char_p, P, I64],
+        "rzk_lightslice_guide": [P, P, C.POINTER(P)],
+        "rzk_guide_destroy": [P],
+        "rzk_batch_create": [P, P, C.POINTER(BatchConfig), C.POINTER(P)],
+        "rzk_batch_destroy": [P],
+        "rzk_batch_set_config": [P, C.POINTER(BatchConfig)],
+        "rzk_batch_get_config": [P, C.POINTER(BatchConfig)],
+        "rzk_batch_render": [P, C.POINTER(RenderJob), I32, P, P, C.POINTER(C.c_float)],
+        "rzk_batch_autotune": [P, C.POINTER(RenderJob), I32, C.POINTER(BatchConfig), C.POINTER(BatchTrial), I32,
+                               C.POINTER(I32)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -402,6 +434,10 @@ class LightSlice:
             return out, None
         return out, {"closest_rays": int(rays[0]), "shadow_rays": int(rays[1])}
 
+    def guide(self, cam):
+        """borrow this LightSlice's guiding data (valid until it is initialised again)"""
+        return Guide.from_lightslice(self, cam)
+
     def count_work(self):
         out = np.zeros(8, np.int64)
         self.ctx.check(self.lib.rzk_lightslice_count_work(self.h, ptr(out)), "rzk_lightslice_count_work")
@@ -418,6 +454,107 @@ class LightSlice:
     def close(self):
         if getattr(self, "h", None):
             self.lib.rzk_lightslice_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Guide:
+    """rzk_guide: the per-slice guiding data of one camera"""
+
+    def __init__(self, ctx, h, width, height, keep=()):
+        self.ctx, self.lib, self.h = ctx, ctx.lib, h
+        self.width, self.height = width, height
+        self._keep = keep
+
+    @classmethod
+    def from_lightslice(cls, ls, cam):
+        cam = f32(cam)
+        h = C.c_void_p()
+        ls.ctx.check(ls.lib.rzk_lightslice_guide(ls.h, ptr(cam), C.byref(h)), "rzk_lightslice_guide")
+        return cls(ls.ctx, h, int(cam[17]), int(cam[18]), keep=(ls,))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rzk_guide_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_jobs(specs):
+    """specs: dicts with guide, window (x0, y0, w, h), spp, max_bounce, direct_lighting, guide_mode, seed, tmin"""
+    arr = (RenderJob * len(specs))()
+    for i, s in enumerate(specs):
+        x0, y0, w, h = s.get("window") or (0, 0, s["guide"].width, s["guide"].height)
+        arr[i] = RenderJob(s["guide"].h, x0, y0, w, h, s.get("spp", 16), s.get("max_bounce", 5),
+                           s.get("direct_lighting", 0), s.get("guide_mode", 1), s.get("seed", 2013),
+                           s.get("tmin", 1e-3))
+    return arr
+
+
+class Batch:
+    """rzk_batch: many guided path tracing jobs per persistent, graph-launched kernel"""
+
+    def __init__(self, ctx, scene, config=None):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = C.c_void_p()
+        cfg = BatchConfig(**config) if config else None
+        ctx.check(self.lib.rzk_batch_create(ctx.h, scene.h, C.byref(cfg) if cfg else None, C.byref(h)),
+                  "rzk_batch_create")
+        self.h = h
+        self.scene = scene
+
+    @property
+    def config(self):
+        c = BatchConfig()
+        self.ctx.check(self.lib.rzk_batch_get_config(self.h, C.byref(c)), "rzk_batch_get_config")
+        return c.as_dict()
+
+    @config.setter
+    def config(self, d):
+        c = BatchConfig(**d)
+        self.ctx.check(self.lib.rzk_batch_set_config(self.h, C.byref(c)), "rzk_batch_set_config")
+
+    def render(self, specs, to_host=True, count_rays=False):
+        """returns (list of [h][w][3] images or None, rays dict or None, device ms)"""
+        jobs = make_jobs(specs)
+        npx = sum(j.w * j.h for j in jobs)
+        out = np.zeros(npx * 3, np.float32) if to_host else None
+        rays = np.zeros(2, np.int64) if count_rays else None
+        ms = C.c_float()
+        self.ctx.check(self.lib.rzk_batch_render(self.h, jobs, len(jobs), ptr(out), ptr(rays), C.byref(ms)),
+                       "rzk_batch_render")
+        imgs = None
+        if to_host:
+            imgs, o = [], 0
+            for j in jobs:
+                n = j.w * j.h * 3
+                imgs.append(out[o:o + n].reshape(j.h, j.w, 3))
+                o += n
+        rd = {"closest_rays": int(rays[0]), "shadow_rays": int(rays[1])} if count_rays else None
+        return imgs, rd, ms.value
+
+    def autotune(self, specs, max_tried=64):
+        jobs = make_jobs(specs)
+        best = BatchConfig()
+        tried = (BatchTrial * max_tried)()
+        n = C.c_int32()
+        self.ctx.check(self.lib.rzk_batch_autotune(self.h, jobs, len(jobs), C.byref(best), tried, max_tried,
+                                                   C.byref(n)), "rzk_batch_autotune")
+        return best.as_dict(), [(tried[i].cfg.as_dict(), tried[i].ms) for i in range(min(n.value, max_tried))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rzk_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -529,6 +529,33 @@ int rzk_lightslice_render_guided(rzk_lightslice* ls, const float* camera, const 
     });
 }
 
+int rzk_lightslice_guide(rzk_lightslice* ls, const float* camera, rzk_guide** out) {
+    if (!ls || !camera || !out) return RZK_E_INVALID;
+    *out = nullptr;
+    rzk_ctx* ctx = ls->ctx;
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(ls->initialized, RZK_E_STATE, "rzk_lightslice_initialize has not run");
+        RZK_CHECK((int)camera[17] == ls->W && (int)camera[18] == ls->H, RZK_E_INVALID, "camera size differs");
+        std::unique_ptr<rzk_guide> g(new rzk_guide());
+        g->ctx = ctx;
+        g->scene = ls->scene;
+        std::memcpy(g->cam, camera, sizeof(g->cam));
+        g->W = ls->W; g->H = ls->H; g->tile = ls->T; g->tiles_x = ls->tx; g->tiles_y = ls->ty;
+        g->res = ls->prm.map_resolution;
+        g->gv.maps = ls->maps.p; g->gv.row_cdf = ls->row_cdf.p; g->gv.col_cdf = ls->col_cdf.p;
+        g->gv.totals = ls->totals.p;
+        *out = g.release();
+    });
+}
+
+int rzk_guide_destroy(rzk_guide* g) {
+    if (!g) return RZK_OK;
+    (void)hipStreamSynchronize(g->ctx->stream);
+    for (void* p : g->owned) (void)hipFree(p);
+    delete g;
+    return RZK_OK;
+}
+
 int rzk_lightslice_count_work(rzk_lightslice* ls, int64_t* out) {
     if (!ls || !out) return RZK_E_INVALID;
     rzk_ctx* ctx = ls->ctx;

@@ -169,6 +169,85 @@ __device__ __forceinline__ Ray camera_sample_ray(const float* c, float x, float 
     return ray_ctor(mk(c[0], c[1], c[2]), d);
 }
 
+// Slice of a pixel: its tile when guide_mode 1 and the tile's map is non-empty,
+// else -1 (the uniform-hemisphere fallback)
+__device__ __forceinline__ int pixel_slice(const PtParams& pp, const GuideView& gv, int x, int y) {
+    if (pp.guide_mode != 1) return -1;
+    int sx = min(x / pp.tile, pp.tiles_x - 1), sy = min(y / pp.tile, pp.tiles_y - 1);
+    int s = sy * pp.tiles_x + sx;
+    return gv.totals[s] > 0.0f ? s : -1;
+}
+
+// samples per axis when spp is a square (Sensor's stratified jitter, sensor.cc:25-41), else 0
+__device__ __forceinline__ int spp_strata(int spp) {
+    int strata = 1;
+    while ((strata + 1) * (strata + 1) <= spp) ++strata;
+    return strata * strata == spp ? strata : 0;
+}
+
+// One path sample of pixel (x, y): trace_wellington unrolled into a
+// throughput product (vpl.cc:934-1041).  Every random number comes from
+// Philox(seed, pixel, sample, bounce), so the value depends on nothing else.
+__device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& pp, const GuideView& gv, int x,
+                         int y, int slice, int strata, int smp, int* stk, int pitch, AnyStack& astk, TravStats& st,
+                         unsigned long long& nclosest, unsigned long long& nshadow) {
+    uint32_t pixel = (uint32_t)(y * pp.width + x);
+    PathRng r0 = path_rng(pp.seed, pixel, (uint32_t)smp, 0u);
+    float jx = u24(r0.u[0]), jy = u24(r0.u[1]);
+    if (strata > 0) {
+        jx = ((float)(smp % strata) + jx) / (float)strata;
+        jy = ((float)(smp / strata) + jy) / (float)strata;
+    }
+    Ray ray = camera_sample_ray(cam, (float)x + jx, (float)y + jy);
+    v3 L = mk(0.f, 0.f, 0.f), T = mk(1.f, 1.f, 1.f);
+    float th = 0.f, tb = 0.f, tg = 0.f;
+    nclosest++;
+    int tri = traverse<false>(sv, ray, pp.tmin, 1e6f, stk, pitch, th, tb, tg, st);
+    for (int bounce = 1; tri >= 0 && bounce <= pp.max_bounce; ++bounce) {
+        float rc[RZK_RECV_FLOATS];
+        int m, lidx;
+        fill_receiver(sv, ray, th, tb, tg, tri, rc, m, lidx);
+        if (lidx >= 0) {
+            if (bounce == 1 && pp.direct) L = L + T * light_query(sv.lights[lidx], ld3(rc + 6), ld3(rc + 9));
+            break;
+        }
+        if (m < 0) break;
+        PathRng rng = path_rng(pp.seed, pixel, (uint32_t)smp, (uint32_t)bounce);
+        bool brdf_sampling = u24(rng.u[0]) < 0.5f;
+        if (bounce > 1 || pp.direct)
+            L = L + T * next_event(sv, rc, m, rng, pp.tmin, stk, pitch, astk, st, nclosest, nshadow);
+        if (bounce >= pp.max_bounce) break;
+        const float* mat = sv.mat + RZK_MAT_FLOATS * m;
+        Onb b; b.u = ld3(rc + 12); b.v = ld3(rc + 15); b.n = ld3(rc + 18);
+        v3 sn = ld3(rc + 6), wo = ld3(rc + 9), p = ld3(rc);
+        int gslice = bounce == 1 ? slice : -1;
+        v3 wi, brdf;
+        float pdf;
+        if (brdf_sampling) brdf = bsdf_sample(mat, b, wo, &rng.u[8], wi, pdf);
+        else brdf = guide_sample(gv, gslice, pp.res, mat, b, sn, wo, &rng.u[8], wi, pdf);
+        if (!(pdf > 0.0f)) break;
+        ray = ray_ctor(p, wi);
+        nclosest++;
+        tri = traverse<false>(sv, ray, pp.tmin, 1e6f, stk, pitch, th, tb, tg, st);
+        if (tri < 0) break;
+        int4 sf = sv.surf[__float_as_int(sv.tri_n[tri].w)];
+        if (sf.y >= 0) break;                               // light: counted by the light sample
+        float pdf_other = brdf_sampling ? guide_pdf(gv, gslice, pp.res, b, sn, wi) : bsdf_pdf(mat, b, wo, wi);
+        float weight = pdf / (pdf + pdf_other);
+        float cosv = fabsf(dot(wi, sn));
+        float s = ((weight * cosv) / pdf) * 2.0f;
+        T = (T * brdf) * s;
+    }
+    return L;
+}
+
+__device__ __forceinline__ void add_ray_counts(unsigned long long* work, unsigned long long a, unsigned long long c) {
+    for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off); c += __shfl_xor(c, off); }
+    if ((threadIdx.x & 63) == 0) { atomicAdd(work, a); atomicAdd(work + 1, c); }
+}
+
+// One lane per pixel of the window, all its samples in order, summed in
+// sample order (MonteCarloIntegrator::integrate, monte_carlo_integrator.cc:45-70)
 __global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, PtParams pp, GuideView gv,
                                                    float* __restrict__ out, unsigned long long* __restrict__ work) {
     extern __shared__ int s_stack[];
@@ -181,81 +260,125 @@ __global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, Pt
     int64_t npx = (int64_t)pp.w * pp.h;
     if (t < npx) {
         int x = pp.x0 + (int)(t % pp.w), y = pp.y0 + (int)(t / pp.w);
-        uint32_t pixel = (uint32_t)(y * pp.width + x);
-        int slice = -1;
-        if (pp.guide_mode == 1) {
-            int sx = min(x / pp.tile, pp.tiles_x - 1), sy = min(y / pp.tile, pp.tiles_y - 1);
-            int s = sy * pp.tiles_x + sx;
-            if (gv.totals[s] > 0.0f) slice = s;
-        }
-        int strata = 1;
-        while ((strata + 1) * (strata + 1) <= pp.spp) ++strata;
-        bool stratified = strata * strata == pp.spp;
+        int slice = pixel_slice(pp, gv, x, y);
+        int strata = spp_strata(pp.spp);
         v3 acc = mk(0.f, 0.f, 0.f);
-        for (int smp = 0; smp < pp.spp; ++smp) {
-            PathRng r0 = path_rng(pp.seed, pixel, (uint32_t)smp, 0u);
-            float jx = u24(r0.u[0]), jy = u24(r0.u[1]);
-            if (stratified) {
-                jx = ((float)(smp % strata) + jx) / (float)strata;
-                jy = ((float)(smp / strata) + jy) / (float)strata;
-            }
-            Ray ray = camera_sample_ray(cam.v, (float)x + jx, (float)y + jy);
-            v3 L = mk(0.f, 0.f, 0.f), T = mk(1.f, 1.f, 1.f);
-            float th = 0.f, tb = 0.f, tg = 0.f;
-            nclosest++;
-            int tri = traverse<false>(sv, ray, pp.tmin, 1e6f, stk, pitch, th, tb, tg, st);
-            for (int bounce = 1; tri >= 0 && bounce <= pp.max_bounce; ++bounce) {
-                float rc[RZK_RECV_FLOATS];
-                int m, lidx;
-                fill_receiver(sv, ray, th, tb, tg, tri, rc, m, lidx);
-                if (lidx >= 0) {
-                    if (bounce == 1 && pp.direct) L = L + T * light_query(sv.lights[lidx], ld3(rc + 6), ld3(rc + 9));
-                    break;
-                }
-                if (m < 0) break;
-                PathRng rng = path_rng(pp.seed, pixel, (uint32_t)smp, (uint32_t)bounce);
-                bool brdf_sampling = u24(rng.u[0]) < 0.5f;
-                if (bounce > 1 || pp.direct)
-                    L = L + T * next_event(sv, rc, m, rng, pp.tmin, stk, pitch, astk, st, nclosest, nshadow);
-                if (bounce >= pp.max_bounce) break;
-                const float* mat = sv.mat + RZK_MAT_FLOATS * m;
-                Onb b; b.u = ld3(rc + 12); b.v = ld3(rc + 15); b.n = ld3(rc + 18);
-                v3 sn = ld3(rc + 6), wo = ld3(rc + 9), p = ld3(rc);
-                int gslice = bounce == 1 ? slice : -1;
-                v3 wi, brdf;
-                float pdf;
-                if (brdf_sampling) brdf = bsdf_sample(mat, b, wo, &rng.u[8], wi, pdf);
-                else brdf = guide_sample(gv, gslice, pp.res, mat, b, sn, wo, &rng.u[8], wi, pdf);
-                if (!(pdf > 0.0f)) break;
-                ray = ray_ctor(p, wi);
-                nclosest++;
-                tri = traverse<false>(sv, ray, pp.tmin, 1e6f, stk, pitch, th, tb, tg, st);
-                if (tri < 0) break;
-                int4 sf = sv.surf[__float_as_int(sv.tri_n[tri].w)];
-                if (sf.y >= 0) break;                               // light: counted by the light sample
-                float pdf_other = brdf_sampling ? guide_pdf(gv, gslice, pp.res, b, sn, wi) : bsdf_pdf(mat, b, wo, wi);
-                float weight = pdf / (pdf + pdf_other);
-                float cosv = fabsf(dot(wi, sn));
-                float s = ((weight * cosv) / pdf) * 2.0f;
-                T = (T * brdf) * s;
-            }
-            acc = acc + L;
-        }
+        for (int smp = 0; smp < pp.spp; ++smp)
+            acc = acc + trace_path(sv, cam.v, pp, gv, x, y, slice, strata, smp, stk, pitch, astk, st, nclosest,
+                                   nshadow);
         int64_t o = (int64_t)y * pp.width + x;
         out[3 * o] = acc.x / (float)pp.spp;
         out[3 * o + 1] = acc.y / (float)pp.spp;
         out[3 * o + 2] = acc.z / (float)pp.spp;
     }
-    if (work) {
-        unsigned long long a = nclosest, c = nshadow;
-        for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off); c += __shfl_xor(c, off); }
-        if ((threadIdx.x & 63) == 0) { atomicAdd(work, a); atomicAdd(work + 1, c); }
+    if (work) add_ray_counts(work, nclosest, nshadow);
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: one persistent kernel over a work queue of (job, 64-pixel
+// block, sample block) items.  A wave takes items_per_dequeue items per
+// atomic, finds each item's job by binary search over item_first, and writes
+// every sample's radiance; k_batch_reduce then sums each pixel's samples in
+// sample order, so a job's image is bit-equal to k_guided_pt's.
+
+__device__ __forceinline__ unsigned long long wave_bcast64(unsigned long long v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__global__ void k_batch_reset(const BatchHeader* __restrict__ hdr) {
+    if (threadIdx.x == 0) {
+        *hdr->counter = 0ull;
+        if (hdr->work) { hdr->work[0] = 0ull; hdr->work[1] = 0ull; }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_pt(SceneView sv, const BatchHeader* __restrict__ hdrp) {
+    extern __shared__ int s_stack[];
+    const BatchHeader hdr = *hdrp;
+    int pitch = blockDim.x;
+    int* stk = s_stack + threadIdx.x;
+    AnyStack astk = any_stack(sv, s_stack);
+    const int lane = threadIdx.x & 63;
+    TravStats st;
+    unsigned long long nclosest = 0, nshadow = 0;
+    const unsigned long long n_items = (unsigned long long)hdr.n_items;
+    const unsigned long long q = (unsigned long long)hdr.items_per_dequeue;
+    const int last_job = (int)hdr.n_jobs - 1;
+    for (;;) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(hdr.counter, q);
+        base = wave_bcast64(base);
+        if (base >= n_items) break;
+        unsigned long long end = base + q < n_items ? base + q : n_items;
+        for (unsigned long long item = base; item < end; ++item) {
+            int lo = 0, hi = last_job;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if ((unsigned long long)hdr.jobs[mid].item_first <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            const JobGpu* J = hdr.jobs + lo;
+            int64_t local = (int64_t)(item - (unsigned long long)J->item_first);
+            int nsb = J->samp_blocks;
+            int64_t pb = local / nsb;
+            int sb = (int)(local - pb * nsb);
+            int64_t p = pb * 64 + lane;
+            const PtParams pp = J->pp;
+            if (p < (int64_t)pp.w * pp.h) {
+                int x = pp.x0 + (int)(p % pp.w), y = pp.y0 + (int)(p / pp.w);
+                const GuideView gv = J->gv;
+                int slice = pixel_slice(pp, gv, x, y);
+                int s0 = sb * J->samples_per_item;
+                int s1 = min(pp.spp, s0 + J->samples_per_item);
+                float* dst = hdr.samples + J->sample_off + p * pp.spp * 3;
+                for (int smp = s0; smp < s1; ++smp) {
+                    v3 L = trace_path(sv, J->cam.v, pp, gv, x, y, slice, J->strata, smp, stk, pitch, astk, st,
+                                      nclosest, nshadow);
+                    dst[3 * smp] = L.x;
+                    dst[3 * smp + 1] = L.y;
+                    dst[3 * smp + 2] = L.z;
+                }
+            }
+        }
+    }
+    if (hdr.work) add_ray_counts(hdr.work, nclosest, nshadow);
+}
+
+__global__ __launch_bounds__(256) void k_batch_reduce(const BatchHeader* __restrict__ hdrp) {
+    const BatchHeader hdr = *hdrp;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int last_job = (int)hdr.n_jobs - 1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < hdr.n_pixels; i += stride) {
+        int lo = 0, hi = last_job;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (hdr.jobs[mid].pix_first <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const JobGpu* J = hdr.jobs + lo;
+        int spp = J->pp.spp;
+        const float* src = hdr.samples + J->sample_off + (i - J->pix_first) * spp * 3;
+        v3 acc = mk(0.f, 0.f, 0.f);
+        for (int smp = 0; smp < spp; ++smp) acc = acc + mk(src[3 * smp], src[3 * smp + 1], src[3 * smp + 2]);
+        hdr.out[3 * i] = acc.x / (float)spp;
+        hdr.out[3 * i + 1] = acc.y / (float)spp;
+        hdr.out[3 * i + 2] = acc.z / (float)spp;
     }
 }
 
 }  // namespace rzk
 
 using namespace rzk;
+
+void launch_batch(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
+                  int reduce_grid) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, d_hdr);
+    size_t lds = sizeof(int) * (size_t)block * (size_t)std::max(sv.depth, kLdsStack);
+    hipLaunchKernelGGL(k_batch_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
+    hipLaunchKernelGGL(k_batch_reduce, dim3(reduce_grid), dim3(256), 0, stream, d_hdr);
+}
 
 void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv_in, const float* cam, const PtParams& pp, const GuideView& gv,
                       float* d_out, unsigned long long* d_work) {
