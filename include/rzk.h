@@ -315,6 +315,87 @@ typedef struct rzk_batch_trial { rzk_batch_config cfg; float ms; } rzk_batch_tri
 int rzk_batch_autotune(rzk_batch *batch, const rzk_render_job *jobs, int32_t n, rzk_batch_config *best,
                        rzk_batch_trial *tried, int32_t max_tried, int32_t *n_tried);
 
+/* ---- served form: resident clusters, per-camera guides, request loop ----
+ * The clusters (representatives and weights of LightSlice's global
+ * clustering) stay resident; a guide for any camera is then built from them
+ * alone: tile-centre primary hits, the reduced matrix restricted to the
+ * representatives (rows x clusters segments), cluster directions and the
+ * per-slice luminance maps (lightslice.cc:1011-1030, 2058-2151).  For the
+ * LightSlice's own camera the guide equals the LightSlice's maps bit for bit. */
+typedef struct rzk_clusters rzk_clusters;
+typedef struct rzk_server rzk_server;
+
+typedef struct rzk_guide_params {
+    int32_t tile;                   /* slice = tile x tile pixels */
+    int32_t map_resolution;         /* 32 (lightslice.cc:57) */
+    float map_epsilon;              /* 1e-3 (lightslice.cc:74) */
+    int32_t clamp_mode;             /* incident radiance clamp of the reduced matrix */
+    float incident_clamp;
+    float tmin;
+    int32_t map_gate;               /* as rzk_lightslice_params.map_gate */
+} rzk_guide_params;
+
+/* keep the LightSlice's kept clusters (representative VPLs, weights) and its
+ * guide parameters as a resident set owned by the returned handle */
+int rzk_lightslice_clusters(const rzk_lightslice *ls, rzk_clusters **out);
+int rzk_clusters_destroy(rzk_clusters *clusters);
+/* params NULL: the parameters of the LightSlice the clusters came from */
+int rzk_guide_create(rzk_ctx *ctx, const rzk_scene *scene, const rzk_clusters *clusters, const float *camera,
+                     const rzk_guide_params *params, rzk_guide **out);
+/* copy a guide table to host: "maps", "col_cdf" (slices*res*res f32),
+ * "row_cdf" (slices*res f32), "totals" (slices f32); bytes must match */
+int rzk_guide_get(const rzk_guide *guide, const char *name, void *dst, int64_t bytes);
+/* the plain per-request drop-in over any guide: rzk_lightslice_render_guided
+ * with the guide's camera (window, spp, depth, seed from params) */
+int rzk_render_guided(rzk_ctx *ctx, const rzk_guide *guide, const rzk_pt_params *params, float *out_rgb,
+                      int64_t *out_rays);
+
+typedef struct rzk_server_params {
+    int64_t memory_budget;          /* bytes of device memory for admitted requests (guides + images) */
+    int32_t round_items;            /* work items (64 pixels x samples_per_item) per round launch */
+    rzk_batch_config launch;        /* launch shape of the round kernel */
+    int32_t use_cluster_params;     /* 1: guide parameters of the clusters' LightSlice, else `guide` */
+    rzk_guide_params guide;
+} rzk_server_params;
+
+typedef struct rzk_request {
+    float camera[19];               /* camera record; width and height give the image size */
+    int32_t spp, max_bounce, direct_lighting, guide_mode;
+    uint32_t seed;
+    float tmin;
+} rzk_request;
+
+typedef struct rzk_request_stats {
+    double submitted, admitted, finished;   /* seconds on the server's clock */
+    int32_t rounds;                 /* round launches the request took part in */
+    int32_t pad;
+    int64_t device_bytes;           /* memory the request held while admitted */
+} rzk_request_stats;
+
+typedef struct rzk_server_stats {
+    int64_t submitted, finished, rounds, items;
+    int64_t peak_bytes;             /* most device memory admitted requests held at once */
+    double busy_seconds;            /* summed device time of the round launches */
+    double now;                     /* the server clock */
+} rzk_server_stats;
+
+/* a server owns a context (its own stream) and a worker thread on `device`;
+ * scene and clusters must outlive it */
+int rzk_server_create(int device, const rzk_scene *scene, const rzk_clusters *clusters,
+                      const rzk_server_params *params, rzk_server **out);
+int rzk_server_destroy(rzk_server *server);
+/* thread-safe; the request is queued at once and admitted when the memory
+ * budget allows, oldest first */
+int rzk_server_submit(rzk_server *server, const rzk_request *request, int64_t *ticket);
+/* block until the request is finished (timeout_s < 0: no limit; on time-out
+ * RZK_E_STATE), copy its [height][width][3] image to out_rgb (may be NULL)
+ * and its stats, and release it */
+int rzk_server_wait(rzk_server *server, int64_t ticket, double timeout_s, float *out_rgb,
+                    rzk_request_stats *stats);
+int rzk_server_get_stats(rzk_server *server, rzk_server_stats *out);
+/* message of the last failure (a failed worker fails every pending request) */
+const char *rzk_server_last_error(rzk_server *server);
+
 #ifdef __cplusplus
 }
 #endif

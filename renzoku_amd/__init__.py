@@ -6,8 +6,8 @@ synthetic scenes the workloads are defined on (`scenes`) and the VPL set
 generator that feeds the gather (`vplgen`).
 """
 from . import _lib, scenes, vplgen  # noqa: F401
-from ._lib import (Batch, Ctx, Guide, LightSlice, RzkError, Scene, Vpls, alphas, cluster_assign,  # noqa: F401
-                   gather, reduced_matrix, render_vpl, visibility)
+from ._lib import (Batch, Clusters, Ctx, Guide, LightSlice, RzkError, Scene, Server, Vpls, alphas,  # noqa: F401
+                   cluster_assign, gather, reduced_matrix, render_vpl, visibility)
 
-__all__ = ["Ctx", "Scene", "Vpls", "LightSlice", "Guide", "Batch", "RzkError", "gather", "render_vpl", "visibility",
+__all__ = ["Ctx", "Scene", "Vpls", "LightSlice", "Guide", "Batch", "Clusters", "Server", "RzkError", "gather", "render_vpl", "visibility",
            "reduced_matrix", "alphas", "cluster_assign", "scenes", "vplgen"]

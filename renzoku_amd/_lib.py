@@ -36,6 +36,9 @@ EXPORTS = [
     "rzk_lightslice_guide", "rzk_guide_destroy",
     "rzk_batch_create", "rzk_batch_destroy", "rzk_batch_set_config", "rzk_batch_get_config", "rzk_batch_render",
     "rzk_batch_autotune",
+    "rzk_lightslice_clusters", "rzk_clusters_destroy", "rzk_guide_create", "rzk_guide_get", "rzk_render_guided",
+    "rzk_server_create", "rzk_server_destroy", "rzk_server_submit", "rzk_server_wait", "rzk_server_get_stats",
+    "rzk_server_last_error",
 ]
 
 
@@ -94,6 +97,31 @@ class BatchConfig(C.Structure):
 
 class BatchTrial(C.Structure):
     _fields_ = [("cfg", BatchConfig), ("ms", C.c_float)]
+
+
+class GuideParams(C.Structure):
+    _fields_ = [("tile", C.c_int32), ("map_resolution", C.c_int32), ("map_epsilon", C.c_float),
+                ("clamp_mode", C.c_int32), ("incident_clamp", C.c_float), ("tmin", C.c_float), ("map_gate", C.c_int32)]
+
+
+class ServerParams(C.Structure):
+    _fields_ = [("memory_budget", C.c_int64), ("round_items", C.c_int32), ("launch", BatchConfig),
+                ("use_cluster_params", C.c_int32), ("guide", GuideParams)]
+
+
+class Request(C.Structure):
+    _fields_ = [("camera", C.c_float * 19), ("spp", C.c_int32), ("max_bounce", C.c_int32),
+                ("direct_lighting", C.c_int32), ("guide_mode", C.c_int32), ("seed", C.c_uint32), ("tmin", C.c_float)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("submitted", C.c_double), ("admitted", C.c_double), ("finished", C.c_double),
+                ("rounds", C.c_int32), ("pad", C.c_int32), ("device_bytes", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("finished", C.c_int64), ("rounds", C.c_int64), ("items", C.c_int64),
+                ("peak_bytes", C.c_int64), ("busy_seconds", C.c_double), ("now", C.c_double)]
 
 
 class LightSliceInfo(C.Structure):
@@ -162,12 +190,24 @@ def load() -> C.CDLL:
         "rzk_batch_render": [P, C.POINTER(RenderJob), I32, P, P, C.POINTER(C.c_float)],
         "rzk_batch_autotune": [P, C.POINTER(RenderJob), I32, C.POINTER(BatchConfig), C.POINTER(BatchTrial), I32,
                                C.POINTER(I32)],
+        "rzk_lightslice_clusters": [P, C.POINTER(P)],
+        "rzk_clusters_destroy": [P],
+        "rzk_guide_create": [P, P, P, P, C.POINTER(GuideParams), C.POINTER(P)],
+        "rzk_guide_get": [P, C.c_char_p, P, I64],
+        "rzk_render_guided": [P, P, C.POINTER(PtParams), P, P],
+        "rzk_server_create": [C.c_int, P, P, C.POINTER(ServerParams), C.POINTER(P)],
+        "rzk_server_destroy": [P],
+        "rzk_server_submit": [P, C.POINTER(Request), C.POINTER(I64)],
+        "rzk_server_wait": [P, I64, C.c_double, P, C.POINTER(RequestStats)],
+        "rzk_server_get_stats": [P, C.POINTER(ServerStats)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = C.c_int
     lib.rzk_ctx_last_error.restype = C.c_char_p
+    lib.rzk_server_last_error.argtypes = [P]
+    lib.rzk_server_last_error.restype = C.c_char_p
     lib.rzk_ctx_stream.restype = C.c_void_p
     _lib = lib
     return lib
@@ -438,6 +478,10 @@ class LightSlice:
         """borrow this LightSlice's guiding data (valid until it is initialised again)"""
         return Guide.from_lightslice(self, cam)
 
+    def clusters(self):
+        """the kept clusters as a resident set (rzk_lightslice_clusters)"""
+        return Clusters(self)
+
     def count_work(self):
         out = np.zeros(8, np.int64)
         self.ctx.check(self.lib.rzk_lightslice_count_work(self.h, ptr(out)), "rzk_lightslice_count_work")
@@ -477,6 +521,33 @@ class Guide:
         h = C.c_void_p()
         ls.ctx.check(ls.lib.rzk_lightslice_guide(ls.h, ptr(cam), C.byref(h)), "rzk_lightslice_guide")
         return cls(ls.ctx, h, int(cam[17]), int(cam[18]), keep=(ls,))
+
+    @classmethod
+    def create(cls, ctx, scene, clusters, cam, params=None):
+        """a guide for `cam` from resident clusters (rzk_guide_create)"""
+        cam = f32(cam)
+        h = C.c_void_p()
+        gp = GuideParams(**params) if params else None
+        ctx.check(ctx.lib.rzk_guide_create(ctx.h, scene.h, clusters.h, ptr(cam), C.byref(gp) if gp else None,
+                                           C.byref(h)), "rzk_guide_create")
+        return cls(ctx, h, int(cam[17]), int(cam[18]), keep=(scene, clusters))
+
+    def get(self, name, count):
+        out = np.zeros(count, np.float32)
+        self.ctx.check(self.lib.rzk_guide_get(self.h, name.encode(), ptr(out), out.nbytes), f"rzk_guide_get({name})")
+        return out
+
+    def render(self, spp=16, max_bounce=5, direct_lighting=0, guide_mode=1, seed=2013, tmin=1e-3, window=None,
+               count_rays=False):
+        """the plain per-request drop-in (rzk_render_guided): full [H][W][3] image"""
+        p = PtParams(spp, max_bounce, direct_lighting, guide_mode, seed, tmin, 0, 0, 0, 0)
+        if window is not None:
+            p.x0, p.y0, p.w, p.h = window
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        rays = np.zeros(2, np.int64) if count_rays else None
+        self.ctx.check(self.lib.rzk_render_guided(self.ctx.h, self.h, C.byref(p), ptr(out), ptr(rays)),
+                       "rzk_render_guided")
+        return out, ({"closest_rays": int(rays[0]), "shadow_rays": int(rays[1])} if count_rays else None)
 
     def close(self):
         if getattr(self, "h", None):
@@ -555,6 +626,91 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             self.lib.rzk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Clusters:
+    """rzk_clusters: a LightSlice's kept clusters, resident on the device"""
+
+    def __init__(self, ls):
+        self.ctx, self.lib = ls.ctx, ls.lib
+        h = C.c_void_p()
+        self.ctx.check(self.lib.rzk_lightslice_clusters(ls.h, C.byref(h)), "rzk_lightslice_clusters")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rzk_clusters_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """rzk_server: continuous-batching request loop over a resident scene and clusters"""
+
+    def __init__(self, device, scene, clusters, memory_budget=8 << 30, round_items=16384, launch=None, guide=None):
+        self.lib = load()
+        self.scene, self.clusters = scene, clusters
+        p = ServerParams()
+        p.memory_budget = int(memory_budget)
+        p.round_items = int(round_items)
+        p.launch = BatchConfig(**(launch or dict(block=256, blocks_per_cu=4, items_per_dequeue=1,
+                                                 samples_per_item=1)))
+        p.use_cluster_params = 0 if guide else 1
+        if guide:
+            p.guide = GuideParams(**guide)
+        h = C.c_void_p()
+        rc = self.lib.rzk_server_create(device, scene.h, clusters.h, C.byref(p), C.byref(h))
+        if rc != RZK_OK:
+            raise RzkError(f"rzk_server_create failed ({rc})")
+        self.h = h
+        self.sizes = {}
+
+    def _check(self, rc, what):
+        if rc != RZK_OK:
+            msg = self.lib.rzk_server_last_error(self.h)
+            raise RzkError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def submit(self, cam, spp=4, max_bounce=5, direct_lighting=0, guide_mode=1, seed=2013, tmin=1e-3):
+        r = Request()
+        cam = f32(cam)
+        for i in range(19):
+            r.camera[i] = float(cam[i])
+        r.spp, r.max_bounce, r.direct_lighting, r.guide_mode, r.seed, r.tmin = (spp, max_bounce, direct_lighting,
+                                                                               guide_mode, seed, tmin)
+        t = C.c_int64()
+        self._check(self.lib.rzk_server_submit(self.h, C.byref(r), C.byref(t)), "rzk_server_submit")
+        self.sizes[t.value] = (int(cam[18]), int(cam[17]))
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0, to_host=True):
+        H, W = self.sizes.pop(ticket)
+        out = np.zeros((H, W, 3), np.float32) if to_host else None
+        st = RequestStats()
+        self._check(self.lib.rzk_server_wait(self.h, ticket, float(timeout), ptr(out), C.byref(st)),
+                    "rzk_server_wait")
+        return out, {"submitted": st.submitted, "admitted": st.admitted, "finished": st.finished,
+                     "rounds": st.rounds, "device_bytes": st.device_bytes}
+
+    def stats(self):
+        st = ServerStats()
+        self._check(self.lib.rzk_server_get_stats(self.h, C.byref(st)), "rzk_server_get_stats")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rzk_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -99,6 +99,15 @@ __global__ void k_slice_pdf_norm(int rows, int nreps, const float* __restrict__ 
     c[nreps - 1] = 1.0f;
 }
 
+// tile-centre pixel of every slice (the row of the reduced matrix)
+__global__ void k_tile_centres(int tx, int ty, int tile, int W, int H, int* __restrict__ px, int* __restrict__ py) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= tx * ty) return;
+    int x = s % tx, y = s / tx;
+    px[s] = min(x * tile + tile / 2, W - 1);
+    py[s] = min(y * tile + tile / 2, H - 1);
+}
+
 // build_map (lightslice.cc:2058-2147) + DiscretePdf2D::set_distribution
 // (pdf.cc:195-233).  One block per slice: the scatter runs in cluster order in
 // one lane (the map sums must keep that order), rows are normalised in parallel.
@@ -269,7 +278,210 @@ static void ls_neighbours(rzk_lightslice* ls, std::vector<int>& nb, int k) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Guides from resident clusters
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+GuideLayout guide_layout(int W, int H, int tile, int res, int C) {
+    GuideLayout L;
+    L.tx = (W + tile - 1) / tile;
+    L.ty = (H + tile - 1) / tile;
+    L.S = L.tx * L.ty;
+    size_t S = (size_t)L.S, Cs = (size_t)std::max(C, 1), r = (size_t)res;
+    size_t tmp[12] = {4 * S, 4 * S, 4 * 6 * S, 16 * S, 4 * RZK_RECV_FLOATS * S, 4 * S, 4 * S,
+                      4 * 3 * S * Cs, 4 * S * Cs, 4 * 3 * S * Cs, 4 * S * Cs, 4 * S * r};
+    size_t tab[4] = {4 * S * r * r, 4 * S * r * r, 4 * S * r, 4 * S};
+    size_t o = 0;
+    for (int i = 0; i < 12; ++i) { L.off_tmp[i] = o; o += align256(tmp[i]); }
+    L.bytes_tmp = o;
+    o = 0;
+    for (int i = 0; i < 4; ++i) { L.off_tab[i] = o; o += align256(tab[i]); }
+    L.bytes_tab = o;
+    return L;
+}
+
+GuideBuffers guide_buffers(const GuideLayout& L, char* tmp, char* tab) {
+    GuideBuffers b;
+    b.px = (int*)(tmp + L.off_tmp[0]);
+    b.py = (int*)(tmp + L.off_tmp[1]);
+    b.rays = (float*)(tmp + L.off_tmp[2]);
+    b.hits = (float4*)(tmp + L.off_tmp[3]);
+    b.recv = (float*)(tmp + L.off_tmp[4]);
+    b.mat = (int*)(tmp + L.off_tmp[5]);
+    b.light = (int*)(tmp + L.off_tmp[6]);
+    b.R = (float*)(tmp + L.off_tmp[7]);
+    b.V = (float*)(tmp + L.off_tmp[8]);
+    b.slice_wi = (float*)(tmp + L.off_tmp[9]);
+    b.upper = (int*)(tmp + L.off_tmp[10]);
+    b.row_pdf = (float*)(tmp + L.off_tmp[11]);
+    b.maps = (float*)(tab + L.off_tab[0]);
+    b.col_cdf = (float*)(tab + L.off_tab[1]);
+    b.row_cdf = (float*)(tab + L.off_tab[2]);
+    b.totals = (float*)(tab + L.off_tab[3]);
+    return b;
+}
+
+// The per-slice stage of LightSlice::initialize for one camera, with the
+// reduced matrix restricted to the resident representatives: the same
+// (row, representative) values the full matrix holds, so the same maps.
+void enqueue_guide_build(rzk_ctx* ctx, const rzk_scene* scene, const rzk_clusters& cl, const float* cam,
+                         const rzk_guide_params& gp, const GuideLayout& L, const GuideBuffers& b) {
+    SceneView sv = scene_view(scene);
+    hipStream_t st = ctx->stream;
+    int W = (int)cam[17], H = (int)cam[18], S = L.S, C = cl.C, res = gp.map_resolution;
+    hipLaunchKernelGGL(k_tile_centres, dim3((S + 63) / 64), dim3(64), 0, st, L.tx, L.ty, gp.tile, W, H, b.px, b.py);
+    launch_camera_rays(ctx, cam, S, b.px, b.py, W, b.rays);
+    launch_trace_closest(ctx, sv, b.rays, S, 1e-3f, 1e6f, b.hits, nullptr);
+    launch_fill_receivers(ctx, sv, b.rays, b.hits, S, b.recv, b.mat, b.light);
+    launch_reduced_matrix(ctx, sv, b.recv, b.mat, S, cl.reps->d_order, cl.reps->d_vpl, cl.reps->d_mat, C,
+                          gp.clamp_mode, gp.incident_clamp, gp.tmin, b.R, b.V, nullptr);
+    int64_t sc = (int64_t)S * C;
+    hipLaunchKernelGGL(k_slice_info, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, st, sv.mat, b.recv, b.mat, S,
+                       b.V, cl.reps->d_vpl, cl.reps->d_mat, cl.d_ident, C, gp.clamp_mode, gp.incident_clamp,
+                       b.slice_wi, b.upper);
+    size_t lds = sizeof(float) * ((size_t)res * res + res + 1);
+    hipLaunchKernelGGL(k_slice_maps, dim3(S), dim3(64), lds, st, res, C, b.R, S, b.recv, cl.d_ident, cl.d_rep_w,
+                       b.slice_wi, b.upper, gp.map_epsilon, gp.map_gate ? 0 : 1, b.maps, b.col_cdf, b.row_pdf,
+                       b.row_cdf, b.totals);
+    RZK_HIP(hipGetLastError());
+}
+
+void guide_set(rzk_guide& g, const rzk_scene* scene, const float* cam, const rzk_guide_params& gp,
+               const GuideLayout& L, const GuideBuffers& b) {
+    g.scene = scene;
+    std::memcpy(g.cam, cam, sizeof(g.cam));
+    g.W = (int)cam[17]; g.H = (int)cam[18];
+    g.tile = gp.tile; g.tiles_x = L.tx; g.tiles_y = L.ty; g.res = gp.map_resolution;
+    g.gv.maps = b.maps; g.gv.row_cdf = b.row_cdf; g.gv.col_cdf = b.col_cdf; g.gv.totals = b.totals;
+    g.device_bytes = (int64_t)L.bytes_tab;
+}
+
+static void check_guide_params(const rzk_guide_params& gp, const float* cam) {
+    RZK_CHECK(gp.tile > 0 && gp.map_resolution > 0 && gp.map_resolution <= 64, RZK_E_INVALID,
+              "bad guide parameters (tile > 0, 0 < map_resolution <= 64)");
+    RZK_CHECK((int)cam[17] > 0 && (int)cam[18] > 0, RZK_E_INVALID, "camera without an image size");
+}
+
 extern "C" {
+
+int rzk_lightslice_clusters(const rzk_lightslice* ls, rzk_clusters** out) {
+    if (!ls || !out) return RZK_E_INVALID;
+    *out = nullptr;
+    rzk_ctx* ctx = ls->ctx;
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(ls->initialized && !ls->h_reps.empty(), RZK_E_STATE, "LightSlice has no clusters");
+        std::unique_ptr<rzk_clusters> cl(new rzk_clusters());
+        cl->ctx = ctx;
+        cl->scene = ls->scene;
+        cl->C = (int)ls->h_reps.size();
+        cl->h_reps = ls->h_reps;
+        cl->h_rep_w = ls->h_rep_w;
+        std::vector<float> vr((size_t)cl->C * RZK_VPL_FLOATS);
+        std::vector<int> mr(cl->C), ident(cl->C);
+        for (int c = 0; c < cl->C; ++c) {
+            int k = ls->h_reps[c];
+            std::memcpy(&vr[(size_t)c * RZK_VPL_FLOATS], &ls->vpls->h_vpl[(size_t)k * RZK_VPL_FLOATS],
+                        sizeof(float) * RZK_VPL_FLOATS);
+            mr[c] = ls->vpls->h_mat[k];
+            ident[c] = c;
+        }
+        int rc = rzk_vpls_create(ctx, vr.data(), mr.data(), cl->C, &cl->reps);
+        if (rc != RZK_OK) throw RzkError{rc, ctx->last_error};
+        RZK_HIP(hipMalloc((void**)&cl->d_rep_w, sizeof(float) * cl->C));
+        RZK_HIP(hipMalloc((void**)&cl->d_ident, sizeof(int) * cl->C));
+        RZK_HIP(hipMemcpy(cl->d_rep_w, cl->h_rep_w.data(), sizeof(float) * cl->C, hipMemcpyHostToDevice));
+        RZK_HIP(hipMemcpy(cl->d_ident, ident.data(), sizeof(int) * cl->C, hipMemcpyHostToDevice));
+        const rzk_lightslice_params& p = ls->prm;
+        cl->n_emitted = p.n_emitted;
+        cl->gp = rzk_guide_params{p.tile, p.map_resolution, p.map_epsilon, p.clamp_mode, p.incident_clamp, p.tmin,
+                                  p.map_gate};
+        *out = cl.release();
+    });
+}
+
+int rzk_clusters_destroy(rzk_clusters* cl) {
+    if (!cl) return RZK_OK;
+    (void)hipDeviceSynchronize();
+    rzk_vpls_destroy(cl->reps);
+    if (cl->d_rep_w) (void)hipFree(cl->d_rep_w);
+    if (cl->d_ident) (void)hipFree(cl->d_ident);
+    delete cl;
+    return RZK_OK;
+}
+
+int rzk_guide_create(rzk_ctx* ctx, const rzk_scene* scene, const rzk_clusters* cl, const float* camera,
+                     const rzk_guide_params* params, rzk_guide** out) {
+    if (!ctx || !scene || !cl || !camera || !out) return RZK_E_INVALID;
+    *out = nullptr;
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(cl->scene == scene, RZK_E_INVALID, "clusters belong to another scene");
+        rzk_guide_params gp = params ? *params : cl->gp;
+        check_guide_params(gp, camera);
+        GuideLayout L = guide_layout((int)camera[17], (int)camera[18], gp.tile, gp.map_resolution, cl->C);
+        char *tmp = nullptr, *tab = nullptr;
+        RZK_HIP(hipMalloc((void**)&tmp, L.bytes_tmp));
+        hipError_t e = hipMalloc((void**)&tab, L.bytes_tab);
+        if (e != hipSuccess) { (void)hipFree(tmp); RZK_HIP(e); }
+        std::unique_ptr<rzk_guide> g(new rzk_guide());
+        g->ctx = ctx;
+        g->owned.push_back(tab);
+        GuideBuffers b = guide_buffers(L, tmp, tab);
+        try {
+            enqueue_guide_build(ctx, scene, *cl, camera, gp, L, b);
+            RZK_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            (void)hipFree(tmp);
+            throw;
+        }
+        RZK_HIP(hipFree(tmp));
+        guide_set(*g, scene, camera, gp, L, b);
+        *out = g.release();
+    });
+}
+
+int rzk_guide_get(const rzk_guide* g, const char* name, void* dst, int64_t bytes) {
+    if (!g || !name || !dst) return RZK_E_INVALID;
+    rzk_ctx* ctx = g->ctx;
+    return ls_guard(ctx, [&] {
+        std::string nm(name);
+        int64_t S = (int64_t)g->tiles_x * g->tiles_y, r = g->res;
+        const void* src = nullptr;
+        int64_t need = 0;
+        if (nm == "maps") { src = g->gv.maps; need = 4 * S * r * r; }
+        else if (nm == "col_cdf") { src = g->gv.col_cdf; need = 4 * S * r * r; }
+        else if (nm == "row_cdf") { src = g->gv.row_cdf; need = 4 * S * r; }
+        else if (nm == "totals") { src = g->gv.totals; need = 4 * S; }
+        RZK_CHECK(src != nullptr, RZK_E_INVALID, "unknown guide array: " + nm);
+        RZK_CHECK(bytes == need, RZK_E_INVALID, "size mismatch for " + nm);
+        RZK_HIP(hipMemcpyAsync(dst, src, (size_t)need, hipMemcpyDeviceToHost, ctx->stream));
+        ls_sync(ctx);
+    });
+}
+
+int rzk_render_guided(rzk_ctx* ctx, const rzk_guide* g, const rzk_pt_params* p, float* out_rgb, int64_t* out_rays) {
+    if (!ctx || !g || !p) return RZK_E_INVALID;
+    return ls_guard(ctx, [&] {
+        rzk_render_job j{g, p->w > 0 ? p->x0 : 0, p->h > 0 ? p->y0 : 0, p->w > 0 ? p->w : g->W, p->h > 0 ? p->h : g->H,
+                         p->spp, p->max_bounce, p->direct_lighting, p->guide_mode, p->seed, p->tmin};
+        check_job(g, j);
+        PtParams pp = job_params(*g, j);
+        int64_t n = (int64_t)g->W * g->H;
+        DBuf<float> img;
+        img.alloc(3 * n);
+        DBuf<unsigned long long> work;
+        work.alloc(2);
+        hipStream_t st = ctx->stream;
+        if ((int64_t)pp.w * pp.h != n) RZK_HIP(hipMemsetAsync(img.p, 0, sizeof(float) * 3 * n, st));
+        if (out_rays) RZK_HIP(hipMemsetAsync(work.p, 0, 2 * sizeof(unsigned long long), st));
+        launch_guided_pt(ctx, scene_view(g->scene), g->cam, pp, g->gv, img.p, out_rays ? work.p : nullptr);
+        if (out_rgb) RZK_HIP(hipMemcpyAsync(out_rgb, img.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st));
+        unsigned long long h[2] = {0, 0};
+        if (out_rays) RZK_HIP(hipMemcpyAsync(h, work.p, sizeof(h), hipMemcpyDeviceToHost, st));
+        ls_sync(ctx);
+        if (out_rays) { out_rays[0] = (int64_t)h[0]; out_rays[1] = (int64_t)h[1]; }
+    });
+}
 
 int rzk_render_vpl(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const float* camera, int clamp_mode,
                    float clamp, float tmin, float n_emitted, float* out_rgb) {

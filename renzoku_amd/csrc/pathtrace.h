@@ -51,6 +51,29 @@ struct BatchHeader {
     int items_per_dequeue, pad;
 };
 
+// One request's share of a served round: items [item_begin, item_begin +
+// count) of the request, in sample-major order (item = sample_block *
+// pix_blocks + pixel_block).  A round holds at most pix_blocks consecutive
+// items of a request, so no pixel appears twice in a round and each pixel's
+// samples are added to `acc` in sample order across rounds.
+struct ServeJobGpu {
+    CamView cam;
+    PtParams pp;
+    GuideView gv;
+    int64_t item_first;            // round-global index of the share's first item
+    int64_t item_begin;            // request-local index of that item
+    int pix_blocks, samples_per_item, strata, pad;
+    float* acc;                    // running per-pixel sums [h][w][3]
+    float* out;                    // per-pixel mean, written with the last sample
+};
+
+struct ServeHeader {
+    const ServeJobGpu* jobs;
+    int64_t n_jobs, n_items;
+    unsigned long long* counter;
+    int items_per_dequeue, pad;
+};
+
 }  // namespace rzk
 
 // A guide: the per-slice guiding distributions of one camera, borrowed from a
@@ -65,11 +88,55 @@ struct rzk_guide {
     int64_t device_bytes = 0;
 };
 
+// Resident cluster set: the kept representatives as a compact VPL set (their
+// own Morton order), weights, identity indices, and the guide parameters of
+// the LightSlice they came from.
+struct rzk_clusters {
+    rzk_ctx* ctx = nullptr;
+    const rzk_scene* scene = nullptr;
+    rzk_vpls* reps = nullptr;      // compact representative VPLs
+    float* d_rep_w = nullptr;
+    int* d_ident = nullptr;        // 0..C-1
+    int C = 0;
+    float n_emitted = 1.f;
+    rzk_guide_params gp{};
+    std::vector<int> h_reps;       // indices into the original VPL set
+    std::vector<float> h_rep_w;
+};
+
+// Device buffers of one guide build.  Temporaries are dead once the build's
+// kernels have run; the tables are the guide.
+struct GuideBuffers {
+    int* px = nullptr; int* py = nullptr;
+    float* rays = nullptr; float4* hits = nullptr;
+    float* recv = nullptr; int* mat = nullptr; int* light = nullptr;
+    float* R = nullptr; float* V = nullptr; float* slice_wi = nullptr; int* upper = nullptr; float* row_pdf = nullptr;
+    float* maps = nullptr; float* col_cdf = nullptr; float* row_cdf = nullptr; float* totals = nullptr;
+};
+
+struct GuideLayout {
+    int S = 0, tx = 0, ty = 0;
+    // byte offsets of every buffer within two regions (temporaries, tables), 256-B aligned
+    size_t off_tmp[12], off_tab[4];
+    size_t bytes_tmp = 0, bytes_tab = 0;
+};
+
+GuideLayout guide_layout(int W, int H, int tile, int res, int C);
+GuideBuffers guide_buffers(const GuideLayout& L, char* tmp, char* tab);
+// enqueue the guide build on ctx's stream (no synchronisation)
+void enqueue_guide_build(rzk_ctx* ctx, const rzk_scene* scene, const rzk_clusters& cl, const float* cam,
+                         const rzk_guide_params& gp, const GuideLayout& L, const GuideBuffers& b);
+// fill a guide handle's camera, sizes and views over the tables of `b`
+void guide_set(rzk_guide& g, const rzk_scene* scene, const float* cam, const rzk_guide_params& gp,
+               const GuideLayout& L, const GuideBuffers& b);
+
 void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv, const float* cam, const rzk::PtParams& pp,
                       const rzk::GuideView& gv, float* d_out, unsigned long long* d_work);
 // the three kernels of a batch on `stream` (captured into a graph by the caller)
 void launch_batch(hipStream_t stream, const SceneView& sv, const rzk::BatchHeader* d_hdr, int grid, int block,
                   int reduce_grid);
+// the two kernels of a served round (captured into a graph by the caller)
+void launch_serve_round(hipStream_t stream, const SceneView& sv, const rzk::ServeHeader* d_hdr, int grid, int block);
 // PtParams of a job over a guide
 rzk::PtParams job_params(const rzk_guide& g, const rzk_render_job& j);
 void check_job(const rzk_guide* g, const rzk_render_job& j);

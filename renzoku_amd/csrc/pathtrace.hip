@@ -368,9 +368,81 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const BatchHeader* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Served form: a round over shares of many requests.  Each lane adds its
+// pixel's samples of one sample block to the running sum in sample order and
+// writes the mean with the last sample, so the image is the plain render's.
+
+__global__ void k_serve_reset(const ServeHeader* __restrict__ hdr) {
+    if (threadIdx.x == 0) *hdr->counter = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_serve_pt(SceneView sv, const ServeHeader* __restrict__ hdrp) {
+    extern __shared__ int s_stack[];
+    const ServeHeader hdr = *hdrp;
+    int pitch = blockDim.x;
+    int* stk = s_stack + threadIdx.x;
+    AnyStack astk = any_stack(sv, s_stack);
+    const int lane = threadIdx.x & 63;
+    TravStats st;
+    unsigned long long nclosest = 0, nshadow = 0;
+    const unsigned long long n_items = (unsigned long long)hdr.n_items;
+    const unsigned long long q = (unsigned long long)hdr.items_per_dequeue;
+    const int last_job = (int)hdr.n_jobs - 1;
+    for (;;) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(hdr.counter, q);
+        base = wave_bcast64(base);
+        if (base >= n_items) break;
+        unsigned long long end = base + q < n_items ? base + q : n_items;
+        for (unsigned long long item = base; item < end; ++item) {
+            int lo = 0, hi = last_job;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if ((unsigned long long)hdr.jobs[mid].item_first <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            const ServeJobGpu* J = hdr.jobs + lo;
+            int64_t local = (int64_t)(item - (unsigned long long)J->item_first) + J->item_begin;
+            int64_t sb = local / J->pix_blocks;
+            int64_t pb = local - sb * J->pix_blocks;
+            int64_t p = pb * 64 + lane;
+            const PtParams pp = J->pp;
+            if (p < (int64_t)pp.w * pp.h) {
+                int x = pp.x0 + (int)(p % pp.w), y = pp.y0 + (int)(p / pp.w);
+                const GuideView gv = J->gv;
+                int slice = pixel_slice(pp, gv, x, y);
+                int s0 = (int)sb * J->samples_per_item;
+                int s1 = min(pp.spp, s0 + J->samples_per_item);
+                float* a = J->acc + 3 * p;
+                v3 acc = s0 == 0 ? mk(0.f, 0.f, 0.f) : mk(a[0], a[1], a[2]);
+                for (int smp = s0; smp < s1; ++smp)
+                    acc = acc + trace_path(sv, J->cam.v, pp, gv, x, y, slice, J->strata, smp, stk, pitch, astk, st,
+                                           nclosest, nshadow);
+                if (s1 == pp.spp) {
+                    float* o = J->out + 3 * p;
+                    o[0] = acc.x / (float)pp.spp;
+                    o[1] = acc.y / (float)pp.spp;
+                    o[2] = acc.z / (float)pp.spp;
+                } else {
+                    a[0] = acc.x;
+                    a[1] = acc.y;
+                    a[2] = acc.z;
+                }
+            }
+        }
+    }
+}
+
 }  // namespace rzk
 
 using namespace rzk;
+
+void launch_serve_round(hipStream_t stream, const SceneView& sv, const ServeHeader* d_hdr, int grid, int block) {
+    hipLaunchKernelGGL(k_serve_reset, dim3(1), dim3(64), 0, stream, d_hdr);
+    size_t lds = sizeof(int) * (size_t)block * (size_t)std::max(sv.depth, kLdsStack);
+    hipLaunchKernelGGL(k_serve_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
+}
 
 void launch_batch(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
                   int reduce_grid) {

@@ -1,0 +1,151 @@
+"""Served form: guides from resident clusters and the continuous-batching
+request loop (rzk_guide_create, rzk_server_*).
+
+* A guide built from the resident clusters for the LightSlice's own camera
+  equals the LightSlice's per-slice tables bit for bit (same (row,
+  representative) transport values, same map build).
+* A request's image is the same, bit for bit, alone and in a mix of
+  concurrent requests of other sizes, spp and cameras, and equals the plain
+  per-request drop-in (rzk_render_guided over rzk_guide_create).
+* Admission never holds more than the memory budget."""
+import math
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def resident(ctx):
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    sc = scenes.room(96, 80, seed=11, target_tris=20_000)
+    gs = rz.Scene(ctx, sc)
+    tracer, filler = vplgen.gpu_tracer(gs)
+    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 3072, seed=4, batch=8192)
+    vp = rz.Vpls(ctx, vpl, vmat)
+    ls = rz.LightSlice(ctx, gs, vp, width=96, height=80, tile=16, num_clusters=48, num_neighbor_slices=4,
+                       n_emitted=ne, seed=2013)
+    ls.initialize(sc.camera)
+    return sc, gs, vp, ls, ls.clusters()
+
+
+def random_camera(rng, w, h):
+    from renzoku_amd import scenes
+    eye = (rng.uniform(150, 850), rng.uniform(60, 240), rng.uniform(150, 850))
+    ang = rng.uniform(0, 2 * math.pi)
+    look = (eye[0] + 300 * math.cos(ang), rng.uniform(20, 200), eye[2] + 300 * math.sin(ang))
+    return scenes.camera(eye, look, (0, 1, 0), rng.uniform(40, 70), w, h)
+
+
+def test_guide_from_clusters_equals_lightslice(ctx, resident):
+    import renzoku_amd as rz
+    sc, gs, vp, ls, cl = resident
+    g = rz.Guide.create(ctx, gs, cl, sc.camera)
+    S, res = ls.info().num_slices, 32
+    for name, n in [("maps", S * res * res), ("col_cdf", S * res * res), ("row_cdf", S * res),
+                    ("totals", S)]:
+        a = g.get(name, n)
+        b = ls.get("map_totals" if name == "totals" else name, np.float32, n)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+    # and the plain render over it is LightSlice's render_guided
+    a, _ = g.render(spp=4, max_bounce=4, direct_lighting=1, seed=9)
+    b, _ = ls.render_guided(sc.camera, spp=4, max_bounce=4, direct_lighting=1, seed=9)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _plain(ctx, gs, cl, cam, kw):
+    import renzoku_amd as rz
+    g = rz.Guide.create(ctx, gs, cl, cam)
+    img, _ = g.render(**kw)
+    return img
+
+
+def test_served_alone_equals_mix_equals_plain(ctx, resident):
+    import renzoku_amd as rz
+    sc, gs, vp, ls, cl = resident
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(7):
+        w, h = int(rng.integers(24, 130)), int(rng.integers(24, 110))
+        reqs.append((random_camera(rng, w, h), dict(spp=int(rng.integers(1, 10)), max_bounce=int(rng.integers(2, 6)),
+                                                    direct_lighting=int(rng.integers(0, 2)), guide_mode=1,
+                                                    seed=int(rng.integers(0, 2 ** 31)))))
+    srv = rz.Server(0, gs, cl, memory_budget=1 << 30, round_items=2000)
+    cam0, kw0 = reqs[0]
+    alone, st_alone = srv.wait(srv.submit(cam0, **kw0))
+    tickets = [srv.submit(cam, **kw) for cam, kw in reqs]
+    mixed = [srv.wait(t) for t in tickets]
+    assert np.array_equal(alone.view(np.uint32), mixed[0][0].view(np.uint32))
+    for (cam, kw), (img, st) in zip(reqs, mixed):
+        ref = _plain(ctx, gs, cl, cam, kw)
+        assert img.shape == ref.shape
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+        assert st["finished"] >= st["admitted"] >= st["submitted"] >= 0
+    # several requests shared rounds (continuous batching, not one at a time)
+    stats = srv.stats()
+    assert stats["finished"] == 8 and stats["rounds"] < sum(st["rounds"] for _, st in mixed) + st_alone["rounds"]
+    print(f"served: {stats}", flush=True)
+    srv.close()
+
+
+def request_bytes(W, H, tile, res, C):
+    """device bytes a request needs at admission: guide tables, build
+    temporaries, running sums and image, each rounded to 256 B"""
+    a = lambda b: (b + 255) // 256 * 256
+    S = -(-W // tile) * -(-H // tile)
+    tmp = [4 * S, 4 * S, 24 * S, 16 * S, 84 * S, 4 * S, 4 * S, 12 * S * C, 4 * S * C, 12 * S * C, 4 * S * C,
+           4 * S * res]
+    tab = [4 * S * res * res, 4 * S * res * res, 4 * S * res, 4 * S]
+    return sum(map(a, tmp)) + sum(map(a, tab)) + 2 * 12 * W * H + 4 * 256
+
+
+def test_served_memory_budget_admission(ctx, resident):
+    """a budget that fits about one request at a time: every request still
+    completes with the right image and the peak stays within the budget"""
+    import renzoku_amd as rz
+    sc, gs, vp, ls, cl = resident
+    rng = np.random.default_rng(6)
+    cams = [random_camera(rng, 96, 80) for _ in range(4)]
+    budget = request_bytes(96, 80, 16, 32, ls.info().num_clusters) + 1000   # less than two requests
+    srv = rz.Server(0, gs, cl, memory_budget=budget, round_items=64)
+    kw = dict(spp=2, max_bounce=3, direct_lighting=1, guide_mode=1, seed=3)
+    tickets = [srv.submit(c, **kw) for c in cams]
+    out = [srv.wait(t, timeout=120.0) for t in tickets]
+    stats = srv.stats()
+    assert 0 < stats["peak_bytes"] <= budget
+    for c, (img, st) in zip(cams, out):
+        assert np.array_equal(img.view(np.uint32), _plain(ctx, gs, cl, c, kw).view(np.uint32))
+    # admission is oldest first and one at a time here: each admitted after the previous finished
+    for (_, a), (_, b) in zip(out, out[1:]):
+        assert b["admitted"] >= a["finished"] - 1e-3
+    with pytest.raises(rz.RzkError):
+        srv.submit(random_camera(rng, 1024, 1024), **kw)     # larger than the whole budget
+    srv.close()
+
+
+def test_served_concurrent_submitters(ctx, resident):
+    """requests submitted from several threads while rounds run"""
+    import renzoku_amd as rz
+    sc, gs, vp, ls, cl = resident
+    rng = np.random.default_rng(7)
+    reqs = [(random_camera(rng, int(rng.integers(16, 90)), int(rng.integers(16, 70))),
+             dict(spp=int(rng.integers(1, 5)), max_bounce=3, seed=i)) for i in range(12)]
+    srv = rz.Server(0, gs, cl, memory_budget=1 << 29, round_items=128)
+    results = {}
+
+    def client(part):
+        for i in part:
+            cam, kw = reqs[i]
+            results[i] = srv.wait(srv.submit(cam, **kw), timeout=120.0)[0]
+
+    ths = [threading.Thread(target=client, args=(range(k, 12, 3),)) for k in range(3)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    for i, (cam, kw) in enumerate(reqs):
+        assert np.array_equal(results[i].view(np.uint32), _plain(ctx, gs, cl, cam, kw).view(np.uint32)), i
+    srv.close()
