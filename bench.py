@@ -154,11 +154,229 @@ def run_guided(args):
     print(json.dumps(out), flush=True)
 
 
+def c3_scene_and_guides(args, levels):
+    """the 1M-triangle instanced scene at 1280x720 with one LightSlice guide per
+    VPL count in `levels` (16x16-pixel slices, 1,000 clusters each)"""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    W, H = 1280, 720
+    t0 = time.perf_counter()
+    sc = scenes.instanced_meshes(W, H, seed=2013, target_tris=args.tris or 1_000_000)
+    ctx = rz.Ctx(0)
+    scene = rz.Scene(ctx, sc)
+    log(f"[bench] scene: {sc.ntri} triangles, BVH depth {scene.info().depth} ({time.perf_counter() - t0:.1f} s)")
+    tracer, filler = vplgen.gpu_tracer(scene)
+    guides = []
+    for nv in levels:
+        vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, nv, seed=2013, batch=65536)
+        vp = rz.Vpls(ctx, vpl, vmat)
+        ls = rz.LightSlice(ctx, scene, vp, width=W, height=H, tile=16, num_clusters=min(1000, nv // 4),
+                           num_neighbor_slices=16, map_resolution=32, n_emitted=n_emitted, seed=2013)
+        t1 = time.perf_counter()
+        ls.initialize(sc.camera)
+        ctx.synchronize()
+        log(f"[bench] guide for {nv} VPLs: {ls.info().num_clusters} clusters, {(time.perf_counter() - t1) * 1e3:.0f} ms")
+        guides.append((nv, vp, ls, ls.guide(sc.camera)))
+    return sc, ctx, scene, guides
+
+
+def run_batched(args):
+    """256 render jobs of uneven size on the 1M-triangle scene: windows 32..256 px
+    a side, 1-64 spp, guides from 4k-100k VPLs; all jobs in one persistent,
+    graph-launched kernel with the autotuned launch shape, against the plain
+    per-request drop-in (one rzk_lightslice_render_guided per job)."""
+    import renzoku_amd as rz
+    levels = [4096, 8192, 16384, 32768, 65536, 100_000]
+    sc, ctx, scene, guides = c3_scene_and_guides(args, levels)
+    rng = np.random.default_rng(2013)
+    specs = []
+    for i in range(args.jobs):
+        w, h = int(rng.integers(32, 257)), int(rng.integers(32, 257))
+        nv, vp, ls, g = guides[int(rng.integers(0, len(guides)))]
+        specs.append(dict(guide=g, ls=ls, vpls=nv, window=(int(rng.integers(0, 1280 - w + 1)),
+                                                          int(rng.integers(0, 720 - h + 1)), w, h),
+                          spp=int(rng.integers(1, 65)), max_bounce=5, direct_lighting=0, guide_mode=1,
+                          seed=int(rng.integers(0, 2 ** 31))))
+    paths = sum(s["window"][2] * s["window"][3] * s["spp"] for s in specs)
+    log(f"[bench] {len(specs)} jobs, {paths / 1e6:.1f} M paths")
+    b = rz.Batch(ctx, scene)
+    tune = [dict(s, spp=max(1, s["spp"] // 4)) for s in specs[:: max(1, len(specs) // 48)]]
+    t0 = time.perf_counter()
+    best, tried = b.autotune(tune)
+    log(f"[bench] autotune: {len(tried)} candidates in {time.perf_counter() - t0:.1f} s -> {best}")
+    _, rays, _ = b.render(specs, to_host=False, count_rays=True)
+    ms = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        _, _, dev_ms = b.render(specs, to_host=False)
+        ms.append((time.perf_counter() - t0) * 1e3)
+        log(f"[bench] batched step {i + 1}/{args.steps}: {ms[-1]:.1f} ms (device {dev_ms:.1f} ms)")
+    batched_ms = float(np.mean(ms))
+    # plain per-request drop-in: one launch per job, back to back
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k, s in enumerate(specs):
+        s["ls"].render_guided(sc.camera, spp=s["spp"], max_bounce=5, direct_lighting=0, guide_mode=1, seed=s["seed"],
+                              window=s["window"], to_host=False, count_rays=False)
+        if (k + 1) % 64 == 0:
+            ctx.synchronize()
+            log(f"[bench] plain: {k + 1}/{len(specs)} jobs, {time.perf_counter() - t0:.1f} s")
+    ctx.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    # equal accuracy: the batched images are the plain ones, bit for bit (checked on a sample)
+    imgs, _, _ = b.render(specs, to_host=True)
+    checked = 0
+    for k in range(0, len(specs), max(1, len(specs) // 8)):
+        s = specs[k]
+        ref, _ = s["ls"].render_guided(sc.camera, spp=s["spp"], max_bounce=5, direct_lighting=0, guide_mode=1,
+                                       seed=s["seed"], window=s["window"], count_rays=False)
+        x0, y0, w, h = s["window"]
+        assert np.array_equal(imgs[k].view(np.uint32), ref[y0:y0 + h, x0:x0 + w].view(np.uint32)), k
+        checked += 1
+    nrays = rays["closest_rays"] + rays["shadow_rays"]
+    out = {"metric": "batched image-requests/s (guided path tracing jobs)",
+           "value": round(len(specs) / (batched_ms / 1e3), 3), "unit": "requests/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": 1, "ms_per_step": round(batched_ms, 1), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded instanced-mesh scene, seeded light-traced VPLs, seeded job mix)",
+           "config": {"workload": f"batched {len(specs)} guided path tracing jobs on 1M tris, windows 32..256 px, "
+                                  "1-64 spp, depth 5, guides from 4k-100k VPLs", "triangles": int(sc.ntri),
+                      "jobs": len(specs), "paths": int(paths), "vpl_levels": levels, "parallelism": "single GPU"},
+           "plain_requests_per_s": round(len(specs) / (plain_ms / 1e3), 3), "plain_ms": round(plain_ms, 1),
+           "speedup_vs_plain": round(plain_ms / batched_ms, 2), "mrays_per_s": round(nrays / batched_ms / 1e3, 1),
+           "rays": rays, "autotuned": best, "autotune_trials": [[c, round(t, 2)] for c, t in tried],
+           "accuracy": f"bit-equal to the plain render on {checked} sampled jobs"}
+    print(json.dumps(out), flush=True)
+
+
+def random_camera(rng, w, h):
+    """a seeded camera pose inside the instanced-mesh room (2000 x 600 x 2000)"""
+    import math
+    from renzoku_amd import scenes
+    eye = (rng.uniform(200, 1800), rng.uniform(120, 480), rng.uniform(100, 900))
+    look = (rng.uniform(200, 1800), rng.uniform(0, 300), rng.uniform(1000, 1900))
+    return scenes.camera(eye, look, (0, 1, 0), float(rng.uniform(45, 75)), w, h)
+
+
+def run_served(args):
+    """Poisson arrivals of render requests (square images 128..1024 px a side,
+    1-32 spp, depth 5, random camera poses) against a 2M-triangle scene with
+    200k VPLs whose 1,000 clusters stay resident; the server admits by memory
+    budget and batches continuously.  The same requests, arrival times and
+    images are then served by the plain per-request drop-in (guide build +
+    render_guided, one request at a time, first come first served)."""
+    import threading
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    t0 = time.perf_counter()
+    sc = scenes.instanced_meshes(1280, 720, seed=2013, target_tris=args.tris or 2_000_000)
+    ctx = rz.Ctx(0)
+    scene = rz.Scene(ctx, sc)
+    log(f"[bench] scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
+    tracer, filler = vplgen.gpu_tracer(scene)
+    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 200_000, seed=2013, batch=65536)
+    vp = rz.Vpls(ctx, vpl, vmat)
+    ls = rz.LightSlice(ctx, scene, vp, width=1280, height=720, tile=32, num_clusters=1000, num_neighbor_slices=16,
+                       n_emitted=n_emitted, seed=2013)
+    ls.initialize(sc.camera)
+    clusters = ls.clusters()
+    gparams = dict(tile=16, map_resolution=32, map_epsilon=1e-3, clamp_mode=rz._lib.CLAMP_FORM_FACTOR,
+                   incident_clamp=0.01, tmin=1e-3, map_gate=0)
+    log(f"[bench] {len(vmat)} VPLs, {ls.info().num_clusters} resident clusters")
+    rng = np.random.default_rng(args.seed)
+    reqs = []
+    for i in range(args.requests):
+        side = int(rng.integers(128, 1025))
+        reqs.append((random_camera(rng, side, side), dict(spp=int(rng.integers(1, 33)), max_bounce=5,
+                                                          direct_lighting=0, guide_mode=1,
+                                                          seed=int(rng.integers(0, 2 ** 31)))))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, len(reqs)))
+    paths = sum(int(c[17]) * int(c[18]) * kw["spp"] for c, kw in reqs)
+    log(f"[bench] {len(reqs)} requests, {paths / 1e6:.1f} M paths, arrivals over {arrivals[-1]:.2f} s "
+        f"at {args.rate}/s")
+    # warm the kernels once
+    g = rz.Guide.create(ctx, scene, clusters, reqs[0][0], gparams)
+    g.render(spp=1, max_bounce=5, window=(0, 0, 64, 64))
+    del g
+    srv = rz.Server(0, scene, clusters, memory_budget=args.budget_gb << 30, round_items=args.round_items,
+                    guide=gparams)
+    srv.wait(srv.submit(reqs[0][0], spp=1, max_bounce=1))
+    served = [None] * len(reqs)
+
+    def submitter():
+        start = time.perf_counter()
+        for i, (cam, kw) in enumerate(reqs):
+            dt = arrivals[i] - (time.perf_counter() - start)
+            if dt > 0:
+                time.sleep(dt)
+            served[i] = srv.submit(cam, **kw)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    s_imgs, s_lat, s_fin, s_sub = [], [], [], []
+    for i, t in enumerate(served):
+        img, st = srv.wait(t)
+        s_imgs.append(img)
+        s_lat.append(st["finished"] - st["submitted"])
+        s_fin.append(st["finished"])
+        s_sub.append(st["submitted"])
+        if (i + 1) % 16 == 0:
+            log(f"[bench] served {i + 1}/{len(reqs)}")
+    sstats = srv.stats()
+    served_span = max(s_fin) - min(s_sub)
+    srv.close()
+    # plain: first come first served, one request at a time, same arrival times
+    p_lat = []
+    start = time.perf_counter()
+    busy_until = 0.0
+    mismatches = 0
+    for i, (cam, kw) in enumerate(reqs):
+        now = time.perf_counter() - start
+        if now < arrivals[i]:
+            time.sleep(arrivals[i] - now)
+        g = rz.Guide.create(ctx, scene, clusters, cam, gparams)
+        img, _ = g.render(**kw)
+        del g
+        end = time.perf_counter() - start
+        p_lat.append(end - arrivals[i])
+        busy_until = end
+        mismatches += int(not np.array_equal(img.view(np.uint32), s_imgs[i].view(np.uint32)))
+        if (i + 1) % 16 == 0:
+            log(f"[bench] plain {i + 1}/{len(reqs)}: {end:.1f} s")
+    plain_span = busy_until - arrivals[0]
+    pct = lambda a, q: round(float(np.percentile(np.asarray(a) * 1e3, q)), 1)
+    out = {"metric": "served image-requests/s with p50/p99 latency",
+           "value": round(len(reqs) / served_span, 3), "unit": "requests/s", "n_gpus": 1,
+           "steps": len(reqs), "warmup": 1, "ms_per_step": round(served_span * 1e3 / len(reqs), 1),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded instanced-mesh scene, seeded VPLs, seeded Poisson request stream)",
+           "config": {"workload": f"served guided path tracing, Poisson {args.rate}/s, images 128..1024 px square, "
+                                  "1-32 spp, depth 5, random cameras, 2M tris, 200k VPLs, 1000 resident clusters",
+                      "triangles": int(sc.ntri), "vpls": int(len(vmat)), "requests": len(reqs),
+                      "paths": int(paths), "rate_per_s": args.rate, "memory_budget_gb": args.budget_gb,
+                      "round_items": args.round_items, "parallelism": "single GPU"},
+           "latency_ms": {"p50": pct(s_lat, 50), "p99": pct(s_lat, 99), "max": pct(s_lat, 100)},
+           "plain_requests_per_s": round(len(reqs) / plain_span, 3),
+           "plain_latency_ms": {"p50": pct(p_lat, 50), "p99": pct(p_lat, 99), "max": pct(p_lat, 100)},
+           "speedup_vs_plain": round(plain_span / served_span, 2),
+           "paths_per_s": round(paths / served_span, 1),
+           "server": {k: sstats[k] for k in ("rounds", "items", "peak_bytes", "busy_seconds")},
+           "accuracy": f"{len(reqs) - mismatches} of {len(reqs)} images bit-equal to the plain drop-in"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided"],
+    ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided", "batched", "served"],
                     help="lightslice: the 200k-triangle LightSlice gather step (the headline line); "
-                         "guided: guided path tracing of the 1M-triangle scene")
+                         "guided: guided path tracing of the 1M-triangle scene; batched: 256 uneven jobs")
+    ap.add_argument("--jobs", type=int, default=256, help="batched workload: number of jobs")
+    ap.add_argument("--requests", type=int, default=64, help="served workload: number of requests")
+    ap.add_argument("--rate", type=float, default=50.0, help="served workload: Poisson arrival rate (1/s)")
+    ap.add_argument("--budget-gb", type=int, default=16, help="served workload: memory budget (GiB)")
+    ap.add_argument("--round-items", type=int, default=16384, help="served workload: work items per round")
+    ap.add_argument("--seed", type=int, default=2013, help="served workload: request stream seed")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
@@ -175,6 +393,10 @@ def main():
         sys.exit(2)
     if args.workload == "guided":
         return run_guided(args)
+    if args.workload == "batched":
+        return run_batched(args)
+    if args.workload == "served":
+        return run_served(args)
 
     import renzoku_amd as rz
     from renzoku_amd import scenes, vplgen

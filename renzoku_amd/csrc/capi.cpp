@@ -107,8 +107,8 @@ void* ctx_scratch(rzk_ctx* ctx, size_t bytes) {
     return ctx->scratch;
 }
 
-void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads) {
-    int64_t extra = sv.stack4 - kLdsStack;
+void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries) {
+    int64_t extra = (entries > 0 ? entries : sv.stack4) - kLdsStack;
     if (extra <= 0 || threads <= 0) { sv.spill = nullptr; sv.spill_pitch = 0; return; }
     sv.spill = static_cast<int*>(ctx_scratch(ctx, sizeof(int) * (size_t)extra * (size_t)threads));
     sv.spill_pitch = threads;

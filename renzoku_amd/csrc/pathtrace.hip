@@ -29,9 +29,8 @@ __device__ __forceinline__ float balance(float a, float b) { return a / (a + b);
 // FirstBounce::radiance with no direction sampler: one light chosen by
 // Scene::sample_light (DiscretePdf, strict <), then AreaLight::first_bounce:
 // light sampling + BSDF sampling combined with the balance heuristic.
-__device__ v3 next_event(const SceneView& sv, const float* rc, int m, const PathRng& rng, float tmin, int* stk,
-                         int pitch, AnyStack& astk, TravStats& st, unsigned long long& nclosest,
-                         unsigned long long& nshadow) {
+__device__ v3 next_event(const SceneView& sv, const float* rc, int m, const PathRng& rng, float tmin,
+                         AnyStack& astk, TravStats& st, unsigned long long& nclosest, unsigned long long& nshadow) {
     float ul = u24(rng.u[1]);
     int k = -1;
     for (int i = 0; i < sv.nlights; ++i)
@@ -79,7 +78,7 @@ __device__ v3 next_event(const SceneView& sv, const float* rc, int m, const Path
         Ray r2 = ray_ctor(p, wi);
         float t = 0.f, tb = 0.f, tg = 0.f;
         nclosest++;
-        int tri = traverse<false>(sv, r2, tmin, 1e6f, stk, pitch, t, tb, tg, st);
+        int tri = closest_hit4(sv, r2, tmin, 1e6f, astk, t, tb, tg, st);
         if (tri >= 0) {
             float4 nf = sv.tri_n[tri];
             int4 sf = sv.surf[__float_as_int(nf.w)];
@@ -189,7 +188,7 @@ __device__ __forceinline__ int spp_strata(int spp) {
 // throughput product (vpl.cc:934-1041).  Every random number comes from
 // Philox(seed, pixel, sample, bounce), so the value depends on nothing else.
 __device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& pp, const GuideView& gv, int x,
-                         int y, int slice, int strata, int smp, int* stk, int pitch, AnyStack& astk, TravStats& st,
+                         int y, int slice, int strata, int smp, AnyStack& astk, TravStats& st,
                          unsigned long long& nclosest, unsigned long long& nshadow) {
     uint32_t pixel = (uint32_t)(y * pp.width + x);
     PathRng r0 = path_rng(pp.seed, pixel, (uint32_t)smp, 0u);
@@ -202,7 +201,7 @@ __device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& 
     v3 L = mk(0.f, 0.f, 0.f), T = mk(1.f, 1.f, 1.f);
     float th = 0.f, tb = 0.f, tg = 0.f;
     nclosest++;
-    int tri = traverse<false>(sv, ray, pp.tmin, 1e6f, stk, pitch, th, tb, tg, st);
+    int tri = closest_hit4(sv, ray, pp.tmin, 1e6f, astk, th, tb, tg, st);
     for (int bounce = 1; tri >= 0 && bounce <= pp.max_bounce; ++bounce) {
         float rc[RZK_RECV_FLOATS];
         int m, lidx;
@@ -215,7 +214,7 @@ __device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& 
         PathRng rng = path_rng(pp.seed, pixel, (uint32_t)smp, (uint32_t)bounce);
         bool brdf_sampling = u24(rng.u[0]) < 0.5f;
         if (bounce > 1 || pp.direct)
-            L = L + T * next_event(sv, rc, m, rng, pp.tmin, stk, pitch, astk, st, nclosest, nshadow);
+            L = L + T * next_event(sv, rc, m, rng, pp.tmin, astk, st, nclosest, nshadow);
         if (bounce >= pp.max_bounce) break;
         const float* mat = sv.mat + RZK_MAT_FLOATS * m;
         Onb b; b.u = ld3(rc + 12); b.v = ld3(rc + 15); b.n = ld3(rc + 18);
@@ -228,7 +227,7 @@ __device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& 
         if (!(pdf > 0.0f)) break;
         ray = ray_ctor(p, wi);
         nclosest++;
-        tri = traverse<false>(sv, ray, pp.tmin, 1e6f, stk, pitch, th, tb, tg, st);
+        tri = closest_hit4(sv, ray, pp.tmin, 1e6f, astk, th, tb, tg, st);
         if (tri < 0) break;
         int4 sf = sv.surf[__float_as_int(sv.tri_n[tri].w)];
         if (sf.y >= 0) break;                               // light: counted by the light sample
@@ -251,8 +250,6 @@ __device__ __forceinline__ void add_ray_counts(unsigned long long* work, unsigne
 __global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, PtParams pp, GuideView gv,
                                                    float* __restrict__ out, unsigned long long* __restrict__ work) {
     extern __shared__ int s_stack[];
-    int pitch = blockDim.x;
-    int* stk = s_stack + threadIdx.x;
     AnyStack astk = any_stack(sv, s_stack);
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     TravStats st;
@@ -264,7 +261,7 @@ __global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, Pt
         int strata = spp_strata(pp.spp);
         v3 acc = mk(0.f, 0.f, 0.f);
         for (int smp = 0; smp < pp.spp; ++smp)
-            acc = acc + trace_path(sv, cam.v, pp, gv, x, y, slice, strata, smp, stk, pitch, astk, st, nclosest,
+            acc = acc + trace_path(sv, cam.v, pp, gv, x, y, slice, strata, smp, astk, st, nclosest,
                                    nshadow);
         int64_t o = (int64_t)y * pp.width + x;
         out[3 * o] = acc.x / (float)pp.spp;
@@ -297,8 +294,6 @@ __global__ void k_batch_reset(const BatchHeader* __restrict__ hdr) {
 __global__ __launch_bounds__(256) void k_batch_pt(SceneView sv, const BatchHeader* __restrict__ hdrp) {
     extern __shared__ int s_stack[];
     const BatchHeader hdr = *hdrp;
-    int pitch = blockDim.x;
-    int* stk = s_stack + threadIdx.x;
     AnyStack astk = any_stack(sv, s_stack);
     const int lane = threadIdx.x & 63;
     TravStats st;
@@ -334,7 +329,7 @@ __global__ __launch_bounds__(256) void k_batch_pt(SceneView sv, const BatchHeade
                 int s1 = min(pp.spp, s0 + J->samples_per_item);
                 float* dst = hdr.samples + J->sample_off + p * pp.spp * 3;
                 for (int smp = s0; smp < s1; ++smp) {
-                    v3 L = trace_path(sv, J->cam.v, pp, gv, x, y, slice, J->strata, smp, stk, pitch, astk, st,
+                    v3 L = trace_path(sv, J->cam.v, pp, gv, x, y, slice, J->strata, smp, astk, st,
                                       nclosest, nshadow);
                     dst[3 * smp] = L.x;
                     dst[3 * smp + 1] = L.y;
@@ -380,8 +375,6 @@ __global__ void k_serve_reset(const ServeHeader* __restrict__ hdr) {
 __global__ __launch_bounds__(256) void k_serve_pt(SceneView sv, const ServeHeader* __restrict__ hdrp) {
     extern __shared__ int s_stack[];
     const ServeHeader hdr = *hdrp;
-    int pitch = blockDim.x;
-    int* stk = s_stack + threadIdx.x;
     AnyStack astk = any_stack(sv, s_stack);
     const int lane = threadIdx.x & 63;
     TravStats st;
@@ -417,7 +410,7 @@ __global__ __launch_bounds__(256) void k_serve_pt(SceneView sv, const ServeHeade
                 float* a = J->acc + 3 * p;
                 v3 acc = s0 == 0 ? mk(0.f, 0.f, 0.f) : mk(a[0], a[1], a[2]);
                 for (int smp = s0; smp < s1; ++smp)
-                    acc = acc + trace_path(sv, J->cam.v, pp, gv, x, y, slice, J->strata, smp, stk, pitch, astk, st,
+                    acc = acc + trace_path(sv, J->cam.v, pp, gv, x, y, slice, J->strata, smp, astk, st,
                                            nclosest, nshadow);
                 if (s1 == pp.spp) {
                     float* o = J->out + 3 * p;
@@ -440,14 +433,14 @@ using namespace rzk;
 
 void launch_serve_round(hipStream_t stream, const SceneView& sv, const ServeHeader* d_hdr, int grid, int block) {
     hipLaunchKernelGGL(k_serve_reset, dim3(1), dim3(64), 0, stream, d_hdr);
-    size_t lds = sizeof(int) * (size_t)block * (size_t)std::max(sv.depth, kLdsStack);
+    size_t lds = sizeof(int) * (size_t)block * (size_t)kLdsStack;
     hipLaunchKernelGGL(k_serve_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
 }
 
 void launch_batch(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
                   int reduce_grid) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, d_hdr);
-    size_t lds = sizeof(int) * (size_t)block * (size_t)std::max(sv.depth, kLdsStack);
+    size_t lds = sizeof(int) * (size_t)block * (size_t)kLdsStack;
     hipLaunchKernelGGL(k_batch_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
     hipLaunchKernelGGL(k_batch_reduce, dim3(reduce_grid), dim3(256), 0, stream, d_hdr);
 }
@@ -459,10 +452,10 @@ void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv_in, const float* cam, co
     SceneView sv = sv_in;
     const int block = 128;
     unsigned grid = (unsigned)((n + block - 1) / block);
-    bind_spill(ctx, sv, (int64_t)grid * block);
+    bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4);
     CamView cv;
     for (int i = 0; i < RZK_CAM_FLOATS; ++i) cv.v[i] = cam[i];
-    size_t lds = sizeof(int) * block * (size_t)std::max(sv.depth, kLdsStack);
+    size_t lds = sizeof(int) * block * (size_t)kLdsStack;
     hipLaunchKernelGGL(k_guided_pt, dim3(grid), dim3(block), lds, ctx->stream, sv, cv, pp, gv, d_out, d_work);
     RZK_HIP(hipGetLastError());
 }

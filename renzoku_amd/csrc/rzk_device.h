@@ -170,6 +170,92 @@ __device__ __forceinline__ bool any_hit4(const SceneView& sv, const Ray& r, floa
     }
 }
 
+// Closest hit over the 4-wide collapse, nearest child first.  Leaf children
+// are tested as soon as their box passes (shrinking tmax); interior children
+// that pass are visited in order of entry distance, the others pushed as
+// (node, entry) pairs and dropped at pop time when their entry lies beyond
+// the current tmax.  The hit is the same triangle, t, beta and gamma as the
+// reference-order traversal whenever the closest triangle is unique (the
+// triangle test is the same arithmetic and every triangle whose box chain
+// passes at the final tmax is tested); on exact ties in t the winner may
+// differ.  The stack holds up to 2 * sv.stack4 ints.
+__device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, float tmin, float tmax,
+                                            AnyStack& stk, float& t_out, float& b_out, float& g_out,
+                                            TravStats& st) {
+    if (!box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]), mk(sv.rbmax[0], sv.rbmax[1], sv.rbmax[2]),
+                 r, tmin, tmax))
+        return -1;
+    int best = -1;
+    int node = sv.root4_ref;
+    float t, b, g;
+    if (node < 0) {
+        const TriGpu* tp = sv.tris + (~node);
+        float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
+        st.tris++;
+        if (tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g)) {
+            t_out = t; b_out = b; g_out = g;
+            best = __float_as_int(p0.w);
+        }
+        return best;
+    }
+    int sp = 0;
+    for (;;) {
+        const Bvh4NodeGpu* np = sv.nodes4 + node;
+        float4 lx = np->lox, ly = np->loy, lz = np->loz, hx = np->hix, hy = np->hiy, hz = np->hiz;
+        int4 rf = np->ref;
+        st.nodes++;
+        const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w};
+        const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+        const int rfa[4] = {rf.x, rf.y, rf.z, rf.w};
+        float cd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+        int cr[4] = {-1, -1, -1, -1};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            int ref = rfa[c];
+            if (ref == kEmptyRef) continue;
+            float te;
+            if (!box_enter(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax, te)) continue;
+            if (ref < 0) {
+                const TriGpu* tp = sv.tris + (~ref);
+                float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
+                st.tris++;
+                if (tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b,
+                            g)) {
+                    tmax = t; t_out = t; b_out = b; g_out = g;
+                    best = __float_as_int(p0.w);
+                }
+            } else {
+                cd[c] = te;
+                cr[c] = ref;
+            }
+        }
+        // ascending entry distance (5-exchange network); empty slots carry +inf
+#define RZK_CX(i, j)                                                              \
+        if (cd[j] < cd[i]) {                                                      \
+            float td = cd[i]; cd[i] = cd[j]; cd[j] = td;                          \
+            int tr = cr[i]; cr[i] = cr[j]; cr[j] = tr;                            \
+        }
+        RZK_CX(0, 1) RZK_CX(2, 3) RZK_CX(0, 2) RZK_CX(1, 3) RZK_CX(1, 2)
+#undef RZK_CX
+        node = -1;
+#pragma unroll
+        for (int c = 3; c >= 1; --c) {
+            if (cr[c] >= 0 && cd[c] <= tmax) {
+                stk.push(sp, cr[c]);
+                stk.push(sp, __float_as_int(cd[c]));
+            }
+        }
+        if (cr[0] >= 0 && cd[0] <= tmax) node = cr[0];
+        while (node < 0) {
+            if (sp == 0) return best;
+            float d = __int_as_float(stk.pop(sp));
+            int ref = stk.pop(sp);
+            if (d <= tmax) node = ref;
+        }
+    }
+}
+
 // Wave-coherent any hit: the 64 lanes walk the 4-wide tree together.  The
 // node index is wave-uniform (readfirstlane), so a node is fetched once per
 // wave through the scalar cache; each lane tests the four child boxes against

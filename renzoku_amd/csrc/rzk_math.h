@@ -86,6 +86,17 @@ RZK_HD bool box_hit(v3 bmin, v3 bmax, const Ray& r, float tmin, float tmax) {
     return (tminbox <= tmaxbox) && (tmax >= tminbox && tmaxbox >= tmin);
 }
 
+// box_hit that also returns the entry distance (max of tmin and the slab entries)
+RZK_HD bool box_enter(v3 bmin, v3 bmax, const Ray& r, float tmin, float tmax, float& tenter) {
+    float x0 = bmin.x * r.inv_d.x + r.org_inv_d.x, x1 = bmax.x * r.inv_d.x + r.org_inv_d.x;
+    float y0 = bmin.y * r.inv_d.y + r.org_inv_d.y, y1 = bmax.y * r.inv_d.y + r.org_inv_d.y;
+    float z0 = bmin.z * r.inv_d.z + r.org_inv_d.z, z1 = bmax.z * r.inv_d.z + r.org_inv_d.z;
+    float tminbox = fmaxf(fmaxf(tmin, fminf(x0, x1)), fmaxf(fminf(y0, y1), fminf(z0, z1)));
+    float tmaxbox = fminf(fminf(tmax, fmaxf(x0, x1)), fminf(fmaxf(y0, y1), fmaxf(z0, z1)));
+    tenter = tminbox;
+    return (tminbox <= tmaxbox) && (tmax >= tminbox && tmaxbox >= tmin);
+}
+
 // Triangle::hit, triangle.cc:158-232, on the precomputed edges p01 = p0 - p1,
 // p02 = p0 - p2 (the same subtractions the reference performs per test).
 RZK_HD bool tri_hit(v3 p0, v3 p01, v3 p02, const Ray& r, float tmin, float tmax,

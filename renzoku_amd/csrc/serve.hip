@@ -436,7 +436,7 @@ int rzk_server_create(int device, const rzk_scene* scene, const rzk_clusters* cl
         s->arena.init((size_t)prm->memory_budget);
         s->sv = scene_view(scene);
         int64_t threads = (int64_t)s->grid() * c.block;
-        int64_t extra = s->sv.stack4 - kLdsStack;
+        int64_t extra = 2 * (int64_t)s->sv.stack4 - kLdsStack;   // closest_hit4 pairs
         if (extra > 0) {
             RZK_HIP(hipMalloc((void**)&s->spill, sizeof(int) * (size_t)(extra * threads)));
             s->sv.spill = s->spill;
