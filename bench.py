@@ -140,6 +140,18 @@ def run_guided(args):
     ctx.synchronize()
     ms_per_step = (time.perf_counter() - t_start) * 1e3 / args.steps
     nrays = rays["closest_rays"] + rays["shadow_rays"]
+    # the same frame as one job of the wavefront engine (bit-equal image)
+    b = rz.Batch(ctx, scene, dict(engine=1))
+    job = [dict(guide=ls.guide(sc.camera), window=(0, 0, W, H), **{k: v for k, v in kw.items() if k != "to_host"})]
+    imgs, wrays, _ = b.render(job, count_rays=True)
+    ref, _ = ls.render_guided(sc.camera, count_rays=False, **dict(kw, to_host=True))
+    same = bool(np.array_equal(imgs[0].view(np.uint32), ref.view(np.uint32)))
+    wf_ms = []
+    for i in range(args.steps):
+        _, _, dev_ms = b.render(job, to_host=False)
+        wf_ms.append(dev_ms)
+        log(f"[bench] wavefront step {i + 1}/{args.steps}: {dev_ms:.2f} ms")
+    wf_ms = float(np.mean(wf_ms))
     out = {"metric": "guided path tracing Mrays/s (closest + shadow)", "value": round(nrays / (ms_per_step / 1e3) / 1e6, 3),
            "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(ms_per_step, 3), "kernel_ms": round(float(np.mean(ms)), 3),
@@ -150,7 +162,9 @@ def run_guided(args):
                       "spp": 16, "max_bounce": 5, "vpls": int(len(vmat)), "slices": int(inf.num_slices),
                       "clusters": int(inf.num_clusters), "parallelism": "single GPU"},
            "rays": rays, "paths_per_s": round(W * H * 16 / (ms_per_step / 1e3), 1),
-           "guide_build_ms": round(init_ms, 1)}
+           "guide_build_ms": round(init_ms, 1),
+           "wavefront": {"ms": round(wf_ms, 3), "mrays_per_s": round(nrays / wf_ms / 1e3, 1),
+                         "same_rays": wrays == rays, "bit_equal_image": same, "config": b.config}}
     print(json.dumps(out), flush=True)
 
 

@@ -290,12 +290,15 @@ typedef struct rzk_render_job {
 
 typedef struct rzk_batch_config {
     int32_t block;                  /* workgroup size: 64, 128, 192 or 256 lanes */
-    int32_t blocks_per_cu;          /* persistent grid = compute units x blocks_per_cu */
-    int32_t items_per_dequeue;      /* queue depth: work items a wave takes per atomic */
-    int32_t samples_per_item;       /* samples per work item (a work item is 64 pixels) */
+    int32_t blocks_per_cu;          /* grid = compute units x blocks_per_cu */
+    int32_t items_per_dequeue;      /* persistent engine: work items a wave takes per atomic */
+    int32_t samples_per_item;       /* persistent engine: samples per work item (64 pixels) */
+    int32_t engine;                 /* 0: persistent one-lane-per-pixel kernel over a work queue;
+                                       1: wavefront engine (paths in slots, compacted ray queues) */
+    int32_t path_slots;             /* wavefront engine: paths in flight (queue depth) */
 } rzk_batch_config;
 
-/* cfg NULL: defaults {128, 4, 1, 4} */
+/* cfg NULL: defaults {256, 8, 1, 1, 1, 1048576} */
 int rzk_batch_create(rzk_ctx *ctx, const rzk_scene *scene, const rzk_batch_config *cfg, rzk_batch **out);
 int rzk_batch_destroy(rzk_batch *batch);
 int rzk_batch_set_config(rzk_batch *batch, const rzk_batch_config *cfg);
@@ -306,9 +309,10 @@ int rzk_batch_get_config(const rzk_batch *batch, rzk_batch_config *out);
  * out_ms (optional): device time of the launch.  Synchronous. */
 int rzk_batch_render(rzk_batch *batch, const rzk_render_job *jobs, int32_t n, float *out_rgb,
                      int64_t *out_rays, float *out_ms);
-/* time candidate configurations on `jobs` (coordinate descent over
- * samples_per_item, block, blocks_per_cu, items_per_dequeue), keep the
- * fastest; tried (optional, max_tried entries) receives each candidate with
+/* time candidate configurations on `jobs` (the engine, then coordinate
+ * descent over its launch shape: block, blocks_per_cu and path_slots for the
+ * wavefront engine; samples_per_item, block, blocks_per_cu and
+ * items_per_dequeue for the persistent one), keep the fastest; tried (optional, max_tried entries) receives each candidate with
  * its time in ms in rzk_batch_trial records; returns the number tried in
  * *n_tried. */
 typedef struct rzk_batch_trial { rzk_batch_config cfg; float ms; } rzk_batch_trial;

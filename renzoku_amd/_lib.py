@@ -89,7 +89,9 @@ class RenderJob(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("items_per_dequeue", C.c_int32),
-                ("samples_per_item", C.c_int32)]
+                ("samples_per_item", C.c_int32), ("engine", C.c_int32), ("path_slots", C.c_int32)]
+    DEFAULTS = dict(block=256, blocks_per_cu=8, items_per_dequeue=1, samples_per_item=1, engine=1,
+                    path_slots=1 << 20)
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -578,7 +580,7 @@ class Batch:
     def __init__(self, ctx, scene, config=None):
         self.ctx, self.lib = ctx, ctx.lib
         h = C.c_void_p()
-        cfg = BatchConfig(**config) if config else None
+        cfg = BatchConfig(**dict(BatchConfig.DEFAULTS, **config)) if config else None
         ctx.check(self.lib.rzk_batch_create(ctx.h, scene.h, C.byref(cfg) if cfg else None, C.byref(h)),
                   "rzk_batch_create")
         self.h = h
@@ -592,7 +594,7 @@ class Batch:
 
     @config.setter
     def config(self, d):
-        c = BatchConfig(**d)
+        c = BatchConfig(**dict(BatchConfig.DEFAULTS, **d))
         self.ctx.check(self.lib.rzk_batch_set_config(self.h, C.byref(c)), "rzk_batch_set_config")
 
     def render(self, specs, to_host=True, count_rays=False):
@@ -665,8 +667,8 @@ class Server:
         p = ServerParams()
         p.memory_budget = int(memory_budget)
         p.round_items = int(round_items)
-        p.launch = BatchConfig(**(launch or dict(block=256, blocks_per_cu=4, items_per_dequeue=1,
-                                                 samples_per_item=1)))
+        p.launch = BatchConfig(**dict(BatchConfig.DEFAULTS, **(launch or dict(block=256, blocks_per_cu=4,
+                                                                           engine=0))))
         p.use_cluster_params = 0 if guide else 1
         if guide:
             p.guide = GuideParams(**guide)

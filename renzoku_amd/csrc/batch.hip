@@ -69,7 +69,13 @@ int batch_guard(rzk_ctx* ctx, F&& f) {
 bool valid_config(const rzk_batch_config& c) {
     return (c.block == 64 || c.block == 128 || c.block == 192 || c.block == 256) && c.blocks_per_cu >= 1 &&
            c.blocks_per_cu <= 64 && c.items_per_dequeue >= 1 && c.items_per_dequeue <= 64 &&
-           c.samples_per_item >= 1 && c.samples_per_item <= 1024;
+           c.samples_per_item >= 1 && c.samples_per_item <= 1024 && (c.engine == 0 || c.engine == 1) &&
+           c.path_slots >= 4096 && c.path_slots <= (1 << 26);
+}
+
+bool same_launch(const rzk_batch_config& a, const rzk_batch_config& b) {
+    return a.block == b.block && a.blocks_per_cu == b.blocks_per_cu && a.engine == b.engine &&
+           a.path_slots == b.path_slots;
 }
 
 }  // namespace
@@ -77,7 +83,8 @@ bool valid_config(const rzk_batch_config& c) {
 struct rzk_batch {
     rzk_ctx* ctx = nullptr;
     const rzk_scene* scene = nullptr;
-    rzk_batch_config cfg{128, 4, 1, 4};
+    rzk_batch_config cfg{256, 8, 1, 1, 1, 1 << 20};
+    WfEngine* wf = nullptr;
     DevArray<JobGpu> jobs;
     DevArray<float> samples, out;
     DevArray<BatchHeader> hdr;
@@ -98,6 +105,8 @@ struct rzk_batch {
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
+        if (wf) wf_destroy(wf);
+        wf = nullptr;
     }
 
     // capture reset + persistent kernel + reduction for the current launch shape
@@ -164,7 +173,9 @@ static void batch_render(rzk_batch* b, const rzk_render_job* jobs, int32_t n, fl
     RZK_HIP(hipStreamSynchronize(ctx->stream));
     b->h_jobs.reserve((size_t)n);
     int64_t items = 0, pixels = 0, samples = 0;
-    const int S = b->cfg.samples_per_item;
+    const bool wave = b->cfg.engine == 1;
+    const int S = wave ? 1 : b->cfg.samples_per_item;
+    int max_bounce = 0;
     for (int32_t i = 0; i < n; ++i) {
         const rzk_render_job& j = jobs[i];
         JobGpu& J = b->h_jobs.p[i];
@@ -183,14 +194,20 @@ static void batch_render(rzk_batch* b, const rzk_render_job* jobs, int32_t n, fl
         J.item_first = items;
         J.pix_first = pixels;
         J.sample_off = samples;
-        items += (int64_t)J.pix_blocks * J.samp_blocks;
+        max_bounce = std::max(max_bounce, j.max_bounce);
+        // persistent engine: work items; wavefront engine: paths (sample-major)
+        items += wave ? npx * j.spp : (int64_t)J.pix_blocks * J.samp_blocks;
         pixels += npx;
         samples += npx * j.spp * 3;
     }
     b->jobs.reserve((size_t)n);
     b->samples.reserve((size_t)samples);
     b->out.reserve((size_t)pixels * 3);
-    if (!b->exec) b->capture();
+    if (wave) {
+        if (!b->wf) b->wf = wf_create(ctx, b->scene, b->cfg.path_slots, b->cfg.block, b->cfg.blocks_per_cu);
+    } else if (!b->exec) {
+        b->capture();
+    }
     BatchHeader& h = *b->h_hdr.p;
     h.jobs = b->jobs.p;
     h.n_jobs = n;
@@ -206,7 +223,14 @@ static void batch_render(rzk_batch* b, const rzk_render_job* jobs, int32_t n, fl
     RZK_HIP(hipMemcpyAsync(b->jobs.p, b->h_jobs.p, sizeof(JobGpu) * n, hipMemcpyHostToDevice, st));
     RZK_HIP(hipMemcpyAsync(b->hdr.p, b->h_hdr.p, sizeof(BatchHeader), hipMemcpyHostToDevice, st));
     RZK_HIP(hipEventRecord(b->ev0, st));
-    RZK_HIP(hipGraphLaunch(b->exec, st));
+    if (wave) {
+        if (out_rays) RZK_HIP(hipMemsetAsync(b->counters.p + 1, 0, 2 * sizeof(unsigned long long), st));
+        wf_run(b->wf, b->jobs.p, n, items, max_bounce, b->samples.p, out_rays ? b->counters.p + 1 : nullptr, nullptr);
+        launch_batch_reduce(st, b->hdr.p, std::max(1, ctx->num_cus) * 8);
+        RZK_HIP(hipGetLastError());
+    } else {
+        RZK_HIP(hipGraphLaunch(b->exec, st));
+    }
     RZK_HIP(hipEventRecord(b->ev1, st));
     if (out_rgb)
         RZK_HIP(hipMemcpyAsync(out_rgb, b->out.p, sizeof(float) * 3 * pixels, hipMemcpyDeviceToHost, st));
@@ -257,7 +281,7 @@ int rzk_batch_set_config(rzk_batch* b, const rzk_batch_config* cfg) {
     return batch_guard(b->ctx, [&] {
         RZK_CHECK(valid_config(*cfg), RZK_E_INVALID, "bad batch configuration");
         RZK_HIP(hipStreamSynchronize(b->ctx->stream));
-        bool shape = cfg->block != b->cfg.block || cfg->blocks_per_cu != b->cfg.blocks_per_cu;
+        bool shape = !same_launch(*cfg, b->cfg);
         b->cfg = *cfg;
         if (shape) b->drop_graph();
     });
@@ -282,7 +306,7 @@ int rzk_batch_autotune(rzk_batch* b, const rzk_render_job* jobs, int32_t n, rzk_
         RZK_CHECK(jobs && n > 0, RZK_E_INVALID, "empty batch");
         int32_t count = 0;
         auto time_cfg = [&](const rzk_batch_config& c) {
-            bool shape = c.block != b->cfg.block || c.blocks_per_cu != b->cfg.blocks_per_cu;
+            bool shape = !same_launch(c, b->cfg);
             b->cfg = c;
             if (shape) b->drop_graph();
             float ms = 0.f;
@@ -303,11 +327,18 @@ int rzk_batch_autotune(rzk_batch* b, const rzk_render_job* jobs, int32_t n, rzk_
                 if (ms < cur_ms) { cur_ms = ms; cur = c; }
             }
         };
-        stage(&rzk_batch_config::samples_per_item, {1, 2, 4, 8, 16});
-        stage(&rzk_batch_config::block, {64, 128, 256});
-        stage(&rzk_batch_config::blocks_per_cu, {2, 4, 8, 16});
-        stage(&rzk_batch_config::items_per_dequeue, {1, 2, 4, 8});
-        bool shape = cur.block != b->cfg.block || cur.blocks_per_cu != b->cfg.blocks_per_cu;
+        stage(&rzk_batch_config::engine, {0, 1});
+        if (cur.engine == 1) {
+            stage(&rzk_batch_config::block, {128, 256});
+            stage(&rzk_batch_config::blocks_per_cu, {4, 8, 16});
+            stage(&rzk_batch_config::path_slots, {1 << 18, 1 << 19, 1 << 20, 1 << 21});
+        } else {
+            stage(&rzk_batch_config::samples_per_item, {1, 2, 4, 8, 16});
+            stage(&rzk_batch_config::block, {64, 128, 256});
+            stage(&rzk_batch_config::blocks_per_cu, {2, 4, 8, 16});
+            stage(&rzk_batch_config::items_per_dequeue, {1, 2, 4, 8});
+        }
+        bool shape = !same_launch(cur, b->cfg);
         b->cfg = cur;
         if (shape) b->drop_graph();
         if (best) *best = cur;

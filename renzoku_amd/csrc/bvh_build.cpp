@@ -35,6 +35,213 @@ static int split_range(const uint32_t* code, int first, int last) {
     return lo;
 }
 
+
+// ---------------------------------------------------------------------------
+// The 4-wide tree the any-hit and path-tracing traversals walk: a binned-SAH
+// binary tree over the triangles' boxes (one triangle per leaf, exact union
+// boxes) collapsed four-wide by repeatedly opening the interior child with
+// the largest surface area.  Any-hit answers do not depend on the topology:
+// a triangle is tested iff its own box passes the slab test (every enclosing
+// box is a superset, and the rounded slab bounds are monotone in the box), so
+// the boolean equals the reference tree's.  Closest hits are the same
+// triangle, t, beta and gamma whenever the nearest triangle is unique.
+
+namespace {
+
+struct SahNode {
+    float lo[3], hi[3];
+    int left = -1, right = -1;       // children, or -1
+    int tri = -1;                    // leaf: position in the leaf-ordered triangle array
+};
+
+inline float half_area(const float lo[3], const float hi[3]) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+struct Bin {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int n = 0;
+    void add(const float* l, const float* h) {
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], l[a]); hi[a] = std::max(hi[a], h[a]); }
+        ++n;
+    }
+    void add(const Bin& b) {
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], b.lo[a]); hi[a] = std::max(hi[a], b.hi[a]); }
+        n += b.n;
+    }
+};
+
+}  // namespace
+
+void build_sah4(int ntri, const float* pos, const std::vector<v3>& bmin, const std::vector<v3>& bmax, BvhHost& out) {
+    constexpr int kBins = 32;
+    std::vector<float> tb(6 * (size_t)ntri), cen(3 * (size_t)ntri);
+    for (int k = 0; k < ntri; ++k) {
+        tb[6 * k + 0] = bmin[k].x; tb[6 * k + 1] = bmin[k].y; tb[6 * k + 2] = bmin[k].z;
+        tb[6 * k + 3] = bmax[k].x; tb[6 * k + 4] = bmax[k].y; tb[6 * k + 5] = bmax[k].z;
+        for (int a = 0; a < 3; ++a) cen[3 * k + a] = 0.5f * (tb[6 * k + a] + tb[6 * k + 3 + a]);
+    }
+    std::vector<int> idx(ntri);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::vector<SahNode> nodes;
+    nodes.reserve(2 * (size_t)ntri);
+    std::vector<int> leaf_tri;           // leaf order -> original triangle
+    leaf_tri.reserve(ntri);
+    struct Task { int node, b, e; };
+    std::vector<Task> work;
+    nodes.emplace_back();
+    work.push_back({0, 0, ntri});
+    while (!work.empty()) {
+        Task t = work.back();
+        work.pop_back();
+        SahNode& nd = nodes[t.node];
+        Bin all, cb;
+        for (int i = t.b; i < t.e; ++i) {
+            int k = idx[i];
+            all.add(&tb[6 * k], &tb[6 * k + 3]);
+            cb.add(&cen[3 * k], &cen[3 * k]);
+        }
+        for (int a = 0; a < 3; ++a) { nd.lo[a] = all.lo[a]; nd.hi[a] = all.hi[a]; }
+        int n = t.e - t.b;
+        if (n == 1) {
+            nd.tri = (int)leaf_tri.size();
+            leaf_tri.push_back(idx[t.b]);
+            continue;
+        }
+        // binned SAH over the three axes
+        float best = INFINITY;
+        int best_axis = -1, best_split = -1;
+        for (int a = 0; a < 3; ++a) {
+            float ext = cb.hi[a] - cb.lo[a];
+            if (!(ext > 0.f)) continue;
+            Bin bins[kBins];
+            float scale = kBins / ext;
+            for (int i = t.b; i < t.e; ++i) {
+                int k = idx[i];
+                int bi = std::min(kBins - 1, (int)((cen[3 * k + a] - cb.lo[a]) * scale));
+                bins[bi].add(&tb[6 * k], &tb[6 * k + 3]);
+            }
+            float right_area[kBins];
+            int right_n[kBins];
+            Bin acc;
+            for (int i = kBins - 1; i > 0; --i) {
+                acc.add(bins[i]);
+                right_area[i] = acc.n ? half_area(acc.lo, acc.hi) : 0.f;
+                right_n[i] = acc.n;
+            }
+            Bin left;
+            for (int i = 0; i < kBins - 1; ++i) {
+                left.add(bins[i]);
+                if (left.n == 0 || right_n[i + 1] == 0) continue;
+                float cost = left.n * half_area(left.lo, left.hi) + right_n[i + 1] * right_area[i + 1];
+                if (cost < best) { best = cost; best_axis = a; best_split = i; }
+            }
+        }
+        int mid;
+        if (best_axis < 0) {
+            mid = t.b + n / 2;                 // all centroids coincide: split the range in half
+        } else {
+            float scale = kBins / (cb.hi[best_axis] - cb.lo[best_axis]);
+            auto it = std::partition(idx.begin() + t.b, idx.begin() + t.e, [&](int k) {
+                int bi = std::min(kBins - 1, (int)((cen[3 * k + best_axis] - cb.lo[best_axis]) * scale));
+                return bi <= best_split;
+            });
+            mid = (int)(it - idx.begin());
+            if (mid == t.b || mid == t.e) mid = t.b + n / 2;
+        }
+        int l = (int)nodes.size();
+        nodes.emplace_back();
+        int r = (int)nodes.size();
+        nodes.emplace_back();
+        nodes[t.node].left = l;
+        nodes[t.node].right = r;
+        work.push_back({l, t.b, mid});
+        work.push_back({r, mid, t.e});
+    }
+
+    // triangles in leaf order
+    out.tris4.resize(ntri);
+    for (int i = 0; i < ntri; ++i) {
+        int ot = leaf_tri[i];
+        v3 p0 = ld3(pos + 9 * ot), p1 = ld3(pos + 9 * ot + 3), p2 = ld3(pos + 9 * ot + 6);
+        v3 e1 = p0 - p1, e2 = p0 - p2;
+        TriGpu tg;
+        float idf;
+        std::memcpy(&idf, &ot, 4);
+        tg.p0 = make_float4(p0.x, p0.y, p0.z, idf);
+        tg.e1 = make_float4(e1.x, e1.y, e1.z, 0.f);
+        tg.e2 = make_float4(e2.x, e2.y, e2.z, 0.f);
+        out.tris4[i] = tg;
+    }
+
+    // 4-wide collapse
+    auto area = [&](int b) { return half_area(nodes[b].lo, nodes[b].hi); };
+    auto leaf = [&](int b) { return nodes[b].left < 0; };
+    auto open4 = [&](int b, int c[4]) {
+        int cnt = 2;
+        c[0] = nodes[b].left;
+        c[1] = nodes[b].right;
+        while (cnt < 4) {
+            int bi = -1;
+            float ba = -1.f;
+            for (int i = 0; i < cnt; ++i)
+                if (!leaf(c[i]) && area(c[i]) > ba) { ba = area(c[i]); bi = i; }
+            if (bi < 0) break;
+            int x = c[bi];
+            for (int j = cnt; j > bi + 1; --j) c[j] = c[j - 1];
+            c[bi] = nodes[x].left;
+            c[bi + 1] = nodes[x].right;
+            ++cnt;
+        }
+        return cnt;
+    };
+    std::vector<int> idx4(nodes.size(), -1), order4;
+    std::vector<std::pair<int, int>> st4;
+    st4.push_back({0, 1});
+    int depth4 = 0;
+    while (!st4.empty()) {
+        auto [b, d] = st4.back();
+        st4.pop_back();
+        idx4[b] = (int)order4.size();
+        order4.push_back(b);
+        depth4 = std::max(depth4, d);
+        int c[4];
+        int cnt = open4(b, c);
+        for (int i = cnt - 1; i >= 0; --i)
+            if (!leaf(c[i])) st4.push_back({c[i], d + 1});
+    }
+    out.nodes4.resize(order4.size());
+    for (size_t j = 0; j < order4.size(); ++j) {
+        int c[4];
+        int cnt = open4(order4[j], c);
+        float lo[3][4], hi[3][4];
+        int rf[4];
+        for (int i = 0; i < 4; ++i) {
+            if (i < cnt) {
+                for (int a = 0; a < 3; ++a) { lo[a][i] = nodes[c[i]].lo[a]; hi[a][i] = nodes[c[i]].hi[a]; }
+                rf[i] = leaf(c[i]) ? ~nodes[c[i]].tri : idx4[c[i]];
+            } else {
+                for (int a = 0; a < 3; ++a) { lo[a][i] = 0.f; hi[a][i] = 0.f; }
+                rf[i] = kEmptyRef;
+            }
+        }
+        Bvh4NodeGpu nd;
+        nd.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        nd.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        nd.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        nd.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        nd.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        nd.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        nd.ref = make_int4(rf[0], rf[1], rf[2], rf[3]);
+        nd.pad = make_int4(0, 0, 0, 0);
+        out.nodes4[j] = nd;
+    }
+    out.root4_ref = 0;
+    out.depth4 = depth4;
+    out.stack4 = 3 * depth4 + 1;
+}
+
 void bvh_build_host(int ntri, const float* pos, BvhHost& out) {
     out = BvhHost();
     out.ntri = ntri;
@@ -129,6 +336,7 @@ void bvh_build_host(int ntri, const float* pos, BvhHost& out) {
         out.root4_ref = ~0;
         out.depth4 = 0;
         out.stack4 = 1;
+        out.tris4 = out.tris;
         return;
     }
     std::vector<int> newidx(nn, -1);
@@ -162,78 +370,5 @@ void bvh_build_host(int ntri, const float* pos, BvhHost& out) {
     }
     out.root_ref = 0;
 
-    // 4-wide collapse for any-hit traversal: a 4-wide node holds the up-to-4
-    // descendants reached by repeatedly opening the interior child with the
-    // largest surface area.  Every triangle keeps its own box as the box of
-    // the slot it sits in, so the set of triangles an any-hit query tests is
-    // exactly the binary tree's (a slab test passing for a box implies it
-    // passes for every enclosing box: the rounded slab bounds are monotone).
-    auto area = [&](int b) {
-        const float* B = &box[6 * b];
-        float dx = B[3] - B[0], dy = B[4] - B[1], dz = B[5] - B[2];
-        return dx * dy + dy * dz + dz * dx;
-    };
-    auto open4 = [&](int b, int c[4]) {
-        int n = 2;
-        c[0] = child[2 * b];
-        c[1] = child[2 * b + 1];
-        while (n < 4) {
-            int best = -1;
-            float ba = -1.f;
-            for (int i = 0; i < n; ++i)
-                if (child[2 * c[i]] >= 0 && area(c[i]) > ba) { ba = area(c[i]); best = i; }
-            if (best < 0) break;
-            int x = c[best];
-            for (int j = n; j > best + 1; --j) c[j] = c[j - 1];
-            c[best] = child[2 * x];
-            c[best + 1] = child[2 * x + 1];
-            ++n;
-        }
-        return n;
-    };
-    std::vector<int> idx4(nn, -1), order4;
-    std::vector<std::pair<int, int>> st4;
-    st4.push_back({0, 1});
-    int depth4 = 0;
-    while (!st4.empty()) {
-        auto [b, d] = st4.back();
-        st4.pop_back();
-        idx4[b] = (int)order4.size();
-        order4.push_back(b);
-        depth4 = std::max(depth4, d);
-        int c[4];
-        int n = open4(b, c);
-        for (int i = n - 1; i >= 0; --i)
-            if (child[2 * c[i]] >= 0) st4.push_back({c[i], d + 1});
-    }
-    out.nodes4.resize(order4.size());
-    for (size_t j = 0; j < order4.size(); ++j) {
-        int c[4];
-        int n = open4(order4[j], c);
-        float lo[3][4], hi[3][4];
-        int rf[4];
-        for (int i = 0; i < 4; ++i) {
-            if (i < n) {
-                const float* B = &box[6 * c[i]];
-                for (int a = 0; a < 3; ++a) { lo[a][i] = B[a]; hi[a][i] = B[3 + a]; }
-                rf[i] = child[2 * c[i]] < 0 ? ~range[2 * c[i]] : idx4[c[i]];
-            } else {
-                for (int a = 0; a < 3; ++a) { lo[a][i] = 0.f; hi[a][i] = 0.f; }
-                rf[i] = kEmptyRef;
-            }
-        }
-        Bvh4NodeGpu nd;
-        nd.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
-        nd.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
-        nd.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
-        nd.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
-        nd.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
-        nd.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
-        nd.ref = make_int4(rf[0], rf[1], rf[2], rf[3]);
-        nd.pad = make_int4(0, 0, 0, 0);
-        out.nodes4[j] = nd;
-    }
-    out.root4_ref = 0;
-    out.depth4 = depth4;
-    out.stack4 = 3 * depth4 + 1;
+    build_sah4(ntri, pos, bmin, bmax, out);
 }

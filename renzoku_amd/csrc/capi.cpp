@@ -79,6 +79,7 @@ SceneView scene_view(const rzk_scene* s) {
     v.nodes = s->d_nodes;
     v.nodes4 = s->d_nodes4;
     v.tris = s->d_tris;
+    v.tris4 = s->d_tris4;
     v.tri_n = s->d_tri_n;
     v.tri_tan = s->d_tri_tan;
     v.surf = s->d_surf;
@@ -229,6 +230,7 @@ int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
         up((void**)&s->d_nodes, s->bvh.nodes.data(), sizeof(BvhNodeGpu) * s->bvh.nodes.size());
         up((void**)&s->d_nodes4, s->bvh.nodes4.data(), sizeof(Bvh4NodeGpu) * s->bvh.nodes4.size());
         up((void**)&s->d_tris, s->bvh.tris.data(), sizeof(TriGpu) * s->bvh.tris.size());
+        up((void**)&s->d_tris4, s->bvh.tris4.data(), sizeof(TriGpu) * s->bvh.tris4.size());
         up((void**)&s->d_tri_n, tn.data(), sizeof(float4) * n);
         up((void**)&s->d_tri_tan, tt.data(), sizeof(float4) * n);
         up((void**)&s->d_surf, surf.data(), sizeof(int4) * s->nsurf);
@@ -296,7 +298,7 @@ int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
 int rzk_scene_destroy(rzk_scene* s) {
     if (!s) return RZK_OK;
     (void)hipSetDevice(s->ctx->device);
-    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_tris, s->d_tri_n, s->d_tri_tan, s->d_surf, s->d_mat,
+    void* ptrs[] = {s->d_nodes, s->d_nodes4, s->d_tris, s->d_tris4, s->d_tri_n, s->d_tri_tan, s->d_surf, s->d_mat,
                     s->d_ltris, s->d_lights};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -312,7 +314,7 @@ int rzk_scene_get_info(const rzk_scene* s, rzk_scene_info* out) {
     for (int i = 0; i < 6; ++i) out->box[i] = s->bvh.box[i];
     out->device_bytes = (int64_t)sizeof(BvhNodeGpu) * s->bvh.num_interior +
                         (int64_t)sizeof(Bvh4NodeGpu) * (int64_t)s->bvh.nodes4.size() +
-                        (int64_t)sizeof(TriGpu) * s->bvh.ntri + (int64_t)2 * sizeof(float4) * s->bvh.ntri;
+                        (int64_t)2 * sizeof(TriGpu) * s->bvh.ntri + (int64_t)2 * sizeof(float4) * s->bvh.ntri;
     return RZK_OK;
 }
 

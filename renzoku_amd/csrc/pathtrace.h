@@ -74,6 +74,34 @@ struct ServeHeader {
     int items_per_dequeue, pad;
 };
 
+// Wavefront engine: path state in structure-of-arrays slots ([3][P] planar
+// vectors), four compacted queues per iteration parity.
+struct WfView {
+    int64_t P;
+    int *job, *pix, *smp, *bounce, *flags;
+    float *ro, *rd, *T, *Tc, *L;        // [3][P]
+    float4* hit;                        // current ray: t, beta, gamma, tri bits
+    float *np, *nq, *lo1, *nwi, *nd, *nbr;   // next-event data, [3][P]
+    float *ndpdf, *ncos;
+    int* nk;
+    float4* nhit;                       // next-event BSDF ray hit
+    int* blocked;
+    int* q[2][4];                       // [parity][extend, nee ray, shadow, nee resolve]
+    int* counts;                        // [parity][4]
+    int* freeq[2];                      // slots freed in an iteration, taken by the next one
+    int* free_counts;                   // [parity]
+    unsigned long long* ctrl;           // {paths handed out, active slots}
+};
+
+// the job table of the run in flight (read by every kernel of the captured iteration)
+struct WfJobs {
+    const JobGpu* jobs;                 // item_first = first path index (paths sample-major per job)
+    int64_t n_jobs, total_paths;
+    float* samples;                     // per-sample radiance, [pixel][spp][3] at each job's sample_off
+    unsigned long long* work;           // {closest rays, shadow rays} or null
+    unsigned int* job_done;             // finished paths per job or null
+};
+
 }  // namespace rzk
 
 // A guide: the per-slice guiding distributions of one camera, borrowed from a
@@ -137,6 +165,23 @@ void launch_batch(hipStream_t stream, const SceneView& sv, const rzk::BatchHeade
                   int reduce_grid);
 // the two kernels of a served round (captured into a graph by the caller)
 void launch_serve_round(hipStream_t stream, const SceneView& sv, const rzk::ServeHeader* d_hdr, int grid, int block);
+// one iteration of the wavefront engine (queues of parity `in` consumed,
+// parity in ^ 1 produced): extend + next-event rays, shadow rays, next-event
+// resolution, bounce shading, path regeneration
+void launch_wf_iteration(hipStream_t stream, const SceneView& sv, const rzk::WfView& v, const rzk::WfJobs* d_jh,
+                         int in, int grid, int block);
+// the wavefront engine (wavefront.hip): `slots` paths in flight, kernels of
+// `block` lanes on compute units x blocks_per_cu workgroups
+struct WfEngine;
+WfEngine* wf_create(rzk_ctx* ctx, const rzk_scene* scene, int64_t slots, int block, int blocks_per_cu);
+void wf_destroy(WfEngine* e);
+int64_t wf_slots(const WfEngine* e);
+// run every path of a device job table (item_first = first path index) to
+// completion on ctx's stream; max_bounce bounds the iterations; returns them
+int64_t wf_run(WfEngine* e, const rzk::JobGpu* d_jobs, int64_t n_jobs, int64_t total_paths, int max_bounce,
+               float* d_samples, unsigned long long* d_work, unsigned int* d_job_done);
+// per-pixel means of finished per-sample radiance (k_batch_reduce)
+void launch_batch_reduce(hipStream_t stream, const rzk::BatchHeader* d_hdr, int grid);
 // PtParams of a job over a guide
 rzk::PtParams job_params(const rzk_guide& g, const rzk_render_job& j);
 void check_job(const rzk_guide* g, const rzk_render_job& j);

@@ -127,6 +127,17 @@ __device__ __forceinline__ float guide_pdf(const GuideView& gv, int slice, int r
     return map_pdf(gv, slice, res, world_to_local(b, wi));
 }
 
+// first i in [0, n) with u < cdf[i], -1 when none (cdf non-decreasing, or all NaN)
+__device__ __forceinline__ int first_above(const float* cdf, int n, float u) {
+    int lo = 0, hi = n;                 // answer in [lo, hi]; hi == n means none
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (u < cdf[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo < n ? lo : -1;
+}
+
 // LightSlice::sample -> multi_slice_sample: sample_map of the slice (receiver
 // frame, same_local_wi) or the uniform-hemisphere fallback; pdf re-evaluated
 // by multi_slice_pdf; returns the receiver BRDF for wi
@@ -138,16 +149,13 @@ __device__ v3 guide_sample(const GuideView& gv, int slice, int res, const float*
         wi = unit(spherical_to_world(b, theta, phi));
     } else {
         // DiscretePdf2D::sample (pdf.cc:171-186): row by the row cdf, column by that row's cdf
+        // the first index with u < cdf[i]; the predicate is monotone in i
+        // (cumulative sums, the last row entry forced to 1 > u), so a binary
+        // search finds the index the reference's linear scan finds
         float ur = u24(u[0]), uc = u24(u[1]);
         const float* rcdf = gv.row_cdf + (int64_t)slice * res;
-        int row = -1, col = -1;
-        for (int i = 0; i < res; ++i)
-            if (ur < rcdf[i]) { row = i; break; }
-        if (row >= 0) {
-            const float* ccdf = gv.col_cdf + ((int64_t)slice * res + row) * res;
-            for (int j = 0; j < res; ++j)
-                if (uc < ccdf[j]) { col = j; break; }
-        }
+        int row = first_above(rcdf, res, ur), col = -1;
+        if (row >= 0) col = first_above(gv.col_cdf + ((int64_t)slice * res + row) * res, res, uc);
         if (row < 0 || col < 0) { wi = mk(0.f, 0.f, 0.f); pdf = 0.f; return mk(0.f, 0.f, 0.f); }
         float sx = ((float)col + u24(u[2])) / (float)res;
         float sy = ((float)row + u24(u[3])) / (float)res;
@@ -427,9 +435,395 @@ __global__ __launch_bounds__(256) void k_serve_pt(SceneView sv, const ServeHeade
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wavefront engine.  A path lives in a slot; each iteration runs
+//   closest (extend + next-event BSDF rays) -> any (shadow segments) ->
+//   shade2 (next-event resolution: L += T * NEE) -> shade1 (the bounce at the
+//   extended hit: miss/light tests, T <- T', receiver, next-event setup,
+//   continuation sampling) -> generate (free slots take new paths).
+// The arithmetic is trace_path's, split at its traversal calls, so every path
+// gives the same bits; the kernels are small enough to run at high occupancy.
+// Queue appends are wave-aggregated: one atomic per wave, ranks by ballot
+// prefix.
+
+enum : int { kWfActive = 1, kWfLo1 = 2, kWfLo2 = 4, kWfFinishAfterNee = 8 };
+enum : int { kQExt = 0, kQNee = 1, kQAny = 2, kQRes = 3 };
+
+__device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63); }
+
+// every lane of the wave calls this; returns the slot of `pred` lanes in the queue
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return -1;
+    int lane = wave_lane();
+    int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    return pred ? base + rank : -1;
+}
+
+__device__ __forceinline__ void wave_sub_active(unsigned long long* active, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    if (mask != 0ull && wave_lane() == __ffsll((long long)mask) - 1)
+        atomicAdd(active, (unsigned long long)(-(long long)__popcll(mask)));
+}
+
+__device__ __forceinline__ v3 ld_p(const float* a, int64_t P, int64_t i) { return mk(a[i], a[P + i], a[2 * P + i]); }
+__device__ __forceinline__ void st_p(float* a, int64_t P, int64_t i, v3 v) {
+    a[i] = v.x; a[P + i] = v.y; a[2 * P + i] = v.z;
+}
+
+// a finished path: its radiance into the per-sample buffer, the slot freed
+__device__ __forceinline__ void wf_finish(const WfView& v, const WfJobs& jh, int64_t slot) {
+    const JobGpu& J = jh.jobs[v.job[slot]];
+    int64_t p = v.pix[slot];
+    int smp = v.smp[slot];
+    float* dst = jh.samples + J.sample_off + (p * J.pp.spp + smp) * 3;
+    v3 L = ld_p(v.L, v.P, slot);
+    dst[0] = L.x; dst[1] = L.y; dst[2] = L.z;
+    v.flags[slot] = 0;
+    if (jh.job_done) atomicAdd(jh.job_done + v.job[slot], 1u);
+}
+
+__global__ void k_wf_begin(WfView v, int out) {
+    if (threadIdx.x < 4) v.counts[out * 4 + threadIdx.x] = 0;
+    if (threadIdx.x == 4) v.free_counts[out] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_wf_closest(SceneView sv, WfView v, const WfJobs* __restrict__ jhp, int in) {
+    extern __shared__ int s_stack[];
+    const WfJobs jh = *jhp;
+    AnyStack astk = any_stack(sv, s_stack);
+    TravStats st;
+    unsigned long long nclosest = 0;
+    const int n1 = v.counts[in * 4 + kQExt], n2 = v.counts[in * 4 + kQNee];
+    const int64_t n = (int64_t)n1 + n2, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        bool nee = i >= n1;
+        int64_t slot = nee ? v.q[in][kQNee][i - n1] : v.q[in][kQExt][i];
+        float tmin = jh.jobs[v.job[slot]].pp.tmin;
+        Ray r = nee ? ray_from_od(ld_p(v.np, v.P, slot), ld_p(v.nd, v.P, slot))
+                    : ray_from_od(ld_p(v.ro, v.P, slot), ld_p(v.rd, v.P, slot));
+        float t = 0.f, b = 0.f, g = 0.f;
+        nclosest++;
+        int tri = closest_hit4(sv, r, tmin, 1e6f, astk, t, b, g, st);
+        float4 h = make_float4(t, b, g, __int_as_float(tri));
+        if (nee) v.nhit[slot] = h;
+        else v.hit[slot] = h;
+    }
+    if (jh.work) add_ray_counts(jh.work, nclosest, 0ull);
+}
+
+__global__ __launch_bounds__(256) void k_wf_any(SceneView sv, WfView v, const WfJobs* __restrict__ jhp, int in) {
+    extern __shared__ int s_stack[];
+    const WfJobs jh = *jhp;
+    AnyStack astk = any_stack(sv, s_stack);
+    TravStats st;
+    unsigned long long nshadow = 0;
+    const int64_t n = v.counts[in * 4 + kQAny], stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        int64_t slot = v.q[in][kQAny][i];
+        float tmin = jh.jobs[v.job[slot]].pp.tmin;
+        nshadow++;
+        v.blocked[slot] = segment_blocked(sv, ld_p(v.np, v.P, slot), ld_p(v.nq, v.P, slot), tmin, astk, st) ? 1 : 0;
+    }
+    if (jh.work) add_ray_counts(jh.work, 0ull, nshadow);
+}
+
+// next-event resolution (the tail of next_event after its two traversals)
+__global__ __launch_bounds__(256) void k_wf_shade2(SceneView sv, WfView v, const WfJobs* __restrict__ jhp, int in) {
+    const WfJobs jh = *jhp;
+    const int64_t n = v.counts[in * 4 + kQRes], stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        int64_t i = i0 + threadIdx.x;
+        bool fin = false;
+        int64_t slot = 0;
+        if (i < n) {
+            slot = v.q[in][kQRes][i];
+            int fl = v.flags[slot];
+            int k = v.nk[slot];
+            v3 nee = mk(0.f, 0.f, 0.f);
+            if (k >= 0) {
+                LightGpu Lk = sv.lights[k];
+                v3 Lo = mk(0.f, 0.f, 0.f);
+                if ((fl & kWfLo1) && !v.blocked[slot]) Lo = Lo + ld_p(v.lo1, v.P, slot);
+                if (fl & kWfLo2) {
+                    float4 h = v.nhit[slot];
+                    int tri = __float_as_int(h.w);
+                    if (tri >= 0) {
+                        float4 nf = sv.tri_n[tri];
+                        int4 sf = sv.surf[__float_as_int(nf.w)];
+                        if (sf.y == k) {
+                            v3 p = ld_p(v.np, v.P, slot), wi = ld_p(v.nwi, v.P, slot);
+                            float t = h.x, tb = h.y, tg = h.z;
+                            v3 qh = p + t * wi;
+                            v3 nn = mk(nf.x, nf.y, nf.z);
+                            v3 qhn = unit((nn + (nn - nn) * tb) + (nn - nn) * tg);
+                            if (sf.z & RZK_SURF_FLIPPED) qhn = -qhn;
+                            v3 Le = light_query(Lk, qhn, -wi);
+                            float pa = 0.f;
+                            for (int qq = 0; qq < Lk.count; ++qq) {
+                                const LightTri& lt = sv.ltris[Lk.first + qq];
+                                if (lt.tri == tri) { pa = lt.ratio * (1.0f / lt.n.w); break; }
+                            }
+                            v3 pq = qh - p;
+                            float lpdf = pa * dot(pq, pq) / fabsf(dot(unit(p - qh), qhn));
+                            float dpdf = v.ndpdf[slot];
+                            float w2 = balance(dpdf, lpdf);
+                            v3 brdf = ld_p(v.nbr, v.P, slot);
+                            float cosv = v.ncos[slot];
+                            Lo = Lo + (((w2 * cosv) / dpdf) * Le) * brdf;
+                        }
+                    }
+                }
+                nee = mk(Lo.x / Lk.sel_pdf, Lo.y / Lk.sel_pdf, Lo.z / Lk.sel_pdf);
+            }
+            v3 T = ld_p(v.T, v.P, slot);
+            st_p(v.L, v.P, slot, ld_p(v.L, v.P, slot) + T * nee);
+            if (fl & kWfFinishAfterNee) {
+                wf_finish(v, jh, slot);
+                fin = true;
+            } else {
+                v.flags[slot] = fl & ~(kWfLo1 | kWfLo2);
+            }
+        }
+        int at = wave_append(v.free_counts + (in ^ 1), fin);
+        if (fin) v.freeq[in ^ 1][at] = (int)slot;
+        wave_sub_active(v.ctrl + 1, fin);
+    }
+}
+
+// the bounce at the extended hit (the body of trace_path's loop)
+__global__ __launch_bounds__(256) void k_wf_shade1(SceneView sv, WfView v, const WfJobs* __restrict__ jhp, int in) {
+    const WfJobs jh = *jhp;
+    const int out = in ^ 1;
+    const int64_t n = v.counts[in * 4 + kQExt], stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        int64_t i = i0 + threadIdx.x;
+        bool e_ext = false, e_nee = false, e_any = false, e_res = false, fin = false;
+        int64_t slot = 0;
+        if (i < n) {
+            slot = v.q[in][kQExt][i];
+            const JobGpu& J = jh.jobs[v.job[slot]];
+            const PtParams pp = J.pp;
+            const int bounce = v.bounce[slot];
+            float4 h = v.hit[slot];
+            int tri = __float_as_int(h.w);
+            bool done = tri < 0;
+            if (!done && bounce > 1) {
+                int4 sf = sv.surf[__float_as_int(sv.tri_n[tri].w)];
+                if (sf.y >= 0) done = true;                 // light: counted by the light sample
+                else st_p(v.T, v.P, slot, ld_p(v.Tc, v.P, slot));
+            }
+            if (!done && bounce > pp.max_bounce) done = true;
+            int fl = kWfActive;
+            if (!done) {
+                Ray ray = ray_from_od(ld_p(v.ro, v.P, slot), ld_p(v.rd, v.P, slot));
+                float rc[RZK_RECV_FLOATS];
+                int m, lidx;
+                fill_receiver(sv, ray, h.x, h.y, h.z, tri, rc, m, lidx);
+                v3 T = ld_p(v.T, v.P, slot);
+                if (lidx >= 0) {
+                    if (bounce == 1 && pp.direct)
+                        st_p(v.L, v.P, slot, ld_p(v.L, v.P, slot) + T * light_query(sv.lights[lidx], ld3(rc + 6),
+                                                                                   ld3(rc + 9)));
+                    done = true;
+                } else if (m < 0) {
+                    done = true;
+                } else {
+                    int64_t p = v.pix[slot];
+                    int x = pp.x0 + (int)(p % pp.w), y = pp.y0 + (int)(p / pp.w);
+                    uint32_t pixel = (uint32_t)(y * pp.width + x);
+                    PathRng rng = path_rng(pp.seed, pixel, (uint32_t)v.smp[slot], (uint32_t)bounce);
+                    bool brdf_sampling = u24(rng.u[0]) < 0.5f;
+                    const float* mat = sv.mat + RZK_MAT_FLOATS * m;
+                    Onb b; b.u = ld3(rc + 12); b.v = ld3(rc + 15); b.n = ld3(rc + 18);
+                    v3 sn = ld3(rc + 6), wo = ld3(rc + 9), p3 = ld3(rc);
+                    bool nee = bounce > 1 || pp.direct;
+                    if (nee) {
+                        e_res = true;
+                        // next_event up to its traversals
+                        float ul = u24(rng.u[1]);
+                        int k = -1;
+                        for (int li = 0; li < sv.nlights; ++li)
+                            if (ul < sv.lights[li].sel_cdf) { k = li; break; }
+                        if (k >= 0 && sv.lights[k].count <= 0) k = -1;
+                        v.nk[slot] = k;
+                        if (k >= 0) {
+                            LightGpu Lk = sv.lights[k];
+                            v3 pn = sn;
+                            st_p(v.np, v.P, slot, p3);
+                            float ut = u24(rng.u[2]);
+                            int ti = Lk.first + Lk.count - 1;
+                            for (int qi = 0; qi < Lk.count; ++qi)
+                                if (ut <= sv.ltris[Lk.first + qi].cum) { ti = Lk.first + qi; break; }
+                            LightTri Tl = sv.ltris[ti];
+                            float tmp = sqrtf(1.0f - u24(rng.u[3]));
+                            float bb = 1.0f - tmp;
+                            float cc = u24(rng.u[4]) * tmp;
+                            float aa = 1.0f - bb - cc;
+                            v3 q = (aa * mk(Tl.p0.x, Tl.p0.y, Tl.p0.z) + bb * mk(Tl.p1.x, Tl.p1.y, Tl.p1.z)) +
+                                   cc * mk(Tl.p2.x, Tl.p2.y, Tl.p2.z);
+                            v3 qn = mk(Tl.n.x, Tl.n.y, Tl.n.z);
+                            float lpdf = Tl.ratio * (1.0f / Tl.n.w);
+                            if (lpdf > 0.0f) {
+                                e_any = true;
+                                st_p(v.nq, v.P, slot, q);
+                                v3 wi = unit(q - p3);
+                                v3 Le = light_query(Lk, qn, -wi);
+                                float G = form_factor_abs(p3, pn, q, qn);
+                                if (G > 0.0f) {
+                                    v3 brdf = bsdf_eval(mat, b, wo, wi);
+                                    float dpdf = bsdf_pdf(mat, b, wo, wi);
+                                    v3 pq = q - p3;
+                                    float w1 = balance(lpdf * dot(pq, pq) / fabsf(dot(unit(p3 - q), qn)), dpdf);
+                                    st_p(v.lo1, v.P, slot, (((w1 * G) / lpdf) * Le) * brdf);
+                                    fl |= kWfLo1;
+                                }
+                            }
+                            v3 wi2;
+                            float dpdf2;
+                            bsdf_sample(mat, b, wo, &rng.u[5], wi2, dpdf2);
+                            if (dpdf2 > 0.0f) {
+                                e_nee = true;
+                                Ray r2 = ray_ctor(p3, wi2);
+                                st_p(v.nd, v.P, slot, r2.d);
+                                st_p(v.nwi, v.P, slot, wi2);
+                                v.ndpdf[slot] = dpdf2;
+                                st_p(v.nbr, v.P, slot, bsdf_eval(mat, b, wo, wi2));
+                                v.ncos[slot] = fabsf(dot(wi2, pn));
+                                fl |= kWfLo2;
+                            }
+                        }
+                    }
+                    bool cont = false;
+                    if (bounce < pp.max_bounce) {
+                        int slice = pixel_slice(pp, J.gv, x, y);
+                        int gslice = bounce == 1 ? slice : -1;
+                        v3 wi, brdf;
+                        float pdf;
+                        if (brdf_sampling) brdf = bsdf_sample(mat, b, wo, &rng.u[8], wi, pdf);
+                        else brdf = guide_sample(J.gv, gslice, pp.res, mat, b, sn, wo, &rng.u[8], wi, pdf);
+                        if (pdf > 0.0f) {
+                            cont = true;
+                            Ray nr = ray_ctor(p3, wi);
+                            st_p(v.ro, v.P, slot, nr.o);
+                            st_p(v.rd, v.P, slot, nr.d);
+                            float pdf_other = brdf_sampling ? guide_pdf(J.gv, gslice, pp.res, b, sn, wi)
+                                                            : bsdf_pdf(mat, b, wo, wi);
+                            float weight = pdf / (pdf + pdf_other);
+                            float cosv = fabsf(dot(wi, sn));
+                            float s = ((weight * cosv) / pdf) * 2.0f;
+                            st_p(v.Tc, v.P, slot, (T * brdf) * s);
+                            v.bounce[slot] = bounce + 1;
+                            e_ext = true;
+                        }
+                    }
+                    if (!cont) {
+                        if (nee) fl |= kWfFinishAfterNee;
+                        else done = true;
+                    }
+                }
+            }
+            if (done) {
+                wf_finish(v, jh, slot);
+                fin = true;
+            } else {
+                v.flags[slot] = fl;
+            }
+        }
+        int at;
+        if ((at = wave_append(v.counts + out * 4 + kQExt, e_ext)) >= 0) v.q[out][kQExt][at] = (int)slot;
+        if ((at = wave_append(v.counts + out * 4 + kQNee, e_nee)) >= 0) v.q[out][kQNee][at] = (int)slot;
+        if ((at = wave_append(v.counts + out * 4 + kQAny, e_any)) >= 0) v.q[out][kQAny][at] = (int)slot;
+        if ((at = wave_append(v.counts + out * 4 + kQRes, e_res)) >= 0) v.q[out][kQRes][at] = (int)slot;
+        if ((at = wave_append(v.free_counts + out, fin)) >= 0) v.freeq[out][at] = (int)slot;
+        wave_sub_active(v.ctrl + 1, fin);
+    }
+}
+
+// slots freed in the previous iteration take the next paths (sample-major
+// within a job); primary rays go to the extend queue.  Slots left without a
+// path stay free for good: the paths are exhausted.
+__global__ __launch_bounds__(256) void k_wf_generate(WfView v, const WfJobs* __restrict__ jhp, int in) {
+    const WfJobs jh = *jhp;
+    const int out = in ^ 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int last_job = (int)jh.n_jobs - 1;
+    const int64_t nfree = v.free_counts[in];
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < nfree; i0 += stride) {
+        int64_t fi = i0 + threadIdx.x;
+        bool free_ = fi < nfree;
+        int64_t slot = free_ ? v.freeq[in][fi] : 0;
+        unsigned long long mask = __ballot(free_);
+        if (mask == 0ull) continue;
+        int lane = wave_lane();
+        int leader = __ffsll((long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(v.ctrl, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader);
+        unsigned long long g = base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+        bool take = free_ && g < (unsigned long long)jh.total_paths && last_job >= 0;
+        if (take) {
+            int lo = 0, hi = last_job;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if ((unsigned long long)jh.jobs[mid].item_first <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            const JobGpu& J = jh.jobs[lo];
+            const PtParams pp = J.pp;
+            int64_t npx = (int64_t)pp.w * pp.h;
+            int64_t local = (int64_t)(g - (unsigned long long)J.item_first);
+            int smp = (int)(local / npx);
+            int64_t p = local - (int64_t)smp * npx;
+            int x = pp.x0 + (int)(p % pp.w), y = pp.y0 + (int)(p / pp.w);
+            uint32_t pixel = (uint32_t)(y * pp.width + x);
+            PathRng r0 = path_rng(pp.seed, pixel, (uint32_t)smp, 0u);
+            float jx = u24(r0.u[0]), jy = u24(r0.u[1]);
+            int strata = J.strata;
+            if (strata > 0) {
+                jx = ((float)(smp % strata) + jx) / (float)strata;
+                jy = ((float)(smp / strata) + jy) / (float)strata;
+            }
+            Ray ray = camera_sample_ray(J.cam.v, (float)x + jx, (float)y + jy);
+            v.job[slot] = lo;
+            v.pix[slot] = (int)p;
+            v.smp[slot] = smp;
+            v.bounce[slot] = 1;
+            v.flags[slot] = kWfActive;
+            st_p(v.ro, v.P, slot, ray.o);
+            st_p(v.rd, v.P, slot, ray.d);
+            st_p(v.T, v.P, slot, mk(1.f, 1.f, 1.f));
+            st_p(v.L, v.P, slot, mk(0.f, 0.f, 0.f));
+        }
+        int at = wave_append(v.counts + out * 4 + kQExt, take);
+        if (take) v.q[out][kQExt][at] = (int)slot;
+        unsigned long long tm = __ballot(take);
+        if (tm != 0ull && lane == __ffsll((long long)tm) - 1) atomicAdd(v.ctrl + 1, (unsigned long long)__popcll(tm));
+    }
+}
+
 }  // namespace rzk
 
 using namespace rzk;
+
+void launch_wf_iteration(hipStream_t stream, const SceneView& sv, const WfView& v, const WfJobs* d_jh, int in,
+                         int grid, int block) {
+    size_t lds = sizeof(int) * (size_t)block * (size_t)kLdsStack;
+    hipLaunchKernelGGL(k_wf_begin, dim3(1), dim3(64), 0, stream, v, in ^ 1);
+    hipLaunchKernelGGL(k_wf_closest, dim3(grid), dim3(block), lds, stream, sv, v, d_jh, in);
+    hipLaunchKernelGGL(k_wf_any, dim3(grid), dim3(block), lds, stream, sv, v, d_jh, in);
+    hipLaunchKernelGGL(k_wf_shade2, dim3(grid), dim3(block), 0, stream, sv, v, d_jh, in);
+    hipLaunchKernelGGL(k_wf_shade1, dim3(grid), dim3(block), 0, stream, sv, v, d_jh, in);
+    hipLaunchKernelGGL(k_wf_generate, dim3(grid), dim3(block), 0, stream, v, d_jh, in);
+}
+
+void launch_batch_reduce(hipStream_t stream, const BatchHeader* d_hdr, int grid) {
+    hipLaunchKernelGGL(k_batch_reduce, dim3(grid), dim3(256), 0, stream, d_hdr);
+}
 
 void launch_serve_round(hipStream_t stream, const SceneView& sv, const ServeHeader* d_hdr, int grid, int block) {
     hipLaunchKernelGGL(k_serve_reset, dim3(1), dim3(64), 0, stream, d_hdr);

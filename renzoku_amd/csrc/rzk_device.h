@@ -128,7 +128,7 @@ __device__ __forceinline__ bool any_hit4(const SceneView& sv, const Ray& r, floa
     int node = sv.root4_ref;
     float t, b, g;
     if (node < 0) {
-        const TriGpu* tp = sv.tris + (~node);
+        const TriGpu* tp = sv.tris4 + (~node);
         float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
         st.tris++;
         return tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g);
@@ -150,7 +150,7 @@ __device__ __forceinline__ bool any_hit4(const SceneView& sv, const Ray& r, floa
             if (ref == kEmptyRef) break;
             if (!box_hit(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax)) continue;
             if (ref < 0) {
-                const TriGpu* tp = sv.tris + (~ref);
+                const TriGpu* tp = sv.tris4 + (~ref);
                 float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
                 st.tris++;
                 if (tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g))
@@ -189,7 +189,7 @@ __device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, f
     int node = sv.root4_ref;
     float t, b, g;
     if (node < 0) {
-        const TriGpu* tp = sv.tris + (~node);
+        const TriGpu* tp = sv.tris4 + (~node);
         float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
         st.tris++;
         if (tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g)) {
@@ -217,7 +217,7 @@ __device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, f
             float te;
             if (!box_enter(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax, te)) continue;
             if (ref < 0) {
-                const TriGpu* tp = sv.tris + (~ref);
+                const TriGpu* tp = sv.tris4 + (~ref);
                 float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
                 st.tris++;
                 if (tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b,
@@ -275,7 +275,7 @@ __device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r,
     int node = sv.root4_ref;
     if (node < 0) {
         if (live) {
-            const TriGpu* tp = sv.tris + (~node);
+            const TriGpu* tp = sv.tris4 + (~node);
             float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
             st.tris++;
             hit = tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g);
@@ -301,7 +301,7 @@ __device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r,
             bool hc = live && box_hit(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax);
             if (__ballot(hc) == 0ull) continue;
             if (ref < 0) {
-                const kconst float* tf = (const kconst float*)(sv.tris + (~ref));
+                const kconst float* tf = (const kconst float*)(sv.tris4 + (~ref));
                 v3 p0 = mk(tf[0], tf[1], tf[2]), e1 = mk(tf[4], tf[5], tf[6]), e2 = mk(tf[8], tf[9], tf[10]);
                 if (hc) {
                     st.tris++;

@@ -44,7 +44,8 @@ struct BvhHost {
     int root4_ref = 0;
     int depth4 = 0;           // 4-wide nodes on the deepest path
     int stack4 = 0;           // bound on any-hit stack entries (3 per 4-wide level)
-    std::vector<TriGpu> tris;
+    std::vector<TriGpu> tris;            // reference-tree leaf (Morton) order
+    std::vector<TriGpu> tris4;           // 4-wide tree leaf order
     std::vector<int> sorted_tri;         // sorted position -> original triangle
     std::vector<uint32_t> sorted_morton;
     // reference-order (BFS) view for parity checks
@@ -73,6 +74,7 @@ struct rzk_scene {
     BvhNodeGpu* d_nodes = nullptr;
     Bvh4NodeGpu* d_nodes4 = nullptr;
     TriGpu* d_tris = nullptr;
+    TriGpu* d_tris4 = nullptr;
     float4* d_tri_n = nullptr;        // per original triangle: (n.xyz, surface id bits)
     float4* d_tri_tan = nullptr;      // per original triangle: (tangent.xyz, 0)
     int4* d_surf = nullptr;           // per surface: (material, light, flags, 0)
@@ -101,7 +103,8 @@ struct rzk_vpls {
 struct SceneView {
     const BvhNodeGpu* nodes;
     const Bvh4NodeGpu* nodes4;
-    const TriGpu* tris;
+    const TriGpu* tris;               // leaves of the reference tree (nodes)
+    const TriGpu* tris4;              // leaves of the 4-wide tree (nodes4)
     const float4* tri_n;
     const float4* tri_tan;
     const int4* surf;
