@@ -312,7 +312,7 @@ def run_served(args):
     g = rz.Guide.create(ctx, scene, clusters, reqs[0][0], gparams)
     g.render(spp=1, max_bounce=5, window=(0, 0, 64, 64))
     del g
-    srv = rz.Server(0, scene, clusters, memory_budget=args.budget_gb << 30, round_items=args.round_items,
+    srv = rz.Server(0, scene, clusters, memory_budget=args.budget_gb << 30, step_iterations=args.step_iterations,
                     guide=gparams)
     srv.wait(srv.submit(reqs[0][0], spp=1, max_bounce=1))
     served = [None] * len(reqs)
@@ -369,7 +369,7 @@ def run_served(args):
                                   "1-32 spp, depth 5, random cameras, 2M tris, 200k VPLs, 1000 resident clusters",
                       "triangles": int(sc.ntri), "vpls": int(len(vmat)), "requests": len(reqs),
                       "paths": int(paths), "rate_per_s": args.rate, "memory_budget_gb": args.budget_gb,
-                      "round_items": args.round_items, "parallelism": "single GPU"},
+                      "step_iterations": args.step_iterations, "parallelism": "single GPU"},
            "latency_ms": {"p50": pct(s_lat, 50), "p99": pct(s_lat, 99), "max": pct(s_lat, 100)},
            "plain_requests_per_s": round(len(reqs) / plain_span, 3),
            "plain_latency_ms": {"p50": pct(p_lat, 50), "p99": pct(p_lat, 99), "max": pct(p_lat, 100)},
@@ -389,7 +389,8 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="served workload: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="served workload: Poisson arrival rate (1/s)")
     ap.add_argument("--budget-gb", type=int, default=16, help="served workload: memory budget (GiB)")
-    ap.add_argument("--round-items", type=int, default=16384, help="served workload: work items per round")
+    ap.add_argument("--step-iterations", type=int, default=2,
+                    help="served workload: engine iterations between looks at finished requests")
     ap.add_argument("--seed", type=int, default=2013, help="served workload: request stream seed")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)

@@ -355,9 +355,10 @@ int rzk_render_guided(rzk_ctx *ctx, const rzk_guide *guide, const rzk_pt_params 
                       int64_t *out_rays);
 
 typedef struct rzk_server_params {
-    int64_t memory_budget;          /* bytes of device memory for admitted requests (guides + images) */
-    int32_t round_items;            /* work items (64 pixels x samples_per_item) per round launch */
-    rzk_batch_config launch;        /* launch shape of the round kernel */
+    int64_t memory_budget;          /* bytes of device memory for admitted requests (guide tables,
+                                       per-sample radiance, image) */
+    int32_t step_iterations;        /* engine iterations between looks at finished requests (1..8) */
+    rzk_batch_config launch;        /* launch shape and path slots of the wavefront engine */
     int32_t use_cluster_params;     /* 1: guide parameters of the clusters' LightSlice, else `guide` */
     rzk_guide_params guide;
 } rzk_server_params;
@@ -371,13 +372,13 @@ typedef struct rzk_request {
 
 typedef struct rzk_request_stats {
     double submitted, admitted, finished;   /* seconds on the server's clock */
-    int32_t rounds;                 /* round launches the request took part in */
+    int32_t rounds;                 /* engine steps the request was open for */
     int32_t pad;
     int64_t device_bytes;           /* memory the request held while admitted */
 } rzk_request_stats;
 
 typedef struct rzk_server_stats {
-    int64_t submitted, finished, rounds, items;
+    int64_t submitted, finished, rounds, items;   /* rounds: engine steps; items: paths finished */
     int64_t peak_bytes;             /* most device memory admitted requests held at once */
     double busy_seconds;            /* summed device time of the round launches */
     double now;                     /* the server clock */

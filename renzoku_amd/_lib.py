@@ -107,7 +107,7 @@ class GuideParams(C.Structure):
 
 
 class ServerParams(C.Structure):
-    _fields_ = [("memory_budget", C.c_int64), ("round_items", C.c_int32), ("launch", BatchConfig),
+    _fields_ = [("memory_budget", C.c_int64), ("step_iterations", C.c_int32), ("launch", BatchConfig),
                 ("use_cluster_params", C.c_int32), ("guide", GuideParams)]
 
 
@@ -661,14 +661,13 @@ class Clusters:
 class Server:
     """rzk_server: continuous-batching request loop over a resident scene and clusters"""
 
-    def __init__(self, device, scene, clusters, memory_budget=8 << 30, round_items=16384, launch=None, guide=None):
+    def __init__(self, device, scene, clusters, memory_budget=8 << 30, step_iterations=2, launch=None, guide=None):
         self.lib = load()
         self.scene, self.clusters = scene, clusters
         p = ServerParams()
         p.memory_budget = int(memory_budget)
-        p.round_items = int(round_items)
-        p.launch = BatchConfig(**dict(BatchConfig.DEFAULTS, **(launch or dict(block=256, blocks_per_cu=4,
-                                                                           engine=0))))
+        p.step_iterations = int(step_iterations)
+        p.launch = BatchConfig(**dict(BatchConfig.DEFAULTS, **(launch or {})))
         p.use_cluster_params = 0 if guide else 1
         if guide:
             p.guide = GuideParams(**guide)

@@ -34,7 +34,7 @@ struct JobGpu {
     GuideView gv;
     int64_t item_first;            // first work item of the job
     int64_t pix_first;             // first output pixel of the job (jobs concatenated)
-    int64_t sample_off;            // first float of the job's per-sample radiance [pixel][spp][3]
+    int64_t sample_off;            // first float of the job's per-sample radiance [spp][pixel][3]
     int pix_blocks, samp_blocks, samples_per_item, strata;
 };
 
@@ -48,29 +48,6 @@ struct BatchHeader {
     float* out;                    // per-pixel mean, jobs concatenated, [h][w][3] each
     unsigned long long* counter;   // work-queue head
     unsigned long long* work;      // {closest rays, shadow rays} or null
-    int items_per_dequeue, pad;
-};
-
-// One request's share of a served round: items [item_begin, item_begin +
-// count) of the request, in sample-major order (item = sample_block *
-// pix_blocks + pixel_block).  A round holds at most pix_blocks consecutive
-// items of a request, so no pixel appears twice in a round and each pixel's
-// samples are added to `acc` in sample order across rounds.
-struct ServeJobGpu {
-    CamView cam;
-    PtParams pp;
-    GuideView gv;
-    int64_t item_first;            // round-global index of the share's first item
-    int64_t item_begin;            // request-local index of that item
-    int pix_blocks, samples_per_item, strata, pad;
-    float* acc;                    // running per-pixel sums [h][w][3]
-    float* out;                    // per-pixel mean, written with the last sample
-};
-
-struct ServeHeader {
-    const ServeJobGpu* jobs;
-    int64_t n_jobs, n_items;
-    unsigned long long* counter;
     int items_per_dequeue, pad;
 };
 
@@ -93,13 +70,19 @@ struct WfView {
     unsigned long long* ctrl;           // {paths handed out, active slots}
 };
 
-// the job table of the run in flight (read by every kernel of the captured iteration)
+// The job table in flight (read by every kernel of the captured iteration).
+// Jobs are numbered 0, 1, 2, ... for the life of the table; job j lives in
+// ring entry j & (job_cap - 1), and paths are numbered across jobs in order
+// (item_first = the job's first path, paths sample-major within a job).  New
+// jobs may be appended between iterations; only jobs [job_lo, n_jobs) can
+// still hand out paths.
 struct WfJobs {
-    const JobGpu* jobs;                 // item_first = first path index (paths sample-major per job)
-    int64_t n_jobs, total_paths;
-    float* samples;                     // per-sample radiance, [pixel][spp][3] at each job's sample_off
+    const JobGpu* jobs;                 // ring of job_cap entries (a power of two)
+    int64_t job_lo, n_jobs, job_cap;
+    int64_t total_paths;
+    float* samples;                     // base of every job's per-sample radiance ([spp][pixel][3] at sample_off)
     unsigned long long* work;           // {closest rays, shadow rays} or null
-    unsigned int* job_done;             // finished paths per job or null
+    unsigned int* job_done;             // finished paths per ring entry, or null
 };
 
 }  // namespace rzk
@@ -163,8 +146,6 @@ void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv, const float* cam, const
 // the three kernels of a batch on `stream` (captured into a graph by the caller)
 void launch_batch(hipStream_t stream, const SceneView& sv, const rzk::BatchHeader* d_hdr, int grid, int block,
                   int reduce_grid);
-// the two kernels of a served round (captured into a graph by the caller)
-void launch_serve_round(hipStream_t stream, const SceneView& sv, const rzk::ServeHeader* d_hdr, int grid, int block);
 // one iteration of the wavefront engine (queues of parity `in` consumed,
 // parity in ^ 1 produced): extend + next-event rays, shadow rays, next-event
 // resolution, bounce shading, path regeneration
@@ -180,6 +161,15 @@ int64_t wf_slots(const WfEngine* e);
 // completion on ctx's stream; max_bounce bounds the iterations; returns them
 int64_t wf_run(WfEngine* e, const rzk::JobGpu* d_jobs, int64_t n_jobs, int64_t total_paths, int max_bounce,
                float* d_samples, unsigned long long* d_work, unsigned int* d_job_done);
+// incremental use (a request loop): reset the slots, publish a job table
+// (queued on the stream), run iterations, read {paths handed out, active
+// slots} as of the last completed wf_counters_async
+void wf_reset(WfEngine* e);
+void wf_publish(WfEngine* e, const rzk::WfJobs& jobs);
+void wf_iterate(WfEngine* e, int n);
+// each job's finished paths: k_job_reduce sums a job's samples per pixel in
+// sample order into out ([h][w][3] means)
+void launch_job_reduce(hipStream_t stream, const rzk::JobGpu* d_job, const float* samples, float* out);
 // per-pixel means of finished per-sample radiance (k_batch_reduce)
 void launch_batch_reduce(hipStream_t stream, const rzk::BatchHeader* d_hdr, int grid);
 // PtParams of a job over a guide

@@ -335,13 +335,14 @@ __global__ __launch_bounds__(256) void k_batch_pt(SceneView sv, const BatchHeade
                 int slice = pixel_slice(pp, gv, x, y);
                 int s0 = sb * J->samples_per_item;
                 int s1 = min(pp.spp, s0 + J->samples_per_item);
-                float* dst = hdr.samples + J->sample_off + p * pp.spp * 3;
+                const int64_t npx = (int64_t)pp.w * pp.h;
                 for (int smp = s0; smp < s1; ++smp) {
                     v3 L = trace_path(sv, J->cam.v, pp, gv, x, y, slice, J->strata, smp, astk, st,
                                       nclosest, nshadow);
-                    dst[3 * smp] = L.x;
-                    dst[3 * smp + 1] = L.y;
-                    dst[3 * smp + 2] = L.z;
+                    float* dst = hdr.samples + J->sample_off + ((int64_t)smp * npx + p) * 3;
+                    dst[0] = L.x;
+                    dst[1] = L.y;
+                    dst[2] = L.z;
                 }
             }
         }
@@ -362,76 +363,13 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const BatchHeader* __restr
         }
         const JobGpu* J = hdr.jobs + lo;
         int spp = J->pp.spp;
-        const float* src = hdr.samples + J->sample_off + (i - J->pix_first) * spp * 3;
+        const int64_t npx = (int64_t)J->pp.w * J->pp.h;
+        const float* src = hdr.samples + J->sample_off + (i - J->pix_first) * 3;
         v3 acc = mk(0.f, 0.f, 0.f);
-        for (int smp = 0; smp < spp; ++smp) acc = acc + mk(src[3 * smp], src[3 * smp + 1], src[3 * smp + 2]);
+        for (int smp = 0; smp < spp; ++smp, src += npx * 3) acc = acc + mk(src[0], src[1], src[2]);
         hdr.out[3 * i] = acc.x / (float)spp;
         hdr.out[3 * i + 1] = acc.y / (float)spp;
         hdr.out[3 * i + 2] = acc.z / (float)spp;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Served form: a round over shares of many requests.  Each lane adds its
-// pixel's samples of one sample block to the running sum in sample order and
-// writes the mean with the last sample, so the image is the plain render's.
-
-__global__ void k_serve_reset(const ServeHeader* __restrict__ hdr) {
-    if (threadIdx.x == 0) *hdr->counter = 0ull;
-}
-
-__global__ __launch_bounds__(256) void k_serve_pt(SceneView sv, const ServeHeader* __restrict__ hdrp) {
-    extern __shared__ int s_stack[];
-    const ServeHeader hdr = *hdrp;
-    AnyStack astk = any_stack(sv, s_stack);
-    const int lane = threadIdx.x & 63;
-    TravStats st;
-    unsigned long long nclosest = 0, nshadow = 0;
-    const unsigned long long n_items = (unsigned long long)hdr.n_items;
-    const unsigned long long q = (unsigned long long)hdr.items_per_dequeue;
-    const int last_job = (int)hdr.n_jobs - 1;
-    for (;;) {
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(hdr.counter, q);
-        base = wave_bcast64(base);
-        if (base >= n_items) break;
-        unsigned long long end = base + q < n_items ? base + q : n_items;
-        for (unsigned long long item = base; item < end; ++item) {
-            int lo = 0, hi = last_job;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if ((unsigned long long)hdr.jobs[mid].item_first <= item) lo = mid;
-                else hi = mid - 1;
-            }
-            const ServeJobGpu* J = hdr.jobs + lo;
-            int64_t local = (int64_t)(item - (unsigned long long)J->item_first) + J->item_begin;
-            int64_t sb = local / J->pix_blocks;
-            int64_t pb = local - sb * J->pix_blocks;
-            int64_t p = pb * 64 + lane;
-            const PtParams pp = J->pp;
-            if (p < (int64_t)pp.w * pp.h) {
-                int x = pp.x0 + (int)(p % pp.w), y = pp.y0 + (int)(p / pp.w);
-                const GuideView gv = J->gv;
-                int slice = pixel_slice(pp, gv, x, y);
-                int s0 = (int)sb * J->samples_per_item;
-                int s1 = min(pp.spp, s0 + J->samples_per_item);
-                float* a = J->acc + 3 * p;
-                v3 acc = s0 == 0 ? mk(0.f, 0.f, 0.f) : mk(a[0], a[1], a[2]);
-                for (int smp = s0; smp < s1; ++smp)
-                    acc = acc + trace_path(sv, J->cam.v, pp, gv, x, y, slice, J->strata, smp, astk, st,
-                                           nclosest, nshadow);
-                if (s1 == pp.spp) {
-                    float* o = J->out + 3 * p;
-                    o[0] = acc.x / (float)pp.spp;
-                    o[1] = acc.y / (float)pp.spp;
-                    o[2] = acc.z / (float)pp.spp;
-                } else {
-                    a[0] = acc.x;
-                    a[1] = acc.y;
-                    a[2] = acc.z;
-                }
-            }
-        }
     }
 }
 
@@ -477,14 +415,28 @@ __device__ __forceinline__ void st_p(float* a, int64_t P, int64_t i, v3 v) {
 
 // a finished path: its radiance into the per-sample buffer, the slot freed
 __device__ __forceinline__ void wf_finish(const WfView& v, const WfJobs& jh, int64_t slot) {
-    const JobGpu& J = jh.jobs[v.job[slot]];
+    const JobGpu& J = jh.jobs[v.job[slot] & (jh.job_cap - 1)];
     int64_t p = v.pix[slot];
     int smp = v.smp[slot];
-    float* dst = jh.samples + J.sample_off + (p * J.pp.spp + smp) * 3;
+    float* dst = jh.samples + J.sample_off + ((int64_t)smp * J.pp.w * J.pp.h + p) * 3;
     v3 L = ld_p(v.L, v.P, slot);
     dst[0] = L.x; dst[1] = L.y; dst[2] = L.z;
     v.flags[slot] = 0;
-    if (jh.job_done) atomicAdd(jh.job_done + v.job[slot], 1u);
+}
+
+// finished paths per job, one atomic per distinct job in the wave (lanes of a
+// wave mostly share one job); every lane of the wave calls this
+__device__ __forceinline__ void wave_count_done(const WfJobs& jh, int job, bool fin) {
+    if (!jh.job_done) return;
+    unsigned long long rem = __ballot(fin);
+    const int lane = wave_lane();
+    while (rem != 0ull) {
+        int leader = __ffsll((long long)rem) - 1;
+        int lj = __shfl(job, leader);
+        unsigned long long m = __ballot(fin && job == lj);
+        if (lane == leader) atomicAdd(jh.job_done + (lj & (jh.job_cap - 1)), (unsigned int)__popcll(m));
+        rem &= ~m;
+    }
 }
 
 __global__ void k_wf_begin(WfView v, int out) {
@@ -503,7 +455,7 @@ __global__ __launch_bounds__(256) void k_wf_closest(SceneView sv, WfView v, cons
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         bool nee = i >= n1;
         int64_t slot = nee ? v.q[in][kQNee][i - n1] : v.q[in][kQExt][i];
-        float tmin = jh.jobs[v.job[slot]].pp.tmin;
+        float tmin = jh.jobs[v.job[slot] & (jh.job_cap - 1)].pp.tmin;
         Ray r = nee ? ray_from_od(ld_p(v.np, v.P, slot), ld_p(v.nd, v.P, slot))
                     : ray_from_od(ld_p(v.ro, v.P, slot), ld_p(v.rd, v.P, slot));
         float t = 0.f, b = 0.f, g = 0.f;
@@ -525,7 +477,7 @@ __global__ __launch_bounds__(256) void k_wf_any(SceneView sv, WfView v, const Wf
     const int64_t n = v.counts[in * 4 + kQAny], stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         int64_t slot = v.q[in][kQAny][i];
-        float tmin = jh.jobs[v.job[slot]].pp.tmin;
+        float tmin = jh.jobs[v.job[slot] & (jh.job_cap - 1)].pp.tmin;
         nshadow++;
         v.blocked[slot] = segment_blocked(sv, ld_p(v.np, v.P, slot), ld_p(v.nq, v.P, slot), tmin, astk, st) ? 1 : 0;
     }
@@ -591,6 +543,7 @@ __global__ __launch_bounds__(256) void k_wf_shade2(SceneView sv, WfView v, const
         }
         int at = wave_append(v.free_counts + (in ^ 1), fin);
         if (fin) v.freeq[in ^ 1][at] = (int)slot;
+        wave_count_done(jh, fin ? v.job[slot] : 0, fin);
         wave_sub_active(v.ctrl + 1, fin);
     }
 }
@@ -606,7 +559,7 @@ __global__ __launch_bounds__(256) void k_wf_shade1(SceneView sv, WfView v, const
         int64_t slot = 0;
         if (i < n) {
             slot = v.q[in][kQExt][i];
-            const JobGpu& J = jh.jobs[v.job[slot]];
+            const JobGpu& J = jh.jobs[v.job[slot] & (jh.job_cap - 1)];
             const PtParams pp = J.pp;
             const int bounce = v.bounce[slot];
             float4 h = v.hit[slot];
@@ -740,6 +693,7 @@ __global__ __launch_bounds__(256) void k_wf_shade1(SceneView sv, WfView v, const
         if ((at = wave_append(v.counts + out * 4 + kQAny, e_any)) >= 0) v.q[out][kQAny][at] = (int)slot;
         if ((at = wave_append(v.counts + out * 4 + kQRes, e_res)) >= 0) v.q[out][kQRes][at] = (int)slot;
         if ((at = wave_append(v.free_counts + out, fin)) >= 0) v.freeq[out][at] = (int)slot;
+        wave_count_done(jh, fin ? v.job[slot] : 0, fin);
         wave_sub_active(v.ctrl + 1, fin);
     }
 }
@@ -751,7 +705,7 @@ __global__ __launch_bounds__(256) void k_wf_generate(WfView v, const WfJobs* __r
     const WfJobs jh = *jhp;
     const int out = in ^ 1;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int last_job = (int)jh.n_jobs - 1;
+    const int64_t mask_j = jh.job_cap - 1;
     const int64_t nfree = v.free_counts[in];
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < nfree; i0 += stride) {
         int64_t fi = i0 + threadIdx.x;
@@ -765,15 +719,15 @@ __global__ __launch_bounds__(256) void k_wf_generate(WfView v, const WfJobs* __r
         if (lane == leader) base = atomicAdd(v.ctrl, (unsigned long long)__popcll(mask));
         base = __shfl(base, leader);
         unsigned long long g = base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
-        bool take = free_ && g < (unsigned long long)jh.total_paths && last_job >= 0;
+        bool take = free_ && g < (unsigned long long)jh.total_paths && jh.n_jobs > jh.job_lo;
         if (take) {
-            int lo = 0, hi = last_job;
+            int64_t lo = jh.job_lo, hi = jh.n_jobs - 1;
             while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if ((unsigned long long)jh.jobs[mid].item_first <= g) lo = mid;
+                int64_t mid = (lo + hi + 1) >> 1;
+                if ((unsigned long long)jh.jobs[mid & mask_j].item_first <= g) lo = mid;
                 else hi = mid - 1;
             }
-            const JobGpu& J = jh.jobs[lo];
+            const JobGpu& J = jh.jobs[lo & mask_j];
             const PtParams pp = J.pp;
             int64_t npx = (int64_t)pp.w * pp.h;
             int64_t local = (int64_t)(g - (unsigned long long)J.item_first);
@@ -789,7 +743,7 @@ __global__ __launch_bounds__(256) void k_wf_generate(WfView v, const WfJobs* __r
                 jy = ((float)(smp / strata) + jy) / (float)strata;
             }
             Ray ray = camera_sample_ray(J.cam.v, (float)x + jx, (float)y + jy);
-            v.job[slot] = lo;
+            v.job[slot] = (int)lo;
             v.pix[slot] = (int)p;
             v.smp[slot] = smp;
             v.bounce[slot] = 1;
@@ -801,8 +755,33 @@ __global__ __launch_bounds__(256) void k_wf_generate(WfView v, const WfJobs* __r
         }
         int at = wave_append(v.counts + out * 4 + kQExt, take);
         if (take) v.q[out][kQExt][at] = (int)slot;
+        // a free slot without a path now waits for the next iteration's paths
+        bool wait = free_ && !take;
+        at = wave_append(v.free_counts + out, wait);
+        if (wait) v.freeq[out][at] = (int)slot;
         unsigned long long tm = __ballot(take);
         if (tm != 0ull && lane == __ffsll((long long)tm) - 1) atomicAdd(v.ctrl + 1, (unsigned long long)__popcll(tm));
+    }
+}
+
+// paths handed out past the table's end were not taken: hand them out again
+__global__ void k_wf_clamp(WfView v, const WfJobs* __restrict__ jhp) {
+    if (threadIdx.x == 0 && v.ctrl[0] > (unsigned long long)jhp->total_paths)
+        v.ctrl[0] = (unsigned long long)jhp->total_paths;
+}
+
+// a finished job: per-pixel mean of its samples, summed in sample order
+__global__ void k_job_reduce(const JobGpu* __restrict__ J, const float* __restrict__ samples,
+                             float* __restrict__ out) {
+    const int64_t npx = (int64_t)J->pp.w * J->pp.h;
+    const int spp = J->pp.spp;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npx; p += (int64_t)gridDim.x * blockDim.x) {
+        const float* src = samples + J->sample_off + p * 3;
+        v3 acc = mk(0.f, 0.f, 0.f);
+        for (int smp = 0; smp < spp; ++smp, src += npx * 3) acc = acc + mk(src[0], src[1], src[2]);
+        out[3 * p] = acc.x / (float)spp;
+        out[3 * p + 1] = acc.y / (float)spp;
+        out[3 * p + 2] = acc.z / (float)spp;
     }
 }
 
@@ -819,17 +798,17 @@ void launch_wf_iteration(hipStream_t stream, const SceneView& sv, const WfView& 
     hipLaunchKernelGGL(k_wf_shade2, dim3(grid), dim3(block), 0, stream, sv, v, d_jh, in);
     hipLaunchKernelGGL(k_wf_shade1, dim3(grid), dim3(block), 0, stream, sv, v, d_jh, in);
     hipLaunchKernelGGL(k_wf_generate, dim3(grid), dim3(block), 0, stream, v, d_jh, in);
+    hipLaunchKernelGGL(k_wf_clamp, dim3(1), dim3(64), 0, stream, v, d_jh);
+}
+
+void launch_job_reduce(hipStream_t stream, const JobGpu* d_job, const float* samples, float* out) {
+    hipLaunchKernelGGL(k_job_reduce, dim3(512), dim3(256), 0, stream, d_job, samples, out);
 }
 
 void launch_batch_reduce(hipStream_t stream, const BatchHeader* d_hdr, int grid) {
     hipLaunchKernelGGL(k_batch_reduce, dim3(grid), dim3(256), 0, stream, d_hdr);
 }
 
-void launch_serve_round(hipStream_t stream, const SceneView& sv, const ServeHeader* d_hdr, int grid, int block) {
-    hipLaunchKernelGGL(k_serve_reset, dim3(1), dim3(64), 0, stream, d_hdr);
-    size_t lds = sizeof(int) * (size_t)block * (size_t)kLdsStack;
-    hipLaunchKernelGGL(k_serve_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
-}
 
 void launch_batch(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
                   int reduce_grid) {

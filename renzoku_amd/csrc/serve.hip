@@ -1,12 +1,17 @@
 // serve.hip — the served form: a request loop over a resident scene, VPL
-// clusters and memory budget.  Requests (camera, spp, depth, seed) are queued
-// by any thread; a worker thread admits them oldest first while the budget
-// allows (guide tables + running sums + image), builds each admitted
-// request's guide from the resident clusters, and runs rounds: each round is
-// one graph launch of a persistent kernel over up to round_items work items
-// drawn from all admitted requests, oldest first.  Requests that arrive while
-// a round runs join the next one; finished images are copied out
-// asynchronously and their memory returns to the budget.
+// clusters and memory budget, feeding one wavefront path-tracing engine.
+//
+// Requests (camera, spp, depth, seed) are queued by any thread.  A worker
+// thread admits them oldest first while the budget allows (guide tables +
+// per-sample radiance + image), builds each admitted request's guide from the
+// resident clusters, and appends the request as a job to the engine's job
+// ring.  The engine never stops between requests: its free path slots take
+// the next paths of whichever jobs are open, so requests that arrive while
+// others are on the GPU start as soon as slots free up (continuous batching
+// at path granularity).  When a job's paths are all finished its pixels are
+// summed in sample order, the image is copied out asynchronously, and its
+// memory returns to the budget.  A request's image is therefore the same,
+// bit for bit, alone or in any mix, and equal to the plain per-request render.
 //
 // Replaces the reference's per-frame loop (Scene::start -> IntegratorScheduler,
 // scene.cc:441, integrator_scheduler.cc:143-229), which renders one image at
@@ -29,6 +34,8 @@
 using namespace rzk;
 
 namespace {
+
+constexpr int64_t kJobRing = 1 << 14;      // jobs open at once (a power of two)
 
 // first-fit allocator over one device block: the memory budget
 struct Arena {
@@ -81,22 +88,26 @@ struct Request {
     rzk_request r{};
     int W = 0, H = 0;
     double t_submit = 0, t_admit = 0, t_finish = 0;
-    int rounds = 0;
-    // admitted state
+    int steps = 0;                   // engine steps the request was open for
     GuideLayout L;
-    size_t off_tab = 0, off_tmp = 0, off_acc = 0, off_out = 0;
-    size_t bytes_img = 0;
+    size_t bytes_samples = 0, bytes_img = 0;
+    size_t off_tab = 0, off_smp = 0, off_out = 0;
     rzk_guide guide;
-    int64_t pix_blocks = 0, samp_blocks = 0, total_items = 0, next_item = 0;
-    int S = 1, strata = 0;
+    int64_t job = -1, paths = 0;
     hipEvent_t done_ev = nullptr;
     float* host_img = nullptr;       // pinned, filled by the final copy
     size_t host_bytes = 0;
-    int state = 0;                   // 0 queued, 1 active, 2 copying, 3 finished, 4 failed
-    int64_t held_bytes() const { return (int64_t)(L.bytes_tab + 2 * bytes_img); }
+    int state = 0;                   // 0 queued, 1 open, 2 copying, 3 finished, 4 failed
+    int64_t held_bytes() const { return (int64_t)(L.bytes_tab + bytes_samples + bytes_img); }
 };
 
-size_t request_bytes(const GuideLayout& L, size_t img) { return L.bytes_tab + L.bytes_tmp + 2 * img + 4 * 256; }
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// device bytes a request takes at admission (the guide-build temporaries are
+// returned right after the build is queued)
+size_t request_bytes(const GuideLayout& L, size_t samples, size_t img) {
+    return round256(L.bytes_tab) + round256(L.bytes_tmp) + round256(samples) + round256(img);
+}
 
 }  // namespace
 
@@ -107,25 +118,22 @@ struct rzk_server {
     rzk_server_params prm{};
     rzk_guide_params gp{};
     Arena arena;
-    SceneView sv{};
-    int* spill = nullptr;
-    // two round slots: device job table + header, pinned staging, graph
-    ServeJobGpu* d_jobs[2] = {nullptr, nullptr};
-    ServeHeader* d_hdr[2] = {nullptr, nullptr};
-    ServeJobGpu* h_jobs[2] = {nullptr, nullptr};
-    ServeHeader* h_hdr[2] = {nullptr, nullptr};
-    size_t job_cap = 0;
-    unsigned long long* d_counter = nullptr;
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipEvent_t ev_start[2] = {nullptr, nullptr}, ev_end[2] = {nullptr, nullptr};
-    bool slot_busy[2] = {false, false};
+    WfEngine* wf = nullptr;
+    JobGpu* d_jobs = nullptr;              // job ring
+    JobGpu* h_jobs = nullptr;              // pinned staging, one entry per ring slot
+    unsigned int* d_done = nullptr;        // finished paths per ring slot
+    unsigned int* h_done = nullptr;        // two pinned snapshots
+    hipEvent_t ev_step[2] = {nullptr, nullptr}, ev_t0[2] = {nullptr, nullptr};
+    bool step_busy[2] = {false, false};
+    int64_t step_jobs[2] = {0, 0};         // jobs admitted before each snapshot was queued
+    int step_slot = 0;
+    int64_t n_jobs = 0, total_paths = 0;
     // request bookkeeping
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> incoming;
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
-    std::deque<Request*> waiting, active, copying;
+    std::deque<Request*> waiting, open, copying;
     std::vector<std::pair<float*, size_t>> pinned_pool;
     int64_t next_ticket = 1;
     std::atomic<bool> stop{false};
@@ -137,7 +145,6 @@ struct rzk_server {
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
-    int grid() const { return std::max(1, ctx->num_cus) * prm.launch.blocks_per_cu; }
 
     // a pooled pinned buffer of at least `bytes` (its size in `bytes`), or null
     float* pinned_get(size_t& bytes) {
@@ -155,52 +162,21 @@ struct rzk_server {
         return nullptr;
     }
 
-    void capture() {
-        for (int k = 0; k < 2; ++k) {
-            hipStream_t st = ctx->stream;
-            RZK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            launch_serve_round(st, sv, d_hdr[k], grid(), prm.launch.block);
-            hipError_t le = hipGetLastError();
-            hipGraph_t g = nullptr;
-            hipError_t ee = hipStreamEndCapture(st, &g);
-            RZK_HIP(le);
-            RZK_HIP(ee);
-            graph[k] = g;
-            RZK_HIP(hipGraphInstantiate(&exec[k], g, nullptr, nullptr, 0));
-        }
+    int64_t oldest_open_job() const { return open.empty() ? n_jobs : open.front()->job; }
+
+    void publish() {
+        wf_publish(wf, WfJobs{d_jobs, oldest_open_job(), n_jobs, kJobRing, total_paths, (float*)arena.base, nullptr,
+                              d_done});
     }
 
-    void ensure_job_cap(size_t n) {
-        if (n <= job_cap) return;
-        size_t cap = std::max(n, job_cap * 2);
-        for (int k = 0; k < 2; ++k) {
-            if (slot_busy[k]) { RZK_HIP(hipEventSynchronize(ev_end[k])); retire_slot(k); }
-            if (d_jobs[k]) RZK_HIP(hipFree(d_jobs[k]));
-            if (h_jobs[k]) RZK_HIP(hipHostFree(h_jobs[k]));
-            RZK_HIP(hipMalloc((void**)&d_jobs[k], sizeof(ServeJobGpu) * cap));
-            RZK_HIP(hipHostMalloc((void**)&h_jobs[k], sizeof(ServeJobGpu) * cap, hipHostMallocDefault));
-        }
-        job_cap = cap;
-    }
-
-    void retire_slot(int k) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ev_start[k], ev_end[k]) == hipSuccess) {
-            std::lock_guard<std::mutex> lk(mu);
-            stats.busy_seconds += ms * 1e-3;
-        }
-        slot_busy[k] = false;
-    }
-
-    // admit queued requests oldest first while the budget allows
+    // admit queued requests oldest first while the budget and the job ring allow
     void admit() {
-        while (!waiting.empty()) {
+        while (!waiting.empty() && n_jobs - oldest_open_job() < kJobRing) {
             Request* q = waiting.front();
-            size_t need = request_bytes(q->L, q->bytes_img);
-            size_t o_tab, o_tmp, o_acc, o_out;
+            size_t o_tab, o_tmp, o_smp, o_out;
             if (!arena.alloc(q->L.bytes_tab, o_tab)) break;
             if (!arena.alloc(q->L.bytes_tmp, o_tmp)) { arena.release(o_tab, q->L.bytes_tab); break; }
-            if (!arena.alloc(q->bytes_img, o_acc)) {
+            if (!arena.alloc(q->bytes_samples, o_smp)) {
                 arena.release(o_tab, q->L.bytes_tab);
                 arena.release(o_tmp, q->L.bytes_tmp);
                 break;
@@ -208,139 +184,157 @@ struct rzk_server {
             if (!arena.alloc(q->bytes_img, o_out)) {
                 arena.release(o_tab, q->L.bytes_tab);
                 arena.release(o_tmp, q->L.bytes_tmp);
-                arena.release(o_acc, q->bytes_img);
+                arena.release(o_smp, q->bytes_samples);
                 break;
             }
-            (void)need;
-            q->off_tab = o_tab; q->off_tmp = o_tmp; q->off_acc = o_acc; q->off_out = o_out;
+            q->off_tab = o_tab; q->off_smp = o_smp; q->off_out = o_out;
+            hipStream_t st = ctx->stream;
             GuideBuffers b = guide_buffers(q->L, arena.base + o_tmp, arena.base + o_tab);
             enqueue_guide_build(ctx, scene, *cl, q->r.camera, gp, q->L, b);
             // temporaries are only read by the kernels just queued: later users are stream-ordered after them
             arena.release(o_tmp, q->L.bytes_tmp);
             guide_set(q->guide, scene, q->r.camera, gp, q->L, b);
             q->guide.ctx = ctx;
+            // the job: paths numbered from total_paths, samples in its arena region
+            int64_t j = n_jobs++;
+            JobGpu& J = h_jobs[j & (kJobRing - 1)];
+            std::memset(&J, 0, sizeof(J));
+            std::memcpy(J.cam.v, q->r.camera, sizeof(J.cam.v));
+            rzk_render_job rj{&q->guide, 0, 0, q->W, q->H, q->r.spp, q->r.max_bounce, q->r.direct_lighting,
+                              q->r.guide_mode, q->r.seed, q->r.tmin};
+            J.pp = job_params(q->guide, rj);
+            J.gv = q->guide.gv;
+            J.item_first = total_paths;
+            J.sample_off = (int64_t)(o_smp / sizeof(float));
+            int strata = 1;
+            while ((strata + 1) * (strata + 1) <= q->r.spp) ++strata;
+            J.strata = strata * strata == q->r.spp ? strata : 0;
+            RZK_HIP(hipMemcpyAsync(d_jobs + (j & (kJobRing - 1)), &J, sizeof(JobGpu), hipMemcpyHostToDevice, st));
+            RZK_HIP(hipMemsetAsync(d_done + (j & (kJobRing - 1)), 0, sizeof(unsigned int), st));
+            q->job = j;
+            q->paths = (int64_t)q->W * q->H * q->r.spp;
+            total_paths += q->paths;
             q->t_admit = now();
             q->state = 1;
             waiting.pop_front();
-            active.push_back(q);
+            open.push_back(q);
         }
         std::lock_guard<std::mutex> lk(mu);
         stats.peak_bytes = std::max<int64_t>(stats.peak_bytes, (int64_t)arena.peak);
     }
 
-    // fill one slot's job table from the active requests, oldest first
-    int64_t plan_round(int k, std::vector<Request*>& finished_now) {
-        int64_t budget = prm.round_items, total = 0;
-        size_t n = 0;
-        std::vector<std::pair<Request*, int64_t>> shares;
-        for (Request* q : active) {
-            if (budget <= 0) break;
-            int64_t left = q->total_items - q->next_item;
-            int64_t take = std::min(std::min(left, q->pix_blocks), budget);
-            if (take <= 0) continue;
-            shares.push_back({q, take});
-            budget -= take;
+    void retire_step(int k) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev_t0[k], ev_step[k]) == hipSuccess) {
+            std::lock_guard<std::mutex> lk(mu);
+            stats.busy_seconds += ms * 1e-3;
         }
-        ensure_job_cap(shares.size());
-        for (auto& sh : shares) {
-            Request* q = sh.first;
-            ServeJobGpu& J = h_jobs[k][n++];
-            std::memset(&J, 0, sizeof(J));
-            std::memcpy(J.cam.v, q->r.camera, sizeof(J.cam.v));
-            rzk_render_job j{&q->guide, 0, 0, q->W, q->H, q->r.spp, q->r.max_bounce, q->r.direct_lighting,
-                             q->r.guide_mode, q->r.seed, q->r.tmin};
-            J.pp = job_params(q->guide, j);
-            J.gv = q->guide.gv;
-            J.item_first = total;
-            J.item_begin = q->next_item;
-            J.pix_blocks = (int)q->pix_blocks;
-            J.samples_per_item = q->S;
-            J.strata = q->strata;
-            J.acc = (float*)(arena.base + q->off_acc);
-            J.out = (float*)(arena.base + q->off_out);
-            total += sh.second;
-            q->next_item += sh.second;
-            q->rounds++;
-            if (q->next_item == q->total_items) finished_now.push_back(q);
+        step_busy[k] = false;
+    }
+
+    // requests whose paths are all finished (as of snapshot `k`): reduce, copy out
+    void finish_done(int k) {
+        const unsigned int* done = h_done + k * kJobRing;
+        hipStream_t st = ctx->stream;
+        int64_t finished_paths = 0;
+        for (auto it = open.begin(); it != open.end();) {
+            Request* q = *it;
+            // a job admitted after the snapshot was queued has no count in it yet
+            if (q->job >= step_jobs[k] || (int64_t)done[q->job & (kJobRing - 1)] < q->paths) {
+                q->steps++;
+                ++it;
+                continue;
+            }
+            q->steps++;
+            launch_job_reduce(st, d_jobs + (q->job & (kJobRing - 1)), (const float*)arena.base,
+                              (float*)(arena.base + q->off_out));
+            RZK_HIP(hipGetLastError());
+            q->host_bytes = q->bytes_img;
+            q->host_img = pinned_get(q->host_bytes);
+            if (!q->host_img) RZK_HIP(hipHostMalloc((void**)&q->host_img, q->host_bytes, hipHostMallocDefault));
+            RZK_HIP(hipMemcpyAsync(q->host_img, arena.base + q->off_out, q->bytes_img, hipMemcpyDeviceToHost, st));
+            RZK_HIP(hipEventRecord(q->done_ev, st));
+            q->state = 2;
+            finished_paths += q->paths;
+            copying.push_back(q);
+            it = open.erase(it);
         }
-        ServeHeader& h = *h_hdr[k];
-        h.jobs = d_jobs[k];
-        h.n_jobs = (int64_t)n;
-        h.n_items = total;
-        h.counter = d_counter;
-        h.items_per_dequeue = prm.launch.items_per_dequeue;
-        h.pad = 0;
-        return total;
+        std::lock_guard<std::mutex> lk(mu);
+        stats.items += finished_paths;
+    }
+
+    void retire_copies() {
+        while (!copying.empty()) {
+            Request* q = copying.front();
+            hipError_t e = hipEventQuery(q->done_ev);
+            if (e == hipErrorNotReady) break;
+            RZK_HIP(e);
+            copying.pop_front();
+            arena.release(q->off_tab, q->L.bytes_tab);
+            arena.release(q->off_smp, q->bytes_samples);
+            arena.release(q->off_out, q->bytes_img);
+            std::lock_guard<std::mutex> lk(mu);
+            q->t_finish = now();
+            q->state = 3;
+            stats.finished++;
+            cv_done.notify_all();
+        }
     }
 
     void loop() {
-        int slot = 0;
+        const int iters = std::max(1, std::min(8, prm.step_iterations));
         while (true) {
             {
                 std::unique_lock<std::mutex> lk(mu);
                 if (stop) break;
                 while (!incoming.empty()) { waiting.push_back(incoming.front()); incoming.pop_front(); }
-                bool idle = waiting.empty() && active.empty() && copying.empty() && !slot_busy[0] && !slot_busy[1];
+                bool idle = waiting.empty() && open.empty() && copying.empty() && !step_busy[0] && !step_busy[1];
                 if (idle) {
                     cv_work.wait_for(lk, std::chrono::milliseconds(50));
                     continue;
                 }
             }
-            // retire finished copies
-            while (!copying.empty()) {
-                Request* q = copying.front();
-                hipError_t e = hipEventQuery(q->done_ev);
-                if (e == hipErrorNotReady) break;
-                RZK_HIP(e);
-                copying.pop_front();
-                arena.release(q->off_tab, q->L.bytes_tab);
-                arena.release(q->off_acc, q->bytes_img);
-                arena.release(q->off_out, q->bytes_img);
-                std::lock_guard<std::mutex> lk(mu);
-                q->t_finish = now();
-                q->state = 3;
-                stats.finished++;
-                cv_done.notify_all();
-            }
+            retire_copies();
             admit();
-            // the slot about to be refilled must have finished its previous round
-            if (slot_busy[slot]) {
-                RZK_HIP(hipEventSynchronize(ev_end[slot]));
-                retire_slot(slot);
-            }
-            std::vector<Request*> fin;
-            int64_t items = active.empty() ? 0 : plan_round(slot, fin);
-            if (items == 0) {
-                // nothing to launch: wait for an outstanding copy or round
-                if (!copying.empty()) RZK_HIP(hipEventSynchronize(copying.front()->done_ev));
-                else if (slot_busy[slot ^ 1]) { RZK_HIP(hipEventSynchronize(ev_end[slot ^ 1])); retire_slot(slot ^ 1); }
-                else std::this_thread::sleep_for(std::chrono::microseconds(200));
+            if (open.empty()) {
+                // nothing on the engine: drain outstanding steps, then copies
+                bool waited = false;
+                for (int k = 0; k < 2; ++k)
+                    if (step_busy[k]) { RZK_HIP(hipEventSynchronize(ev_step[k])); retire_step(k); waited = true; }
+                if (!waited) {
+                    if (!copying.empty()) RZK_HIP(hipEventSynchronize(copying.front()->done_ev));
+                    else std::this_thread::sleep_for(std::chrono::microseconds(200));
+                }
                 continue;
             }
+            publish();
+            // one engine step: a few iterations, then a snapshot of the per-job finished counts
+            int k = step_slot;
+            step_slot ^= 1;
+            if (step_busy[k]) {
+                RZK_HIP(hipEventSynchronize(ev_step[k]));
+                retire_step(k);
+                finish_done(k);
+            }
             hipStream_t st = ctx->stream;
-            RZK_HIP(hipMemcpyAsync(d_jobs[slot], h_jobs[slot], sizeof(ServeJobGpu) * h_hdr[slot]->n_jobs,
-                                   hipMemcpyHostToDevice, st));
-            RZK_HIP(hipMemcpyAsync(d_hdr[slot], h_hdr[slot], sizeof(ServeHeader), hipMemcpyHostToDevice, st));
-            RZK_HIP(hipEventRecord(ev_start[slot], st));
-            RZK_HIP(hipGraphLaunch(exec[slot], st));
-            RZK_HIP(hipEventRecord(ev_end[slot], st));
-            slot_busy[slot] = true;
+            RZK_HIP(hipEventRecord(ev_t0[k], st));
+            wf_iterate(wf, iters);
+            RZK_HIP(hipMemcpyAsync(h_done + k * kJobRing, d_done, sizeof(unsigned int) * kJobRing,
+                                   hipMemcpyDeviceToHost, st));
+            RZK_HIP(hipEventRecord(ev_step[k], st));
+            step_busy[k] = true;
+            step_jobs[k] = n_jobs;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 stats.rounds++;
-                stats.items += items;
             }
-            for (Request* q : fin) {
-                active.erase(std::find(active.begin(), active.end(), q));
-                q->host_bytes = q->bytes_img;
-                q->host_img = pinned_get(q->host_bytes);
-                if (!q->host_img) RZK_HIP(hipHostMalloc((void**)&q->host_img, q->host_bytes, hipHostMallocDefault));
-                RZK_HIP(hipMemcpyAsync(q->host_img, arena.base + q->off_out, q->bytes_img, hipMemcpyDeviceToHost, st));
-                RZK_HIP(hipEventRecord(q->done_ev, st));
-                q->state = 2;
-                copying.push_back(q);
+            // act on the previous step's snapshot while this one runs
+            int prev = k ^ 1;
+            if (step_busy[prev]) {
+                RZK_HIP(hipEventSynchronize(ev_step[prev]));
+                retire_step(prev);
+                finish_done(prev);
             }
-            slot ^= 1;
         }
     }
 
@@ -368,23 +362,20 @@ struct rzk_server {
         cv_work.notify_all();
         if (worker.joinable()) worker.join();
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        if (wf) wf_destroy(wf);
         for (int k = 0; k < 2; ++k) {
-            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
-            if (graph[k]) (void)hipGraphDestroy(graph[k]);
-            if (ev_start[k]) (void)hipEventDestroy(ev_start[k]);
-            if (ev_end[k]) (void)hipEventDestroy(ev_end[k]);
-            if (d_jobs[k]) (void)hipFree(d_jobs[k]);
-            if (d_hdr[k]) (void)hipFree(d_hdr[k]);
-            if (h_jobs[k]) (void)hipHostFree(h_jobs[k]);
-            if (h_hdr[k]) (void)hipHostFree(h_hdr[k]);
+            if (ev_step[k]) (void)hipEventDestroy(ev_step[k]);
+            if (ev_t0[k]) (void)hipEventDestroy(ev_t0[k]);
         }
         for (auto& kv : reqs) {
             if (kv.second->done_ev) (void)hipEventDestroy(kv.second->done_ev);
             if (kv.second->host_img) (void)hipHostFree(kv.second->host_img);
         }
         for (auto& p : pinned_pool) (void)hipHostFree(p.first);
-        if (d_counter) (void)hipFree(d_counter);
-        if (spill) (void)hipFree(spill);
+        if (d_jobs) (void)hipFree(d_jobs);
+        if (d_done) (void)hipFree(d_done);
+        if (h_jobs) (void)hipHostFree(h_jobs);
+        if (h_done) (void)hipHostFree(h_done);
         if (ctx) rzk_ctx_destroy(ctx);
     }
 };
@@ -421,10 +412,10 @@ int rzk_server_create(int device, const rzk_scene* scene, const rzk_clusters* cl
     if (rc != RZK_OK) return rc;
     rc = serve_guard(s.get(), [&] {
         RZK_CHECK(cl->scene == scene, RZK_E_INVALID, "clusters belong to another scene");
-        RZK_CHECK(prm->memory_budget > 0 && prm->round_items > 0, RZK_E_INVALID, "bad server parameters");
+        RZK_CHECK(prm->memory_budget > 0, RZK_E_INVALID, "bad server parameters");
         const rzk_batch_config& c = prm->launch;
         RZK_CHECK((c.block == 64 || c.block == 128 || c.block == 192 || c.block == 256) && c.blocks_per_cu >= 1 &&
-                      c.blocks_per_cu <= 64 && c.items_per_dequeue >= 1 && c.samples_per_item >= 1,
+                      c.blocks_per_cu <= 64 && c.path_slots >= 4096 && c.path_slots <= (1 << 26),
                   RZK_E_INVALID, "bad launch shape");
         s->scene = scene;
         s->cl = cl;
@@ -434,26 +425,19 @@ int rzk_server_create(int device, const rzk_scene* scene, const rzk_clusters* cl
                   "bad guide parameters");
         s->t0 = std::chrono::steady_clock::now();
         s->arena.init((size_t)prm->memory_budget);
-        s->sv = scene_view(scene);
-        int64_t threads = (int64_t)s->grid() * c.block;
-        int64_t extra = 2 * (int64_t)s->sv.stack4 - kLdsStack;   // closest_hit4 pairs
-        if (extra > 0) {
-            RZK_HIP(hipMalloc((void**)&s->spill, sizeof(int) * (size_t)(extra * threads)));
-            s->sv.spill = s->spill;
-            s->sv.spill_pitch = threads;
-        } else {
-            s->sv.spill = nullptr;
-            s->sv.spill_pitch = 0;
-        }
-        RZK_HIP(hipMalloc((void**)&s->d_counter, sizeof(unsigned long long)));
+        RZK_HIP(hipMalloc((void**)&s->d_jobs, sizeof(JobGpu) * kJobRing));
+        RZK_HIP(hipMalloc((void**)&s->d_done, sizeof(unsigned int) * kJobRing));
+        RZK_HIP(hipMemset(s->d_done, 0, sizeof(unsigned int) * kJobRing));
+        RZK_HIP(hipHostMalloc((void**)&s->h_jobs, sizeof(JobGpu) * kJobRing, hipHostMallocDefault));
+        RZK_HIP(hipHostMalloc((void**)&s->h_done, 2 * sizeof(unsigned int) * kJobRing, hipHostMallocDefault));
         for (int k = 0; k < 2; ++k) {
-            RZK_HIP(hipMalloc((void**)&s->d_hdr[k], sizeof(ServeHeader)));
-            RZK_HIP(hipHostMalloc((void**)&s->h_hdr[k], sizeof(ServeHeader), hipHostMallocDefault));
-            RZK_HIP(hipEventCreate(&s->ev_start[k]));
-            RZK_HIP(hipEventCreate(&s->ev_end[k]));
+            RZK_HIP(hipEventCreate(&s->ev_step[k]));
+            RZK_HIP(hipEventCreate(&s->ev_t0[k]));
         }
-        s->ensure_job_cap(64);
-        s->capture();
+        s->wf = wf_create(s->ctx, scene, c.path_slots, c.block, c.blocks_per_cu);
+        wf_reset(s->wf);
+        s->publish();
+        RZK_HIP(hipStreamSynchronize(s->ctx->stream));
         rzk_server* raw = s.get();
         s->worker = std::thread([raw] { raw->run(); });
     });
@@ -479,16 +463,9 @@ int rzk_server_submit(rzk_server* s, const rzk_request* r, int64_t* ticket) {
                   RZK_E_INVALID, "bad request");
         q->L = guide_layout(q->W, q->H, s->gp.tile, s->gp.map_resolution, s->cl->C);
         q->bytes_img = sizeof(float) * 3 * (size_t)q->W * q->H;
-        RZK_CHECK(request_bytes(q->L, q->bytes_img) <= s->arena.size, RZK_E_OOM,
+        q->bytes_samples = q->bytes_img * (size_t)r->spp;
+        RZK_CHECK(request_bytes(q->L, q->bytes_samples, q->bytes_img) <= s->arena.size, RZK_E_OOM,
                   "request needs more device memory than the whole budget");
-        int64_t npx = (int64_t)q->W * q->H;
-        q->pix_blocks = (npx + 63) / 64;
-        q->S = std::min(s->prm.launch.samples_per_item, r->spp);
-        q->samp_blocks = (r->spp + q->S - 1) / q->S;
-        q->total_items = q->pix_blocks * q->samp_blocks;
-        int strata = 1;
-        while ((strata + 1) * (strata + 1) <= r->spp) ++strata;
-        q->strata = strata * strata == r->spp ? strata : 0;
         RZK_HIP(hipEventCreateWithFlags(&q->done_ev, hipEventDisableTiming));
         std::lock_guard<std::mutex> lk(s->mu);
         RZK_CHECK(s->error.empty(), RZK_E_HIP, s->error);
@@ -517,7 +494,7 @@ int rzk_server_wait(rzk_server* s, int64_t ticket, double timeout_s, float* out_
         st->submitted = q->t_submit;
         st->admitted = q->t_admit;
         st->finished = q->t_finish;
-        st->rounds = q->rounds;
+        st->rounds = q->steps;
         st->pad = 0;
         st->device_bytes = q->held_bytes();
     }

@@ -25,7 +25,10 @@ struct WfEngine {
     std::vector<void*> bufs;
     WfView v{};
     WfJobs* d_jh = nullptr;
-    WfJobs* h_jh = nullptr;
+    WfJobs* h_jh = nullptr;                 // 2 pinned staging entries, reused after their copy completed
+    hipEvent_t jh_ev[2] = {nullptr, nullptr};
+    int jh_next = 0;
+    int parity = 0;                         // queue parity of the next iteration
     unsigned long long* h_ctrl = nullptr;   // 2 x {handed out, active}
     int* h_fc = nullptr;                    // initial free counts {P, 0}
     int* all_slots = nullptr;               // 0 .. P-1
@@ -47,6 +50,7 @@ struct WfEngine {
             if (ex[k]) (void)hipGraphExecDestroy(ex[k]);
             if (g[k]) (void)hipGraphDestroy(g[k]);
             if (ev[k]) (void)hipEventDestroy(ev[k]);
+            if (jh_ev[k]) (void)hipEventDestroy(jh_ev[k]);
         }
         for (void* p : bufs) (void)hipFree(p);
         if (h_jh) (void)hipHostFree(h_jh);
@@ -95,7 +99,8 @@ WfEngine* wf_create(rzk_ctx* ctx, const rzk_scene* scene, int64_t slots, int blo
     v.free_counts = e->alloc<int>(2);
     v.ctrl = e->alloc<unsigned long long>(2);
     e->d_jh = e->alloc<WfJobs>(1);
-    RZK_HIP(hipHostMalloc((void**)&e->h_jh, sizeof(WfJobs), hipHostMallocDefault));
+    RZK_HIP(hipHostMalloc((void**)&e->h_jh, 2 * sizeof(WfJobs), hipHostMallocDefault));
+    for (int k = 0; k < 2; ++k) RZK_HIP(hipEventCreateWithFlags(&e->jh_ev[k], hipEventDisableTiming));
     RZK_HIP(hipHostMalloc((void**)&e->h_ctrl, 4 * sizeof(unsigned long long), hipHostMallocDefault));
     RZK_HIP(hipHostMalloc((void**)&e->h_fc, 2 * sizeof(int), hipHostMallocDefault));
     e->h_fc[0] = (int)P;
@@ -140,6 +145,36 @@ void wf_destroy(WfEngine* e) { delete e; }
 
 int64_t wf_slots(const WfEngine* e) { return e->P; }
 
+void wf_reset(WfEngine* e) {
+    hipStream_t st = e->ctx->stream;
+    RZK_HIP(hipMemsetAsync(e->v.flags, 0, sizeof(int) * e->P, st));
+    RZK_HIP(hipMemsetAsync(e->v.counts, 0, sizeof(int) * 8, st));
+    RZK_HIP(hipMemsetAsync(e->v.ctrl, 0, sizeof(unsigned long long) * 2, st));
+    // every slot starts free, queued for the first iteration (parity 0)
+    RZK_HIP(hipMemcpyAsync(e->v.freeq[0], e->all_slots, sizeof(int) * e->P, hipMemcpyDeviceToDevice, st));
+    RZK_HIP(hipMemcpyAsync(e->v.free_counts, e->h_fc, 2 * sizeof(int), hipMemcpyHostToDevice, st));
+    e->parity = 0;
+}
+
+void wf_publish(WfEngine* e, const WfJobs& jobs) {
+    RZK_CHECK(jobs.job_cap > 0 && (jobs.job_cap & (jobs.job_cap - 1)) == 0, RZK_E_INVALID,
+              "job ring capacity must be a power of two");
+    hipStream_t st = e->ctx->stream;
+    int k = e->jh_next;
+    e->jh_next ^= 1;
+    RZK_HIP(hipEventSynchronize(e->jh_ev[k]));     // the staging entry's previous copy is done
+    e->h_jh[k] = jobs;
+    RZK_HIP(hipMemcpyAsync(e->d_jh, e->h_jh + k, sizeof(WfJobs), hipMemcpyHostToDevice, st));
+    RZK_HIP(hipEventRecord(e->jh_ev[k], st));
+}
+
+void wf_iterate(WfEngine* e, int n) {
+    for (int i = 0; i < n; ++i) {
+        RZK_HIP(hipGraphLaunch(e->ex[e->parity], e->ctx->stream));
+        e->parity ^= 1;
+    }
+}
+
 // Every path of the job table, to completion.  d_jobs[j].item_first is the
 // job's first path index; paths of a job are sample-major.  Returns the
 // number of iterations.
@@ -150,25 +185,20 @@ int64_t wf_run(WfEngine* e, const JobGpu* d_jobs, int64_t n_jobs, int64_t total_
     // and each such window frees every slot at least once
     const int64_t bound = (total_paths / e->P + 2) * (int64_t)(std::max(max_bounce, 0) + 3) + 16;
     hipStream_t st = e->ctx->stream;
-    RZK_HIP(hipStreamSynchronize(st));          // the pinned job header may be rewritten
-    *e->h_jh = WfJobs{d_jobs, n_jobs, total_paths, d_samples, d_work, d_job_done};
-    RZK_HIP(hipMemcpyAsync(e->d_jh, e->h_jh, sizeof(WfJobs), hipMemcpyHostToDevice, st));
-    RZK_HIP(hipMemsetAsync(e->v.flags, 0, sizeof(int) * e->P, st));
-    RZK_HIP(hipMemsetAsync(e->v.counts, 0, sizeof(int) * 8, st));
-    RZK_HIP(hipMemsetAsync(e->v.ctrl, 0, sizeof(unsigned long long) * 2, st));
-    // every slot starts free, queued for the first iteration (parity 0)
-    RZK_HIP(hipMemcpyAsync(e->v.freeq[0], e->all_slots, sizeof(int) * e->P, hipMemcpyDeviceToDevice, st));
-    RZK_HIP(hipMemcpyAsync(e->v.free_counts, e->h_fc, 2 * sizeof(int), hipMemcpyHostToDevice, st));
+    int64_t cap = 1;
+    while (cap < n_jobs) cap <<= 1;
+    wf_reset(e);
+    wf_publish(e, WfJobs{d_jobs, 0, n_jobs, cap, total_paths, d_samples, d_work, d_job_done});
     int64_t it = 0;
     for (;; ++it) {
-        int par = (int)(it & 1);
-        RZK_HIP(hipGraphLaunch(e->ex[par], st));
-        RZK_HIP(hipMemcpyAsync(e->h_ctrl + 2 * par, e->v.ctrl, 2 * sizeof(unsigned long long),
+        int k = (int)(it & 1);
+        wf_iterate(e, 1);
+        RZK_HIP(hipMemcpyAsync(e->h_ctrl + 2 * k, e->v.ctrl, 2 * sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, st));
-        RZK_HIP(hipEventRecord(e->ev[par], st));
+        RZK_HIP(hipEventRecord(e->ev[k], st));
         if (it == 0) continue;
         // counters after the previous iteration (one iteration stays queued)
-        int prev = par ^ 1;
+        int prev = k ^ 1;
         RZK_HIP(hipEventSynchronize(e->ev[prev]));
         unsigned long long handed = e->h_ctrl[2 * prev], active = e->h_ctrl[2 * prev + 1];
         if (handed >= (unsigned long long)total_paths && active == 0) break;

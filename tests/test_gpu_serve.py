@@ -73,7 +73,7 @@ def test_served_alone_equals_mix_equals_plain(ctx, resident):
         reqs.append((random_camera(rng, w, h), dict(spp=int(rng.integers(1, 10)), max_bounce=int(rng.integers(2, 6)),
                                                     direct_lighting=int(rng.integers(0, 2)), guide_mode=1,
                                                     seed=int(rng.integers(0, 2 ** 31)))))
-    srv = rz.Server(0, gs, cl, memory_budget=1 << 30, round_items=2000)
+    srv = rz.Server(0, gs, cl, memory_budget=1 << 30, launch=dict(path_slots=8192))
     cam0, kw0 = reqs[0]
     alone, st_alone = srv.wait(srv.submit(cam0, **kw0))
     tickets = [srv.submit(cam, **kw) for cam, kw in reqs]
@@ -84,22 +84,24 @@ def test_served_alone_equals_mix_equals_plain(ctx, resident):
         assert img.shape == ref.shape
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
         assert st["finished"] >= st["admitted"] >= st["submitted"] >= 0
-    # several requests shared rounds (continuous batching, not one at a time)
+    # the mix shared engine steps (continuous batching, not one at a time)
     stats = srv.stats()
     assert stats["finished"] == 8 and stats["rounds"] < sum(st["rounds"] for _, st in mixed) + st_alone["rounds"]
+    assert stats["items"] == sum(int(c[17]) * int(c[18]) * kw["spp"] for c, kw in reqs) + \
+        int(cam0[17]) * int(cam0[18]) * kw0["spp"]
     print(f"served: {stats}", flush=True)
     srv.close()
 
 
-def request_bytes(W, H, tile, res, C):
+def request_bytes(W, H, spp, tile, res, C):
     """device bytes a request needs at admission: guide tables, build
-    temporaries, running sums and image, each rounded to 256 B"""
+    temporaries, per-sample radiance and image, each rounded to 256 B"""
     a = lambda b: (b + 255) // 256 * 256
     S = -(-W // tile) * -(-H // tile)
     tmp = [4 * S, 4 * S, 24 * S, 16 * S, 84 * S, 4 * S, 4 * S, 12 * S * C, 4 * S * C, 12 * S * C, 4 * S * C,
            4 * S * res]
     tab = [4 * S * res * res, 4 * S * res * res, 4 * S * res, 4 * S]
-    return sum(map(a, tmp)) + sum(map(a, tab)) + 2 * 12 * W * H + 4 * 256
+    return a(sum(map(a, tmp))) + a(sum(map(a, tab))) + a(12 * W * H * spp) + a(12 * W * H)
 
 
 def test_served_memory_budget_admission(ctx, resident):
@@ -109,8 +111,8 @@ def test_served_memory_budget_admission(ctx, resident):
     sc, gs, vp, ls, cl = resident
     rng = np.random.default_rng(6)
     cams = [random_camera(rng, 96, 80) for _ in range(4)]
-    budget = request_bytes(96, 80, 16, 32, ls.info().num_clusters) + 1000   # less than two requests
-    srv = rz.Server(0, gs, cl, memory_budget=budget, round_items=64)
+    budget = request_bytes(96, 80, 2, 16, 32, ls.info().num_clusters) + 1000   # less than two requests
+    srv = rz.Server(0, gs, cl, memory_budget=budget, launch=dict(path_slots=4096))
     kw = dict(spp=2, max_bounce=3, direct_lighting=1, guide_mode=1, seed=3)
     tickets = [srv.submit(c, **kw) for c in cams]
     out = [srv.wait(t, timeout=120.0) for t in tickets]
@@ -133,7 +135,7 @@ def test_served_concurrent_submitters(ctx, resident):
     rng = np.random.default_rng(7)
     reqs = [(random_camera(rng, int(rng.integers(16, 90)), int(rng.integers(16, 70))),
              dict(spp=int(rng.integers(1, 5)), max_bounce=3, seed=i)) for i in range(12)]
-    srv = rz.Server(0, gs, cl, memory_budget=1 << 29, round_items=128)
+    srv = rz.Server(0, gs, cl, memory_budget=1 << 29, step_iterations=1)
     results = {}
 
     def client(part):
