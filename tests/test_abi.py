@@ -53,3 +53,13 @@ def test_scene_generators():
     assert 18_000 < r.ntri <= 20_000
     kinds = set(r.materials[:, 0].astype(int).tolist())
     assert kinds == {0, 1}
+
+
+def test_cpp_adapter_builds_against_the_abi():
+    """the C++ Aggregate adapter compiles and links against librzk.so with a
+    plain C++ compiler (no HIP headers): the boundary is the C ABI only"""
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run(["make", "-s", "-C", os.path.join(here, "cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert os.path.exists(os.path.join(here, "cpp", "adapter_test"))
