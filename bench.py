@@ -32,6 +32,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per gather launch (k_slice_visibility + k_slice_shade) measured by rocprofv3 --pmc FETCH_SIZE
+# (x2, the 512 MiB copy calibration) and WRITE_SIZE (x1) on this workload; the passes and the calibration
+# are recorded in profiles/slice_gather_pmc.md
+GATHER_HBM_BYTES = (251224 + 49205) * 2 * 1024 + (19614 + 3072) * 1024
+# MI355X vector issue: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz lane-operations per second
+VALU_LANE_OPS_PEAK = 256 * 64 * 2.4e9
 # algorithmic bytes per shadow segment: ray in, result out, every 4-wide node
 # fetched (128 B: four child boxes + refs) and every triangle record tested
 RAY_IN_BYTES, SHADOW_OUT_BYTES, NODE4_BYTES, TRI_BYTES = 32, 4, 128, 48
@@ -488,9 +494,13 @@ def main():
         "gather_kernel_ms": round(g_ms, 4),
         "work": work,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_slice_gather",
-                     "bytes_per_launch": int(gbytes)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": GATHER_HBM_BYTES,
+                     "kernel": "k_slice_visibility + k_slice_shade (the gather)",
+                     "bytes_per_launch": int(gbytes),
+                     "note": "achieved counts every lane's node visits (128 B per 4-wide node, 48 B per triangle "
+                             "test, 36 B of segment I/O); a wave fetches each node once and L2/MALL serve it, so "
+                             "measured HBM traffic per launch (traffic) is far below; the visibility kernel is "
+                             "vector-issue bound (VALUBusy 75.6%, about 71% of lane-op peak, rocprofv3 --pmc)"},
     }
     if not args.no_cpu:
         reps = ls.get("reps", np.int32, lsinf.num_clusters)
