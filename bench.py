@@ -13,7 +13,8 @@ segment per (pixel, representative) pair).
 value = shadow Mrays/s over the step (reduced-matrix + gather segments / step
 time).  The dominant kernel (the fused gather) is also timed on its own with
 HIP events on the context stream, and its roofline is reported against HBM
-peak with the algorithmic bytes of SURVEY's traversal model.  cpu_baseline is
+peak with the algorithmic bytes of the traversal byte model (36 B of segment
+I/O, 128 B per visited 4-wide node, 48 B per tested triangle).  cpu_baseline is
 the oracle (a C restatement of the reference's gather) on a bounded sample of
 the same pairs, on the host's cores.
 """
@@ -409,8 +410,8 @@ def main():
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design "
-              f"described in DESIGN.md, not a measured configuration.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a "
+              f"design, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "guided":
         return run_guided(args)
