@@ -236,7 +236,8 @@ int rzk_lightslice_count_work(rzk_lightslice *ls, int64_t *out);
  *  "centers" (i32), "center_w" (f32), "assign" (V i32), "reps" (i32 per kept cluster),
  *  "rep_w" (f32), "rows_recv" (rows*21 f32), "rows_mat" (rows i32),
  *  "slice_pdf" (slices*clusters f32), "slice_upper" (slices*clusters i32),
- *  "slice_wi" (slices*clusters*3 f32), "maps" (slices*res*res f32), "image" (W*H*3 f32)
+ *  "slice_wi" (slices*clusters*3 f32), "maps" (slices*res*res f32), "image" (W*H*3 f32),
+ *  "recv" (W*H*21 f32), "recv_mat" (W*H i32), "hits" (W*H x {t, beta, gamma, tri as i32 bits})
  * bytes must equal the array size. */
 int rzk_lightslice_get(const rzk_lightslice *ls, const char *name, void *dst, int64_t bytes);
 

@@ -60,54 +60,75 @@ __global__ __launch_bounds__(256) void k_mag_transpose(int rows, int64_t cols, c
     }
 }
 
-// out[i] = Σ_j x[i][j] in order j = 0, 1, ... (one float add after another,
-// matrix_c.h:236-245 / light_transport_matrix.cc:99-100).  One wave per row:
-// the whole wave streams 1024-element chunks into LDS, one lane performs the
-// ordered additions.
-__global__ __launch_bounds__(64) void k_ordered_row_sums(int rows, int64_t cols, const float* __restrict__ x,
-                                                         float* __restrict__ out) {
+// Ordered sums (matrix_c.h:236-245 / light_transport_matrix.cc:99-100): one
+// float add after another, j = 0, 1, ..., for the rows of x (out[i], blocks
+// 0 .. rows-1) and, in block `rows`, for the vector x1 (out1[0]); every sum
+// has `cols` terms.  One wave per sum.  The chain of dependent adds is the
+// bound, so every lane runs the same chain (no lane-0 bottleneck) on values
+// it reads from LDS with wave-uniform addresses (broadcast reads, 16 float4
+// ahead of the adds), while the global loads of the next 1,024-value chunk
+// are in flight.
+__global__ __launch_bounds__(64) void k_ordered_sums(int rows, int64_t cols, const float* __restrict__ x,
+                                                     float* __restrict__ out, const float* __restrict__ x1,
+                                                     float* __restrict__ out1) {
     __shared__ float4 buf[2][256];
-    int i = blockIdx.x;
-    if (i >= rows) return;
-    const float* row = x + (int64_t)i * cols;
-    int lane = threadIdx.x;
-    float acc = 0.f;
-    int64_t nchunk = (cols + 1023) / 1024;
-    auto stage = [&](int64_t c, int slot) {
+    const int i = blockIdx.x;
+    if (i > rows || (i == rows && !x1)) return;
+    const float* row = i < rows ? x + (int64_t)i * cols : x1;
+    const int lane = threadIdx.x;
+    const int64_t nchunk = (cols + 1023) / 1024;
+    float4 pre[4];
+    auto fetch = [&](int64_t c) {
+#pragma unroll
         for (int q = 0; q < 4; ++q) {
             int64_t e = c * 1024 + (int64_t)(q * 64 + lane) * 4;
-            float4 v;
-            v.x = e < cols ? row[e] : 0.f;
-            v.y = e + 1 < cols ? row[e + 1] : 0.f;
-            v.z = e + 2 < cols ? row[e + 2] : 0.f;
-            v.w = e + 3 < cols ? row[e + 3] : 0.f;
-            buf[slot][q * 64 + lane] = v;
+            pre[q].x = e < cols ? row[e] : 0.f;
+            pre[q].y = e + 1 < cols ? row[e + 1] : 0.f;
+            pre[q].z = e + 2 < cols ? row[e + 2] : 0.f;
+            pre[q].w = e + 3 < cols ? row[e + 3] : 0.f;
         }
     };
-    stage(0, 0);
-    __syncthreads();
-    for (int64_t c = 0; c < nchunk; ++c) {
-        int slot = (int)(c & 1);
-        if (c + 1 < nchunk) stage(c + 1, slot ^ 1);
-        if (lane == 0) {
-            int64_t n = cols - c * 1024 < 1024 ? cols - c * 1024 : 1024;
-            int64_t q = 0;
-            for (; q + 4 <= n; q += 4) {
-                float4 v = buf[slot][q >> 2];
-                acc += v.x;
-                acc += v.y;
-                acc += v.z;
-                acc += v.w;
-            }
-            for (; q < n; ++q) {
-                float4 v = buf[slot][q >> 2];
-                float w = (q & 3) == 0 ? v.x : (q & 3) == 1 ? v.y : (q & 3) == 2 ? v.z : v.w;
-                acc += w;
-            }
-        }
+    auto put = [&](int slot) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) buf[slot][q * 64 + lane] = pre[q];
+    };
+    float acc = 0.f;
+    if (nchunk > 0) {
+        fetch(0);
+        put(0);
         __syncthreads();
     }
-    if (lane == 0) out[i] = acc;
+    for (int64_t c = 0; c < nchunk; ++c) {
+        const int slot = (int)(c & 1);
+        if (c + 1 < nchunk) fetch(c + 1);
+        const int64_t n = cols - c * 1024 < 1024 ? cols - c * 1024 : 1024;
+        if (n == 1024) {
+            for (int q = 0; q < 256; q += 16) {
+                float4 v[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) v[k] = buf[slot][q + k];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    acc += v[k].x;
+                    acc += v[k].y;
+                    acc += v[k].z;
+                    acc += v[k].w;
+                }
+            }
+        } else {
+            for (int64_t e = 0; e < n; ++e) {
+                float4 v = buf[slot][e >> 2];
+                int k = (int)(e & 3);
+                acc += k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+            }
+        }
+        if (c + 1 < nchunk) put(slot ^ 1);
+        __syncthreads();
+    }
+    if (lane == 0) {
+        if (i < rows) out[i] = acc;
+        else out1[0] = acc;
+    }
 }
 
 // per column: v_j = Σ_i m_ij * u_i, i sequential (matrix_c.h:224-233 on Rt)
@@ -121,7 +142,7 @@ __global__ void k_col_dot_u(int rows, int64_t cols, const float* __restrict__ R,
     v[j] = acc;
 }
 
-// alpha_j = norm_j * c0 - v_j, c0 = Σ_j norm_j in order (k_ordered_row_sums, one row)
+// alpha_j = norm_j * c0 - v_j, c0 = Σ_j norm_j in order (k_ordered_sums, block `rows`)
 __global__ void k_alphas(int64_t cols, const float* __restrict__ norms, const float* __restrict__ c0,
                          const float* __restrict__ v, float* __restrict__ alphas) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -252,9 +273,10 @@ void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, 
                        d_mags);
     hipLaunchKernelGGL(k_mag_transpose, dim3(nblk(cols, 64), nblk(rows, 64)), dim3(256), 0, ctx->stream, rows, cols,
                        d_R, d_mT);
-    hipLaunchKernelGGL(k_ordered_row_sums, dim3(rows), dim3(64), 0, ctx->stream, rows, cols, d_mT, d_u);
+    // u_i = Σ_j m_ij (rows of m^T) and c0 = Σ_j norm_j: rows + 1 independent ordered sums in one launch
+    hipLaunchKernelGGL(k_ordered_sums, dim3(rows + 1), dim3(64), 0, ctx->stream, rows, cols, d_mT, d_u, d_norms,
+                       d_c0);
     hipLaunchKernelGGL(k_col_dot_u, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, rows, cols, d_R, d_u, d_v);
-    hipLaunchKernelGGL(k_ordered_row_sums, dim3(1), dim3(64), 0, ctx->stream, 1, cols, d_norms, d_c0);
     hipLaunchKernelGGL(k_alphas, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, cols, d_norms, d_c0, d_v,
                        d_alphas);
     RZK_HIP(hipGetLastError());

@@ -506,7 +506,7 @@ int rzk_render_vpl(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const fl
         RZK_HIP(hipMemcpyAsync(order.p, ord.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
         SceneView sv = scene_view(s);
         launch_camera_image_rays(ctx, camera, W, H, rays.p);
-        launch_trace_closest(ctx, sv, rays.p, n, 1e-3f, 1e6f, hits.p, nullptr);
+        launch_trace_closest_image(ctx, sv, rays.p, W, H, 1e-3f, 1e6f, hits.p, nullptr);
         launch_fill_receivers(ctx, sv, rays.p, hits.p, n, recv.p, mat.p, light.p);
         launch_gather_fused(ctx, sv, recv.p, mat.p, n, order.p, v->d_vpl, v->d_mat, v->n, clamp_mode, clamp, tmin,
                             n_emitted, img.p, nullptr);
@@ -580,7 +580,7 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         hipStream_t st = ctx->stream;
         // primary hits for every pixel (camera.cc:52-65 pixel centres)
         launch_camera_image_rays(ctx, camera, ls->W, ls->H, ls->rays.p);
-        launch_trace_closest(ctx, sv, ls->rays.p, n, 1e-3f, 1e6f, ls->hits.p, nullptr);
+        launch_trace_closest_image(ctx, sv, ls->rays.p, ls->W, ls->H, 1e-3f, 1e6f, ls->hits.p, nullptr);
         launch_fill_receivers(ctx, sv, ls->rays.p, ls->hits.p, n, ls->recv.p, ls->mat.p, ls->light.p);
         hipLaunchKernelGGL(k_pick_rows, dim3((S + 63) / 64), dim3(64), 0, st, ls->recv.p, ls->mat.p, ls->row_pix.p, S,
                            ls->rows_recv.p, ls->rows_mat.p);
@@ -847,6 +847,7 @@ int rzk_lightslice_get(const rzk_lightslice* ls, const char* name, void* dst, in
         else if (nm == "image") { src = ls->image.p; need = 12 * n; }
         else if (nm == "recv") { src = ls->recv.p; need = 4 * (int64_t)RZK_RECV_FLOATS * n; }
         else if (nm == "recv_mat") { src = ls->mat.p; need = 4 * n; }
+        else if (nm == "hits") { src = ls->hits.p; need = 16 * n; }
         else throw RzkError{RZK_E_INVALID, "unknown array name " + nm};
         RZK_CHECK(bytes == need, RZK_E_INVALID, "size mismatch for " + nm + ": need " + std::to_string(need));
         if (need == 0) return;

@@ -142,6 +142,10 @@ void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries = 0);
 // kernel launchers (traverse.hip, shade.hip, cluster.hip)
 void launch_trace_closest(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int64_t n,
                           float tmin, float tmax, float4* d_hits, int* d_stats);
+// closest hits of a width x height row-major image of rays, one wave per 8x8 tile
+// (same results as launch_trace_closest)
+void launch_trace_closest_image(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int width, int height,
+                                float tmin, float tmax, float4* d_hits, int* d_stats);
 void launch_trace_any(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int64_t n,
                       float tmin, float tmax, uint8_t* d_out, int* d_stats);
 void launch_trace_segments(rzk_ctx* ctx, const SceneView& sv, const float* d_p, const float* d_q,

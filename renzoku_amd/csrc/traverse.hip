@@ -40,6 +40,107 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView sv, con
     record_stats<kStats>(d_stats, i, st);
 }
 
+// 64-bit words of one wave's stack of the tiled closest-hit walk: depth lane
+// masks and depth node refs.  One pending entry per tree level at most.
+__host__ __device__ constexpr size_t tile_stack_u64(int depth) { return (size_t)depth + ((size_t)depth + 1) / 2; }
+
+// Closest hits of a W x H image of rays (row-major), one wave per 8x8 pixel
+// tile.  The wave walks the reference tree in the reference's depth-first order
+// (aggregate_bvh.cc:402-448: both child boxes tested against the current tmax,
+// right child first, left deferred) with one shared schedule: the node index is
+// wave-uniform, so nodes and triangles are fetched once per wave through the
+// scalar cache, and the wave stack holds (node, lanes) pairs.  A lane takes
+// part only in the nodes its own walk would visit, with its own tmax, in the
+// same order (the depth-first order of the whole tree restricted to the lane's
+// visits is the lane's own order), so t, beta, gamma and the triangle id,
+// ties included, are those of k_trace_closest.  LDS per wave: depth masks
+// (uint64) followed by depth node refs (tile_stack_u64 words).
+template <int kStats>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_closest_tiles(SceneView sv, const float* __restrict__ rays,
+                                                                     int width, int height, float tmin,
+                                                                     float tmax0, float4* __restrict__ hits,
+                                                                     int* __restrict__ d_stats) {
+    extern __shared__ unsigned long long s_wave[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long* wmask = s_wave + (size_t)wave * tile_stack_u64(sv.depth);
+    int* wnode = reinterpret_cast<int*>(wmask + sv.depth);
+    const int tiles_x = (width + 7) / 8;
+    const int64_t ntiles = (int64_t)tiles_x * ((height + 7) / 8);
+    const int64_t tile = (int64_t)blockIdx.x * (kTraceBlock / 64) + wave;
+    int64_t pix = -1;
+    if (tile < ntiles) {
+        int px = (int)(tile % tiles_x) * 8 + (lane & 7), py = (int)(tile / tiles_x) * 8 + (lane >> 3);
+        if (px < width && py < height) pix = (int64_t)py * width + px;
+    }
+    TravStats st;
+    Ray r = ray_from_od(mk(0.f, 0.f, 1.f), mk(0.f, 0.f, 1.f));
+    if (pix >= 0) {
+        const float* rp = rays + RZK_RAY_FLOATS * pix;
+        r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
+    }
+    float tmax = tmax0, tb = 0.f, bb = 0.f, gb = 0.f;
+    int best = -1;
+    bool live = pix >= 0 && box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]),
+                                    mk(sv.rbmax[0], sv.rbmax[1], sv.rbmax[2]), r, tmin, tmax);
+    unsigned long long mask = __ballot(live);
+    if (mask != 0ull) {
+        int node = sv.root_ref;
+        int sp = 0;
+        for (;;) {
+            node = __builtin_amdgcn_readfirstlane(node);
+            const bool in = (mask >> lane) & 1ull;
+            bool descend = false;
+            if (node < 0) {
+                const kconst float* tf = (const kconst float*)(sv.tris + (~node));
+                v3 p0 = mk(tf[0], tf[1], tf[2]), e1 = mk(tf[4], tf[5], tf[6]), e2 = mk(tf[8], tf[9], tf[10]);
+                int id = __float_as_int(tf[3]);
+                if (in) {
+                    st.tris++;
+                    float t, b, g;
+                    if (tri_hit(p0, e1, e2, r, tmin, tmax, t, b, g)) {
+                        best = id;
+                        tmax = t; tb = t; bb = b; gb = g;
+                    }
+                }
+            } else {
+                const kconst float* nf = (const kconst float*)(sv.nodes + node);
+                const kconst int* ni = (const kconst int*)(sv.nodes + node);
+                v3 llo = mk(nf[0], nf[1], nf[2]), lhi = mk(nf[3], nf[4], nf[5]);
+                v3 rlo = mk(nf[6], nf[7], nf[8]), rhi = mk(nf[9], nf[10], nf[11]);
+                int left = ni[12], right = ni[13];
+                if (in) st.nodes++;
+                bool hl = in && box_hit(llo, lhi, r, tmin, tmax);
+                bool hr = in && box_hit(rlo, rhi, r, tmin, tmax);
+                unsigned long long ml = __ballot(hl), mr = __ballot(hr);
+                if (mr != 0ull) {
+                    if (ml != 0ull) {
+                        wmask[sp] = ml;
+                        wnode[sp] = left;
+                        ++sp;
+                    }
+                    node = right;
+                    mask = mr;
+                    descend = true;
+                } else if (ml != 0ull) {
+                    node = left;
+                    mask = ml;
+                    descend = true;
+                }
+            }
+            if (!descend) {
+                if (sp == 0) break;
+                --sp;
+                node = wnode[sp];
+                mask = wmask[sp];
+            }
+        }
+    }
+    if (pix >= 0)
+        hits[pix] = best >= 0 ? make_float4(tb, bb, gb, __int_as_float(best))
+                              : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+    record_stats<kStats == 2 ? 2 : 0>(d_stats, pix, st);
+}
+
 template <int kStats>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_any(SceneView sv, const float* __restrict__ rays,
                                                            int64_t n, float tmin, float tmax,
@@ -148,6 +249,22 @@ void launch_trace_closest(rzk_ctx* ctx, const SceneView& sv, const float* d_rays
     else
         hipLaunchKernelGGL(k_trace_closest<0>, dim3(grid_for(n, kTraceBlock)), dim3(kTraceBlock), lds,
                            ctx->stream, sv, d_rays, n, tmin, tmax, d_hits, d_stats);
+    RZK_HIP(hipGetLastError());
+}
+
+void launch_trace_closest_image(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int width, int height,
+                                float tmin, float tmax, float4* d_hits, int* d_stats) {
+    const int64_t tiles = (int64_t)((width + 7) / 8) * ((height + 7) / 8);
+    if (tiles <= 0) return;
+    const int waves = kTraceBlock / 64;
+    size_t lds = sizeof(unsigned long long) * waves * tile_stack_u64(sv.depth);
+    unsigned grid = (unsigned)((tiles + waves - 1) / waves);
+    if (d_stats)
+        hipLaunchKernelGGL(k_trace_closest_tiles<2>, dim3(grid), dim3(kTraceBlock), lds, ctx->stream, sv, d_rays,
+                           width, height, tmin, tmax, d_hits, d_stats);
+    else
+        hipLaunchKernelGGL(k_trace_closest_tiles<0>, dim3(grid), dim3(kTraceBlock), lds, ctx->stream, sv, d_rays,
+                           width, height, tmin, tmax, d_hits, d_stats);
     RZK_HIP(hipGetLastError());
 }
 

@@ -153,6 +153,26 @@ def test_render_vpl_parity_cornell(ctx, scenes_gpu, cornell_vpls):
     assert img.mean() > 0
 
 
+def test_render_vpl_ragged_image(ctx, scenes_gpu, cornell_vpls):
+    """an image whose sides are not multiples of the 8x8 wave tiles (partial
+    tiles in the primary-hit walk and the gather)"""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes
+    sc, gs, os_ = scenes_gpu["cornell"]
+    vpl, vmat, ne, vp = cornell_vpls
+    cam = np.array(sc.camera, np.float32).copy()
+    W, H = 61, 45
+    cam[17], cam[18] = W, H
+    img = rz.render_vpl(ctx, gs, vp, cam, 1, 0.01, 1e-3, ne)
+    px, py = scenes.pixel_grid(W, H)
+    rays = oracle.camera_rays(cam, px, py)
+    t, b, g, tri = os_.trace_closest(rays)
+    recv, m, _ = os_.fill_receivers(rays, t, b, g, tri)
+    ref = os_.gather(recv, m, vpl, vmat, 1, 0.01, 1e-3, ne, nthreads=8).reshape(H, W, 3)
+    assert img.shape == (H, W, 3)
+    assert rel_l2(img, ref) <= 1e-6
+
+
 @pytest.fixture(scope="module")
 def room_setup(ctx, scenes_gpu):
     import renzoku_amd as rz
@@ -183,6 +203,18 @@ def test_reduced_matrix_alphas_assign_parity(ctx, room_setup):
     w = rng.uniform(0.5, 50, 40).astype(np.float32)
     asg = rz.cluster_assign(ctx, oR, centers, w)
     assert np.array_equal(asg, oracle.cluster_assign(oR, centers, w, nthreads=8))
+
+
+@pytest.mark.parametrize("cols,rows", [(1, 1), (3, 2), (1023, 3), (1024, 1), (2051, 5), (5000, 17)])
+def test_alphas_ragged_shapes(ctx, cols, rows):
+    """alphas and magnitudes bit-exact for sums whose length is not a multiple
+    of the 1,024-value chunks (or of 4) of the ordered-sum kernel"""
+    import renzoku_amd as rz
+    rng = np.random.default_rng(cols * 31 + rows)
+    R = rng.uniform(0, 3, (cols, rows, 3)).astype(np.float32) * (rng.uniform(size=(cols, rows, 1)) < 0.8)
+    a, mags = rz.alphas(ctx, R)
+    assert np.array_equal(a.view(np.uint32), oracle.alphas(R).view(np.uint32))
+    assert np.array_equal(mags.view(np.uint32), oracle.column_magnitudes(R).view(np.uint32))
 
 
 def vplgen_rows(w, h, tile):
@@ -220,6 +252,12 @@ def test_lightslice_pipeline_parity(ctx, room_setup):
     pxa, pya = scenes.pixel_grid(W, H)
     prays = oracle.camera_rays(sc.camera, pxa, pya)
     t, b, g, tri = os_.trace_closest(prays)
+    # the primary hits (one wave per 8x8 tile walking the reference tree) are the oracle's, bit for bit
+    hits = ls.get("hits", np.float32, W * H * 4).reshape(-1, 4)
+    assert np.array_equal(hits[:, 3].view(np.int32), tri)
+    ok = tri >= 0
+    for k, ref in enumerate((t, b, g)):
+        assert np.array_equal(hits[ok, k].view(np.uint32), ref[ok].view(np.uint32))
     recv, m, _ = os_.fill_receivers(prays, t, b, g, tri)
     vsub = (vpl[reps] * 1.0).astype(np.float32)
     vsub[:, 18:21] = (vpl[reps, 18:21] * rep_w[:, None]).astype(np.float32)
