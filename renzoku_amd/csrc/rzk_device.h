@@ -293,30 +293,39 @@ __device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r,
         const float hya[4] = {nf[16], nf[17], nf[18], nf[19]}, hza[4] = {nf[20], nf[21], nf[22], nf[23]};
         const int rfa[4] = {ni[24], ni[25], ni[26], ni[27]};
         if (live) st.nodes++;
+        // the four child tests first, as straight-line code: every lane evaluates
+        // them (idle lanes hold finite rays; empty slots hold finite zero boxes and
+        // are masked out below), so the node arrives in one scalar round trip and
+        // the four tests interleave
+        bool box[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            box[c] = box_hit(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax);
+        // lanes still walking, as a wave mask (refreshed after a leaf's tests)
+        unsigned long long lm = __builtin_amdgcn_ballot_w64(live);
         int next = -1;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             int ref = rfa[c];
-            if (ref == kEmptyRef) break;
-            bool hc = live && box_hit(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax);
-            if (__ballot(hc) == 0ull) continue;
+            if (ref == kEmptyRef || (__builtin_amdgcn_ballot_w64(box[c]) & lm) == 0ull) continue;
             if (ref < 0) {
                 const kconst float* tf = (const kconst float*)(sv.tris4 + (~ref));
                 v3 p0 = mk(tf[0], tf[1], tf[2]), e1 = mk(tf[4], tf[5], tf[6]), e2 = mk(tf[8], tf[9], tf[10]);
-                if (hc) {
+                if (box[c] & live) {
                     st.tris++;
                     if (tri_hit(p0, e1, e2, r, tmin, tmax, t, b, g)) {
                         hit = true;
                         live = false;
                     }
                 }
+                lm = __builtin_amdgcn_ballot_w64(live);
             } else if (next < 0) {
                 next = ref;
             } else {
                 wstack[sp++] = ref;
             }
         }
-        if (__ballot(live) == 0ull) return hit;
+        if (lm == 0ull) return hit;
         if (next >= 0) {
             node = next;
         } else {

@@ -76,14 +76,17 @@ RZK_HD Ray ray_ctor(v3 o, v3 d_in) {
 // slab values are never NaN (inv_d is finite and non-zero, boxes and origins
 // are finite), so fminf/fmaxf (v_min3/v_max3 on gfx950) pick the same values
 // as the reference's std::min/std::max up to the sign of a zero, which no
-// comparison below can see: the test's outcome is identical.
+// comparison below can see: the test's outcome is identical.  The reference
+// also requires tmax >= tminbox and tmaxbox >= tmin; with tminbox >= tmin and
+// tmaxbox <= tmax (no NaN), both follow from tminbox <= tmaxbox, so the one
+// comparison decides.
 RZK_HD bool box_hit(v3 bmin, v3 bmax, const Ray& r, float tmin, float tmax) {
     float x0 = bmin.x * r.inv_d.x + r.org_inv_d.x, x1 = bmax.x * r.inv_d.x + r.org_inv_d.x;
     float y0 = bmin.y * r.inv_d.y + r.org_inv_d.y, y1 = bmax.y * r.inv_d.y + r.org_inv_d.y;
     float z0 = bmin.z * r.inv_d.z + r.org_inv_d.z, z1 = bmax.z * r.inv_d.z + r.org_inv_d.z;
     float tminbox = fmaxf(fmaxf(tmin, fminf(x0, x1)), fmaxf(fminf(y0, y1), fminf(z0, z1)));
     float tmaxbox = fminf(fminf(tmax, fmaxf(x0, x1)), fminf(fmaxf(y0, y1), fmaxf(z0, z1)));
-    return (tminbox <= tmaxbox) && (tmax >= tminbox && tmaxbox >= tmin);
+    return tminbox <= tmaxbox;
 }
 
 // box_hit that also returns the entry distance (max of tmin and the slab entries)
@@ -94,7 +97,7 @@ RZK_HD bool box_enter(v3 bmin, v3 bmax, const Ray& r, float tmin, float tmax, fl
     float tminbox = fmaxf(fmaxf(tmin, fminf(x0, x1)), fmaxf(fminf(y0, y1), fminf(z0, z1)));
     float tmaxbox = fminf(fminf(tmax, fmaxf(x0, x1)), fminf(fmaxf(y0, y1), fmaxf(z0, z1)));
     tenter = tminbox;
-    return (tminbox <= tmaxbox) && (tmax >= tminbox && tmaxbox >= tmin);
+    return tminbox <= tmaxbox;     // the other two conditions follow (box_hit)
 }
 
 // Triangle::hit, triangle.cc:158-232, on the precomputed edges p01 = p0 - p1,

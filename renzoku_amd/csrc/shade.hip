@@ -160,6 +160,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_gather_fused(SceneView sv, cons
 // A wave holds one row and 64 VPLs that are consecutive in Morton order
 // (k = vorder[kk]), so its 64 segments leave one receiver in a narrow cone.
 // Outputs go to the column-major position of the original VPL index.
+// kCount: traversal work added to d_stats (the counting pass); otherwise the
+// counters are dead code.
+template <bool kCount>
 __global__ __launch_bounds__(kShadeBlock) void k_reduced_matrix(SceneView sv, const float* __restrict__ recv,
                                                                const int* __restrict__ rmat, int rows,
                                                                const int* __restrict__ vorder,
@@ -202,7 +205,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_reduced_matrix(SceneView sv, co
         R[3 * o] = rv; R[3 * o + 1] = g; R[3 * o + 2] = b;
         V[o] = vis;
     }
-    if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
+    if (kCount) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
 // Pixel of lane `lane` of 8x8 tile `tile` (-1 outside the image)
@@ -218,6 +221,7 @@ __device__ __forceinline__ int64_t tile_pixel(int64_t tile, int lane, int width,
 // representatives in order and records a 64-bit blocked mask per
 // representative: bits[c * ntiles + tile].  Only the segment endpoints are
 // live here, which keeps the traversal kernel small (high occupancy).
+template <bool kCount>
 __global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, const float* __restrict__ recv,
                                                                  const int* __restrict__ rmat, int width,
                                                                  int height, const float* __restrict__ vpl,
@@ -253,7 +257,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, 
             if (lane == 0 && tile < ntiles) bits[(int64_t)(c0 + cc) * ntiles + tile] = b;
         }
     }
-    if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
+    if (kCount) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
 // ---- phase 2: Σ_c weight_c * gather_point(rep_c) over the visible
@@ -362,9 +366,12 @@ void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv_in, const float* d_
     SceneView sv = sv_in;
     unsigned grid = blocks_for(n, kShadeBlock);
     size_t lds = sizeof(int) * (kShadeBlock / 64) * (size_t)sv.stack4;
-    hipLaunchKernelGGL(k_reduced_matrix, dim3(grid), dim3(kShadeBlock), lds,
-                       ctx->stream, sv, d_recv, d_rmat, rows, d_vorder, d_vpl, d_vmat, nvpl, clamp_mode, clamp,
-                       tmin, d_R, d_V, d_stats);
+    if (d_stats)
+        hipLaunchKernelGGL(k_reduced_matrix<true>, dim3(grid), dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
+                           d_rmat, rows, d_vorder, d_vpl, d_vmat, nvpl, clamp_mode, clamp, tmin, d_R, d_V, d_stats);
+    else
+        hipLaunchKernelGGL(k_reduced_matrix<false>, dim3(grid), dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
+                           d_rmat, rows, d_vorder, d_vpl, d_vmat, nvpl, clamp_mode, clamp, tmin, d_R, d_V, d_stats);
     RZK_HIP(hipGetLastError());
 }
 
@@ -386,8 +393,12 @@ void launch_slice_gather(rzk_ctx* ctx, const SceneView& sv_in, const float* d_re
     // wave-coherent walk (measured: one wave per tile 23.1 ms, four
     // representatives per wave 7.3 ms for 512x512 pixels x 300 representatives)
     const int groups = std::max(1, std::min((nreps + 3) / 4, 65535));
-    hipLaunchKernelGGL(k_slice_visibility, dim3(grid, groups), dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
-                       d_rmat, width, height, d_vpl, d_reps, nreps, tmin, d_bits, d_stats);
+    if (d_stats)
+        hipLaunchKernelGGL(k_slice_visibility<true>, dim3(grid, groups), dim3(kShadeBlock), lds, ctx->stream, sv,
+                           d_recv, d_rmat, width, height, d_vpl, d_reps, nreps, tmin, d_bits, d_stats);
+    else
+        hipLaunchKernelGGL(k_slice_visibility<false>, dim3(grid, groups), dim3(kShadeBlock), lds, ctx->stream, sv,
+                           d_recv, d_rmat, width, height, d_vpl, d_reps, nreps, tmin, d_bits, d_stats);
     RZK_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_slice_shade, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, sv.mat, d_recv, d_rmat, width,
                        height, d_vpl, d_vmat, d_reps, d_rep_w, nreps, d_bits, clamp_mode, clamp, n_emitted, d_out);

@@ -109,8 +109,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest_tiles(SceneView s
                 v3 rlo = mk(nf[6], nf[7], nf[8]), rhi = mk(nf[9], nf[10], nf[11]);
                 int left = ni[12], right = ni[13];
                 if (in) st.nodes++;
-                bool hl = in && box_hit(llo, lhi, r, tmin, tmax);
-                bool hr = in && box_hit(rlo, rhi, r, tmin, tmax);
+                bool hl = box_hit(llo, lhi, r, tmin, tmax) & in;     // idle lanes hold a finite ray
+                bool hr = box_hit(rlo, rhi, r, tmin, tmax) & in;
                 unsigned long long ml = __ballot(hl), mr = __ballot(hr);
                 if (mr != 0ull) {
                     if (ml != 0ull) {
