@@ -262,16 +262,23 @@ __device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, f
 // its own ray, a ballot decides which children the wave descends into, and a
 // lane leaves the walk once it has found an occluder.  Each lane still tests
 // exactly the triangles whose box chain its own ray passes (the result is the
-// per-lane any_hit4 result); only the schedule is shared.  Every lane of the
-// wave must call this together; `active` false lanes ride along idle.
-// `wstack` is this wave's LDS stack of sv.stack4 ints.
-__device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r, float tmin, float tmax,
-                                              bool active, int* wstack, TravStats& st) {
-    bool live = active && box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]),
-                                  mk(sv.rbmax[0], sv.rbmax[1], sv.rbmax[2]), r, tmin, tmax);
+// per-lane any_hit4 result); only the schedule is shared.  kSigned: every
+// walking lane's direction has the wave-uniform component signs (nx, ny, nz:
+// true for a negative component), so the child tests use box_hit_signed
+// (same outcomes, fewer operations).
+template <bool kSigned>
+__device__ __forceinline__ bool any_hit4_wave_walk(const SceneView& sv, const Ray& r, float tmin, float tmax,
+                                                   bool live, bool nx, bool ny, bool nz, int* wstack,
+                                                   TravStats& st) {
+    // float offsets in a node of the entry (e*) and exit (x*) plane arrays of
+    // each axis: lo.x[4] at 0, lo.y at 4, lo.z at 8, hi.x at 12, hi.y at 16,
+    // hi.z at 20; a negative component swaps lo and hi.  Picking the planes by
+    // load address keeps the selection out of the per-node instructions.
+    const int ex = nx ? 12 : 0, xx = nx ? 0 : 12;
+    const int ey = ny ? 16 : 4, xy = ny ? 4 : 16;
+    const int ez = nz ? 20 : 8, xz = nz ? 8 : 20;
     bool hit = false;
     float t, b, g;
-    if (__ballot(live) == 0ull) return false;
     int node = sv.root4_ref;
     if (node < 0) {
         if (live) {
@@ -288,19 +295,31 @@ __device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r,
         // wave-uniform address in the constant address space: scalar loads
         const kconst float* nf = (const kconst float*)(sv.nodes4 + node);
         const kconst int* ni = (const kconst int*)(sv.nodes4 + node);
-        const float lxa[4] = {nf[0], nf[1], nf[2], nf[3]}, lya[4] = {nf[4], nf[5], nf[6], nf[7]};
-        const float lza[4] = {nf[8], nf[9], nf[10], nf[11]}, hxa[4] = {nf[12], nf[13], nf[14], nf[15]};
-        const float hya[4] = {nf[16], nf[17], nf[18], nf[19]}, hza[4] = {nf[20], nf[21], nf[22], nf[23]};
+        // kSigned: lxa..hza hold the entry planes (lxa, lya, lza) and exit planes
+        // (hxa, hya, hza) rather than lo and hi
+        const int o0 = kSigned ? ex : 0, o1 = kSigned ? ey : 4, o2 = kSigned ? ez : 8;
+        const int o3 = kSigned ? xx : 12, o4 = kSigned ? xy : 16, o5 = kSigned ? xz : 20;
+        const float lxa[4] = {nf[o0], nf[o0 + 1], nf[o0 + 2], nf[o0 + 3]};
+        const float lya[4] = {nf[o1], nf[o1 + 1], nf[o1 + 2], nf[o1 + 3]};
+        const float lza[4] = {nf[o2], nf[o2 + 1], nf[o2 + 2], nf[o2 + 3]};
+        const float hxa[4] = {nf[o3], nf[o3 + 1], nf[o3 + 2], nf[o3 + 3]};
+        const float hya[4] = {nf[o4], nf[o4 + 1], nf[o4 + 2], nf[o4 + 3]};
+        const float hza[4] = {nf[o5], nf[o5 + 1], nf[o5 + 2], nf[o5 + 3]};
         const int rfa[4] = {ni[24], ni[25], ni[26], ni[27]};
         if (live) st.nodes++;
         // the four child tests first, as straight-line code: every lane evaluates
         // them (idle lanes hold finite rays; empty slots hold finite zero boxes and
-        // are masked out below), so the node arrives in one scalar round trip and
-        // the four tests interleave
+        // are masked out below), so the node arrives in one scalar round trip
         bool box[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-            box[c] = box_hit(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax);
+        for (int c = 0; c < 4; ++c) {
+            if (kSigned) {
+                // box_hit_signed, the planes picked by the loads above
+                box[c] = box_hit_planes(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax);
+            } else {
+                box[c] = box_hit(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax);
+            }
+        }
         // lanes still walking, as a wave mask (refreshed after a leaf's tests)
         unsigned long long lm = __builtin_amdgcn_ballot_w64(live);
         int next = -1;
@@ -333,6 +352,23 @@ __device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r,
             node = wstack[--sp];
         }
     }
+}
+
+// Every lane of the wave must call this together; `active` false lanes ride
+// along idle.  `wstack` is this wave's LDS stack of sv.stack4 ints.
+__device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r, float tmin, float tmax,
+                                              bool active, int* wstack, TravStats& st) {
+    bool live = active && box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]),
+                                  mk(sv.rbmax[0], sv.rbmax[1], sv.rbmax[2]), r, tmin, tmax);
+    const unsigned long long lm = __builtin_amdgcn_ballot_w64(live);
+    if (lm == 0ull) return false;
+    // direction signs shared by every walking lane?
+    const unsigned long long mx = __builtin_amdgcn_ballot_w64(r.inv_d.x < 0.f) & lm;
+    const unsigned long long my = __builtin_amdgcn_ballot_w64(r.inv_d.y < 0.f) & lm;
+    const unsigned long long mz = __builtin_amdgcn_ballot_w64(r.inv_d.z < 0.f) & lm;
+    if ((mx == 0ull || mx == lm) && (my == 0ull || my == lm) && (mz == 0ull || mz == lm))
+        return any_hit4_wave_walk<true>(sv, r, tmin, tmax, live, mx != 0ull, my != 0ull, mz != 0ull, wstack, st);
+    return any_hit4_wave_walk<false>(sv, r, tmin, tmax, live, false, false, false, wstack, st);
 }
 
 // Segment visibility p -> q (aggregate.cc:84-90): true when blocked.

@@ -100,6 +100,28 @@ RZK_HD bool box_enter(v3 bmin, v3 bmax, const Ray& r, float tmin, float tmax, fl
     return tminbox <= tmaxbox;     // the other two conditions follow (box_hit)
 }
 
+// Slab test of box_hit when the sign of every direction component is known
+// (nx: inv_d.x < 0, ...): the entry plane of each slab is then lo (positive
+// component) or hi (negative), so the per-axis min/max drop out.  The values
+// are box_hit's bit for bit: with inv_d.x > 0 and lo <= hi, lo * inv + org <=
+// hi * inv + org in fp32 (rounded mul and add are monotone), so fminf of the
+// two slab values is the lo one; with inv_d.x < 0 it is the hi one.
+// The slab test on pre-selected entry corner `en` and exit corner `ex`
+// (per axis: lo and hi for a positive component, hi and lo for a negative one).
+RZK_HD bool box_hit_planes(v3 en, v3 ex, const Ray& r, float tmin, float tmax) {
+    float xn = en.x * r.inv_d.x + r.org_inv_d.x, xf = ex.x * r.inv_d.x + r.org_inv_d.x;
+    float yn = en.y * r.inv_d.y + r.org_inv_d.y, yf = ex.y * r.inv_d.y + r.org_inv_d.y;
+    float zn = en.z * r.inv_d.z + r.org_inv_d.z, zf = ex.z * r.inv_d.z + r.org_inv_d.z;
+    float tminbox = fmaxf(fmaxf(tmin, xn), fmaxf(yn, zn));
+    float tmaxbox = fminf(fminf(tmax, xf), fminf(yf, zf));
+    return tminbox <= tmaxbox;
+}
+
+RZK_HD bool box_hit_signed(v3 bmin, v3 bmax, bool nx, bool ny, bool nz, const Ray& r, float tmin, float tmax) {
+    return box_hit_planes(mk(nx ? bmax.x : bmin.x, ny ? bmax.y : bmin.y, nz ? bmax.z : bmin.z),
+                          mk(nx ? bmin.x : bmax.x, ny ? bmin.y : bmax.y, nz ? bmin.z : bmax.z), r, tmin, tmax);
+}
+
 // Triangle::hit, triangle.cc:158-232, on the precomputed edges p01 = p0 - p1,
 // p02 = p0 - p2 (the same subtractions the reference performs per test).
 RZK_HD bool tri_hit(v3 p0, v3 p01, v3 p02, const Ray& r, float tmin, float tmax,

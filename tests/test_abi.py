@@ -63,3 +63,17 @@ def test_cpp_adapter_builds_against_the_abi():
     r = subprocess.run(["make", "-s", "-C", os.path.join(here, "cpp")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert os.path.exists(os.path.join(here, "cpp", "adapter_test"))
+
+
+def test_signed_slab_test_equals_min_max_form():
+    """box_hit_signed (entry/exit planes picked from the direction signs, used
+    by the wave-coherent walk when a wave's signs agree) gives box_hit's
+    boolean on 2M random and adversarial rays and boxes (host build of the
+    kernels' own math header; no GPU)"""
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run(["make", "-s", "-C", os.path.join(here, "cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([os.path.join(here, "cpp", "box_signed_test")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 mismatches" in r.stdout
