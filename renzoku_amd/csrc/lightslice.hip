@@ -79,24 +79,46 @@ __global__ void k_slice_pdf_weights(const float* __restrict__ R, int rows, const
     wout[t] = v;
 }
 
-// DiscretePdf::set_distribution per slice (pdf.cc:18-52), one lane per slice
-__global__ void k_slice_pdf_norm(int rows, int nreps, const float* __restrict__ w, float* __restrict__ pdf,
-                                 float* __restrict__ cdf) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+// DiscretePdf::set_distribution per slice (pdf.cc:18-52): sum of the weights
+// in order, pdf = w / sum, cdf[i] = cdf[i-1] + pdf[i], cdf[last] = 1.  One
+// wave per slice: the weights pass through LDS 64 at a time, every lane runs
+// the same ordered chains (sum, then cdf) and lane k keeps the cdf value of
+// element k of the chunk, so writes are coalesced and the adds keep their order.
+__global__ __launch_bounds__(64) void k_slice_pdf_norm(int rows, int nreps, const float* __restrict__ w,
+                                                       float* __restrict__ pdf, float* __restrict__ cdf) {
+    __shared__ float s_v[64];
+    const int s = blockIdx.x, lane = threadIdx.x;
     if (s >= rows) return;
     const float* a = w + (int64_t)s * nreps;
     float* p = pdf + (int64_t)s * nreps;
     float* c = cdf + (int64_t)s * nreps;
     float sum = 0.f;
-    for (int i = 0; i < nreps; ++i) sum += a[i];
+    for (int base = 0; base < nreps; base += 64) {
+        const int i = base + lane, n = min(64, nreps - base);
+        s_v[lane] = i < nreps ? a[i] : 0.f;
+        __syncthreads();
+        for (int k = 0; k < n; ++k) sum += s_v[k];
+        __syncthreads();
+    }
     if (sum == 0.0f) {
-        for (int i = 0; i < nreps; ++i) { p[i] = 0.f; c[i] = 0.f; }
+        for (int i = lane; i < nreps; i += 64) { p[i] = 0.f; c[i] = 0.f; }
         return;
     }
-    for (int i = 0; i < nreps; ++i) p[i] = a[i] / sum;
-    c[0] = p[0];
-    for (int i = 1; i < nreps; ++i) c[i] = c[i - 1] + p[i];
-    c[nreps - 1] = 1.0f;
+    float run = 0.f;
+    for (int base = 0; base < nreps; base += 64) {
+        const int i = base + lane, n = min(64, nreps - base);
+        const float pv = i < nreps ? a[i] / sum : 0.f;
+        if (i < nreps) p[i] = pv;
+        s_v[lane] = pv;
+        __syncthreads();
+        float mine = 0.f;
+        for (int k = 0; k < n; ++k) {
+            run = (base + k == 0) ? s_v[0] : run + s_v[k];
+            if (k == lane) mine = run;
+        }
+        if (i < nreps) c[i] = i == nreps - 1 ? 1.0f : mine;
+        __syncthreads();
+    }
 }
 
 // tile-centre pixel of every slice (the row of the reduced matrix)
@@ -109,9 +131,15 @@ __global__ void k_tile_centres(int tx, int ty, int tile, int W, int H, int* __re
 }
 
 // build_map (lightslice.cc:2058-2147) + DiscretePdf2D::set_distribution
-// (pdf.cc:195-233).  One block per slice: the scatter runs in cluster order in
-// one lane (the map sums must keep that order), rows are normalised in parallel.
-__global__ void k_slice_maps(int res, int nreps, const float* __restrict__ R, int rows,
+// (pdf.cc:195-233).  One wave per slice.  The clusters' map bins and
+// luminances are computed across the lanes (kMapChunk at a time, staged in
+// LDS); the scatter then walks the clusters in order, each bin updated only by
+// the lane that owns it (bin mod 64) and the total summed by every lane, so
+// every map value and the total keep the reference's cluster-order sums.  Rows
+// are normalised in parallel.  LDS: res*res bins, res + 1 sums, 2 * kMapChunk.
+constexpr int kMapChunk = 256;
+
+__global__ __launch_bounds__(64) void k_slice_maps(int res, int nreps, const float* __restrict__ R, int rows,
                              const float* __restrict__ rrecv, const int* __restrict__ reps,
                              const float* __restrict__ rep_w, const float* __restrict__ slice_wi,
                              const int* __restrict__ upper, float eps, int valid_all,
@@ -120,35 +148,55 @@ __global__ void k_slice_maps(int res, int nreps, const float* __restrict__ R, in
                              float* __restrict__ totals) {
     extern __shared__ float s_map[];
     float* s_rows = s_map + res * res;
-    int s = blockIdx.x;
-    for (int e = threadIdx.x; e < res * res; e += blockDim.x) s_map[e] = 0.f;
+    float* s_lum = s_rows + res + 1;
+    int* s_bin = reinterpret_cast<int*>(s_lum + kMapChunk);
+    const int s = blockIdx.x, lane = threadIdx.x;
+    for (int e = lane; e < res * res; e += 64) s_map[e] = 0.f;
     __syncthreads();
-    if (threadIdx.x == 0 && valid_all) {
+    if (valid_all) {
         const float* rc = rrecv + RZK_RECV_FLOATS * s;
         Onb b;
         b.u = ld3(rc + 12); b.v = ld3(rc + 15); b.n = ld3(rc + 18);
         float sum = 0.f;
-        for (int c = 0; c < nreps; ++c) {
-            int64_t t = (int64_t)s * nreps + c;
-            if (!upper[t]) continue;
-            v3 lwi = world_to_local(b, ld3(slice_wi + 3 * t));
-            if (lwi.z <= 0.0f) continue;
-            const float* e = R + 3 * ((int64_t)reps[c] * rows + s);
-            v3 rad = ld3(e) * rep_w[c];
-            float sx, sy;
-            direction_to_square(lwi, sx, sy);
-            int x = (int)(sx * res), y = (int)(sy * res);
-            if (x >= res) x = res - 1;
-            if (y >= res) y = res - 1;
-            if (x < 0) x = 0;
-            if (y < 0) y = 0;
-            float lum = rgb_value(rad);
-            s_map[y * res + x] += lum;
-            sum += lum;
+        for (int c0 = 0; c0 < nreps; c0 += kMapChunk) {
+            const int cnt = min(kMapChunk, nreps - c0);
+            for (int k = lane; k < cnt; k += 64) {
+                const int c = c0 + k;
+                const int64_t t = (int64_t)s * nreps + c;
+                int bin = -1;
+                float lum = 0.f;
+                if (upper[t]) {
+                    v3 lwi = world_to_local(b, ld3(slice_wi + 3 * t));
+                    if (!(lwi.z <= 0.0f)) {
+                        const float* e = R + 3 * ((int64_t)reps[c] * rows + s);
+                        v3 rad = ld3(e) * rep_w[c];
+                        float sx, sy;
+                        direction_to_square(lwi, sx, sy);
+                        int x = (int)(sx * res), y = (int)(sy * res);
+                        if (x >= res) x = res - 1;
+                        if (y >= res) y = res - 1;
+                        if (x < 0) x = 0;
+                        if (y < 0) y = 0;
+                        bin = y * res + x;
+                        lum = rgb_value(rad);
+                    }
+                }
+                s_bin[k] = bin;
+                s_lum[k] = lum;
+            }
+            __syncthreads();
+            for (int k = 0; k < cnt; ++k) {
+                const int bin = s_bin[k];
+                if (bin < 0) continue;
+                const float lum = s_lum[k];
+                if ((bin & 63) == lane) s_map[bin] += lum;
+                sum += lum;
+            }
+            __syncthreads();
         }
-        s_rows[0] = sum;
+        if (lane == 0) s_rows[0] = sum;
+        __syncthreads();
     }
-    __syncthreads();
     if (valid_all) {
         float add = eps * s_rows[0];
         __syncthreads();
@@ -340,7 +388,7 @@ void enqueue_guide_build(rzk_ctx* ctx, const rzk_scene* scene, const rzk_cluster
     hipLaunchKernelGGL(k_slice_info, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, st, sv.mat, b.recv, b.mat, S,
                        b.V, cl.reps->d_vpl, cl.reps->d_mat, cl.d_ident, C, gp.clamp_mode, gp.incident_clamp,
                        b.slice_wi, b.upper);
-    size_t lds = sizeof(float) * ((size_t)res * res + res + 1);
+    size_t lds = sizeof(float) * ((size_t)res * res + res + 1 + 2 * kMapChunk);
     hipLaunchKernelGGL(k_slice_maps, dim3(S), dim3(64), lds, st, res, C, b.R, S, b.recv, cl.d_ident, cl.d_rep_w,
                        b.slice_wi, b.upper, gp.map_epsilon, gp.map_gate ? 0 : 1, b.maps, b.col_cdf, b.row_pdf,
                        b.row_cdf, b.totals);
@@ -665,11 +713,11 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
             hipLaunchKernelGGL(k_slice_pdf_weights, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, st, ls->R.p, S,
                                ls->nb.p, p.num_neighbor_slices, ls->reps.p, ls->rep_w.p, nreps, ls->upper.p,
                                ls->slice_w.p);
-            hipLaunchKernelGGL(k_slice_pdf_norm, dim3((S + 63) / 64), dim3(64), 0, st, S, nreps, ls->slice_w.p,
+            hipLaunchKernelGGL(k_slice_pdf_norm, dim3(S), dim3(64), 0, st, S, nreps, ls->slice_w.p,
                                ls->slice_pdf.p, ls->slice_cdf.p);
         }
         int res = p.map_resolution;
-        size_t lds = sizeof(float) * ((size_t)res * res + res + 1);
+        size_t lds = sizeof(float) * ((size_t)res * res + res + 1 + 2 * kMapChunk);
         hipLaunchKernelGGL(k_slice_maps, dim3(S), dim3(64), lds, st, res, nreps, ls->R.p, S, ls->rows_recv.p,
                            ls->reps.p, ls->rep_w.p, ls->slice_wi.p, ls->upper.p, p.map_epsilon, p.map_gate ? 0 : 1,
                            ls->maps.p, ls->col_cdf.p, ls->row_pdf.p, ls->row_cdf.p, ls->totals.p);

@@ -268,10 +268,26 @@ def test_lightslice_pipeline_parity(ctx, room_setup):
     assert np.array_equal(nb, vplgen.tile_neighbours(W, H, 16, 4))
     upper = ls.get("slice_upper", np.int32, S * len(reps)).reshape(S, -1)
     Vd = ls.get("V", np.float32, S * V).reshape(V, S)
+    nr = len(reps)
+    gpdf = ls.get("slice_pdf", np.float32, S * nr).reshape(S, -1)
+    gcdf = ls.get("slice_cdf", np.float32, S * nr).reshape(S, -1)
+    swi = ls.get("slice_wi", np.float32, S * nr * 3).reshape(S, nr, 3)
+    res = 32
+    gmaps = ls.get("maps", np.float32, S * res * res).reshape(S, res, res)
+    growc = ls.get("row_cdf", np.float32, S * res).reshape(S, res)
+    gcolc = ls.get("col_cdf", np.float32, S * res * res).reshape(S, res, res)
+    gtot = ls.get("map_totals", np.float32, S)
+    rows_recv = ls.get("rows_recv", np.float32, S * 21).reshape(S, 21)
     for s in range(0, S, 5):
         wi, up = os_.slice_cluster_info(rrecv[s], reps, vpl, vmat, Vd[:, s].copy(), 1, 0.01)
         assert np.array_equal(up, upper[s])
         w = oracle.slice_cluster_pdf(R, nb[s], reps, rep_w, up)
         ok, pdf, cdf = oracle.pdf_build(w)
-        gp = ls.get("slice_pdf", np.float32, S * len(reps)).reshape(S, -1)[s]
-        assert np.array_equal(gp.view(np.uint32), pdf.view(np.uint32))
+        assert np.array_equal(gpdf[s].view(np.uint32), pdf.view(np.uint32))
+        assert np.array_equal(gcdf[s].view(np.uint32), cdf.view(np.uint32))
+        # the slice's luminance map and its 2-D distribution (build_map, lightslice.cc:2058-2151)
+        om = oracle.build_map(res, swi[s], upper[s], rep_w, R[reps, s, :], rows_recv[s, 12:21], 1e-3, 1)
+        assert np.array_equal(gmaps[s].view(np.uint32), om["map"].view(np.uint32)), f"slice {s} map"
+        assert np.array_equal(growc[s].view(np.uint32), om["row_cdf"].view(np.uint32))
+        assert np.array_equal(gcolc[s].view(np.uint32), om["col_cdf"].view(np.uint32))
+        assert gtot[s] == np.float32(om["total"])
