@@ -73,6 +73,8 @@ def lib():
             "orc_cluster_assign": ([I, I, P, I, P, P, P, I], None),
             "orc_pdf_build": ([I, P, P, P], I), "orc_pdf_sample": ([I, P, P, F, P, P], None),
             "orc_cluster_by_sampling": ([I, I, P, I, P, P, P, P, P, P, I], I),
+            "orc_refine_slices": ([I, I, P, I, P, I, P, P, P, P, P, I, P, P, P, P, P, P], I),
+            "orc_slice_cost": ([I, P, I, P, I, P], F),
             "orc_direction_to_square": ([P, P], None), "orc_square_to_direction": ([P, P], None),
             "orc_slice_cluster_info": ([P, P, I, P, P, P, P, I, F, P, P], None),
             "orc_slice_cluster_pdf": ([I, P, I, P, I, P, P, P, P], None),
@@ -347,6 +349,59 @@ def cluster_by_sampling(R, num_clusters, seed, nthreads=1):
     ncl = code // 65536
     return dict(centers=centers[:ncl], center_w=cw[:ncl], assign=assign, reps=reps[:ncl][kept[:ncl] > 0],
                 rep_w=rw[:ncl][kept[:ncl] > 0], ncl=ncl)
+
+
+def kept_cluster_members(assign, ncl, kept):
+    """CSR (offsets, members in increasing VPL order) of the kept clusters,
+    renumbered 0.. in cluster order (cluster_builder.cc:223, :264-272)"""
+    assign = np.asarray(assign)
+    off, mem = [0], []
+    for c in range(ncl):
+        if not kept[c]:
+            continue
+        m = np.nonzero(assign == c)[0]
+        mem.extend(m.tolist())
+        off.append(len(mem))
+    return np.array(off, np.int32), np.array(mem, np.int32)
+
+
+def slice_cost(R, nb_rows, idx):
+    """compute_distance (lightslice.cc:336-357) of columns idx over rows nb_rows"""
+    R, nb_rows, idx = _f(R), _i(nb_rows), _i(idx)
+    return float(lib().orc_slice_cost(R.shape[1], _p(R), len(nb_rows), _p(nb_rows), len(idx), _p(idx)))
+
+
+def cluster_and_refine(R, num_clusters, max_slice_clusters, nb, seed, nthreads=1, with_perm=False):
+    """cluster_by_sampling then the per-slice refinement on one MT stream seeded
+    with `seed` (lightslice.cc:757-902).  R: (V, S, 3); nb: (S, nnb) rows."""
+    R = _f(R)
+    V, S = R.shape[0], R.shape[1]
+    nb = _i(nb)
+    L = lib()
+    L.orc_mt_seed32(seed)
+    centers, cw = np.zeros(num_clusters, np.int32), np.zeros(num_clusters, np.float32)
+    assign = np.zeros(V, np.int32)
+    reps, rw, kept = (np.zeros(num_clusters, np.int32), np.zeros(num_clusters, np.float32),
+                      np.zeros(num_clusters, np.int32))
+    code = L.orc_cluster_by_sampling(S, V, _p(R), num_clusters, _p(centers), _p(cw), _p(assign), _p(reps), _p(rw),
+                                     _p(kept), nthreads)
+    ncl = code // 65536
+    off, mem = kept_cluster_members(assign, ncl, kept[:ncl])
+    k = kept[:ncl] > 0
+    creps, crw = _i(reps[:ncl][k]), _f(rw[:ncl][k])
+    mags = column_magnitudes(R)
+    M = max_slice_clusters
+    count, rep, w = np.zeros(S, np.int32), np.zeros(S * M, np.int32), np.zeros(S * M, np.float32)
+    size, draws = np.zeros(S * M, np.int32), np.zeros(S, np.int32)
+    perm = np.zeros(S * V, np.int32) if with_perm else None
+    rc = L.orc_refine_slices(S, V, _p(R), nb.shape[1], _p(nb), len(creps), _p(off), _p(mem), _p(creps), _p(crw),
+                             _p(mags), M, _p(count), _p(rep), _p(w), _p(size), _p(perm), _p(draws))
+    assert rc == 0, rc
+    out = dict(centers=centers[:ncl], assign=assign, reps=creps, rep_w=crw, off=off, mem=mem, count=count,
+               rep=rep.reshape(S, M), w=w.reshape(S, M), size=size.reshape(S, M), draws=draws)
+    if with_perm:
+        out["perm"] = perm.reshape(S, V)
+    return out
 
 
 def direction_to_square(wi):

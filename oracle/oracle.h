@@ -108,6 +108,13 @@ void orc_pdf_sample(int n, const float *pdf, const float *cdf, float u, int *k, 
 int  orc_cluster_by_sampling(int rows, int cols, const float *R, int num_clusters,
                              int *centers, float *center_w, int *assign, int *reps,
                              float *rep_w, int *kept, int nthreads);
+/* per-slice refinement (refine.cpp, lightslice.cc:837-902); draws continue the
+ * global MT stream */
+int  orc_refine_slices(int S, int V, const float *R, int nnb, const int *nb, int ncl,
+                       const int *cl_off, const int *cl_mem, const int *cl_rep, const float *cl_w,
+                       const float *mags, int max_slice_clusters, int *out_count, int *out_rep,
+                       float *out_w, int *out_size, int *out_perm, int *out_draws);
+float orc_slice_cost(int S, const float *R, int nnb, const int *nb, int n, const int *idx);
 
 /* ---- guiding distributions ---- */
 void orc_direction_to_square(const float wi[3], float out[2]);

@@ -77,3 +77,16 @@ def test_signed_slab_test_equals_min_max_form():
     r = subprocess.run([os.path.join(here, "cpp", "box_signed_test")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 mismatches" in r.stdout
+
+
+def test_std_sort_and_priority_queue_restatement():
+    """the device restatement of std::sort and std::priority_queue (whose tie
+    orders the LightSlice refinement inherits) moves elements exactly as this
+    toolchain's libstdc++ does, on tie-heavy inputs (host build; no GPU)"""
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run(["make", "-s", "-C", os.path.join(here, "cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([os.path.join(here, "cpp", "std_order_test")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert " 0 mismatches" in r.stdout
