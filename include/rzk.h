@@ -186,6 +186,20 @@ int rzk_alphas(rzk_ctx *ctx, int32_t rows, int64_t cols, const float *R, float *
  * cluster_builder.cc:158-225, strict < with the first minimum kept. */
 int rzk_cluster_assign(rzk_ctx *ctx, int32_t rows, int64_t cols, const float *R, int32_t ncl,
                        const int32_t *centers, const float *weights, int32_t *out_assign);
+/* LightSlice per-slice refinement (lightslice.cc:837-902, split_cluster
+ * :405-488, compute_distance :336-357, choose_representative :385-400): the
+ * ncl kept clusters (members cl_mem[cl_off[c] .. cl_off[c+1]) in order,
+ * representatives cl_rep, weights cl_w) are split per slice, most expensive
+ * first, until a slice holds max_slice_clusters clusters.  R: column-major Rgb
+ * (R[(k*S + r)*3 + ch], V columns, S rows); nb: S x nnb neighbour rows (the
+ * slice matrix).  draws: the MT numbers that follow the clustering's in the
+ * reference's stream, at least 4 * (max_slice_clusters - ncl) * S of them.
+ * Outputs: out_count[S]; out_rep, out_w: S x max_slice_clusters, row s holding
+ * slice s's clusters in the reference's order. */
+int rzk_refine_slices(rzk_ctx *ctx, int32_t S, int64_t V, const float *R, int32_t nnb, const int32_t *nb,
+                      int32_t ncl, const int32_t *cl_off, const int32_t *cl_mem, const int32_t *cl_rep,
+                      const float *cl_w, int32_t max_slice_clusters, const float *draws, int64_t ndraws,
+                      int32_t *out_count, int32_t *out_rep, float *out_w);
 
 /* ---- LightSlice (lightslice.h, lightslice.cc:589-1345) ------------------ */
 typedef struct rzk_lightslice_params {
@@ -204,6 +218,9 @@ typedef struct rzk_lightslice_params {
     int32_t map_gate;               /* 1: build maps only where sampling_records[s].is_valid
                                        would be true (lightslice.cc:2070); 0: every
                                        slice with a non-empty map */
+    int32_t max_slice_clusters;     /* per-slice refinement bound (lightslice.cc:29, :869):
+                                       clusters are split per slice until a slice holds this
+                                       many; 0 or <= the kept clusters: no refinement */
 } rzk_lightslice_params;
 
 typedef struct rzk_lightslice_info {
@@ -215,6 +232,7 @@ typedef struct rzk_lightslice_info {
     int64_t gather_pairs;           /* pixels with a material x clusters */
 } rzk_lightslice_info;
 
+/* the LightSlice keeps scene and vpls by pointer: both must outlive it */
 int rzk_lightslice_create(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls,
                           const rzk_lightslice_params *params, rzk_lightslice **out);
 int rzk_lightslice_destroy(rzk_lightslice *ls);
@@ -237,7 +255,10 @@ int rzk_lightslice_count_work(rzk_lightslice *ls, int64_t *out);
  *  "rep_w" (f32), "rows_recv" (rows*21 f32), "rows_mat" (rows i32),
  *  "slice_pdf" (slices*clusters f32), "slice_upper" (slices*clusters i32),
  *  "slice_wi" (slices*clusters*3 f32), "maps" (slices*res*res f32), "image" (W*H*3 f32),
- *  "recv" (W*H*21 f32), "recv_mat" (W*H i32), "hits" (W*H x {t, beta, gamma, tri as i32 bits})
+ *  "recv" (W*H*21 f32), "recv_mat" (W*H i32), "hits" (W*H x {t, beta, gamma, tri as i32 bits}),
+ *  "slice_count" (slices i32), "slice_reps" (slices*C i32), "slice_rep_w" (slices*C f32): each
+ *  slice's clusters after refinement (the kept clusters when there is none).  C, the row length
+ *  of every slice_* array, is max_slice_clusters when refinement ran, else the kept clusters.
  * bytes must equal the array size. */
 int rzk_lightslice_get(const rzk_lightslice *ls, const char *name, void *dst, int64_t bytes);
 
@@ -341,7 +362,8 @@ typedef struct rzk_guide_params {
 } rzk_guide_params;
 
 /* keep the LightSlice's kept clusters (representative VPLs, weights) and its
- * guide parameters as a resident set owned by the returned handle */
+ * guide parameters as a resident set owned by the returned handle; it refers
+ * to the LightSlice's scene, which must outlive it (the LightSlice need not) */
 int rzk_lightslice_clusters(const rzk_lightslice *ls, rzk_clusters **out);
 int rzk_clusters_destroy(rzk_clusters *clusters);
 /* params NULL: the parameters of the LightSlice the clusters came from */

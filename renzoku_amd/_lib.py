@@ -29,7 +29,7 @@ EXPORTS = [
     "rzk_fill_receivers", "rzk_camera_primary",
     "rzk_vpls_create", "rzk_vpls_destroy",
     "rzk_gather", "rzk_visibility", "rzk_render_vpl",
-    "rzk_reduced_matrix", "rzk_alphas", "rzk_cluster_assign",
+    "rzk_reduced_matrix", "rzk_alphas", "rzk_cluster_assign", "rzk_refine_slices",
     "rzk_lightslice_create", "rzk_lightslice_destroy", "rzk_lightslice_initialize",
     "rzk_lightslice_gather", "rzk_lightslice_get_info", "rzk_lightslice_count_work", "rzk_lightslice_get",
     "rzk_lightslice_render_guided",
@@ -67,7 +67,7 @@ class LightSliceParams(C.Structure):
         ("width", C.c_int32), ("height", C.c_int32), ("tile", C.c_int32), ("num_clusters", C.c_int32),
         ("num_neighbor_slices", C.c_int32), ("map_resolution", C.c_int32), ("map_epsilon", C.c_float),
         ("clamp_mode", C.c_int32), ("clamp", C.c_float), ("incident_clamp", C.c_float), ("tmin", C.c_float),
-        ("n_emitted", C.c_float), ("seed", C.c_uint32), ("map_gate", C.c_int32),
+        ("n_emitted", C.c_float), ("seed", C.c_uint32), ("map_gate", C.c_int32), ("max_slice_clusters", C.c_int32),
     ]
 
 
@@ -175,6 +175,7 @@ def load() -> C.CDLL:
         "rzk_reduced_matrix": [P, P, P, P, P, I32, C.c_int, F, F, P, P],
         "rzk_alphas": [P, I32, I64, P, P, P],
         "rzk_cluster_assign": [P, I32, I64, P, I32, P, P, P],
+        "rzk_refine_slices": [P, I32, I64, P, I32, P, I32, P, P, P, P, I32, P, I64, P, P, P],
         "rzk_lightslice_create": [P, P, P, C.POINTER(LightSliceParams), C.POINTER(P)],
         "rzk_lightslice_destroy": [P],
         "rzk_lightslice_initialize": [P, P],
@@ -430,13 +431,27 @@ def cluster_assign(ctx, R, centers, weights):
     return out
 
 
+def refine_slices(ctx, R, nb, cl_off, cl_mem, cl_rep, cl_w, max_slice_clusters, draws):
+    """rzk_refine_slices: per-slice refinement of the kept clusters.
+    R: (V, S, 3); nb: (S, nnb); returns (count (S,), reps (S, M), weights (S, M))."""
+    R, nb = f32(R), i32(nb)
+    V, S = R.shape[0], R.shape[1]
+    cl_off, cl_mem, cl_rep, cl_w, draws = i32(cl_off), i32(cl_mem), i32(cl_rep), f32(cl_w), f32(draws)
+    M = int(max_slice_clusters)
+    count, rep, w = np.zeros(S, np.int32), np.zeros((S, M), np.int32), np.zeros((S, M), np.float32)
+    ctx.check(ctx.lib.rzk_refine_slices(ctx.h, S, V, ptr(R), nb.shape[1], ptr(nb), len(cl_rep), ptr(cl_off),
+                                        ptr(cl_mem), ptr(cl_rep), ptr(cl_w), M, ptr(draws), len(draws), ptr(count),
+                                        ptr(rep), ptr(w)), "rzk_refine_slices")
+    return count, rep, w
+
+
 class LightSlice:
     def __init__(self, ctx, scene, vpls, **kw):
         self.ctx, self.lib = ctx, ctx.lib
         p = LightSliceParams()
         defaults = dict(width=512, height=512, tile=32, num_clusters=300, num_neighbor_slices=16,
                         map_resolution=32, map_epsilon=1e-3, clamp_mode=CLAMP_FORM_FACTOR, clamp=0.01,
-                        incident_clamp=0.01, tmin=1e-3, n_emitted=1.0, seed=2013, map_gate=0)
+                        incident_clamp=0.01, tmin=1e-3, n_emitted=1.0, seed=2013, map_gate=0, max_slice_clusters=0)
         defaults.update(kw)
         for k, v in defaults.items():
             setattr(p, k, v)
@@ -445,6 +460,8 @@ class LightSlice:
         ctx.check(self.lib.rzk_lightslice_create(ctx.h, scene.h, vpls.h, C.byref(p), C.byref(h)),
                   "rzk_lightslice_create")
         self.h = h
+        # the native object points at both: they must outlive it
+        self.scene, self.vpls = scene, vpls
         self.w, self.hgt = defaults["width"], defaults["height"]
         self.res = defaults["map_resolution"]
 
@@ -645,6 +662,7 @@ class Clusters:
         h = C.c_void_p()
         self.ctx.check(self.lib.rzk_lightslice_clusters(ls.h, C.byref(h)), "rzk_lightslice_clusters")
         self.h = h
+        self.scene = ls.scene              # referred to by the native object
 
     def close(self):
         if getattr(self, "h", None):

@@ -38,6 +38,18 @@ __global__ void k_col_norms(int rows, int64_t cols, const float* __restrict__ R,
     mags[j] = sqrtf(s2);
 }
 
+// per column: sqrt(sum_i (r^2 + g^2 + b^2)), the per-row sum grouped
+// (LightTransportMatrix::get_column_norms, light_transport_matrix.cc:117-130);
+// the refinement samples representatives by it
+__global__ void k_col_norms_R(int rows, int64_t cols, const float* __restrict__ R, float* __restrict__ out) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cols) return;
+    const float* c = R + 3 * j * rows;
+    float mag = 0.f;
+    for (int i = 0; i < rows; ++i) mag += c[3 * i] * c[3 * i] + c[3 * i + 1] * c[3 * i + 1] + c[3 * i + 2] * c[3 * i + 2];
+    out[j] = sqrtf(mag);
+}
+
 // m^T[i][j] = |R(i, j)|: 64 x 64 tiles through LDS so that both the column
 // reads of R and the row writes of m^T are coalesced
 __global__ __launch_bounds__(256) void k_mag_transpose(int rows, int64_t cols, const float* __restrict__ R,
@@ -279,6 +291,12 @@ void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, 
     hipLaunchKernelGGL(k_col_dot_u, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, rows, cols, d_R, d_u, d_v);
     hipLaunchKernelGGL(k_alphas, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, cols, d_norms, d_c0, d_v,
                        d_alphas);
+    RZK_HIP(hipGetLastError());
+}
+
+void launch_column_norms_R(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_out) {
+    if (cols <= 0 || rows <= 0) return;
+    hipLaunchKernelGGL(k_col_norms_R, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, rows, cols, d_R, d_out);
     RZK_HIP(hipGetLastError());
 }
 

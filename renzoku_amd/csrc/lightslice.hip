@@ -1,9 +1,10 @@
 // lightslice.hip — LightSlice with pixel-tile slices (lightslice.h,
 // lightslice.cc:589-1345): whole-image primary hits, the reduced matrix on
 // the slice-representative rows, alphas, centre sampling, assignment,
-// representatives, per-slice cluster pdfs and luminance maps, and the final
-// per-pixel gather over cluster representatives.  Also the plain whole-image
-// VPL render (every VPL a weight-1 cluster).
+// representatives, the per-slice refinement of the clusters (refine.hip),
+// per-slice cluster pdfs and luminance maps, and the final per-pixel gather
+// over each slice's cluster representatives.  Also the plain whole-image VPL
+// render (every VPL a weight-1 cluster).
 #pragma clang fp contract(off)
 
 #include <algorithm>
@@ -23,6 +24,12 @@ void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, 
 void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl, const int* d_centers,
                         const float* d_weights, const float* d_mags, float* d_K, float* d_Knorm,
                         int* d_assign);
+void launch_column_norms_R(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_out);
+int64_t refine_draws_needed(int S, int ncl, int maxC);
+void launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb, int nnb, int ncl, const int* d_off,
+                          const int* d_mem, int nmem, const int* d_grep, const float* d_gw, const float* d_mags,
+                          const float* d_norms_R, int maxC, const std::vector<float>& h_draws, int* d_rep,
+                          float* d_w, int* d_count);
 
 namespace rzk {
 
@@ -37,16 +44,22 @@ __global__ void k_pick_rows(const float* __restrict__ recv, const int* __restric
 }
 
 // lightslice.cc:1011-1030 for each (slice s, cluster c): slice_wi = direction
-// to the representative, upper = visible && above the shading normal && non-black
-__global__ void k_slice_info(const float* __restrict__ mat, const float* __restrict__ rrecv,
-                             const int* __restrict__ rrmat, int rows, const float* __restrict__ V,
-                             const float* __restrict__ vpl, const int* __restrict__ vmat,
-                             const int* __restrict__ reps, int nreps, int mode, float thr,
+// to the representative, upper = visible && above the shading normal && non-black.
+// Entries are [s * sr.width() + c]; clusters beyond a slice's count get upper = 0.
+__global__ void k_slice_info(const float* __restrict__ mat, const float* __restrict__ rrecv, int rows,
+                             const float* __restrict__ V, const float* __restrict__ vpl,
+                             const int* __restrict__ vmat, SliceReps sr, int mode, float thr,
                              float* __restrict__ slice_wi, int* __restrict__ upper) {
+    const int C = sr.width();
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)rows * nreps) return;
-    int s = (int)(t / nreps), c = (int)(t - (int64_t)s * nreps);
-    int k = reps[c];
+    if (t >= (int64_t)rows * C) return;
+    int s = (int)(t / C), c = (int)(t - (int64_t)s * C);
+    if (c >= sr.count(s)) {
+        st3(slice_wi + 3 * t, mk(0.f, 0.f, 0.f));
+        upper[t] = 0;
+        return;
+    }
+    int k = sr.rep(s, c);
     const float* rc = rrecv + RZK_RECV_FLOATS * s;
     const float* vp = vpl + (int64_t)RZK_VPL_FLOATS * k;
     float vis = V[(int64_t)k * rows + s];
@@ -55,26 +68,26 @@ __global__ void k_slice_info(const float* __restrict__ mat, const float* __restr
     st3(slice_wi + 3 * t, wi);
     bool nonblack = inc.x != 0.f || inc.y != 0.f || inc.z != 0.f;
     upper[t] = (vis > 0.0f && dot(ld3(rc + 6), wi) > kZeroEps && nonblack) ? 1 : 0;
-    (void)rrmat;
 }
 
 // lightslice.cc:1114-1179: cluster weight per slice = norm of the
 // representative column over the neighbour rows (matrix_c.cc:74-88 order)
 // x weight, 0 below the horizon
 __global__ void k_slice_pdf_weights(const float* __restrict__ R, int rows, const int* __restrict__ nb,
-                                    int nnb, const int* __restrict__ reps, const float* __restrict__ rep_w,
-                                    int nreps, const int* __restrict__ upper, float* __restrict__ wout) {
+                                    int nnb, SliceReps sr, const int* __restrict__ upper, float* __restrict__ wout) {
+    const int C = sr.width();
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)rows * nreps) return;
-    int s = (int)(t / nreps), c = (int)(t - (int64_t)s * nreps);
-    const float* col = R + 3 * (int64_t)reps[c] * rows;
+    if (t >= (int64_t)rows * C) return;
+    int s = (int)(t / C), c = (int)(t - (int64_t)s * C);
+    if (c >= sr.count(s)) { wout[t] = 0.f; return; }
+    const float* col = R + 3 * (int64_t)sr.rep(s, c) * rows;
     float mag = 0.f;
     for (int q = 0; q < nnb; ++q) {
         const float* e = col + 3 * nb[s * nnb + q];
         mag += e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
     }
     float v = sqrtf(mag);
-    v *= rep_w[c];
+    v *= sr.weight(s, c);
     if (!upper[t]) v = 0.0f;
     wout[t] = v;
 }
@@ -84,14 +97,16 @@ __global__ void k_slice_pdf_weights(const float* __restrict__ R, int rows, const
 // wave per slice: the weights pass through LDS 64 at a time, every lane runs
 // the same ordered chains (sum, then cdf) and lane k keeps the cdf value of
 // element k of the chunk, so writes are coalesced and the adds keep their order.
-__global__ __launch_bounds__(64) void k_slice_pdf_norm(int rows, int nreps, const float* __restrict__ w,
+__global__ __launch_bounds__(64) void k_slice_pdf_norm(int rows, SliceReps sr, const float* __restrict__ w,
                                                        float* __restrict__ pdf, float* __restrict__ cdf) {
     __shared__ float s_v[64];
     const int s = blockIdx.x, lane = threadIdx.x;
     if (s >= rows) return;
-    const float* a = w + (int64_t)s * nreps;
-    float* p = pdf + (int64_t)s * nreps;
-    float* c = cdf + (int64_t)s * nreps;
+    const int C = sr.width(), nreps = sr.count(s);
+    const float* a = w + (int64_t)s * C;
+    float* p = pdf + (int64_t)s * C;
+    float* c = cdf + (int64_t)s * C;
+    for (int i = nreps + lane; i < C; i += 64) { p[i] = 0.f; c[i] = 0.f; }
     float sum = 0.f;
     for (int base = 0; base < nreps; base += 64) {
         const int i = base + lane, n = min(64, nreps - base);
@@ -139,9 +154,8 @@ __global__ void k_tile_centres(int tx, int ty, int tile, int W, int H, int* __re
 // are normalised in parallel.  LDS: res*res bins, res + 1 sums, 2 * kMapChunk.
 constexpr int kMapChunk = 256;
 
-__global__ __launch_bounds__(64) void k_slice_maps(int res, int nreps, const float* __restrict__ R, int rows,
-                             const float* __restrict__ rrecv, const int* __restrict__ reps,
-                             const float* __restrict__ rep_w, const float* __restrict__ slice_wi,
+__global__ __launch_bounds__(64) void k_slice_maps(int res, SliceReps sr, const float* __restrict__ R, int rows,
+                             const float* __restrict__ rrecv, const float* __restrict__ slice_wi,
                              const int* __restrict__ upper, float eps, int valid_all,
                              float* __restrict__ maps, float* __restrict__ col_cdf,
                              float* __restrict__ row_pdf, float* __restrict__ row_cdf,
@@ -151,6 +165,7 @@ __global__ __launch_bounds__(64) void k_slice_maps(int res, int nreps, const flo
     float* s_lum = s_rows + res + 1;
     int* s_bin = reinterpret_cast<int*>(s_lum + kMapChunk);
     const int s = blockIdx.x, lane = threadIdx.x;
+    const int C = sr.width(), nreps = sr.count(s);
     for (int e = lane; e < res * res; e += 64) s_map[e] = 0.f;
     __syncthreads();
     if (valid_all) {
@@ -162,14 +177,14 @@ __global__ __launch_bounds__(64) void k_slice_maps(int res, int nreps, const flo
             const int cnt = min(kMapChunk, nreps - c0);
             for (int k = lane; k < cnt; k += 64) {
                 const int c = c0 + k;
-                const int64_t t = (int64_t)s * nreps + c;
+                const int64_t t = (int64_t)s * C + c;
                 int bin = -1;
                 float lum = 0.f;
                 if (upper[t]) {
                     v3 lwi = world_to_local(b, ld3(slice_wi + 3 * t));
                     if (!(lwi.z <= 0.0f)) {
-                        const float* e = R + 3 * ((int64_t)reps[c] * rows + s);
-                        v3 rad = ld3(e) * rep_w[c];
+                        const float* e = R + 3 * ((int64_t)sr.rep(s, c) * rows + s);
+                        v3 rad = ld3(e) * sr.weight(s, c);
                         float sx, sy;
                         direction_to_square(lwi, sx, sy);
                         int x = (int)(sx * res), y = (int)(sy * res);
@@ -280,6 +295,16 @@ struct rzk_lightslice {
     DBuf<unsigned long long> pt_work;
     std::vector<int> h_centers, h_assign, h_reps;
     std::vector<float> h_center_w, h_rep_w, h_alphas, h_mags;
+    // per-slice clusters (refinement): S x ld representatives / weights and counts
+    DBuf<int> sreps, scount, cl_off, cl_mem;
+    DBuf<float> srep_w, norms_R;
+    std::vector<int> h_scount;
+    int ld = 0;                        // 0: every slice gathers the kept clusters
+    int cw = 0;                        // row length of the slice_* arrays
+    SliceReps slice_reps() const {
+        if (ld) return SliceReps{sreps.p, srep_w.p, scount.p, ld, ld, T, tx};
+        return SliceReps{reps.p, rep_w.p, nullptr, (int)h_reps.size(), 0, T, tx};
+    }
 };
 
 static void ls_sync(rzk_ctx* ctx) { RZK_HIP(hipStreamSynchronize(ctx->stream)); }
@@ -385,13 +410,13 @@ void enqueue_guide_build(rzk_ctx* ctx, const rzk_scene* scene, const rzk_cluster
     launch_reduced_matrix(ctx, sv, b.recv, b.mat, S, cl.reps->d_order, cl.reps->d_vpl, cl.reps->d_mat, C,
                           gp.clamp_mode, gp.incident_clamp, gp.tmin, b.R, b.V, nullptr);
     int64_t sc = (int64_t)S * C;
-    hipLaunchKernelGGL(k_slice_info, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, st, sv.mat, b.recv, b.mat, S,
-                       b.V, cl.reps->d_vpl, cl.reps->d_mat, cl.d_ident, C, gp.clamp_mode, gp.incident_clamp,
-                       b.slice_wi, b.upper);
+    // the resident representatives are VPLs 0 .. C-1 of cl.reps, shared by every slice
+    const SliceReps gsr{cl.d_ident, cl.d_rep_w, nullptr, C, 0, gp.tile, L.tx};
+    hipLaunchKernelGGL(k_slice_info, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, st, sv.mat, b.recv, S, b.V,
+                       cl.reps->d_vpl, cl.reps->d_mat, gsr, gp.clamp_mode, gp.incident_clamp, b.slice_wi, b.upper);
     size_t lds = sizeof(float) * ((size_t)res * res + res + 1 + 2 * kMapChunk);
-    hipLaunchKernelGGL(k_slice_maps, dim3(S), dim3(64), lds, st, res, C, b.R, S, b.recv, cl.d_ident, cl.d_rep_w,
-                       b.slice_wi, b.upper, gp.map_epsilon, gp.map_gate ? 0 : 1, b.maps, b.col_cdf, b.row_pdf,
-                       b.row_cdf, b.totals);
+    hipLaunchKernelGGL(k_slice_maps, dim3(S), dim3(64), lds, st, res, gsr, b.R, S, b.recv, b.slice_wi, b.upper,
+                       gp.map_epsilon, gp.map_gate ? 0 : 1, b.maps, b.col_cdf, b.row_pdf, b.row_cdf, b.totals);
     RZK_HIP(hipGetLastError());
 }
 
@@ -563,6 +588,52 @@ int rzk_render_vpl(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const fl
     });
 }
 
+int rzk_refine_slices(rzk_ctx* ctx, int32_t S, int64_t V, const float* R, int32_t nnb, const int32_t* nb,
+                      int32_t ncl, const int32_t* cl_off, const int32_t* cl_mem, const int32_t* cl_rep,
+                      const float* cl_w, int32_t max_slice_clusters, const float* draws, int64_t ndraws,
+                      int32_t* out_count, int32_t* out_rep, float* out_w) {
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(ctx && R && nb && cl_off && cl_mem && cl_rep && cl_w && out_count && out_rep && out_w,
+                  RZK_E_INVALID, "null argument");
+        RZK_CHECK(S > 0 && V > 0 && V < INT32_MAX && ncl > 0 && max_slice_clusters >= ncl, RZK_E_INVALID,
+                  "bad refinement sizes");
+        RZK_CHECK(ndraws >= refine_draws_needed(S, ncl, max_slice_clusters) && (ndraws == 0 || draws), RZK_E_INVALID,
+                  "too few random numbers for the refinement");
+        for (int s = 0; s < S * nnb; ++s) RZK_CHECK(nb[s] >= 0 && nb[s] < S, RZK_E_INVALID, "bad neighbour row");
+        RZK_CHECK(cl_off[0] == 0, RZK_E_INVALID, "bad cluster offsets");
+        for (int c = 0; c < ncl; ++c) RZK_CHECK(cl_off[c + 1] > cl_off[c], RZK_E_INVALID, "empty cluster");
+        const int nmem = cl_off[ncl];
+        RZK_CHECK(nmem <= V, RZK_E_INVALID, "more members than VPLs");
+        for (int e = 0; e < nmem; ++e) RZK_CHECK(cl_mem[e] >= 0 && cl_mem[e] < V, RZK_E_INVALID, "bad member");
+        hipStream_t st = ctx->stream;
+        DBuf<float> dR, mags, norms, u, v, c0, alphas, mT, normsR, w;
+        DBuf<int> dnb, doff, dmem, drep, rep, count;
+        dR.alloc(3 * (size_t)S * V); dnb.alloc((size_t)S * nnb); doff.alloc(ncl + 1); dmem.alloc(nmem);
+        drep.alloc(ncl); w.alloc(ncl);
+        mags.alloc(V); norms.alloc(V); u.alloc(S); v.alloc(V); c0.alloc(1); alphas.alloc(V); mT.alloc((size_t)S * V);
+        normsR.alloc(V);
+        rep.alloc((size_t)S * max_slice_clusters); count.alloc(S);
+        DBuf<float> rw;
+        rw.alloc((size_t)S * max_slice_clusters);
+        RZK_HIP(hipMemcpyAsync(dR.p, R, sizeof(float) * 3 * (size_t)S * V, hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(dnb.p, nb, sizeof(int) * (size_t)S * nnb, hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(doff.p, cl_off, sizeof(int) * (ncl + 1), hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(dmem.p, cl_mem, sizeof(int) * nmem, hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(drep.p, cl_rep, sizeof(int) * ncl, hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(w.p, cl_w, sizeof(float) * ncl, hipMemcpyHostToDevice, st));
+        // flattened column magnitudes (the kept clusters' pdfs) and column norms
+        launch_alphas_full(ctx, S, V, dR.p, alphas.p, mags.p, norms.p, u.p, v.p, c0.p, mT.p);
+        launch_column_norms_R(ctx, S, V, dR.p, normsR.p);
+        std::vector<float> h(draws, draws + ndraws);
+        launch_refine_slices(ctx, dR.p, S, dnb.p, nnb, ncl, doff.p, dmem.p, nmem, drep.p, w.p, mags.p, normsR.p,
+                             max_slice_clusters, h, rep.p, rw.p, count.p);
+        RZK_HIP(hipMemcpyAsync(out_count, count.p, sizeof(int) * S, hipMemcpyDeviceToHost, st));
+        RZK_HIP(hipMemcpyAsync(out_rep, rep.p, sizeof(int) * (size_t)S * max_slice_clusters, hipMemcpyDeviceToHost, st));
+        RZK_HIP(hipMemcpyAsync(out_w, rw.p, sizeof(float) * (size_t)S * max_slice_clusters, hipMemcpyDeviceToHost, st));
+        ls_sync(ctx);
+    });
+}
+
 int rzk_lightslice_create(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const rzk_lightslice_params* p,
                           rzk_lightslice** out) {
     return ls_guard(ctx, [&] {
@@ -584,18 +655,25 @@ int rzk_lightslice_create(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, c
         int64_t n = (int64_t)ls->W * ls->H;
         int S = ls->S, C = p->num_clusters, res = p->map_resolution;
         int64_t V = ls->V;
+        RZK_CHECK(p->max_slice_clusters >= 0 && (int64_t)p->max_slice_clusters <= V, RZK_E_INVALID,
+                  "max_slice_clusters must lie in [0, number of VPLs]");
+        const int CW = std::max(C, p->max_slice_clusters);      // widest per-slice cluster list
         ls->rays.alloc(6 * n); ls->recv.alloc(RZK_RECV_FLOATS * n); ls->image.alloc(3 * n);
         ls->hits.alloc(n); ls->mat.alloc(n); ls->light.alloc(n);
         ls->row_pix.alloc(S); ls->rows_mat.alloc(S);
-        ls->vis_bits.alloc((size_t)C * ((ls->W + 7) / 8) * ((ls->H + 7) / 8)); ls->rows_recv.alloc(RZK_RECV_FLOATS * S);
+        ls->vis_bits.alloc((size_t)CW * ((ls->W + 7) / 8) * ((ls->H + 7) / 8)); ls->rows_recv.alloc(RZK_RECV_FLOATS * S);
         ls->R.alloc(3 * (size_t)S * V); ls->Vm.alloc((size_t)S * V);
         ls->alphas.alloc(V); ls->mags.alloc(V); ls->norms.alloc(V); ls->u.alloc(S); ls->v.alloc(V); ls->c0.alloc(1);
         ls->mT.alloc((size_t)S * V);
         ls->centers.alloc(C); ls->center_w.alloc(C); ls->K.alloc(3 * (size_t)S * C); ls->Knorm.alloc(C);
         ls->assign.alloc(V); ls->reps.alloc(C); ls->rep_w.alloc(C);
         ls->nb.alloc((size_t)S * p->num_neighbor_slices);
-        ls->upper.alloc((size_t)S * C); ls->slice_wi.alloc(3 * (size_t)S * C); ls->slice_w.alloc((size_t)S * C);
-        ls->slice_pdf.alloc((size_t)S * C); ls->slice_cdf.alloc((size_t)S * C);
+        ls->upper.alloc((size_t)S * CW); ls->slice_wi.alloc(3 * (size_t)S * CW); ls->slice_w.alloc((size_t)S * CW);
+        ls->slice_pdf.alloc((size_t)S * CW); ls->slice_cdf.alloc((size_t)S * CW);
+        if (p->max_slice_clusters > 0) {
+            ls->sreps.alloc((size_t)S * p->max_slice_clusters); ls->srep_w.alloc((size_t)S * p->max_slice_clusters);
+            ls->scount.alloc(S); ls->norms_R.alloc(V); ls->cl_off.alloc(C + 1); ls->cl_mem.alloc(V);
+        }
         ls->maps.alloc((size_t)S * res * res); ls->col_cdf.alloc((size_t)S * res * res);
         ls->row_pdf.alloc((size_t)S * res); ls->row_cdf.alloc((size_t)S * res); ls->totals.alloc(S);
         std::vector<int> pix, nbh;
@@ -704,22 +782,48 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         int nreps = (int)ls->h_reps.size();
         RZK_HIP(hipMemcpyAsync(ls->reps.p, ls->h_reps.data(), sizeof(int) * nreps, hipMemcpyHostToDevice, st));
         RZK_HIP(hipMemcpyAsync(ls->rep_w.p, ls->h_rep_w.data(), sizeof(float) * nreps, hipMemcpyHostToDevice, st));
+        // per-slice refinement (lightslice.cc:837-902) of the kept clusters, the
+        // MT stream continuing after the representatives
+        ls->ld = 0;
+        if (p.max_slice_clusters > nreps && nreps > 0) {
+            std::vector<int> koff(1, 0), kmem;
+            kmem.reserve(V);
+            for (int c = 0; c < ncl; ++c) {
+                if (cnt[c] == 0) continue;                 // dropped: no representative
+                kmem.insert(kmem.end(), mem.begin() + off[c], mem.begin() + off[c] + cnt[c]);
+                koff.push_back((int)kmem.size());
+            }
+            RZK_HIP(hipMemcpyAsync(ls->cl_off.p, koff.data(), sizeof(int) * koff.size(), hipMemcpyHostToDevice, st));
+            RZK_HIP(hipMemcpyAsync(ls->cl_mem.p, kmem.data(), sizeof(int) * kmem.size(), hipMemcpyHostToDevice, st));
+            launch_column_norms_R(ctx, S, V, ls->R.p, ls->norms_R.p);
+            std::vector<float> draws((size_t)refine_draws_needed(S, nreps, p.max_slice_clusters));
+            for (auto& x : draws) x = rd.next_float();
+            launch_refine_slices(ctx, ls->R.p, S, ls->nb.p, p.num_neighbor_slices, nreps, ls->cl_off.p, ls->cl_mem.p,
+                                 (int)kmem.size(), ls->reps.p, ls->rep_w.p, ls->mags.p, ls->norms_R.p,
+                                 p.max_slice_clusters, draws, ls->sreps.p, ls->srep_w.p, ls->scount.p);
+            ls->ld = p.max_slice_clusters;
+            ls->h_scount.resize(S);
+            RZK_HIP(hipMemcpyAsync(ls->h_scount.data(), ls->scount.p, sizeof(int) * S, hipMemcpyDeviceToHost, st));
+        } else {
+            ls->h_scount.assign(S, nreps);
+        }
+        const SliceReps sr = ls->slice_reps();
+        ls->cw = sr.width();
         // per-slice cluster data, pdfs and luminance maps
-        int64_t sc = (int64_t)S * nreps;
+        int64_t sc = (int64_t)S * ls->cw;
         if (nreps > 0) {
             hipLaunchKernelGGL(k_slice_info, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, st, sv.mat, ls->rows_recv.p,
-                               ls->rows_mat.p, S, ls->Vm.p, ls->vpls->d_vpl, ls->vpls->d_mat, ls->reps.p, nreps,
-                               p.clamp_mode, p.incident_clamp, ls->slice_wi.p, ls->upper.p);
+                               S, ls->Vm.p, ls->vpls->d_vpl, ls->vpls->d_mat, sr, p.clamp_mode, p.incident_clamp,
+                               ls->slice_wi.p, ls->upper.p);
             hipLaunchKernelGGL(k_slice_pdf_weights, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, st, ls->R.p, S,
-                               ls->nb.p, p.num_neighbor_slices, ls->reps.p, ls->rep_w.p, nreps, ls->upper.p,
-                               ls->slice_w.p);
-            hipLaunchKernelGGL(k_slice_pdf_norm, dim3(S), dim3(64), 0, st, S, nreps, ls->slice_w.p,
+                               ls->nb.p, p.num_neighbor_slices, sr, ls->upper.p, ls->slice_w.p);
+            hipLaunchKernelGGL(k_slice_pdf_norm, dim3(S), dim3(64), 0, st, S, sr, ls->slice_w.p,
                                ls->slice_pdf.p, ls->slice_cdf.p);
         }
         int res = p.map_resolution;
         size_t lds = sizeof(float) * ((size_t)res * res + res + 1 + 2 * kMapChunk);
-        hipLaunchKernelGGL(k_slice_maps, dim3(S), dim3(64), lds, st, res, nreps, ls->R.p, S, ls->rows_recv.p,
-                           ls->reps.p, ls->rep_w.p, ls->slice_wi.p, ls->upper.p, p.map_epsilon, p.map_gate ? 0 : 1,
+        hipLaunchKernelGGL(k_slice_maps, dim3(S), dim3(64), lds, st, res, sr, ls->R.p, S, ls->rows_recv.p,
+                           ls->slice_wi.p, ls->upper.p, p.map_epsilon, p.map_gate ? 0 : 1,
                            ls->maps.p, ls->col_cdf.p, ls->row_pdf.p, ls->row_cdf.p, ls->totals.p);
         RZK_HIP(hipGetLastError());
         ls_sync(ctx);
@@ -738,9 +842,9 @@ int rzk_lightslice_gather(rzk_lightslice* ls, float* out_rgb) {
         RZK_CHECK(ls->initialized, RZK_E_STATE, "rzk_lightslice_initialize has not run");
         const rzk_lightslice_params& p = ls->prm;
         SceneView sv = scene_view(ls->scene);
-        launch_slice_gather(ctx, sv, ls->recv.p, ls->mat.p, ls->W, ls->H, ls->T, ls->vpls->d_vpl, ls->vpls->d_mat,
-                            ls->reps.p, ls->rep_w.p, (int)ls->h_reps.size(), p.clamp_mode, p.clamp, p.tmin,
-                            p.n_emitted, ls->image.p, ls->vis_bits.p, nullptr);
+        launch_slice_gather(ctx, sv, ls->recv.p, ls->mat.p, ls->W, ls->H, ls->vpls->d_vpl, ls->vpls->d_mat,
+                            ls->slice_reps(), p.clamp_mode, p.clamp, p.tmin, p.n_emitted, ls->image.p,
+                            ls->vis_bits.p, nullptr);
         int64_t n = (int64_t)ls->W * ls->H;
         if (out_rgb) {
             RZK_HIP(hipMemcpyAsync(out_rgb, ls->image.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
@@ -831,8 +935,8 @@ int rzk_lightslice_count_work(rzk_lightslice* ls, int64_t* out) {
         RZK_HIP(hipMemsetAsync(tot.p, 0, sizeof(unsigned long long) * 4, ctx->stream));
         launch_reduced_matrix(ctx, sv, ls->rows_recv.p, ls->rows_mat.p, S, ls->vpls->d_order, ls->vpls->d_vpl, ls->vpls->d_mat, V,
                               p.clamp_mode, p.incident_clamp, p.tmin, ls->R.p, ls->Vm.p, (int*)tot.p);
-        launch_slice_gather(ctx, sv, ls->recv.p, ls->mat.p, ls->W, ls->H, ls->T, ls->vpls->d_vpl, ls->vpls->d_mat,
-                            ls->reps.p, ls->rep_w.p, nreps, p.clamp_mode, p.clamp, p.tmin, p.n_emitted, ls->image.p,
+        launch_slice_gather(ctx, sv, ls->recv.p, ls->mat.p, ls->W, ls->H, ls->vpls->d_vpl, ls->vpls->d_mat,
+                            ls->slice_reps(), p.clamp_mode, p.clamp, p.tmin, p.n_emitted, ls->image.p,
                             ls->vis_bits.p, (int*)(tot.p + 2));
         unsigned long long h[4];
         std::vector<int> rm(S), pm(n);
@@ -840,14 +944,15 @@ int rzk_lightslice_count_work(rzk_lightslice* ls, int64_t* out) {
         RZK_HIP(hipMemcpyAsync(rm.data(), ls->rows_mat.p, sizeof(int) * S, hipMemcpyDeviceToHost, ctx->stream));
         RZK_HIP(hipMemcpyAsync(pm.data(), ls->mat.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
         ls_sync(ctx);
-        int64_t rows_ok = 0, pix_ok = 0;
+        int64_t rows_ok = 0, pairs = 0;
         for (int v : rm) rows_ok += v >= 0;
-        for (int v : pm) pix_ok += v >= 0;
+        for (int64_t i = 0; i < n; ++i)        // pixels with a material x their slice's clusters
+            if (pm[i] >= 0) pairs += ls->h_scount[((i / ls->W) / ls->T) * ls->tx + (i % ls->W) / ls->T];
         out[0] = n;
         out[1] = rows_ok * V;
         out[2] = (int64_t)h[0];
         out[3] = (int64_t)h[1];
-        out[4] = pix_ok * nreps;
+        out[4] = pairs;
         out[5] = (int64_t)h[2];
         out[6] = (int64_t)h[3];
         out[7] = nreps;
@@ -868,7 +973,8 @@ int rzk_lightslice_get(const rzk_lightslice* ls, const char* name, void* dst, in
         std::string nm(name);
         int S = ls->S, res = ls->prm.map_resolution;
         int64_t V = ls->V, n = (int64_t)ls->W * ls->H;
-        int nreps = (int)ls->h_reps.size();
+        const int64_t cw = ls->cw;
+        const int nreps = (int)ls->h_reps.size();
         const void* src = nullptr;
         int64_t need = 0;
         bool host = false;
@@ -883,10 +989,26 @@ int rzk_lightslice_get(const rzk_lightslice* ls, const char* name, void* dst, in
         else if (nm == "rep_w") { src = ls->h_rep_w.data(); need = 4 * (int64_t)nreps; host = true; }
         else if (nm == "rows_recv") { src = ls->rows_recv.p; need = 4 * (int64_t)RZK_RECV_FLOATS * S; }
         else if (nm == "rows_mat") { src = ls->rows_mat.p; need = 4 * (int64_t)S; }
-        else if (nm == "slice_pdf") { src = ls->slice_pdf.p; need = 4 * (int64_t)S * nreps; }
-        else if (nm == "slice_cdf") { src = ls->slice_cdf.p; need = 4 * (int64_t)S * nreps; }
-        else if (nm == "slice_upper") { src = ls->upper.p; need = 4 * (int64_t)S * nreps; }
-        else if (nm == "slice_wi") { src = ls->slice_wi.p; need = 12 * (int64_t)S * nreps; }
+        else if (nm == "slice_pdf") { src = ls->slice_pdf.p; need = 4 * (int64_t)S * cw; }
+        else if (nm == "slice_cdf") { src = ls->slice_cdf.p; need = 4 * (int64_t)S * cw; }
+        else if (nm == "slice_upper") { src = ls->upper.p; need = 4 * (int64_t)S * cw; }
+        else if (nm == "slice_wi") { src = ls->slice_wi.p; need = 12 * (int64_t)S * cw; }
+        else if (nm == "slice_count") { src = ls->h_scount.data(); need = 4 * (int64_t)ls->h_scount.size(); host = true; }
+        else if (nm == "slice_reps" || nm == "slice_rep_w") {
+            need = 4 * (int64_t)S * cw;
+            RZK_CHECK(bytes == need, RZK_E_INVALID, "size mismatch for " + nm + ": need " + std::to_string(need));
+            if (ls->ld) {
+                RZK_HIP(hipMemcpyAsync(dst, nm == "slice_reps" ? (const void*)ls->sreps.p : (const void*)ls->srep_w.p,
+                                       (size_t)need, hipMemcpyDeviceToHost, ctx->stream));
+                ls_sync(ctx);
+            } else {
+                for (int s = 0; s < S; ++s)
+                    std::memcpy((char*)dst + (size_t)s * cw * 4,
+                                nm == "slice_reps" ? (const void*)ls->h_reps.data() : (const void*)ls->h_rep_w.data(),
+                                (size_t)cw * 4);
+            }
+            return;
+        }
         else if (nm == "maps") { src = ls->maps.p; need = 4 * (int64_t)S * res * res; }
         else if (nm == "map_totals") { src = ls->totals.p; need = 4 * (int64_t)S; }
         else if (nm == "row_cdf") { src = ls->row_cdf.p; need = 4 * (int64_t)S * res; }

@@ -171,8 +171,28 @@ void launch_alphas(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float
                    float* d_mags);
 void launch_cluster_assign(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl,
                            const int* d_centers, const float* d_weights, int* d_assign);
+// The clusters each slice gathers from: one list shared by every slice
+// (ld = 0: nreps representatives), or a list per slice after refinement
+// (row s of length ld holds counts[s] representatives).  Slices are tile x
+// tile pixel squares, tiles_x per image row.
+struct SliceReps {
+    const int* reps;
+    const float* rep_w;
+    const int* counts;
+    int nreps;
+    int ld;
+    int tile, tiles_x;
+    __host__ __device__ int width() const { return ld ? ld : nreps; }
+    __device__ int slice_of(int64_t pix, int img_width) const {
+        int px = (int)(pix % img_width), py = (int)(pix / img_width);
+        return (py / tile) * tiles_x + px / tile;
+    }
+    __device__ int count(int s) const { return ld ? counts[s] : nreps; }
+    __device__ int rep(int s, int c) const { return reps[(int64_t)s * ld + c]; }
+    __device__ float weight(int s, int c) const { return rep_w[(int64_t)s * ld + c]; }
+};
+
 void launch_slice_gather(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
-                         const int* d_rmat, int width, int height, int tile,
-                         const float* d_vpl, const int* d_vmat, const int* d_reps,
-                         const float* d_rep_w, int nreps, int clamp_mode, float clamp,
+                         const int* d_rmat, int width, int height, const float* d_vpl, const int* d_vmat,
+                         const SliceReps& sr, int clamp_mode, float clamp,
                          float tmin, float n_emitted, float* d_out, uint64_t* d_bits, int* d_stats);

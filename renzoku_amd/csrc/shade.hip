@@ -217,16 +217,20 @@ __device__ __forceinline__ int64_t tile_pixel(int64_t tile, int lane, int width,
 }
 
 // ---- LightSlice::gather_clusters, phase 1: visibility of every (pixel,
-// representative) pair.  One wave = one 8x8 pixel tile; it walks the
-// representatives in order and records a 64-bit blocked mask per
-// representative: bits[c * ntiles + tile].  Only the segment endpoints are
-// live here, which keeps the traversal kernel small (high occupancy).
-template <bool kCount>
+// representative) pair.  One wave = one 8x8 pixel tile and a share of the
+// representatives (blockIdx.y); it records a 64-bit blocked mask per
+// representative: bits[c * ntiles + tile], bit = lane.  Only the segment
+// endpoints are live here, which keeps the traversal kernel small (high
+// occupancy).  kShared: one representative list for every slice, staged in
+// LDS per block; otherwise each lane reads its own slice's list (clusters
+// refined per slice).  A pixel without a material, or a slice with fewer
+// representatives than c, gets a set bit.
+template <bool kCount, bool kShared>
 __global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, const float* __restrict__ recv,
                                                                  const int* __restrict__ rmat, int width,
                                                                  int height, const float* __restrict__ vpl,
-                                                                 const int* __restrict__ reps, int nreps,
-                                                                 float tmin, uint64_t* __restrict__ bits,
+                                                                 SliceReps sr, float tmin,
+                                                                 uint64_t* __restrict__ bits,
                                                                  int* __restrict__ d_stats) {
     extern __shared__ int s_dyn[];
     float* s_q = reinterpret_cast<float*>(s_dyn);            // kVplTile positions
@@ -238,21 +242,29 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, 
     int64_t i = tile < ntiles ? tile_pixel(tile, lane, width, height) : -1;
     int m = i >= 0 ? rmat[i] : -1;
     v3 p = m >= 0 ? ld3(recv + RZK_RECV_FLOATS * i) : mk(0.f, 0.f, 0.f);
+    const int sl = (!kShared && i >= 0) ? sr.slice_of(i, width) : 0;
+    const int mine = i >= 0 ? sr.count(sl) : 0;
     TravStats st;
     // blockIdx.y picks this block's share of the representatives
-    const int per = (nreps + gridDim.y - 1) / gridDim.y;
-    const int c_begin = (int)blockIdx.y * per, c_end = min(nreps, c_begin + per);
+    const int C = sr.width();
+    const int per = (C + gridDim.y - 1) / gridDim.y;
+    const int c_begin = (int)blockIdx.y * per, c_end = min(C, c_begin + per);
     for (int c0 = c_begin; c0 < c_end; c0 += kVplTile) {
         int cnt = (c_end - c0) < kVplTile ? (c_end - c0) : kVplTile;
-        __syncthreads();
-        for (int e = threadIdx.x; e < cnt * 3; e += blockDim.x) {
-            int c = e / 3, f = e - c * 3;
-            s_q[e] = vpl[(int64_t)reps[c0 + c] * RZK_VPL_FLOATS + f];
+        if (kShared) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < cnt * 3; e += blockDim.x) {
+                int c = e / 3, f = e - c * 3;
+                s_q[e] = vpl[(int64_t)sr.reps[c0 + c] * RZK_VPL_FLOATS + f];
+            }
+            __syncthreads();
         }
-        __syncthreads();
         for (int cc = 0; cc < cnt; ++cc) {
-            bool blocked = segment_blocked_wave(sv, p, mk(s_q[3 * cc], s_q[3 * cc + 1], s_q[3 * cc + 2]), tmin,
-                                                m >= 0, wstk, st) || m < 0;
+            const bool active = m >= 0 && c0 + cc < mine;
+            v3 q;
+            if (kShared) q = mk(s_q[3 * cc], s_q[3 * cc + 1], s_q[3 * cc + 2]);
+            else q = active ? ld3(vpl + (int64_t)sr.rep(sl, c0 + cc) * RZK_VPL_FLOATS) : mk(0.f, 0.f, 0.f);
+            bool blocked = segment_blocked_wave(sv, p, q, tmin, active, wstk, st) || !active;
             uint64_t b = __ballot(blocked);
             if (lane == 0 && tile < ntiles) bits[(int64_t)(c0 + cc) * ntiles + tile] = b;
         }
@@ -262,14 +274,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, 
 
 // ---- phase 2: Σ_c weight_c * gather_point(rep_c) over the visible
 // representatives, in cluster order, then / num_emitted (lightslice.cc:1325-1339,
-// vpl.cc:488-489).  Same pixel-to-lane mapping as phase 1.
+// vpl.cc:488-489).  Same pixel-to-lane mapping as phase 1; kShared as there.
+template <bool kShared>
 __global__ __launch_bounds__(kShadeBlock) void k_slice_shade(const float* __restrict__ mat,
                                                             const float* __restrict__ recv,
                                                             const int* __restrict__ rmat, int width, int height,
                                                             const float* __restrict__ vpl,
-                                                            const int* __restrict__ vmat,
-                                                            const int* __restrict__ reps,
-                                                            const float* __restrict__ rep_w, int nreps,
+                                                            const int* __restrict__ vmat, SliceReps sr,
                                                             const uint64_t* __restrict__ bits, int mode,
                                                             float thr, float n_emitted, float* __restrict__ out) {
     __shared__ float s_vpl[kVplTile * (kVplRec + 1)];
@@ -282,28 +293,42 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_shade(const float* __rest
     float rc[RZK_RECV_FLOATS];
     if (m >= 0) load_record(rc, recv + RZK_RECV_FLOATS * i, RZK_RECV_FLOATS);
     v3 acc = mk(0.f, 0.f, 0.f);
-    for (int c0 = 0; c0 < nreps; c0 += kVplTile) {
-        int cnt = (nreps - c0) < kVplTile ? (nreps - c0) : kVplTile;
-        __syncthreads();
-        for (int e = threadIdx.x; e < cnt * (kVplRec + 1); e += blockDim.x) {
-            int c = e / (kVplRec + 1), f = e - c * (kVplRec + 1);
-            int k = reps[c0 + c];
-            float v;
-            if (f < RZK_VPL_FLOATS) v = vpl[(int64_t)k * RZK_VPL_FLOATS + f];
-            else if (f == RZK_VPL_FLOATS) v = __int_as_float(vmat[k]);
-            else v = rep_w[c0 + c];
-            s_vpl[e] = v;
-        }
-        __syncthreads();
-        if (m >= 0) {
-            for (int cc = 0; cc < cnt; ++cc) {
-                uint64_t w = bits[(int64_t)(c0 + cc) * ntiles + tile];
-                if ((w >> lane) & 1ull) continue;
-                float vp[kVplRec + 1];
-                load_record(vp, s_vpl + cc * (kVplRec + 1), kVplRec + 1);
-                v3 lo = gather_shade(mat, rc, m, vp, __float_as_int(vp[21]), mode, thr);
-                acc = acc + lo * vp[22];
+    if (kShared) {
+        const int nreps = sr.nreps;
+        for (int c0 = 0; c0 < nreps; c0 += kVplTile) {
+            int cnt = (nreps - c0) < kVplTile ? (nreps - c0) : kVplTile;
+            __syncthreads();
+            for (int e = threadIdx.x; e < cnt * (kVplRec + 1); e += blockDim.x) {
+                int c = e / (kVplRec + 1), f = e - c * (kVplRec + 1);
+                int k = sr.reps[c0 + c];
+                float v;
+                if (f < RZK_VPL_FLOATS) v = vpl[(int64_t)k * RZK_VPL_FLOATS + f];
+                else if (f == RZK_VPL_FLOATS) v = __int_as_float(vmat[k]);
+                else v = sr.rep_w[c0 + c];
+                s_vpl[e] = v;
             }
+            __syncthreads();
+            if (m >= 0) {
+                for (int cc = 0; cc < cnt; ++cc) {
+                    uint64_t w = bits[(int64_t)(c0 + cc) * ntiles + tile];
+                    if ((w >> lane) & 1ull) continue;
+                    float vp[kVplRec + 1];
+                    load_record(vp, s_vpl + cc * (kVplRec + 1), kVplRec + 1);
+                    v3 lo = gather_shade(mat, rc, m, vp, __float_as_int(vp[21]), mode, thr);
+                    acc = acc + lo * vp[22];
+                }
+            }
+        }
+    } else if (m >= 0) {
+        const int sl = sr.slice_of(i, width), n = sr.count(sl);
+        for (int c = 0; c < n; ++c) {
+            uint64_t w = bits[(int64_t)c * ntiles + tile];
+            if ((w >> lane) & 1ull) continue;
+            const int k = sr.rep(sl, c);
+            float vp[RZK_VPL_FLOATS];
+            load_record(vp, vpl + (int64_t)k * RZK_VPL_FLOATS, RZK_VPL_FLOATS);
+            v3 lo = gather_shade(mat, rc, m, vp, vmat[k], mode, thr);
+            acc = acc + lo * sr.weight(sl, c);
         }
     }
     if (i >= 0) {
@@ -376,11 +401,9 @@ void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv_in, const float* d_
 }
 
 void launch_slice_gather(rzk_ctx* ctx, const SceneView& sv_in, const float* d_recv, const int* d_rmat,
-                         int width, int height, int tile, const float* d_vpl, const int* d_vmat,
-                         const int* d_reps, const float* d_rep_w, int nreps, int clamp_mode,
-                         float clamp, float tmin, float n_emitted, float* d_out, uint64_t* d_bits,
+                         int width, int height, const float* d_vpl, const int* d_vmat, const SliceReps& sr,
+                         int clamp_mode, float clamp, float tmin, float n_emitted, float* d_out, uint64_t* d_bits,
                          int* d_stats) {
-    (void)tile;
     int64_t tiles = (int64_t)((width + 7) / 8) * ((height + 7) / 8);
     int64_t threads = tiles * 64;
     if (threads <= 0) return;
@@ -392,15 +415,27 @@ void launch_slice_gather(rzk_ctx* ctx, const SceneView& sv_in, const float* d_re
     // keep every compute unit full and hide the node-fetch latency of the
     // wave-coherent walk (measured: one wave per tile 23.1 ms, four
     // representatives per wave 7.3 ms for 512x512 pixels x 300 representatives)
-    const int groups = std::max(1, std::min((nreps + 3) / 4, 65535));
-    if (d_stats)
-        hipLaunchKernelGGL(k_slice_visibility<true>, dim3(grid, groups), dim3(kShadeBlock), lds, ctx->stream, sv,
-                           d_recv, d_rmat, width, height, d_vpl, d_reps, nreps, tmin, d_bits, d_stats);
+    const int groups = std::max(1, std::min((sr.width() + 3) / 4, 65535));
+    const bool shared = sr.ld == 0;
+    dim3 g2(grid, groups);
+    if (d_stats && shared)
+        hipLaunchKernelGGL((k_slice_visibility<true, true>), g2, dim3(kShadeBlock), lds, ctx->stream, sv, d_recv, d_rmat,
+                           width, height, d_vpl, sr, tmin, d_bits, d_stats);
+    else if (d_stats)
+        hipLaunchKernelGGL((k_slice_visibility<true, false>), g2, dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
+                           d_rmat, width, height, d_vpl, sr, tmin, d_bits, d_stats);
+    else if (shared)
+        hipLaunchKernelGGL((k_slice_visibility<false, true>), g2, dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
+                           d_rmat, width, height, d_vpl, sr, tmin, d_bits, d_stats);
     else
-        hipLaunchKernelGGL(k_slice_visibility<false>, dim3(grid, groups), dim3(kShadeBlock), lds, ctx->stream, sv,
-                           d_recv, d_rmat, width, height, d_vpl, d_reps, nreps, tmin, d_bits, d_stats);
+        hipLaunchKernelGGL((k_slice_visibility<false, false>), g2, dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
+                           d_rmat, width, height, d_vpl, sr, tmin, d_bits, d_stats);
     RZK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_slice_shade, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, sv.mat, d_recv, d_rmat, width,
-                       height, d_vpl, d_vmat, d_reps, d_rep_w, nreps, d_bits, clamp_mode, clamp, n_emitted, d_out);
+    if (shared)
+        hipLaunchKernelGGL(k_slice_shade<true>, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, sv.mat, d_recv, d_rmat,
+                           width, height, d_vpl, d_vmat, sr, d_bits, clamp_mode, clamp, n_emitted, d_out);
+    else
+        hipLaunchKernelGGL(k_slice_shade<false>, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, sv.mat, d_recv, d_rmat,
+                           width, height, d_vpl, d_vmat, sr, d_bits, clamp_mode, clamp, n_emitted, d_out);
     RZK_HIP(hipGetLastError());
 }
