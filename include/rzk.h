@@ -334,9 +334,10 @@ int rzk_batch_render(rzk_batch *batch, const rzk_render_job *jobs, int32_t n, fl
 /* time candidate configurations on `jobs` (the engine, then coordinate
  * descent over its launch shape: block, blocks_per_cu and path_slots for the
  * wavefront engine; samples_per_item, block, blocks_per_cu and
- * items_per_dequeue for the persistent one), keep the fastest; tried (optional, max_tried entries) receives each candidate with
- * its time in ms in rzk_batch_trial records; returns the number tried in
- * *n_tried. */
+ * items_per_dequeue for the persistent one), keep the fastest; each candidate
+ * is timed once (the fastest of three renders after a warm-up); tried
+ * (optional, max_tried entries) receives each candidate with its time in ms
+ * in rzk_batch_trial records; returns the number tried in *n_tried. */
 typedef struct rzk_batch_trial { rzk_batch_config cfg; float ms; } rzk_batch_trial;
 int rzk_batch_autotune(rzk_batch *batch, const rzk_render_job *jobs, int32_t n, rzk_batch_config *best,
                        rzk_batch_trial *tried, int32_t max_tried, int32_t *n_tried);

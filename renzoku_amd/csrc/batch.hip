@@ -265,6 +265,7 @@ int rzk_batch_create(rzk_ctx* ctx, const rzk_scene* scene, const rzk_batch_confi
         b->h_counters.reserve(3);
         RZK_HIP(hipEventCreate(&b->ev0));
         RZK_HIP(hipEventCreate(&b->ev1));
+        ctx_retain(ctx);
         *out = b.release();
     });
 }
@@ -272,7 +273,9 @@ int rzk_batch_create(rzk_ctx* ctx, const rzk_scene* scene, const rzk_batch_confi
 int rzk_batch_destroy(rzk_batch* b) {
     if (!b) return RZK_OK;
     (void)hipStreamSynchronize(b->ctx->stream);
+    rzk_ctx* ctx = b->ctx;
     delete b;
+    ctx_release(ctx);
     return RZK_OK;
 }
 
@@ -299,6 +302,9 @@ int rzk_batch_render(rzk_batch* b, const rzk_render_job* jobs, int32_t n, float*
     return batch_guard(b->ctx, [&] { batch_render(b, jobs, n, out_rgb, out_rays, out_ms); });
 }
 
+// timed renders per autotuner candidate (after one warm-up render)
+static constexpr int kTuneRuns = 3;
+
 int rzk_batch_autotune(rzk_batch* b, const rzk_render_job* jobs, int32_t n, rzk_batch_config* best,
                        rzk_batch_trial* tried, int32_t max_tried, int32_t* n_tried) {
     if (!b) return RZK_E_INVALID;
@@ -309,18 +315,25 @@ int rzk_batch_autotune(rzk_batch* b, const rzk_render_job* jobs, int32_t n, rzk_
             bool shape = !same_launch(c, b->cfg);
             b->cfg = c;
             if (shape) b->drop_graph();
-            float ms = 0.f;
             batch_render(b, jobs, n, nullptr, nullptr, nullptr);   // instantiate + warm
-            batch_render(b, jobs, n, nullptr, nullptr, &ms);
+            float ms = 0.f;
+            for (int r = 0; r < kTuneRuns; ++r) {                  // the fastest of a few runs
+                float t = 0.f;
+                batch_render(b, jobs, n, nullptr, nullptr, &t);
+                ms = r == 0 ? t : std::min(ms, t);
+            }
             if (tried && count < max_tried) tried[count] = rzk_batch_trial{c, ms};
             count++;
             return ms;
         };
         rzk_batch_config cur = b->cfg;
         float cur_ms = time_cfg(cur);
+        // one field at a time; the value a stage starts from is already timed
+        // (every candidate differs from all earlier ones)
         auto stage = [&](int rzk_batch_config::*field, std::initializer_list<int> values) {
+            const int start = cur.*field;
             for (int v : values) {
-                if (cur.*field == v) continue;
+                if (v == start) continue;
                 rzk_batch_config c = cur;
                 c.*field = v;
                 float ms = time_cfg(c);

@@ -108,6 +108,17 @@ void* ctx_scratch(rzk_ctx* ctx, size_t bytes) {
     return ctx->scratch;
 }
 
+void ctx_retain(rzk_ctx* ctx) { ctx->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void ctx_release(rzk_ctx* ctx) {
+    if (!ctx || ctx->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
 void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries) {
     int64_t extra = (entries > 0 ? entries : sv.stack4) - kLdsStack;
     if (extra <= 0 || threads <= 0) { sv.spill = nullptr; sv.spill_pitch = 0; return; }
@@ -134,11 +145,7 @@ int rzk_ctx_create(int device, rzk_ctx** out) {
 }
 
 int rzk_ctx_destroy(rzk_ctx* ctx) {
-    if (!ctx) return RZK_OK;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->scratch) (void)hipFree(ctx->scratch);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    ctx_release(ctx);
     return RZK_OK;
 }
 
@@ -291,6 +298,7 @@ int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
         s->nltri = (int)s->h_ltris.size();
         up((void**)&s->d_ltris, s->h_ltris.empty() ? nullptr : s->h_ltris.data(), sizeof(LightTri) * s->h_ltris.size());
         up((void**)&s->d_lights, s->h_lights.empty() ? nullptr : s->h_lights.data(), sizeof(LightGpu) * s->h_lights.size());
+        ctx_retain(ctx);
         *out = s.release();
     });
 }
@@ -302,7 +310,9 @@ int rzk_scene_destroy(rzk_scene* s) {
                     s->d_ltris, s->d_lights};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    rzk_ctx* ctx = s->ctx;
     delete s;
+    ctx_release(ctx);
     return RZK_OK;
 }
 
@@ -512,6 +522,7 @@ int rzk_vpls_create(rzk_ctx* ctx, const float* vpl, const int32_t* vpl_mat, int6
             RZK_HIP(hipMemcpy(v->d_mat, vpl_mat, sizeof(int) * n, hipMemcpyHostToDevice));
             RZK_HIP(hipMemcpy(v->d_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
         }
+        ctx_retain(ctx);
         *out = v.release();
     });
 }
@@ -521,7 +532,9 @@ int rzk_vpls_destroy(rzk_vpls* v) {
     if (v->d_vpl) (void)hipFree(v->d_vpl);
     if (v->d_mat) (void)hipFree(v->d_mat);
     if (v->d_order) (void)hipFree(v->d_order);
+    rzk_ctx* ctx = v->ctx;
     delete v;
+    ctx_release(ctx);
     return RZK_OK;
 }
 

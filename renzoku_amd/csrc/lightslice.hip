@@ -469,6 +469,7 @@ int rzk_lightslice_clusters(const rzk_lightslice* ls, rzk_clusters** out) {
         cl->n_emitted = p.n_emitted;
         cl->gp = rzk_guide_params{p.tile, p.map_resolution, p.map_epsilon, p.clamp_mode, p.incident_clamp, p.tmin,
                                   p.map_gate};
+        ctx_retain(ctx);
         *out = cl.release();
     });
 }
@@ -479,7 +480,9 @@ int rzk_clusters_destroy(rzk_clusters* cl) {
     rzk_vpls_destroy(cl->reps);
     if (cl->d_rep_w) (void)hipFree(cl->d_rep_w);
     if (cl->d_ident) (void)hipFree(cl->d_ident);
+    rzk_ctx* ctx = cl->ctx;
     delete cl;
+    ctx_release(ctx);
     return RZK_OK;
 }
 
@@ -509,6 +512,7 @@ int rzk_guide_create(rzk_ctx* ctx, const rzk_scene* scene, const rzk_clusters* c
         }
         RZK_HIP(hipFree(tmp));
         guide_set(*g, scene, camera, gp, L, b);
+        ctx_retain(ctx);
         *out = g.release();
     });
 }
@@ -683,12 +687,16 @@ int rzk_lightslice_create(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, c
         RZK_HIP(hipMemcpy(ls->nb.p, nbh.data(), sizeof(int) * nbh.size(), hipMemcpyHostToDevice));
         std::memset(&ls->info, 0, sizeof(ls->info));
         ls->info.num_slices = S;
+        ctx_retain(ctx);
         *out = ls.release();
     });
 }
 
 int rzk_lightslice_destroy(rzk_lightslice* ls) {
+    if (!ls) return RZK_OK;
+    rzk_ctx* ctx = ls->ctx;
     delete ls;
+    ctx_release(ctx);
     return RZK_OK;
 }
 
@@ -908,6 +916,7 @@ int rzk_lightslice_guide(rzk_lightslice* ls, const float* camera, rzk_guide** ou
         g->res = ls->prm.map_resolution;
         g->gv.maps = ls->maps.p; g->gv.row_cdf = ls->row_cdf.p; g->gv.col_cdf = ls->col_cdf.p;
         g->gv.totals = ls->totals.p;
+        ctx_retain(ctx);
         *out = g.release();
     });
 }
@@ -916,7 +925,9 @@ int rzk_guide_destroy(rzk_guide* g) {
     if (!g) return RZK_OK;
     (void)hipStreamSynchronize(g->ctx->stream);
     for (void* p : g->owned) (void)hipFree(p);
+    rzk_ctx* ctx = g->ctx;
     delete g;
+    ctx_release(ctx);
     return RZK_OK;
 }
 

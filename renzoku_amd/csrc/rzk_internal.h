@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -64,7 +65,14 @@ struct rzk_ctx {
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     int num_cus = 0;
+    // held by the creator and by every object created on the context; the
+    // stream and scratch go when the last holder lets go, so handles may be
+    // destroyed in any order
+    std::atomic<int> refs{1};
 };
+
+void ctx_retain(rzk_ctx* ctx);
+void ctx_release(rzk_ctx* ctx);
 
 struct rzk_scene {
     rzk_ctx* ctx = nullptr;
