@@ -8,12 +8,15 @@
 //
 // One 256-lane workgroup refines one slice.  The numeric steps run across the
 // lanes with every floating-point sum kept in the reference's order: columns
-// are gathered into LDS 64 at a time; per-column sums (norms, projections,
-// products) run one lane per column, per-row sums one lane per row, and the
-// sums over the cluster (the cost, the norm total, the pdf total and cdf) run
-// as one identical chain in every lane.  The sort of the projections and the
-// cost heap follow std::sort and std::priority_queue move for move (one lane,
-// std_order.h), because the reference's result depends on their tie orders.
+// are gathered into LDS up to 256 at a time, one lane per column; per-column
+// sums (norms, projections, products) run one lane per column, per-row sums
+// one lane per row, and the sums over the cluster (the cost, the norm total,
+// the pdf total and cdf) run as one identical chain in every lane over values
+// held in LDS.  The sort of the projections follows std::sort's moves (the
+// block-parallel form in par_sort.h, up to 4,096 members; the one-lane form in
+// std_order.h beyond) and the cost heap std::priority_queue's (one lane, on a
+// table in LDS when it fits), because the reference's result depends on their
+// tie orders.
 //
 // Draws: the reference takes them from one MT stream, slice after slice.  A
 // slice takes 4 per split as a rule (2 to pick the line, 1 per new
@@ -27,20 +30,21 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <memory>
 
 #include "host_sampling.h"
 #include "rzk_device.h"
+#include "par_sort.h"
 #include "std_order.h"
 
 namespace rzk {
 
 constexpr int kRefBlock = 256;
-constexpr int kRefChunk = 64;          // columns gathered per pass
-constexpr int kRefSortLds = 4096;      // projections sorted in LDS (longer lines sort in HBM scratch)
+constexpr int kRefSortLds = 4096;      // clusters of up to this many members are worked on in LDS
 constexpr int kRefMaxNb = 32;          // neighbour rows per slice (3 * 32 flattened rows)
+constexpr unsigned kRefLdsMax = 160u * 1024u;   // LDS of one gfx950 compute unit
 
 struct RefineArgs {
-    const float* R;          // column-major Rgb reduced matrix, R[(k * S + r) * 3 + c]
     int S;
     const int* nb;           // S x nnb neighbour rows
     int nnb;
@@ -50,9 +54,11 @@ struct RefineArgs {
     int nmem;
     const int* grep;
     const float* gw;
-    const float* mags;       // flattened column magnitudes (pdfs of the global clusters)
-    const float* norms_R;    // column norms (pdfs of split clusters)
+    const float* mags;       // per member position: flattened column magnitude (pdfs of the global clusters)
+    const float* norms_R;    // per member position: column norm (pdfs of split clusters)
+    const float* SM;         // per scratch row: the slice matrix, nmem x 3*nnb, member by member
     int maxC;
+    int lds_clusters;        // cluster table and cost heap in LDS (else in the scratch rows)
     const float* draws;      // pre-generated MT numbers
     const int64_t* draw_off; // per slice
     int s0;                  // first slice of the launch
@@ -63,101 +69,343 @@ struct RefineArgs {
     int* count;
     int* used;               // draws taken
     // per-slice state (scratch rows of this batch)
-    int* perm;               // nmem
-    int* perm_tmp;           // nmem
-    float* nrm;              // nmem
-    float* line_t;           // nmem (long lines)
+    int* perm;               // nmem member positions (index into mem), cluster by cluster
+    int* perm_tmp;           // nmem (clusters over kRefSortLds members)
+    float* nrm;              // nmem (idem)
+    float* line_t;           // nmem (idem)
     int* line_k;             // nmem
     int* cst;                // maxC
     int* csz;
     int* cflag;              // 0: pdf over mags (global cluster), 1: over norms_R
     float* hkey;             // maxC
     int* hval;
+    // computed once per slice row (they do not depend on the draws)
+    const int* iota;         // 0 .. nmem - 1: the member order before any split
+    float* cost0;            // ncl costs of the kept clusters
+    const float* hkey0;      // the cost heap after pushing them in order (ncl entries)
+    const int* hval0;
 };
+
+// LDS layout of one refinement block, shared by the kernel and its launcher
+struct RefLayout {
+    int ldx, kc;
+    unsigned oX, oU, oA, oT, oPm, oMap, oNb, oInt, oKey, oVal, oSeg, oLa, oLb, oWs, oCl, oPosL, oPosR, oRid, oFl, bytes;
+};
+
+RZK_HD unsigned ref_take(unsigned& o, unsigned bytes) {
+    const unsigned r = o;
+    o += (bytes + 15u) & ~15u;
+    return r;
+}
+
+RZK_HD RefLayout ref_layout(int nnb, int maxC, bool lds_clusters) {
+    RefLayout L;
+    L.ldx = 3 * nnb + 1;
+    L.kc = 4 * 256 * L.ldx <= 64 * 1024 ? 256 : 128;     // columns staged per pass
+    unsigned o = 0;
+    L.oX = ref_take(o, 4u * L.kc * L.ldx);
+    L.oU = ref_take(o, 4u * 3 * kRefMaxNb);
+    L.oA = ref_take(o, 4u * 3 * kRefMaxNb);
+    L.oT = ref_take(o, 4u * kRefBlock);
+    L.oPm = ref_take(o, 4u * kRefBlock);
+    L.oMap = ref_take(o, 4u * kRefBlock);
+    L.oNb = ref_take(o, 4u * kRefMaxNb);
+    L.oInt = ref_take(o, 4u * 16);
+    L.oKey = ref_take(o, 4u * kRefSortLds);
+    L.oVal = ref_take(o, 4u * kRefSortLds);
+    L.oSeg = ref_take(o, 4u * 18 * kRefBlock);
+    L.oLa = ref_take(o, 4u * kRefBlock);
+    L.oLb = ref_take(o, 4u * kRefBlock);
+    L.oWs = ref_take(o, 4u * (2 * kRefBlock / 64 + 1));
+    L.oCl = lds_clusters ? ref_take(o, 20u * (unsigned)maxC) : 0u;
+    L.oPosL = ref_take(o, 2u * kRefSortLds);
+    L.oPosR = ref_take(o, 2u * kRefSortLds);
+    L.oRid = ref_take(o, 2u * kRefSortLds);
+    L.oFl = ref_take(o, kRefSortLds);
+    L.bytes = o;
+    return L;
+}
 
 struct RefineShared {
-    float* sX;      // kRefChunk x ldx gathered columns
-    int ldx;
+    float* sX;      // kc x ldx gathered columns
+    int ldx, kc;
     float* sU;      // flattened rows: row sums / the line direction
     float* sA;      // column a
-    float* sT;      // kRefBlock chain staging
-    float* sKey;    // kRefSortLds
-    int* sVal;
+    float* sT;      // kRefBlock per-column values of a pass
+    int* sPm;       // kRefBlock member positions of the staged columns
+    int* sMap;      // kRefBlock: after a split staged whole, the staged column of each new position
+    const float* sm;   // this slice's matrix copy (nmem x 3*nnb)
+    int* sNb;       // the slice's neighbour rows
     int* sInt;      // small broadcast area
+    float* sKey;    // kRefSortLds: projections / pdf values / column norms / new member order
+    int* sVal;
+    psort::Work<float, int> pw;   // the sort's buffers (sKey, sVal and the rest)
 };
 
-// gather columns perm[st + c0 .. st + c0 + cnt) of the slice matrix into sX
-__device__ __forceinline__ void ref_stage(const RefineArgs& a, const int* nbs, const int* perm, int st, int c0,
-                                          int cnt, const RefineShared& sh) {
-    const int M = 3 * a.nnb;
-    for (int e = threadIdx.x; e < cnt * M; e += kRefBlock) {
-        int col = e / M, r = e - col * M, q = r / 3, ch = r - 3 * q;
-        int k = perm[st + c0 + col];
-        sh.sX[col * sh.ldx + r] = a.R[((int64_t)k * a.S + nbs[q]) * 3 + ch];
+// Ordered fp32 chains over LDS values.  The loads of 16 elements are issued
+// before their adds, so a chain waits for LDS once per 16 elements instead of
+// once per element; the adds keep their order.
+__device__ __forceinline__ float chain_sum(const float* v, int n, float acc) {
+    int k = 0;
+    for (; k + 16 <= n; k += 16) {
+        float x[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) x[q] = v[k + q];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc += x[q];
+    }
+    for (; k < n; ++k) acc += v[k];
+    return acc;
+}
+
+// sum_k v[k * stride] (a column of a row-major LDS tile)
+__device__ __forceinline__ float chain_sum_strided(const float* v, int n, int stride, float acc) {
+    int k = 0;
+    for (; k + 16 <= n; k += 16) {
+        float x[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) x[q] = v[(k + q) * stride];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc += x[q];
+    }
+    for (; k < n; ++k) acc += v[k * stride];
+    return acc;
+}
+
+// sum_k v[map[k] * stride] (a row of the staged columns, in a new column order)
+__device__ __forceinline__ float chain_sum_mapped(const float* v, const int* map, int n, int stride, float acc) {
+    int k = 0;
+    for (; k + 16 <= n; k += 16) {
+        int m[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) m[q] = map[k + q];
+        float x[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) x[q] = v[m[q] * stride];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc += x[q];
+    }
+    for (; k < n; ++k) acc += v[map[k] * stride];
+    return acc;
+}
+
+// sum_r x[r] * x[r]
+__device__ __forceinline__ float chain_sumsq(const float* x, int n) {
+    float acc = 0.f;
+    int k = 0;
+    for (; k + 16 <= n; k += 16) {
+        float y[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) y[q] = x[k + q];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc += y[q] * y[q];
+    }
+    for (; k < n; ++k) acc += x[k] * x[k];
+    return acc;
+}
+
+// sum_r x[r] * y[r]
+__device__ __forceinline__ float chain_dot(const float* x, const float* y, int n) {
+    float acc = 0.f;
+    int k = 0;
+    for (; k + 16 <= n; k += 16) {
+        float p[16], q2[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) { p[q] = x[k + q]; q2[q] = y[k + q]; }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc += p[q] * q2[q];
+    }
+    for (; k < n; ++k) acc += x[k] * y[k];
+    return acc;
+}
+
+// sum_r (a[r] - x[r]) * d[r]
+__device__ __forceinline__ float chain_proj(const float* a, const float* x, const float* d, int n) {
+    float acc = 0.f;
+    int k = 0;
+    for (; k + 16 <= n; k += 16) {
+        float p[16], q2[16], r2[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) { p[q] = a[k + q]; q2[q] = x[k + q]; r2[q] = d[k + q]; }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc += (p[q] - q2[q]) * r2[q];
+    }
+    for (; k < n; ++k) acc += (a[k] - x[k]) * d[k];
+    return acc;
+}
+
+// columns perm[st + c0 .. st + c0 + cnt) of the slice matrix into sX (cnt <= kc),
+// copied from the slice's contiguous copy: consecutive lanes read consecutive
+// values of a column, eight loads in flight per lane
+__device__ __forceinline__ void ref_stage(const RefineArgs& a, const RefineShared& sh, const int* perm, int st, int c0,
+                                          int cnt) {
+    const int M = 3 * a.nnb, t = threadIdx.x;
+    if (t < cnt) sh.sPm[t] = perm[st + c0 + t];
+    __syncthreads();
+    const int total = cnt * M, dcol = kRefBlock / M, dr = kRefBlock - dcol * M;
+    int col = t / M, r = t - col * M;
+    for (int e0 = 0; e0 < total; e0 += 8 * kRefBlock) {
+        float v[8];
+        int dst[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const bool ok = e0 + q * kRefBlock + t < total;
+            dst[q] = ok ? col * sh.ldx + r : -1;
+            v[q] = ok ? sh.sm[(int64_t)sh.sPm[col] * M + r] : 0.f;
+            col += dcol;
+            r += dr;
+            if (r >= M) { r -= M; ++col; }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (dst[q] >= 0) sh.sX[dst[q]] = v[q];
     }
 }
 
 // compute_distance (lightslice.cc:336-357) of columns perm[st .. st + n)
-__device__ float ref_cost(const RefineArgs& a, const int* nbs, const int* perm, float* nrm, int st, int n,
-                          const RefineShared& sh) {
-    const int M = 3 * a.nnb, tid = threadIdx.x;
+__device__ float ref_cost(const RefineArgs& a, const RefineShared& sh, const int* perm, float* nrm_g, int st, int n) {
+    const int M = 3 * a.nnb, t = threadIdx.x, kc = sh.kc;
+    float* nrm = n <= kRefSortLds ? sh.sKey : nrm_g + st;     // column norms between the passes
     float u = 0.f, c = 0.f;
     // pass 1: column norms (get_column_norm), row sums (row_sum), their total (sum)
-    for (int c0 = 0; c0 < n; c0 += kRefChunk) {
-        const int cnt = min(kRefChunk, n - c0);
-        ref_stage(a, nbs, perm, st, c0, cnt, sh);
+    for (int c0 = 0; c0 < n; c0 += kc) {
+        const int cnt = min(kc, n - c0);
+        ref_stage(a, sh, perm, st, c0, cnt);
         __syncthreads();
-        if (tid < cnt) {
-            float mag = 0.f;
-            for (int r = 0; r < M; ++r) {
-                float x = sh.sX[tid * sh.ldx + r];
-                mag += x * x;
-            }
-            float nv = sqrtf(mag);
-            nrm[c0 + tid] = nv;
-            sh.sT[tid] = nv;
+        if (t < cnt) {
+            const float nv = sqrtf(chain_sumsq(sh.sX + t * sh.ldx, M));
+            nrm[c0 + t] = nv;
+            sh.sT[t] = nv;
         }
-        if (tid < M)
-            for (int col = 0; col < cnt; ++col) u += sh.sX[col * sh.ldx + tid];
+        if (t < M) u = chain_sum_strided(sh.sX + t, cnt, sh.ldx, u);
         __syncthreads();
-        for (int k = 0; k < cnt; ++k) c += sh.sT[k];
+        c = chain_sum(sh.sT, cnt, c);
         __syncthreads();
     }
-    if (tid < M) sh.sU[tid] = u;
+    if (t < M) sh.sU[t] = u;
     __syncthreads();
     // pass 2: v_j = sum_r x_rj u_r (Rt.mul), cost = sum_j (norm_j * c - v_j)
     float cost = 0.f;
-    for (int c0 = 0; c0 < n; c0 += kRefChunk) {
-        const int cnt = min(kRefChunk, n - c0);
-        ref_stage(a, nbs, perm, st, c0, cnt, sh);
+    for (int c0 = 0; c0 < n; c0 += kc) {
+        const int cnt = min(kc, n - c0);
+        ref_stage(a, sh, perm, st, c0, cnt);
         __syncthreads();
-        if (tid < cnt) {
-            float v = 0.f;
-            for (int r = 0; r < M; ++r) v += sh.sX[tid * sh.ldx + r] * sh.sU[r];
-            sh.sT[tid] = nrm[c0 + tid] * c - v;
-        }
+        if (t < cnt) sh.sT[t] = nrm[c0 + t] * c - chain_dot(sh.sX + t * sh.ldx, sh.sU, M);
         __syncthreads();
-        for (int k = 0; k < cnt; ++k) cost += sh.sT[k];
+        cost = chain_sum(sh.sT, cnt, cost);
         __syncthreads();
     }
+    return cost;
+}
+
+// compute_distance of the staged columns sX[sMap[off + j]], j < n <= kc: the
+// same sums in the same order as ref_cost (one chunk), without staging again
+__device__ float ref_cost_local(const RefineArgs& a, const RefineShared& sh, int off, int n) {
+    const int M = 3 * a.nnb, t = threadIdx.x;
+    if (t < n) sh.sT[t] = sqrtf(chain_sumsq(sh.sX + sh.sMap[off + t] * sh.ldx, M));
+    float u = 0.f;
+    if (t < M) u = chain_sum_mapped(sh.sX + t, sh.sMap + off, n, sh.ldx, 0.f);
+    __syncthreads();
+    const float c = chain_sum(sh.sT, n, 0.f);
+    if (t < M) sh.sU[t] = u;
+    __syncthreads();
+    if (t < n) sh.sT[t] = sh.sT[t] * c - chain_dot(sh.sX + sh.sMap[off + t] * sh.ldx, sh.sU, M);
+    __syncthreads();
+    const float cost = chain_sum(sh.sT, n, 0.f);
+    __syncthreads();
     return cost;
 }
 
 // DiscretePdf over src[perm[st + k]] (k < n) sampled at nu (1 or 2) numbers:
 // out_k[i] = first k with u[i] < cdf[k] (-1 if none), out_p[i] = pdf[k]
 // (pdf.cc:18-77: ordered total, pdf = a / total, cdf running sum, last = 1).
-__device__ void ref_pdf_sample(const float* src, const int* perm, int st, int n, int nu, const float* u,
-                               int* out_k, float* out_p, const RefineShared& sh) {
-    const int tid = threadIdx.x;
+// Every lane runs the same ordered chains over the values staged in LDS.
+__device__ void ref_pdf_sample(const RefineShared& sh, const float* src, const int* perm, int st, int n, int nu,
+                               const float* u, int* out_k, float* out_p) {
+    const int t = threadIdx.x;
+    if (n <= kRefSortLds) {
+        float* v = sh.sKey;
+        for (int k0 = t; k0 < n; k0 += 4 * kRefBlock) {
+            int pk[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) pk[q] = k0 + q * kRefBlock < n ? perm[st + k0 + q * kRefBlock] : 0;
+            float x[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[q] = k0 + q * kRefBlock < n ? src[pk[q]] : 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (k0 + q * kRefBlock < n) v[k0 + q * kRefBlock] = x[q];
+        }
+        __syncthreads();
+        const float sum = chain_sum(v, n, 0.f);
+        __syncthreads();
+        if (sum == 0.0f) {
+            for (int i = 0; i < nu; ++i) { out_k[i] = -1; out_p[i] = 0.f; }
+            return;
+        }
+        for (int k = t; k < n; k += kRefBlock) v[k] = v[k] / sum;
+        __syncthreads();
+        // the cdf, as one ordered chain (lane 0), then every lane looks for the first k with u < cdf[k]
+        float* cdf = reinterpret_cast<float*>(sh.sVal);
+        if (t == 0) {
+            float run = 0.f;
+            int k = 0;
+            for (; k + 16 <= n; k += 16) {
+                float x[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) x[q] = v[k + q];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    run = k + q == 0 ? x[q] : run + x[q];
+                    cdf[k + q] = run;
+                }
+            }
+            for (; k < n; ++k) {
+                run = k == 0 ? v[0] : run + v[k];
+                cdf[k] = run;
+            }
+            cdf[n - 1] = 1.0f;
+        }
+        __syncthreads();
+        int k0 = INT_MAX, k1 = INT_MAX;
+        for (int k = t; k < n; k += kRefBlock) {
+            const float ck = cdf[k];
+            if (k0 == INT_MAX && u[0] < ck) k0 = k;
+            if (nu > 1 && k1 == INT_MAX && u[1] < ck) k1 = k;
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            k0 = min(k0, __shfl_xor(k0, o, 64));
+            k1 = min(k1, __shfl_xor(k1, o, 64));
+        }
+        if ((t & 63) == 0) { sh.sInt[8 + 2 * (t >> 6)] = k0; sh.sInt[9 + 2 * (t >> 6)] = k1; }
+        __syncthreads();
+        k0 = INT_MAX;
+        k1 = INT_MAX;
+        for (int w2 = 0; w2 < kRefBlock / 64; ++w2) {
+            k0 = min(k0, sh.sInt[8 + 2 * w2]);
+            k1 = min(k1, sh.sInt[9 + 2 * w2]);
+        }
+        if (k0 == INT_MAX) k0 = -1;
+        if (k1 == INT_MAX) k1 = -1;
+        out_k[0] = k0;
+        out_p[0] = k0 >= 0 ? v[k0] : 0.f;
+        if (nu > 1) {
+            out_k[1] = k1;
+            out_p[1] = k1 >= 0 ? v[k1] : 0.f;
+        }
+        __syncthreads();
+        return;
+    }
+    // long clusters: 256 values at a time through sT, first hits by LDS atomicMin
     float sum = 0.f;
     for (int c0 = 0; c0 < n; c0 += kRefBlock) {
         const int cnt = min(kRefBlock, n - c0);
-        sh.sT[tid] = tid < cnt ? src[perm[st + c0 + tid]] : 0.f;
+        sh.sT[t] = t < cnt ? src[perm[st + c0 + t]] : 0.f;
         __syncthreads();
         for (int k = 0; k < cnt; ++k) sum += sh.sT[k];
         __syncthreads();
     }
-    if (tid == 0) { sh.sInt[0] = INT_MAX; sh.sInt[1] = INT_MAX; }
+    if (t == 0) { sh.sInt[0] = INT_MAX; sh.sInt[1] = INT_MAX; }
     __syncthreads();
     if (sum == 0.0f) {
         for (int i = 0; i < nu; ++i) { out_k[i] = -1; out_p[i] = 0.f; }
@@ -166,151 +414,281 @@ __device__ void ref_pdf_sample(const float* src, const int* perm, int st, int n,
     float run = 0.f;
     for (int c0 = 0; c0 < n; c0 += kRefBlock) {
         const int cnt = min(kRefBlock, n - c0);
-        const float p = tid < cnt ? src[perm[st + c0 + tid]] / sum : 0.f;
-        sh.sT[tid] = p;
+        const float p = t < cnt ? src[perm[st + c0 + t]] / sum : 0.f;
+        sh.sT[t] = p;
         __syncthreads();
         float mine = 0.f;
         for (int k = 0; k < cnt; ++k) {
             run = (c0 + k == 0) ? sh.sT[0] : run + sh.sT[k];
-            if (k == tid) mine = run;
+            if (k == t) mine = run;
         }
-        if (c0 + tid == n - 1) mine = 1.0f;
-        if (tid < cnt)
+        if (c0 + t == n - 1) mine = 1.0f;
+        if (t < cnt)
             for (int i = 0; i < nu; ++i)
-                if (u[i] < mine) atomicMin(&sh.sInt[i], c0 + tid);
+                if (u[i] < mine) atomicMin(&sh.sInt[i], c0 + t);
         __syncthreads();
         if (sh.sInt[0] != INT_MAX && (nu < 2 || sh.sInt[1] != INT_MAX)) break;    // uniform
         __syncthreads();
     }
     __syncthreads();
     for (int i = 0; i < nu; ++i) {
-        int k = sh.sInt[i];
+        const int k = sh.sInt[i];
         if (k == INT_MAX) { out_k[i] = -1; out_p[i] = 0.f; }
         else { out_k[i] = k; out_p[i] = src[perm[st + k]] / sum; }
     }
     __syncthreads();
 }
 
+// the first k in [1, nl) with the largest gap key[k] - key[k-1] > 0, or -1
+// (the sequential scan at lightslice.cc:466-473: strict >, starting from 0)
+__device__ int ref_max_gap(const RefineShared& sh, const float* key, int nl) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    float best = 0.f;
+    int bk = -1;
+    for (int k = 1 + t; k < nl; k += kRefBlock) {
+        const float d = key[k] - key[k - 1];
+        if (d > best) { best = d; bk = k; }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float ob = __shfl_xor(best, o, 64);
+        const int ok = __shfl_xor(bk, o, 64);
+        if (ok >= 0 && (bk < 0 || ob > best || (ob == best && ok < bk))) { best = ob; bk = ok; }
+    }
+    if (lane == 0) {
+        sh.sInt[8 + 2 * wv] = __float_as_int(best);
+        sh.sInt[9 + 2 * wv] = bk;
+    }
+    __syncthreads();
+    best = 0.f;
+    bk = -1;
+    for (int w2 = 0; w2 < kRefBlock / 64; ++w2) {
+        const float ob = __int_as_float(sh.sInt[8 + 2 * w2]);
+        const int ok = sh.sInt[9 + 2 * w2];
+        if (ok >= 0 && (bk < 0 || ob > best || (ob == best && ok < bk))) { best = ob; bk = ok; }
+    }
+    __syncthreads();
+    return bk;
+}
+
 // split_cluster (lightslice.cc:405-488) of perm[st .. st + n), n >= 2: the
 // range is rewritten as indices1 then indices2; returns |indices1|.
-__device__ int ref_split(const RefineArgs& a, const int* nbs, int* perm, int* perm_tmp, float* line_t,
-                         int* line_k, int st, int n, int flag, int64_t draw0, int& drawn, const RefineShared& sh) {
-    const int M = 3 * a.nnb, tid = threadIdx.x;
+// local: on return, sX holds the cluster's columns in their old order and
+// sMap[p] gives the staged column of every new position p (clusters of at
+// most kc members that reached the projection)
+__device__ int ref_split(const RefineArgs& a, const RefineShared& sh, int* perm, int* perm_tmp, float* line_t,
+                         int* line_k, int st, int n, int flag, int64_t draw0, int& drawn, bool& local) {
+    const int M = 3 * a.nnb, t = threadIdx.x, kc = sh.kc;
+    local = false;
     if (n == 2) return 1;
-    float u[2] = {a.draws[draw0 + drawn], a.draws[draw0 + drawn + 1]};
+    const float u[2] = {a.draws[draw0 + drawn], a.draws[draw0 + drawn + 1]};
     drawn += 2;
     int ij[2];
     float pij[2];
-    ref_pdf_sample(flag ? a.norms_R : a.mags, perm, st, n, 2, u, ij, pij, sh);
+    ref_pdf_sample(sh, flag ? a.norms_R : a.mags, perm, st, n, 2, u, ij, pij);
     const int i = ij[0], j = ij[1];
     if (i < 0 || j < 0) return n / 2;                                    // split by half, order kept
     // a, a - b
-    for (int r = tid; r < M; r += kRefBlock) {
-        int q = r / 3, ch = r - 3 * q;
-        float av = a.R[((int64_t)perm[st + i] * a.S + nbs[q]) * 3 + ch];
-        float bv = a.R[((int64_t)perm[st + j] * a.S + nbs[q]) * 3 + ch];
-        sh.sA[r] = av;
-        sh.sU[r] = av - bv;
+    if (t < M) {
+        const float av = sh.sm[(int64_t)perm[st + i] * M + t];
+        const float bv = sh.sm[(int64_t)perm[st + j] * M + t];
+        sh.sA[t] = av;
+        sh.sU[t] = av - bv;
     }
     __syncthreads();
     // t_k = (a - c_k) . (a - b) for k != i, j, at line position k - [k > i] - [k > j]
     const int nl = n - 2;
-    float* key = nl <= kRefSortLds ? sh.sKey : line_t;
-    int* val = nl <= kRefSortLds ? sh.sVal : line_k;
-    for (int c0 = 0; c0 < n; c0 += kRefChunk) {
-        const int cnt = min(kRefChunk, n - c0);
-        ref_stage(a, nbs, perm, st, c0, cnt, sh);
+    const bool in_lds = n <= kRefSortLds;
+    float* key = in_lds ? sh.sKey : line_t;
+    int* val = in_lds ? sh.sVal : line_k;
+    for (int c0 = 0; c0 < n; c0 += kc) {
+        const int cnt = min(kc, n - c0);
+        ref_stage(a, sh, perm, st, c0, cnt);
         __syncthreads();
-        if (tid < cnt) {
-            const int k = c0 + tid;
+        if (t < cnt) {
+            const int k = c0 + t;
             if (k != i && k != j) {
-                float t = 0.0f;
-                for (int r = 0; r < M; ++r) t += (sh.sA[r] - sh.sX[tid * sh.ldx + r]) * sh.sU[r];
+                const float tk = chain_proj(sh.sA, sh.sX + t * sh.ldx, sh.sU, M);
                 const int p = k - (k > i ? 1 : 0) - (k > j ? 1 : 0);
-                key[p] = t;
+                key[p] = tk;
                 val[p] = k;
             }
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        stdord::sort_pairs(key, val, nl);
-        // largest gap, first on ties, only if positive
-        float max_dist = 0.0f;
-        int max_k = -1;
-        for (int k = 1; k < nl; ++k) {
-            float d = key[k] - key[k - 1];
-            if (d > max_dist) { max_dist = d; max_k = k; }
+    if (in_lds) {
+        psort::sort_pairs<psort::DevLanes<kRefBlock>>(sh.pw, nl);
+    } else {
+        if (t == 0) stdord::sort_pairs(line_t, line_k, nl);
+        __syncthreads();
+    }
+    const int cut = ref_max_gap(sh, key, nl);
+    local = n <= kc;
+    if (cut < 0) {
+        if (local) {
+            for (int p = t; p < n; p += kRefBlock) sh.sMap[p] = p;
+            __syncthreads();
         }
-        sh.sInt[2] = max_k;
+        return n / 2;
     }
-    __syncthreads();
-    const int cut = sh.sInt[2];
-    if (cut < 0) return n / 2;
     // indices1 = [i, line[0 .. cut)], indices2 = [j, line[cut ..)]
-    for (int p = tid; p < n; p += kRefBlock) {
-        int src;
-        if (p == 0) src = i;
-        else if (p <= cut) src = val[p - 1];
-        else if (p == cut + 1) src = j;
-        else src = val[p - 2];
-        perm_tmp[p] = perm[st + src];
+    int* tmp = in_lds ? reinterpret_cast<int*>(sh.sKey) : perm_tmp;
+    for (int p0 = t; p0 < n; p0 += 4 * kRefBlock) {
+        int x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int p = p0 + q * kRefBlock;
+            int src = 0;
+            if (p < n) {
+                if (p == 0) src = i;
+                else if (p <= cut) src = val[p - 1];
+                else if (p == cut + 1) src = j;
+                else src = val[p - 2];
+            }
+            x[q] = p < n ? perm[st + src] : 0;
+            if (local && p < n) sh.sMap[p] = src;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (p0 + q * kRefBlock < n) tmp[p0 + q * kRefBlock] = x[q];
     }
     __syncthreads();
-    for (int p = tid; p < n; p += kRefBlock) perm[st + p] = perm_tmp[p];
+    for (int p = t; p < n; p += kRefBlock) perm[st + p] = tmp[p];
     __syncthreads();
     return cut + 1;
 }
 
-__global__ __launch_bounds__(kRefBlock) void k_refine_slices(RefineArgs a) {
-    extern __shared__ float s_ref[];
-    const int M = 3 * a.nnb, tid = threadIdx.x;
-    RefineShared sh;
-    sh.ldx = M + 1;
-    sh.sX = s_ref;
-    sh.sU = sh.sX + kRefChunk * sh.ldx;
-    sh.sA = sh.sU + 3 * kRefMaxNb;
-    sh.sT = sh.sA + 3 * kRefMaxNb;
-    sh.sKey = sh.sT + kRefBlock;
-    sh.sVal = reinterpret_cast<int*>(sh.sKey + kRefSortLds);
-    sh.sInt = sh.sVal + kRefSortLds;
+// the slice matrix of slices s0 .. s0 + gridDim.y - 1, member by member:
+// SM[row][p][3 q + c] = R[(mem[p] * S + nb[s][q]) * 3 + c], row = s - s0
+__global__ __launch_bounds__(256) void k_slice_columns(const float* __restrict__ R, int S,
+                                                      const int* __restrict__ nb, int nnb,
+                                                      const int* __restrict__ mem, int nmem, int s0,
+                                                      float* __restrict__ SM) {
+    const int M = 3 * nnb, s = s0 + (int)blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nmem * M) return;
+    const int p = (int)(e / M), r = (int)(e - (int64_t)p * M), q = r / 3, ch = r - 3 * q;
+    SM[(int64_t)blockIdx.y * nmem * M + e] = R[((int64_t)mem[p] * S + nb[(int64_t)s * nnb + q]) * 3 + ch];
+}
 
+// src[mem[p]] for every member position p
+__global__ void k_gather_members(const float* __restrict__ src, const int* __restrict__ mem, int nmem,
+                                 float* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nmem) out[p] = src[mem[p]];
+}
+
+// the LDS views of one block (the layout is the same for every kernel here;
+// k_initial_costs allocates only up to oVal)
+__device__ __forceinline__ RefineShared ref_shared(unsigned char* s_ref, const RefLayout& L, const RefineArgs& a,
+                                                   int row) {
+    RefineShared sh;
+    sh.ldx = L.ldx;
+    sh.kc = L.kc;
+    sh.sX = reinterpret_cast<float*>(s_ref + L.oX);
+    sh.sU = reinterpret_cast<float*>(s_ref + L.oU);
+    sh.sA = reinterpret_cast<float*>(s_ref + L.oA);
+    sh.sT = reinterpret_cast<float*>(s_ref + L.oT);
+    sh.sPm = reinterpret_cast<int*>(s_ref + L.oPm);
+    sh.sMap = reinterpret_cast<int*>(s_ref + L.oMap);
+    sh.sm = a.SM + (int64_t)row * a.nmem * (3 * a.nnb);
+    sh.sNb = reinterpret_cast<int*>(s_ref + L.oNb);
+    sh.sInt = reinterpret_cast<int*>(s_ref + L.oInt);
+    sh.sKey = reinterpret_cast<float*>(s_ref + L.oKey);
+    sh.sVal = reinterpret_cast<int*>(s_ref + L.oVal);
+    return sh;
+}
+
+// compute_distance of every kept cluster in every slice of the launch: block
+// (c, y) costs cluster c of slice s0 + y (scratch row s0 + y - batch0)
+__global__ __launch_bounds__(kRefBlock) void k_initial_costs(RefineArgs a) {
+    extern __shared__ __align__(16) unsigned char s_ref[];
+    const RefLayout L = ref_layout(a.nnb, a.maxC, false);
+    const int c = (int)blockIdx.x, row = a.s0 + (int)blockIdx.y - a.batch0;
+    const RefineShared sh = ref_shared(s_ref, L, a, row);
+    const int st = a.off[c], n = a.off[c + 1] - a.off[c];
+    const float cost = ref_cost(a, sh, a.iota, a.nrm + (int64_t)row * a.nmem, st, n);
+    if (threadIdx.x == 0) a.cost0[(int64_t)row * a.ncl + c] = cost;
+}
+
+// the cost heap of each slice after pushing its kept clusters' costs in
+// order (lightslice.cc:862-866), built by one lane in LDS
+__global__ __launch_bounds__(64) void k_initial_heap(const float* __restrict__ cost0, int ncl, float* __restrict__ hkey0,
+                                                     int* __restrict__ hval0) {
+    extern __shared__ __align__(16) unsigned char s_heap[];
+    float* hk = reinterpret_cast<float*>(s_heap);
+    int* hv = reinterpret_cast<int*>(hk + ncl);
+    const int64_t row = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int hn = 0;
+        for (int c = 0; c < ncl; ++c) stdord::heap_push(hk, hv, hn, cost0[row * ncl + c], c);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncl; c += 64) {
+        hkey0[row * ncl + c] = hk[c];
+        hval0[row * ncl + c] = hv[c];
+    }
+}
+
+__global__ __launch_bounds__(kRefBlock) void k_refine_slices(RefineArgs a) {
+    extern __shared__ __align__(16) unsigned char s_ref[];
+    const RefLayout L = ref_layout(a.nnb, a.maxC, a.lds_clusters != 0);
+    const int t = threadIdx.x;
     const int s = a.s0 + (int)blockIdx.x;
-    if (s >= a.S) return;
     const int row = s - a.batch0;
-    const int* nbs = a.nb + (int64_t)s * a.nnb;
+    RefineShared sh = ref_shared(s_ref, L, a, row);
+    sh.pw.key = sh.sKey;
+    sh.pw.val = sh.sVal;
+    sh.pw.posL = reinterpret_cast<unsigned short*>(s_ref + L.oPosL);
+    sh.pw.posR = reinterpret_cast<unsigned short*>(s_ref + L.oPosR);
+    sh.pw.rid = reinterpret_cast<unsigned short*>(s_ref + L.oRid);
+    sh.pw.fl = s_ref + L.oFl;
+    sh.pw.seg = reinterpret_cast<int*>(s_ref + L.oSeg);
+    sh.pw.lane_a = reinterpret_cast<int*>(s_ref + L.oLa);
+    sh.pw.lane_b = reinterpret_cast<int*>(s_ref + L.oLb);
+    sh.pw.wsum = reinterpret_cast<int*>(s_ref + L.oWs);
+
+    if (s >= a.S) return;
     int* perm = a.perm + (int64_t)row * a.nmem;
     int* perm_tmp = a.perm_tmp + (int64_t)row * a.nmem;
     float* nrm = a.nrm + (int64_t)row * a.nmem;
     float* line_t = a.line_t + (int64_t)row * a.nmem;
     int* line_k = a.line_k + (int64_t)row * a.nmem;
-    int* cst = a.cst + (int64_t)row * a.maxC;
-    int* csz = a.csz + (int64_t)row * a.maxC;
-    int* cflag = a.cflag + (int64_t)row * a.maxC;
-    float* hkey = a.hkey + (int64_t)row * a.maxC;
-    int* hval = a.hval + (int64_t)row * a.maxC;
+    int *cst, *csz, *cflag, *hval;
+    float* hkey;
+    if (a.lds_clusters) {
+        cst = reinterpret_cast<int*>(s_ref + L.oCl);
+        csz = cst + a.maxC;
+        cflag = csz + a.maxC;
+        hkey = reinterpret_cast<float*>(cflag + a.maxC);
+        hval = reinterpret_cast<int*>(hkey + a.maxC);
+    } else {
+        cst = a.cst + (int64_t)row * a.maxC;
+        csz = a.csz + (int64_t)row * a.maxC;
+        cflag = a.cflag + (int64_t)row * a.maxC;
+        hkey = a.hkey + (int64_t)row * a.maxC;
+        hval = a.hval + (int64_t)row * a.maxC;
+    }
     int* rep = a.rep + (int64_t)s * a.maxC;
     float* w = a.w + (int64_t)s * a.maxC;
     const int64_t draw0 = a.draw_off[s];
 
-    for (int e = tid; e < a.nmem; e += kRefBlock) perm[e] = a.mem[e];
-    for (int c = tid; c < a.ncl; c += kRefBlock) {
+    if (t < a.nnb) sh.sNb[t] = a.nb[(int64_t)s * a.nnb + t];
+    for (int e = t; e < a.nmem; e += kRefBlock) perm[e] = e;
+    for (int c = t; c < a.ncl; c += kRefBlock) {
         cst[c] = a.off[c];
         csz[c] = a.off[c + 1] - a.off[c];
         cflag[c] = 0;
         rep[c] = a.grep[c];
         w[c] = a.gw[c];
+        hkey[c] = a.hkey0[(int64_t)row * a.ncl + c];      // the kept clusters' costs, pushed in order
+        hval[c] = a.hval0[(int64_t)row * a.ncl + c];
     }
     __syncthreads();
-    int hn = 0;                    // heap size (lane 0's copy is the one used)
-    for (int c = 0; c < a.ncl; ++c) {
-        float cost = ref_cost(a, nbs, perm, nrm, cst[c], csz[c], sh);
-        if (tid == 0) stdord::heap_push(hkey, hval, hn, cost, c);
-    }
+    int hn = a.ncl;                // heap size (lane 0's copy is the one used)
     int cnt = a.ncl, drawn = 0;
     for (;;) {
         __syncthreads();
-        if (tid == 0) {
+        if (t == 0) {
             int stop = (hn == 0 || cnt >= a.maxC) ? 1 : 0;
             if (!stop && csz[hval[0]] <= 1) stop = 1;
             sh.sInt[3] = stop;
@@ -321,25 +699,26 @@ __global__ __launch_bounds__(kRefBlock) void k_refine_slices(RefineArgs a) {
         const int top = sh.sInt[4];
         const int st = cst[top], n = csz[top], flag = cflag[top];
         __syncthreads();
-        const int n1 = ref_split(a, nbs, perm, perm_tmp, line_t, line_k, st, n, flag, draw0, drawn, sh);
-        const float cost1 = ref_cost(a, nbs, perm, nrm, st, n1, sh);
-        const float cost2 = ref_cost(a, nbs, perm, nrm, st + n1, n - n1, sh);
+        bool local;
+        const int n1 = ref_split(a, sh, perm, perm_tmp, line_t, line_k, st, n, flag, draw0, drawn, local);
+        const float cost1 = local ? ref_cost_local(a, sh, 0, n1) : ref_cost(a, sh, perm, nrm, st, n1);
+        const float cost2 = local ? ref_cost_local(a, sh, n1, n - n1) : ref_cost(a, sh, perm, nrm, st + n1, n - n1);
         int k1, k2;
         float p1, p2;
-        float u1 = a.draws[draw0 + drawn], u2 = a.draws[draw0 + drawn + 1];
+        const float u1 = a.draws[draw0 + drawn], u2 = a.draws[draw0 + drawn + 1];
         drawn += 2;
-        ref_pdf_sample(a.norms_R, perm, st, n1, 1, &u1, &k1, &p1, sh);
-        ref_pdf_sample(a.norms_R, perm, st + n1, n - n1, 1, &u2, &k2, &p2, sh);
-        if (tid == 0) {
+        ref_pdf_sample(sh, a.norms_R, perm, st, n1, 1, &u1, &k1, &p1);
+        ref_pdf_sample(sh, a.norms_R, perm, st + n1, n - n1, 1, &u2, &k2, &p2);
+        if (t == 0) {
             const int n2 = n - n1;
             csz[top] = n1;
             cflag[top] = 1;
-            rep[top] = k1 < 0 ? perm[st + n1 / 2] : perm[st + k1];
+            rep[top] = a.mem[k1 < 0 ? perm[st + n1 / 2] : perm[st + k1]];
             w[top] = k1 < 0 ? 0.0f : 1.0f / p1;
             cst[cnt] = st + n1;
             csz[cnt] = n2;
             cflag[cnt] = 1;
-            rep[cnt] = k2 < 0 ? perm[st + n1 + n2 / 2] : perm[st + n1 + k2];
+            rep[cnt] = a.mem[k2 < 0 ? perm[st + n1 + n2 / 2] : perm[st + n1 + k2]];
             w[cnt] = k2 < 0 ? 0.0f : 1.0f / p2;
             stdord::heap_pop(hkey, hval, hn);
             stdord::heap_push(hkey, hval, hn, cost1, top);
@@ -347,7 +726,7 @@ __global__ __launch_bounds__(kRefBlock) void k_refine_slices(RefineArgs a) {
         }
         ++cnt;
     }
-    if (tid == 0) {
+    if (t == 0) {
         a.count[s] = cnt;
         a.used[s] = drawn;
     }
@@ -356,12 +735,6 @@ __global__ __launch_bounds__(kRefBlock) void k_refine_slices(RefineArgs a) {
 }  // namespace rzk
 
 using namespace rzk;
-
-size_t refine_lds_bytes(int nnb) {
-    const int M = 3 * nnb;
-    return sizeof(float) * ((size_t)kRefChunk * (M + 1) + 2 * 3 * kRefMaxNb + kRefBlock + kRefSortLds) +
-           sizeof(int) * ((size_t)kRefSortLds + 8);
-}
 
 int64_t refine_draws_needed(int S, int ncl, int maxC) { return 4 * (int64_t)(maxC - ncl) * S; }
 
@@ -379,25 +752,40 @@ void launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb
     RZK_CHECK((int64_t)h_draws_in.size() >= refine_draws_needed(S, ncl, maxC), RZK_E_INVALID,
               "too few random numbers for the refinement");
     hipStream_t st = ctx->stream;
+    const int M = 3 * nnb;
     const int64_t per = 4 * (int64_t)(maxC - ncl);         // draws per slice under the rule
     const std::vector<float> h_draws = h_draws_in.empty() ? std::vector<float>(1, 0.f) : h_draws_in;
     std::vector<int64_t> h_off(S);
     for (int s = 0; s < S; ++s) h_off[s] = per * s;
-    // slice batches bound the scratch to about 1 GiB
-    const size_t row_bytes = (size_t)nmem * 20 + (size_t)maxC * 20;
-    const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)S, ((size_t)1 << 30) / row_bytes));
-    // 13 regions below, each rounded up to 256 bytes
-    char* scratch = (char*)ctx_scratch(ctx, row_bytes * batch + sizeof(float) * h_draws.size() +
-                                               sizeof(int64_t) * S + sizeof(int) * S + 16 * 256);
-    char* p = scratch;
+    // per slice in flight: its slice-matrix copy and state rows; as many slices
+    // as half the free memory (at most 16 GiB) holds
+    const size_t row_bytes = (size_t)nmem * (20 + 4 * (size_t)M) + (size_t)maxC * 20 + (size_t)ncl * 12 + 20 * 256;
+    size_t free_b = 0, total_b = 0;
+    RZK_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t budget = std::min(free_b / 2, (size_t)16 << 30);
+    const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)S, budget / row_bytes));
+    const bool resident = batch >= S;                      // every slice's rows at once: copies built once
+    // 20 regions below, each rounded up to 256 bytes
+    const size_t shared_bytes = sizeof(float) * h_draws.size() + sizeof(int64_t) * S + sizeof(int) * S +
+                                sizeof(float) * 3 * (size_t)nmem + 8 * 256;
+    void* raw = nullptr;
+    RZK_HIP(hipMalloc(&raw, shared_bytes + row_bytes * batch));
+    std::unique_ptr<void, void (*)(void*)> hold(raw, [](void* q) { (void)hipFree(q); });
+    char* p = static_cast<char*>(raw);
     auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) & ~(size_t)255; return (void*)q; };
     RefineArgs a{};
-    a.R = d_R; a.S = S; a.nb = d_nb; a.nnb = nnb; a.ncl = ncl; a.off = d_off; a.mem = d_mem; a.nmem = nmem;
-    a.grep = d_grep; a.gw = d_gw; a.mags = d_mags; a.norms_R = d_norms_R; a.maxC = maxC;
+    a.S = S; a.nb = d_nb; a.nnb = nnb; a.ncl = ncl; a.off = d_off; a.mem = d_mem; a.nmem = nmem;
+    a.grep = d_grep; a.gw = d_gw; a.maxC = maxC;
     float* d_draws = (float*)take(sizeof(float) * h_draws.size());
     int64_t* d_doff = (int64_t*)take(sizeof(int64_t) * S);
     int* d_used = (int*)take(sizeof(int) * S);
+    float* d_mm = (float*)take(sizeof(float) * (size_t)nmem);
+    float* d_nn = (float*)take(sizeof(float) * (size_t)nmem);
+    int* d_iota = (int*)take(sizeof(int) * (size_t)nmem);
     a.draws = d_draws; a.draw_off = d_doff; a.rep = d_rep; a.w = d_w; a.count = d_count; a.used = d_used;
+    a.mags = d_mm; a.norms_R = d_nn;
+    float* d_SM = (float*)take(sizeof(float) * (size_t)nmem * M * batch);
+    a.SM = d_SM;
     a.perm = (int*)take(sizeof(int) * (size_t)nmem * batch);
     a.perm_tmp = (int*)take(sizeof(int) * (size_t)nmem * batch);
     a.nrm = (float*)take(sizeof(float) * (size_t)nmem * batch);
@@ -408,8 +796,50 @@ void launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb
     a.cflag = (int*)take(sizeof(int) * (size_t)maxC * batch);
     a.hkey = (float*)take(sizeof(float) * (size_t)maxC * batch);
     a.hval = (int*)take(sizeof(int) * (size_t)maxC * batch);
+    float* d_cost0 = (float*)take(sizeof(float) * (size_t)ncl * batch);
+    float* d_hkey0 = (float*)take(sizeof(float) * (size_t)ncl * batch);
+    int* d_hval0 = (int*)take(sizeof(int) * (size_t)ncl * batch);
+    a.iota = d_iota; a.cost0 = d_cost0; a.hkey0 = d_hkey0; a.hval0 = d_hval0;
+    RZK_CHECK((size_t)(p - static_cast<char*>(raw)) <= shared_bytes + row_bytes * batch, RZK_E_STATE,
+              "refinement scratch layout overflow");
     RZK_HIP(hipMemcpyAsync(d_draws, h_draws.data(), sizeof(float) * h_draws.size(), hipMemcpyHostToDevice, st));
-    const size_t lds = refine_lds_bytes(nnb);
+    std::vector<int> h_iota(nmem);
+    for (int e = 0; e < nmem; ++e) h_iota[e] = e;
+    RZK_HIP(hipMemcpyAsync(d_iota, h_iota.data(), sizeof(int) * (size_t)nmem, hipMemcpyHostToDevice, st));
+    const unsigned gm = (unsigned)((nmem + 255) / 256);
+    hipLaunchKernelGGL(k_gather_members, dim3(gm), dim3(256), 0, st, d_mags, d_mem, nmem, d_mm);
+    hipLaunchKernelGGL(k_gather_members, dim3(gm), dim3(256), 0, st, d_norms_R, d_mem, nmem, d_nn);
+    RZK_HIP(hipGetLastError());
+    // the cluster table and cost heap go to LDS when they fit beside the rest
+    const bool lds_cl = ref_layout(nnb, maxC, true).bytes <= kRefLdsMax;
+    const RefLayout lay = ref_layout(nnb, maxC, lds_cl);
+    RZK_CHECK(lay.bytes <= kRefLdsMax, RZK_E_INVALID, "refinement LDS layout exceeds a compute unit");
+    a.lds_clusters = lds_cl ? 1 : 0;
+    const size_t lds = lay.bytes, lds_cost = lay.oVal;
+    RZK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_refine_slices),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    RZK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_initial_costs),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cost));
+    // per-row data that does not depend on the draws, for slices s0 .. s0 + n - 1
+    // in rows s0 - batch0 ..: the slice-matrix copies, the kept clusters'
+    // costs and the heap they make
+    auto prepare_rows = [&](int s0, int n, int batch0) {
+        const unsigned gx = (unsigned)(((int64_t)nmem * M + 255) / 256);
+        for (int y0 = 0; y0 < n; y0 += 65535) {
+            const int ny = std::min(65535, n - y0);
+            hipLaunchKernelGGL(k_slice_columns, dim3(gx, (unsigned)ny), dim3(256), 0, st, d_R, S, d_nb, nnb, d_mem, nmem,
+                               s0 + y0, d_SM + (size_t)(s0 + y0 - batch0) * nmem * M);
+            RefineArgs b = a;
+            b.s0 = s0 + y0;
+            b.batch0 = batch0;
+            hipLaunchKernelGGL(k_initial_costs, dim3((unsigned)ncl, (unsigned)ny), dim3(kRefBlock), lds_cost, st, b);
+            hipLaunchKernelGGL(k_initial_heap, dim3((unsigned)ny), dim3(64), sizeof(float) * 2 * (size_t)ncl, st,
+                               d_cost0 + (size_t)(s0 + y0 - batch0) * ncl, ncl, d_hkey0 + (size_t)(s0 + y0 - batch0) * ncl,
+                               d_hval0 + (size_t)(s0 + y0 - batch0) * ncl);
+        }
+        RZK_HIP(hipGetLastError());
+    };
+    if (resident) prepare_rows(0, S, 0);
     std::vector<int> h_used(S);
     int first = 0;                                         // slices before `first` are final
     for (int pass = 0; first < S; ++pass) {
@@ -417,9 +847,11 @@ void launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb
         RZK_HIP(hipMemcpyAsync(d_doff + first, h_off.data() + first, sizeof(int64_t) * (S - first),
                                hipMemcpyHostToDevice, st));
         for (int b0 = first; b0 < S; b0 += batch) {
+            const int nb0 = std::min(batch, S - b0);
             a.s0 = b0;
-            a.batch0 = b0;
-            hipLaunchKernelGGL(k_refine_slices, dim3(std::min(batch, S - b0)), dim3(kRefBlock), lds, st, a);
+            a.batch0 = resident ? 0 : b0;
+            if (!resident) prepare_rows(b0, nb0, b0);
+            hipLaunchKernelGGL(k_refine_slices, dim3(nb0), dim3(kRefBlock), lds, st, a);
             RZK_HIP(hipGetLastError());
         }
         RZK_HIP(hipMemcpyAsync(h_used.data() + first, d_used + first, sizeof(int) * (S - first),

@@ -185,14 +185,16 @@ RZK_HD int floor_log2(int n) {
 // std::sort(key/val[0, n)) by key.  The reference's recursion on the right
 // part is replaced by a stack of pending ranges: the parts are disjoint, so
 // the order they are finished in does not change any element's final place.
+// depth0 < 0: the introsort depth limit 2*floor(log2 n) (tests pass a smaller
+// one to reach the heap-sort fallback).
 template <typename K, typename V>
-RZK_HD void sort_pairs(K* key, V* val, int n) {
+RZK_HD void sort_pairs(K* key, V* val, int n, int depth0 = -1) {
     if (n <= 1) return;
     int st_first[kSortStack], st_last[kSortStack], st_depth[kSortStack];
     int sp = 0;
     st_first[sp] = 0;
     st_last[sp] = n;
-    st_depth[sp] = 2 * floor_log2(n);
+    st_depth[sp] = depth0 >= 0 ? depth0 : 2 * floor_log2(n);
     ++sp;
     while (sp > 0) {
         --sp;

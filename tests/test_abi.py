@@ -90,3 +90,16 @@ def test_std_sort_and_priority_queue_restatement():
     r = subprocess.run([os.path.join(here, "cpp", "std_order_test")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert " 0 mismatches" in r.stdout
+
+
+def test_block_parallel_sort_restatement():
+    """the block-parallel std::sort used by the refinement kernel (run lane by
+    lane on the host) gives the one-lane restatement's permutation, and at the
+    default depth limit std::sort's own, ties and heap-sort fallbacks included"""
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run(["make", "-s", "-C", os.path.join(here, "cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([os.path.join(here, "cpp", "par_sort_test")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert " 0 mismatches" in r.stdout
