@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per gather launch (k_slice_visibility + k_slice_shade) measured by rocprofv3 --pmc FETCH_SIZE
 # (x2, the 512 MiB copy calibration) and WRITE_SIZE (x1) on this workload; the passes and the calibration
 # are recorded in profiles/slice_gather_pmc.md
-GATHER_HBM_BYTES = (251224 + 49205) * 2 * 1024 + (19614 + 3072) * 1024
+GATHER_HBM_BYTES = (250621 + 49215) * 2 * 1024 + (19626 + 3072) * 1024
 # MI355X vector issue: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz lane-operations per second
 VALU_LANE_OPS_PEAK = 256 * 64 * 2.4e9
 # algorithmic bytes per shadow segment: ray in, result out, every 4-wide node
@@ -172,6 +172,79 @@ def run_guided(args):
            "guide_build_ms": round(init_ms, 1),
            "wavefront": {"ms": round(wf_ms, 3), "mrays_per_s": round(nrays / wf_ms / 1e3, 1),
                          "same_rays": wrays == rays, "bit_equal_image": same, "config": b.config}}
+    print(json.dumps(out), flush=True)
+
+
+def run_refine(args):
+    """LightSlice per-slice refinement on the 200k-triangle LightSlice workload
+    (512x512, 65,536 VPLs, 32x32-pixel slices, 16 neighbour slices, 300 kept
+    clusters refined to 600 per slice, the reference driver's 2x rule).  A step
+    is one initialize with refinement; the refinement time is that minus an
+    initialize without it (same LightSlice inputs).  The refinement reads the
+    reference's single MT stream slice after slice, so it takes one pass per
+    slice whose number count differs from 4 per split; the pass count is
+    reported.  cpu_baseline: the oracle refining the first slices (the
+    sequential computation's own prefix) on one thread."""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    import oracle
+    W = H = 512
+    maxc = 600
+    sc = scenes.room(W, H, seed=2013, target_tris=args.tris or 200_000)
+    ctx = rz.Ctx(0)
+    scene = rz.Scene(ctx, sc)
+    tracer, filler = vplgen.gpu_tracer(scene)
+    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 65_536, seed=2013, batch=65536)
+    vpls = rz.Vpls(ctx, vpl, vmat)
+    params = dict(width=W, height=H, tile=32, num_clusters=300, num_neighbor_slices=16, map_resolution=32,
+                  n_emitted=n_emitted, seed=2013)
+    plain = rz.LightSlice(ctx, scene, vpls, **params)
+    refined = rz.LightSlice(ctx, scene, vpls, max_slice_clusters=maxc, **params)
+
+    def timed(ls):
+        t0 = time.perf_counter()
+        ls.initialize(sc.camera)
+        ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+    for i in range(args.warmup):
+        timed(plain)
+        timed(refined)
+    plain_ms, refined_ms = [], []
+    for i in range(args.steps):
+        plain_ms.append(timed(plain))
+        refined_ms.append(timed(refined))
+        log(f"[bench] step {i + 1}/{args.steps}: initialize {plain_ms[-1]:.1f} ms, with refinement "
+            f"{refined_ms[-1]:.1f} ms")
+    inf = refined.info()
+    S = inf.num_slices
+    refine_ms = float(np.mean(refined_ms)) - float(np.mean(plain_ms))
+    cnt = refined.get("slice_count", np.int32, S)
+    out = {"metric": "LightSlice per-slice refinement, slices/s", "value": round(S / (refine_ms / 1e3), 3),
+           "unit": "slices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(float(np.mean(refined_ms)), 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded room scene, seeded light-traced VPLs)",
+           "config": {"workload": "lightslice refinement room 200k tris 512x512 65536 VPLs 32x32 slices "
+                                  "16 neighbours 300 -> 600 clusters", "triangles": int(sc.ntri), "vpls": int(len(vmat)),
+                      "slices": S, "kept_clusters": int(inf.num_clusters), "max_slice_clusters": maxc,
+                      "parallelism": "single GPU"},
+           "refine_ms": round(refine_ms, 1), "initialize_ms": round(float(np.mean(plain_ms)), 2),
+           "refine_passes": int(inf.refine_passes), "slice_clusters": [int(cnt.min()), int(cnt.max())]}
+    if not args.no_cpu:
+        R = refined.get("R", np.float32, S * len(vmat) * 3).reshape(len(vmat), S, 3)
+        nb = refined.get("neighbors", np.int32, S * 16).reshape(S, 16)
+        k = 24                                   # slices in the sample
+        t0 = time.perf_counter()
+        oracle.cluster_by_sampling(R, 300, 2013, nthreads=16)
+        t1 = time.perf_counter()
+        o = oracle.cluster_and_refine(R[:, :, :], 300, maxc, nb, 2013, nthreads=16, slices=k)
+        dt = time.perf_counter() - t1 - (t1 - t0)      # the refinement part of the oracle's run
+        out["cpu_baseline"] = {"value": round(k / dt, 3), "unit": "slices/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle refinement of the first {k} slices (the sequential run's prefix, "
+                                         f"one thread; clustering on 16 threads subtracted), {dt:.1f} s"}
+        same = all(np.array_equal(o["rep"][s, :o["count"][s]],
+                                  refined.get("slice_reps", np.int32, S * maxc).reshape(S, maxc)[s, :o["count"][s]])
+                   for s in range(k))
+        out["cpu_baseline"]["same_representatives"] = bool(same)
     print(json.dumps(out), flush=True)
 
 
@@ -389,7 +462,7 @@ def run_served(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided", "batched", "served"],
+    ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided", "batched", "served", "refine"],
                     help="lightslice: the 200k-triangle LightSlice gather step (the headline line); "
                          "guided: guided path tracing of the 1M-triangle scene; batched: 256 uneven jobs")
     ap.add_argument("--jobs", type=int, default=256, help="batched workload: number of jobs")
@@ -419,6 +492,8 @@ def main():
         return run_batched(args)
     if args.workload == "served":
         return run_served(args)
+    if args.workload == "refine":
+        return run_refine(args)
 
     import renzoku_amd as rz
     from renzoku_amd import scenes, vplgen
@@ -501,7 +576,7 @@ def main():
                      "note": "achieved counts every lane's node visits (128 B per 4-wide node, 48 B per triangle "
                              "test, 36 B of segment I/O); a wave fetches each node once and L2/MALL serve it, so "
                              "measured HBM traffic per launch (traffic) is far below; the visibility kernel is "
-                             "vector-issue bound (VALUBusy 75.6%, about 71% of lane-op peak, rocprofv3 --pmc)"},
+                             "vector-issue bound (VALUBusy 79.4%, about 77% of lane-op peak, rocprofv3 --pmc)"},
     }
     if not args.no_cpu:
         reps = ls.get("reps", np.int32, lsinf.num_clusters)

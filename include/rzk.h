@@ -230,6 +230,9 @@ typedef struct rzk_lightslice_info {
     int32_t draws;                  /* centre draws taken */
     int64_t reduced_pairs;          /* rows x VPLs */
     int64_t gather_pairs;           /* pixels with a material x clusters */
+    int32_t refine_passes;          /* passes of the per-slice refinement (0: no refinement); each pass
+                                       re-runs the slices after the first whose random-number count
+                                       differed from 4 per split */
 } rzk_lightslice_info;
 
 /* the LightSlice keeps scene and vpls by pointer: both must outlive it */

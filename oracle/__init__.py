@@ -74,6 +74,7 @@ def lib():
             "orc_pdf_build": ([I, P, P, P], I), "orc_pdf_sample": ([I, P, P, F, P, P], None),
             "orc_cluster_by_sampling": ([I, I, P, I, P, P, P, P, P, P, I], I),
             "orc_refine_slices": ([I, I, P, I, P, I, P, P, P, P, P, I, P, P, P, P, P, P], I),
+            "orc_refine_slices_n": ([I, I, I, P, I, P, I, P, P, P, P, P, I, P, P, P, P, P, P], I),
             "orc_slice_cost": ([I, P, I, P, I, P], F),
             "orc_direction_to_square": ([P, P], None), "orc_square_to_direction": ([P, P], None),
             "orc_slice_cluster_info": ([P, P, I, P, P, P, P, I, F, P, P], None),
@@ -371,9 +372,10 @@ def slice_cost(R, nb_rows, idx):
     return float(lib().orc_slice_cost(R.shape[1], _p(R), len(nb_rows), _p(nb_rows), len(idx), _p(idx)))
 
 
-def cluster_and_refine(R, num_clusters, max_slice_clusters, nb, seed, nthreads=1, with_perm=False):
+def cluster_and_refine(R, num_clusters, max_slice_clusters, nb, seed, nthreads=1, with_perm=False, slices=None):
     """cluster_by_sampling then the per-slice refinement on one MT stream seeded
-    with `seed` (lightslice.cc:757-902).  R: (V, S, 3); nb: (S, nnb) rows."""
+    with `seed` (lightslice.cc:757-902).  R: (V, S, 3); nb: (S, nnb) rows;
+    slices: refine only the first `slices` slices (default: all)."""
     R = _f(R)
     V, S = R.shape[0], R.shape[1]
     nb = _i(nb)
@@ -394,8 +396,9 @@ def cluster_and_refine(R, num_clusters, max_slice_clusters, nb, seed, nthreads=1
     count, rep, w = np.zeros(S, np.int32), np.zeros(S * M, np.int32), np.zeros(S * M, np.float32)
     size, draws = np.zeros(S * M, np.int32), np.zeros(S, np.int32)
     perm = np.zeros(S * V, np.int32) if with_perm else None
-    rc = L.orc_refine_slices(S, V, _p(R), nb.shape[1], _p(nb), len(creps), _p(off), _p(mem), _p(creps), _p(crw),
-                             _p(mags), M, _p(count), _p(rep), _p(w), _p(size), _p(perm), _p(draws))
+    ns = S if slices is None else int(slices)
+    rc = L.orc_refine_slices_n(S, ns, V, _p(R), nb.shape[1], _p(nb), len(creps), _p(off), _p(mem), _p(creps),
+                               _p(crw), _p(mags), M, _p(count), _p(rep), _p(w), _p(size), _p(perm), _p(draws))
     assert rc == 0, rc
     out = dict(centers=centers[:ncl], assign=assign, reps=creps, rep_w=crw, off=off, mem=mem, count=count,
                rep=rep.reshape(S, M), w=w.reshape(S, M), size=size.reshape(S, M), draws=draws)

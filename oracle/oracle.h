@@ -114,6 +114,11 @@ int  orc_refine_slices(int S, int V, const float *R, int nnb, const int *nb, int
                        const int *cl_off, const int *cl_mem, const int *cl_rep, const float *cl_w,
                        const float *mags, int max_slice_clusters, int *out_count, int *out_rep,
                        float *out_w, int *out_size, int *out_perm, int *out_draws);
+/* the same for slices 0 .. n_slices - 1 only (a prefix of the sequential run) */
+int  orc_refine_slices_n(int S, int n_slices, int V, const float *R, int nnb, const int *nb, int ncl,
+                         const int *cl_off, const int *cl_mem, const int *cl_rep, const float *cl_w,
+                         const float *mags, int max_slice_clusters, int *out_count, int *out_rep,
+                         float *out_w, int *out_size, int *out_perm, int *out_draws);
 float orc_slice_cost(int S, const float *R, int nnb, const int *nb, int n, const int *idx);
 
 /* ---- guiding distributions ---- */

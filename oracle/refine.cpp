@@ -162,7 +162,8 @@ extern "C" float orc_slice_cost(int S, const float *R, int nnb, const int *nb, i
     return compute_distance(L, std::vector<int>(idx, idx + n));
 }
 
-/* Refine the kept global clusters per slice (lightslice.cc:837-902).
+/* Refine the kept global clusters per slice (lightslice.cc:837-902), slices
+ * 0 .. n_slices - 1 (a prefix of the sequential run: same numbers, same results).
  *   R        : column-major Rgb reduced matrix, R[(k * S + r) * 3 + c]
  *   nb       : S x nnb neighbour rows of every slice (the slice matrix L)
  *   cl_off/cl_mem : the ncl kept clusters' members, in order (CSR)
@@ -172,11 +173,12 @@ extern "C" float orc_slice_cost(int S, const float *R, int nnb, const int *nb, i
  * Outputs per slice s (max_slice_clusters entries each): count, rep, weight,
  * size; perm (optional, S x V): the members cluster by cluster; draws: MT
  * numbers consumed.  Returns 0, or -1 on bad sizes. */
-extern "C" int orc_refine_slices(int S, int V, const float *R, int nnb, const int *nb, int ncl, const int *cl_off,
-                                 const int *cl_mem, const int *cl_rep, const float *cl_w, const float *mags,
-                                 int max_slice_clusters, int *out_count, int *out_rep, float *out_w, int *out_size,
-                                 int *out_perm, int *out_draws) {
-    if (S <= 0 || V <= 0 || nnb <= 0 || ncl <= 0 || max_slice_clusters < ncl) return -1;
+extern "C" int orc_refine_slices_n(int S, int n_slices, int V, const float *R, int nnb, const int *nb, int ncl,
+                                   const int *cl_off, const int *cl_mem, const int *cl_rep, const float *cl_w,
+                                   const float *mags, int max_slice_clusters, int *out_count, int *out_rep,
+                                   float *out_w, int *out_size, int *out_perm, int *out_draws) {
+    if (S <= 0 || V <= 0 || nnb <= 0 || ncl <= 0 || max_slice_clusters < ncl || n_slices < 0 || n_slices > S)
+        return -1;
     /* norms of R's columns (LightTransportMatrix::get_column_norms,
      * light_transport_matrix.cc:117-130): per row r^2 + g^2 + b^2, then summed */
     std::vector<float> norms_R(V);
@@ -200,7 +202,7 @@ extern "C" int orc_refine_slices(int S, int V, const float *R, int nnb, const in
     }
     std::vector<int> i1, i2;
     std::vector<float> nrm;
-    for (int s = 0; s < S; ++s) {
+    for (int s = 0; s < n_slices; ++s) {
         SliceMatrix L{R, S, nnb, nb + (int64_t)s * nnb};
         std::vector<Cluster> cls(global);
         std::priority_queue<ClusterCost, std::vector<ClusterCost>, ClusterCostLess> heap;
@@ -244,4 +246,13 @@ extern "C" int orc_refine_slices(int S, int V, const float *R, int nnb, const in
         }
     }
     return 0;
+}
+
+/* every slice: orc_refine_slices_n with n_slices = S */
+extern "C" int orc_refine_slices(int S, int V, const float *R, int nnb, const int *nb, int ncl, const int *cl_off,
+                                 const int *cl_mem, const int *cl_rep, const float *cl_w, const float *mags,
+                                 int max_slice_clusters, int *out_count, int *out_rep, float *out_w, int *out_size,
+                                 int *out_perm, int *out_draws) {
+    return orc_refine_slices_n(S, S, V, R, nnb, nb, ncl, cl_off, cl_mem, cl_rep, cl_w, mags, max_slice_clusters,
+                               out_count, out_rep, out_w, out_size, out_perm, out_draws);
 }

@@ -129,7 +129,7 @@ class ServerStats(C.Structure):
 class LightSliceInfo(C.Structure):
     _fields_ = [
         ("num_slices", C.c_int32), ("num_centers", C.c_int32), ("num_clusters", C.c_int32),
-        ("draws", C.c_int32), ("reduced_pairs", C.c_int64), ("gather_pairs", C.c_int64),
+        ("draws", C.c_int32), ("reduced_pairs", C.c_int64), ("gather_pairs", C.c_int64), ("refine_passes", C.c_int32),
     ]
 
 

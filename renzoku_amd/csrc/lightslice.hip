@@ -26,7 +26,7 @@ void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, 
                         int* d_assign);
 void launch_column_norms_R(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_out);
 int64_t refine_draws_needed(int S, int ncl, int maxC);
-void launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb, int nnb, int ncl, const int* d_off,
+int launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb, int nnb, int ncl, const int* d_off,
                           const int* d_mem, int nmem, const int* d_grep, const float* d_gw, const float* d_mags,
                           const float* d_norms_R, int maxC, const std::vector<float>& h_draws, int* d_rep,
                           float* d_w, int* d_count);
@@ -793,6 +793,7 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         // per-slice refinement (lightslice.cc:837-902) of the kept clusters, the
         // MT stream continuing after the representatives
         ls->ld = 0;
+        ls->info.refine_passes = 0;
         if (p.max_slice_clusters > nreps && nreps > 0) {
             std::vector<int> koff(1, 0), kmem;
             kmem.reserve(V);
@@ -806,9 +807,11 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
             launch_column_norms_R(ctx, S, V, ls->R.p, ls->norms_R.p);
             std::vector<float> draws((size_t)refine_draws_needed(S, nreps, p.max_slice_clusters));
             for (auto& x : draws) x = rd.next_float();
-            launch_refine_slices(ctx, ls->R.p, S, ls->nb.p, p.num_neighbor_slices, nreps, ls->cl_off.p, ls->cl_mem.p,
-                                 (int)kmem.size(), ls->reps.p, ls->rep_w.p, ls->mags.p, ls->norms_R.p,
-                                 p.max_slice_clusters, draws, ls->sreps.p, ls->srep_w.p, ls->scount.p);
+            ls->info.refine_passes =
+                launch_refine_slices(ctx, ls->R.p, S, ls->nb.p, p.num_neighbor_slices, nreps, ls->cl_off.p,
+                                     ls->cl_mem.p, (int)kmem.size(), ls->reps.p, ls->rep_w.p, ls->mags.p,
+                                     ls->norms_R.p, p.max_slice_clusters, draws, ls->sreps.p, ls->srep_w.p,
+                                     ls->scount.p);
             ls->ld = p.max_slice_clusters;
             ls->h_scount.resize(S);
             RZK_HIP(hipMemcpyAsync(ls->h_scount.data(), ls->scount.p, sizeof(int) * S, hipMemcpyDeviceToHost, st));

@@ -742,8 +742,9 @@ int64_t refine_draws_needed(int S, int ncl, int maxC) { return 4 * (int64_t)(max
 // representatives/weights d_grep/d_gw) into at most maxC clusters per slice.
 // h_draws: the MT numbers that follow the clustering's, at least
 // refine_draws_needed(S, ncl, maxC) of them; outputs d_rep/d_w (S x maxC) and
-// d_count (S).
-void launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb, int nnb, int ncl, const int* d_off,
+// d_count (S).  Returns the number of passes over the slices (1 when every
+// slice took 4 numbers per split).
+int launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb, int nnb, int ncl, const int* d_off,
                           const int* d_mem, int nmem, const int* d_grep, const float* d_gw, const float* d_mags,
                           const float* d_norms_R, int maxC, const std::vector<float>& h_draws_in, int* d_rep,
                           float* d_w, int* d_count) {
@@ -861,9 +862,10 @@ void launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb
         int wrong = S;
         for (int s = first; s + 1 < S; ++s)
             if (h_off[s] + h_used[s] != h_off[s + 1]) { wrong = s + 1; break; }
-        if (wrong == S) break;
+        if (wrong == S) return pass + 1;
         h_off[wrong] = h_off[wrong - 1] + h_used[wrong - 1];
         for (int s = wrong + 1; s < S; ++s) h_off[s] = h_off[s - 1] + per;
         first = wrong;
     }
+    return 0;
 }
