@@ -248,6 +248,62 @@ def run_refine(args):
     print(json.dumps(out), flush=True)
 
 
+def run_vplgen(args):
+    """Bounce-1 VPL generation on the device (rzk_generate_vpls) for the
+    200k-triangle LightSlice scene: 65,536 VPLs, one closest ray per light
+    sample on the reference tree.  A step is one whole generation (light
+    sampling, tracing, ordered compaction, copy of the records to the host).
+    Roofline: the closest-ray byte model (48 B of ray I/O, 64 B per visited
+    binary node, 48 B per tested triangle) over the step time.  cpu_baseline:
+    the oracle generating the same VPLs on one thread (the whole workload, well
+    under a second of CPU work).  value counts the light samples used; the
+    byte model counts every ray traced."""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes
+    import oracle
+    W = H = 512
+    nv = args.vpls or 65_536
+    sc = scenes.room(W, H, seed=2013, target_tris=args.tris or 200_000)
+    ctx = rz.Ctx(0)
+    scene = rz.Scene(ctx, sc)
+    _, _, used, work = rz.generate_vpls(ctx, scene, nv, seed=2013, count_work=True)
+    for i in range(args.warmup):
+        rz.generate_vpls(ctx, scene, nv, seed=2013)
+    ms = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        rz.generate_vpls(ctx, scene, nv, seed=2013)
+        ms.append((time.perf_counter() - t0) * 1e3)
+        log(f"[bench] step {i + 1}/{args.steps}: {ms[-1]:.2f} ms")
+    ms_per_step = float(np.mean(ms))
+    rays = work["closest_rays"]           # traced, with the last batch's samples past the last one used
+    gbytes = rays * (RAY_IN_BYTES + 16) + work["nodes"] * 64 + work["tris"] * TRI_BYTES
+    achieved = gbytes / (ms_per_step / 1e3) / 1e9
+    out = {"metric": "bounce-1 VPL generation, light paths/s (closest rays)", "value": round(used / (ms_per_step / 1e3) / 1e6, 3),
+           "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (seeded room scene)",
+           "config": {"workload": "vpl generation room 200k tris 65536 bounce-1 VPLs", "triangles": int(sc.ntri),
+                      "vpls": nv, "light_samples": int(used), "parallelism": "single GPU"},
+           "vpls_per_s": round(nv / (ms_per_step / 1e3), 1), "work": work,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "rzk_generate_vpls (k_light_vpls, k_vpl_order, k_vpl_scatter, host copy)",
+                        "note": "step time includes the ordered compaction and the copy of the records to the host"}}
+    if not args.no_cpu:
+        k = nv
+        osc = oracle.OrcScene(sc)
+        t0 = time.perf_counter()
+        ov, om, ou = osc.generate_vpls(k, seed=2013)
+        dt = time.perf_counter() - t0
+        gv, gm, gu = rz.generate_vpls(ctx, scene, k, seed=2013)
+        out["cpu_baseline"] = {"value": round(ou / dt / 1e6, 4), "unit": "Mrays/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle generation of the first {k} VPLs ({ou} light samples) on one "
+                                         f"thread, {dt:.2f} s",
+                               "same_vpls": bool(gu == ou and np.array_equal(gv.view(np.uint32), ov.view(np.uint32)))}
+    print(json.dumps(out), flush=True)
+
+
 def c3_scene_and_guides(args, levels):
     """the 1M-triangle instanced scene at 1280x720 with one LightSlice guide per
     VPL count in `levels` (16x16-pixel slices, 1,000 clusters each)"""
@@ -462,7 +518,7 @@ def run_served(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided", "batched", "served", "refine"],
+    ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided", "batched", "served", "refine", "vplgen"],
                     help="lightslice: the 200k-triangle LightSlice gather step (the headline line); "
                          "guided: guided path tracing of the 1M-triangle scene; batched: 256 uneven jobs")
     ap.add_argument("--jobs", type=int, default=256, help="batched workload: number of jobs")
@@ -494,6 +550,8 @@ def main():
         return run_served(args)
     if args.workload == "refine":
         return run_refine(args)
+    if args.workload == "vplgen":
+        return run_vplgen(args)
 
     import renzoku_amd as rz
     from renzoku_amd import scenes, vplgen

@@ -152,6 +152,23 @@ int rzk_vpls_create(rzk_ctx *ctx, const float *vpl, const int32_t *vpl_mat, int6
                     rzk_vpls **out);
 int rzk_vpls_destroy(rzk_vpls *vpls);
 
+/* VplGenerator::spawn with max_bounce = 2 over area lights (vpl_generator.cc:127-181;
+ * SubpathGenerator::sample_direct_vertex, path.cc:171-220, 305-334; AreaLight::sample,
+ * area_light.cc:26-44): light samples 0, 1, 2, ... each pick a light (Scene::sample_light),
+ * a triangle of it by area and a uniform point, and a cosine-weighted direction in the light
+ * surface's basis; the closest hit of a sample on a surface with a material (not a light)
+ * becomes a bounce-1 VPL (RZK_VPL_FLOATS layout: p, shading normal, uvn, wi = -ray direction,
+ * power = Le |n_q . wo| / (pdf_light pdf_q pdf_wo)).  Samples are taken in order until num_vpls
+ * VPLs are stored (at most max_samples samples, else RZK_E_STATE).  Each sample's numbers come
+ * from Philox4x32-10 keyed by seed with the sample index as counter (the reference draws them
+ * from its global MT stream).  out_samples: light samples used (the gather's
+ * num_emitted_photons); out_work (optional, 3 values): closest rays traced (batches may trace
+ * a few samples past the last one used), interior nodes visited, triangles tested.  Device
+ * buffers come from the context's scratch. */
+int rzk_generate_vpls(rzk_ctx *ctx, const rzk_scene *scene, int64_t num_vpls, uint32_t seed,
+                      int64_t max_samples, float tmin, float *out_vpl, int32_t *out_mat,
+                      int64_t *out_samples, int64_t *out_work);
+
 /* ---- gather (vpl.cc:661-688 gather_point, summed over the VPL set as
  * lightslice.cc:1325-1339 does over cluster representatives of weight 1,
  * then / num_emitted as vpl.cc:488-489) ----------------------------------- */

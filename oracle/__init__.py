@@ -81,6 +81,9 @@ def lib():
             "orc_slice_cluster_pdf": ([I, P, I, P, I, P, P, P, P], None),
             "orc_build_map": ([I, I, P, P, P, P, P, F, I, P, P, P, P, P], I),
             "orc_scene_set_lights": ([P, I, P], None), "orc_philox": ([P, C.c_uint32, C.c_uint32], None),
+            "orc_sincos_turn": ([C.c_float, P, P], None),
+            "orc_generate_vpls": ([P, C.c_longlong, C.c_uint32, C.c_longlong, C.c_float, P, P,
+                                   C.POINTER(C.c_longlong)], I),
             "orc_bsdf_pdf": ([P, P, P, P, P], None), "orc_bsdf_sample": ([P, P, P, P, P, P, P], None),
             "orc_guided_pt": ([P, P, I, I, I, I, C.c_uint32, F, I, I, P, P, P, P, I, I, I, I, P, I], None),
         }
@@ -112,6 +115,17 @@ class OrcScene:
             self.L.orc_scene_destroy(self.h)
         except Exception:
             pass
+
+    def generate_vpls(self, num_vpls, seed=2013, max_samples=1 << 26, tmin=1e-3):
+        """orc_generate_vpls: (vpl (n, 21) f32, vmat (n,) i32, light samples used)"""
+        vpl = np.zeros((int(num_vpls), 21), np.float32)
+        vmat = np.zeros(int(num_vpls), np.int32)
+        used = C.c_longlong()
+        rc = self.L.orc_generate_vpls(self.h, int(num_vpls), int(seed) & 0xffffffff, int(max_samples), float(tmin),
+                                      _p(vpl), _p(vmat), C.byref(used))
+        if rc != 0:
+            raise RuntimeError(f"only part of {num_vpls} VPLs after {used.value} light samples")
+        return vpl, vmat, int(used.value)
 
     def export_bvh(self):
         n = self.ntri
@@ -405,6 +419,13 @@ def cluster_and_refine(R, num_clusters, max_slice_clusters, nb, seed, nthreads=1
     if with_perm:
         out["perm"] = perm.reshape(S, V)
     return out
+
+
+def sincos_turn(u):
+    """the engine's cos/sin of 2 pi u, restated: (sin, cos)"""
+    a, b = C.c_float(), C.c_float()
+    lib().orc_sincos_turn(float(u), C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 def direction_to_square(wi):

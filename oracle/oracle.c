@@ -1502,3 +1502,108 @@ void orc_guided_pt(const orc_scene *s, const float cam[ORC_CAM_STRIDE], int spp,
                  (W + tile - 1) / tile, (H + tile - 1) / tile, W, x0, y0, w, h, maps, row_cdf, col_cdf, totals, out};
     parallel_for(w * h, nthreads, pt_range, &a);
 }
+
+/* ======================================================================= */
+/* Bounce-1 VPL generation                                                 */
+/* ======================================================================= */
+/* cos and sin of 2 pi u, u in [0, 1): octant reduction and Taylor polynomials
+ * of degree 9 / 10 on [0, pi/4] (the engine's own routine, restated; a few ulp
+ * from libm's cos/sin of 2 pi u) */
+void orc_sincos_turn(float u, float *s, float *c) {
+    const float x = u * 8.0f;
+    int o = (int)x;
+    if (o > 7) o = 7;
+    if (o < 0) o = 0;
+    const float r = x - (float)o;
+    const float f = (o & 1) ? 1.0f - r : r;
+    const float a = f * 0.785398163397448f;
+    const float a2 = a * a;
+    const float sa =
+        a * (1.0f + a2 * (-0.166666667f + a2 * (0.00833333333f + a2 * (-0.000198412698f + a2 * 2.75573192e-6f))));
+    const float ca = 1.0f + a2 * (-0.5f + a2 * (0.0416666667f + a2 * (-0.00138888889f +
+                                                                     a2 * (2.48015873e-5f + a2 * -2.75573192e-7f))));
+    switch (o) {
+        case 0: *s = sa; *c = ca; break;
+        case 1: *s = ca; *c = sa; break;
+        case 2: *s = ca; *c = -sa; break;
+        case 3: *s = sa; *c = -ca; break;
+        case 4: *s = -sa; *c = -ca; break;
+        case 5: *s = -ca; *c = -sa; break;
+        case 6: *s = -ca; *c = sa; break;
+        default: *s = -sa; *c = ca; break;
+    }
+}
+
+/* the 8 numbers of light sample `index` (Philox4x32-10 keyed by the seed) */
+static void vpl_rng(uint32_t seed, uint64_t index, uint32_t u[8]) {
+    for (uint32_t b = 0; b < 2; ++b) {
+        uint32_t c[4] = {(uint32_t)index, (uint32_t)(index >> 32), b, 0x5650u};
+        orc_philox(c, seed ^ 0x3c6ef372u, 0x4c475650u);
+        u[4 * b] = c[0]; u[4 * b + 1] = c[1]; u[4 * b + 2] = c[2]; u[4 * b + 3] = c[3];
+    }
+}
+
+/* VplGenerator::spawn with max_bounce = 2 over area lights (vpl_generator.cc:127-181):
+ * per light sample, Scene::sample_light (strict <), SubpathGenerator::sample_direct_vertex
+ * (path.cc:171-220, 305-334) with AreaLight::sample (area_light.cc:26-44: triangle by
+ * cumulative area ratio t <= ratio (quad.cc:75-87), uniform point (triangle.cc:298-308),
+ * cosine-weighted direction in init_from_nu(shape normal, shape tangent)), the closest hit;
+ * a hit on a surface with a material that is not a light stores a bounce-1 VPL with
+ * power Le |n_q . wo| / (pdf_light (pdf_q pdf_wo)) and wi = -ray direction.  Samples are
+ * taken in order until num_vpls are stored.  Returns 0, or -1 if max_samples ran out. */
+int orc_generate_vpls(const orc_scene *s, long long num_vpls, uint32_t seed, long long max_samples, float tmin,
+                      float *out_vpl, int *out_mat, long long *out_samples) {
+    long long have = 0, i = 0;
+    for (; i < max_samples && have < num_vpls; ++i) {
+        uint32_t u[8];
+        vpl_rng(seed, (uint64_t)i, u);
+        const float ul = u24(u[0]);
+        int k = -1;
+        for (int q = 0; q < s->nlights; ++q)
+            if (ul < s->lights[q].sel_cdf) { k = q; break; }
+        if (k < 0 || s->lights[k].count <= 0) continue;
+        const light_t *L = &s->lights[k];
+        const float ut = u24(u[2]);
+        int ti = L->first + L->count - 1;
+        for (int q = 0; q < L->count; ++q)
+            if (ut <= s->ltris[L->first + q].cum) { ti = L->first + q; break; }
+        const ltri_t *T = &s->ltris[ti];
+        const float tmp = sqrtf(1.0f - u24(u[3]));
+        const float bb = 1.0f - tmp;
+        const float cc = u24(u[4]) * tmp;
+        const float aa = 1.0f - bb - cc;
+        const v3 q = add(add(scl(T->p0, aa), scl(T->p1, bb)), scl(T->p2, cc));
+        const float pdf_q = T->ratio * (1.0f / T->area);
+        float sp, cp;
+        orc_sincos_turn(u24(u[5]), &sp, &cp);
+        const float sin_t = sqrtf(u24(u[6]));
+        const float x = sin_t * cp, y = sin_t * sp;
+        const float z = sqrtf(1.f - sin_t * sin_t);
+        v3 bu, bv, bw;
+        onb_from_nu(s->n[T->tri], s->tan[T->tri], &bu, &bv, &bw);
+        const v3 wo = unit(l2w(bu, bv, bw, mk(x, y, z)));
+        const float pdf_wo = z * INV_PI;
+        if (!(pdf_q > 0.0f && pdf_wo > 0.0f)) continue;
+        const float acc = L->sel_pdf * (pdf_q * pdf_wo);
+        const float f = fabsf(dot(T->n, wo));
+        const v3 power = mk((L->le.x * f) / acc, (L->le.y * f) / acc, (L->le.z * f) / acc);
+        const ray_t r = ray_ctor(q, wo);
+        float t = 0.f, b = 0.f, g = 0.f;
+        int tri = -1;
+        bvh_closest(s, &r, tmin, 1e6f, &t, &b, &g, &tri, NULL);
+        if (tri < 0) continue;
+        float rc[ORC_RECV_STRIDE];
+        int m = -1, light = -1;
+        fill_receiver(s, &r, t, b, g, tri, rc, &m, &light);
+        if (light >= 0 || m < 0) continue;
+        float *o = out_vpl + 21 * have;
+        for (int e = 0; e < 3; ++e) { o[e] = rc[e]; o[3 + e] = rc[6 + e]; }
+        for (int e = 0; e < 9; ++e) o[6 + e] = rc[12 + e];
+        st(o + 15, neg(r.d));
+        st(o + 18, power);
+        out_mat[have] = m;
+        ++have;
+    }
+    *out_samples = i;
+    return have == num_vpls ? 0 : -1;
+}

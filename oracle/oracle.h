@@ -147,7 +147,13 @@ void orc_guided_pt(const orc_scene *s, const float cam[ORC_CAM_STRIDE], int spp,
                    const float *row_cdf, const float *col_cdf, const float *totals, int x0, int y0, int w,
                    int h, float *out, int nthreads);
 
+/* bounce-1 VPL generation (vpl_generator.cc:127-181); see oracle.c */
+void orc_sincos_turn(float u, float *s, float *c);
+int  orc_generate_vpls(const orc_scene *s, long long num_vpls, uint32_t seed, long long max_samples, float tmin,
+                       float *out_vpl, int *out_mat, long long *out_samples);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

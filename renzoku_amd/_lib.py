@@ -27,7 +27,7 @@ EXPORTS = [
     "rzk_trace_closest", "rzk_trace_any", "rzk_trace_segments",
     "rzk_trace_closest_dev", "rzk_trace_any_dev", "rzk_trace_segments_dev", "rzk_trace_stats",
     "rzk_fill_receivers", "rzk_camera_primary",
-    "rzk_vpls_create", "rzk_vpls_destroy",
+    "rzk_vpls_create", "rzk_vpls_destroy", "rzk_generate_vpls",
     "rzk_gather", "rzk_visibility", "rzk_render_vpl",
     "rzk_reduced_matrix", "rzk_alphas", "rzk_cluster_assign", "rzk_refine_slices",
     "rzk_lightslice_create", "rzk_lightslice_destroy", "rzk_lightslice_initialize",
@@ -168,6 +168,7 @@ def load() -> C.CDLL:
         "rzk_fill_receivers": [P, P, P, P, I64, P, P, P],
         "rzk_camera_primary": [P, P, P, I64, P, P, P, P, P, P, P],
         "rzk_vpls_create": [P, P, P, I64, C.POINTER(P)],
+        "rzk_generate_vpls": [P, P, I64, C.c_uint32, I64, F, P, P, C.POINTER(I64), P],
         "rzk_vpls_destroy": [P],
         "rzk_gather": [P, P, P, P, P, I64, C.c_int, F, F, F, P],
         "rzk_visibility": [P, P, P, P, P, I64, F, P],
@@ -376,6 +377,21 @@ class Vpls:
             self.close()
         except Exception:
             pass
+
+
+def generate_vpls(ctx, scene, num_vpls, seed=2013, max_samples=1 << 26, tmin=1e-3, count_work=False):
+    """rzk_generate_vpls: (vpl (num_vpls, 21) f32, vmat (num_vpls,) i32, light samples used
+    [, work dict with closest_rays / nodes / tris])"""
+    vpl = np.zeros((int(num_vpls), VPL_FLOATS), np.float32)
+    vmat = np.zeros(int(num_vpls), np.int32)
+    used = C.c_int64()
+    work = np.zeros(3, np.int64) if count_work else None
+    ctx.check(ctx.lib.rzk_generate_vpls(ctx.h, scene.h, int(num_vpls), int(seed) & 0xffffffff, int(max_samples),
+                                        float(tmin), ptr(vpl), ptr(vmat), C.byref(used), ptr(work)),
+              "rzk_generate_vpls")
+    if count_work:
+        return vpl, vmat, int(used.value), {"closest_rays": int(work[0]), "nodes": int(work[1]), "tris": int(work[2])}
+    return vpl, vmat, int(used.value)
 
 
 def gather(ctx, scene, vpls, recv, rmat, clamp_mode, clamp, tmin, n_emitted):

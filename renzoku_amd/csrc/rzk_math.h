@@ -287,6 +287,49 @@ RZK_HD PathRng path_rng(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t
     return o;
 }
 
+// cos and sin of 2*pi*u for u in [0, 1).  The turn is split into octants; the
+// angle inside an octant (at most pi/4, reflected in odd octants) goes through
+// Taylor polynomials of degree 9 (sin) and 10 (cos), whose truncation error is
+// below 2e-9, so the results are within a few ulp of the C library's cos and
+// sin of 2*pi*u.  Every operation is a correctly rounded fp32 add or multiply
+// in a fixed order, so host and device give the same bits (no contraction).
+RZK_HD void sincos_turn(float u, float& s, float& c) {
+    const float x = u * 8.0f;
+    int o = (int)x;
+    if (o > 7) o = 7;
+    if (o < 0) o = 0;
+    const float r = x - (float)o;
+    const float f = (o & 1) ? 1.0f - r : r;
+    const float a = f * 0.785398163397448f;
+    const float a2 = a * a;
+    const float sa =
+        a * (1.0f + a2 * (-0.166666667f + a2 * (0.00833333333f + a2 * (-0.000198412698f + a2 * 2.75573192e-6f))));
+    const float ca = 1.0f + a2 * (-0.5f + a2 * (0.0416666667f + a2 * (-0.00138888889f +
+                                                                     a2 * (2.48015873e-5f + a2 * -2.75573192e-7f))));
+    switch (o) {
+        case 0: s = sa; c = ca; break;
+        case 1: s = ca; c = sa; break;
+        case 2: s = ca; c = -sa; break;
+        case 3: s = sa; c = -ca; break;
+        case 4: s = -sa; c = -ca; break;
+        case 5: s = -ca; c = -sa; break;
+        case 6: s = -ca; c = sa; break;
+        default: s = -sa; c = ca; break;
+    }
+}
+
+// the 8 numbers of light sample `index` of a VPL generation keyed by `seed`
+struct VplRng { uint32_t u[8]; };
+RZK_HD VplRng vpl_rng(uint32_t seed, uint64_t index) {
+    VplRng o;
+    for (uint32_t b = 0; b < 2; ++b) {
+        uint32_t c[4] = {(uint32_t)index, (uint32_t)(index >> 32), b, 0x5650u};
+        philox4x32_10(c, seed ^ 0x3c6ef372u, 0x4c475650u);
+        o.u[4 * b] = c[0]; o.u[4 * b + 1] = c[1]; o.u[4 * b + 2] = c[2]; o.u[4 * b + 3] = c[3];
+    }
+    return o;
+}
+
 // Rgb::max (rgb.h:42-47)
 RZK_HD float rgb_max(v3 c) { float t = c.x; if (c.y > t) t = c.y; if (c.z > t) t = c.z; return t; }
 
