@@ -34,7 +34,11 @@ __device__ __forceinline__ int traverse(const SceneView& sv, const Ray& r, float
         return -1;
     int node = sv.root_ref;
     int sp = 0;
+    // one flat loop: a step is a leaf test or a node test, then either the
+    // next node is known or the stack is popped (one pop site, so the loop
+    // keeps a single level and lanes on leaves and on nodes step together)
     for (;;) {
+        bool pop;
         if (node < 0) {
             const TriGpu* tp = sv.tris + (~node);
             float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
@@ -46,8 +50,7 @@ __device__ __forceinline__ int traverse(const SceneView& sv, const Ray& r, float
                 if (kAny) return best;
                 tmax = t; t_out = t; b_out = b; g_out = g;
             }
-            if (sp == 0) break;
-            node = stk[(--sp) * pitch];
+            pop = true;
         } else {
             const BvhNodeGpu* np = sv.nodes + node;
             float4 a = np->a, b4 = np->b, c = np->c;
@@ -55,17 +58,13 @@ __device__ __forceinline__ int traverse(const SceneView& sv, const Ray& r, float
             st.nodes++;
             bool hl = box_hit(mk(a.x, a.y, a.z), mk(a.w, b4.x, b4.y), r, tmin, tmax);
             bool hr = box_hit(mk(b4.z, b4.w, c.x), mk(c.y, c.z, c.w), r, tmin, tmax);
-            if (hl && hr) {
-                stk[(sp++) * pitch] = ch.x;
-                node = ch.y;
-            } else if (hr) {
-                node = ch.y;
-            } else if (hl) {
-                node = ch.x;
-            } else {
-                if (sp == 0) break;
-                node = stk[(--sp) * pitch];
-            }
+            if (hl && hr) stk[(sp++) * pitch] = ch.x;
+            node = hr ? ch.y : ch.x;
+            pop = !(hl || hr);
+        }
+        if (pop) {
+            if (sp == 0) break;
+            node = stk[(--sp) * pitch];
         }
     }
     return best;
