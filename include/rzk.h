@@ -169,6 +169,39 @@ int rzk_generate_vpls(rzk_ctx *ctx, const rzk_scene *scene, int64_t num_vpls, ui
                       int64_t max_samples, float tmin, float *out_vpl, int32_t *out_mat,
                       int64_t *out_samples, int64_t *out_work);
 
+/* ---- 6-D kd-tree slices (LightSlice::generate_slices, lightslice.cc:252-303) --
+ * rzk_map_vec6: map_vec6 (lightslice.cc:142-188) of n receiver records with
+ * their material ids (-1: none): position, then the directionality (shading
+ * normal; for Phong / Ward the mirror direction of wo scaled by
+ * min(4, max(1, gloss exponent / 1000))) times diagonal / 16.  out: n x 6. */
+int rzk_map_vec6(rzk_ctx *ctx, const rzk_scene *scene, const float *recv, const int32_t *mat, int64_t n,
+                 float diagonal, float *out);
+
+/* KdTreeN(max_cluster_size, ...) (kdtree_n.h:399-429, cluster_kdtree :285-365,
+ * quick_select_k :181-242): a kd-tree over n points (n x 6, host) split at the
+ * median of the longest axis; the first node of each path with at most
+ * max_slice_size (>= 3) points is a slice, numbered depth-first.  Round t of
+ * the median selection at node [s, e] takes its pivot from Philox4x32-10 keyed
+ * by seed with counter (s, e, t) (the reference draws from its global MT
+ * stream).  Arrays (rzk_kdslices_get, bytes must match): "order" (n i32: the
+ * point at each kd position), "slices" (S x 2 i32: first, last kd position),
+ * "boxes" (S x 12 f32: min[6], max[6]), and the query tree as a binary heap of
+ * heap_used nodes: "axis" (i32), "plane" (f32), "cl" (i32 slice of a slice
+ * node, else -1). */
+typedef struct rzk_kdslices rzk_kdslices;
+int rzk_kdslices_build(rzk_ctx *ctx, const float *points, int32_t n, int32_t max_slice_size, uint32_t seed,
+                       rzk_kdslices **out);
+int rzk_kdslices_destroy(rzk_kdslices *kd);
+int rzk_kdslices_info(const rzk_kdslices *kd, int32_t *out_slices, int32_t *out_heap_used);
+int rzk_kdslices_get(const rzk_kdslices *kd, const char *name, void *dst, int64_t bytes);
+/* find_cluster (kdtree_n.h:540-572): the slice of each of m query points
+ * (m x 6), descending by plane - 1e-3 (PLANE_BIAS, kdtree_n.h:171) */
+int rzk_kdslices_find(const rzk_kdslices *kd, const float *queries, int32_t m, int32_t *out);
+/* find_nearest_cluster (kdtree_n.h:575-647): the k (1..32) slices whose boxes
+ * are nearest each query, in std::priority_queue pop order (furthest first),
+ * padded with -1.  out: m x k. */
+int rzk_kdslices_nearest(const rzk_kdslices *kd, const float *queries, int32_t m, int32_t k, int32_t *out);
+
 /* ---- gather (vpl.cc:661-688 gather_point, summed over the VPL set as
  * lightslice.cc:1325-1339 does over cluster representatives of weight 1,
  * then / num_emitted as vpl.cc:488-489) ----------------------------------- */

@@ -76,6 +76,10 @@ def lib():
             "orc_refine_slices": ([I, I, P, I, P, I, P, P, P, P, P, I, P, P, P, P, P, P], I),
             "orc_refine_slices_n": ([I, I, I, P, I, P, I, P, P, P, P, P, I, P, P, P, P, P, P], I),
             "orc_slice_cost": ([I, P, I, P, I, P], F),
+            "orc_map_vec6": ([P, P, F, P], None),
+            "orc_kd_build": ([P, I, I, C.c_uint32, P, P, P, I, P, P, P, I, P], I),
+            "orc_kd_find": ([P, P, P, I, P, I, P], None),
+            "orc_kd_nearest": ([P, P, P, I, P, I, P, I, I, P], None),
             "orc_direction_to_square": ([P, P], None), "orc_square_to_direction": ([P, P], None),
             "orc_slice_cluster_info": ([P, P, I, P, P, P, P, I, F, P, P], None),
             "orc_slice_cluster_pdf": ([I, P, I, P, I, P, P, P, P], None),
@@ -459,3 +463,63 @@ def build_map(res, slice_wi, upper, rep_w, R_rep_rgb, uvn, eps, valid):
                              eps, int(valid), _p(m), _p(rc), _p(rp), _p(cc), _p(tot))
     return dict(ok=ok, map=m.reshape(res, res), row_cdf=rc, row_pdf=rp, col_cdf=cc.reshape(res, res),
                 total=float(tot[0]))
+
+
+# ---- 6-D kd-tree slicing (kdslice.cpp) ----------------------------------------
+
+def map_vec6(recv, mats, materials, diagonal):
+    """LightSlice map_vec6 of receiver records (n x RECV) with material ids
+    (-1: none) into n x 6 points"""
+    L = lib()
+    recv = _f(recv).reshape(-1, 21)
+    mats = _i(mats)
+    materials = _f(materials).reshape(-1, 12)
+    out = np.zeros((len(recv), 6), np.float32)
+    for i in range(len(recv)):
+        m = int(mats[i])
+        mp = _p(materials[m]) if m >= 0 else None
+        L.orc_map_vec6(_p(recv[i]), mp, float(diagonal), _p(out[i]))
+    return out
+
+
+class KdSlices:
+    """cluster_kdtree over n 6-D points: order, slices (S x 2), boxes (S x 12),
+    heap arrays axis / plane / cl of length heap_used"""
+
+    def __init__(self, pts, max_slice_size=256, seed=2013):
+        L = lib()
+        pts = _f(pts).reshape(-1, 6)
+        n = len(pts)
+        cap_heap = 4 * max(n, 1) + 4
+        self.order = np.zeros(n, np.int32)
+        self.slices_buf = np.zeros((max(n, 1), 2), np.int32)
+        self.boxes_buf = np.zeros((max(n, 1), 12), np.float32)
+        axis = np.zeros(cap_heap, np.int32)
+        plane = np.zeros(cap_heap, np.float32)
+        cl = np.zeros(cap_heap, np.int32)
+        used = C.c_int(0)
+        S = L.orc_kd_build(_p(pts), n, int(max_slice_size), int(seed) & 0xffffffff, _p(self.order),
+                           _p(self.slices_buf), _p(self.boxes_buf), max(n, 1), _p(axis), _p(plane), _p(cl),
+                           cap_heap, C.byref(used))
+        if S < 0:
+            raise ValueError("orc_kd_build: bad arguments")
+        self.n, self.S, self.heap_used = n, S, used.value
+        self.slices = self.slices_buf[:S].copy()
+        self.boxes = self.boxes_buf[:S].copy()
+        self.axis = axis[:used.value].copy()
+        self.plane = plane[:used.value].copy()
+        self.cl = cl[:used.value].copy()
+        self.axis_all, self.plane_all = axis, plane
+
+    def find(self, q):
+        q = _f(q).reshape(-1, 6)
+        out = np.zeros(len(q), np.int32)
+        lib().orc_kd_find(_p(self.axis), _p(self.plane), _p(self.cl), self.heap_used, _p(q), len(q), _p(out))
+        return out
+
+    def nearest(self, q, k):
+        q = _f(q).reshape(-1, 6)
+        out = np.zeros((len(q), k), np.int32)
+        lib().orc_kd_nearest(_p(self.axis), _p(self.plane), _p(self.cl), self.heap_used, _p(self.boxes), self.S,
+                             _p(q), len(q), int(k), _p(out))
+        return out

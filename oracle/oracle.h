@@ -121,6 +121,16 @@ int  orc_refine_slices_n(int S, int n_slices, int V, const float *R, int nnb, co
                          float *out_w, int *out_size, int *out_perm, int *out_draws);
 float orc_slice_cost(int S, const float *R, int nnb, const int *nb, int n, const int *idx);
 
+/* ---- 6-D kd-tree slicing (kdslice.cpp, kdtree_n.h, lightslice.cc:142-303) ---- */
+void orc_map_vec6(const float *rec, const float *mat, float diagonal, float *out);
+int  orc_kd_build(const float *pts, int n, int max_size, uint32_t seed, int *order, int *slices,
+                  float *boxes, int cap_slices, int *axis, float *plane, int *cl, int cap_heap,
+                  int *heap_used);
+void orc_kd_find(const int *axis, const float *plane, const int *cl, int heap_used, const float *q,
+                 int m, int *out);
+void orc_kd_nearest(const int *axis, const float *plane, const int *cl, int heap_used,
+                    const float *boxes, int nslices, const float *q, int m, int k, int *out);
+
 /* ---- guiding distributions ---- */
 void orc_direction_to_square(const float wi[3], float out[2]);
 void orc_square_to_direction(const float s[2], float out[3]);

@@ -39,6 +39,8 @@ EXPORTS = [
     "rzk_lightslice_clusters", "rzk_clusters_destroy", "rzk_guide_create", "rzk_guide_get", "rzk_render_guided",
     "rzk_server_create", "rzk_server_destroy", "rzk_server_submit", "rzk_server_wait", "rzk_server_get_stats",
     "rzk_server_last_error",
+    "rzk_kdslices_build", "rzk_kdslices_destroy", "rzk_kdslices_info", "rzk_kdslices_get", "rzk_kdslices_find",
+    "rzk_kdslices_nearest", "rzk_map_vec6",
 ]
 
 
@@ -170,6 +172,13 @@ def load() -> C.CDLL:
         "rzk_vpls_create": [P, P, P, I64, C.POINTER(P)],
         "rzk_generate_vpls": [P, P, I64, C.c_uint32, I64, F, P, P, C.POINTER(I64), P],
         "rzk_vpls_destroy": [P],
+        "rzk_kdslices_build": [P, P, C.c_int32, C.c_int32, C.c_uint32, P],
+        "rzk_kdslices_destroy": [P],
+        "rzk_kdslices_info": [P, P, P],
+        "rzk_kdslices_get": [P, C.c_char_p, P, I64],
+        "rzk_kdslices_find": [P, P, C.c_int32, P],
+        "rzk_kdslices_nearest": [P, P, C.c_int32, C.c_int32, P],
+        "rzk_map_vec6": [P, P, P, P, I64, F, P],
         "rzk_gather": [P, P, P, P, P, I64, C.c_int, F, F, F, P],
         "rzk_visibility": [P, P, P, P, P, I64, F, P],
         "rzk_render_vpl": [P, P, P, P, C.c_int, F, F, F, P],
@@ -377,6 +386,65 @@ class Vpls:
             self.close()
         except Exception:
             pass
+
+
+class KdSlices:
+    """rzk_kdslices: LightSlice's slicing of 6-D gather points by a kd-tree
+    (max_slice_size points per slice at most); arrays order, slices (S x 2),
+    boxes (S x 12) and the heap arrays axis / plane / cl of the query tree"""
+
+    def __init__(self, ctx: Ctx, points, max_slice_size=256, seed=2013):
+        self.ctx, self.lib = ctx, ctx.lib
+        pts = f32(points).reshape(-1, 6)
+        h = C.c_void_p()
+        ctx.check(self.lib.rzk_kdslices_build(ctx.h, ptr(pts), len(pts), int(max_slice_size), int(seed) & 0xffffffff,
+                                              C.byref(h)), "rzk_kdslices_build")
+        self.h = h
+        self.n = len(pts)
+        S, H = C.c_int32(0), C.c_int32(0)
+        ctx.check(self.lib.rzk_kdslices_info(self.h, C.byref(S), C.byref(H)), "rzk_kdslices_info")
+        self.S, self.heap_used = S.value, H.value
+
+    def get(self, name):
+        shape, dt = {"order": ((self.n,), np.int32), "slices": ((self.S, 2), np.int32),
+                     "boxes": ((self.S, 12), np.float32), "axis": ((self.heap_used,), np.int32),
+                     "plane": ((self.heap_used,), np.float32), "cl": ((self.heap_used,), np.int32)}[name]
+        out = np.zeros(shape, dt)
+        self.ctx.check(self.lib.rzk_kdslices_get(self.h, name.encode(), ptr(out), out.nbytes), "rzk_kdslices_get")
+        return out
+
+    def find(self, q):
+        q = f32(q).reshape(-1, 6)
+        out = np.zeros(len(q), np.int32)
+        self.ctx.check(self.lib.rzk_kdslices_find(self.h, ptr(q), len(q), ptr(out)), "rzk_kdslices_find")
+        return out
+
+    def nearest(self, q, k):
+        q = f32(q).reshape(-1, 6)
+        out = np.zeros((len(q), int(k)), np.int32)
+        self.ctx.check(self.lib.rzk_kdslices_nearest(self.h, ptr(q), len(q), int(k), ptr(out)), "rzk_kdslices_nearest")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rzk_kdslices_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def map_vec6(ctx, scene, recv, mats, diagonal):
+    """rzk_map_vec6: LightSlice's 6-D point of each receiver record"""
+    recv = f32(recv).reshape(-1, RECV_FLOATS)
+    mats = i32(mats)
+    out = np.zeros((len(recv), 6), np.float32)
+    ctx.check(ctx.lib.rzk_map_vec6(ctx.h, scene.h, ptr(recv), ptr(mats), len(recv), float(diagonal), ptr(out)),
+              "rzk_map_vec6")
+    return out
 
 
 def generate_vpls(ctx, scene, num_vpls, seed=2013, max_samples=1 << 26, tmin=1e-3, count_work=False):
