@@ -304,6 +304,78 @@ def run_vplgen(args):
     print(json.dumps(out), flush=True)
 
 
+def scene_diagonal(sc):
+    """BoundingBox::diagonal of the scene box (boundingbox.h:109-111), float32"""
+    p = np.asarray(sc.positions, np.float32).reshape(-1, 3)
+    d = p.max(0) - p.min(0)
+    return float(np.sqrt(np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]), dtype=np.float32))
+
+
+def run_kdslice(args):
+    """LightSlice's slicing of the gather points by the 6-D kd-tree
+    (generate_slices, lightslice.cc:252-303) on the 512x512 room: every pixel's
+    primary hit with a material mapped by map_vec6, then the kd-tree build with
+    slices of at most 256 points.  A step is one build from points resident in
+    device memory (rzk_kdslices_build_dev).  Roofline: 36 B per point and depth
+    (its 6 coordinates for the node box, its key, its index read and written)
+    over the step time.  cpu_baseline: the oracle's build of the same points on
+    one thread."""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes
+    import oracle
+    W = H = 512
+    sc = scenes.room(W, H, seed=2013, target_tris=args.tris or 200_000)
+    ctx = rz.Ctx(0)
+    scene = rz.Scene(ctx, sc)
+    px, py = np.meshgrid(np.arange(W, dtype=np.int32), np.arange(H, dtype=np.int32))
+    _, _, recv, m, _ = scene.camera_primary(sc.camera, px.ravel(), py.ravel())
+    keep = m >= 0                         # a hit with a material (lightslice.cc:270)
+    diag = scene_diagonal(sc)
+    pts = rz.map_vec6(ctx, scene, recv[keep], m[keep], diag)
+    n = len(pts)
+    dptr = C.c_void_p()
+    ctx.check(ctx.lib.rzk_device_alloc(ctx.h, pts.nbytes, C.byref(dptr)), "rzk_device_alloc")
+    ctx.check(ctx.lib.rzk_copy_h2d(ctx.h, dptr, pts.ctypes.data_as(C.c_void_p), pts.nbytes), "rzk_copy_h2d")
+    for i in range(args.warmup):
+        rz.KdSlices(ctx, dptr.value, 256, 2013, n=n).close()
+    ms = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        kd = rz.KdSlices(ctx, dptr.value, 256, 2013, n=n)
+        ms.append((time.perf_counter() - t0) * 1e3)
+        log(f"[bench] step {i + 1}/{args.steps}: {ms[-1]:.2f} ms")
+        if i + 1 < args.steps:
+            kd.close()
+    ms_per_step = float(np.mean(ms))
+    depths, mm = 0, n
+    while mm >= 2:
+        depths, mm = depths + 1, (mm + 1) // 2
+    gbytes = 36.0 * n * depths
+    achieved = gbytes / (ms_per_step / 1e3) / 1e9
+    out = {"metric": "LightSlice kd-tree slicing, gather points/s", "value": round(n / (ms_per_step / 1e3) / 1e6, 3),
+           "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (seeded room scene)",
+           "config": {"workload": "kd-tree slicing room 200k tris 512x512 gather points, slices of 256",
+                      "points": n, "slices": kd.S, "depths": depths, "parallelism": "single GPU"},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "rzk_kdslices_build_dev (k_kd_big_*, k_kd_level_lds)",
+                        "note": "step time includes the build's own allocations and the host checks of the "
+                                "large-node rounds"}}
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        o = oracle.KdSlices(pts, 256, 2013)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / dt / 1e6, 4), "unit": "Mpoints/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle build of the same {n} points on one thread, {dt:.2f} s",
+                               "same_slices": bool(np.array_equal(kd.get("order"), o.order)
+                                                   and np.array_equal(kd.get("slices"), o.slices))}
+    kd.close()
+    ctx.lib.rzk_device_free(ctx.h, dptr)
+    print(json.dumps(out), flush=True)
+
+
 def c3_scene_and_guides(args, levels):
     """the 1M-triangle instanced scene at 1280x720 with one LightSlice guide per
     VPL count in `levels` (16x16-pixel slices, 1,000 clusters each)"""
@@ -518,7 +590,8 @@ def run_served(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided", "batched", "served", "refine", "vplgen"],
+    ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided", "batched", "served", "refine", "vplgen",
+                                                                 "kdslice"],
                     help="lightslice: the 200k-triangle LightSlice gather step (the headline line); "
                          "guided: guided path tracing of the 1M-triangle scene; batched: 256 uneven jobs")
     ap.add_argument("--jobs", type=int, default=256, help="batched workload: number of jobs")
@@ -552,6 +625,8 @@ def main():
         return run_refine(args)
     if args.workload == "vplgen":
         return run_vplgen(args)
+    if args.workload == "kdslice":
+        return run_kdslice(args)
 
     import renzoku_amd as rz
     from renzoku_amd import scenes, vplgen

@@ -191,6 +191,9 @@ int rzk_map_vec6(rzk_ctx *ctx, const rzk_scene *scene, const float *recv, const 
 typedef struct rzk_kdslices rzk_kdslices;
 int rzk_kdslices_build(rzk_ctx *ctx, const float *points, int32_t n, int32_t max_slice_size, uint32_t seed,
                        rzk_kdslices **out);
+/* the same with the points already in device memory (n x 6, on the context's stream) */
+int rzk_kdslices_build_dev(rzk_ctx *ctx, const float *d_points, int32_t n, int32_t max_slice_size, uint32_t seed,
+                           rzk_kdslices **out);
 int rzk_kdslices_destroy(rzk_kdslices *kd);
 int rzk_kdslices_info(const rzk_kdslices *kd, int32_t *out_slices, int32_t *out_heap_used);
 int rzk_kdslices_get(const rzk_kdslices *kd, const char *name, void *dst, int64_t bytes);

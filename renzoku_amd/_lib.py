@@ -39,7 +39,7 @@ EXPORTS = [
     "rzk_lightslice_clusters", "rzk_clusters_destroy", "rzk_guide_create", "rzk_guide_get", "rzk_render_guided",
     "rzk_server_create", "rzk_server_destroy", "rzk_server_submit", "rzk_server_wait", "rzk_server_get_stats",
     "rzk_server_last_error",
-    "rzk_kdslices_build", "rzk_kdslices_destroy", "rzk_kdslices_info", "rzk_kdslices_get", "rzk_kdslices_find",
+    "rzk_kdslices_build", "rzk_kdslices_build_dev", "rzk_kdslices_destroy", "rzk_kdslices_info", "rzk_kdslices_get", "rzk_kdslices_find",
     "rzk_kdslices_nearest", "rzk_map_vec6",
 ]
 
@@ -173,6 +173,7 @@ def load() -> C.CDLL:
         "rzk_generate_vpls": [P, P, I64, C.c_uint32, I64, F, P, P, C.POINTER(I64), P],
         "rzk_vpls_destroy": [P],
         "rzk_kdslices_build": [P, P, C.c_int32, C.c_int32, C.c_uint32, P],
+        "rzk_kdslices_build_dev": [P, P, C.c_int32, C.c_int32, C.c_uint32, P],
         "rzk_kdslices_destroy": [P],
         "rzk_kdslices_info": [P, P, P],
         "rzk_kdslices_get": [P, C.c_char_p, P, I64],
@@ -393,14 +394,20 @@ class KdSlices:
     (max_slice_size points per slice at most); arrays order, slices (S x 2),
     boxes (S x 12) and the heap arrays axis / plane / cl of the query tree"""
 
-    def __init__(self, ctx: Ctx, points, max_slice_size=256, seed=2013):
+    def __init__(self, ctx: Ctx, points, max_slice_size=256, seed=2013, n=None):
+        """points: host array (n x 6), or a device address (int) of n x 6 floats"""
         self.ctx, self.lib = ctx, ctx.lib
-        pts = f32(points).reshape(-1, 6)
         h = C.c_void_p()
-        ctx.check(self.lib.rzk_kdslices_build(ctx.h, ptr(pts), len(pts), int(max_slice_size), int(seed) & 0xffffffff,
-                                              C.byref(h)), "rzk_kdslices_build")
+        if isinstance(points, int):
+            self.n = int(n)
+            ctx.check(self.lib.rzk_kdslices_build_dev(ctx.h, C.c_void_p(points), self.n, int(max_slice_size),
+                                                      int(seed) & 0xffffffff, C.byref(h)), "rzk_kdslices_build_dev")
+        else:
+            pts = f32(points).reshape(-1, 6)
+            self.n = len(pts)
+            ctx.check(self.lib.rzk_kdslices_build(ctx.h, ptr(pts), self.n, int(max_slice_size),
+                                                  int(seed) & 0xffffffff, C.byref(h)), "rzk_kdslices_build")
         self.h = h
-        self.n = len(pts)
         S, H = C.c_int32(0), C.c_int32(0)
         ctx.check(self.lib.rzk_kdslices_info(self.h, C.byref(S), C.byref(H)), "rzk_kdslices_info")
         self.S, self.heap_used = S.value, H.value

@@ -143,6 +143,44 @@ __device__ int kd_block_scan(int v, int* sc, int* total) {
     return before + x - v;
 }
 
+// Selection rounds t0, t0 + 1, ... of node [s, e] on LDS positions [lo, hi]
+// (ids and keys ping-pong between A and B) until position k holds the median:
+// each round a stable 3-way partition around the round's pivot.
+template <int NT>
+__device__ void kd_select_lds(int* idA, float* keyA, int* idB, float* keyB, int lo, int hi, int k, uint32_t seed,
+                              int s, int e, int t0, int* s_sc) {
+    const int tid = (int)threadIdx.x;
+    for (int t = t0;; ++t) {
+        const int len = hi - lo + 1;
+        const float v = keyA[kd_pivot(seed, s, e, t, lo, hi)];
+        const int per = (len + NT - 1) / NT;
+        const int b0 = lo + tid * per, b1 = min(hi + 1, b0 + per);
+        int lt = 0, eq = 0;
+        for (int i = b0; i < b1; ++i) {
+            const float x = keyA[i];
+            lt += x < v;
+            eq += x == v;
+        }
+        int tot;
+        const int pre = kd_block_scan<NT>(lt | (eq << 16), s_sc, &tot);
+        const int LT = tot & 0xffff, EQ = tot >> 16;
+        int pl = lo + (pre & 0xffff), pe = lo + LT + (pre >> 16);
+        int pg = lo + LT + EQ + ((b0 - lo) - (pre & 0xffff) - (pre >> 16));
+        for (int i = b0; i < b1; ++i) {
+            const float x = keyA[i];
+            const int pos = x < v ? pl++ : (x == v ? pe++ : pg++);
+            idB[pos] = idA[i];
+            keyB[pos] = x;
+        }
+        __syncthreads();
+        for (int i = lo + tid; i <= hi; i += NT) { idA[i] = idB[i]; keyA[i] = keyB[i]; }
+        __syncthreads();
+        if (k < lo + LT) hi = lo + LT - 1;
+        else if (k < lo + LT + EQ) break;
+        else lo = lo + LT + EQ;
+    }
+}
+
 // ---- depths with nodes of at most kKdLds points: one block per node --------
 // LDS: ids and keys, twice (ping-pong), cap entries each
 template <int NT>
@@ -175,36 +213,7 @@ __global__ __launch_bounds__(NT) void k_kd_level_lds(const float* __restrict__ P
     for (int i = tid; i < m; i += NT) keyA[i] = P[(int64_t)ax * n + idA[i]];
     __syncthreads();
     const int k = (m - 1) >> 1;              // (s + e) / 2 - s
-    int lo = 0, hi = m - 1;
-    for (int t = 0;; ++t) {
-        const int len = hi - lo + 1;
-        const float v = keyA[kd_pivot(seed, s, e, t, lo, hi)];
-        const int per = (len + NT - 1) / NT;
-        const int b0 = lo + tid * per, b1 = min(hi + 1, b0 + per);
-        int lt = 0, eq = 0;
-        for (int i = b0; i < b1; ++i) {
-            const float x = keyA[i];
-            lt += x < v;
-            eq += x == v;
-        }
-        int tot;
-        const int pre = kd_block_scan<NT>(lt | (eq << 16), s_sc, &tot);
-        const int LT = tot & 0xffff, EQ = tot >> 16;
-        int pl = lo + (pre & 0xffff), pe = lo + LT + (pre >> 16);
-        int pg = lo + LT + EQ + ((b0 - lo) - (pre & 0xffff) - (pre >> 16));
-        for (int i = b0; i < b1; ++i) {
-            const float x = keyA[i];
-            const int pos = x < v ? pl++ : (x == v ? pe++ : pg++);
-            idB[pos] = idA[i];
-            keyB[pos] = x;
-        }
-        __syncthreads();
-        for (int i = lo + tid; i <= hi; i += NT) { idA[i] = idB[i]; keyA[i] = keyB[i]; }
-        __syncthreads();
-        if (k < lo + LT) hi = lo + LT - 1;
-        else if (k < lo + LT + EQ) break;
-        else lo = lo + LT + EQ;
-    }
+    kd_select_lds<NT>(idA, keyA, idB, keyB, 0, m - 1, k, seed, s, e, 0, s_sc);
     if (tid == 0 && heap < H) { axis_out[heap] = ax; plane_out[heap] = keyA[k]; }
     for (int i = tid; i < m; i += NT) ord[s + i] = idA[i];
 }
@@ -386,6 +395,33 @@ __global__ __launch_bounds__(kKdBigBlock) void k_kd_big_scatter(KdBig g, int r) 
         }
     }
     if (bk == 0 && tid == 0) *nxt = ns;
+}
+
+// the rest of a depth's selection once every active range fits in LDS: one
+// block per node continues from round r in LDS and writes the range's final
+// order to ord and the plane
+__global__ __launch_bounds__(kKdBigBlock) void k_kd_big_finish(KdBig g, int r) {
+    extern __shared__ int s_fin[];
+    __shared__ int s_sc[kKdBigBlock / 64 + 1];
+    const int j = (int)blockIdx.x, tid = (int)threadIdx.x;
+    int s, e;
+    if (!kd_range(g.n, g.d, j, s, e) || s >= e) return;
+    const int4 S = g.st[(r & 1) * g.N + j];
+    if (S.z) return;
+    const int len = S.y - S.x + 1;
+    int* idA = s_fin;
+    float* keyA = reinterpret_cast<float*>(s_fin + kKdLds);
+    int* idB = s_fin + 2 * kKdLds;
+    float* keyB = reinterpret_cast<float*>(s_fin + 3 * kKdLds);
+    const int* id = (r & 1) ? g.idB : g.idA;
+    const float* key = (r & 1) ? g.keyB : g.keyA;
+    for (int i = tid; i < len; i += kKdBigBlock) { idA[i] = id[S.x + i]; keyA[i] = key[S.x + i]; }
+    __syncthreads();
+    const int k = ((s + e) >> 1) - S.x;
+    kd_select_lds<kKdBigBlock>(idA, keyA, idB, keyB, 0, len - 1, k, g.seed, s, e, r, s_sc);
+    for (int i = tid; i < len; i += kKdBigBlock) g.ord[S.x + i] = idA[i];
+    const int heap = (1 << g.d) - 1 + j;
+    if (tid == 0 && heap < g.H) g.plane_out[heap] = keyA[k];
 }
 
 // ---- queries ------------------------------------------------------------------
@@ -657,6 +693,9 @@ void kd_build_device(rzk_kdslices* kd) {
         axis_lvl.alloc(Nmax);
     }
     std::vector<int4> h_st;
+    if (big_depths > 0)
+        RZK_HIP(hipFuncSetAttribute((const void*)k_kd_big_finish, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(int) * 4 * kKdLds)));
     for (int d = 0; d < (int)maxm.size(); ++d) {
         const int m = maxm[d];
         if (m > kKdLds) {
@@ -671,22 +710,33 @@ void kd_build_device(rzk_kdslices* kd) {
             hipLaunchKernelGGL(k_kd_big_axis, dim3(g.N), dim3(64), 0, st, g);
             hipLaunchKernelGGL(k_kd_big_keys, grid, dim3(kKdBigBlock), 0, st, g);
             RZK_HIP(hipGetLastError());
-            // every round takes the pivot out of the active range, so at most m rounds
+            // rounds over HBM until every active range fits in LDS, then one
+            // launch finishes the selections there; every round takes the pivot
+            // out of the active range, so at most m rounds
             h_st.resize(g.N);
-            bool all_done = false;
-            for (int r = 0; r < m && !all_done; ++r) {
+            bool settled = false;
+            for (int r = 0; r < m && !settled; ++r) {
                 hipLaunchKernelGGL(k_kd_big_count, grid, dim3(kKdBigBlock), 0, st, g, r);
                 hipLaunchKernelGGL(k_kd_big_scatter, grid, dim3(kKdBigBlock), 0, st, g, r);
                 RZK_HIP(hipGetLastError());
-                if ((r & 7) == 7 || r + 1 == m) {
+                if ((r & 1) == 1 || r + 1 == m) {
                     RZK_HIP(hipMemcpyAsync(h_st.data(), g.st + ((r + 1) & 1) * g.N, sizeof(int4) * g.N,
                                            hipMemcpyDeviceToHost, st));
                     RZK_HIP(hipStreamSynchronize(st));
-                    all_done = true;
-                    for (int q = 0; q < g.N; ++q) all_done = all_done && h_st[q].z != 0;
+                    bool all_done = true, fits = true;
+                    for (int q = 0; q < g.N; ++q) {
+                        all_done = all_done && h_st[q].z != 0;
+                        fits = fits && (h_st[q].z != 0 || h_st[q].y - h_st[q].x + 1 <= kKdLds);
+                    }
+                    if (!all_done && fits) {
+                        hipLaunchKernelGGL(k_kd_big_finish, dim3(g.N), dim3(kKdBigBlock), sizeof(int) * 4 * kKdLds,
+                                           st, g, r + 1);
+                        RZK_HIP(hipGetLastError());
+                    }
+                    settled = all_done || fits;
                 }
             }
-            RZK_CHECK(all_done, RZK_E_STATE, "kd-tree selection did not settle");
+            RZK_CHECK(settled, RZK_E_STATE, "kd-tree selection did not settle");
         } else {
             const size_t lds = sizeof(int) * 4 * (size_t)m;
             const unsigned grid = 1u << d;
@@ -707,8 +757,8 @@ void kd_build_device(rzk_kdslices* kd) {
 
 extern "C" {
 
-int rzk_kdslices_build(rzk_ctx* ctx, const float* points, int32_t n, int32_t max_slice_size, uint32_t seed,
-                       rzk_kdslices** out) {
+static int kd_build_entry(rzk_ctx* ctx, const float* points, bool on_device, int32_t n, int32_t max_slice_size,
+                          uint32_t seed, rzk_kdslices** out) {
     if (!ctx || !out) return RZK_E_INVALID;
     *out = nullptr;
     return kd_guard(ctx, [&] {
@@ -723,17 +773,30 @@ int rzk_kdslices_build(rzk_ctx* ctx, const float* points, int32_t n, int32_t max
         kd->P.alloc((size_t)kKdDim * n);
         if (n > 0) {
             KdBuf<float> aos;
-            aos.alloc((size_t)kKdDim * n);
-            RZK_HIP(hipMemcpyAsync(aos.p, points, sizeof(float) * kKdDim * (size_t)n, hipMemcpyHostToDevice,
-                                   ctx->stream));
-            hipLaunchKernelGGL(k_kd_planes, dim3(kd_blocks(n, 256)), dim3(256), 0, ctx->stream, aos.p, n, kd->P.p);
+            const float* src = points;
+            if (!on_device) {
+                aos.alloc((size_t)kKdDim * n);
+                RZK_HIP(hipMemcpyAsync(aos.p, points, sizeof(float) * kKdDim * (size_t)n, hipMemcpyHostToDevice,
+                                       ctx->stream));
+                src = aos.p;
+            }
+            hipLaunchKernelGGL(k_kd_planes, dim3(kd_blocks(n, 256)), dim3(256), 0, ctx->stream, src, n, kd->P.p);
             RZK_HIP(hipGetLastError());
-            RZK_HIP(hipStreamSynchronize(ctx->stream));
         }
         kd_build_device(kd.get());
         ctx_retain(ctx);
         *out = kd.release();
     });
+}
+
+int rzk_kdslices_build(rzk_ctx* ctx, const float* points, int32_t n, int32_t max_slice_size, uint32_t seed,
+                       rzk_kdslices** out) {
+    return kd_build_entry(ctx, points, false, n, max_slice_size, seed, out);
+}
+
+int rzk_kdslices_build_dev(rzk_ctx* ctx, const float* d_points, int32_t n, int32_t max_slice_size, uint32_t seed,
+                           rzk_kdslices** out) {
+    return kd_build_entry(ctx, d_points, true, n, max_slice_size, seed, out);
 }
 
 int rzk_kdslices_destroy(rzk_kdslices* kd) {

@@ -87,3 +87,18 @@ def test_map_vec6_matches_oracle(ctx):
     got = rz.map_vec6(ctx, scene, recv, mats, diag)
     want = oracle.map_vec6(recv, mats, osc.materials, diag)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_kd_build_from_device_points(ctx):
+    import ctypes as C
+    import renzoku_amd as rz
+    P = _points(20_000, 9, True)
+    d = C.c_void_p()
+    ctx.check(ctx.lib.rzk_device_alloc(ctx.h, P.nbytes, C.byref(d)), "alloc")
+    try:
+        ctx.check(ctx.lib.rzk_copy_h2d(ctx.h, d, P.ctypes.data_as(C.c_void_p), P.nbytes), "h2d")
+        g = rz.KdSlices(ctx, d.value, 128, 9, n=len(P))
+        _same_tree(g, oracle.KdSlices(P, 128, 9))
+        g.close()
+    finally:
+        ctx.lib.rzk_device_free(ctx.h, d)
