@@ -274,6 +274,12 @@ typedef struct rzk_lightslice_params {
     int32_t max_slice_clusters;     /* per-slice refinement bound (lightslice.cc:29, :869):
                                        clusters are split per slice until a slice holds this
                                        many; 0 or <= the kept clusters: no refinement */
+    int32_t kd_max_slice_size;      /* 0: slices are pixel tiles; >= 3: the reference's
+                                       slices (generate_slices, lightslice.cc:252-303): a 6-D
+                                       kd-tree over the gather points, slices of at most this
+                                       many points (max_slice_size = 256, lightslice.cc:27),
+                                       a random representative per slice and its nearest
+                                       slices as neighbours; tile is then unused */
 } rzk_lightslice_params;
 
 typedef struct rzk_lightslice_info {

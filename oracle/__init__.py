@@ -353,12 +353,15 @@ def pdf_sample(pdf, cdf, u):
     return int(k[0]), float(pk[0])
 
 
-def cluster_by_sampling(R, num_clusters, seed, nthreads=1):
-    """ClusterBuilder::cluster_by_sampling on a freshly seeded MT stream."""
+def cluster_by_sampling(R, num_clusters, seed, nthreads=1, skip=0):
+    """ClusterBuilder::cluster_by_sampling on a freshly seeded MT stream
+    (after `skip` draws)."""
     R = _f(R)
     cols, rows = R.shape[0], R.shape[1]
     L = lib()
     L.orc_mt_seed32(seed)
+    for _ in range(int(skip)):
+        L.orc_mt_rand()
     centers, cw = np.zeros(num_clusters, np.int32), np.zeros(num_clusters, np.float32)
     assign = np.zeros(cols, np.int32)
     reps, rw, kept = (np.zeros(num_clusters, np.int32), np.zeros(num_clusters, np.float32),
@@ -390,15 +393,18 @@ def slice_cost(R, nb_rows, idx):
     return float(lib().orc_slice_cost(R.shape[1], _p(R), len(nb_rows), _p(nb_rows), len(idx), _p(idx)))
 
 
-def cluster_and_refine(R, num_clusters, max_slice_clusters, nb, seed, nthreads=1, with_perm=False, slices=None):
+def cluster_and_refine(R, num_clusters, max_slice_clusters, nb, seed, nthreads=1, with_perm=False, slices=None,
+                       skip=0):
     """cluster_by_sampling then the per-slice refinement on one MT stream seeded
-    with `seed` (lightslice.cc:757-902).  R: (V, S, 3); nb: (S, nnb) rows;
-    slices: refine only the first `slices` slices (default: all)."""
+    with `seed` (lightslice.cc:757-902), after `skip` draws.  R: (V, S, 3);
+    nb: (S, nnb) rows; slices: refine only the first `slices` slices (default: all)."""
     R = _f(R)
     V, S = R.shape[0], R.shape[1]
     nb = _i(nb)
     L = lib()
     L.orc_mt_seed32(seed)
+    for _ in range(int(skip)):
+        L.orc_mt_rand()
     centers, cw = np.zeros(num_clusters, np.int32), np.zeros(num_clusters, np.float32)
     assign = np.zeros(V, np.int32)
     reps, rw, kept = (np.zeros(num_clusters, np.int32), np.zeros(num_clusters, np.float32),
@@ -523,3 +529,19 @@ class KdSlices:
         lib().orc_kd_nearest(_p(self.axis), _p(self.plane), _p(self.cl), self.heap_used, _p(self.boxes), self.S,
                              _p(q), len(q), int(k), _p(out))
         return out
+
+
+def kd_representatives(slices, n_points, seed):
+    """LightSlice::generate_slices' representatives (lightslice.cc:293-301) on a
+    freshly seeded MT stream: per slice start + rd() * size in float, truncated
+    (kept inside the point array), then the colour's draw.  Returns the kd
+    positions and the number of draws taken (2 per slice)."""
+    L = lib()
+    L.orc_mt_seed32(seed)
+    reps = np.zeros(len(slices), np.int32)
+    for s, (a, b) in enumerate(np.asarray(slices)):
+        u = np.float32(L.orc_mt_rand())
+        r = int(np.float32(np.float32(a) + np.float32(u * np.float32(b - a + 1))))
+        reps[s] = min(r, n_points - 1)
+        L.orc_mt_rand()
+    return reps, 2 * len(slices)

@@ -70,6 +70,7 @@ class LightSliceParams(C.Structure):
         ("num_neighbor_slices", C.c_int32), ("map_resolution", C.c_int32), ("map_epsilon", C.c_float),
         ("clamp_mode", C.c_int32), ("clamp", C.c_float), ("incident_clamp", C.c_float), ("tmin", C.c_float),
         ("n_emitted", C.c_float), ("seed", C.c_uint32), ("map_gate", C.c_int32), ("max_slice_clusters", C.c_int32),
+        ("kd_max_slice_size", C.c_int32),
     ]
 
 
@@ -542,7 +543,8 @@ class LightSlice:
         p = LightSliceParams()
         defaults = dict(width=512, height=512, tile=32, num_clusters=300, num_neighbor_slices=16,
                         map_resolution=32, map_epsilon=1e-3, clamp_mode=CLAMP_FORM_FACTOR, clamp=0.01,
-                        incident_clamp=0.01, tmin=1e-3, n_emitted=1.0, seed=2013, map_gate=0, max_slice_clusters=0)
+                        incident_clamp=0.01, tmin=1e-3, n_emitted=1.0, seed=2013, map_gate=0, max_slice_clusters=0,
+                        kd_max_slice_size=0)
         defaults.update(kw)
         for k, v in defaults.items():
             setattr(p, k, v)

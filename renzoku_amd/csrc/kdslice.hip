@@ -35,6 +35,7 @@
 #include <string>
 #include <vector>
 
+#include "kdslice.h"
 #include "rzk_device.h"
 #include "std_order.h"
 
@@ -464,10 +465,23 @@ __global__ void k_kd_find(KdQuery t, const float* __restrict__ q, int m, int* __
     out[i] = kd_find(t, p);
 }
 
+// the slice of every point i of the planes, written to out[dst[i]]
+__global__ void k_kd_find_points(KdQuery t, const float* __restrict__ P, int n, const int* __restrict__ dst,
+                                 int* __restrict__ out) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    float p[kKdDim];
+    for (int a = 0; a < kKdDim; ++a) p[a] = P[(int64_t)a * n + i];
+    out[dst[i]] = kd_find(t, p);
+}
+
 // find_nearest_cluster_rec with an explicit stack of (second child, plane
 // distance); the heap is std::priority_queue's (push_heap / pop_heap moves)
-__global__ __launch_bounds__(64) void k_kd_nearest(KdQuery t, const float* __restrict__ q, int m, int k,
-                                                   int* __restrict__ out) {
+// queries: q (m x 6), or when pos is set the points at kd positions pos[i]
+// (planes P of n floats, ord = the point at each kd position)
+__global__ __launch_bounds__(64) void k_kd_nearest(KdQuery t, const float* __restrict__ q, const float* __restrict__ P,
+                                                   int n, const int* __restrict__ ord, const int* __restrict__ pos,
+                                                   int m, int k, int* __restrict__ out) {
     __shared__ float s_hk[64 * kKdMaxK];
     __shared__ int s_hv[64 * kKdMaxK];
     __shared__ int s_sn[64 * kKdStack];
@@ -479,7 +493,12 @@ __global__ __launch_bounds__(64) void k_kd_nearest(KdQuery t, const float* __res
     int* sn = s_sn + threadIdx.x * kKdStack;
     float* sd = s_sd + threadIdx.x * kKdStack;
     float p[kKdDim];
-    for (int a = 0; a < kKdDim; ++a) p[a] = q[(int64_t)kKdDim * i + a];
+    if (pos) {
+        const int id = ord[pos[i]];
+        for (int a = 0; a < kKdDim; ++a) p[a] = P[(int64_t)a * n + id];
+    } else {
+        for (int a = 0; a < kKdDim; ++a) p[a] = q[(int64_t)kKdDim * i + a];
+    }
     int hn = 0, sp = 0, node = 0;
     bool ok = t.H > 0;
     while (ok) {
@@ -522,11 +541,12 @@ __global__ __launch_bounds__(64) void k_kd_nearest(KdQuery t, const float* __res
 }
 
 // map_vec6 of receiver records (one lane per record), into 6 planes of n floats
-__global__ void k_map_vec6(const float* __restrict__ recv, const int* __restrict__ rmat,
+__global__ void k_map_vec6(const float* __restrict__ recv, const int* __restrict__ rmat, const int* __restrict__ idx,
                            const float* __restrict__ mat, int64_t n, float diagonal, float* __restrict__ out,
                            int64_t pitch) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const int64_t i = idx ? (int64_t)idx[o] : o;           // record of output point o
     const float* rc = recv + RZK_RECV_FLOATS * i;
     const float weight = 1.0f / 16.0f * diagonal;
     const float nx = rc[6], ny = rc[7], nz = rc[8], wx = rc[9], wy = rc[10], wz = rc[11];
@@ -550,12 +570,12 @@ __global__ void k_map_vec6(const float* __restrict__ recv, const int* __restrict
         dy = (d2 * ny - wy) * gs;
         dz = (d2 * nz - wz) * gs;
     }
-    out[i] = rc[0];
-    out[pitch + i] = rc[1];
-    out[2 * pitch + i] = rc[2];
-    out[3 * pitch + i] = dx * weight;
-    out[4 * pitch + i] = dy * weight;
-    out[5 * pitch + i] = dz * weight;
+    out[o] = rc[0];
+    out[pitch + o] = rc[1];
+    out[2 * pitch + o] = rc[2];
+    out[3 * pitch + o] = dx * weight;
+    out[4 * pitch + o] = dy * weight;
+    out[5 * pitch + o] = dz * weight;
 }
 
 // AoS (n x 6) to 6 planes
@@ -579,27 +599,6 @@ __global__ void k_kd_fill(T* __restrict__ p, int n, T v) {
 }  // namespace rzk
 
 using namespace rzk;
-
-// device buffer owned by a slicing
-template <typename T>
-struct KdBuf {
-    T* p = nullptr;
-    void alloc(size_t n) {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        RZK_HIP(hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1)));
-    }
-    ~KdBuf() { if (p) (void)hipFree(p); }
-};
-
-struct rzk_kdslices {
-    rzk_ctx* ctx = nullptr;
-    int n = 0, max_size = 0, S = 0, H = 0;
-    uint32_t seed = 0;
-    KdBuf<float> P, boxes, plane;
-    KdBuf<int> ord, axis, cl, slices;
-    std::vector<int> h_slices;
-};
 
 namespace {
 
@@ -755,6 +754,33 @@ void kd_build_device(rzk_kdslices* kd) {
     RZK_HIP(hipStreamSynchronize(st));
 }
 
+static KdQuery kd_query(const rzk_kdslices* kd) {
+    return KdQuery{kd->axis.p, kd->plane.p, kd->cl.p, kd->H, kd->boxes.p};
+}
+
+void kd_launch_map_vec6(rzk_ctx* ctx, const float* d_mat, const float* d_recv, const int* d_rmat, const int* d_idx,
+                        int np, float diagonal, float* d_planes) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_map_vec6, dim3(kd_blocks(np, 256)), dim3(256), 0, ctx->stream, d_recv, d_rmat, d_idx, d_mat,
+                       (int64_t)np, diagonal, d_planes, (int64_t)np);
+    RZK_HIP(hipGetLastError());
+}
+
+void kd_launch_nearest_at(const rzk_kdslices* kd, const int* d_pos, int m, int k, int* d_out) {
+    RZK_CHECK(k >= 1 && k <= kKdMaxK, RZK_E_INVALID, "k must be in 1 .. " + std::to_string(kKdMaxK));
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_kd_nearest, dim3(kd_blocks(m, 64)), dim3(64), 0, kd->ctx->stream, kd_query(kd),
+                       (const float*)nullptr, kd->P.p, kd->n, kd->ord.p, d_pos, m, k, d_out);
+    RZK_HIP(hipGetLastError());
+}
+
+void kd_launch_find_points(const rzk_kdslices* kd, const int* d_dst, int* d_out) {
+    if (kd->n <= 0) return;
+    hipLaunchKernelGGL(k_kd_find_points, dim3(kd_blocks(kd->n, 256)), dim3(256), 0, kd->ctx->stream, kd_query(kd),
+                       kd->P.p, kd->n, d_dst, d_out);
+    RZK_HIP(hipGetLastError());
+}
+
 extern "C" {
 
 static int kd_build_entry(rzk_ctx* ctx, const float* points, bool on_device, int32_t n, int32_t max_slice_size,
@@ -838,10 +864,6 @@ int rzk_kdslices_get(const rzk_kdslices* kd, const char* name, void* dst, int64_
     });
 }
 
-static KdQuery kd_query(const rzk_kdslices* kd) {
-    return KdQuery{kd->axis.p, kd->plane.p, kd->cl.p, kd->H, kd->boxes.p};
-}
-
 int rzk_kdslices_find(const rzk_kdslices* kd, const float* queries, int32_t m, int32_t* out) {
     if (!kd) return RZK_E_INVALID;
     rzk_ctx* ctx = kd->ctx;
@@ -872,7 +894,8 @@ int rzk_kdslices_nearest(const rzk_kdslices* kd, const float* queries, int32_t m
         q.alloc((size_t)kKdDim * m);
         o.alloc((size_t)k * m);
         RZK_HIP(hipMemcpyAsync(q.p, queries, sizeof(float) * kKdDim * (size_t)m, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_kd_nearest, dim3(kd_blocks(m, 64)), dim3(64), 0, ctx->stream, kd_query(kd), q.p, m, k, o.p);
+        hipLaunchKernelGGL(k_kd_nearest, dim3(kd_blocks(m, 64)), dim3(64), 0, ctx->stream, kd_query(kd), q.p,
+                           (const float*)nullptr, 0, (const int*)nullptr, (const int*)nullptr, m, k, o.p);
         RZK_HIP(hipGetLastError());
         RZK_HIP(hipMemcpyAsync(out, o.p, sizeof(int) * (size_t)k * m, hipMemcpyDeviceToHost, ctx->stream));
         RZK_HIP(hipStreamSynchronize(ctx->stream));
@@ -896,8 +919,8 @@ int rzk_map_vec6(rzk_ctx* ctx, const rzk_scene* scene, const float* recv, const 
         RZK_HIP(hipMemcpyAsync(r.p, recv, sizeof(float) * RZK_RECV_FLOATS * (size_t)n, hipMemcpyHostToDevice,
                                ctx->stream));
         RZK_HIP(hipMemcpyAsync(mm.p, mat, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_map_vec6, dim3(kd_blocks(n, 256)), dim3(256), 0, ctx->stream, r.p, mm.p, sv.mat, n, diagonal,
-                           o.p, n);
+        hipLaunchKernelGGL(k_map_vec6, dim3(kd_blocks(n, 256)), dim3(256), 0, ctx->stream, r.p, mm.p, (const int*)nullptr,
+                           sv.mat, n, diagonal, o.p, n);
         RZK_HIP(hipGetLastError());
         std::vector<float> planes((size_t)kKdDim * n);
         RZK_HIP(hipMemcpyAsync(planes.data(), o.p, sizeof(float) * kKdDim * (size_t)n, hipMemcpyDeviceToHost,

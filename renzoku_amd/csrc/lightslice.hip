@@ -12,6 +12,7 @@
 #include <memory>
 
 #include "host_sampling.h"
+#include "kdslice.h"
 #include "pathtrace.h"
 #include "rzk_device.h"
 
@@ -134,6 +135,11 @@ __global__ __launch_bounds__(64) void k_slice_pdf_norm(int rows, SliceReps sr, c
         if (i < nreps) c[i] = i == nreps - 1 ? 1.0f : mine;
         __syncthreads();
     }
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // tile-centre pixel of every slice (the row of the reduced matrix)
@@ -301,9 +307,19 @@ struct rzk_lightslice {
     std::vector<int> h_scount;
     int ld = 0;                        // 0: every slice gathers the kept clusters
     int cw = 0;                        // row length of the slice_* arrays
+    // kd-tree slices: the gather points (pixels with a material, pixel order),
+    // the tree, each slice's representative (kd position), each pixel's slice
+    bool kd = false;
+    std::unique_ptr<rzk_kdslices> kdt;
+    DBuf<int> gp_pix, rep_pos, pix_slice;
+    std::vector<int> h_gp_pix, h_row_pix, h_pix_slice;
     SliceReps slice_reps() const {
-        if (ld) return SliceReps{sreps.p, srep_w.p, scount.p, ld, ld, T, tx};
-        return SliceReps{reps.p, rep_w.p, nullptr, (int)h_reps.size(), 0, T, tx};
+        const int* ps = kd ? pix_slice.p : nullptr;
+        if (ld) return SliceReps{sreps.p, srep_w.p, scount.p, ld, ld, T, tx, ps};
+        return SliceReps{reps.p, rep_w.p, nullptr, (int)h_reps.size(), 0, T, tx, ps};
+    }
+    int slice_of_pixel(int64_t i) const {
+        return kd ? h_pix_slice[i] : (int)(((i / W) / T) * tx + (i % W) / T);
     }
 };
 
@@ -322,6 +338,92 @@ static int ls_guard(rzk_ctx* ctx, F&& f) {
         if (ctx) ctx->last_error = e.what();
         return RZK_E_INVALID;
     }
+}
+
+// buffers sized by the slice count S
+static void ls_alloc_slices(rzk_lightslice* ls) {
+    const rzk_lightslice_params& p = ls->prm;
+    const size_t S = (size_t)ls->S, V = (size_t)ls->V, C = (size_t)p.num_clusters, res = (size_t)p.map_resolution;
+    const size_t CW = std::max<size_t>(C, (size_t)p.max_slice_clusters);
+    ls->row_pix.alloc(S); ls->rows_mat.alloc(S); ls->rows_recv.alloc(RZK_RECV_FLOATS * S);
+    ls->R.alloc(3 * S * V); ls->Vm.alloc(S * V); ls->u.alloc(S); ls->mT.alloc(S * V); ls->K.alloc(3 * S * C);
+    ls->nb.alloc(S * p.num_neighbor_slices);
+    ls->upper.alloc(S * CW); ls->slice_wi.alloc(3 * S * CW); ls->slice_w.alloc(S * CW);
+    ls->slice_pdf.alloc(S * CW); ls->slice_cdf.alloc(S * CW);
+    if (p.max_slice_clusters > 0) {
+        ls->sreps.alloc(S * p.max_slice_clusters); ls->srep_w.alloc(S * p.max_slice_clusters); ls->scount.alloc(S);
+    }
+    ls->maps.alloc(S * res * res); ls->col_cdf.alloc(S * res * res);
+    ls->row_pdf.alloc(S * res); ls->row_cdf.alloc(S * res); ls->totals.alloc(S);
+}
+
+// The reference's slices (generate_slices, lightslice.cc:252-303): gather
+// points = pixels whose primary hit has a material, in pixel order; the 6-D
+// kd-tree over their map_vec6 points; per slice a representative point drawn
+// from the MT stream (start + rd() * size, then the colour's draw); rows =
+// the representatives' pixels; neighbours = find_nearest_cluster of the
+// representative (furthest first); each pixel's slice = find_cluster of its
+// point.  Uses the primary hits already in ls->recv / ls->mat.
+static void ls_kd_slices(rzk_lightslice* ls, Mt19937& rd) {
+    rzk_ctx* ctx = ls->ctx;
+    hipStream_t st = ctx->stream;
+    const rzk_lightslice_params& p = ls->prm;
+    const int64_t n = (int64_t)ls->W * ls->H;
+    std::vector<int> hm((size_t)n);
+    RZK_HIP(hipMemcpyAsync(hm.data(), ls->mat.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    ls_sync(ctx);
+    ls->h_gp_pix.clear();
+    for (int64_t i = 0; i < n; ++i)
+        if (hm[i] >= 0) ls->h_gp_pix.push_back((int)i);
+    const int np = (int)ls->h_gp_pix.size();
+    RZK_CHECK(np >= 2, RZK_E_STATE, "fewer than two gather points");
+    RZK_HIP(hipMemcpyAsync(ls->gp_pix.p, ls->h_gp_pix.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
+    // BoundingBox::diagonal of the scene box (boundingbox.h:109-111)
+    const float* rb = ls->scene->bvh.root_box;
+    const float dx = rb[3] - rb[0], dy = rb[4] - rb[1], dz = rb[5] - rb[2];
+    const float diag = sqrtf(dx * dx + dy * dy + dz * dz);
+    ls->kdt.reset(new rzk_kdslices());
+    rzk_kdslices* kd = ls->kdt.get();
+    kd->ctx = ctx;
+    kd->n = np;
+    kd->max_size = p.kd_max_slice_size;
+    kd->seed = p.seed;
+    kd->P.alloc((size_t)6 * np);
+    SceneView sv = scene_view(ls->scene);
+    kd_launch_map_vec6(ctx, sv.mat, ls->recv.p, ls->mat.p, ls->gp_pix.p, np, diag, kd->P.p);
+    kd_build_device(kd);
+    const int S = kd->S;
+    RZK_CHECK(S >= p.num_neighbor_slices, RZK_E_STATE,
+              "fewer kd-tree slices (" + std::to_string(S) + ") than neighbour slices");
+    if (S != ls->S || !ls->row_pix.p) {
+        ls->S = S;
+        ls_alloc_slices(ls);
+    }
+    std::vector<int> order((size_t)np);
+    RZK_HIP(hipMemcpyAsync(order.data(), kd->ord.p, sizeof(int) * np, hipMemcpyDeviceToHost, st));
+    ls_sync(ctx);
+    std::vector<int> rpos(S);
+    ls->h_row_pix.resize(S);
+    for (int s = 0; s < S; ++s) {
+        const int a = kd->h_slices[2 * s], b = kd->h_slices[2 * s + 1];
+        const float u = rd.next_float();
+        int r = (int)((float)a + u * (float)(b - a + 1));
+        r = std::min(r, np - 1);          // float rounding can give b + 1; the last slice stays in the array
+        (void)rd.next_float();            // Rgb::from_hsv(rd() * 360, ...) colour
+        rpos[s] = r;
+        ls->h_row_pix[s] = ls->h_gp_pix[order[r]];
+    }
+    ls->rep_pos.alloc(S);
+    RZK_HIP(hipMemcpyAsync(ls->rep_pos.p, rpos.data(), sizeof(int) * S, hipMemcpyHostToDevice, st));
+    RZK_HIP(hipMemcpyAsync(ls->row_pix.p, ls->h_row_pix.data(), sizeof(int) * S, hipMemcpyHostToDevice, st));
+    kd_launch_nearest_at(kd, ls->rep_pos.p, S, p.num_neighbor_slices, ls->nb.p);
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ls->pix_slice.p, n, -1);
+    kd_launch_find_points(kd, ls->gp_pix.p, ls->pix_slice.p);
+    RZK_HIP(hipGetLastError());
+    ls->h_pix_slice.resize((size_t)n);
+    RZK_HIP(hipMemcpyAsync(ls->h_pix_slice.data(), ls->pix_slice.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    ls_sync(ctx);
+    ls->info.num_slices = S;
 }
 
 // rows = pixel-tile slice representatives (the tile centre)
@@ -643,7 +745,10 @@ int rzk_lightslice_create(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, c
     return ls_guard(ctx, [&] {
         RZK_CHECK(ctx && s && v && p && out, RZK_E_INVALID, "null argument");
         *out = nullptr;
-        RZK_CHECK(p->width > 0 && p->height > 0 && p->tile > 0, RZK_E_INVALID, "bad image or tile size");
+        RZK_CHECK(p->kd_max_slice_size == 0 || p->kd_max_slice_size >= 3, RZK_E_INVALID,
+                  "kd_max_slice_size must be 0 (pixel tiles) or at least 3");
+        RZK_CHECK(p->width > 0 && p->height > 0 && (p->tile > 0 || p->kd_max_slice_size > 0), RZK_E_INVALID,
+                  "bad image or tile size");
         RZK_CHECK(p->num_clusters > 0 && p->num_neighbor_slices > 0 && p->map_resolution > 0, RZK_E_INVALID,
                   "bad clustering parameters");
         RZK_CHECK(v->n > 0, RZK_E_INVALID, "There are no VPLs to cluster");
@@ -651,41 +756,41 @@ int rzk_lightslice_create(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, c
         RZK_CHECK(s->bvh.ntri > 0, RZK_E_INVALID, "empty scene");
         std::unique_ptr<rzk_lightslice> ls(new rzk_lightslice());
         ls->ctx = ctx; ls->scene = s; ls->vpls = v; ls->prm = *p;
-        ls->W = p->width; ls->H = p->height; ls->T = p->tile;
+        ls->W = p->width; ls->H = p->height;
+        ls->kd = p->kd_max_slice_size > 0;
+        ls->T = ls->kd ? std::max(1, p->tile) : p->tile;
         ls->tx = (ls->W + ls->T - 1) / ls->T; ls->ty = (ls->H + ls->T - 1) / ls->T;
-        ls->S = ls->tx * ls->ty;
+        ls->S = ls->kd ? 0 : ls->tx * ls->ty;           // kd-tree slices: known after initialize
         ls->V = v->n;
-        RZK_CHECK(p->num_neighbor_slices <= ls->S, RZK_E_INVALID, "more neighbour slices than slices");
+        RZK_CHECK(ls->kd || p->num_neighbor_slices <= ls->S, RZK_E_INVALID, "more neighbour slices than slices");
+        RZK_CHECK(p->num_neighbor_slices <= 32 || !ls->kd, RZK_E_INVALID,
+                  "at most 32 neighbour slices with kd-tree slices");
         int64_t n = (int64_t)ls->W * ls->H;
-        int S = ls->S, C = p->num_clusters, res = p->map_resolution;
+        int S = ls->S, C = p->num_clusters;
         int64_t V = ls->V;
         RZK_CHECK(p->max_slice_clusters >= 0 && (int64_t)p->max_slice_clusters <= V, RZK_E_INVALID,
                   "max_slice_clusters must lie in [0, number of VPLs]");
         const int CW = std::max(C, p->max_slice_clusters);      // widest per-slice cluster list
         ls->rays.alloc(6 * n); ls->recv.alloc(RZK_RECV_FLOATS * n); ls->image.alloc(3 * n);
         ls->hits.alloc(n); ls->mat.alloc(n); ls->light.alloc(n);
-        ls->row_pix.alloc(S); ls->rows_mat.alloc(S);
-        ls->vis_bits.alloc((size_t)CW * ((ls->W + 7) / 8) * ((ls->H + 7) / 8)); ls->rows_recv.alloc(RZK_RECV_FLOATS * S);
-        ls->R.alloc(3 * (size_t)S * V); ls->Vm.alloc((size_t)S * V);
-        ls->alphas.alloc(V); ls->mags.alloc(V); ls->norms.alloc(V); ls->u.alloc(S); ls->v.alloc(V); ls->c0.alloc(1);
-        ls->mT.alloc((size_t)S * V);
-        ls->centers.alloc(C); ls->center_w.alloc(C); ls->K.alloc(3 * (size_t)S * C); ls->Knorm.alloc(C);
+        ls->vis_bits.alloc((size_t)CW * ((ls->W + 7) / 8) * ((ls->H + 7) / 8));
+        ls->alphas.alloc(V); ls->mags.alloc(V); ls->norms.alloc(V); ls->v.alloc(V); ls->c0.alloc(1);
+        ls->centers.alloc(C); ls->center_w.alloc(C); ls->Knorm.alloc(C);
         ls->assign.alloc(V); ls->reps.alloc(C); ls->rep_w.alloc(C);
-        ls->nb.alloc((size_t)S * p->num_neighbor_slices);
-        ls->upper.alloc((size_t)S * CW); ls->slice_wi.alloc(3 * (size_t)S * CW); ls->slice_w.alloc((size_t)S * CW);
-        ls->slice_pdf.alloc((size_t)S * CW); ls->slice_cdf.alloc((size_t)S * CW);
-        if (p->max_slice_clusters > 0) {
-            ls->sreps.alloc((size_t)S * p->max_slice_clusters); ls->srep_w.alloc((size_t)S * p->max_slice_clusters);
-            ls->scount.alloc(S); ls->norms_R.alloc(V); ls->cl_off.alloc(C + 1); ls->cl_mem.alloc(V);
-        }
-        ls->maps.alloc((size_t)S * res * res); ls->col_cdf.alloc((size_t)S * res * res);
-        ls->row_pdf.alloc((size_t)S * res); ls->row_cdf.alloc((size_t)S * res); ls->totals.alloc(S);
-        std::vector<int> pix, nbh;
-        ls_rows(ls.get(), pix);
-        ls_neighbours(ls.get(), nbh, p->num_neighbor_slices);
-        RZK_HIP(hipMemcpy(ls->row_pix.p, pix.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-        RZK_HIP(hipMemcpy(ls->nb.p, nbh.data(), sizeof(int) * nbh.size(), hipMemcpyHostToDevice));
+        if (p->max_slice_clusters > 0) { ls->norms_R.alloc(V); ls->cl_off.alloc(C + 1); ls->cl_mem.alloc(V); }
         std::memset(&ls->info, 0, sizeof(ls->info));
+        if (!ls->kd) {
+            ls_alloc_slices(ls.get());
+            std::vector<int> pix, nbh;
+            ls_rows(ls.get(), pix);
+            ls_neighbours(ls.get(), nbh, p->num_neighbor_slices);
+            RZK_HIP(hipMemcpy(ls->row_pix.p, pix.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+            RZK_HIP(hipMemcpy(ls->nb.p, nbh.data(), sizeof(int) * nbh.size(), hipMemcpyHostToDevice));
+            ls->h_row_pix = pix;
+        } else {
+            ls->pix_slice.alloc(n);
+            ls->gp_pix.alloc(n);
+        }
         ls->info.num_slices = S;
         ctx_retain(ctx);
         *out = ls.release();
@@ -709,13 +814,17 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
                   "camera image size differs from the LightSlice parameters");
         const rzk_lightslice_params& p = ls->prm;
         SceneView sv = scene_view(ls->scene);
-        int S = ls->S;
         int64_t V = ls->V, n = (int64_t)ls->W * ls->H;
         hipStream_t st = ctx->stream;
+        // one MT stream for the slice representatives, the centres and the refinement
+        Mt19937 rd(p.seed);
+        ls->initialized = false;
         // primary hits for every pixel (camera.cc:52-65 pixel centres)
         launch_camera_image_rays(ctx, camera, ls->W, ls->H, ls->rays.p);
         launch_trace_closest_image(ctx, sv, ls->rays.p, ls->W, ls->H, 1e-3f, 1e6f, ls->hits.p, nullptr);
         launch_fill_receivers(ctx, sv, ls->rays.p, ls->hits.p, n, ls->recv.p, ls->mat.p, ls->light.p);
+        if (ls->kd) ls_kd_slices(ls, rd);
+        int S = ls->S;
         hipLaunchKernelGGL(k_pick_rows, dim3((S + 63) / 64), dim3(64), 0, st, ls->recv.p, ls->mat.p, ls->row_pix.p, S,
                            ls->rows_recv.p, ls->rows_mat.p);
         // reduced matrix (lightslice.cc:721-748)
@@ -728,7 +837,6 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         RZK_HIP(hipMemcpyAsync(ls->h_alphas.data(), ls->alphas.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
         ls_sync(ctx);
         // centre sampling (cluster_builder.cc:114-156) on the host MT stream
-        Mt19937 rd(p.seed);
         DiscretePdf apdf;
         apdf.build(ls->h_alphas.data(), (size_t)V);
         int C = p.num_clusters;
@@ -877,6 +985,7 @@ int rzk_lightslice_render_guided(rzk_lightslice* ls, const float* camera, const 
         pp.width = ls->W; pp.height = ls->H; pp.spp = p->spp; pp.max_bounce = p->max_bounce;
         pp.direct = p->direct_lighting ? 1 : 0; pp.guide_mode = p->guide_mode; pp.tile = ls->T;
         pp.res = ls->prm.map_resolution; pp.tiles_x = ls->tx; pp.tiles_y = ls->ty;
+        pp.pix_slice = ls->kd ? ls->pix_slice.p : nullptr;
         pp.x0 = p->w > 0 ? p->x0 : 0; pp.y0 = p->h > 0 ? p->y0 : 0;
         pp.w = p->w > 0 ? p->w : ls->W; pp.h = p->h > 0 ? p->h : ls->H;
         RZK_CHECK(pp.x0 >= 0 && pp.y0 >= 0 && pp.x0 + pp.w <= ls->W && pp.y0 + pp.h <= ls->H, RZK_E_INVALID,
@@ -911,6 +1020,7 @@ int rzk_lightslice_guide(rzk_lightslice* ls, const float* camera, rzk_guide** ou
     return ls_guard(ctx, [&] {
         RZK_CHECK(ls->initialized, RZK_E_STATE, "rzk_lightslice_initialize has not run");
         RZK_CHECK((int)camera[17] == ls->W && (int)camera[18] == ls->H, RZK_E_INVALID, "camera size differs");
+        RZK_CHECK(!ls->kd, RZK_E_STATE, "guides are laid out by pixel tiles; this LightSlice has kd-tree slices");
         std::unique_ptr<rzk_guide> g(new rzk_guide());
         g->ctx = ctx;
         g->scene = ls->scene;
@@ -961,7 +1071,7 @@ int rzk_lightslice_count_work(rzk_lightslice* ls, int64_t* out) {
         int64_t rows_ok = 0, pairs = 0;
         for (int v : rm) rows_ok += v >= 0;
         for (int64_t i = 0; i < n; ++i)        // pixels with a material x their slice's clusters
-            if (pm[i] >= 0) pairs += ls->h_scount[((i / ls->W) / ls->T) * ls->tx + (i % ls->W) / ls->T];
+            if (pm[i] >= 0) pairs += ls->h_scount[ls->slice_of_pixel(i)];
         out[0] = n;
         out[1] = rows_ok * V;
         out[2] = (int64_t)h[0];
@@ -1032,6 +1142,15 @@ int rzk_lightslice_get(const rzk_lightslice* ls, const char* name, void* dst, in
         else if (nm == "recv") { src = ls->recv.p; need = 4 * (int64_t)RZK_RECV_FLOATS * n; }
         else if (nm == "recv_mat") { src = ls->mat.p; need = 4 * n; }
         else if (nm == "hits") { src = ls->hits.p; need = 16 * n; }
+        else if (nm == "row_pix") { src = ls->h_row_pix.data(); need = 4 * (int64_t)ls->h_row_pix.size(); host = true; }
+        else if (nm == "pixel_slice") {
+            RZK_CHECK(ls->kd, RZK_E_INVALID, "pixel_slice: this LightSlice has pixel-tile slices");
+            src = ls->h_pix_slice.data(); need = 4 * (int64_t)ls->h_pix_slice.size(); host = true;
+        }
+        else if (nm == "gather_pix") {
+            RZK_CHECK(ls->kd, RZK_E_INVALID, "gather_pix: this LightSlice has pixel-tile slices");
+            src = ls->h_gp_pix.data(); need = 4 * (int64_t)ls->h_gp_pix.size(); host = true;
+        }
         else throw RzkError{RZK_E_INVALID, "unknown array name " + nm};
         RZK_CHECK(bytes == need, RZK_E_INVALID, "size mismatch for " + nm + ": need " + std::to_string(need));
         if (need == 0) return;

@@ -14,6 +14,7 @@ struct PtParams {
     int x0, y0, w, h;              // rendered window of the image
     uint32_t seed;
     float tmin;
+    const int* pix_slice = nullptr;   // kd-tree slices: each pixel's slice (-1: none), else tiles
 };
 
 struct GuideView {

@@ -176,12 +176,19 @@ __device__ __forceinline__ Ray camera_sample_ray(const float* c, float x, float 
     return ray_ctor(mk(c[0], c[1], c[2]), d);
 }
 
-// Slice of a pixel: its tile when guide_mode 1 and the tile's map is non-empty,
+// Slice of a pixel: its tile (or, with kd-tree slices, the slice of the
+// pixel's gather point) when guide_mode 1 and the slice's map is non-empty,
 // else -1 (the uniform-hemisphere fallback)
 __device__ __forceinline__ int pixel_slice(const PtParams& pp, const GuideView& gv, int x, int y) {
     if (pp.guide_mode != 1) return -1;
-    int sx = min(x / pp.tile, pp.tiles_x - 1), sy = min(y / pp.tile, pp.tiles_y - 1);
-    int s = sy * pp.tiles_x + sx;
+    int s;
+    if (pp.pix_slice) {
+        s = pp.pix_slice[(int64_t)y * pp.width + x];
+        if (s < 0) return -1;
+    } else {
+        int sx = min(x / pp.tile, pp.tiles_x - 1), sy = min(y / pp.tile, pp.tiles_y - 1);
+        s = sy * pp.tiles_x + sx;
+    }
     return gv.totals[s] > 0.0f ? s : -1;
 }
 

@@ -190,8 +190,10 @@ struct SliceReps {
     int nreps;
     int ld;
     int tile, tiles_x;
+    const int* pix_slice = nullptr;    // kd-tree slices: each pixel's slice (-1: no gather point)
     __host__ __device__ int width() const { return ld ? ld : nreps; }
     __device__ int slice_of(int64_t pix, int img_width) const {
+        if (pix_slice) return pix_slice[pix];
         int px = (int)(pix % img_width), py = (int)(pix / img_width);
         return (py / tile) * tiles_x + px / tile;
     }

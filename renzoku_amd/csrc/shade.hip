@@ -242,8 +242,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, 
     int64_t i = tile < ntiles ? tile_pixel(tile, lane, width, height) : -1;
     int m = i >= 0 ? rmat[i] : -1;
     v3 p = m >= 0 ? ld3(recv + RZK_RECV_FLOATS * i) : mk(0.f, 0.f, 0.f);
-    const int sl = (!kShared && i >= 0) ? sr.slice_of(i, width) : 0;
-    const int mine = i >= 0 ? sr.count(sl) : 0;
+    const int sl = (!kShared && m >= 0) ? sr.slice_of(i, width) : 0;
+    const int mine = m >= 0 ? sr.count(sl) : 0;
     TravStats st;
     // blockIdx.y picks this block's share of the representatives
     const int C = sr.width();

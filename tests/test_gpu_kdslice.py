@@ -102,3 +102,77 @@ def test_kd_build_from_device_points(ctx):
         g.close()
     finally:
         ctx.lib.rzk_device_free(ctx.h, d)
+
+
+def _rel_l2(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_lightslice_with_kd_slices_matches_oracle(ctx):
+    """LightSlice with the reference's slices: gather points, kd-tree, MT
+    representatives and rows, nearest-slice neighbours, each pixel's slice,
+    then clustering and per-slice refinement on the continuing MT stream and
+    the gather over each pixel's own slice"""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    W = H = 128
+    sc = scenes.room(W, H, seed=11, target_tris=20_000)
+    gs = rz.Scene(ctx, sc)
+    tracer, filler = vplgen.gpu_tracer(gs)
+    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 2048, seed=5, batch=8192)
+    vp = rz.Vpls(ctx, vpl, vmat)
+    ls = rz.LightSlice(ctx, gs, vp, width=W, height=H, num_clusters=32, num_neighbor_slices=4, n_emitted=ne,
+                       seed=2013, max_slice_clusters=64, kd_max_slice_size=128)
+    ls.initialize(sc.camera)
+    inf = ls.info()
+    S, V = inf.num_slices, len(vmat)
+    # oracle: receivers, gather points, points, tree
+    osc = oracle.OrcScene(sc)
+    px, py = scenes.pixel_grid(W, H)
+    rays = oracle.camera_rays(sc.camera, px, py)
+    t, b, g, tri = osc.trace_closest(rays)
+    recv, m, _ = osc.fill_receivers(rays, t, b, g, tri)
+    gp = np.nonzero(m >= 0)[0].astype(np.int32)
+    assert np.array_equal(ls.get("gather_pix", np.int32, len(gp)), gp)
+    p = np.asarray(sc.positions, np.float32).reshape(-1, 3)
+    d = p.max(0) - p.min(0)
+    diag = float(np.sqrt(np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]), dtype=np.float32))
+    pts = oracle.map_vec6(recv[gp], m[gp], osc.materials, diag)
+    okd = oracle.KdSlices(pts, 128, 2013)
+    assert S == okd.S and S >= 64
+    reps, ndraw = oracle.kd_representatives(okd.slices, len(gp), 2013)
+    assert np.array_equal(ls.get("row_pix", np.int32, S), gp[okd.order[reps]])
+    nb = ls.get("neighbors", np.int32, S * 4).reshape(S, 4)
+    assert np.array_equal(nb, okd.nearest(pts[okd.order[reps]], 4))
+    psl = np.full(W * H, -1, np.int32)
+    psl[gp] = okd.find(pts)
+    assert np.array_equal(ls.get("pixel_slice", np.int32, W * H), psl)
+    # clustering and refinement continue the MT stream after the representatives
+    R = ls.get("R", np.float32, S * V * 3).reshape(V, S, 3)
+    o = oracle.cluster_and_refine(R, 32, 64, nb, 2013, skip=ndraw)
+    cnt = ls.get("slice_count", np.int32, S)
+    sreps = ls.get("slice_reps", np.int32, S * 64).reshape(S, 64)
+    rw = ls.get("slice_rep_w", np.float32, S * 64).reshape(S, 64)
+    assert np.array_equal(ls.get("centers", np.int32, inf.num_centers), o["centers"])
+    assert np.array_equal(cnt, o["count"])
+    for s in range(S):
+        assert np.array_equal(sreps[s, :cnt[s]], o["rep"][s, :cnt[s]])
+        assert np.array_equal(rw[s, :cnt[s]].view(np.uint32), o["w"][s, :cnt[s]].view(np.uint32))
+    # the gather: each pixel over its own slice's representatives
+    img = ls.gather()
+    ref = np.zeros((W * H, 3), np.float32)
+    for s in range(S):
+        sel = np.nonzero(psl == s)[0]
+        n = cnt[s]
+        vsub = vpl[sreps[s, :n]].copy()
+        vsub[:, 18:21] = vpl[sreps[s, :n], 18:21] * rw[s, :n, None]
+        ref[sel] = osc.gather(recv[sel], m[sel], vsub, vmat[sreps[s, :n]], 1, 0.01, 1e-3, ne, nthreads=8)
+    assert _rel_l2(img.reshape(-1, 3), ref) <= 1e-4
+    work = ls.count_work()
+    assert work["gather_pairs"] == int(cnt[psl[gp]].sum())
+    # guided rendering reads each pixel's slice
+    out, rays_ = ls.render_guided(sc.camera, spp=4, max_bounce=2, guide_mode=1, seed=3)
+    assert out.shape == (H, W, 3) and np.isfinite(out).all() and out.sum() > 0
+    with pytest.raises(rz.RzkError):
+        ls.guide(sc.camera)
