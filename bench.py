@@ -606,6 +606,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tris", type=int, default=0, help="triangle count (default: the workload's)")
+    ap.add_argument("--kd", type=int, default=0,
+                    help="lightslice workload with the reference's kd-tree slices of at most this many points "
+                         "(0: 32x32 pixel tiles)")
     ap.add_argument("--vpls", type=int, default=0, help="VPL count (default: the workload's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device once (a known-size transfer for "
@@ -634,7 +637,7 @@ def main():
     W = H = 512
     params = dict(width=W, height=H, tile=32, num_clusters=300, num_neighbor_slices=16, map_resolution=32,
                   map_epsilon=1e-3, clamp_mode=rz._lib.CLAMP_FORM_FACTOR, clamp=0.01, incident_clamp=0.01,
-                  tmin=1e-3, seed=2013, map_gate=0)
+                  tmin=1e-3, seed=2013, map_gate=0, kd_max_slice_size=args.kd)
     t0 = time.perf_counter()
     sc = scenes.room(W, H, seed=2013, target_tris=args.tris or 200_000)
     log(f"[bench] scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
@@ -694,7 +697,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded room scene, seeded light-traced VPLs)",
-        "config": {"workload": "lightslice-gather room 200k tris 512x512 65536 VPLs 32x32 slices 300 clusters",
+        "config": {"workload": ("lightslice-gather room 200k tris 512x512 65536 VPLs 32x32 slices 300 clusters"
+                                if not args.kd else
+                                f"lightslice-gather room 200k tris 512x512 65536 VPLs kd-tree slices of {args.kd} "
+                                "points 300 clusters"),
                    "triangles": int(sc.ntri), "width": W, "height": H, "vpls": int(len(vmat)),
                    "slices": int(lsinf.num_slices), "clusters": int(lsinf.num_clusters),
                    "parallelism": "single GPU"},
