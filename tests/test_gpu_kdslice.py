@@ -176,3 +176,49 @@ def test_lightslice_with_kd_slices_matches_oracle(ctx):
     assert out.shape == (H, W, 3) and np.isfinite(out).all() and out.sum() > 0
     with pytest.raises(rz.RzkError):
         ls.guide(sc.camera)
+
+
+def test_lightslice_kd_slices_without_refinement(ctx):
+    """every slice gathers the kept clusters: the image equals the shared-list
+    gather, and the slice table, rows and neighbours still follow the tree"""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    W, H = 96, 80
+    sc = scenes.cornell_box(W, H)
+    gs = rz.Scene(ctx, sc)
+    tracer, filler = vplgen.gpu_tracer(gs)
+    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 1024, seed=3, batch=4096)
+    vp = rz.Vpls(ctx, vpl, vmat)
+    ls = rz.LightSlice(ctx, gs, vp, width=W, height=H, num_clusters=24, num_neighbor_slices=8, n_emitted=ne,
+                       seed=99, kd_max_slice_size=64)
+    ls.initialize(sc.camera)
+    inf = ls.info()
+    S = inf.num_slices
+    osc = oracle.OrcScene(sc)
+    px, py = scenes.pixel_grid(W, H)
+    rays = oracle.camera_rays(sc.camera, px, py)
+    t, b, g, tri = osc.trace_closest(rays)
+    recv, m, _ = osc.fill_receivers(rays, t, b, g, tri)
+    gp = np.nonzero(m >= 0)[0].astype(np.int32)
+    p = np.asarray(sc.positions, np.float32).reshape(-1, 3)
+    d = p.max(0) - p.min(0)
+    diag = float(np.sqrt(np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]), dtype=np.float32))
+    pts = oracle.map_vec6(recv[gp], m[gp], osc.materials, diag)
+    okd = oracle.KdSlices(pts, 64, 99)
+    assert S == okd.S
+    reps, ndraw = oracle.kd_representatives(okd.slices, len(gp), 99)
+    assert np.array_equal(ls.get("row_pix", np.int32, S), gp[okd.order[reps]])
+    assert np.array_equal(ls.get("neighbors", np.int32, S * 8).reshape(S, 8), okd.nearest(pts[okd.order[reps]], 8))
+    V = len(vmat)
+    R = ls.get("R", np.float32, S * V * 3).reshape(V, S, 3)
+    oc = oracle.cluster_by_sampling(R, 24, 99, nthreads=8, skip=ndraw)
+    creps = ls.get("reps", np.int32, inf.num_clusters)
+    crw = ls.get("rep_w", np.float32, inf.num_clusters)
+    assert np.array_equal(creps, oc["reps"]) and np.array_equal(crw.view(np.uint32), oc["rep_w"].view(np.uint32))
+    img = ls.gather()
+    vsub = vpl[creps].copy()
+    vsub[:, 18:21] = vpl[creps, 18:21] * crw[:, None]
+    ref = osc.gather(recv, m, vsub, vmat[creps], 1, 0.01, 1e-3, ne, nthreads=8).reshape(H, W, 3)
+    assert _rel_l2(img, ref) <= 1e-4
+    with pytest.raises(rz.RzkError):
+        rz.LightSlice(ctx, gs, vp, width=W, height=H, num_clusters=24, kd_max_slice_size=2)
