@@ -162,8 +162,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_gather_fused(SceneView sv, cons
 // Outputs go to the column-major position of the original VPL index.
 // kCount: traversal work added to d_stats (the counting pass); otherwise the
 // counters are dead code.
+constexpr int kRmBlock = 256;
+constexpr int kRmRows = kRmBlock / 64;         // rows sharing a block's VPL chunk
+
 template <bool kCount>
-__global__ __launch_bounds__(kShadeBlock) void k_reduced_matrix(SceneView sv, const float* __restrict__ recv,
+__global__ __launch_bounds__(kRmBlock) void k_reduced_matrix(SceneView sv, const float* __restrict__ recv,
                                                                const int* __restrict__ rmat, int rows,
                                                                const int* __restrict__ vorder,
                                                                const float* __restrict__ vpl,
@@ -173,16 +176,24 @@ __global__ __launch_bounds__(kShadeBlock) void k_reduced_matrix(SceneView sv, co
                                                                int* __restrict__ d_stats) {
     extern __shared__ int s_stack[];
     int* wstk = wave_stack(s_stack, sv);
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // wave W takes 64 consecutive (Morton-ordered) VPLs of one row; the
+    // kRmRows waves of a block take the same VPLs for kRmRows consecutive rows,
+    // so their R / V writes for one VPL land side by side
+    const int64_t W = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t nc = (nvpl + 63) >> 6;                         // VPL chunks per row
+    const int64_t grp = W / (kRmRows * nc), idx = W - grp * (kRmRows * nc);
+    const int64_t chunk = idx / kRmRows;
+    const int r_ = (int)(grp * kRmRows + (idx - chunk * kRmRows));
+    const int64_t kk = chunk * 64 + lane;
     TravStats st;
-    bool inside = t < (int64_t)rows * nvpl;
+    bool inside = r_ < rows && kk < nvpl;
     int r = 0;
     int64_t k = 0;
     bool active = false;
     v3 p = mk(0.f, 0.f, 0.f), q = mk(0.f, 0.f, 0.f);
     if (inside) {
-        r = (int)(t / nvpl);
-        int64_t kk = t - (int64_t)r * nvpl;
+        r = r_;
         k = vorder ? vorder[kk] : kk;
         active = rmat[r] >= 0;
         if (active) {
@@ -386,16 +397,17 @@ void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv_in, const float* d_
                            int rows, const int* d_vorder, const float* d_vpl, const int* d_vmat,
                            int64_t nvpl, int clamp_mode, float clamp, float tmin, float* d_R, float* d_V,
                            int* d_stats) {
-    int64_t n = (int64_t)rows * nvpl;
-    if (n <= 0) return;
+    if ((int64_t)rows * nvpl <= 0) return;
     SceneView sv = sv_in;
-    unsigned grid = blocks_for(n, kShadeBlock);
-    size_t lds = sizeof(int) * (kShadeBlock / 64) * (size_t)sv.stack4;
+    // waves: row groups of kRmRows x 64-VPL chunks x kRmRows rows (see the kernel)
+    const int64_t waves = (((int64_t)rows + kRmRows - 1) / kRmRows) * kRmRows * ((nvpl + 63) / 64);
+    unsigned grid = blocks_for(waves * 64, kRmBlock);
+    size_t lds = sizeof(int) * (kRmBlock / 64) * (size_t)sv.stack4;
     if (d_stats)
-        hipLaunchKernelGGL(k_reduced_matrix<true>, dim3(grid), dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
+        hipLaunchKernelGGL(k_reduced_matrix<true>, dim3(grid), dim3(kRmBlock), lds, ctx->stream, sv, d_recv,
                            d_rmat, rows, d_vorder, d_vpl, d_vmat, nvpl, clamp_mode, clamp, tmin, d_R, d_V, d_stats);
     else
-        hipLaunchKernelGGL(k_reduced_matrix<false>, dim3(grid), dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
+        hipLaunchKernelGGL(k_reduced_matrix<false>, dim3(grid), dim3(kRmBlock), lds, ctx->stream, sv, d_recv,
                            d_rmat, rows, d_vorder, d_vpl, d_vmat, nvpl, clamp_mode, clamp, tmin, d_R, d_V, d_stats);
     RZK_HIP(hipGetLastError());
 }
