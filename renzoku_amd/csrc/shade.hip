@@ -163,7 +163,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_gather_fused(SceneView sv, cons
 // kCount: traversal work added to d_stats (the counting pass); otherwise the
 // counters are dead code.
 constexpr int kRmBlock = 256;
-constexpr int kRmRows = kRmBlock / 64;         // rows sharing a block's VPL chunk
+constexpr int kRmXcd = 8;                      // XCDs; workgroup b runs on XCD b % 8
+constexpr int kRmPair = 2;                     // blocks per row group, kRmXcd apart
+constexpr int kRmRows = kRmPair * (kRmBlock / 64);   // rows sharing a VPL chunk
 
 template <bool kCount>
 __global__ __launch_bounds__(kRmBlock) void k_reduced_matrix(SceneView sv, const float* __restrict__ recv,
@@ -176,15 +178,19 @@ __global__ __launch_bounds__(kRmBlock) void k_reduced_matrix(SceneView sv, const
                                                                int* __restrict__ d_stats) {
     extern __shared__ int s_stack[];
     int* wstk = wave_stack(s_stack, sv);
-    // wave W takes 64 consecutive (Morton-ordered) VPLs of one row; the
-    // kRmRows waves of a block take the same VPLs for kRmRows consecutive rows,
-    // so their R / V writes for one VPL land side by side
-    const int64_t W = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // a wave takes 64 consecutive (Morton-ordered) VPLs of one row.  A group
+    // of kRmRows waves takes the same VPLs for kRmRows consecutive rows, so
+    // their R / V writes for one VPL land side by side: the group is the
+    // waves of blocks b and b + kRmXcd, which the dispatcher places on the
+    // same XCD (workgroups go round-robin over the XCDs), so one L2 sees them
+    const int64_t b = blockIdx.x;
+    const int64_t x = b % kRmXcd, bx = b / kRmXcd;
+    const int64_t G = (bx / kRmPair) * kRmXcd + x;               // (row group, chunk) pair
+    const int gw = (int)((bx % kRmPair) * (kRmBlock / 64) + (threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63);
     const int64_t nc = (nvpl + 63) >> 6;                         // VPL chunks per row
-    const int64_t grp = W / (kRmRows * nc), idx = W - grp * (kRmRows * nc);
-    const int64_t chunk = idx / kRmRows;
-    const int r_ = (int)(grp * kRmRows + (idx - chunk * kRmRows));
+    const int64_t rg = G / nc, chunk = G - rg * nc;
+    const int r_ = (int)(rg * kRmRows + gw);
     const int64_t kk = chunk * 64 + lane;
     TravStats st;
     bool inside = r_ < rows && kk < nvpl;
@@ -399,9 +405,11 @@ void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv_in, const float* d_
                            int* d_stats) {
     if ((int64_t)rows * nvpl <= 0) return;
     SceneView sv = sv_in;
-    // waves: row groups of kRmRows x 64-VPL chunks x kRmRows rows (see the kernel)
-    const int64_t waves = (((int64_t)rows + kRmRows - 1) / kRmRows) * kRmRows * ((nvpl + 63) / 64);
-    unsigned grid = blocks_for(waves * 64, kRmBlock);
+    // (row group, chunk) pairs, each kRmPair blocks kRmXcd apart (see the kernel);
+    // the pairs are padded to a multiple of kRmXcd so every block has its partner
+    const int64_t pairs = (((int64_t)rows + kRmRows - 1) / kRmRows) * ((nvpl + 63) / 64);
+    const int64_t padded = (pairs + kRmXcd - 1) / kRmXcd * kRmXcd;
+    unsigned grid = (unsigned)(padded * kRmPair);
     size_t lds = sizeof(int) * (kRmBlock / 64) * (size_t)sv.stack4;
     if (d_stats)
         hipLaunchKernelGGL(k_reduced_matrix<true>, dim3(grid), dim3(kRmBlock), lds, ctx->stream, sv, d_recv,
