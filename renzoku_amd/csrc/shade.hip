@@ -248,11 +248,18 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, 
                                                                  int height, const float* __restrict__ vpl,
                                                                  SliceReps sr, float tmin,
                                                                  uint64_t* __restrict__ bits,
-                                                                 int* __restrict__ d_stats) {
+                                                                 int* __restrict__ d_stats, int groups) {
     extern __shared__ int s_dyn[];
     float* s_q = reinterpret_cast<float*>(s_dyn);            // kVplTile positions
     int* wstk = wave_stack(s_dyn + 3 * kVplTile, sv);
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // one flat grid: block B runs on XCD B % 8 (round-robin dispatch); the
+    // `groups` representative shares of one tile block take consecutive
+    // blocks of one XCD, so they run together over one L2 and share the
+    // pixel-side nodes: B = ((tb / 8) * groups + g) * 8 + tb % 8
+    const int64_t B = blockIdx.x, q = B / 8;
+    const int gidx = (int)(q % groups);
+    const int64_t tb = (q / groups) * 8 + (B % 8);
+    int64_t t = tb * blockDim.x + threadIdx.x;
     int64_t ntiles = (int64_t)((width + 7) / 8) * ((height + 7) / 8);
     int64_t tile = t / 64;
     int lane = (int)(t & 63);
@@ -262,10 +269,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, 
     const int sl = (!kShared && m >= 0) ? sr.slice_of(i, width) : 0;
     const int mine = m >= 0 ? sr.count(sl) : 0;
     TravStats st;
-    // blockIdx.y picks this block's share of the representatives
+    // gidx picks this block's share of the representatives
     const int C = sr.width();
-    const int per = (C + gridDim.y - 1) / gridDim.y;
-    const int c_begin = (int)blockIdx.y * per, c_end = min(C, c_begin + per);
+    const int per = (C + groups - 1) / groups;
+    const int c_begin = gidx * per, c_end = min(C, c_begin + per);
     for (int c0 = c_begin; c0 < c_end; c0 += kVplTile) {
         int cnt = (c_end - c0) < kVplTile ? (c_end - c0) : kVplTile;
         if (kShared) {
@@ -437,19 +444,22 @@ void launch_slice_gather(rzk_ctx* ctx, const SceneView& sv_in, const float* d_re
     // representatives per wave 7.3 ms for 512x512 pixels x 300 representatives)
     const int groups = std::max(1, std::min((sr.width() + 3) / 4, 65535));
     const bool shared = sr.ld == 0;
-    dim3 g2(grid, groups);
+    // flat grid of (tile block, share) pairs, the tile blocks padded to a multiple of 8 (see the kernel)
+    const int64_t flat = ((int64_t)grid + 7) / 8 * 8 * groups;
+    RZK_CHECK(flat < ((int64_t)1 << 31), RZK_E_INVALID, "image too large for one gather launch");
+    dim3 g2((unsigned)flat);
     if (d_stats && shared)
         hipLaunchKernelGGL((k_slice_visibility<true, true>), g2, dim3(kShadeBlock), lds, ctx->stream, sv, d_recv, d_rmat,
-                           width, height, d_vpl, sr, tmin, d_bits, d_stats);
+                           width, height, d_vpl, sr, tmin, d_bits, d_stats, groups);
     else if (d_stats)
         hipLaunchKernelGGL((k_slice_visibility<true, false>), g2, dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
-                           d_rmat, width, height, d_vpl, sr, tmin, d_bits, d_stats);
+                           d_rmat, width, height, d_vpl, sr, tmin, d_bits, d_stats, groups);
     else if (shared)
         hipLaunchKernelGGL((k_slice_visibility<false, true>), g2, dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
-                           d_rmat, width, height, d_vpl, sr, tmin, d_bits, d_stats);
+                           d_rmat, width, height, d_vpl, sr, tmin, d_bits, d_stats, groups);
     else
         hipLaunchKernelGGL((k_slice_visibility<false, false>), g2, dim3(kShadeBlock), lds, ctx->stream, sv, d_recv,
-                           d_rmat, width, height, d_vpl, sr, tmin, d_bits, d_stats);
+                           d_rmat, width, height, d_vpl, sr, tmin, d_bits, d_stats, groups);
     RZK_HIP(hipGetLastError());
     if (shared)
         hipLaunchKernelGGL(k_slice_shade<true>, dim3(grid), dim3(kShadeBlock), 0, ctx->stream, sv.mat, d_recv, d_rmat,
