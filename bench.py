@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per gather launch (k_slice_visibility + k_slice_shade) measured by rocprofv3 --pmc FETCH_SIZE
 # (x2, the 512 MiB copy calibration) and WRITE_SIZE (x1) on this workload; the passes and the calibration
 # are recorded in profiles/slice_gather_pmc.md
-GATHER_HBM_BYTES = (250621 + 49215) * 2 * 1024 + (19626 + 3072) * 1024
+GATHER_HBM_BYTES = (30397 + 49212) * 2 * 1024 + (19206 + 3072) * 1024
 # MI355X vector issue: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz lane-operations per second
 VALU_LANE_OPS_PEAK = 256 * 64 * 2.4e9
 # algorithmic bytes per shadow segment: ray in, result out, every 4-wide node
@@ -715,7 +715,7 @@ def main():
                      "note": "achieved counts every lane's node visits (128 B per 4-wide node, 48 B per triangle "
                              "test, 36 B of segment I/O); a wave fetches each node once and L2/MALL serve it, so "
                              "measured HBM traffic per launch (traffic) is far below; the visibility kernel is "
-                             "vector-issue bound (VALUBusy 79.4%, about 77% of lane-op peak, rocprofv3 --pmc)"},
+                             "vector-issue bound (VALUBusy reads 108.6%, saturated, rocprofv3 --pmc)"},
     }
     if not args.no_cpu:
         reps = ls.get("reps", np.int32, lsinf.num_clusters)
