@@ -211,6 +211,18 @@ int rzk_kdslices_nearest(const rzk_kdslices *kd, const float *queries, int32_t m
 int rzk_gather(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls, const float *recv,
                const int32_t *recv_mat, int64_t npts, int clamp_mode, float clamp, float tmin,
                float n_emitted, float *out_rgb);
+/* No counterpart in the reference: the rzk_gather above for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Receivers (d_recv, d_recv_mat) and the
+ * output (d_out_rgb, npts x 3) are device memory owned by the caller; launches
+ * are queued on the context stream and the call returns without synchronising.
+ * The visibility bits live in a static 32 MiB device array per GPU, so
+ * ceil(npts / 64) x VPLs must be at most 4,194,304 (else RZK_E_INVALID).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * streams or contexts) race; calls queued on one stream do not. */
+int rzk_gather_dev_noalloc(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls, const float *d_recv,
+                           const int32_t *d_recv_mat, int64_t npts, int clamp_mode, float clamp, float tmin,
+                           float n_emitted, float *d_out_rgb);
 /* visibility of every (receiver, VPL) pair as a bit matrix: word (k, i/64) bit
  * i%64 set when the segment receiver i -> VPL k is blocked (vpl.cc:672). */
 int rzk_visibility(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls, const float *recv,

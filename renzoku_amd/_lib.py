@@ -28,7 +28,7 @@ EXPORTS = [
     "rzk_trace_closest_dev", "rzk_trace_any_dev", "rzk_trace_segments_dev", "rzk_trace_stats",
     "rzk_fill_receivers", "rzk_camera_primary",
     "rzk_vpls_create", "rzk_vpls_destroy", "rzk_generate_vpls",
-    "rzk_gather", "rzk_visibility", "rzk_render_vpl",
+    "rzk_gather", "rzk_gather_dev_noalloc", "rzk_visibility", "rzk_render_vpl",
     "rzk_reduced_matrix", "rzk_alphas", "rzk_cluster_assign", "rzk_refine_slices",
     "rzk_lightslice_create", "rzk_lightslice_destroy", "rzk_lightslice_initialize",
     "rzk_lightslice_gather", "rzk_lightslice_get_info", "rzk_lightslice_count_work", "rzk_lightslice_get",
@@ -182,6 +182,7 @@ def load() -> C.CDLL:
         "rzk_kdslices_nearest": [P, P, C.c_int32, C.c_int32, P],
         "rzk_map_vec6": [P, P, P, P, I64, F, P],
         "rzk_gather": [P, P, P, P, P, I64, C.c_int, F, F, F, P],
+        "rzk_gather_dev_noalloc": [P, P, P, P, P, I64, C.c_int, F, F, F, P],
         "rzk_visibility": [P, P, P, P, P, I64, F, P],
         "rzk_render_vpl": [P, P, P, P, C.c_int, F, F, F, P],
         "rzk_reduced_matrix": [P, P, P, P, P, I32, C.c_int, F, F, P, P],

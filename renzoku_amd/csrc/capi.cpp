@@ -140,6 +140,13 @@ int rzk_ctx_create(int device, rzk_ctx** out) {
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return RZK_E_HIP;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    // the no-allocation gather's scratch address on this device (resolved once per device)
+    try {
+        gather_noalloc_setup(device);
+    } catch (const RzkError&) {
+        (void)hipStreamDestroy(ctx->stream);
+        return RZK_E_HIP;
+    }
     *out = ctx.release();
     return RZK_OK;
 }

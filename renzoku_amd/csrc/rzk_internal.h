@@ -143,6 +143,9 @@ struct RzkError { int code; std::string msg; };
     do { if (!(cond)) throw RzkError{code, msg}; } while (0)
 
 void* ctx_scratch(rzk_ctx* ctx, size_t bytes);
+// resolves the device address of the no-allocation gather's static scratch on
+// `device` (current), once per device (gather_noalloc.hip)
+void gather_noalloc_setup(int device);
 // size sv.spill for a launch of `threads` lanes whose traversal stack holds up
 // to `entries` ints (0: sv.stack4, the any-hit bound; closest_hit4 needs 2 * stack4)
 void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries = 0);
