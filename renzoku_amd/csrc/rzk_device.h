@@ -264,8 +264,11 @@ __device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, f
 // per-lane any_hit4 result); only the schedule is shared.  kSigned: every
 // walking lane's direction has the wave-uniform component signs (nx, ny, nz:
 // true for a negative component), so the child tests use box_hit_signed
-// (same outcomes, fewer operations).
-template <bool kSigned>
+// (same outcomes, fewer operations).  kCount: st.nodes counts, per lane, the
+// nodes its own ray reaches (the parent's test of that child box passed for
+// this lane), not every node the wave fetches; own[] keeps that bit for each
+// stack entry (entries past 64 are counted as reached).
+template <bool kSigned, bool kCount>
 __device__ __forceinline__ bool any_hit4_wave_walk(const SceneView& sv, const Ray& r, float tmin, float tmax,
                                                    bool live, bool nx, bool ny, bool nz, int* wstack,
                                                    TravStats& st) {
@@ -289,6 +292,8 @@ __device__ __forceinline__ bool any_hit4_wave_walk(const SceneView& sv, const Ra
         return hit;
     }
     int sp = 0;
+    unsigned long long own = 0ull;      // kCount: bit e = this lane's ray passed stack entry e's box
+    bool cur_own = true, next_own = false;
     for (;;) {
         node = __builtin_amdgcn_readfirstlane(node);
         // wave-uniform address in the constant address space: scalar loads
@@ -305,7 +310,11 @@ __device__ __forceinline__ bool any_hit4_wave_walk(const SceneView& sv, const Ra
         const float hya[4] = {nf[o4], nf[o4 + 1], nf[o4 + 2], nf[o4 + 3]};
         const float hza[4] = {nf[o5], nf[o5 + 1], nf[o5 + 2], nf[o5 + 3]};
         const int rfa[4] = {ni[24], ni[25], ni[26], ni[27]};
-        if (live) st.nodes++;
+        if (kCount) {
+            if (live && cur_own) st.nodes++;
+        } else if (live) {
+            st.nodes++;
+        }
         // the four child tests first, as straight-line code: every lane evaluates
         // them (idle lanes hold finite rays; empty slots hold finite zero boxes and
         // are masked out below), so the node arrives in one scalar round trip
@@ -339,22 +348,30 @@ __device__ __forceinline__ bool any_hit4_wave_walk(const SceneView& sv, const Ra
                 lm = __builtin_amdgcn_ballot_w64(live);
             } else if (next < 0) {
                 next = ref;
+                if (kCount) next_own = box[c];
             } else {
+                if (kCount && sp < 64) {
+                    const unsigned long long bit = 1ull << sp;
+                    own = box[c] ? (own | bit) : (own & ~bit);
+                }
                 wstack[sp++] = ref;
             }
         }
         if (lm == 0ull) return hit;
         if (next >= 0) {
             node = next;
+            if (kCount) cur_own = next_own;
         } else {
             if (sp == 0) return hit;
             node = wstack[--sp];
+            if (kCount) cur_own = sp < 64 ? ((own >> sp) & 1ull) != 0ull : true;
         }
     }
 }
 
 // Every lane of the wave must call this together; `active` false lanes ride
 // along idle.  `wstack` is this wave's LDS stack of sv.stack4 ints.
+template <bool kCount = false>
 __device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r, float tmin, float tmax,
                                               bool active, int* wstack, TravStats& st) {
     bool live = active && box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]),
@@ -366,8 +383,9 @@ __device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r,
     const unsigned long long my = __builtin_amdgcn_ballot_w64(r.inv_d.y < 0.f) & lm;
     const unsigned long long mz = __builtin_amdgcn_ballot_w64(r.inv_d.z < 0.f) & lm;
     if ((mx == 0ull || mx == lm) && (my == 0ull || my == lm) && (mz == 0ull || mz == lm))
-        return any_hit4_wave_walk<true>(sv, r, tmin, tmax, live, mx != 0ull, my != 0ull, mz != 0ull, wstack, st);
-    return any_hit4_wave_walk<false>(sv, r, tmin, tmax, live, false, false, false, wstack, st);
+        return any_hit4_wave_walk<true, kCount>(sv, r, tmin, tmax, live, mx != 0ull, my != 0ull, mz != 0ull, wstack,
+                                                st);
+    return any_hit4_wave_walk<false, kCount>(sv, r, tmin, tmax, live, false, false, false, wstack, st);
 }
 
 // Segment visibility p -> q (aggregate.cc:84-90): true when blocked.
@@ -379,11 +397,12 @@ __device__ __forceinline__ bool segment_blocked(const SceneView& sv, v3 p, v3 q,
 }
 
 // Segment visibility for a whole wave of segments (see any_hit4_wave)
+template <bool kCount = false>
 __device__ __forceinline__ bool segment_blocked_wave(const SceneView& sv, v3 p, v3 q, float tmin, bool active,
                                                      int* wstack, TravStats& st) {
     Ray r = ray_ctor(p, q - p);
     float tmax = length(q - p) - tmin;
-    return any_hit4_wave(sv, r, tmin, tmax, active, wstack, st);
+    return any_hit4_wave<kCount>(sv, r, tmin, tmax, active, wstack, st);
 }
 
 // Receiver record from a closest hit (triangle.cc:277-296, surface.cc:62-96,

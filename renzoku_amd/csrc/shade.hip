@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kRmBlock) void k_reduced_matrix(SceneView sv, const
             q = ld3(vpl + RZK_VPL_FLOATS * k);
         }
     }
-    bool blocked = segment_blocked_wave(sv, p, q, tmin, active, wstk, st);
+    bool blocked = segment_blocked_wave<kCount>(sv, p, q, tmin, active, wstk, st);
     if (inside) {
         int64_t o = k * rows + r;
         float rv = 0.f, g = 0.f, b = 0.f, vis = 0.f;
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_visibility(SceneView sv, 
             v3 q;
             if (kShared) q = mk(s_q[3 * cc], s_q[3 * cc + 1], s_q[3 * cc + 2]);
             else q = active ? ld3(vpl + (int64_t)sr.rep(sl, c0 + cc) * RZK_VPL_FLOATS) : mk(0.f, 0.f, 0.f);
-            bool blocked = segment_blocked_wave(sv, p, q, tmin, active, wstk, st) || !active;
+            bool blocked = segment_blocked_wave<kCount>(sv, p, q, tmin, active, wstk, st) || !active;
             uint64_t b = __ballot(blocked);
             if (lane == 0 && tile < ntiles) bits[(int64_t)(c0 + cc) * ntiles + tile] = b;
         }
