@@ -1,22 +1,31 @@
-"""Benchmark: one LightSlice gather step on one MI355X.
+"""Benchmark of the hot path on one MI355X.
 
-Workload (the 200k-triangle LightSlice configuration): seeded room of ~200k
-triangles (tessellated boxes and UV spheres, half Lambertian, half Phong),
-512x512 pixels, 65,536 bounce-1 VPLs, 32x32-pixel slices (256 rows), 300
-cluster centres.  A step is the whole hot path with the scene BVH and the VPL
-set resident in HBM: primary closest hits for every pixel, the reduced matrix
-(256 x 65,536 shadow segments + incident radiance), alphas, centre sampling,
-assignment, representatives, per-slice cluster pdfs and luminance maps, and
-the final per-pixel gather over the cluster representatives (one shadow
-segment per (pixel, representative) pair).
+Default run (no --workload): the headline LightSlice step, then compact legs
+for the other configurations, then live rocprofv3 counter passes; one JSON line.
 
+Headline (C2): seeded room of ~200k triangles (tessellated boxes and UV
+spheres, half Lambertian, half Phong), 512x512 pixels, 65,536 bounce-1 VPLs,
+32x32-pixel slices (256 rows), 300 cluster centres.  A step is the whole hot
+path with the scene BVH and the VPL set resident in HBM: primary closest hits
+for every pixel, the reduced matrix (256 x 65,536 shadow segments + incident
+radiance), alphas, centre sampling, assignment, representatives, per-slice
+cluster pdfs and luminance maps, and the final per-pixel gather over the
+cluster representatives (one shadow segment per (pixel, representative) pair).
 value = shadow Mrays/s over the step (reduced-matrix + gather segments / step
-time).  The dominant kernel (the fused gather) is also timed on its own with
-HIP events on the context stream, and its roofline is reported against HBM
-peak with the algorithmic bytes of the traversal byte model (36 B of segment
-I/O, 128 B per visited 4-wide node, 48 B per tested triangle).  cpu_baseline is
-the oracle (a C restatement of the reference's gather) on a bounded sample of
-the same pairs, on the host's cores.
+time).
+
+Roofline: the gather (k_slice_visibility + k_slice_shade) is timed with HIP
+events on the context stream and is bound by vector issue, so its roofline is
+SQ_INSTS_VALU x 64 lanes per launch (a live rocprofv3 counter pass of the same
+build and inputs) / launch time against 256 CUs x 64 lanes x 2.4 GHz; its HBM
+traffic is FETCH_SIZE + WRITE_SIZE of the same launch, each scaled by what the
+counter reads for a 512 MiB device copy.  cpu_baseline: the whole step on the
+oracle (a C restatement of the reference) on every usable host core.
+
+Legs: C1 plain VPL render (Cornell box, 256x256, 1,024 VPLs), C3 guided path
+tracing (1M triangles, 1280x720, 16 spp, depth 5) with an HBM roofline over the
+traversal byte model, C4 batched jobs and C5 served requests against the plain
+per-request drop-in.
 """
 from __future__ import annotations
 
@@ -33,12 +42,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per gather launch (k_slice_visibility + k_slice_shade) measured by rocprofv3 --pmc FETCH_SIZE
-# (x2, the 512 MiB copy calibration) and WRITE_SIZE (x1) on this workload; the passes and the calibration
-# are recorded in profiles/slice_gather_pmc.md
-GATHER_HBM_BYTES = (30397 + 49212) * 2 * 1024 + (19206 + 3072) * 1024
-# MI355X vector issue: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz lane-operations per second
+# MI355X vector issue: 256 CUs x 4 SIMDs, each retiring one wave64 VALU instruction per 4 cycles, at
+# 2.4 GHz: 256 x 64 lane-operations per cycle
 VALU_LANE_OPS_PEAK = 256 * 64 * 2.4e9
+# counter passes of the live PMC leg (one rocprofv3 run each; at most 8 SQ / 4 TCC counters per pass)
+PMC_PASSES = [["SQ_INSTS_VALU", "SQ_WAVES"], ["VALUUtilization", "VALUBusy"], ["FETCH_SIZE"], ["WRITE_SIZE"]]
+# kernels whose per-launch counters the PMC child reports (substring of the kernel name)
+# (substring of the kernel name, which dispatch): the PMC child runs the C2 step first (its reduced matrix
+# is the first one), then the C1 gather, then C3 (whose guide build has a reduced matrix of its own), then
+# the calibration copy (the last copy dispatch)
+PMC_KERNELS = {"slice_visibility": ("k_slice_visibility", "first"), "slice_shade": ("k_slice_shade", "first"),
+               "reduced_matrix": ("k_reduced_matrix", "first"), "visibility": ("k_visibility", "first"),
+               "gather_shade": ("k_gather_shade", "first"), "guided_pt": ("k_guided_pt", "first"),
+               "copy": ("copyBuffer", "last")}
+PMC_COPY_MIB = 512
 # algorithmic bytes per shadow segment: ray in, result out, every 4-wide node
 # fetched (128 B: four child boxes + refs) and every triangle record tested
 RAY_IN_BYTES, SHADOW_OUT_BYTES, NODE4_BYTES, TRI_BYTES = 32, 4, 128, 48
@@ -72,6 +89,166 @@ class HipEvents:
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """(model name, usable cores): the CPU affinity set, capped by a cgroup CPU quota if there is one"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            cores = max(1, min(cores, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return model, cores
+
+
+def _pmc_parse(outdir):
+    """{kernel key: {counter: value}} for the last dispatch of each PMC_KERNELS key, from the
+    counter-collection CSVs rocprofv3 wrote under outdir (values of one dispatch summed)"""
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection*.csv"), recursive=True):
+        with open(path, newline="") as f:
+            rows.extend(csv.DictReader(f))
+    if not rows:
+        return {}
+    keys = {k.lower(): k for k in rows[0].keys()}
+    kn, cn, cv = keys.get("kernel_name"), keys.get("counter_name"), keys.get("counter_value")
+    di = keys.get("dispatch_id") or keys.get("correlation_id")
+    ts, te = keys.get("start_timestamp"), keys.get("end_timestamp")
+    disp = {}
+    for r in rows:
+        d = disp.setdefault(int(r[di]), {"name": r[kn], "c": {}})
+        d["c"][r[cn]] = d["c"].get(r[cn], 0.0) + float(r[cv])
+        if ts and te and r.get(ts) and r.get(te):
+            d["ns"] = float(r[te]) - float(r[ts])
+    out = {}
+    for key, (sub, which) in PMC_KERNELS.items():
+        ids = [i for i, d in disp.items() if sub in d["name"]]
+        if ids:
+            d = disp[min(ids) if which == "first" else max(ids)]
+            out[key] = dict(d["c"])
+            if "ns" in d:
+                out[key]["_ns"] = d["ns"]
+    return out
+
+
+def pmc_collect(child_args, deadline):
+    """Runs this file as a child under `rocprofv3 --pmc`, one counter pass at a time (PMC_PASSES), each
+    child launching every measured kernel once and then copying PMC_COPY_MIB MiB device to device (the
+    known-size transfer that scales FETCH_SIZE / WRITE_SIZE).  Returns ({kernel key: {counter: value}},
+    note).  A pass that overruns its time limit is killed with its process group."""
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return {}, "rocprofv3 not found"
+    merged, notes = {}, []
+    for counters in PMC_PASSES:
+        left = deadline - time.perf_counter()
+        if left < 60:
+            notes.append(f"pass {'+'.join(counters)} skipped (time budget)")
+            continue
+        out = tempfile.mkdtemp(prefix="rzk_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+        cmd = [exe, "--pmc", *counters, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+               sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", *child_args]
+        log(f"[bench] PMC pass {'+'.join(counters)} ...")
+        t0 = time.perf_counter()
+        proc = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, start_new_session=True, cwd=out)
+        try:
+            rc = proc.wait(timeout=min(180.0, left))
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.wait()
+            notes.append(f"pass {'+'.join(counters)} killed at its time limit")
+            shutil.rmtree(out, ignore_errors=True)
+            break
+        got = _pmc_parse(out)
+        shutil.rmtree(out, ignore_errors=True)
+        if rc != 0 and not got:
+            notes.append(f"pass {'+'.join(counters)} exited {rc}")
+            continue
+        for k, c in got.items():
+            merged.setdefault(k, {}).update(c)
+        log(f"[bench] PMC pass {'+'.join(counters)}: {len(got)} kernels in {time.perf_counter() - t0:.1f} s")
+    return merged, "; ".join(notes)
+
+
+def pmc_traffic(pmc, keys):
+    """HBM bytes of one launch of the kernels `keys`: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
+    scaled by what it reads for the PMC_COPY_MIB MiB copy; None without both passes"""
+    cp = pmc.get("copy", {})
+    if "FETCH_SIZE" not in cp or "WRITE_SIZE" not in cp or cp["FETCH_SIZE"] <= 0 or cp["WRITE_SIZE"] <= 0:
+        return None, None
+    kib = PMC_COPY_MIB * 1024.0
+    ff, fw = kib / cp["FETCH_SIZE"], kib / cp["WRITE_SIZE"]
+    total = 0.0
+    for k in keys:
+        c = pmc.get(k, {})
+        if "FETCH_SIZE" not in c or "WRITE_SIZE" not in c:
+            return None, None
+        total += (c["FETCH_SIZE"] * ff + c["WRITE_SIZE"] * fw) * 1024.0
+    return total, {"fetch_scale": round(ff, 4), "write_scale": round(fw, 4)}
+
+
+def valu_roofline(pmc, keys, launch_ms, kernel_name, byte_model=None):
+    """roofline of a vector-issue-bound launch: SQ_INSTS_VALU x 64 lanes per launch / launch time against
+    VALU_LANE_OPS_PEAK; measured HBM traffic beside it"""
+    insts = [pmc.get(k, {}).get("SQ_INSTS_VALU") for k in keys]
+    traffic, scales = pmc_traffic(pmc, keys)
+    r = {"bound": "valu", "unit": "T VALU lane-ops/s", "peak": round(VALU_LANE_OPS_PEAK / 1e12, 3),
+         "achieved": None, "frac": None, "traffic": None if traffic is None else int(traffic),
+         "kernel": kernel_name, "launch_ms": round(launch_ms, 4)}
+    if all(v is not None for v in insts):
+        ops = sum(insts) * 64.0
+        r["achieved"] = round(ops / (launch_ms / 1e3) / 1e12, 3)
+        r["frac"] = round(ops / (launch_ms / 1e3) / VALU_LANE_OPS_PEAK, 4)
+        r["sq_insts_valu"] = {k: int(pmc[k]["SQ_INSTS_VALU"]) for k in keys}
+    for k in keys:
+        for c in ("VALUUtilization", "VALUBusy"):
+            if c in pmc.get(k, {}):
+                r.setdefault(c.lower(), {})[k] = round(pmc[k][c], 2)
+    if traffic is not None:
+        r["hbm_gbs"] = round(traffic / (launch_ms / 1e3) / 1e9, 2)
+        r["hbm_frac"] = round(traffic / (launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+        r["traffic_scales"] = scales
+    if byte_model is not None:
+        r["byte_model"] = byte_model
+    return r
+
+
+def hbm_roofline(pmc, keys, launch_ms, algo_bytes, kernel_name, note):
+    """roofline of a traversal launch against HBM peak: algorithmic bytes (the traversal byte model over
+    each lane's own counted work) per launch / launch time; measured traffic beside it"""
+    traffic, scales = pmc_traffic(pmc, keys)
+    achieved = algo_bytes / (launch_ms / 1e3) / 1e9
+    r = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None if traffic is None else int(traffic),
+         "kernel": kernel_name, "launch_ms": round(launch_ms, 4), "bytes_per_launch": int(algo_bytes), "note": note}
+    if traffic is not None:
+        r["measured_gbs"] = round(traffic / (launch_ms / 1e3) / 1e9, 2)
+        r["traffic_scales"] = scales
+    insts = [pmc.get(k, {}).get("SQ_INSTS_VALU") for k in keys]
+    if all(v is not None for v in insts):
+        r["valu_frac"] = round(sum(insts) * 64.0 / (launch_ms / 1e3) / VALU_LANE_OPS_PEAK, 4)
+    for k in keys:
+        if "VALUUtilization" in pmc.get(k, {}):
+            r.setdefault("valuutilization", {})[k] = round(pmc[k]["VALUUtilization"], 2)
+    return r
 
 
 def cpu_baseline(sc, ls, vpl, vmat, reps, rep_w, params, target_s=8.0):
@@ -376,43 +553,37 @@ def run_kdslice(args):
     print(json.dumps(out), flush=True)
 
 
-def c3_scene_and_guides(args, levels):
-    """the 1M-triangle instanced scene at 1280x720 with one LightSlice guide per
-    VPL count in `levels` (16x16-pixel slices, 1,000 clusters each)"""
+def guides_for(ctx, sc, scene, levels):
+    """one LightSlice guide per VPL count in `levels` on a 1280x720 scene (16x16-pixel slices, up to
+    1,000 clusters each)"""
     import renzoku_amd as rz
-    from renzoku_amd import scenes, vplgen
-    W, H = 1280, 720
-    t0 = time.perf_counter()
-    sc = scenes.instanced_meshes(W, H, seed=2013, target_tris=args.tris or 1_000_000)
-    ctx = rz.Ctx(0)
-    scene = rz.Scene(ctx, sc)
-    log(f"[bench] scene: {sc.ntri} triangles, BVH depth {scene.info().depth} ({time.perf_counter() - t0:.1f} s)")
+    from renzoku_amd import vplgen
     tracer, filler = vplgen.gpu_tracer(scene)
     guides = []
     for nv in levels:
         vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, nv, seed=2013, batch=65536)
         vp = rz.Vpls(ctx, vpl, vmat)
-        ls = rz.LightSlice(ctx, scene, vp, width=W, height=H, tile=16, num_clusters=min(1000, nv // 4),
+        ls = rz.LightSlice(ctx, scene, vp, width=1280, height=720, tile=16, num_clusters=min(1000, nv // 4),
                            num_neighbor_slices=16, map_resolution=32, n_emitted=n_emitted, seed=2013)
         t1 = time.perf_counter()
         ls.initialize(sc.camera)
         ctx.synchronize()
         log(f"[bench] guide for {nv} VPLs: {ls.info().num_clusters} clusters, {(time.perf_counter() - t1) * 1e3:.0f} ms")
         guides.append((nv, vp, ls, ls.guide(sc.camera)))
-    return sc, ctx, scene, guides
+    return guides
 
 
-def run_batched(args):
-    """256 render jobs of uneven size on the 1M-triangle scene: windows 32..256 px
-    a side, 1-64 spp, guides from 4k-100k VPLs; all jobs in one persistent,
-    graph-launched kernel with the autotuned launch shape, against the plain
-    per-request drop-in (one rzk_lightslice_render_guided per job)."""
+BATCH_LEVELS = [4096, 8192, 16384, 32768, 65536, 100_000]
+
+
+def measure_batched(args, sc, ctx, scene, guides, njobs, tune_max, steps):
+    """njobs render jobs of uneven size on a 1280x720 scene: windows 32..256 px a side, 1-64 spp, guides
+    from the given VPL levels; all jobs in one persistent, graph-launched kernel with the autotuned launch
+    shape, against the plain per-request drop-in (one rzk_lightslice_render_guided per job)."""
     import renzoku_amd as rz
-    levels = [4096, 8192, 16384, 32768, 65536, 100_000]
-    sc, ctx, scene, guides = c3_scene_and_guides(args, levels)
     rng = np.random.default_rng(2013)
     specs = []
-    for i in range(args.jobs):
+    for i in range(njobs):
         w, h = int(rng.integers(32, 257)), int(rng.integers(32, 257))
         nv, vp, ls, g = guides[int(rng.integers(0, len(guides)))]
         specs.append(dict(guide=g, ls=ls, vpls=nv, window=(int(rng.integers(0, 1280 - w + 1)),
@@ -422,17 +593,19 @@ def run_batched(args):
     paths = sum(s["window"][2] * s["window"][3] * s["spp"] for s in specs)
     log(f"[bench] {len(specs)} jobs, {paths / 1e6:.1f} M paths")
     b = rz.Batch(ctx, scene)
-    tune = [dict(s, spp=max(1, s["spp"] // 4)) for s in specs[:: max(1, len(specs) // 48)]]
-    t0 = time.perf_counter()
-    best, tried = b.autotune(tune)
-    log(f"[bench] autotune: {len(tried)} candidates in {time.perf_counter() - t0:.1f} s -> {best}")
+    best, tried = None, []
+    if tune_max > 0:
+        tune = [dict(s, spp=max(1, s["spp"] // 4)) for s in specs[:: max(1, len(specs) // 48)]]
+        t0 = time.perf_counter()
+        best, tried = b.autotune(tune, max_tried=tune_max)
+        log(f"[bench] autotune: {len(tried)} candidates in {time.perf_counter() - t0:.1f} s -> {best}")
     _, rays, _ = b.render(specs, to_host=False, count_rays=True)
     ms = []
-    for i in range(args.steps):
+    for i in range(steps):
         t0 = time.perf_counter()
         _, _, dev_ms = b.render(specs, to_host=False)
         ms.append((time.perf_counter() - t0) * 1e3)
-        log(f"[bench] batched step {i + 1}/{args.steps}: {ms[-1]:.1f} ms (device {dev_ms:.1f} ms)")
+        log(f"[bench] batched step {i + 1}/{steps}: {ms[-1]:.1f} ms (device {dev_ms:.1f} ms)")
     batched_ms = float(np.mean(ms))
     # plain per-request drop-in: one launch per job, back to back
     ctx.synchronize()
@@ -456,18 +629,31 @@ def run_batched(args):
         assert np.array_equal(imgs[k].view(np.uint32), ref[y0:y0 + h, x0:x0 + w].view(np.uint32)), k
         checked += 1
     nrays = rays["closest_rays"] + rays["shadow_rays"]
-    out = {"metric": "batched image-requests/s (guided path tracing jobs)",
-           "value": round(len(specs) / (batched_ms / 1e3), 3), "unit": "requests/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": 1, "ms_per_step": round(batched_ms, 1), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-           "data": "synthetic (seeded instanced-mesh scene, seeded light-traced VPLs, seeded job mix)",
-           "config": {"workload": f"batched {len(specs)} guided path tracing jobs on 1M tris, windows 32..256 px, "
-                                  "1-64 spp, depth 5, guides from 4k-100k VPLs", "triangles": int(sc.ntri),
-                      "jobs": len(specs), "paths": int(paths), "vpl_levels": levels, "parallelism": "single GPU"},
-           "plain_requests_per_s": round(len(specs) / (plain_ms / 1e3), 3), "plain_ms": round(plain_ms, 1),
-           "speedup_vs_plain": round(plain_ms / batched_ms, 2), "mrays_per_s": round(nrays / batched_ms / 1e3, 1),
-           "rays": rays, "autotuned": best, "autotune_trials": [[c, round(t, 2)] for c, t in tried],
-           "accuracy": f"bit-equal to the plain render on {checked} sampled jobs"}
+    return {"metric": "batched image-requests/s (guided path tracing jobs)",
+            "value": round(len(specs) / (batched_ms / 1e3), 3), "unit": "requests/s", "n_gpus": 1,
+            "steps": steps, "warmup": 1, "ms_per_step": round(batched_ms, 1), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded instanced-mesh scene, seeded light-traced VPLs, seeded job mix)",
+            "config": {"workload": f"batched {len(specs)} guided path tracing jobs on {sc.ntri} tris, windows "
+                                   "32..256 px, 1-64 spp, depth 5, guides from 4k-100k VPLs", "triangles": int(sc.ntri),
+                       "jobs": len(specs), "paths": int(paths), "vpl_levels": [g[0] for g in guides],
+                       "parallelism": "single GPU"},
+            "plain_requests_per_s": round(len(specs) / (plain_ms / 1e3), 3), "plain_ms": round(plain_ms, 1),
+            "speedup_vs_plain": round(plain_ms / batched_ms, 2), "mrays_per_s": round(nrays / batched_ms / 1e3, 1),
+            "rays": rays, "autotuned": best, "autotune_trials": [[c, round(t, 2)] for c, t in tried],
+            "accuracy": f"bit-equal to the plain render on {checked} sampled jobs", "_paths": int(paths)}
+
+
+def run_batched(args):
+    """256 render jobs of uneven size on the 1M-triangle scene (see measure_batched)"""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes
+    sc = scenes.instanced_meshes(1280, 720, seed=2013, target_tris=args.tris or 1_000_000)
+    ctx = rz.Ctx(0)
+    scene = rz.Scene(ctx, sc)
+    guides = guides_for(ctx, sc, scene, BATCH_LEVELS)
+    out = measure_batched(args, sc, ctx, scene, guides, args.jobs, 64, args.steps)
+    out.pop("_paths")
     print(json.dumps(out), flush=True)
 
 
@@ -480,42 +666,46 @@ def random_camera(rng, w, h):
     return scenes.camera(eye, look, (0, 1, 0), float(rng.uniform(45, 75)), w, h)
 
 
-def run_served(args):
-    """Poisson arrivals of render requests (square images 128..1024 px a side,
-    1-32 spp, depth 5, random camera poses) against a 2M-triangle scene with
-    200k VPLs whose 1,000 clusters stay resident; the server admits by memory
-    budget and batches continuously.  The same requests, arrival times and
-    images are then served by the plain per-request drop-in (guide build +
-    render_guided, one request at a time, first come first served)."""
-    import threading
+def setup_c5(args, ctx):
+    """the 2M-triangle served scene with 200k VPLs whose 1,000 clusters stay resident"""
     import renzoku_amd as rz
     from renzoku_amd import scenes, vplgen
     t0 = time.perf_counter()
-    sc = scenes.instanced_meshes(1280, 720, seed=2013, target_tris=args.tris or 2_000_000)
-    ctx = rz.Ctx(0)
+    sc = scenes.instanced_meshes(1280, 720, seed=2013, target_tris=args.tris5 or 2_000_000)
     scene = rz.Scene(ctx, sc)
-    log(f"[bench] scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
+    log(f"[bench] C5 scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
     tracer, filler = vplgen.gpu_tracer(scene)
-    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 200_000, seed=2013, batch=65536)
+    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls5 or 200_000, seed=2013, batch=65536)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, scene, vp, width=1280, height=720, tile=32, num_clusters=1000, num_neighbor_slices=16,
                        n_emitted=n_emitted, seed=2013)
     ls.initialize(sc.camera)
     clusters = ls.clusters()
+    log(f"[bench] C5: {len(vmat)} VPLs, {ls.info().num_clusters} resident clusters")
+    return dict(sc=sc, scene=scene, vp=vp, ls=ls, clusters=clusters, nvpl=len(vmat))
+
+
+def measure_served(args, ctx, c5, nreq, rate, seed):
+    """Poisson arrivals (rate per second) of nreq render requests (square images 128..1024 px a side, 1-32
+    spp, depth 5, random camera poses) against the resident scene, VPLs and clusters; the server admits by
+    memory budget and batches continuously.  The same requests, arrival times and images are then served
+    by the plain per-request drop-in (guide build + render_guided, one request at a time, first come
+    first served)."""
+    import threading
+    import renzoku_amd as rz
+    sc, scene, clusters = c5["sc"], c5["scene"], c5["clusters"]
     gparams = dict(tile=16, map_resolution=32, map_epsilon=1e-3, clamp_mode=rz._lib.CLAMP_FORM_FACTOR,
                    incident_clamp=0.01, tmin=1e-3, map_gate=0)
-    log(f"[bench] {len(vmat)} VPLs, {ls.info().num_clusters} resident clusters")
-    rng = np.random.default_rng(args.seed)
+    rng = np.random.default_rng(seed)
     reqs = []
-    for i in range(args.requests):
+    for i in range(nreq):
         side = int(rng.integers(128, 1025))
         reqs.append((random_camera(rng, side, side), dict(spp=int(rng.integers(1, 33)), max_bounce=5,
                                                           direct_lighting=0, guide_mode=1,
                                                           seed=int(rng.integers(0, 2 ** 31)))))
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, len(reqs)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
     paths = sum(int(c[17]) * int(c[18]) * kw["spp"] for c, kw in reqs)
-    log(f"[bench] {len(reqs)} requests, {paths / 1e6:.1f} M paths, arrivals over {arrivals[-1]:.2f} s "
-        f"at {args.rate}/s")
+    log(f"[bench] {len(reqs)} requests, {paths / 1e6:.1f} M paths, arrivals over {arrivals[-1]:.2f} s at {rate}/s")
     # warm the kernels once
     g = rz.Guide.create(ctx, scene, clusters, reqs[0][0], gparams)
     g.render(spp=1, max_bounce=5, window=(0, 0, 64, 64))
@@ -567,26 +757,469 @@ def run_served(args):
         if (i + 1) % 16 == 0:
             log(f"[bench] plain {i + 1}/{len(reqs)}: {end:.1f} s")
     plain_span = busy_until - arrivals[0]
-    pct = lambda a, q: round(float(np.percentile(np.asarray(a) * 1e3, q)), 1)
-    out = {"metric": "served image-requests/s with p50/p99 latency",
-           "value": round(len(reqs) / served_span, 3), "unit": "requests/s", "n_gpus": 1,
-           "steps": len(reqs), "warmup": 1, "ms_per_step": round(served_span * 1e3 / len(reqs), 1),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-           "data": "synthetic (seeded instanced-mesh scene, seeded VPLs, seeded Poisson request stream)",
-           "config": {"workload": f"served guided path tracing, Poisson {args.rate}/s, images 128..1024 px square, "
-                                  "1-32 spp, depth 5, random cameras, 2M tris, 200k VPLs, 1000 resident clusters",
-                      "triangles": int(sc.ntri), "vpls": int(len(vmat)), "requests": len(reqs),
-                      "paths": int(paths), "rate_per_s": args.rate, "memory_budget_gb": args.budget_gb,
-                      "step_iterations": args.step_iterations, "parallelism": "single GPU"},
-           "latency_ms": {"p50": pct(s_lat, 50), "p99": pct(s_lat, 99), "max": pct(s_lat, 100)},
-           "plain_requests_per_s": round(len(reqs) / plain_span, 3),
-           "plain_latency_ms": {"p50": pct(p_lat, 50), "p99": pct(p_lat, 99), "max": pct(p_lat, 100)},
-           "speedup_vs_plain": round(plain_span / served_span, 2),
-           "paths_per_s": round(paths / served_span, 1),
-           "server": {k: sstats[k] for k in ("rounds", "items", "peak_bytes", "busy_seconds")},
-           "accuracy": f"{len(reqs) - mismatches} of {len(reqs)} images bit-equal to the plain drop-in"}
-    print(json.dumps(out), flush=True)
+    pct = lambda a, q: round(float(np.percentile(np.asarray(a) * 1e3, q)), 1)  # noqa: E731
+    return {"metric": "served image-requests/s with p50/p99 latency",
+            "value": round(len(reqs) / served_span, 3), "unit": "requests/s", "n_gpus": 1,
+            "steps": len(reqs), "warmup": 1, "ms_per_step": round(served_span * 1e3 / len(reqs), 1),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded instanced-mesh scene, seeded VPLs, seeded Poisson request stream)",
+            "config": {"workload": f"served guided path tracing, Poisson {rate}/s, images 128..1024 px square, "
+                                   f"1-32 spp, depth 5, random cameras, {sc.ntri} tris, {c5['nvpl']} VPLs, 1000 "
+                                   "resident clusters",
+                       "triangles": int(sc.ntri), "vpls": int(c5["nvpl"]), "requests": len(reqs),
+                       "paths": int(paths), "rate_per_s": rate, "memory_budget_gb": args.budget_gb,
+                       "step_iterations": args.step_iterations, "parallelism": "single GPU"},
+            "latency_ms": {"p50": pct(s_lat, 50), "p99": pct(s_lat, 99), "max": pct(s_lat, 100)},
+            "plain_requests_per_s": round(len(reqs) / plain_span, 3),
+            "plain_latency_ms": {"p50": pct(p_lat, 50), "p99": pct(p_lat, 99), "max": pct(p_lat, 100)},
+            "speedup_vs_plain": round(plain_span / served_span, 2),
+            "paths_per_s": round(paths / served_span, 1),
+            "server": {k: sstats[k] for k in ("rounds", "items", "peak_bytes", "busy_seconds")},
+            "accuracy": f"{len(reqs) - mismatches} of {len(reqs)} images bit-equal to the plain drop-in",
+            "_paths": int(paths)}
 
+
+def run_served(args):
+    """the served workload at each rate of --rates (see measure_served), one JSON line per rate"""
+    import renzoku_amd as rz
+    ctx = rz.Ctx(0)
+    c5 = setup_c5(args, ctx)
+    for rate in [float(r) for r in args.rates.split(",")]:
+        out = measure_served(args, ctx, c5, args.requests, rate, args.seed)
+        out.pop("_paths")
+        print(json.dumps(out), flush=True)
+
+
+# --------------------------------------------------------------------------- C2 (the headline)
+C2_DESC = "lightslice-gather room 200k tris 512x512 65536 VPLs 32x32 slices 300 clusters"
+
+
+def setup_c2(args):
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    W = H = 512
+    params = dict(width=W, height=H, tile=32, num_clusters=300, num_neighbor_slices=16, map_resolution=32,
+                  map_epsilon=1e-3, clamp_mode=rz._lib.CLAMP_FORM_FACTOR, clamp=0.01, incident_clamp=0.01,
+                  tmin=1e-3, seed=2013, map_gate=0, kd_max_slice_size=args.kd)
+    t0 = time.perf_counter()
+    sc = scenes.room(W, H, seed=2013, target_tris=args.tris or 200_000)
+    ctx = rz.Ctx(0)
+    scene = rz.Scene(ctx, sc)
+    log(f"[bench] C2 scene: {sc.ntri} triangles, {scene.info().num_interior_nodes} interior nodes "
+        f"({time.perf_counter() - t0:.1f} s)")
+    tracer, filler = vplgen.gpu_tracer(scene)
+    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 65_536, seed=2013, batch=65536)
+    params["n_emitted"] = n_emitted
+    vpls = rz.Vpls(ctx, vpl, vmat)
+    log(f"[bench] C2 VPLs: {len(vmat)} from {n_emitted:.0f} light samples")
+    ls = rz.LightSlice(ctx, scene, vpls, **params)
+    return dict(sc=sc, ctx=ctx, scene=scene, vpl=vpl, vmat=vmat, vpls=vpls, ls=ls, params=params)
+
+
+def cpu_baseline_c2(c2, lsinf, cores, model):
+    """the whole LightSlice step on the oracle (a C restatement of the reference), all usable host cores:
+    primary hits of every pixel, the reduced matrix of the slice rows, alphas + centre sampling +
+    assignment + representatives, and the final gather over the representatives; plus the gather on one
+    thread over a seeded sample of pixels"""
+    import oracle
+    from renzoku_amd import scenes
+    sc, ls, vpl, vmat, prm = c2["sc"], c2["ls"], c2["vpl"], c2["vmat"], c2["params"]
+    W, H = prm["width"], prm["height"]
+    osc = oracle.OrcScene(sc)
+    parts = {}
+    t_all = time.perf_counter()
+    t0 = time.perf_counter()
+    px, py = scenes.pixel_grid(W, H)
+    rays = oracle.camera_rays(sc.camera, px, py)
+    t, b, g, tri = osc.trace_closest(rays)
+    recv, m, _ = osc.fill_receivers(rays, t, b, g, tri)
+    parts["primary_s"] = time.perf_counter() - t0
+    row_pix = ls.get("row_pix", np.int32, lsinf.num_slices)
+    rows_recv, rows_mat = recv[row_pix], m[row_pix]
+    t0 = time.perf_counter()
+    R, _ = osc.reduced_matrix(rows_recv, rows_mat, vpl, vmat, prm["clamp_mode"], prm["incident_clamp"], prm["tmin"],
+                              nthreads=cores)
+    parts["reduced_matrix_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cl = oracle.cluster_by_sampling(R, prm["num_clusters"], prm["seed"], nthreads=cores)
+    parts["clustering_s"] = time.perf_counter() - t0
+    reps, rep_w = cl["reps"], cl["rep_w"]
+    vsub = vpl[reps].copy()
+    vsub[:, 18:21] = vpl[reps, 18:21] * rep_w[:, None]
+    vm = vmat[reps].copy()
+    t0 = time.perf_counter()
+    osc.gather(recv, m, vsub, vm, prm["clamp_mode"], prm["clamp"], prm["tmin"], prm["n_emitted"], nthreads=cores)
+    parts["gather_s"] = time.perf_counter() - t0
+    step_s = time.perf_counter() - t_all
+    lit = int((m >= 0).sum())
+    rows_ok = int((rows_mat >= 0).sum())
+    segs = rows_ok * len(vmat) + lit * len(reps)
+    # one thread: the gather over a seeded pixel sample, about 5 s
+    rng = np.random.default_rng(1)
+    ok = np.nonzero(m >= 0)[0]
+    n1 = 256
+    while True:
+        sel = np.sort(rng.choice(ok, min(n1, len(ok)), replace=False))
+        t0 = time.perf_counter()
+        osc.gather(recv[sel], m[sel], vsub, vm, prm["clamp_mode"], prm["clamp"], prm["tmin"], prm["n_emitted"],
+                   nthreads=1)
+        dt1 = time.perf_counter() - t0
+        if dt1 >= 2.5 or n1 >= len(ok):
+            break
+        n1 = int(min(len(ok), n1 * max(2.0, 5.0 / max(dt1, 1e-3))))
+    return {"value": round(segs / step_s / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "cpu": model,
+            "sample": f"the whole step on the oracle: {W * H} primary hits, reduced matrix {rows_ok} rows x "
+                      f"{len(vmat)} VPLs, clustering into {prm['num_clusters']}, gather {lit} pixels x {len(reps)} "
+                      f"representatives ({segs} shadow segments) on {cores} threads, {step_s:.1f} s",
+            "step_s": round(step_s, 3), "parts_s": {k: round(v, 3) for k, v in parts.items()},
+            "one_thread": {"value": round(len(sel) * len(reps) / dt1 / 1e6, 3), "unit": "Mrays/s",
+                           "sample": f"oracle gather of {len(sel)} seeded pixels x {len(reps)} representatives on "
+                                     f"one thread, {dt1:.1f} s"}}
+
+
+def measure_c2(args, c2):
+    import renzoku_amd as rz  # noqa: F401
+    ctx, ls, sc = c2["ctx"], c2["ls"], c2["sc"]
+    ev = HipEvents()
+    stream = ctx.stream
+    e0, e1 = ev.create(), ev.create()
+    for i in range(args.warmup):
+        ls.initialize(sc.camera)
+        ls.gather(to_host=False)
+        ctx.synchronize()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    work = ls.count_work()
+    lsinf = ls.info()
+    log(f"[bench] work per step: {work}")
+    ctx.synchronize()
+    gather_ms = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        ls.initialize(sc.camera)
+        ev.record(e0, stream)
+        ls.gather(to_host=False)
+        ev.record(e1, stream)
+        gather_ms.append(ev.elapsed_ms(e0, e1))
+    ctx.synchronize()
+    total_s = time.perf_counter() - t_start
+    ms_per_step = total_s * 1e3 / args.steps
+    g_ms = float(np.mean(gather_ms))
+    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, gather kernel {g_ms:.3f} ms")
+    shadow_rays = work["reduced_pairs"] + work["gather_pairs"]
+    value = shadow_rays / (ms_per_step / 1e3) / 1e6
+    out = {
+        "metric": "shadow Mrays/s + VPL-gather Mpairs/s; image req/s + p50/p99 latency vs plain",
+        "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded room scene, seeded light-traced VPLs)",
+        "config": {"workload": (C2_DESC if not args.kd else
+                                f"lightslice-gather room 200k tris 512x512 65536 VPLs kd-tree slices of {args.kd} "
+                                "points 300 clusters"),
+                   "triangles": int(sc.ntri), "width": 512, "height": 512, "vpls": int(len(c2["vmat"])),
+                   "slices": int(lsinf.num_slices), "clusters": int(lsinf.num_clusters),
+                   "parallelism": "single GPU"},
+        "gather_mpairs_per_s": round(work["gather_pairs"] / (g_ms / 1e3) / 1e6, 3),
+        "step_mpairs_per_s": round(shadow_rays / (ms_per_step / 1e3) / 1e6, 3),
+        "gather_kernel_ms": round(g_ms, 4), "work": work}
+    # the traversal byte model over each lane's own work (36 B segment I/O, 128 B per 4-wide node reached,
+    # 48 B per triangle tested); node fetches are shared by the 64 lanes of a wave (scalar loads) and served
+    # by L2 / MALL, so this is logical traffic, far above what reaches HBM
+    gbytes = (work["gather_pairs"] * (RAY_IN_BYTES + SHADOW_OUT_BYTES) + work["gather_nodes"] * NODE4_BYTES +
+              work["gather_tris"] * TRI_BYTES)
+    out["_byte_model"] = {"bytes_per_launch": int(gbytes), "gbs": round(gbytes / (g_ms / 1e3) / 1e9, 1),
+                          "nodes_per_pair": round(work["gather_nodes"] / max(1, work["gather_pairs"]), 3),
+                          "tris_per_pair": round(work["gather_tris"] / max(1, work["gather_pairs"]), 3)}
+    return out, lsinf, g_ms
+
+
+# --------------------------------------------------------------------------- C1 (plain VPL render)
+def setup_c1(ctx):
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    sc = scenes.cornell_box(256, 256)
+    scene = rz.Scene(ctx, sc)
+    tracer, filler = vplgen.gpu_tracer(scene)
+    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 1024, seed=2013, batch=4096)
+    vpls = rz.Vpls(ctx, vpl, vmat)
+    px, py = scenes.pixel_grid(256, 256)
+    _, _, recv, m, _ = scene.camera_primary(sc.camera, px, py)
+    dev = []
+    for a in (np.ascontiguousarray(recv, np.float32), np.ascontiguousarray(m, np.int32),
+              np.zeros((len(m), 3), np.float32)):
+        p = C.c_void_p()
+        ctx.check(ctx.lib.rzk_device_alloc(ctx.h, a.nbytes, C.byref(p)), "rzk_device_alloc")
+        ctx.check(ctx.lib.rzk_copy_h2d(ctx.h, p, a.ctypes.data_as(C.c_void_p), a.nbytes), "rzk_copy_h2d")
+        dev.append(p)
+    return dict(sc=sc, scene=scene, vpl=vpl, vmat=vmat, ne=ne, vpls=vpls, recv=recv, m=m, dev=dev)
+
+
+def c1_gather_dev(ctx, c1):
+    """the gather of every pixel's receiver over the 1,024 VPLs, device resident (no allocation)"""
+    d_recv, d_mat, d_out = c1["dev"]
+    ctx.check(ctx.lib.rzk_gather_dev_noalloc(ctx.h, c1["scene"].h, c1["vpls"].h, d_recv, d_mat, len(c1["m"]), 1,
+                                             0.01, 1e-3, c1["ne"], d_out), "rzk_gather_dev_noalloc")
+
+
+def measure_c1(args, ctx, pmc_hook, cores, model):
+    """Plain VPL render of the seeded Cornell box: 256x256 gather points at pixel centres, 1,024 bounce-1
+    VPLs, form-factor clamp 0.01.  A step is one rzk_render_vpl (primary hits + the fused gather, image
+    left on the device); the gather kernels alone (k_visibility + k_gather_shade) are timed with HIP
+    events around the device-resident no-allocation gather of the same receivers."""
+    c1 = setup_c1(ctx)
+    sc, cam = c1["sc"], np.ascontiguousarray(c1["sc"].camera, np.float32)
+    render = lambda: ctx.check(ctx.lib.rzk_render_vpl(ctx.h, c1["scene"].h, c1["vpls"].h,  # noqa: E731
+                                                      cam.ctypes.data_as(C.c_void_p), 1, 0.01, 1e-3, c1["ne"], None),
+                               "rzk_render_vpl")
+    for _ in range(2):
+        render()
+        c1_gather_dev(ctx, c1)
+    ctx.synchronize()
+    steps = max(3, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        render()
+    ctx.synchronize()
+    render_ms = (time.perf_counter() - t0) * 1e3 / steps
+    ev = HipEvents()
+    e0, e1 = ev.create(), ev.create()
+    g = []
+    for _ in range(steps):
+        ev.record(e0, ctx.stream)
+        c1_gather_dev(ctx, c1)
+        ev.record(e1, ctx.stream)
+        g.append(ev.elapsed_ms(e0, e1))
+    g_ms = float(np.mean(g))
+    lit = int((c1["m"] >= 0).sum())
+    pairs = lit * len(c1["vmat"])
+    leg = {"workload": "plain VPL render Cornell box 34 tris + light 256x256 1024 VPLs G clamp 0.01",
+           "value": round(pairs / (render_ms / 1e3) / 1e6, 3), "unit": "Mpairs/s (= Mrays/s, one shadow segment "
+           "per pair)", "ms_per_step": round(render_ms, 4), "gather_kernel_ms": round(g_ms, 4),
+           "gather_mpairs_per_s": round(pairs / (g_ms / 1e3) / 1e6, 3), "pairs": pairs}
+    pmc_hook("c1", leg, lambda pmc: valu_roofline(pmc, ["visibility", "gather_shade"], g_ms,
+                                                  "k_visibility + k_gather_shade (the C1 gather)"))
+    if not args.no_cpu:
+        import oracle
+        osc = oracle.OrcScene(sc)
+        t0 = time.perf_counter()
+        osc.gather(c1["recv"], c1["m"], c1["vpl"], c1["vmat"], 1, 0.01, 1e-3, c1["ne"], nthreads=cores)
+        dt = time.perf_counter() - t0
+        sel = np.nonzero(c1["m"] >= 0)[0][::16]
+        t1 = time.perf_counter()
+        osc.gather(c1["recv"][sel], c1["m"][sel], c1["vpl"], c1["vmat"], 1, 0.01, 1e-3, c1["ne"], nthreads=1)
+        dt1 = time.perf_counter() - t1
+        leg["cpu_baseline"] = {"value": round(pairs / dt / 1e6, 3), "unit": "Mpairs/s", "cores": cores,
+                               "kind": "port", "cpu": model,
+                               "sample": f"the whole image on the oracle: {lit} pixels x {len(c1['vmat'])} VPLs "
+                                         f"on {cores} threads, {dt:.2f} s",
+                               "one_thread": {"value": round(len(sel) * len(c1["vmat"]) / dt1 / 1e6, 3),
+                                              "unit": "Mpairs/s",
+                                              "sample": f"{len(sel)} pixels x {len(c1['vmat'])} VPLs, {dt1:.2f} s"}}
+    for p in c1["dev"]:
+        ctx.lib.rzk_device_free(ctx.h, p)
+    return leg
+
+
+# --------------------------------------------------------------------------- C3 (guided path tracing)
+def setup_c3(args, ctx):
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    W, H = 1280, 720
+    t0 = time.perf_counter()
+    sc = scenes.instanced_meshes(W, H, seed=2013, target_tris=args.tris3 or 1_000_000)
+    scene = rz.Scene(ctx, sc)
+    log(f"[bench] C3 scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
+    tracer, filler = vplgen.gpu_tracer(scene)
+    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 100_000, seed=2013, batch=65536)
+    vpls = rz.Vpls(ctx, vpl, vmat)
+    ls = rz.LightSlice(ctx, scene, vpls, width=W, height=H, tile=16, num_clusters=1000, num_neighbor_slices=16,
+                       map_resolution=32, n_emitted=ne, seed=2013)
+    t1 = time.perf_counter()
+    ls.initialize(sc.camera)
+    ctx.synchronize()
+    init_ms = (time.perf_counter() - t1) * 1e3
+    return dict(sc=sc, scene=scene, vpls=vpls, ls=ls, init_ms=init_ms, tracer=tracer, filler=filler)
+
+
+C3_KW = dict(spp=16, max_bounce=5, direct_lighting=0, guide_mode=1, seed=2013, tmin=1e-3)
+
+
+def measure_c3(args, ctx, c3, pmc_hook, cores, model):
+    """Guided path tracing of the seeded 1M-triangle instanced scene (Ward BRDFs), 1280x720, 16 spp, depth 5,
+    guided by 100k VPLs in 1,000 clusters per 16x16-pixel slice.  A step is one frame of the plain drop-in
+    (rzk_lightslice_render_guided: one k_guided_pt launch), timed with HIP events."""
+    sc, ls = c3["sc"], c3["ls"]
+    W, H = 1280, 720
+    ls.render_guided(sc.camera, count_rays=False, to_host=False, **C3_KW)
+    _, rays = ls.render_guided(sc.camera, count_rays=True, to_host=False, **C3_KW)
+    pw = ls.get("pt_work", np.int64, 4)
+    ev = HipEvents()
+    e0, e1 = ev.create(), ev.create()
+    ms = []
+    for i in range(max(2, min(args.steps, 4))):
+        ev.record(e0, ctx.stream)
+        ls.render_guided(sc.camera, count_rays=False, to_host=False, **C3_KW)
+        ev.record(e1, ctx.stream)
+        ms.append(ev.elapsed_ms(e0, e1))
+        log(f"[bench] C3 frame {i + 1}: {ms[-1]:.1f} ms")
+    k_ms = float(np.mean(ms))
+    nrays = rays["closest_rays"] + rays["shadow_rays"]
+    paths = W * H * C3_KW["spp"]
+    algo = (rays["closest_rays"] * (RAY_IN_BYTES + 16) + rays["shadow_rays"] * (RAY_IN_BYTES + SHADOW_OUT_BYTES) +
+            int(pw[2]) * NODE4_BYTES + int(pw[3]) * TRI_BYTES)
+    leg = {"workload": "guided path tracing 1M tris 1280x720 16 spp depth 5 100k VPLs 16x16 slices 1000 clusters",
+           "value": round(nrays / (k_ms / 1e3) / 1e6, 3), "unit": "Mrays/s (closest + shadow)",
+           "ms_per_step": round(k_ms, 3), "paths_per_s": round(paths / (k_ms / 1e3), 1),
+           "guide_build_ms": round(c3["init_ms"], 1), "rays": rays,
+           "work": {"nodes4": int(pw[2]), "tris": int(pw[3]),
+                    "nodes_per_ray": round(int(pw[2]) / max(1, nrays), 2),
+                    "tris_per_ray": round(int(pw[3]) / max(1, nrays), 2)}}
+    pmc_hook("c3", leg, lambda pmc: hbm_roofline(
+        pmc, ["guided_pt"], k_ms, algo, "k_guided_pt (one lane per pixel, all bounces)",
+        "48 B per closest ray + 36 B per shadow ray of I/O, 128 B per 4-wide node and 48 B per triangle "
+        "each lane's own traversal reaches (counted by the kernel)"))
+    if not args.no_cpu:
+        import oracle
+        osc = oracle.OrcScene(sc)
+        S = ls.info().num_slices
+        res = 32
+        guide = dict(res=res, maps=ls.get("maps", np.float32, S * res * res),
+                     row_cdf=ls.get("row_cdf", np.float32, S * res),
+                     col_cdf=ls.get("col_cdf", np.float32, S * res * res),
+                     totals=ls.get("map_totals", np.float32, S))
+        side = 16
+        while True:
+            win = (W // 2 - side // 2, H // 2 - side // 2, side, side)
+            t0 = time.perf_counter()
+            oracle.guided_pt(osc, sc.camera, C3_KW["spp"], C3_KW["max_bounce"], 0, 1, C3_KW["seed"], C3_KW["tmin"],
+                             16, window=win, nthreads=cores, **guide)
+            dt = time.perf_counter() - t0
+            if dt >= 4.0 or side >= 256:
+                break
+            side = min(256, int(side * max(1.5, min(4.0, (8.0 / max(dt, 1e-3)) ** 0.5))))
+        cpu_pps = side * side * C3_KW["spp"] / dt
+        leg["cpu_baseline"] = {"value": round(cpu_pps / 1e6, 4), "unit": "Mpaths/s", "cores": cores,
+                               "kind": "port", "cpu": model,
+                               "sample": f"oracle guided path tracing of a centred {side}x{side} window x "
+                                         f"{C3_KW['spp']} spp, depth 5, on {cores} threads, {dt:.1f} s",
+                               "gpu_mpaths_per_s": round(paths / (k_ms / 1e3) / 1e6, 3)}
+        c3["cpu_paths_per_s"] = cpu_pps
+    return leg
+
+
+# --------------------------------------------------------------------------- the default run
+def pmc_child(args):
+    """the kernels the PMC passes count, each launched once (C2 step, C1 gather, C3 frame), then the
+    known-size copy"""
+    legs = [x for x in args.legs.split(",") if x]
+    c2 = setup_c2(args)
+    ctx, ls = c2["ctx"], c2["ls"]
+    ls.initialize(c2["sc"].camera)
+    ls.gather(to_host=False)
+    ctx.synchronize()
+    if "c1" in legs:
+        c1 = setup_c1(ctx)
+        c1_gather_dev(ctx, c1)
+        ctx.synchronize()
+    if "c3" in legs:
+        c3 = setup_c3(args, ctx)
+        c3["ls"].render_guided(c3["sc"].camera, count_rays=False, to_host=False, **C3_KW)
+        ctx.synchronize()
+    device_copy(ctx, PMC_COPY_MIB)
+    log("[bench] PMC child done")
+
+
+# rough seconds each leg needs on the box (setup + timing + its CPU baseline), for the time budget
+LEG_SECONDS = {"c1": 20, "c3": 60, "c4": 90, "c5": 90}
+
+
+def run_default(args):
+    """The headline C2 LightSlice step, then compact C1 / C3 / C4 / C5 legs, then the live PMC passes that
+    give every roofline its measured counters; one JSON line."""
+    import traceback
+    t_begin = time.perf_counter()
+    deadline = t_begin + args.time_budget
+    model, cores = host_cpu()
+    wanted = [x for x in args.legs.split(",") if x]
+    c2 = setup_c2(args)
+    ctx = c2["ctx"]
+    out, lsinf, g_ms = measure_c2(args, c2)
+    byte_model = out.pop("_byte_model")
+    if not args.no_cpu:
+        log("[bench] C2 CPU baseline (the whole step on the oracle) ...")
+        out["cpu_baseline"] = cpu_baseline_c2(c2, lsinf, cores, model)
+        log(f"[bench] C2 CPU baseline: {out['cpu_baseline']['value']} Mrays/s on {cores} threads")
+    pending = []
+
+    def pmc_hook(name, leg, fn):
+        pending.append((leg, fn))
+
+    legs, c3 = {}, None
+    for name in wanted:
+        left = deadline - time.perf_counter()
+        if left < LEG_SECONDS.get(name, 60) + (0 if args.no_pmc else 150):
+            legs[name] = {"skipped": f"time budget ({left:.0f} s left)"}
+            continue
+        log(f"[bench] leg {name} ...")
+        try:
+            if name == "c1":
+                legs[name] = measure_c1(args, ctx, pmc_hook, cores, model)
+            elif name == "c3":
+                c3 = setup_c3(args, ctx)
+                legs[name] = measure_c3(args, ctx, c3, pmc_hook, cores, model)
+            elif name == "c4":
+                if c3 is None:
+                    c3 = setup_c3(args, ctx)
+                guides = guides_for(ctx, c3["sc"], c3["scene"], BATCH_LEVELS)
+                r = measure_batched(args, c3["sc"], ctx, c3["scene"], guides, args.jobs, 12, 2)
+                leg = {k: r[k] for k in ("value", "unit", "ms_per_step", "plain_requests_per_s", "speedup_vs_plain",
+                                         "mrays_per_s", "autotuned", "accuracy")}
+                leg["workload"] = r["config"]["workload"]
+                if "cpu_paths_per_s" in c3:
+                    leg["cpu_baseline"] = {
+                        "value": round(args.jobs / (r["_paths"] / c3["cpu_paths_per_s"]), 4), "unit": "requests/s",
+                        "cores": cores, "kind": "port", "cpu": model,
+                        "sample": f"the job mix's {r['_paths']} paths at the oracle's guided path tracing rate "
+                                  "measured in the C3 leg on the same scene"}
+                legs[name] = leg
+            elif name == "c5":
+                c5 = setup_c5(args, ctx)
+                r = measure_served(args, ctx, c5, args.requests, args.rate, args.seed)
+                leg = {k: r[k] for k in ("value", "unit", "latency_ms", "plain_requests_per_s", "plain_latency_ms",
+                                         "speedup_vs_plain", "paths_per_s", "accuracy")}
+                leg["workload"] = r["config"]["workload"]
+                if c3 is not None and "cpu_paths_per_s" in c3:
+                    leg["cpu_baseline"] = {
+                        "value": round(args.requests / (r["_paths"] / c3["cpu_paths_per_s"]), 4),
+                        "unit": "requests/s", "cores": cores, "kind": "port", "cpu": model,
+                        "sample": f"the request stream's {r['_paths']} paths at the oracle's guided path tracing "
+                                  "rate measured in the C3 leg (the 1M-triangle half of this scene)"}
+                legs[name] = leg
+            log(f"[bench] leg {name}: {legs[name].get('value')} {legs[name].get('unit')}")
+        except Exception as e:  # a failed leg is reported, the headline line still prints
+            traceback.print_exc()
+            legs[name] = {"error": f"{type(e).__name__}: {e}"}
+    pmc, note = {}, "PMC passes not run (--no-pmc)"
+    if not args.no_pmc:
+        child = ["--legs", ",".join(x for x in ("c1", "c3") if x in legs and "error" not in legs[x]
+                                    and "skipped" not in legs[x])]
+        for flag, v in (("--tris", args.tris), ("--vpls", args.vpls), ("--kd", args.kd), ("--tris3", args.tris3)):
+            if v:
+                child += [flag, str(v)]
+        pmc, note = pmc_collect(child, deadline)
+    out["roofline"] = valu_roofline(pmc, ["slice_visibility", "slice_shade"], g_ms,
+                                    "k_slice_visibility + k_slice_shade (the gather)", byte_model)
+    if note:
+        out["roofline"]["pmc_note"] = note
+    rm = pmc.get("reduced_matrix", {})
+    if "_ns" in rm:
+        out["roofline_reduced_matrix"] = valu_roofline(pmc, ["reduced_matrix"], rm["_ns"] / 1e6,
+                                                       "k_reduced_matrix (launch time from the counter pass)")
+    for leg, fn in pending:
+        leg["roofline"] = fn(pmc)
+    out["legs"] = legs
+    out["bench_seconds"] = round(time.perf_counter() - t_begin, 1)
+    print(json.dumps(out), flush=True)
 
 def main():
     ap = argparse.ArgumentParser()
@@ -595,8 +1228,8 @@ def main():
                     help="lightslice: the 200k-triangle LightSlice gather step (the headline line); "
                          "guided: guided path tracing of the 1M-triangle scene; batched: 256 uneven jobs")
     ap.add_argument("--jobs", type=int, default=256, help="batched workload: number of jobs")
-    ap.add_argument("--requests", type=int, default=64, help="served workload: number of requests")
-    ap.add_argument("--rate", type=float, default=50.0, help="served workload: Poisson arrival rate (1/s)")
+    ap.add_argument("--requests", type=int, default=32, help="served workload / C5 leg: number of requests")
+    ap.add_argument("--rate", type=float, default=200.0, help="C5 leg: Poisson arrival rate (1/s)")
     ap.add_argument("--budget-gb", type=int, default=16, help="served workload: memory budget (GiB)")
     ap.add_argument("--step-iterations", type=int, default=2,
                     help="served workload: engine iterations between looks at finished requests")
@@ -610,10 +1243,18 @@ def main():
                     help="lightslice workload with the reference's kd-tree slices of at most this many points "
                          "(0: 32x32 pixel tiles)")
     ap.add_argument("--vpls", type=int, default=0, help="VPL count (default: the workload's)")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="after the run, copy MIB MiB device-to-device once (a known-size transfer for "
-                         "scaling FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--tris3", type=int, default=0, help="C3 / C4 scene triangles (default 1M)")
+    ap.add_argument("--tris5", type=int, default=0, help="C5 scene triangles (default 2M)")
+    ap.add_argument("--vpls5", type=int, default=0, help="C5 VPL count (default 200k)")
+    ap.add_argument("--rates", default="50", help="served workload: comma-separated Poisson rates (1/s)")
+    ap.add_argument("--legs", default="c1,c3,c4,c5", help="default run: the legs beside the C2 headline")
+    ap.add_argument("--no-pmc", action="store_true", help="default run: skip the live rocprofv3 counter passes")
+    ap.add_argument("--time-budget", type=float, default=540.0,
+                    help="default run: seconds after which legs and counter passes are skipped")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        return pmc_child(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a "
               f"design, not a measured configuration.", file=sys.stderr)
@@ -631,112 +1272,9 @@ def main():
     if args.workload == "kdslice":
         return run_kdslice(args)
 
-    import renzoku_amd as rz
-    from renzoku_amd import scenes, vplgen
+    return run_default(args)
 
-    W = H = 512
-    params = dict(width=W, height=H, tile=32, num_clusters=300, num_neighbor_slices=16, map_resolution=32,
-                  map_epsilon=1e-3, clamp_mode=rz._lib.CLAMP_FORM_FACTOR, clamp=0.01, incident_clamp=0.01,
-                  tmin=1e-3, seed=2013, map_gate=0, kd_max_slice_size=args.kd)
-    t0 = time.perf_counter()
-    sc = scenes.room(W, H, seed=2013, target_tris=args.tris or 200_000)
-    log(f"[bench] scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
-    ctx = rz.Ctx(0)
-    scene = rz.Scene(ctx, sc)
-    inf = scene.info()
-    log(f"[bench] BVH: {inf.num_interior_nodes} interior nodes, depth {inf.depth}")
-    tracer, filler = vplgen.gpu_tracer(scene)
-    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 65_536, seed=2013, batch=65536)
-    params["n_emitted"] = n_emitted
-    vpls = rz.Vpls(ctx, vpl, vmat)
-    log(f"[bench] VPLs: {len(vmat)} from {n_emitted:.0f} light samples")
-    ls = rz.LightSlice(ctx, scene, vpls, **params)
 
-    ev = HipEvents()
-    stream = ctx.stream
-    e0, e1 = ev.create(), ev.create()
-    for i in range(args.warmup):
-        ls.initialize(sc.camera)
-        ls.gather(to_host=False)
-        ctx.synchronize()
-        log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    work = ls.count_work()
-    lsinf = ls.info()
-    log(f"[bench] work per step: {work}")
-
-    ctx.synchronize()
-    gather_ms = []
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        ls.initialize(sc.camera)
-        ev.record(e0, stream)
-        ls.gather(to_host=False)
-        ev.record(e1, stream)
-        gather_ms.append(ev.elapsed_ms(e0, e1))
-    ctx.synchronize()
-    total_s = time.perf_counter() - t_start
-    ms_per_step = total_s * 1e3 / args.steps
-    g_ms = float(np.mean(gather_ms))
-    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, gather kernel {g_ms:.3f} ms")
-
-    shadow_rays = work["reduced_pairs"] + work["gather_pairs"]
-    value = shadow_rays / (ms_per_step / 1e3) / 1e6
-    gbytes = (work["gather_pairs"] * (RAY_IN_BYTES + SHADOW_OUT_BYTES) + work["gather_nodes"] * NODE4_BYTES +
-              work["gather_tris"] * TRI_BYTES)
-    achieved = gbytes / (g_ms / 1e3) / 1e9
-    out = {
-        "metric": "shadow Mrays/s + VPL-gather Mpairs/s; image req/s + p50/p99 latency vs plain",
-        "value": round(value, 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded room scene, seeded light-traced VPLs)",
-        "config": {"workload": ("lightslice-gather room 200k tris 512x512 65536 VPLs 32x32 slices 300 clusters"
-                                if not args.kd else
-                                f"lightslice-gather room 200k tris 512x512 65536 VPLs kd-tree slices of {args.kd} "
-                                "points 300 clusters"),
-                   "triangles": int(sc.ntri), "width": W, "height": H, "vpls": int(len(vmat)),
-                   "slices": int(lsinf.num_slices), "clusters": int(lsinf.num_clusters),
-                   "parallelism": "single GPU"},
-        "gather_mpairs_per_s": round(work["gather_pairs"] / (g_ms / 1e3) / 1e6, 3),
-        "step_mpairs_per_s": round(shadow_rays / (ms_per_step / 1e3) / 1e6, 3),
-        "gather_kernel_ms": round(g_ms, 4),
-        "work": work,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": GATHER_HBM_BYTES,
-                     "kernel": "k_slice_visibility + k_slice_shade (the gather)",
-                     "bytes_per_launch": int(gbytes),
-                     "note": "achieved counts every lane's node visits (128 B per 4-wide node, 48 B per triangle "
-                             "test, 36 B of segment I/O); a wave fetches each node once and L2/MALL serve it, so "
-                             "measured HBM traffic per launch (traffic) is far below; the visibility kernel is "
-                             "vector-issue bound (VALUBusy reads 108.6%, saturated, rocprofv3 --pmc)"},
-    }
-    if not args.no_cpu:
-        reps = ls.get("reps", np.int32, lsinf.num_clusters)
-        rep_w = ls.get("rep_w", np.float32, lsinf.num_clusters)
-        log("[bench] CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(sc, ls, vpl, vmat, reps, rep_w, params)
-        log(f"[bench] CPU baseline: {out['cpu_baseline']}")
-    print(json.dumps(out), flush=True)
-    if args.copy_calibration > 0:
-        nb = args.copy_calibration << 20
-        src, dst = C.c_void_p(), C.c_void_p()
-        lib = ctx.lib
-        ctx.check(lib.rzk_device_alloc(ctx.h, nb, C.byref(src)), "alloc")
-        ctx.check(lib.rzk_device_alloc(ctx.h, nb, C.byref(dst)), "alloc")
-        hip = C.CDLL("libamdhip64.so")
-        hip.hipMemcpyDtoD.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
-        assert hip.hipMemcpyDtoD(dst, src, nb) == 0
-        ctx.synchronize()
-        log(f"[bench] copy calibration: {args.copy_calibration} MiB device-to-device")
-        lib.rzk_device_free(ctx.h, src)
-        lib.rzk_device_free(ctx.h, dst)
 
 
 if __name__ == "__main__":

@@ -332,7 +332,9 @@ int rzk_lightslice_count_work(rzk_lightslice *ls, int64_t *out);
  *  "recv" (W*H*21 f32), "recv_mat" (W*H i32), "hits" (W*H x {t, beta, gamma, tri as i32 bits}),
  *  "slice_count" (slices i32), "slice_reps" (slices*C i32), "slice_rep_w" (slices*C f32): each
  *  slice's clusters after refinement (the kept clusters when there is none).  C, the row length
- *  of every slice_* array, is max_slice_clusters when refinement ran, else the kept clusters.
+ *  of every slice_* array, is max_slice_clusters when refinement ran, else the kept clusters;
+ *  "pt_work" (4 i64): the last rzk_lightslice_render_guided that counted rays: {closest rays,
+ *  shadow rays, 4-wide nodes visited, triangles tested}, each lane's own traversal.
  * bytes must equal the array size. */
 int rzk_lightslice_get(const rzk_lightslice *ls, const char *name, void *dst, int64_t bytes);
 

@@ -299,6 +299,7 @@ struct rzk_lightslice {
     DBuf<float> slice_wi, slice_w, slice_pdf, slice_cdf, maps, col_cdf, row_pdf, row_cdf, totals;
     DBuf<float> pt_image;
     DBuf<unsigned long long> pt_work;
+    int64_t h_pt_work[4] = {0, 0, 0, 0};   // last counted render_guided: closest, shadow rays, nodes, tris
     std::vector<int> h_centers, h_assign, h_reps;
     std::vector<float> h_center_w, h_rep_w, h_alphas, h_mags;
     // per-slice clusters (refinement): S x ld representatives / weights and counts
@@ -649,10 +650,10 @@ int rzk_render_guided(rzk_ctx* ctx, const rzk_guide* g, const rzk_pt_params* p, 
         DBuf<float> img;
         img.alloc(3 * n);
         DBuf<unsigned long long> work;
-        work.alloc(2);
+        work.alloc(4);
         hipStream_t st = ctx->stream;
         if ((int64_t)pp.w * pp.h != n) RZK_HIP(hipMemsetAsync(img.p, 0, sizeof(float) * 3 * n, st));
-        if (out_rays) RZK_HIP(hipMemsetAsync(work.p, 0, 2 * sizeof(unsigned long long), st));
+        if (out_rays) RZK_HIP(hipMemsetAsync(work.p, 0, 4 * sizeof(unsigned long long), st));
         launch_guided_pt(ctx, scene_view(g->scene), g->cam, pp, g->gv, img.p, out_rays ? work.p : nullptr);
         if (out_rgb) RZK_HIP(hipMemcpyAsync(out_rgb, img.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st));
         unsigned long long h[2] = {0, 0};
@@ -995,19 +996,20 @@ int rzk_lightslice_render_guided(rzk_lightslice* ls, const float* camera, const 
         gv.maps = ls->maps.p; gv.row_cdf = ls->row_cdf.p; gv.col_cdf = ls->col_cdf.p; gv.totals = ls->totals.p;
         int64_t n = (int64_t)ls->W * ls->H;
         if (ls->pt_image.n != (size_t)(3 * n)) ls->pt_image.alloc(3 * n);
-        if (ls->pt_work.n == 0) ls->pt_work.alloc(2);
+        if (ls->pt_work.n == 0) ls->pt_work.alloc(4);
         if ((int64_t)pp.w * pp.h != n)   // pixels outside the window read back as 0
             RZK_HIP(hipMemsetAsync(ls->pt_image.p, 0, sizeof(float) * 3 * n, ctx->stream));
-        if (out_rays) RZK_HIP(hipMemsetAsync(ls->pt_work.p, 0, 2 * sizeof(unsigned long long), ctx->stream));
+        if (out_rays) RZK_HIP(hipMemsetAsync(ls->pt_work.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
         launch_guided_pt(ctx, scene_view(ls->scene), camera, pp, gv, ls->pt_image.p, out_rays ? ls->pt_work.p : nullptr);
         if (out_rgb) RZK_HIP(hipMemcpyAsync(out_rgb, ls->pt_image.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost,
                                             ctx->stream));
         if (out_rays) {
-            unsigned long long h[2];
+            unsigned long long h[4];
             RZK_HIP(hipMemcpyAsync(h, ls->pt_work.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
             ls_sync(ctx);
             out_rays[0] = (int64_t)h[0];
             out_rays[1] = (int64_t)h[1];
+            for (int i = 0; i < 4; ++i) ls->h_pt_work[i] = (int64_t)h[i];
         }
         if (out_rgb) ls_sync(ctx);
     });
@@ -1137,6 +1139,7 @@ int rzk_lightslice_get(const rzk_lightslice* ls, const char* name, void* dst, in
         else if (nm == "map_totals") { src = ls->totals.p; need = 4 * (int64_t)S; }
         else if (nm == "row_cdf") { src = ls->row_cdf.p; need = 4 * (int64_t)S * res; }
         else if (nm == "col_cdf") { src = ls->col_cdf.p; need = 4 * (int64_t)S * res * res; }
+        else if (nm == "pt_work") { src = ls->h_pt_work; need = 8 * 4; host = true; }
         else if (nm == "neighbors") { src = ls->nb.p; need = 4 * (int64_t)S * ls->prm.num_neighbor_slices; }
         else if (nm == "image") { src = ls->image.p; need = 12 * n; }
         else if (nm == "recv") { src = ls->recv.p; need = 4 * (int64_t)RZK_RECV_FLOATS * n; }

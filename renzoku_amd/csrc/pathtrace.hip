@@ -283,7 +283,10 @@ __global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, Pt
         out[3 * o + 1] = acc.y / (float)pp.spp;
         out[3 * o + 2] = acc.z / (float)pp.spp;
     }
-    if (work) add_ray_counts(work, nclosest, nshadow);
+    if (work) {
+        add_ray_counts(work, nclosest, nshadow);
+        add_work_totals(work + 2, st);        // 4-wide nodes visited, triangles tested (per lane)
+    }
 }
 
 // ---------------------------------------------------------------------------
