@@ -54,7 +54,9 @@ PMC_PASSES = [["SQ_INSTS_VALU", "SQ_WAVES"], ["VALUUtilization", "VALUBusy"], ["
 PMC_KERNELS = {"slice_visibility": ("k_slice_visibility", "first"), "slice_shade": ("k_slice_shade", "first"),
                "reduced_matrix": ("k_reduced_matrix", "first"), "visibility": ("k_visibility", "first"),
                "gather_shade": ("k_gather_shade", "first"), "guided_pt": ("k_guided_pt", "first"),
-               "copy": ("copyBuffer", "last")}
+               "copy": ("copyBuffer", "last"), "probe": ("k_valu_probe", "last")}
+# the vector-issue probe launch (renzoku_amd/libprobe.so): 2,048 blocks of 256 lanes (8 waves per SIMD)
+PROBE_BLOCKS, PROBE_ITERS = 2048, 8192
 PMC_COPY_MIB = 512
 # algorithmic bytes per shadow segment: ray in, result out, every 4-wide node
 # fetched (128 B: four child boxes + refs) and every triangle record tested
@@ -205,18 +207,47 @@ def pmc_traffic(pmc, keys):
     return total, {"fetch_scale": round(ff, 4), "write_scale": round(fw, 4)}
 
 
-def valu_roofline(pmc, keys, launch_ms, kernel_name, byte_model=None):
+def probe_ms():
+    """launch time of the vector-issue probe (libprobe.so), or None"""
+    path = os.path.join(ROOT, "renzoku_amd", "libprobe.so")
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    lib.rzk_probe_valu.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_float)]
+    ms = C.c_float()
+    best = None
+    for _ in range(3):
+        if lib.rzk_probe_valu(PROBE_BLOCKS, PROBE_ITERS, C.byref(ms)) != 0:
+            return None
+        best = ms.value if best is None else min(best, ms.value)
+    return best
+
+
+def valu_peak(pmc, probe_launch_ms):
+    """(sustained VALU issue rate in lane-ops/s, how it was obtained): the probe's SQ_INSTS_VALU x 64 over
+    its launch time when both were measured and the rate is at least the clock model, else the model"""
+    insts = pmc.get("probe", {}).get("SQ_INSTS_VALU")
+    if insts and probe_launch_ms:
+        rate = insts * 64.0 / (probe_launch_ms / 1e3)
+        if rate >= VALU_LANE_OPS_PEAK:
+            return rate, (f"measured: the independent-FMA probe issued {int(insts)} wave instructions in "
+                          f"{probe_launch_ms:.4f} ms ({rate / VALU_LANE_OPS_PEAK:.4f} x the 2.4 GHz model)")
+    return VALU_LANE_OPS_PEAK, "model: 256 CUs x 64 lanes x 2.4 GHz"
+
+
+def valu_roofline(pmc, keys, launch_ms, kernel_name, byte_model=None, peak=None):
     """roofline of a vector-issue-bound launch: SQ_INSTS_VALU x 64 lanes per launch / launch time against
-    VALU_LANE_OPS_PEAK; measured HBM traffic beside it"""
+    the sustained issue peak (valu_peak); measured HBM traffic beside it"""
+    peak_rate, peak_src = peak if peak is not None else (VALU_LANE_OPS_PEAK, "model: 256 CUs x 64 lanes x 2.4 GHz")
     insts = [pmc.get(k, {}).get("SQ_INSTS_VALU") for k in keys]
     traffic, scales = pmc_traffic(pmc, keys)
-    r = {"bound": "valu", "unit": "T VALU lane-ops/s", "peak": round(VALU_LANE_OPS_PEAK / 1e12, 3),
+    r = {"bound": "valu", "unit": "T VALU lane-ops/s", "peak": round(peak_rate / 1e12, 3),
          "achieved": None, "frac": None, "traffic": None if traffic is None else int(traffic),
-         "kernel": kernel_name, "launch_ms": round(launch_ms, 4)}
+         "kernel": kernel_name, "launch_ms": round(launch_ms, 4), "peak_source": peak_src}
     if all(v is not None for v in insts):
         ops = sum(insts) * 64.0
         r["achieved"] = round(ops / (launch_ms / 1e3) / 1e12, 3)
-        r["frac"] = round(ops / (launch_ms / 1e3) / VALU_LANE_OPS_PEAK, 4)
+        r["frac"] = round(ops / (launch_ms / 1e3) / peak_rate, 4)
         r["sq_insts_valu"] = {k: int(pmc[k]["SQ_INSTS_VALU"]) for k in keys}
     for k in keys:
         for c in ("VALUUtilization", "VALUBusy"):
@@ -231,7 +262,7 @@ def valu_roofline(pmc, keys, launch_ms, kernel_name, byte_model=None):
     return r
 
 
-def hbm_roofline(pmc, keys, launch_ms, algo_bytes, kernel_name, note):
+def hbm_roofline(pmc, keys, launch_ms, algo_bytes, kernel_name, note, peak=None):
     """roofline of a traversal launch against HBM peak: algorithmic bytes (the traversal byte model over
     each lane's own counted work) per launch / launch time; measured traffic beside it"""
     traffic, scales = pmc_traffic(pmc, keys)
@@ -244,7 +275,7 @@ def hbm_roofline(pmc, keys, launch_ms, algo_bytes, kernel_name, note):
         r["traffic_scales"] = scales
     insts = [pmc.get(k, {}).get("SQ_INSTS_VALU") for k in keys]
     if all(v is not None for v in insts):
-        r["valu_frac"] = round(sum(insts) * 64.0 / (launch_ms / 1e3) / VALU_LANE_OPS_PEAK, 4)
+        r["valu_frac"] = round(sum(insts) * 64.0 / (launch_ms / 1e3) / (peak[0] if peak else VALU_LANE_OPS_PEAK), 4)
     for k in keys:
         if "VALUUtilization" in pmc.get(k, {}):
             r.setdefault("valuutilization", {})[k] = round(pmc[k]["VALUUtilization"], 2)
@@ -790,6 +821,21 @@ def run_served(args):
         print(json.dumps(out), flush=True)
 
 
+def device_copy(ctx, mib):
+    """one MIB MiB device-to-device copy (a known-size transfer for scaling FETCH_SIZE / WRITE_SIZE)"""
+    nb = int(mib) << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    lib = ctx.lib
+    ctx.check(lib.rzk_device_alloc(ctx.h, nb, C.byref(src)), "rzk_device_alloc")
+    ctx.check(lib.rzk_device_alloc(ctx.h, nb, C.byref(dst)), "rzk_device_alloc")
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpyDtoD.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    assert hip.hipMemcpyDtoD(dst, src, nb) == 0
+    ctx.synchronize()
+    lib.rzk_device_free(ctx.h, src)
+    lib.rzk_device_free(ctx.h, dst)
+
+
 # --------------------------------------------------------------------------- C2 (the headline)
 C2_DESC = "lightslice-gather room 200k tris 512x512 65536 VPLs 32x32 slices 300 clusters"
 
@@ -996,8 +1042,8 @@ def measure_c1(args, ctx, pmc_hook, cores, model):
            "value": round(pairs / (render_ms / 1e3) / 1e6, 3), "unit": "Mpairs/s (= Mrays/s, one shadow segment "
            "per pair)", "ms_per_step": round(render_ms, 4), "gather_kernel_ms": round(g_ms, 4),
            "gather_mpairs_per_s": round(pairs / (g_ms / 1e3) / 1e6, 3), "pairs": pairs}
-    pmc_hook("c1", leg, lambda pmc: valu_roofline(pmc, ["visibility", "gather_shade"], g_ms,
-                                                  "k_visibility + k_gather_shade (the C1 gather)"))
+    pmc_hook("c1", leg, lambda pmc, peak: valu_roofline(pmc, ["visibility", "gather_shade"], g_ms,
+                                                        "k_visibility + k_gather_shade (the C1 gather)", peak=peak))
     if not args.no_cpu:
         import oracle
         osc = oracle.OrcScene(sc)
@@ -1074,10 +1120,10 @@ def measure_c3(args, ctx, c3, pmc_hook, cores, model):
            "work": {"nodes4": int(pw[2]), "tris": int(pw[3]),
                     "nodes_per_ray": round(int(pw[2]) / max(1, nrays), 2),
                     "tris_per_ray": round(int(pw[3]) / max(1, nrays), 2)}}
-    pmc_hook("c3", leg, lambda pmc: hbm_roofline(
+    pmc_hook("c3", leg, lambda pmc, peak: hbm_roofline(
         pmc, ["guided_pt"], k_ms, algo, "k_guided_pt (one lane per pixel, all bounces)",
         "48 B per closest ray + 36 B per shadow ray of I/O, 128 B per 4-wide node and 48 B per triangle "
-        "each lane's own traversal reaches (counted by the kernel)"))
+        "each lane's own traversal reaches (counted by the kernel)", peak))
     if not args.no_cpu:
         import oracle
         osc = oracle.OrcScene(sc)
@@ -1126,6 +1172,7 @@ def pmc_child(args):
         c3["ls"].render_guided(c3["sc"].camera, count_rays=False, to_host=False, **C3_KW)
         ctx.synchronize()
     device_copy(ctx, PMC_COPY_MIB)
+    probe_ms()
     log("[bench] PMC child done")
 
 
@@ -1199,6 +1246,7 @@ def run_default(args):
         except Exception as e:  # a failed leg is reported, the headline line still prints
             traceback.print_exc()
             legs[name] = {"error": f"{type(e).__name__}: {e}"}
+    p_ms = probe_ms()
     pmc, note = {}, "PMC passes not run (--no-pmc)"
     if not args.no_pmc:
         child = ["--legs", ",".join(x for x in ("c1", "c3") if x in legs and "error" not in legs[x]
@@ -1207,16 +1255,18 @@ def run_default(args):
             if v:
                 child += [flag, str(v)]
         pmc, note = pmc_collect(child, deadline)
+    peak = valu_peak(pmc, p_ms)
     out["roofline"] = valu_roofline(pmc, ["slice_visibility", "slice_shade"], g_ms,
-                                    "k_slice_visibility + k_slice_shade (the gather)", byte_model)
+                                    "k_slice_visibility + k_slice_shade (the gather)", byte_model, peak)
     if note:
         out["roofline"]["pmc_note"] = note
     rm = pmc.get("reduced_matrix", {})
     if "_ns" in rm:
         out["roofline_reduced_matrix"] = valu_roofline(pmc, ["reduced_matrix"], rm["_ns"] / 1e6,
-                                                       "k_reduced_matrix (launch time from the counter pass)")
+                                                       "k_reduced_matrix (launch time from the counter pass)",
+                                                       peak=peak)
     for leg, fn in pending:
-        leg["roofline"] = fn(pmc)
+        leg["roofline"] = fn(pmc, peak)
     out["legs"] = legs
     out["bench_seconds"] = round(time.perf_counter() - t_begin, 1)
     print(json.dumps(out), flush=True)
