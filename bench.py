@@ -54,9 +54,12 @@ PMC_PASSES = [["SQ_INSTS_VALU", "SQ_WAVES"], ["VALUUtilization", "VALUBusy"], ["
 PMC_KERNELS = {"slice_visibility": ("k_slice_visibility", "first"), "slice_shade": ("k_slice_shade", "first"),
                "reduced_matrix": ("k_reduced_matrix", "first"), "visibility": ("k_visibility", "first"),
                "gather_shade": ("k_gather_shade", "first"), "guided_pt": ("k_guided_pt", "first"),
-               "copy": ("copyBuffer", "last"), "probe": ("k_valu_probe", "last")}
-# the vector-issue probe launch (renzoku_amd/libprobe.so): 2,048 blocks of 256 lanes (8 waves per SIMD)
-PROBE_BLOCKS, PROBE_ITERS = 2048, 8192
+               "copy": ("copyBuffer", "last"), "probe_fma": ("k_valu_probe(", "last"),
+               "probe_slab": ("k_valu_probe_slab(", "last")}
+PROBES = {"probe_fma": 0, "probe_slab": 1}
+# the vector-issue probe launch (renzoku_amd/libprobe.so): 4,096 blocks of 256 lanes (16 waves per SIMD),
+# 16 independent FMA chains per lane, about 8 ms
+PROBE_BLOCKS, PROBE_ITERS = 4096, 8192
 PMC_COPY_MIB = 512
 # algorithmic bytes per shadow segment: ray in, result out, every 4-wide node
 # fetched (128 B: four child boxes + refs) and every triangle record tested
@@ -208,31 +211,40 @@ def pmc_traffic(pmc, keys):
 
 
 def probe_ms():
-    """launch time of the vector-issue probe (libprobe.so), or None"""
+    """{probe key: launch time} of the vector-issue probes (libprobe.so: independent FMA chains, and the
+    slab-test mix of subtract, multiply, min, max, compare, select); {} without the library"""
     path = os.path.join(ROOT, "renzoku_amd", "libprobe.so")
     if not os.path.exists(path):
-        return None
+        return {}
     lib = C.CDLL(path)
-    lib.rzk_probe_valu.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_float)]
-    ms = C.c_float()
-    best = None
-    for _ in range(3):
-        if lib.rzk_probe_valu(PROBE_BLOCKS, PROBE_ITERS, C.byref(ms)) != 0:
-            return None
-        best = ms.value if best is None else min(best, ms.value)
-    return best
+    lib.rzk_probe_valu.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    out = {}
+    for key, kind in PROBES.items():
+        ms = C.c_float()
+        best = None
+        for _ in range(3):
+            if lib.rzk_probe_valu(kind, PROBE_BLOCKS, PROBE_ITERS, C.byref(ms)) != 0:
+                break
+            best = ms.value if best is None else min(best, ms.value)
+        if best:
+            out[key] = best
+    return out
 
 
 def valu_peak(pmc, probe_launch_ms):
-    """(sustained VALU issue rate in lane-ops/s, how it was obtained): the probe's SQ_INSTS_VALU x 64 over
-    its launch time when both were measured and the rate is at least the clock model, else the model"""
-    insts = pmc.get("probe", {}).get("SQ_INSTS_VALU")
-    if insts and probe_launch_ms:
-        rate = insts * 64.0 / (probe_launch_ms / 1e3)
-        if rate >= VALU_LANE_OPS_PEAK:
-            return rate, (f"measured: the independent-FMA probe issued {int(insts)} wave instructions in "
-                          f"{probe_launch_ms:.4f} ms ({rate / VALU_LANE_OPS_PEAK:.4f} x the 2.4 GHz model)")
-    return VALU_LANE_OPS_PEAK, "model: 256 CUs x 64 lanes x 2.4 GHz"
+    """(sustained VALU issue rate in lane-ops/s, how it was obtained): the best probe's SQ_INSTS_VALU x 64
+    over its launch time when the probes ran in the counter pass, else the 2.4 GHz clock model"""
+    rates, desc = {}, []
+    for key in PROBES:
+        insts, ms = pmc.get(key, {}).get("SQ_INSTS_VALU"), probe_launch_ms.get(key)
+        if insts and ms:
+            rates[key] = insts * 64.0 / (ms / 1e3)
+            desc.append(f"{key}: {int(insts)} wave instructions in {ms:.4f} ms = "
+                        f"{rates[key] / VALU_LANE_OPS_PEAK:.4f} x the clock model")
+    if rates:
+        best = max(rates, key=rates.get)
+        return rates[best], f"measured ({best}); " + "; ".join(desc)
+    return VALU_LANE_OPS_PEAK, "model: 256 CUs x 64 lanes x 2.4 GHz (no probe counts)"
 
 
 def valu_roofline(pmc, keys, launch_ms, kernel_name, byte_model=None, peak=None):
