@@ -15,10 +15,6 @@ using namespace rzk;
 // rzk_internal.h
 void launch_trace_stats(rzk_ctx* ctx, const SceneView& sv, int kind, const float* d_rays, int64_t n,
                         float tmin, float tmax, void* d_scratch_out, int* d_stats);
-void launch_gather_fused(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
-                         int64_t npts, const int* d_order, const float* d_vpl, const int* d_vmat,
-                         int64_t nvpl, int clamp_mode, float clamp, float tmin, float n_emitted,
-                         float* d_out, int* d_stats);
 void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_alphas,
                         float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0, float* d_mT);
 void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl, const int* d_centers,
@@ -559,7 +555,7 @@ int rzk_visibility(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const fl
         DevBuf<uint64_t> db(ngroups * v->n);
         dv.upload(ctx, recv, RZK_RECV_FLOATS * npts);
         dm.upload(ctx, recv_mat, npts);
-        launch_visibility(ctx, scene_view(s), dv.p, dm.p, npts, v->d_vpl, v->n, tmin, db.p, nullptr);
+        launch_visibility(ctx, scene_view(s), dv.p, dm.p, npts, nullptr, v->d_vpl, v->n, tmin, db.p, nullptr);
         db.download(ctx, out_bits, ngroups * v->n);
         sync(ctx);
     });
@@ -580,9 +576,9 @@ int rzk_gather(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const float*
         dv.upload(ctx, recv, RZK_RECV_FLOATS * npts);
         dm.upload(ctx, recv_mat, npts);
         SceneView sv = scene_view(s);
-        launch_visibility(ctx, sv, dv.p, dm.p, npts, v->d_vpl, v->n, tmin, db.p, nullptr);
-        launch_gather_shade(ctx, sv, dv.p, dm.p, npts, v->d_vpl, v->d_mat, v->n, db.p, clamp_mode, clamp,
-                            n_emitted, dout.p);
+        launch_visibility(ctx, sv, dv.p, dm.p, npts, nullptr, v->d_vpl, v->n, tmin, db.p, nullptr);
+        launch_gather_shade(ctx, sv, dv.p, dm.p, npts, nullptr, v->d_vpl, v->d_mat, v->n, db.p, clamp_mode, clamp,
+                            n_emitted, nullptr, nullptr, dout.p);
         dout.download(ctx, out_rgb, 3 * npts);
         sync(ctx);
     });

@@ -7,7 +7,7 @@
 // call it.
 //
 // `scratch` is a host-side stand-in in host code: kernels get the device
-// address resolved by hipGetSymbolAddress for each device (scratch_setup, run
+// address resolved by hipGetSymbolAddress for each device (gather_noalloc_setup, run
 // by rzk_ctx_create with the device current), never the array's name.
 #pragma clang fp contract(off)
 
@@ -68,9 +68,9 @@ extern "C" int rzk_gather_dev_noalloc(rzk_ctx* ctx, const rzk_scene* s, const rz
             // k_visibility writes every (VPL, group) word it covers, so the
             // scratch needs no clearing between calls
             SceneView sv = scene_view(s);
-            launch_visibility(ctx, sv, d_recv, d_recv_mat, npts, v->d_vpl, v->n, tmin, bits, nullptr);
-            launch_gather_shade(ctx, sv, d_recv, d_recv_mat, npts, v->d_vpl, v->d_mat, v->n, bits, clamp_mode,
-                                clamp, n_emitted, d_out_rgb);
+            launch_visibility(ctx, sv, d_recv, d_recv_mat, npts, nullptr, v->d_vpl, v->n, tmin, bits, nullptr);
+            launch_gather_shade(ctx, sv, d_recv, d_recv_mat, npts, nullptr, v->d_vpl, v->d_mat, v->n, bits,
+                                clamp_mode, clamp, n_emitted, nullptr, nullptr, d_out_rgb);
         }
         ctx->last_error.clear();
         return RZK_OK;

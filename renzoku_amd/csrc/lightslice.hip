@@ -16,10 +16,6 @@
 #include "pathtrace.h"
 #include "rzk_device.h"
 
-void launch_gather_fused(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
-                         int64_t npts, const int* d_order, const float* d_vpl, const int* d_vmat,
-                         int64_t nvpl, int clamp_mode, float clamp, float tmin, float n_emitted,
-                         float* d_out, int* d_stats);
 void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_alphas,
                         float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0, float* d_mT);
 void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl, const int* d_centers,
@@ -688,8 +684,29 @@ int rzk_render_vpl(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const fl
         launch_camera_image_rays(ctx, camera, W, H, rays.p);
         launch_trace_closest_image(ctx, sv, rays.p, W, H, 1e-3f, 1e6f, hits.p, nullptr);
         launch_fill_receivers(ctx, sv, rays.p, hits.p, n, recv.p, mat.p, light.p);
-        launch_gather_fused(ctx, sv, recv.p, mat.p, n, order.p, v->d_vpl, v->d_mat, v->n, clamp_mode, clamp, tmin,
-                            n_emitted, img.p, nullptr);
+        // the gather as visibility bits then the shading sum, VPLs in chunks whose
+        // bit matrix stays within 256 MiB; the running sums carry across chunks in
+        // VPL order, so the image is that of one launch over all VPLs
+        const int64_t ngroups = (n + 63) / 64;
+        const int64_t budget_words = (int64_t)32 << 20;
+        int64_t chunk = std::max<int64_t>(64, budget_words / ngroups / 64 * 64);
+        if (v->n > 0) chunk = std::min(chunk, v->n);
+        DBuf<uint64_t> bits;
+        bits.alloc(ngroups * chunk);
+        DBuf<float> acc;
+        if (v->n > chunk) acc.alloc(3 * n);
+        if (v->n == 0)
+            launch_gather_shade(ctx, sv, recv.p, mat.p, n, order.p, v->d_vpl, v->d_mat, 0, bits.p, clamp_mode, clamp,
+                                n_emitted, nullptr, nullptr, img.p);
+        for (int64_t k0 = 0; k0 < v->n; k0 += chunk) {
+            const int64_t cnt = std::min(chunk, v->n - k0);
+            const bool first = k0 == 0, last = k0 + cnt >= v->n;
+            launch_visibility(ctx, sv, recv.p, mat.p, n, order.p, v->d_vpl + (int64_t)RZK_VPL_FLOATS * k0, cnt, tmin,
+                              bits.p, nullptr);
+            launch_gather_shade(ctx, sv, recv.p, mat.p, n, order.p, v->d_vpl + (int64_t)RZK_VPL_FLOATS * k0,
+                                v->d_mat + k0, cnt, bits.p, clamp_mode, clamp, n_emitted, first ? nullptr : acc.p,
+                                last ? nullptr : acc.p, img.p);
+        }
         if (out_rgb) RZK_HIP(hipMemcpyAsync(out_rgb, img.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
         ls_sync(ctx);
     });

@@ -166,13 +166,16 @@ void launch_fill_receivers(rzk_ctx* ctx, const SceneView& sv, const float* d_ray
 void launch_camera_rays(rzk_ctx* ctx, const float* cam, int64_t n, const int* d_px,
                         const int* d_py, int width, float* d_rays);
 void launch_camera_image_rays(rzk_ctx* ctx, const float* cam, int width, int height, float* d_rays);
+// pair visibility bits of npts receivers (in d_order's order when given) x nvpl VPLs
 void launch_visibility(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
-                       int64_t npts, const float* d_vpl, int64_t nvpl, float tmin,
+                       int64_t npts, const int* d_order, const float* d_vpl, int64_t nvpl, float tmin,
                        uint64_t* d_bits, int* d_stats);
+// the gather's shading sum over nvpl VPLs with those bits; d_acc_in / d_acc_out
+// (optional) carry running sums between the chunks of a longer VPL list
 void launch_gather_shade(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
-                         int64_t npts, const float* d_vpl, const int* d_vmat, int64_t nvpl,
+                         int64_t npts, const int* d_order, const float* d_vpl, const int* d_vmat, int64_t nvpl,
                          const uint64_t* d_bits, int clamp_mode, float clamp, float n_emitted,
-                         float* d_out);
+                         const float* d_acc_in, float* d_acc_out, float* d_out);
 void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
                            const int* d_rmat, int rows, const int* d_vorder, const float* d_vpl,
                            const int* d_vmat,

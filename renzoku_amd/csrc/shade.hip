@@ -33,9 +33,12 @@ __device__ __forceinline__ void stage_vpls(float* s_vpl, const float* __restrict
 
 // ---- pair visibility: one wave = 64 receivers x one VPL ------------------
 // bits[k * ngroups + g] bit l: segment (receiver 64g+l) -> VPL k blocked (or
-// the receiver has no material / lies past the end)
+// the receiver has no material / lies past the end).  order (optional): the
+// receiver at position t is order[t] (64 consecutive positions form a wave;
+// an 8x8 pixel tile per wave keeps its 64 segments coherent).
 __global__ __launch_bounds__(kShadeBlock) void k_visibility(SceneView sv, const float* __restrict__ recv,
                                                            const int* __restrict__ rmat, int64_t npts,
+                                                           const int* __restrict__ order,
                                                            const float* __restrict__ vpl, int64_t nvpl,
                                                            float tmin, uint64_t* __restrict__ bits,
                                                            int* __restrict__ d_stats) {
@@ -52,8 +55,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_visibility(SceneView sv, const 
     if (live) {
         k = wave / ngroups;
         g = wave - k * ngroups;
-        int64_t i = g * 64 + lane;
-        if (i < npts && rmat[i] >= 0) {
+        int64_t t = g * 64 + lane;
+        int64_t i = t < npts ? (order ? order[t] : t) : -1;
+        if (i >= 0 && rmat[i] >= 0) {
             active = true;
             p = ld3(recv + RZK_RECV_FLOATS * i);
             q = ld3(vpl + RZK_VPL_FLOATS * k);
@@ -65,34 +69,44 @@ __global__ __launch_bounds__(kShadeBlock) void k_visibility(SceneView sv, const 
     if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
-// ---- gather shading over all VPLs with precomputed visibility -------------
-// Σ_k (visible ? gather_point : 0) in VPL order, then / num_emitted
+// ---- gather shading over a VPL range with precomputed visibility ---------
+// Σ_k (visible ? gather_point : 0) in VPL order, then / num_emitted.  The
+// range may be one chunk of a longer VPL list: acc_in (optional) holds the
+// receivers' running sums over the earlier chunks, and acc_out (optional)
+// takes the running sums instead of the final radiance, so a list gathered in
+// chunks sums in the same order, to the same bits, as in one launch.  order:
+// as k_visibility (thread t shades receiver order[t]; bits of group t / 64).
 __global__ __launch_bounds__(kShadeBlock) void k_gather_shade(const float* __restrict__ mat,
                                                              const float* __restrict__ recv,
                                                              const int* __restrict__ rmat, int64_t npts,
+                                                             const int* __restrict__ order,
                                                              const float* __restrict__ vpl,
                                                              const int* __restrict__ vmat, int64_t nvpl,
                                                              const uint64_t* __restrict__ bits,
                                                              int mode, float thr, float n_emitted,
+                                                             const float* __restrict__ acc_in,
+                                                             float* __restrict__ acc_out,
                                                              float* __restrict__ out) {
     __shared__ float s_vpl[kVplTile * kVplRec];
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t ngroups = (npts + 63) / 64;
+    int64_t i = t < npts ? (order ? order[t] : t) : -1;
     float rc[RZK_RECV_FLOATS];
     int m = -1;
-    if (i < npts) {
+    v3 acc = mk(0.f, 0.f, 0.f);
+    if (i >= 0) {
         load_record(rc, recv + RZK_RECV_FLOATS * i, RZK_RECV_FLOATS);
         m = rmat[i];
+        if (acc_in) acc = ld3(acc_in + 3 * i);
     }
-    v3 acc = mk(0.f, 0.f, 0.f);
-    int64_t g = i / 64;
-    int lane = (int)(i & 63);
+    int64_t g = t / 64;
+    int lane = (int)(t & 63);
     for (int64_t k0 = 0; k0 < nvpl; k0 += kVplTile) {
         int cnt = (int)((nvpl - k0) < kVplTile ? (nvpl - k0) : kVplTile);
         __syncthreads();
         stage_vpls(s_vpl, vpl, vmat, k0, cnt);
         __syncthreads();
-        if (i < npts && m >= 0) {
+        if (i >= 0 && m >= 0) {
             for (int kk = 0; kk < cnt; ++kk) {
                 uint64_t w = bits[(k0 + kk) * ngroups + g];
                 if ((w >> lane) & 1ull) continue;
@@ -103,57 +117,17 @@ __global__ __launch_bounds__(kShadeBlock) void k_gather_shade(const float* __res
             }
         }
     }
-    if (i < npts) {
-        out[3 * i] = acc.x / n_emitted;
-        out[3 * i + 1] = acc.y / n_emitted;
-        out[3 * i + 2] = acc.z / n_emitted;
-    }
-}
-
-// ---- fused gather: per receiver, for each VPL in order, trace the segment
-// and shade when visible (one wave = 64 neighbouring receivers, so the 64
-// segments of a step share an end point and stay coherent) -----------------
-__global__ __launch_bounds__(kShadeBlock) void k_gather_fused(SceneView sv, const float* __restrict__ recv,
-                                                             const int* __restrict__ rmat, int64_t npts,
-                                                             const int* __restrict__ order,
-                                                             const float* __restrict__ vpl,
-                                                             const int* __restrict__ vmat, int64_t nvpl,
-                                                             int mode, float thr, float tmin,
-                                                             float n_emitted, float* __restrict__ out,
-                                                             int* __restrict__ d_stats) {
-    extern __shared__ int s_dyn[];
-    float* s_vpl = reinterpret_cast<float*>(s_dyn);
-    int* wstk = wave_stack(s_dyn + kVplTile * kVplRec, sv);
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t i = t < npts ? (order ? order[t] : t) : -1;
-    const float* rcm = recv + RZK_RECV_FLOATS * (i >= 0 ? i : 0);
-    int m = i >= 0 ? rmat[i] : -1;
-    v3 p = m >= 0 ? ld3(rcm) : mk(0.f, 0.f, 0.f);
-    TravStats st;
-    v3 acc = mk(0.f, 0.f, 0.f);
-    for (int64_t k0 = 0; k0 < nvpl; k0 += kVplTile) {
-        int cnt = (int)((nvpl - k0) < kVplTile ? (nvpl - k0) : kVplTile);
-        __syncthreads();
-        stage_vpls(s_vpl, vpl, vmat, k0, cnt);
-        __syncthreads();
-        for (int kk = 0; kk < cnt; ++kk) {
-            const float* sv_rec = s_vpl + kk * kVplRec;
-            bool blocked = segment_blocked_wave(sv, p, mk(sv_rec[0], sv_rec[1], sv_rec[2]), tmin, m >= 0, wstk, st);
-            if (m < 0 || blocked) continue;
-            float rc[RZK_RECV_FLOATS], vp[kVplRec];
-            __asm__ volatile("" ::: "memory");   // re-read the receiver here, do not keep it live
-            load_record(rc, rcm, RZK_RECV_FLOATS);
-            load_record(vp, sv_rec, kVplRec);
-            v3 lo = gather_shade(sv.mat, rc, m, vp, __float_as_int(vp[21]), mode, thr);
-            acc = acc + lo * 1.0f;
+    if (i >= 0) {
+        if (acc_out) {
+            acc_out[3 * i] = acc.x;
+            acc_out[3 * i + 1] = acc.y;
+            acc_out[3 * i + 2] = acc.z;
+        } else {
+            out[3 * i] = acc.x / n_emitted;
+            out[3 * i + 1] = acc.y / n_emitted;
+            out[3 * i + 2] = acc.z / n_emitted;
         }
     }
-    if (i >= 0) {
-        out[3 * i] = acc.x / n_emitted;
-        out[3 * i + 1] = acc.y / n_emitted;
-        out[3 * i + 2] = acc.z / n_emitted;
-    }
-    if (d_stats) add_work_totals(reinterpret_cast<unsigned long long*>(d_stats), st);
 }
 
 // ---- LightSlice reduced matrix (lightslice.cc:721-748): pair (row r, VPL k).
@@ -369,40 +343,26 @@ using namespace rzk;
 static unsigned blocks_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 void launch_visibility(rzk_ctx* ctx, const SceneView& sv_in, const float* d_recv, const int* d_rmat,
-                       int64_t npts, const float* d_vpl, int64_t nvpl, float tmin, uint64_t* d_bits,
-                       int* d_stats) {
+                       int64_t npts, const int* d_order, const float* d_vpl, int64_t nvpl, float tmin,
+                       uint64_t* d_bits, int* d_stats) {
     int64_t waves = ((npts + 63) / 64) * nvpl;
     if (waves <= 0) return;
     SceneView sv = sv_in;
     unsigned grid = blocks_for(waves, kShadeBlock / 64);
     size_t lds = sizeof(int) * (kShadeBlock / 64) * (size_t)sv.stack4;
     hipLaunchKernelGGL(k_visibility, dim3(grid), dim3(kShadeBlock), lds,
-                       ctx->stream, sv, d_recv, d_rmat, npts, d_vpl, nvpl, tmin, d_bits, d_stats);
+                       ctx->stream, sv, d_recv, d_rmat, npts, d_order, d_vpl, nvpl, tmin, d_bits, d_stats);
     RZK_HIP(hipGetLastError());
 }
 
 void launch_gather_shade(rzk_ctx* ctx, const SceneView& sv, const float* d_recv, const int* d_rmat,
-                         int64_t npts, const float* d_vpl, const int* d_vmat, int64_t nvpl,
+                         int64_t npts, const int* d_order, const float* d_vpl, const int* d_vmat, int64_t nvpl,
                          const uint64_t* d_bits, int clamp_mode, float clamp, float n_emitted,
-                         float* d_out) {
+                         const float* d_acc_in, float* d_acc_out, float* d_out) {
     if (npts <= 0) return;
     hipLaunchKernelGGL(k_gather_shade, dim3(blocks_for(npts, kShadeBlock)), dim3(kShadeBlock), 0,
-                       ctx->stream, sv.mat, d_recv, d_rmat, npts, d_vpl, d_vmat, nvpl, d_bits, clamp_mode,
-                       clamp, n_emitted, d_out);
-    RZK_HIP(hipGetLastError());
-}
-
-void launch_gather_fused(rzk_ctx* ctx, const SceneView& sv_in, const float* d_recv, const int* d_rmat,
-                         int64_t npts, const int* d_order, const float* d_vpl, const int* d_vmat,
-                         int64_t nvpl, int clamp_mode, float clamp, float tmin, float n_emitted,
-                         float* d_out, int* d_stats) {
-    if (npts <= 0) return;
-    SceneView sv = sv_in;
-    unsigned grid = blocks_for(npts, kShadeBlock);
-    size_t lds = sizeof(float) * kVplTile * kVplRec + sizeof(int) * (kShadeBlock / 64) * (size_t)sv.stack4;
-    hipLaunchKernelGGL(k_gather_fused, dim3(grid), dim3(kShadeBlock), lds,
-                       ctx->stream, sv, d_recv, d_rmat, npts, d_order, d_vpl, d_vmat, nvpl, clamp_mode,
-                       clamp, tmin, n_emitted, d_out, d_stats);
+                       ctx->stream, sv.mat, d_recv, d_rmat, npts, d_order, d_vpl, d_vmat, nvpl, d_bits, clamp_mode,
+                       clamp, n_emitted, d_acc_in, d_acc_out, d_out);
     RZK_HIP(hipGetLastError());
 }
 

@@ -291,3 +291,22 @@ def test_lightslice_pipeline_parity(ctx, room_setup):
         assert np.array_equal(growc[s].view(np.uint32), om["row_cdf"].view(np.uint32))
         assert np.array_equal(gcolc[s].view(np.uint32), om["col_cdf"].view(np.uint32))
         assert gtot[s] == np.float32(om["total"])
+
+
+def test_render_vpl_chunked_gather_is_one_launch(ctx, scenes_gpu):
+    """a 1024x1024 render over 2,560 VPLs gathers in two VPL chunks (its bit
+    matrix would pass 256 MiB in one); the chunked running sums give the bits of
+    one gather launch over all VPLs on the same receivers"""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    sc, gs, os_ = scenes_gpu["cornell"]
+    tracer, filler = vplgen.gpu_tracer(gs)
+    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 2560, seed=8, batch=8192)
+    vp = rz.Vpls(ctx, vpl, vmat)
+    cam = np.array(scenes.cornell_box(1024, 1024).camera, np.float32)   # same box, 1024x1024 camera
+    img = rz.render_vpl(ctx, gs, vp, cam, 1, 0.01, 1e-3, ne)
+    px, py = scenes.pixel_grid(1024, 1024)
+    _, _, recv, m, _ = gs.camera_primary(cam, px, py)
+    one = rz.gather(ctx, gs, vp, recv, m, 1, 0.01, 1e-3, ne).reshape(1024, 1024, 3)
+    assert np.array_equal(img.view(np.uint32), one.view(np.uint32))
+    assert img.mean() > 0
