@@ -294,35 +294,6 @@ def hbm_roofline(pmc, keys, launch_ms, algo_bytes, kernel_name, note, peak=None)
     return r
 
 
-def cpu_baseline(sc, ls, vpl, vmat, reps, rep_w, params, target_s=8.0):
-    """oracle gather_clusters on a seeded sample of pixels, all host cores (<= 16)"""
-    import oracle
-    W, H = params["width"], params["height"]
-    recv = ls.get("recv", np.float32, W * H * 21).reshape(-1, 21)
-    rmat = ls.get("recv_mat", np.int32, W * H)
-    osc = oracle.OrcScene(sc)
-    vsub = vpl[reps].copy()
-    vsub[:, 18:21] = vpl[reps, 18:21] * rep_w[:, None]
-    vm = vmat[reps].copy()
-    cores = max(1, min(16, os.cpu_count() or 1))
-    rng = np.random.default_rng(1)
-    ok = np.nonzero(rmat >= 0)[0]
-    n = 16 * cores
-    while True:
-        sel = np.sort(rng.choice(ok, min(n, len(ok)), replace=False))
-        t0 = time.perf_counter()
-        osc.gather(recv[sel], rmat[sel], vsub, vm, params["clamp_mode"], params["clamp"], params["tmin"],
-                   params["n_emitted"], nthreads=cores)
-        dt = time.perf_counter() - t0
-        if dt >= target_s * 0.5 or n >= len(ok):
-            break
-        n = int(min(len(ok), n * max(2.0, target_s / max(dt, 1e-3))))
-    pairs = len(sel) * len(reps)
-    return {"value": pairs / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"oracle gather over {len(sel)} seeded pixels x {len(reps)} cluster representatives "
-                      f"({pairs} shadow segments + shadings) on {cores} threads, {dt:.1f} s"}
-
-
 def run_guided(args):
     """Guided path tracing of the 1M-triangle instanced scene (Ward BRDFs):
     1280x720, 16 spp, depth 5, 100k VPLs clustered into 1,000 clusters with
@@ -902,11 +873,10 @@ def cpu_baseline_c2(c2, lsinf, cores, model):
     cl = oracle.cluster_by_sampling(R, prm["num_clusters"], prm["seed"], nthreads=cores)
     parts["clustering_s"] = time.perf_counter() - t0
     reps, rep_w = cl["reps"], cl["rep_w"]
-    vsub = vpl[reps].copy()
-    vsub[:, 18:21] = vpl[reps, 18:21] * rep_w[:, None]
-    vm = vmat[reps].copy()
+    vr, vm = vpl[reps].copy(), vmat[reps].copy()
     t0 = time.perf_counter()
-    osc.gather(recv, m, vsub, vm, prm["clamp_mode"], prm["clamp"], prm["tmin"], prm["n_emitted"], nthreads=cores)
+    osc.gather_clusters(recv, m, vr, vm, rep_w, prm["clamp_mode"], prm["clamp"], prm["tmin"], prm["n_emitted"],
+                        nthreads=cores)
     parts["gather_s"] = time.perf_counter() - t0
     step_s = time.perf_counter() - t_all
     lit = int((m >= 0).sum())
@@ -919,8 +889,8 @@ def cpu_baseline_c2(c2, lsinf, cores, model):
     while True:
         sel = np.sort(rng.choice(ok, min(n1, len(ok)), replace=False))
         t0 = time.perf_counter()
-        osc.gather(recv[sel], m[sel], vsub, vm, prm["clamp_mode"], prm["clamp"], prm["tmin"], prm["n_emitted"],
-                   nthreads=1)
+        osc.gather_clusters(recv[sel], m[sel], vr, vm, rep_w, prm["clamp_mode"], prm["clamp"], prm["tmin"],
+                            prm["n_emitted"], nthreads=1)
         dt1 = time.perf_counter() - t0
         if dt1 >= 2.5 or n1 >= len(ok):
             break

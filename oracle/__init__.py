@@ -68,6 +68,7 @@ def lib():
             "orc_gather_point": ([P, P, I, P, I, I, F, F, P], I),
             "orc_incident_radiance": ([P, P, P, I, I, F, P], None),
             "orc_gather": ([P, I, P, P, I, P, P, I, F, F, F, P, I], None),
+            "orc_gather_weighted": ([P, I, P, P, I, P, P, P, I, F, F, F, P, I], None),
             "orc_reduced_matrix": ([P, I, P, P, I, P, P, I, F, F, P, P, I], None),
             "orc_alphas": ([I, I, P, P], None), "orc_column_magnitudes": ([I, I, P, P], None),
             "orc_cluster_assign": ([I, I, P, I, P, P, P, I], None),
@@ -185,6 +186,15 @@ class OrcScene:
         out = np.zeros((len(rmat), 3), np.float32)
         self.L.orc_gather(self.h, len(rmat), _p(recv), _p(rmat), len(vmat), _p(vpl), _p(vmat), clamp_mode, clamp,
                           tmin, n_emitted, _p(out), nthreads)
+        return out
+
+    def gather_clusters(self, recv, rmat, vpl, vmat, weights, clamp_mode, clamp, tmin, n_emitted, nthreads=1):
+        """Σ_c w_c gather_point(rep_c) / n_emitted per receiver (lightslice.cc:1325-1339): vpl/vmat are
+        the representatives' records, the weight multiplies after the clamp"""
+        recv, rmat, vpl, vmat, w = _f(recv), _i(rmat), _f(vpl), _i(vmat), _f(weights)
+        out = np.zeros((len(rmat), 3), np.float32)
+        self.L.orc_gather_weighted(self.h, len(rmat), _p(recv), _p(rmat), len(vmat), _p(vpl), _p(vmat), _p(w),
+                                   clamp_mode, clamp, tmin, n_emitted, _p(out), nthreads)
         return out
 
     def gather_point(self, recv, rmat, vpl, vmat, clamp_mode, clamp, tmin):

@@ -737,11 +737,12 @@ static void parallel_for(int n, int nthreads, void (*fn)(void *, int, int), void
 
 typedef struct {
     const orc_scene *s; const float *recv; const int *rmat; int nvpl; const float *vpl; const int *vmat;
-    int mode; float thr, tmin, nem; float *out;
+    const float *w; int mode; float thr, tmin, nem; float *out;
 } gather_args;
 
-/* Full gather: Σ_k gather_point / num_emitted (lightslice.cc:1325-1339 with every
- * VPL its own weight-1 cluster; vpl.cc:488-489 for the division) */
+/* Σ_k w_k gather_point / num_emitted: LightSlice::gather_clusters (lightslice.cc:1325-1339,
+ * radiance += weight * Lo after the clamp inside gather_point), vpl.cc:488-489 for the
+ * division; without weights every VPL is its own weight-1 cluster */
 static void gather_range(void *a_, int lo, int hi) {
     gather_args *a = a_;
     for (int i = lo; i < hi; ++i) {
@@ -751,7 +752,7 @@ static void gather_range(void *a_, int lo, int hi) {
             for (int k = 0; k < a->nvpl; ++k) {
                 v3 lo3;
                 if (gather_point(a->s, rc, a->rmat[i], a->vpl + ORC_VPL_STRIDE * k, a->vmat[k], a->mode, a->thr, a->tmin, &lo3))
-                    acc = add(acc, scl(lo3, 1.0f));
+                    acc = add(acc, scl(lo3, a->w ? a->w[k] : 1.0f));
             }
         }
         a->out[3 * i] = acc.x / a->nem; a->out[3 * i + 1] = acc.y / a->nem; a->out[3 * i + 2] = acc.z / a->nem;
@@ -761,7 +762,14 @@ static void gather_range(void *a_, int lo, int hi) {
 void orc_gather(const orc_scene *s, int npts, const float *recv, const int *recv_mat,
                 int nvpl, const float *vpl, const int *vpl_mat, int clamp_mode, float clamp,
                 float tmin, float n_emitted, float *out_rgb, int nthreads) {
-    gather_args a = {s, recv, recv_mat, nvpl, vpl, vpl_mat, clamp_mode, clamp, tmin, n_emitted, out_rgb};
+    gather_args a = {s, recv, recv_mat, nvpl, vpl, vpl_mat, NULL, clamp_mode, clamp, tmin, n_emitted, out_rgb};
+    parallel_for(npts, nthreads, gather_range, &a);
+}
+
+void orc_gather_weighted(const orc_scene *s, int npts, const float *recv, const int *recv_mat,
+                         int nvpl, const float *vpl, const int *vpl_mat, const float *vpl_w, int clamp_mode,
+                         float clamp, float tmin, float n_emitted, float *out_rgb, int nthreads) {
+    gather_args a = {s, recv, recv_mat, nvpl, vpl, vpl_mat, vpl_w, clamp_mode, clamp, tmin, n_emitted, out_rgb};
     parallel_for(npts, nthreads, gather_range, &a);
 }
 

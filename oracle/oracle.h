@@ -94,6 +94,12 @@ void  orc_incident_radiance(const orc_scene *s, const float recv[ORC_RECV_STRIDE
 void  orc_gather(const orc_scene *s, int npts, const float *recv, const int *recv_mat,
                  int nvpl, const float *vpl, const int *vpl_mat, int clamp_mode, float clamp,
                  float tmin, float n_emitted, float *out_rgb, int nthreads);
+/* the same with a weight per VPL applied after the clamp (gather_clusters over
+ * cluster representatives, lightslice.cc:1325-1339) */
+void  orc_gather_weighted(const orc_scene *s, int npts, const float *recv, const int *recv_mat,
+                          int nvpl, const float *vpl, const int *vpl_mat, const float *vpl_w,
+                          int clamp_mode, float clamp, float tmin, float n_emitted, float *out_rgb,
+                          int nthreads);
 
 /* ---- light transport matrix + clustering ---- */
 void orc_reduced_matrix(const orc_scene *s, int rows, const float *recv, const int *recv_mat,

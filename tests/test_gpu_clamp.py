@@ -135,9 +135,8 @@ def test_lightslice_clamp_modes_room(ctx, small_room, room_setup, mode, clamp, i
     prays = oracle.camera_rays(small_room.camera, pxa, pya)
     t, b, g, tri = osc.trace_closest(prays)
     recv, m, _ = osc.fill_receivers(prays, t, b, g, tri)
-    vsub = vpl[reps].astype(np.float32)
-    vsub[:, 18:21] = (vpl[reps, 18:21] * rep_w[:, None]).astype(np.float32)
-    ref = osc.gather(recv, m, vsub, vmat[reps], mode, clamp, 1e-3, ne, nthreads=8).reshape(H, W, 3)
+    ref = osc.gather_clusters(recv, m, vpl[reps], vmat[reps], rep_w, mode, clamp, 1e-3, ne,
+                              nthreads=8).reshape(H, W, 3)
     assert rel_l2(img, ref) <= 1e-4
     # the per-slice upper-hemisphere flags use the incident clamp of this mode
     upper = ls.get("slice_upper", np.int32, S * len(reps)).reshape(S, -1)

@@ -63,9 +63,8 @@ def test_c2_headline_configuration(ctx):
     assert np.array_equal(reps, oc["reps"])
     assert np.array_equal(rep_w.view(np.uint32), oc["rep_w"].view(np.uint32))
     # the final gather: every pixel over the 300 representatives
-    vsub = vpl[reps].astype(np.float32)
-    vsub[:, 18:21] = (vpl[reps, 18:21] * rep_w[:, None]).astype(np.float32)
-    ref = osc.gather(recv, m, vsub, vmat[reps], 1, 0.01, 1e-3, ne, nthreads=16).reshape(H, W, 3)
+    ref = osc.gather_clusters(recv, m, vpl[reps], vmat[reps], rep_w, 1, 0.01, 1e-3, ne,
+                              nthreads=16).reshape(H, W, 3)
     err = rel_l2(img, ref)
     print(f"C2 full size: gather rel-L2 {err:.3e}", flush=True)
     assert err <= 1e-4 and ref.sum() > 0

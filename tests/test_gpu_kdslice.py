@@ -165,9 +165,8 @@ def test_lightslice_with_kd_slices_matches_oracle(ctx):
     for s in range(S):
         sel = np.nonzero(psl == s)[0]
         n = cnt[s]
-        vsub = vpl[sreps[s, :n]].copy()
-        vsub[:, 18:21] = vpl[sreps[s, :n], 18:21] * rw[s, :n, None]
-        ref[sel] = osc.gather(recv[sel], m[sel], vsub, vmat[sreps[s, :n]], 1, 0.01, 1e-3, ne, nthreads=8)
+        ref[sel] = osc.gather_clusters(recv[sel], m[sel], vpl[sreps[s, :n]], vmat[sreps[s, :n]], rw[s, :n], 1, 0.01,
+                                       1e-3, ne, nthreads=8)
     assert _rel_l2(img.reshape(-1, 3), ref) <= 1e-4
     work = ls.count_work()
     assert work["gather_pairs"] == int(cnt[psl[gp]].sum())
@@ -216,9 +215,7 @@ def test_lightslice_kd_slices_without_refinement(ctx):
     crw = ls.get("rep_w", np.float32, inf.num_clusters)
     assert np.array_equal(creps, oc["reps"]) and np.array_equal(crw.view(np.uint32), oc["rep_w"].view(np.uint32))
     img = ls.gather()
-    vsub = vpl[creps].copy()
-    vsub[:, 18:21] = vpl[creps, 18:21] * crw[:, None]
-    ref = osc.gather(recv, m, vsub, vmat[creps], 1, 0.01, 1e-3, ne, nthreads=8).reshape(H, W, 3)
+    ref = osc.gather_clusters(recv, m, vpl[creps], vmat[creps], crw, 1, 0.01, 1e-3, ne, nthreads=8).reshape(H, W, 3)
     assert _rel_l2(img, ref) <= 1e-4
     with pytest.raises(rz.RzkError):
         rz.LightSlice(ctx, gs, vp, width=W, height=H, num_clusters=24, kd_max_slice_size=2)

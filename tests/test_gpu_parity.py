@@ -259,9 +259,7 @@ def test_lightslice_pipeline_parity(ctx, room_setup):
     for k, ref in enumerate((t, b, g)):
         assert np.array_equal(hits[ok, k].view(np.uint32), ref[ok].view(np.uint32))
     recv, m, _ = os_.fill_receivers(prays, t, b, g, tri)
-    vsub = (vpl[reps] * 1.0).astype(np.float32)
-    vsub[:, 18:21] = (vpl[reps, 18:21] * rep_w[:, None]).astype(np.float32)
-    ref = os_.gather(recv, m, vsub, vmat[reps], 1, 0.01, 1e-3, ne, nthreads=8).reshape(H, W, 3)
+    ref = os_.gather_clusters(recv, m, vpl[reps], vmat[reps], rep_w, 1, 0.01, 1e-3, ne, nthreads=8).reshape(H, W, 3)
     assert rel_l2(img, ref) <= 1e-4
     # per-slice guiding data against the oracle's restatement
     nb = ls.get("neighbors", np.int32, S * 4).reshape(S, 4)

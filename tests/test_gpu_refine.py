@@ -145,9 +145,8 @@ def test_lightslice_refined_pipeline(ctx, room_refined):
     for s in range(S):
         sel = np.nonzero(sl == s)[0]
         n = cnt[s]
-        vsub = vpl[reps[s, :n]].copy()
-        vsub[:, 18:21] = vpl[reps[s, :n], 18:21] * rw[s, :n, None]
-        ref[sel] = osc.gather(recv[sel], m[sel], vsub, vmat[reps[s, :n]], 1, 0.01, 1e-3, ne, nthreads=8)
+        ref[sel] = osc.gather_clusters(recv[sel], m[sel], vpl[reps[s, :n]], vmat[reps[s, :n]], rw[s, :n], 1, 0.01,
+                                       1e-3, ne, nthreads=8)
     assert rel_l2(img.reshape(-1, 3), ref) <= 1e-4
     work = ls.count_work()
     assert work["gather_pairs"] == int(sum(cnt[sl[i]] for i in np.nonzero(m >= 0)[0]))

@@ -391,3 +391,39 @@ def test_cluster_by_sampling_draw_order(seed):
     nxt = oracle.lib().orc_mt_rand()
     assert nxt == pytest.approx(u[t + C], abs=0)
     assert repeats >= 0
+
+
+# ---------------------------------------------------------- gather_clusters ----------
+def test_gather_clusters_weights_after_the_clamp(cornell):
+    """lightslice.cc:1325-1339: radiance += weight * Lo, Lo already clamped inside
+    gather_point.  Unit weights give the plain gather bit for bit; other weights give
+    the weighted float64 sum of the clamped terms, which differs from folding the
+    weight into the VPL power once the RADIANCE clamp (not linear in power) engages."""
+    from renzoku_amd import scenes, vplgen
+    osc = oracle.OrcScene(cornell)
+    px, py = scenes.pixel_grid(256, 256)
+    sel = np.arange(0, 65536, 1499)
+    rays = oracle.camera_rays(cornell.camera, px[sel], py[sel])
+    t, b, g, tri = osc.trace_closest(rays)
+    recv, m, _ = osc.fill_receivers(rays, t, b, g, tri)
+    vpl, vmat, ne = vplgen.generate_vpls(cornell, osc.trace_closest, osc.fill_receivers, 48, seed=3, batch=512)
+    w = np.random.default_rng(2).uniform(0.5, 30.0, 48).astype(np.float32)
+    for mode, thr in ((1, 3e-6), (3, 0.04)):
+        ones = osc.gather_clusters(recv, m, vpl, vmat, np.ones(48, np.float32), mode, thr, 1e-3, ne)
+        assert np.array_equal(ones.view(np.uint32), osc.gather(recv, m, vpl, vmat, mode, thr, 1e-3, ne).view(np.uint32))
+        got = osc.gather_clusters(recv, m, vpl, vmat, w, mode, thr, 1e-3, ne)
+        exp = np.zeros((len(sel), 3))
+        for i in range(len(sel)):
+            for k in range(48):
+                lo, vis = osc.gather_point(recv[i], m[i], vpl[k], vmat[k], mode, thr, 1e-3)
+                if vis:
+                    exp[i] += float(w[k]) * lo.astype(np.float64)
+        exp /= ne
+        assert np.allclose(got, exp, rtol=1e-5, atol=1e-30)
+        folded = vpl.copy()
+        folded[:, 18:21] *= w[:, None]
+        alt = osc.gather(recv, m, folded, vmat, mode, thr, 1e-3, ne)
+        if mode == 3:
+            assert not np.allclose(alt, exp, rtol=1e-3)       # the clamp is not linear in power
+        else:
+            assert np.allclose(alt, exp, rtol=1e-5, atol=1e-30)
