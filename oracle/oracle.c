@@ -25,8 +25,12 @@
  *   DiscretePdf / maps ........ CDF invariants (monotone, last = 1, strict <)
  *   concentric map ............ round-trip invariant + Shirley-Chiu closed form
  * Functions whose parity with the reference binary rests only on the reading
- * of the source: glibc powf/expf/atan2f inside Phong/Ward/concentric (ULP level,
- * "parity unpinned" beyond the tolerance stated in the tests).
+ * of the source: glibc powf/expf inside the Phong/Ward evaluations (ULP level,
+ * "parity unpinned" beyond the tolerance stated in the tests).  The
+ * transcendentals that pick a sampled direction (sin, cos, tan, atan, atan2,
+ * acos, log, pow) are evaluated in double and rounded once, as the kernels do:
+ * the correctly rounded float, which glibc's float functions return outside
+ * hard-to-round cases.
  */
 #include "oracle.h"
 
@@ -980,6 +984,20 @@ int orc_cluster_by_sampling(int rows, int cols, const float *R, int num_clusters
 /* ======================================================================= */
 /* Concentric map and per-slice guiding distributions                       */
 /* ======================================================================= */
+/* Transcendentals that pick a ray direction: evaluated in double and rounded once
+ * to float, as the kernels do (the same float from host and device double
+ * functions, except within a double ulp of a float rounding boundary); the
+ * correctly rounded value, which glibc's float functions also return outside
+ * hard-to-round cases */
+static inline float sin_rd(float x) { return (float)sin((double)x); }
+static inline float cos_rd(float x) { return (float)cos((double)x); }
+static inline float tan_rd(float x) { return (float)tan((double)x); }
+static inline float atan_rd(float x) { return (float)atan((double)x); }
+static inline float atan2_rd(float y, float x) { return (float)atan2((double)y, (double)x); }
+static inline float acos_rd(float x) { return (float)acos((double)x); }
+static inline float log_rd(float x) { return (float)log((double)x); }
+static inline float pow_rd(float x, float y) { return (float)pow((double)x, (double)y); }
+
 /* concentric_mapping.h:16-47 */
 static void square_to_disk(float sx, float sy, float *u, float *v) {
     float phi, r;
@@ -991,14 +1009,14 @@ static void square_to_disk(float sx, float sy, float *u, float *v) {
         if (a < b) { r = -a; phi = (A_PI / 4) * (4 + (b / a)); }
         else { r = -b; if (b != 0.0f) phi = (A_PI / 4) * (6 - (a / b)); else phi = 0.0f; }
     }
-    *u = r * cosf(phi);
-    *v = r * sinf(phi);
+    *u = r * cos_rd(phi);
+    *v = r * sin_rd(phi);
 }
 
 /* concentric_mapping.h:49-74 */
 static void disk_to_square(float dx, float dy, float *x, float *y) {
     float r = sqrtf(dx * dx + dy * dy);
-    float phi = atan2f(dy, dx);
+    float phi = atan2_rd(dy, dx);
     if (phi < -A_PI / 4) phi += TWO_PI;
     float a, b;
     if (phi < A_PI / 4) { a = r; b = phi * a / (A_PI / 4); }
@@ -1194,7 +1212,7 @@ static v3 l2w(v3 u, v3 v, v3 w, v3 l) { return add(add(scl(u, l.x), scl(v, l.y))
 static v3 w2l(v3 u, v3 v, v3 w, v3 x) { return mk(dot(u, x), dot(v, x), dot(w, x)); }
 
 static v3 sph2w(v3 u, v3 v, v3 w, float theta, float phi) {       /* onb.cc:155-163 */
-    float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);
+    float st = sin_rd(theta), ct = cos_rd(theta), sp = sin_rd(phi), cp = cos_rd(phi);
     return l2w(u, v, w, mk(st * cp, st * sp, ct));
 }
 
@@ -1245,7 +1263,7 @@ static v3 bsdf_sample(const float *m, v3 u, v3 v, v3 w, v3 wo, const uint32_t *r
     if (type == ORC_LAMBERT) {
         float phi = TWO_PI * u24(r[1]);
         float st = sqrtf(u24(r[2]));
-        float x = st * cosf(phi), y = st * sinf(phi);
+        float x = st * cos_rd(phi), y = st * sin_rd(phi);
         float z = sqrtf(1.0f - st * st);
         *wi = unit(l2w(u, v, w, mk(x, y, z)));
         *pdf = z * INV_PI;
@@ -1256,7 +1274,7 @@ static v3 bsdf_sample(const float *m, v3 u, v3 v, v3 w, v3 wo, const uint32_t *r
     if (u24(r[0]) < rd) {
         float phi = TWO_PI * u24(r[1]);
         float st = sqrtf(u24(r[2]));
-        float x = st * cosf(phi), y = st * sinf(phi);
+        float x = st * cos_rd(phi), y = st * sin_rd(phi);
         float z = sqrtf(1.0f - st * st);
         *wi = unit(l2w(u, v, w, mk(x, y, z)));
     } else if (type == ORC_PHONG) {
@@ -1264,7 +1282,7 @@ static v3 bsdf_sample(const float *m, v3 u, v3 v, v3 w, v3 wo, const uint32_t *r
         v3 ou, ov, ow;
         onb_from_nu(wo, w, &ou, &ov, &ow);
         float phi = TWO_PI * u24(r[1]);
-        float alpha = acosf(powf(u24(r[2]), 1.0f / (ne + 1.0f)));
+        float alpha = acos_rd(pow_rd(u24(r[2]), 1.0f / (ne + 1.0f)));
         v3 wr = sph2w(ou, ov, ow, alpha, phi);
         float d = dot(wr, w);
         if (d < 0.0f) return mk(0.f, 0.f, 0.f);
@@ -1275,11 +1293,11 @@ static v3 bsdf_sample(const float *m, v3 u, v3 v, v3 w, v3 wo, const uint32_t *r
         float s0 = u24o(r[1]), s1 = u24o(r[2]);
         v3 lwo = unit(w2l(u, v, w, wo));
         float angle = TWO_PI * s0;
-        float phi = atanf(ay_ax * tanf(angle));
-        if (sinf(phi) * sinf(angle) < 0 || cosf(phi) * cosf(angle) < 0) phi += A_PI;
-        float cp = cosf(phi), sp = sinf(phi);
-        float theta = atanf(sqrtf(-logf(s1) / (cp * cp / ax2 + sp * sp / ay2)));
-        float st = sinf(theta), ct = cosf(theta);
+        float phi = atan_rd(ay_ax * tan_rd(angle));
+        if (sin_rd(phi) * sin_rd(angle) < 0 || cos_rd(phi) * cos_rd(angle) < 0) phi += A_PI;
+        float cp = cos_rd(phi), sp = sin_rd(phi);
+        float theta = atan_rd(sqrtf(-log_rd(s1) / (cp * cp / ax2 + sp * sp / ay2)));
+        float st = sin_rd(theta), ct = cos_rd(theta);
         v3 wh = mk(st * cp, st * sp, ct);
         v3 lwi = sub(scl(wh, 2.0f * dot(lwo, wh)), lwo);
         *wi = l2w(u, v, w, lwi);
@@ -1395,7 +1413,7 @@ static v3 guide_sample(const pt_args *a, int slice, const float *mat, v3 u, v3 v
                        const uint32_t *r, v3 *wi, float *pdf) {
     if (slice < 0) {
         float phi = TWO_PI * u24(r[0]);
-        float theta = acosf(u24(r[1]));
+        float theta = acos_rd(u24(r[1]));
         *wi = unit(sph2w(u, v, w, theta, phi));
     } else {
         int res = a->res;

@@ -341,9 +341,25 @@ RZK_HD void lobe_weights(const float* m, float& rho_d, float& rho_s) {
     if (total > 0.0f) { rho_d /= total; rho_s /= total; }
 }
 
+// Transcendentals that pick a ray direction, evaluated in double and rounded
+// once to float.  Host (C library) and device double functions are both within
+// one double ulp, so these give the same float on both sides except where the
+// double result lies within a double ulp of a float rounding boundary (about
+// 2^-29 of arguments): the host restatement traces the same rays as the kernels.
+// The rounded value is the correctly rounded float result, which the C
+// library's float functions also return outside hard-to-round cases.
+RZK_HD float sin_rd(float x) { return (float)sin((double)x); }
+RZK_HD float cos_rd(float x) { return (float)cos((double)x); }
+RZK_HD float tan_rd(float x) { return (float)tan((double)x); }
+RZK_HD float atan_rd(float x) { return (float)atan((double)x); }
+RZK_HD float atan2_rd(float y, float x) { return (float)atan2((double)y, (double)x); }
+RZK_HD float acos_rd(float x) { return (float)acos((double)x); }
+RZK_HD float log_rd(float x) { return (float)log((double)x); }
+RZK_HD float pow_rd(float x, float y) { return (float)pow((double)x, (double)y); }
+
 // Onb::spherical_to_world (onb.cc:155-163)
 RZK_HD v3 spherical_to_world(const Onb& b, float theta, float phi) {
-    float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);
+    float st = sin_rd(theta), ct = cos_rd(theta), sp = sin_rd(phi), cp = cos_rd(phi);
     return local_to_world(b, mk(st * cp, st * sp, ct));
 }
 
@@ -379,8 +395,8 @@ RZK_HD float bsdf_pdf(const float* m, const Onb& b, v3 wo, v3 wi) {
 RZK_HD v3 cosine_dir(const Onb& b, float u0, float u1) {
     float phi = kTwoPi * u0;
     float sin_theta = sqrtf(u1);
-    float x = sin_theta * cosf(phi);
-    float y = sin_theta * sinf(phi);
+    float x = sin_theta * cos_rd(phi);
+    float y = sin_theta * sin_rd(phi);
     float z = sqrtf(1.0f - sin_theta * sin_theta);
     return unit(local_to_world(b, mk(x, y, z)));
 }
@@ -395,7 +411,7 @@ RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3
     if (type == kLambert) {
         float phi = kTwoPi * u24(u[1]);
         float sin_theta = sqrtf(u24(u[2]));
-        float x = sin_theta * cosf(phi), y = sin_theta * sinf(phi);
+        float x = sin_theta * cos_rd(phi), y = sin_theta * sin_rd(phi);
         float z = sqrtf(1.0f - sin_theta * sin_theta);
         wi = unit(local_to_world(b, mk(x, y, z)));
         pdf = z * kInvPi;
@@ -409,7 +425,7 @@ RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3
         float ne = m[7];
         Onb o = onb_from_nu(wo, b.n);
         float phi = kTwoPi * u24(u[1]);
-        float alpha = acosf(powf(u24(u[2]), 1.0f / (ne + 1.0f)));
+        float alpha = acos_rd(pow_rd(u24(u[2]), 1.0f / (ne + 1.0f)));
         v3 wr = spherical_to_world(o, alpha, phi);
         float d = dot(wr, b.n);
         if (d < 0.0f) return mk(0.f, 0.f, 0.f);
@@ -421,11 +437,11 @@ RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3
         float s0 = u24_open(u[1]), s1 = u24_open(u[2]);
         v3 lwo = unit(world_to_local(b, wo));
         float angle = kTwoPi * s0;
-        float phi = atanf(ay_ax * tanf(angle));
-        if (sinf(phi) * sinf(angle) < 0 || cosf(phi) * cosf(angle) < 0) phi += kPi;
-        float cp = cosf(phi), sp = sinf(phi);
-        float theta = atanf(sqrtf(-logf(s1) / (cp * cp / ax2 + sp * sp / ay2)));
-        float st = sinf(theta), ct = cosf(theta);
+        float phi = atan_rd(ay_ax * tan_rd(angle));
+        if (sin_rd(phi) * sin_rd(angle) < 0 || cos_rd(phi) * cos_rd(angle) < 0) phi += kPi;
+        float cp = cos_rd(phi), sp = sin_rd(phi);
+        float theta = atan_rd(sqrtf(-log_rd(s1) / (cp * cp / ax2 + sp * sp / ay2)));
+        float st = sin_rd(theta), ct = cos_rd(theta);
         v3 wh = mk(st * cp, st * sp, ct);
         v3 lwi = wh * (2.0f * dot(lwo, wh)) - lwo;
         wi = local_to_world(b, lwi);
@@ -439,7 +455,7 @@ RZK_HD void direction_to_square(v3 wi, float& sx, float& sy) {
     float r2 = 1.0f - wi.z;
     float dx = wi.x / sqrtf(2.0f - r2), dy = wi.y / sqrtf(2.0f - r2);
     float r = sqrtf(dx * dx + dy * dy);
-    float phi = atan2f(dy, dx);
+    float phi = atan2_rd(dy, dx);
     if (phi < -kPi / 4) phi += kTwoPi;
     float a, bb;
     if (phi < kPi / 4) { a = r; bb = phi * a / (kPi / 4); }
@@ -460,7 +476,7 @@ RZK_HD v3 square_to_direction(float sx, float sy) {
         if (a < bb) { r = -a; phi = (kPi / 4) * (4 + (bb / a)); }
         else { r = -bb; if (bb != 0.0f) phi = (kPi / 4) * (6 - (a / bb)); else phi = 0.0f; }
     }
-    float dx = r * cosf(phi), dy = r * sinf(phi);
+    float dx = r * cos_rd(phi), dy = r * sin_rd(phi);
     float r2 = dx * dx + dy * dy;
     return mk(dx * sqrtf(2.0f - r2), dy * sqrtf(2.0f - r2), 1.0f - r2);
 }
