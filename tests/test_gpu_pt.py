@@ -3,11 +3,12 @@ oracle's restatement (vpl.cc:934-1041, area_light.cc:48-142) on the same
 scene, the same per-slice maps read back from the device and the same Philox
 streams.
 
-Bar: per-image radiance rel-L2 <= 1e-4 (fp32).  Ray arithmetic is restated
-bit-for-bit; sinf/cosf/acosf/powf/expf/atanf/logf come from different math
-libraries (glibc vs the device's), so a path can occasionally take another
-branch where a 1-ulp difference decides a hit; the test reports how many
-pixels differ at all."""
+Bar: per-image radiance rel-L2 <= 1e-6 (fp32).  Ray arithmetic is restated
+bit-for-bit, and the transcendentals that pick a sampled direction are the
+correctly rounded float of the double function on both sides, so both trace
+the same rays; what remains is the ulp-level difference of the float expf /
+powf inside the Phong and Ward evaluations (glibc vs the device library).  The
+test reports how many pixels differ at all."""
 import numpy as np
 import pytest
 
@@ -51,7 +52,7 @@ def _compare(sc, ls, osc, tile, **kw):
           f"rays {rays}", flush=True)
     assert np.isfinite(img).all()
     assert ref.sum() > 0
-    assert err <= 1e-4
+    assert err <= 1e-6
     return img, ref, rays
 
 
