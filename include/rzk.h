@@ -454,6 +454,17 @@ int rzk_guide_get(const rzk_guide *guide, const char *name, void *dst, int64_t b
 int rzk_render_guided(rzk_ctx *ctx, const rzk_guide *guide, const rzk_pt_params *params, float *out_rgb,
                       int64_t *out_rays);
 
+/* MonteCarloIntegrator::radiance(PrimaryRay&) for guided path tracing
+ * (monte_carlo_integrator.cc:40-43, vpl.cc:443-461 with trace_wellington,
+ * vpl.cc:934-1041) over a batch of given primary rays, the rays of a
+ * PrimaryRayBundle (sensor.h:124-149): ray i (a ray record) is sample smp[i] of
+ * pixel (pix[2i] = x, pix[2i+1] = y) of the guide's image; out_rgb[3i..] gets
+ * its radiance, unaveraged.  Of params only max_bounce, direct_lighting,
+ * guide_mode, seed and tmin are read; bounce b of ray i draws from
+ * Philox(seed, pixel, smp[i], b) as rzk_render_guided does. */
+int rzk_render_guided_rays(rzk_ctx *ctx, const rzk_guide *guide, const rzk_pt_params *params, const float *rays,
+                           const int32_t *pix, const int32_t *smp, int64_t n, float *out_rgb);
+
 typedef struct rzk_server_params {
     int64_t memory_budget;          /* bytes of device memory for admitted requests (guide tables,
                                        per-sample radiance, image) */

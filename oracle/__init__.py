@@ -91,6 +91,8 @@ def lib():
                                    C.POINTER(C.c_longlong)], I),
             "orc_bsdf_pdf": ([P, P, P, P, P], None), "orc_bsdf_sample": ([P, P, P, P, P, P, P], None),
             "orc_guided_pt": ([P, P, I, I, I, I, C.c_uint32, F, I, I, P, P, P, P, I, I, I, I, P, I], None),
+            "orc_guided_pt_rays": ([P, I, I, I, I, I, C.c_uint32, F, I, I, P, P, P, P, I, P, P, P, P, I], None),
+            "orc_camera_sample_rays": ([P, I, C.c_uint32, I, P, P, P, P], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -237,6 +239,26 @@ def guided_pt(osc, cam, spp, max_bounce, direct, guide_mode, seed, tmin, tile, r
     arrs = [None if a is None else _f(a) for a in (maps, row_cdf, col_cdf, totals)]
     lib().orc_guided_pt(osc.h, _p(cam), spp, max_bounce, direct, guide_mode, seed, tmin, tile, res,
                         *[_p(a) for a in arrs], x0, y0, w, h, _p(out), nthreads)
+    return out
+
+
+def guided_pt_rays(osc, width, height, max_bounce, direct, guide_mode, seed, tmin, tile, rays, pix, smp, res=0,
+                   maps=None, row_cdf=None, col_cdf=None, totals=None, nthreads=8):
+    """the guided path sampler from explicit primary rays (ray records): ray i is sample smp[i] of
+    pixel pix[i] = (x, y); returns each ray's radiance (n, 3)"""
+    rays, pix, smp = _f(rays).reshape(-1, 6), _i(pix).reshape(-1, 2), _i(smp)
+    out = np.zeros((len(rays), 3), np.float32)
+    arrs = [None if a is None else _f(a) for a in (maps, row_cdf, col_cdf, totals)]
+    lib().orc_guided_pt_rays(osc.h, width, height, max_bounce, direct, guide_mode, seed, tmin, tile, res,
+                             *[_p(a) for a in arrs], len(rays), _p(rays), _p(pix), _p(smp), _p(out), nthreads)
+    return out
+
+
+def camera_sample_rays(cam, spp, seed, px, py, smp):
+    """the primary ray records guided_pt draws for sample smp of pixel (px, py)"""
+    px, py, smp = _i(px), _i(py), _i(smp)
+    out = np.zeros((len(px), 6), np.float32)
+    lib().orc_camera_sample_rays(_p(_f(cam)), spp, seed, len(px), _p(px), _p(py), _p(smp), _p(out))
     return out
 
 

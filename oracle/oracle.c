@@ -1445,20 +1445,68 @@ static ray_t camera_sample_ray(const float *c, float x, float y) {
     return ray_ctor(ld(c), d);
 }
 
+/* the slice of pixel (x, y) when guiding (its tile, if the tile's map is not empty), else -1 */
+static int pt_pixel_slice(const pt_args *a, int x, int y) {
+    if (a->guide_mode != 1) return -1;
+    int sx = x / a->tile, sy = y / a->tile;
+    if (sx > a->tiles_x - 1) sx = a->tiles_x - 1;
+    if (sy > a->tiles_y - 1) sy = a->tiles_y - 1;
+    int sl = sy * a->tiles_x + sx;
+    return a->totals[sl] > 0.0f ? sl : -1;
+}
+
+/* One path sample from primary ray `ray`: VirtualPointLight::trace_wellington
+ * (vpl.cc:934-1041) unrolled into a throughput product; the numbers of bounce b
+ * come from Philox(seed, pixel, smp, b) */
+static v3 pt_trace_from(const pt_args *a, ray_t ray, uint32_t pixel, int slice, int smp) {
+    const orc_scene *s = a->s;
+    v3 L = mk(0.f, 0.f, 0.f), T = mk(1.f, 1.f, 1.f);
+    float th = 0, tb = 0, tg = 0;
+    int tri = -1;
+    bvh_closest(s, &ray, a->tmin, 1e6f, &th, &tb, &tg, &tri, NULL);
+    for (int bounce = 1; tri >= 0 && bounce <= a->max_bounce; ++bounce) {
+        float rc[ORC_RECV_STRIDE];
+        int m, lidx;
+        fill_receiver(s, &ray, th, tb, tg, tri, rc, &m, &lidx);
+        if (lidx >= 0) {
+            if (bounce == 1 && a->direct) L = add(L, mulv(T, light_query(&s->lights[lidx], ld(rc + 6), ld(rc + 9))));
+            break;
+        }
+        if (m < 0) break;
+        uint32_t u[12];
+        path_rng(a->seed, pixel, (uint32_t)smp, (uint32_t)bounce, u);
+        int brdf_sampling = u24(u[0]) < 0.5f;
+        if (bounce > 1 || a->direct) L = add(L, mulv(T, next_event(s, rc, m, u, a->tmin)));
+        if (bounce >= a->max_bounce) break;
+        const float *mat = s->mat + ORC_MAT_STRIDE * m;
+        v3 bu = ld(rc + 12), bv = ld(rc + 15), bw = ld(rc + 18);
+        v3 sn = ld(rc + 6), wo = ld(rc + 9), p = ld(rc);
+        int gslice = bounce == 1 ? slice : -1;
+        v3 wi, brdf;
+        float pdf;
+        if (brdf_sampling) brdf = bsdf_sample(mat, bu, bv, bw, wo, u + 8, &wi, &pdf);
+        else brdf = guide_sample(a, gslice, mat, bu, bv, bw, sn, wo, u + 8, &wi, &pdf);
+        if (!(pdf > 0.0f)) break;
+        ray = ray_ctor(p, wi);
+        tri = -1;
+        bvh_closest(s, &ray, a->tmin, 1e6f, &th, &tb, &tg, &tri, NULL);
+        if (tri < 0) break;
+        if (s->surf_light[s->tri_surface[tri]] >= 0) break;
+        float po = brdf_sampling ? guide_pdf(a, gslice, bu, bv, bw, sn, wi) : bsdf_pdf(mat, bu, bv, bw, wo, wi);
+        float weight = pdf / (pdf + po);
+        float cosv = fabsf(dot(wi, sn));
+        float sc = ((weight * cosv) / pdf) * 2.0f;
+        T = scl(mulv(T, brdf), sc);
+    }
+    return L;
+}
+
 static void pt_range(void *a_, int lo, int hi) {
     pt_args *a = a_;
-    const orc_scene *s = a->s;
     for (int t = lo; t < hi; ++t) {
         int x = a->x0 + t % a->w, y = a->y0 + t / a->w;
         uint32_t pixel = (uint32_t)(y * a->W + x);
-        int slice = -1;
-        if (a->guide_mode == 1) {
-            int sx = x / a->tile, sy = y / a->tile;
-            if (sx > a->tiles_x - 1) sx = a->tiles_x - 1;
-            if (sy > a->tiles_y - 1) sy = a->tiles_y - 1;
-            int sl = sy * a->tiles_x + sx;
-            if (a->totals[sl] > 0.0f) slice = sl;
-        }
+        int slice = pt_pixel_slice(a, x, y);
         int strata = 1;
         while ((strata + 1) * (strata + 1) <= a->spp) ++strata;
         int stratified = strata * strata == a->spp;
@@ -1472,45 +1520,7 @@ static void pt_range(void *a_, int lo, int hi) {
                 jy = ((float)(smp / strata) + jy) / (float)strata;
             }
             ray_t ray = camera_sample_ray(a->cam, (float)x + jx, (float)y + jy);
-            v3 L = mk(0.f, 0.f, 0.f), T = mk(1.f, 1.f, 1.f);
-            float th = 0, tb = 0, tg = 0;
-            int tri = -1;
-            bvh_closest(s, &ray, a->tmin, 1e6f, &th, &tb, &tg, &tri, NULL);
-            for (int bounce = 1; tri >= 0 && bounce <= a->max_bounce; ++bounce) {
-                float rc[ORC_RECV_STRIDE];
-                int m, lidx;
-                fill_receiver(s, &ray, th, tb, tg, tri, rc, &m, &lidx);
-                if (lidx >= 0) {
-                    if (bounce == 1 && a->direct) L = add(L, mulv(T, light_query(&s->lights[lidx], ld(rc + 6), ld(rc + 9))));
-                    break;
-                }
-                if (m < 0) break;
-                uint32_t u[12];
-                path_rng(a->seed, pixel, (uint32_t)smp, (uint32_t)bounce, u);
-                int brdf_sampling = u24(u[0]) < 0.5f;
-                if (bounce > 1 || a->direct) L = add(L, mulv(T, next_event(s, rc, m, u, a->tmin)));
-                if (bounce >= a->max_bounce) break;
-                const float *mat = s->mat + ORC_MAT_STRIDE * m;
-                v3 bu = ld(rc + 12), bv = ld(rc + 15), bw = ld(rc + 18);
-                v3 sn = ld(rc + 6), wo = ld(rc + 9), p = ld(rc);
-                int gslice = bounce == 1 ? slice : -1;
-                v3 wi, brdf;
-                float pdf;
-                if (brdf_sampling) brdf = bsdf_sample(mat, bu, bv, bw, wo, u + 8, &wi, &pdf);
-                else brdf = guide_sample(a, gslice, mat, bu, bv, bw, sn, wo, u + 8, &wi, &pdf);
-                if (!(pdf > 0.0f)) break;
-                ray = ray_ctor(p, wi);
-                tri = -1;
-                bvh_closest(s, &ray, a->tmin, 1e6f, &th, &tb, &tg, &tri, NULL);
-                if (tri < 0) break;
-                if (s->surf_light[s->tri_surface[tri]] >= 0) break;
-                float po = brdf_sampling ? guide_pdf(a, gslice, bu, bv, bw, sn, wi) : bsdf_pdf(mat, bu, bv, bw, wo, wi);
-                float weight = pdf / (pdf + po);
-                float cosv = fabsf(dot(wi, sn));
-                float sc = ((weight * cosv) / pdf) * 2.0f;
-                T = scl(mulv(T, brdf), sc);
-            }
-            acc = add(acc, L);
+            acc = add(acc, pt_trace_from(a, ray, pixel, slice, smp));
         }
         size_t o = (size_t)y * a->W + x;
         a->out[3 * o] = acc.x / (float)a->spp;
@@ -1527,6 +1537,52 @@ void orc_guided_pt(const orc_scene *s, const float cam[ORC_CAM_STRIDE], int spp,
     pt_args a = {s, cam, spp, max_bounce, direct, guide_mode, seed, tmin, tile, res,
                  (W + tile - 1) / tile, (H + tile - 1) / tile, W, x0, y0, w, h, maps, row_cdf, col_cdf, totals, out};
     parallel_for(w * h, nthreads, pt_range, &a);
+}
+
+/* the primary ray record of sample smp[i] of pixel (px[i], py[i]) as orc_guided_pt
+ * draws it for spp samples per pixel (jitter from Philox(seed, pixel, sample, 0),
+ * 2-D strata when spp is a square; Camera::shoot_ray(SensorSample), camera.cc:67-80) */
+void orc_camera_sample_rays(const float cam[ORC_CAM_STRIDE], int spp, uint32_t seed, int n, const int *px,
+                            const int *py, const int *smp, float *out) {
+    int W = (int)cam[17];
+    int strata = 1;
+    while ((strata + 1) * (strata + 1) <= spp) ++strata;
+    int stratified = strata * strata == spp;
+    for (int i = 0; i < n; ++i) {
+        uint32_t r0[12];
+        path_rng(seed, (uint32_t)(py[i] * W + px[i]), (uint32_t)smp[i], 0u, r0);
+        float jx = u24(r0[0]), jy = u24(r0[1]);
+        if (stratified) {
+            jx = ((float)(smp[i] % strata) + jx) / (float)strata;
+            jy = ((float)(smp[i] / strata) + jy) / (float)strata;
+        }
+        ray_t r = camera_sample_ray(cam, (float)px[i] + jx, (float)py[i] + jy);
+        st(out + ORC_RAY_STRIDE * i, r.o);
+        st(out + ORC_RAY_STRIDE * i + 3, r.d);
+    }
+}
+
+typedef struct { const pt_args *a; const float *rays; const int *pix; const int *smp; float *out; } ptr_args;
+
+static void ptr_range(void *a_, int lo, int hi) {
+    ptr_args *q = a_;
+    for (int i = lo; i < hi; ++i) {
+        int x = q->pix[2 * i], y = q->pix[2 * i + 1];
+        ray_t ray = ray_from_record(q->rays + ORC_RAY_STRIDE * i);
+        v3 L = pt_trace_from(q->a, ray, (uint32_t)(y * q->a->W + x), pt_pixel_slice(q->a, x, y), q->smp[i]);
+        q->out[3 * i] = L.x; q->out[3 * i + 1] = L.y; q->out[3 * i + 2] = L.z;
+    }
+}
+
+void orc_guided_pt_rays(const orc_scene *s, int width, int height, int max_bounce, int direct, int guide_mode,
+                        uint32_t seed, float tmin, int tile, int res, const float *maps, const float *row_cdf,
+                        const float *col_cdf, const float *totals, int n, const float *rays, const int *pix,
+                        const int *smp, float *out, int nthreads) {
+    pt_args a = {s, NULL, 1, max_bounce, direct, guide_mode, seed, tmin, tile, res,
+                 (width + tile - 1) / tile, (height + tile - 1) / tile, width, 0, 0, 0, 0, maps, row_cdf, col_cdf,
+                 totals, NULL};
+    ptr_args q = {&a, rays, pix, smp, out};
+    parallel_for(n, nthreads, ptr_range, &q);
 }
 
 /* ======================================================================= */

@@ -162,6 +162,17 @@ void orc_guided_pt(const orc_scene *s, const float cam[ORC_CAM_STRIDE], int spp,
                    int guide_mode, uint32_t seed, float tmin, int tile, int res, const float *maps,
                    const float *row_cdf, const float *col_cdf, const float *totals, int x0, int y0, int w,
                    int h, float *out, int nthreads);
+/* the same path sampler from explicit primary rays: ray i (a ray record) belongs
+ * to pixel (pix[2i], pix[2i+1]) of a width x height image and is that pixel's
+ * sample smp[i]; out[3i..] is its radiance (MonteCarloIntegrator::radiance(PrimaryRay&),
+ * monte_carlo_integrator.cc:40-43) */
+void orc_guided_pt_rays(const orc_scene *s, int width, int height, int max_bounce, int direct, int guide_mode,
+                        uint32_t seed, float tmin, int tile, int res, const float *maps, const float *row_cdf,
+                        const float *col_cdf, const float *totals, int n, const float *rays, const int *pix,
+                        const int *smp, float *out, int nthreads);
+/* the primary ray records orc_guided_pt draws: sample smp[i] of pixel (px[i], py[i]) at spp samples per pixel */
+void orc_camera_sample_rays(const float cam[ORC_CAM_STRIDE], int spp, uint32_t seed, int n, const int *px,
+                            const int *py, const int *smp, float *out);
 
 /* bounce-1 VPL generation (vpl_generator.cc:127-181); see oracle.c */
 void orc_sincos_turn(float u, float *s, float *c);

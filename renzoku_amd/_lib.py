@@ -36,7 +36,7 @@ EXPORTS = [
     "rzk_lightslice_guide", "rzk_guide_destroy",
     "rzk_batch_create", "rzk_batch_destroy", "rzk_batch_set_config", "rzk_batch_get_config", "rzk_batch_render",
     "rzk_batch_autotune",
-    "rzk_lightslice_clusters", "rzk_clusters_destroy", "rzk_guide_create", "rzk_guide_get", "rzk_render_guided",
+    "rzk_lightslice_clusters", "rzk_clusters_destroy", "rzk_guide_create", "rzk_guide_get", "rzk_render_guided", "rzk_render_guided_rays",
     "rzk_server_create", "rzk_server_destroy", "rzk_server_submit", "rzk_server_wait", "rzk_server_get_stats",
     "rzk_server_last_error",
     "rzk_kdslices_build", "rzk_kdslices_build_dev", "rzk_kdslices_destroy", "rzk_kdslices_info", "rzk_kdslices_get", "rzk_kdslices_find",
@@ -211,6 +211,7 @@ def load() -> C.CDLL:
         "rzk_guide_create": [P, P, P, P, C.POINTER(GuideParams), C.POINTER(P)],
         "rzk_guide_get": [P, C.c_char_p, P, I64],
         "rzk_render_guided": [P, P, C.POINTER(PtParams), P, P],
+        "rzk_render_guided_rays": [P, P, C.POINTER(PtParams), P, P, P, I64, P],
         "rzk_server_create": [C.c_int, P, P, C.POINTER(ServerParams), C.POINTER(P)],
         "rzk_server_destroy": [P],
         "rzk_server_submit": [P, C.POINTER(Request), C.POINTER(I64)],

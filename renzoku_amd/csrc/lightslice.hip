@@ -659,6 +659,34 @@ int rzk_render_guided(rzk_ctx* ctx, const rzk_guide* g, const rzk_pt_params* p, 
     });
 }
 
+int rzk_render_guided_rays(rzk_ctx* ctx, const rzk_guide* g, const rzk_pt_params* p, const float* rays,
+                           const int32_t* pix, const int32_t* smp, int64_t n, float* out_rgb) {
+    if (!ctx || !g || !p) return RZK_E_INVALID;
+    return ls_guard(ctx, [&] {
+        RZK_CHECK(n >= 0 && (n == 0 || (rays && pix && smp && out_rgb)), RZK_E_INVALID, "bad ray arguments");
+        RZK_CHECK(p->max_bounce >= 0 && (p->guide_mode == 0 || p->guide_mode == 1), RZK_E_INVALID,
+                  "bad path tracing parameters");
+        RZK_CHECK(g->tile > 0 && g->tiles_x > 0 && g->tiles_y > 0, RZK_E_INVALID, "guide without slices");
+        for (int64_t i = 0; i < n; ++i)
+            RZK_CHECK(pix[2 * i] >= 0 && pix[2 * i] < g->W && pix[2 * i + 1] >= 0 && pix[2 * i + 1] < g->H &&
+                          smp[i] >= 0,
+                      RZK_E_INVALID, "ray pixel outside the guide's image");
+        if (n == 0) return;
+        rzk_render_job j{g, 0, 0, g->W, g->H, 1, p->max_bounce, p->direct_lighting, p->guide_mode, p->seed, p->tmin};
+        PtParams pp = job_params(*g, j);
+        DBuf<float> d_rays, d_out;
+        DBuf<int> d_pix, d_smp;
+        d_rays.alloc(6 * n); d_out.alloc(3 * n); d_pix.alloc(2 * n); d_smp.alloc(n);
+        hipStream_t st = ctx->stream;
+        RZK_HIP(hipMemcpyAsync(d_rays.p, rays, sizeof(float) * 6 * n, hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(d_pix.p, pix, sizeof(int) * 2 * n, hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(d_smp.p, smp, sizeof(int) * n, hipMemcpyHostToDevice, st));
+        launch_guided_pt_rays(ctx, scene_view(g->scene), pp, g->gv, d_rays.p, d_pix.p, d_smp.p, n, d_out.p);
+        RZK_HIP(hipMemcpyAsync(out_rgb, d_out.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st));
+        ls_sync(ctx);
+    });
+}
+
 int rzk_render_vpl(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const float* camera, int clamp_mode,
                    float clamp, float tmin, float n_emitted, float* out_rgb) {
     return ls_guard(ctx, [&] {

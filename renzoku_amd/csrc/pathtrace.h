@@ -142,6 +142,8 @@ void enqueue_guide_build(rzk_ctx* ctx, const rzk_scene* scene, const rzk_cluster
 void guide_set(rzk_guide& g, const rzk_scene* scene, const float* cam, const rzk_guide_params& gp,
                const GuideLayout& L, const GuideBuffers& b);
 
+void launch_guided_pt_rays(rzk_ctx* ctx, const SceneView& sv, const rzk::PtParams& pp, const rzk::GuideView& gv,
+                           const float* d_rays, const int* d_pix, const int* d_smp, int64_t n, float* d_out);
 void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv, const float* cam, const rzk::PtParams& pp,
                       const rzk::GuideView& gv, float* d_out, unsigned long long* d_work);
 // the three kernels of a batch on `stream` (captured into a graph by the caller)

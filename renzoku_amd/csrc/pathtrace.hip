@@ -199,20 +199,13 @@ __device__ __forceinline__ int spp_strata(int spp) {
     return strata * strata == spp ? strata : 0;
 }
 
-// One path sample of pixel (x, y): trace_wellington unrolled into a
-// throughput product (vpl.cc:934-1041).  Every random number comes from
-// Philox(seed, pixel, sample, bounce), so the value depends on nothing else.
-__device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& pp, const GuideView& gv, int x,
-                         int y, int slice, int strata, int smp, AnyStack& astk, TravStats& st,
-                         unsigned long long& nclosest, unsigned long long& nshadow) {
+// One path sample from primary ray `ray` of pixel (x, y): trace_wellington
+// unrolled into a throughput product (vpl.cc:934-1041).  The numbers of bounce
+// b come from Philox(seed, pixel, sample, b), so the value depends on nothing else.
+__device__ v3 trace_from(const SceneView& sv, const PtParams& pp, const GuideView& gv, Ray ray, int x, int y,
+                         int slice, int smp, AnyStack& astk, TravStats& st, unsigned long long& nclosest,
+                         unsigned long long& nshadow) {
     uint32_t pixel = (uint32_t)(y * pp.width + x);
-    PathRng r0 = path_rng(pp.seed, pixel, (uint32_t)smp, 0u);
-    float jx = u24(r0.u[0]), jy = u24(r0.u[1]);
-    if (strata > 0) {
-        jx = ((float)(smp % strata) + jx) / (float)strata;
-        jy = ((float)(smp / strata) + jy) / (float)strata;
-    }
-    Ray ray = camera_sample_ray(cam, (float)x + jx, (float)y + jy);
     v3 L = mk(0.f, 0.f, 0.f), T = mk(1.f, 1.f, 1.f);
     float th = 0.f, tb = 0.f, tg = 0.f;
     nclosest++;
@@ -255,6 +248,22 @@ __device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& 
     return L;
 }
 
+// One path sample of pixel (x, y): the camera ray through the sample's jittered
+// position (2-D strata when spp is a square), then trace_from.
+__device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& pp, const GuideView& gv, int x,
+                         int y, int slice, int strata, int smp, AnyStack& astk, TravStats& st,
+                         unsigned long long& nclosest, unsigned long long& nshadow) {
+    uint32_t pixel = (uint32_t)(y * pp.width + x);
+    PathRng r0 = path_rng(pp.seed, pixel, (uint32_t)smp, 0u);
+    float jx = u24(r0.u[0]), jy = u24(r0.u[1]);
+    if (strata > 0) {
+        jx = ((float)(smp % strata) + jx) / (float)strata;
+        jy = ((float)(smp / strata) + jy) / (float)strata;
+    }
+    Ray ray = camera_sample_ray(cam, (float)x + jx, (float)y + jy);
+    return trace_from(sv, pp, gv, ray, x, y, slice, smp, astk, st, nclosest, nshadow);
+}
+
 __device__ __forceinline__ void add_ray_counts(unsigned long long* work, unsigned long long a, unsigned long long c) {
     for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off); c += __shfl_xor(c, off); }
     if ((threadIdx.x & 63) == 0) { atomicAdd(work, a); atomicAdd(work + 1, c); }
@@ -286,6 +295,29 @@ __global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, Pt
     if (work) {
         add_ray_counts(work, nclosest, nshadow);
         add_work_totals(work + 2, st);        // 4-wide nodes visited, triangles tested (per lane)
+    }
+}
+
+// One lane per given primary ray: ray i (a ray record) is sample smp[i] of pixel
+// (pix[2i], pix[2i+1]); out[3i..] is its radiance, unaveraged
+// (MonteCarloIntegrator::radiance(PrimaryRay&), monte_carlo_integrator.cc:40-43).
+__global__ __launch_bounds__(128) void k_guided_pt_rays(SceneView sv, PtParams pp, GuideView gv,
+                                                        const float* __restrict__ rays, const int* __restrict__ pix,
+                                                        const int* __restrict__ smp, int64_t n,
+                                                        float* __restrict__ out) {
+    extern __shared__ int s_stack[];
+    AnyStack astk = any_stack(sv, s_stack);
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    TravStats st;
+    unsigned long long nclosest = 0, nshadow = 0;
+    if (i < n) {
+        const float* rp = rays + 6 * i;
+        Ray r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
+        int x = pix[2 * i], y = pix[2 * i + 1];
+        v3 L = trace_from(sv, pp, gv, r, x, y, pixel_slice(pp, gv, x, y), smp[i], astk, st, nclosest, nshadow);
+        out[3 * i] = L.x;
+        out[3 * i + 1] = L.y;
+        out[3 * i + 2] = L.z;
     }
 }
 
@@ -826,6 +858,19 @@ void launch_batch(hipStream_t stream, const SceneView& sv, const BatchHeader* d_
     size_t lds = sizeof(int) * (size_t)block * (size_t)kLdsStack;
     hipLaunchKernelGGL(k_batch_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
     hipLaunchKernelGGL(k_batch_reduce, dim3(reduce_grid), dim3(256), 0, stream, d_hdr);
+}
+
+void launch_guided_pt_rays(rzk_ctx* ctx, const SceneView& sv_in, const PtParams& pp, const GuideView& gv,
+                           const float* d_rays, const int* d_pix, const int* d_smp, int64_t n, float* d_out) {
+    if (n <= 0) return;
+    SceneView sv = sv_in;
+    const int block = 128;
+    unsigned grid = (unsigned)((n + block - 1) / block);
+    bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4);
+    size_t lds = sizeof(int) * block * (size_t)kLdsStack;
+    hipLaunchKernelGGL(k_guided_pt_rays, dim3(grid), dim3(block), lds, ctx->stream, sv, pp, gv, d_rays, d_pix, d_smp,
+                       n, d_out);
+    RZK_HIP(hipGetLastError());
 }
 
 void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv_in, const float* cam, const PtParams& pp, const GuideView& gv,

@@ -6,7 +6,9 @@
 // input : int32 ntri, nsurf, nmat, nlight, nray, nseg
 //         f32 pos[ntri*9], tan[ntri*3]; i32 tri_surface[ntri], surf_material, surf_light, surf_flags [nsurf]
 //         f32 mat[nmat*12], radiance[nlight*3], rays[nray*6] (o, d unnormalised), seg_p[nseg*3], seg_q[nseg*3]
-// output: per ray f32 t, beta, gamma, i32 tri, u8 any; per segment u8 blocked
+// output: per ray f32 t, beta, gamma, i32 tri, u8 any; per segment u8 blocked; per ray f32 normal[3],
+//         shading_normal[3], uvn[9] (u, v, n), tangent[3], i32 surface index of record.surface and of
+//         record.material (-1 for null), the record's pointers checked against the surface bindings
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -43,12 +45,21 @@ int main(int argc, char** argv) {
     d.num_materials = nmat; d.materials = mat.data(); d.num_lights = nlight; d.light_radiance = rad.data();
     int status = 0;
     {
-        AggregateGpu gpu(ctx, d);
+        // one binding per surface: distinct addresses standing for the reference's Surface /
+        // Material / AreaLight objects (material and light null where the surface has none)
+        std::vector<char> surf_obj(nsurf + 1), mat_obj(nmat + 1), light_obj(nlight + 1);
+        std::vector<SurfaceBinding> bind(nsurf);
+        for (int k = 0; k < nsurf; ++k) {
+            bind[k].surface = reinterpret_cast<Surface*>(&surf_obj[k]);
+            bind[k].material = sm[k] >= 0 ? reinterpret_cast<Material*>(&mat_obj[sm[k]]) : nullptr;
+            bind[k].light = sl[k] >= 0 ? reinterpret_cast<AreaLight*>(&light_obj[sl[k]]) : nullptr;
+        }
+        AggregateGpu gpu(ctx, d, bind);
         const Aggregate* agg = &gpu;                  // callers see only the reference's interface
         const Float tmin = 1e-3f, tmax = 1e6f;
         std::vector<Ray> rs(nray);
-        std::vector<float> t(nray), b(nray), g(nray);
-        std::vector<int32_t> tri(nray);
+        std::vector<float> t(nray), b(nray), g(nray), frame(18 * (size_t)nray);
+        std::vector<int32_t> tri(nray), sidx(nray), midx(nray);
         std::vector<uint8_t> any(nray), blocked(nseg);
         for (int i = 0; i < nray; ++i) {
             const float* r = &rays[6 * (size_t)i];
@@ -56,9 +67,22 @@ int main(int argc, char** argv) {
             HitRecord rec;
             bool h = agg->hit(rs[i], tmin, tmax, 0.f, rec);
             t[i] = h ? rec.t : 0.f; b[i] = h ? rec.beta : 0.f; g[i] = h ? rec.gamma : 0.f;
-            tri[i] = h ? rec.triangle : -1;
             any[i] = agg->hit(rs[i], tmin, tmax, 0.f) ? 1 : 0;
             if (any[i] != (h ? 1 : 0)) status = 1;    // any hit <=> a closest hit in [tmin, tmax]
+            Vec3 v[6] = {rec.normal, rec.shading_normal, rec.uvn.u(), rec.uvn.v(), rec.uvn.n(), rec.tangent};
+            for (int k = 0; k < 6; ++k)
+                for (int c = 0; c < 3; ++c) frame[18 * (size_t)i + 3 * k + c] = h ? v[k][c] : 0.f;
+            sidx[i] = rec.surface ? (int32_t)(reinterpret_cast<char*>(rec.surface) - surf_obj.data()) : -1;
+            midx[i] = rec.material ? (int32_t)(reinterpret_cast<char*>(rec.material) - mat_obj.data()) : -1;
+            if (h && sidx[i] >= 0 && (rec.material != bind[sidx[i]].material || rec.light != bind[sidx[i]].light))
+                status = 1;                           // the record carries its surface's bindings
+        }
+        // triangle indices come out of band from the batched form
+        {
+            std::vector<HitRecord> hb;
+            std::vector<int> tb;
+            gpu.hit_batch(rs, tmin, tmax, hb, &tb);
+            for (int i = 0; i < nray; ++i) tri[i] = tb[i];
         }
         std::vector<Vec3> P(nseg), Q(nseg);
         for (int i = 0; i < nseg; ++i) {
@@ -70,7 +94,7 @@ int main(int argc, char** argv) {
         std::vector<HitRecord> hb;
         gpu.hit_batch(rs, tmin, tmax, hb);
         for (int i = 0; i < nray; ++i)
-            if (hb[i].triangle != tri[i] || (tri[i] >= 0 && memcmp(&hb[i].t, &t[i], 4) != 0)) status = 1;
+            if (hb[i].hit != (tri[i] >= 0) || (tri[i] >= 0 && memcmp(&hb[i].t, &t[i], 4) != 0)) status = 1;
         std::vector<uint8_t> vb;
         gpu.visible_batch(P, Q, tmin, vb);
         if (vb != blocked) status = 1;
@@ -78,6 +102,7 @@ int main(int argc, char** argv) {
         if (!o) return 2;
         fwrite(t.data(), 4, nray, o); fwrite(b.data(), 4, nray, o); fwrite(g.data(), 4, nray, o);
         fwrite(tri.data(), 4, nray, o); fwrite(any.data(), 1, nray, o); fwrite(blocked.data(), 1, nseg, o);
+        fwrite(frame.data(), 4, 18 * (size_t)nray, o); fwrite(sidx.data(), 4, nray, o); fwrite(midx.data(), 4, nray, o);
         fclose(o);
     }
     rzk_ctx_destroy(ctx);

@@ -1,7 +1,9 @@
 """The C++ drop-in adapter (renzoku_amd/adapters/aggregate_gpu.h): a compiled
 program calls the three Aggregate::hit overloads through a base-class pointer
 (closest, any, segment; aggregate.h:20-26) and the batched forms; answers are
-compared with the oracle (hit ids and t bit-exact, booleans exact)."""
+compared with the oracle (hit ids and t bit-exact, booleans exact), and the
+closest hit's whole HitRecord (normals, uvn frame, tangent, surface and
+material bindings) with the oracle's receiver of the same hit."""
 import os
 import subprocess
 
@@ -51,3 +53,17 @@ def test_aggregate_adapter_matches_oracle(tmp_path, small_room):
     assert np.array_equal(t[hit].view(np.uint32), ot[hit].view(np.uint32))
     assert np.array_equal(anyh, osc.trace_any(orays))
     assert np.array_equal(blocked, osc.trace_segments(p, q))
+    # the whole record: normals, uvn frame and tangent (triangle.cc:277-296, surface.cc:62-96) are the
+    # oracle's receiver fields bit for bit; surface and material pointers are the hit surface's bindings
+    off = 17 * n + m
+    frame = np.frombuffer(raw, np.float32, 18 * n, off).reshape(n, 18)
+    sidx = np.frombuffer(raw, np.int32, n, off + 72 * n)
+    midx = np.frombuffer(raw, np.int32, n, off + 76 * n)
+    recv, om, _ = osc.fill_receivers(orays, ot, ob, og, otri)
+    assert np.array_equal(frame[hit, 0:6].view(np.uint32), recv[hit, 3:9].view(np.uint32))      # n, n_shading
+    assert np.array_equal(frame[hit, 6:15].view(np.uint32), recv[hit, 12:21].view(np.uint32))   # uvn
+    assert np.array_equal(frame[hit, 15:18], np.asarray(sc.tangents, np.float32).reshape(-1, 3)[otri[hit]])
+    surf = np.asarray(sc.tri_surface)[otri[hit]]
+    assert np.array_equal(sidx[hit], surf)
+    assert np.array_equal(midx[hit], om[hit])
+    assert (sidx[~hit] == -1).all() and (midx[~hit] == -1).all()

@@ -179,3 +179,31 @@ def test_window_and_determinism():
     assert np.array_equal(full[3:16, 5:16], win[3:16, 5:16])
     assert not win[:3].any() and not win[:, :5].any()
     assert full.sum() > 0
+
+
+def test_rays_form_is_the_image_form():
+    """guided_pt_rays on the very rays guided_pt draws (camera_sample_rays), summed
+    per pixel in sample order and divided by spp, is guided_pt's image bit for bit
+    (guided, stratified 4 spp); another sample number changes the value"""
+    cs = scenes.cornell_box(24, 20)
+    osc = oracle.OrcScene(cs)
+    tile, res, spp = 8, 8, 4
+    ns = 3 * 3
+    maps, row_cdf, col_cdf, totals = _random_guide(np.random.default_rng(4), ns, res)
+    g = dict(res=res, maps=maps, row_cdf=row_cdf, col_cdf=col_cdf, totals=totals)
+    img = oracle.guided_pt(osc, cs.camera, spp, 4, 1, 1, 11, 1e-3, tile, nthreads=8, **g)
+    py, px = np.mgrid[0:20, 0:24]
+    px, py = px.ravel().astype(np.int32), py.ravel().astype(np.int32)
+    acc = np.zeros((len(px), 3), np.float32)
+    for s in range(spp):
+        smp = np.full(len(px), s, np.int32)
+        rays = oracle.camera_sample_rays(cs.camera, spp, 11, px, py, smp)
+        acc += oracle.guided_pt_rays(osc, 24, 20, 4, 1, 1, 11, 1e-3, tile, rays, np.stack([px, py], 1), smp, **g)
+    acc /= np.float32(spp)
+    assert np.array_equal(acc.reshape(20, 24, 3).view(np.uint32), img.view(np.uint32))
+    rays = oracle.camera_sample_rays(cs.camera, spp, 11, px, py, np.zeros(len(px), np.int32))
+    a = oracle.guided_pt_rays(osc, 24, 20, 4, 1, 1, 11, 1e-3, tile, rays, np.stack([px, py], 1),
+                              np.zeros(len(px), np.int32), **g)
+    b = oracle.guided_pt_rays(osc, 24, 20, 4, 1, 1, 11, 1e-3, tile, rays, np.stack([px, py], 1),
+                              np.full(len(px), 7, np.int32), **g)
+    assert not np.array_equal(a, b)
