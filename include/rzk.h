@@ -191,7 +191,15 @@ int rzk_map_vec6(rzk_ctx *ctx, const rzk_scene *scene, const float *recv, const 
 typedef struct rzk_kdslices rzk_kdslices;
 int rzk_kdslices_build(rzk_ctx *ctx, const float *points, int32_t n, int32_t max_slice_size, uint32_t seed,
                        rzk_kdslices **out);
-/* the same with the points already in device memory (n x 6, on the context's stream) */
+/* the same with every pivot drawn from one MT19937 stream seeded with seed, in
+ * cluster_kdtree's depth-first recursion order, as the reference draws them from
+ * the scene's Random (kdtree_n.h:192, :399-429).  The draw count of each node
+ * depends on its points, so the build is sequential and runs on the host; the
+ * tree lands on the device in the same layout.  *out_draws (may be NULL) = the
+ * numbers drawn; the representatives' draws (lightslice.cc:293-301) come next. */
+int rzk_kdslices_build_mt(rzk_ctx *ctx, const float *points, int32_t n, int32_t max_slice_size, uint32_t seed,
+                          int64_t *out_draws, rzk_kdslices **out);
+/* the same as rzk_kdslices_build with the points already in device memory (n x 6, on the context's stream) */
 int rzk_kdslices_build_dev(rzk_ctx *ctx, const float *d_points, int32_t n, int32_t max_slice_size, uint32_t seed,
                            rzk_kdslices **out);
 int rzk_kdslices_destroy(rzk_kdslices *kd);
@@ -292,6 +300,12 @@ typedef struct rzk_lightslice_params {
                                        many points (max_slice_size = 256, lightslice.cc:27),
                                        a random representative per slice and its nearest
                                        slices as neighbours; tile is then unused */
+    int32_t kd_pivots;              /* kd-tree pivot stream: 0 = round t of node [s, e] from
+                                       Philox keyed by (seed, s, e, t), every node of a depth at
+                                       once on the device; 1 = the reference's: every pivot the
+                                       next number of the LightSlice's MT stream in depth-first
+                                       recursion order (kdtree_n.h:192), before the
+                                       representatives' draws; a sequential host build */
 } rzk_lightslice_params;
 
 typedef struct rzk_lightslice_info {

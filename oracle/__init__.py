@@ -79,6 +79,7 @@ def lib():
             "orc_slice_cost": ([I, P, I, P, I, P], F),
             "orc_map_vec6": ([P, P, F, P], None),
             "orc_kd_build": ([P, I, I, C.c_uint32, P, P, P, I, P, P, P, I, P], I),
+            "orc_kd_build_mt": ([P, I, I, P, P, P, I, P, P, P, I, P, P], I),
             "orc_kd_find": ([P, P, P, I, P, I, P], None),
             "orc_kd_nearest": ([P, P, P, I, P, I, P, I, I, P], None),
             "orc_direction_to_square": ([P, P], None), "orc_square_to_direction": ([P, P], None),
@@ -522,9 +523,12 @@ def map_vec6(recv, mats, materials, diagonal):
 
 class KdSlices:
     """cluster_kdtree over n 6-D points: order, slices (S x 2), boxes (S x 12),
-    heap arrays axis / plane / cl of length heap_used"""
+    heap arrays axis / plane / cl of length heap_used.  pivots="philox": round t
+    of node [s, e] keyed by (seed, s, e, t); pivots="mt": the global MT stream
+    seeded with seed, drawn in recursion order as the reference does (draws =
+    numbers taken; the stream is left where the build stopped)."""
 
-    def __init__(self, pts, max_slice_size=256, seed=2013):
+    def __init__(self, pts, max_slice_size=256, seed=2013, pivots="philox"):
         L = lib()
         pts = _f(pts).reshape(-1, 6)
         n = len(pts)
@@ -536,9 +540,20 @@ class KdSlices:
         plane = np.zeros(cap_heap, np.float32)
         cl = np.zeros(cap_heap, np.int32)
         used = C.c_int(0)
-        S = L.orc_kd_build(_p(pts), n, int(max_slice_size), int(seed) & 0xffffffff, _p(self.order),
-                           _p(self.slices_buf), _p(self.boxes_buf), max(n, 1), _p(axis), _p(plane), _p(cl),
-                           cap_heap, C.byref(used))
+        self.draws = 0
+        if pivots == "mt":
+            draws = C.c_int64(0)
+            L.orc_mt_seed32(int(seed) & 0xffffffff)
+            S = L.orc_kd_build_mt(_p(pts), n, int(max_slice_size), _p(self.order), _p(self.slices_buf),
+                                  _p(self.boxes_buf), max(n, 1), _p(axis), _p(plane), _p(cl), cap_heap,
+                                  C.byref(used), C.byref(draws))
+            self.draws = draws.value
+        elif pivots == "philox":
+            S = L.orc_kd_build(_p(pts), n, int(max_slice_size), int(seed) & 0xffffffff, _p(self.order),
+                               _p(self.slices_buf), _p(self.boxes_buf), max(n, 1), _p(axis), _p(plane), _p(cl),
+                               cap_heap, C.byref(used))
+        else:
+            raise ValueError(f"unknown pivot stream {pivots!r}")
         if S < 0:
             raise ValueError("orc_kd_build: bad arguments")
         self.n, self.S, self.heap_used = n, S, used.value
@@ -563,17 +578,21 @@ class KdSlices:
         return out
 
 
-def kd_representatives(slices, n_points, seed):
-    """LightSlice::generate_slices' representatives (lightslice.cc:293-301) on a
-    freshly seeded MT stream: per slice start + rd() * size in float, truncated
-    (kept inside the point array), then the colour's draw.  Returns the kd
-    positions and the number of draws taken (2 per slice)."""
+def kd_representatives(slices, n_points, seed, skip=0):
+    """LightSlice::generate_slices' representatives (lightslice.cc:293-301) on
+    the MT stream seeded with seed after its first skip numbers (the tree's
+    pivots when they came from the same stream): per slice start + rd() * size
+    in float, truncated (kept inside the point array), then the colour's draw.
+    Returns the kd positions and the number of draws taken from the seed on
+    (skip + 2 per slice)."""
     L = lib()
     L.orc_mt_seed32(seed)
+    for _ in range(int(skip)):
+        L.orc_mt_rand()
     reps = np.zeros(len(slices), np.int32)
     for s, (a, b) in enumerate(np.asarray(slices)):
         u = np.float32(L.orc_mt_rand())
         r = int(np.float32(np.float32(a) + np.float32(u * np.float32(b - a + 1))))
         reps[s] = min(r, n_points - 1)
         L.orc_mt_rand()
-    return reps, 2 * len(slices)
+    return reps, int(skip) + 2 * len(slices)

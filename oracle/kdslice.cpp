@@ -12,12 +12,13 @@
 //                         distance to the cluster's box; furthest first)
 //
 // The recursion, the ping-pong buffers and the heap are the reference's own
-// shapes.  One change: the reference takes each pivot from its global MT stream
-// in recursion order; here round t of the selection at node [s, e] takes its
-// number from Philox4x32-10 keyed by the build seed with counter (s, e, t), so
-// that every node can be built on its own.  Nodes are numbered as a binary heap
-// (children 2i+1, 2i+2), which fixes the same tree as the reference's
-// allocation order.
+// shapes.  Two pivot streams: orc_kd_build_mt takes every pivot from the
+// oracle's global MT stream (orc_mt_rand) in recursion order, as the reference
+// does (kdtree_n.h:192, the scene's Random); orc_kd_build takes round t of the
+// selection at node [s, e] from Philox4x32-10 keyed by the build seed with
+// counter (s, e, t), so that every node can be built on its own.  Nodes are
+// numbered as a binary heap (children 2i+1, 2i+2), which fixes the same tree as
+// the reference's allocation order.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -25,6 +26,7 @@
 #include <vector>
 
 extern "C" void orc_philox(uint32_t c[4], uint32_t k0, uint32_t k1);
+extern "C" float orc_mt_rand(void);
 
 namespace {
 
@@ -89,11 +91,20 @@ struct Builder {
     std::vector<int> sl_start, sl_end;
     std::vector<Box> boxes;
     int max_heap = -1;
+    bool mt = false;          // pivots from the global MT stream in recursion order
+    int64_t draws = 0;        // MT numbers taken
 
     // quick_select_k: round t of node [s0, e0]
     void quick_select(int s0, int e0, int t, int axis_, int k, Pt* in, int start, int end, Pt* out) {
         if (start > end) return;
-        int p = start + (int)(pivot_u(seed, s0, e0, t) * (float)(end - start + 1));
+        float u;
+        if (mt) {
+            u = orc_mt_rand();
+            ++draws;
+        } else {
+            u = pivot_u(seed, s0, e0, t);
+        }
+        int p = start + (int)(u * (float)(end - start + 1));
         const Pt v = in[p];
         int pos = start;
         for (int i = start; i <= end; ++i)
@@ -156,6 +167,35 @@ struct ClusterLess {
 
 }  // namespace
 
+static int kd_build(bool mt, const float* pts, int n, int max_size, uint32_t seed, int* order, int* slices,
+                    float* boxes, int cap_slices, int* axis, float* plane, int* cl, int cap_heap, int* heap_used,
+                    int64_t* draws) {
+    if (n < 0 || max_size < 3) return -1;
+    for (int i = 0; i < cap_heap; ++i) { axis[i] = 127; plane[i] = 3.402823466e+38f; cl[i] = -1; }
+    std::vector<Pt> data((size_t)n), median_data;
+    for (int i = 0; i < n; ++i) {
+        std::memcpy(data[i].p, pts + (size_t)kDim * i, sizeof(float) * kDim);
+        data[i].index = i;
+    }
+    median_data = data;
+    Builder b;
+    b.seed = seed; b.max_size = max_size; b.cap_heap = cap_heap;
+    b.axis = axis; b.plane = plane; b.cl = cl; b.mt = mt;
+    if (n > 0) b.build(0, false, data.data(), 0, n - 1, median_data.data());
+    if (draws) *draws = b.draws;
+    for (int i = 0; i < n; ++i) order[i] = data[i].index;
+    const int S = (int)b.sl_start.size();
+    if (S > cap_slices) return -1;
+    for (int s = 0; s < S; ++s) {
+        slices[2 * s] = b.sl_start[s];
+        slices[2 * s + 1] = b.sl_end[s];
+        std::memcpy(boxes + 12 * s, b.boxes[s].vmin, sizeof(float) * kDim);
+        std::memcpy(boxes + 12 * s + kDim, b.boxes[s].vmax, sizeof(float) * kDim);
+    }
+    *heap_used = b.max_heap + 1;
+    return S;
+}
+
 extern "C" {
 
 // LightSlice map_vec6 of one receiver record (p at 0, shading normal at 6, wo
@@ -194,29 +234,16 @@ void orc_map_vec6(const float* rec, const float* mat, float diagonal, float* out
 // = 1 + the largest heap index of a slice node.  -1 on bad arguments.
 int orc_kd_build(const float* pts, int n, int max_size, uint32_t seed, int* order, int* slices, float* boxes,
                  int cap_slices, int* axis, float* plane, int* cl, int cap_heap, int* heap_used) {
-    if (n < 0 || max_size < 3) return -1;
-    for (int i = 0; i < cap_heap; ++i) { axis[i] = 127; plane[i] = 3.402823466e+38f; cl[i] = -1; }
-    std::vector<Pt> data((size_t)n), median_data;
-    for (int i = 0; i < n; ++i) {
-        std::memcpy(data[i].p, pts + (size_t)kDim * i, sizeof(float) * kDim);
-        data[i].index = i;
-    }
-    median_data = data;
-    Builder b;
-    b.seed = seed; b.max_size = max_size; b.cap_heap = cap_heap;
-    b.axis = axis; b.plane = plane; b.cl = cl;
-    if (n > 0) b.build(0, false, data.data(), 0, n - 1, median_data.data());
-    for (int i = 0; i < n; ++i) order[i] = data[i].index;
-    const int S = (int)b.sl_start.size();
-    if (S > cap_slices) return -1;
-    for (int s = 0; s < S; ++s) {
-        slices[2 * s] = b.sl_start[s];
-        slices[2 * s + 1] = b.sl_end[s];
-        std::memcpy(boxes + 12 * s, b.boxes[s].vmin, sizeof(float) * kDim);
-        std::memcpy(boxes + 12 * s + kDim, b.boxes[s].vmax, sizeof(float) * kDim);
-    }
-    *heap_used = b.max_heap + 1;
-    return S;
+    return kd_build(false, pts, n, max_size, seed, order, slices, boxes, cap_slices, axis, plane, cl, cap_heap,
+                    heap_used, nullptr);
+}
+
+// the same with every pivot drawn from the global MT stream (orc_mt_seed32 /
+// orc_mt_rand) in the reference's recursion order; *draws = numbers taken
+int orc_kd_build_mt(const float* pts, int n, int max_size, int* order, int* slices, float* boxes, int cap_slices,
+                    int* axis, float* plane, int* cl, int cap_heap, int* heap_used, int64_t* draws) {
+    return kd_build(true, pts, n, max_size, 0, order, slices, boxes, cap_slices, axis, plane, cl, cap_heap,
+                    heap_used, draws);
 }
 
 // find_cluster for m query points; -1 where the tree has no slice

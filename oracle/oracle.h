@@ -132,6 +132,10 @@ void orc_map_vec6(const float *rec, const float *mat, float diagonal, float *out
 int  orc_kd_build(const float *pts, int n, int max_size, uint32_t seed, int *order, int *slices,
                   float *boxes, int cap_slices, int *axis, float *plane, int *cl, int cap_heap,
                   int *heap_used);
+/* the same with the pivots drawn from the global MT stream in recursion order */
+int  orc_kd_build_mt(const float *pts, int n, int max_size, int *order, int *slices, float *boxes,
+                     int cap_slices, int *axis, float *plane, int *cl, int cap_heap, int *heap_used,
+                     int64_t *draws);
 void orc_kd_find(const int *axis, const float *plane, const int *cl, int heap_used, const float *q,
                  int m, int *out);
 void orc_kd_nearest(const int *axis, const float *plane, const int *cl, int heap_used,

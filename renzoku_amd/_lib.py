@@ -39,7 +39,7 @@ EXPORTS = [
     "rzk_lightslice_clusters", "rzk_clusters_destroy", "rzk_guide_create", "rzk_guide_get", "rzk_render_guided", "rzk_render_guided_rays",
     "rzk_server_create", "rzk_server_destroy", "rzk_server_submit", "rzk_server_wait", "rzk_server_get_stats",
     "rzk_server_last_error",
-    "rzk_kdslices_build", "rzk_kdslices_build_dev", "rzk_kdslices_destroy", "rzk_kdslices_info", "rzk_kdslices_get", "rzk_kdslices_find",
+    "rzk_kdslices_build", "rzk_kdslices_build_mt", "rzk_kdslices_build_dev", "rzk_kdslices_destroy", "rzk_kdslices_info", "rzk_kdslices_get", "rzk_kdslices_find",
     "rzk_kdslices_nearest", "rzk_map_vec6",
 ]
 
@@ -70,7 +70,7 @@ class LightSliceParams(C.Structure):
         ("num_neighbor_slices", C.c_int32), ("map_resolution", C.c_int32), ("map_epsilon", C.c_float),
         ("clamp_mode", C.c_int32), ("clamp", C.c_float), ("incident_clamp", C.c_float), ("tmin", C.c_float),
         ("n_emitted", C.c_float), ("seed", C.c_uint32), ("map_gate", C.c_int32), ("max_slice_clusters", C.c_int32),
-        ("kd_max_slice_size", C.c_int32),
+        ("kd_max_slice_size", C.c_int32), ("kd_pivots", C.c_int32),
     ]
 
 
@@ -175,6 +175,7 @@ def load() -> C.CDLL:
         "rzk_vpls_destroy": [P],
         "rzk_kdslices_build": [P, P, C.c_int32, C.c_int32, C.c_uint32, P],
         "rzk_kdslices_build_dev": [P, P, C.c_int32, C.c_int32, C.c_uint32, P],
+        "rzk_kdslices_build_mt": [P, P, C.c_int32, C.c_int32, C.c_uint32, P, P],
         "rzk_kdslices_destroy": [P],
         "rzk_kdslices_info": [P, P, P],
         "rzk_kdslices_get": [P, C.c_char_p, P, I64],
@@ -397,11 +398,27 @@ class KdSlices:
     (max_slice_size points per slice at most); arrays order, slices (S x 2),
     boxes (S x 12) and the heap arrays axis / plane / cl of the query tree"""
 
-    def __init__(self, ctx: Ctx, points, max_slice_size=256, seed=2013, n=None):
-        """points: host array (n x 6), or a device address (int) of n x 6 floats"""
+    def __init__(self, ctx: Ctx, points, max_slice_size=256, seed=2013, n=None, pivots="philox"):
+        """points: host array (n x 6), or a device address (int) of n x 6 floats.
+        pivots="philox": per-node pivot numbers, the parallel device build;
+        pivots="mt": one MT stream in the reference's recursion order, the host
+        build (host points only); draws = the numbers it took"""
         self.ctx, self.lib = ctx, ctx.lib
         h = C.c_void_p()
-        if isinstance(points, int):
+        self.draws = 0
+        if pivots not in ("philox", "mt"):
+            raise ValueError(f"unknown pivot stream {pivots!r}")
+        if pivots == "mt":
+            if isinstance(points, int):
+                raise ValueError("the MT-order build takes host points")
+            pts = f32(points).reshape(-1, 6)
+            self.n = len(pts)
+            draws = C.c_int64(0)
+            ctx.check(self.lib.rzk_kdslices_build_mt(ctx.h, ptr(pts), self.n, int(max_slice_size),
+                                                     int(seed) & 0xffffffff, C.byref(draws), C.byref(h)),
+                      "rzk_kdslices_build_mt")
+            self.draws = draws.value
+        elif isinstance(points, int):
             self.n = int(n)
             ctx.check(self.lib.rzk_kdslices_build_dev(ctx.h, C.c_void_p(points), self.n, int(max_slice_size),
                                                       int(seed) & 0xffffffff, C.byref(h)), "rzk_kdslices_build_dev")
@@ -546,7 +563,7 @@ class LightSlice:
         defaults = dict(width=512, height=512, tile=32, num_clusters=300, num_neighbor_slices=16,
                         map_resolution=32, map_epsilon=1e-3, clamp_mode=CLAMP_FORM_FACTOR, clamp=0.01,
                         incident_clamp=0.01, tmin=1e-3, n_emitted=1.0, seed=2013, map_gate=0, max_slice_clusters=0,
-                        kd_max_slice_size=0)
+                        kd_max_slice_size=0, kd_pivots=0)
         defaults.update(kw)
         for k, v in defaults.items():
             setattr(p, k, v)

@@ -7,6 +7,8 @@
 
 #include "rzk_internal.h"
 
+namespace rzk { class Mt19937; }
+
 // device buffer owned by a slicing
 template <typename T>
 struct KdBuf {
@@ -31,6 +33,10 @@ struct rzk_kdslices {
 // the build over kd->P (n, max_size and seed set); on the context's stream,
 // synchronized at the end
 void kd_build_device(rzk_kdslices* kd);
+// the build with every pivot drawn from rd in the reference's recursion order,
+// on the host over h_planes (kd->P's points, 6 planes of n floats); leaves the
+// same device arrays as kd_build_device; returns the numbers drawn
+int64_t kd_build_host_mt(rzk_kdslices* kd, const float* h_planes, rzk::Mt19937& rd);
 // map_vec6 of receiver records idx[i] (i < np) into planes of pitch np
 void kd_launch_map_vec6(rzk_ctx* ctx, const float* d_mat, const float* d_recv, const int* d_rmat, const int* d_idx,
                         int np, float diagonal, float* d_planes);

@@ -35,6 +35,7 @@
 #include <string>
 #include <vector>
 
+#include "host_sampling.h"
 #include "kdslice.h"
 #include "rzk_device.h"
 #include "std_order.h"
@@ -782,6 +783,34 @@ void kd_launch_find_points(const rzk_kdslices* kd, const int* d_dst, int* d_out)
 }
 
 extern "C" {
+
+int rzk_kdslices_build_mt(rzk_ctx* ctx, const float* points, int32_t n, int32_t max_slice_size, uint32_t seed,
+                          int64_t* out_draws, rzk_kdslices** out) {
+    if (!ctx || !out) return RZK_E_INVALID;
+    *out = nullptr;
+    return kd_guard(ctx, [&] {
+        RZK_CHECK(n >= 0 && (n == 0 || points), RZK_E_INVALID, "null points");
+        RZK_CHECK(max_slice_size >= 3, RZK_E_INVALID, "max_slice_size must be at least 3");
+        RZK_CHECK(n < (1 << 30), RZK_E_INVALID, "too many points");
+        std::unique_ptr<rzk_kdslices> kd(new rzk_kdslices());
+        kd->ctx = ctx;
+        kd->n = n;
+        kd->max_size = max_slice_size;
+        kd->seed = seed;
+        std::vector<float> planes((size_t)kKdDim * n);
+        for (int i = 0; i < n; ++i)
+            for (int a = 0; a < kKdDim; ++a) planes[(size_t)a * n + i] = points[(size_t)kKdDim * i + a];
+        kd->P.alloc((size_t)kKdDim * n);
+        if (n > 0)
+            RZK_HIP(hipMemcpyAsync(kd->P.p, planes.data(), sizeof(float) * planes.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        Mt19937 rd(seed);
+        const int64_t draws = kd_build_host_mt(kd.get(), planes.data(), rd);
+        if (out_draws) *out_draws = draws;
+        ctx_retain(ctx);
+        *out = kd.release();
+    });
+}
 
 static int kd_build_entry(rzk_ctx* ctx, const float* points, bool on_device, int32_t n, int32_t max_slice_size,
                           uint32_t seed, rzk_kdslices** out) {

@@ -388,7 +388,15 @@ static void ls_kd_slices(rzk_lightslice* ls, Mt19937& rd) {
     kd->P.alloc((size_t)6 * np);
     SceneView sv = scene_view(ls->scene);
     kd_launch_map_vec6(ctx, sv.mat, ls->recv.p, ls->mat.p, ls->gp_pix.p, np, diag, kd->P.p);
-    kd_build_device(kd);
+    if (p.kd_pivots == 1) {
+        // the reference's pivot stream: the tree's draws come first on rd
+        std::vector<float> planes((size_t)6 * np);
+        RZK_HIP(hipMemcpyAsync(planes.data(), kd->P.p, sizeof(float) * planes.size(), hipMemcpyDeviceToHost, st));
+        ls_sync(ctx);
+        (void)kd_build_host_mt(kd, planes.data(), rd);
+    } else {
+        kd_build_device(kd);
+    }
     const int S = kd->S;
     RZK_CHECK(S >= p.num_neighbor_slices, RZK_E_STATE,
               "fewer kd-tree slices (" + std::to_string(S) + ") than neighbour slices");
@@ -793,6 +801,8 @@ int rzk_lightslice_create(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, c
         *out = nullptr;
         RZK_CHECK(p->kd_max_slice_size == 0 || p->kd_max_slice_size >= 3, RZK_E_INVALID,
                   "kd_max_slice_size must be 0 (pixel tiles) or at least 3");
+        RZK_CHECK(p->kd_pivots == 0 || p->kd_pivots == 1, RZK_E_INVALID,
+                  "kd_pivots must be 0 (per-node Philox) or 1 (one MT stream in recursion order)");
         RZK_CHECK(p->width > 0 && p->height > 0 && (p->tile > 0 || p->kd_max_slice_size > 0), RZK_E_INVALID,
                   "bad image or tile size");
         RZK_CHECK(p->num_clusters > 0 && p->num_neighbor_slices > 0 && p->map_resolution > 0, RZK_E_INVALID,

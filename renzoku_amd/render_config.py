@@ -62,9 +62,11 @@ class RenderConfig:
     def clamp_mode(self) -> int:
         return CLAMP_MODES[self.clamp_type]
 
-    def lightslice_params(self, width, height, n_emitted, seed=2013, tile=32, map_resolution=32):
+    def lightslice_params(self, width, height, n_emitted, seed=2013, tile=32, map_resolution=32, kd_pivots=0):
         """keyword arguments of renzoku_amd.LightSlice (rzk_lightslice_params): the slices are the
-        reference's kd-tree slices of at most slice_size gather points when slice_size > 0"""
+        reference's kd-tree slices of at most slice_size gather points when slice_size > 0;
+        kd_pivots=1 draws the tree's pivots from the LightSlice's MT stream in the
+        reference's recursion order (a sequential host build), 0 from per-node streams"""
         if self.integrator != "vpl":
             raise RenderConfigError("LightSlice parameters come from a vpl integrator")
         ls = self.lightslice
@@ -73,7 +75,7 @@ class RenderConfig:
                     clamp_mode=self.clamp_mode, clamp=self.clamp_threshold,
                     incident_clamp=self.incident_clamp_threshold, n_emitted=n_emitted, seed=seed,
                     max_slice_clusters=ls.max_clusters,
-                    kd_max_slice_size=ls.slice_size)
+                    kd_max_slice_size=ls.slice_size, kd_pivots=int(kd_pivots))
 
     def path_params(self, spp=None, seed=2013, tmin=1e-3):
         """keyword arguments of LightSlice.render_guided / Guide.render (rzk_pt_params)"""

@@ -51,6 +51,33 @@ def test_kd_build_matches_oracle(ctx, n, max_size, ties, seed):
         assert np.array_equal(g.nearest(q[:600], k), o.nearest(q[:600], k))
 
 
+# the reference's pivot order: one MT stream drawn depth-first (host build)
+@pytest.mark.parametrize("n,max_size,ties,seed", [(3000, 256, False, 2013), (50_000, 256, True, 2),
+                                                  (250_000, 256, False, 3), (1000, 3, True, 5), (2, 3, False, 6),
+                                                  (200, 256, False, 7)])
+def test_kd_build_mt_matches_oracle(ctx, n, max_size, ties, seed):
+    import renzoku_amd as rz
+    print(f"kd build (MT pivots) n={n} max={max_size} ties={ties} seed={seed}", flush=True)
+    P = _points(n, seed, ties)
+    g = rz.KdSlices(ctx, P, max_size, seed, pivots="mt")
+    o = oracle.KdSlices(P, max_size, seed, pivots="mt")
+    _same_tree(g, o)
+    assert g.draws == o.draws and (n < 2 or g.draws >= g.S)
+    q = np.concatenate([P[:: max(1, n // 2000)], _points(500, seed + 100, ties)])
+    assert np.array_equal(g.find(q), o.find(q))
+    for k in (1, 16):
+        assert np.array_equal(g.nearest(q[:600], k), o.nearest(q[:600], k))
+    g.close()
+
+
+def test_kd_build_mt_rejects_non_finite_points(ctx):
+    import renzoku_amd as rz
+    P = _points(100, 1, False)
+    P[17, 3] = np.nan
+    with pytest.raises(rz.RzkError):
+        rz.KdSlices(ctx, P, 16, 1, pivots="mt")
+
+
 def test_kd_empty_and_bad_input(ctx):
     import renzoku_amd as rz
     for n in (0, 1):
@@ -109,11 +136,14 @@ def _rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_lightslice_with_kd_slices_matches_oracle(ctx):
+@pytest.mark.parametrize("pivots", ["philox", "mt"])
+def test_lightslice_with_kd_slices_matches_oracle(ctx, pivots):
     """LightSlice with the reference's slices: gather points, kd-tree, MT
     representatives and rows, nearest-slice neighbours, each pixel's slice,
     then clustering and per-slice refinement on the continuing MT stream and
-    the gather over each pixel's own slice"""
+    the gather over each pixel's own slice.  pivots="mt": the tree's pivots are
+    the first numbers of the same MT stream, drawn in the reference's recursion
+    order, and everything after them moves up the stream by their count."""
     import renzoku_amd as rz
     from renzoku_amd import scenes, vplgen
     W = H = 128
@@ -123,7 +153,7 @@ def test_lightslice_with_kd_slices_matches_oracle(ctx):
     vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 2048, seed=5, batch=8192)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, gs, vp, width=W, height=H, num_clusters=32, num_neighbor_slices=4, n_emitted=ne,
-                       seed=2013, max_slice_clusters=64, kd_max_slice_size=128)
+                       seed=2013, max_slice_clusters=64, kd_max_slice_size=128, kd_pivots=int(pivots == "mt"))
     ls.initialize(sc.camera)
     inf = ls.info()
     S, V = inf.num_slices, len(vmat)
@@ -139,9 +169,9 @@ def test_lightslice_with_kd_slices_matches_oracle(ctx):
     d = p.max(0) - p.min(0)
     diag = float(np.sqrt(np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]), dtype=np.float32))
     pts = oracle.map_vec6(recv[gp], m[gp], osc.materials, diag)
-    okd = oracle.KdSlices(pts, 128, 2013)
+    okd = oracle.KdSlices(pts, 128, 2013, pivots=pivots)
     assert S == okd.S and S >= 64
-    reps, ndraw = oracle.kd_representatives(okd.slices, len(gp), 2013)
+    reps, ndraw = oracle.kd_representatives(okd.slices, len(gp), 2013, skip=okd.draws)
     assert np.array_equal(ls.get("row_pix", np.int32, S), gp[okd.order[reps]])
     nb = ls.get("neighbors", np.int32, S * 4).reshape(S, 4)
     assert np.array_equal(nb, okd.nearest(pts[okd.order[reps]], 4))
@@ -219,3 +249,5 @@ def test_lightslice_kd_slices_without_refinement(ctx):
     assert _rel_l2(img, ref) <= 1e-4
     with pytest.raises(rz.RzkError):
         rz.LightSlice(ctx, gs, vp, width=W, height=H, num_clusters=24, kd_max_slice_size=2)
+    with pytest.raises(rz.RzkError):
+        rz.LightSlice(ctx, gs, vp, width=W, height=H, num_clusters=24, kd_max_slice_size=64, kd_pivots=2)

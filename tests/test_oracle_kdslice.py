@@ -8,6 +8,10 @@ What pins it:
   * the selection's moves: a second formulation (depth by depth, in-place
     stable partitions of each node, numpy) gives the same order, planes and
     axes as the oracle's ping-pong recursion, with many equal keys;
+  * the reference's pivot order (pivots from one MT stream, kdtree_n.h:192): a
+    depth-first numpy build drawing from the MT stream (itself pinned on the
+    reference's mtwist golden vectors) takes the same numbers in the same order
+    and gives the same order, planes and axes;
   * find: every point farther than the plane bias from the planes on its path
     lands in its own slice;
   * nearest: on integer data with queries a quarter off the grid, where the
@@ -111,10 +115,84 @@ def test_selection_moves_match_a_depth_by_depth_build(n, max_size, ties):
             assert kd.axis[h] == 127
 
 
+def _dfs_build_mt(P, seed):
+    """depth-first, pivots taken in order from the MT stream seeded with seed;
+    returns order, axes, planes and the numbers taken"""
+    n = len(P)
+    stream = (oracle.mt_stream(seed, 40 * n + 64) & 0xffffff).astype(np.float32) * np.float32(1.0 / 16777216.0)
+    order = np.arange(n)
+    axes, planes = {}, {}
+    taken = 0
+
+    def node(heap, s, e):
+        nonlocal taken
+        if s >= e:
+            return
+        ids = order[s:e + 1].copy()
+        pts = P[ids]
+        ext = np.abs(pts.max(0) - pts.min(0))
+        ax, best = 0, np.float32(0)
+        for a in range(6):
+            if ext[a] > best:
+                ax, best = a, ext[a]
+        key = pts[:, ax].copy()
+        k, lo, hi = (e - s) >> 1, 0, e - s
+        while True:
+            u = stream[taken]
+            taken += 1
+            v = key[lo + int(np.float32(u * np.float32(hi - lo + 1)))]
+            kk, ii = key[lo:hi + 1], ids[lo:hi + 1]
+            L, E, G = kk < v, kk == v, kk > v
+            key[lo:hi + 1] = np.concatenate([kk[L], kk[E], kk[G]])
+            ids[lo:hi + 1] = np.concatenate([ii[L], ii[E], ii[G]])
+            LT, EQ = int(L.sum()), int(E.sum())
+            if k < lo + LT:
+                hi = lo + LT - 1
+            elif k < lo + LT + EQ:
+                break
+            else:
+                lo += LT + EQ
+        axes[heap], planes[heap] = ax, key[k]
+        order[s:e + 1] = ids
+        m = (s + e) // 2
+        node(2 * heap + 1, s, m)
+        node(2 * heap + 2, m + 1, e)
+
+    node(0, 0, n - 1)
+    assert taken < len(stream)
+    return order, axes, planes, taken
+
+
+@pytest.mark.parametrize("n,max_size,ties,seed", [(2500, 256, False, 2013), (4000, 100, True, 7), (513, 3, True, 1),
+                                                  (2, 3, False, 5), (200, 256, True, 11)])
+def test_mt_pivots_follow_the_reference_recursion(n, max_size, ties, seed):
+    P = _points(n, 3 * n + 1, ties)
+    kd = oracle.KdSlices(P, max_size, seed, pivots="mt")
+    order, axes, planes, taken = _dfs_build_mt(P, seed)
+    assert kd.draws == taken
+    assert np.array_equal(order, kd.order)
+    for h in range(kd.heap_used):
+        if h in axes:
+            assert kd.axis[h] == axes[h] and kd.plane[h].view(np.uint32) == np.float32(planes[h]).view(np.uint32)
+        else:
+            assert kd.axis[h] == 127
+    # the stream goes on with the representatives: the first one draws number taken + 1
+    if kd.S:
+        reps, nd = oracle.kd_representatives(kd.slices, n, seed, skip=kd.draws)
+        u = np.float32(oracle.mt_stream(seed, taken + 1)[taken] & 0xffffff) * np.float32(1.0 / 16777216.0)
+        a, b = kd.slices[0]
+        assert nd == taken + 2 * kd.S
+        assert reps[0] == min(int(np.float32(np.float32(a) + np.float32(u * np.float32(b - a + 1)))), n - 1)
+    # the same points under the per-node stream give another tree of the same shape
+    kp = oracle.KdSlices(P, max_size, seed)
+    assert np.array_equal(kp.slices, kd.slices) and kp.heap_used == kd.heap_used
+
+
+@pytest.mark.parametrize("pivots", ["philox", "mt"])
 @pytest.mark.parametrize("n,max_size,ties", [(5000, 256, False), (4096, 64, True), (1000, 7, True)])
-def test_tree_invariants(n, max_size, ties):
+def test_tree_invariants(n, max_size, ties, pivots):
     P = _points(n, 7 * n, ties)
-    kd = oracle.KdSlices(P, max_size, 5)
+    kd = oracle.KdSlices(P, max_size, 5, pivots=pivots)
     assert np.array_equal(np.sort(kd.order), np.arange(n))
     Q = P[kd.order]
     # slices: depth-first first nodes of at most max_size points, tiling [0, n)

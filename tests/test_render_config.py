@@ -39,7 +39,8 @@ def test_mapping_onto_engine_parameters():
     c = parse_render_config(VPL)
     kw = c.lightslice_params(512, 512, n_emitted=70000.0)
     assert kw["num_clusters"] == 300 and kw["max_slice_clusters"] == 600 and kw["kd_max_slice_size"] == 256
-    assert kw["num_neighbor_slices"] == 16
+    assert kw["num_neighbor_slices"] == 16 and kw["kd_pivots"] == 0
+    assert c.lightslice_params(512, 512, n_emitted=1.0, kd_pivots=1)["kd_pivots"] == 1
     assert kw["clamp_mode"] == 3 and kw["clamp"] == 0.05 and kw["incident_clamp"] == 0.5
     names = {f for f, _ in _lib.LightSliceParams._fields_}
     assert set(kw) <= names                       # every key is a field of rzk_lightslice_params
