@@ -372,8 +372,9 @@ def run_refine(args):
     clusters refined to 600 per slice, the reference driver's 2x rule).  A step
     is one initialize with refinement; the refinement time is that minus an
     initialize without it (same LightSlice inputs).  The refinement reads the
-    reference's single MT stream slice after slice, so it takes one pass per
-    slice whose number count differs from 4 per split; the pass count is
+    reference's single MT stream slice after slice, so a slice's offset is
+    known only once the slices before it ran at theirs; passes run slices at
+    guessed offsets and keep the chain that guessed right; the pass count is
     reported.  cpu_baseline: the oracle refining the first slices (the
     sequential computation's own prefix) on one thread."""
     import renzoku_amd as rz

@@ -315,9 +315,10 @@ typedef struct rzk_lightslice_info {
     int32_t draws;                  /* centre draws taken */
     int64_t reduced_pairs;          /* rows x VPLs */
     int64_t gather_pairs;           /* pixels with a material x clusters */
-    int32_t refine_passes;          /* passes of the per-slice refinement (0: no refinement); each pass
-                                       re-runs the slices after the first whose random-number count
-                                       differed from 4 per split */
+    int32_t refine_passes;          /* passes of the per-slice refinement (0: no refinement); a pass
+                                       runs slices at guessed stream offsets and keeps those on the
+                                       chain of exact offsets (1 when every slice took 4 numbers per
+                                       split) */
 } rzk_lightslice_info;
 
 /* the LightSlice keeps scene and vpls by pointer: both must outlive it */

@@ -21,10 +21,17 @@
 // Draws: the reference takes them from one MT stream, slice after slice.  A
 // slice takes 4 per split as a rule (2 to pick the line, 1 per new
 // representative) but fewer when a split cluster has two members or the
-// refinement stops early, so slice s reads the pre-generated stream from an
-// offset that assumes the rule for every earlier slice; the host checks the
-// counts afterwards and re-runs, with corrected offsets, the slices after the
-// first one that took a different number.
+// refinement stops early, and how many depends on the numbers it reads, so a
+// slice's offset in the pre-generated stream is known only once every earlier
+// slice has run at its own exact offset.  Passes run (slice, offset) jobs, one
+// workgroup each: the first pass every slice at the rule's offset; each later
+// pass the first unresolved slice at its exact offset and the slices after it
+// at every offset near their estimate (the draws each slice took on its runs
+// so far, averaged), the window widening with the distance.  After a pass the
+// host follows the chain of exact offsets through the jobs that guessed right
+// and keeps their results.  When the slice matrices do not all fit in memory,
+// slices run in batches at offsets that assume the rule after the first wrong
+// one, re-running from there.
 #pragma clang fp contract(off)
 
 #include <algorithm>
@@ -62,13 +69,18 @@ struct RefineArgs {
     const float* draws;      // pre-generated MT numbers
     const int64_t* draw_off; // per slice
     int s0;                  // first slice of the launch
-    int batch0;              // slice index of scratch row 0
-    // per-slice results (S x maxC, S)
+    int batch0;              // slice index of slice-data row 0 (SM, the initial heap)
+    // speculative jobs: block j refines slice job_slice[j] from draw offset
+    // job_off[j] in scratch row j and writes result row j; null: block j is
+    // slice s0 + j at draw_off[s], in scratch row s - batch0, result row s
+    const int* job_slice;
+    const int64_t* job_off;
+    // results (rows x maxC, rows)
     int* rep;
     float* w;
     int* count;
     int* used;               // draws taken
-    // per-slice state (scratch rows of this batch)
+    // per-slice state (scratch rows of this batch, or of the jobs)
     int* perm;               // nmem member positions (index into mem), cluster by cluster
     int* perm_tmp;           // nmem (clusters over kRefSortLds members)
     float* nrm;              // nmem (idem)
@@ -633,8 +645,11 @@ __global__ __launch_bounds__(kRefBlock) void k_refine_slices(RefineArgs a) {
     extern __shared__ __align__(16) unsigned char s_ref[];
     const RefLayout L = ref_layout(a.nnb, a.maxC, a.lds_clusters != 0);
     const int t = threadIdx.x;
-    const int s = a.s0 + (int)blockIdx.x;
-    const int row = s - a.batch0;
+    const int j = (int)blockIdx.x;
+    const int s = a.job_slice ? a.job_slice[j] : a.s0 + j;
+    const int row = s - a.batch0;                  // slice data
+    const int jrow = a.job_slice ? j : row;        // scratch
+    const int orow = a.job_slice ? j : s;          // results
     RefineShared sh = ref_shared(s_ref, L, a, row);
     sh.pw.key = sh.sKey;
     sh.pw.val = sh.sVal;
@@ -648,11 +663,11 @@ __global__ __launch_bounds__(kRefBlock) void k_refine_slices(RefineArgs a) {
     sh.pw.wsum = reinterpret_cast<int*>(s_ref + L.oWs);
 
     if (s >= a.S) return;
-    int* perm = a.perm + (int64_t)row * a.nmem;
-    int* perm_tmp = a.perm_tmp + (int64_t)row * a.nmem;
-    float* nrm = a.nrm + (int64_t)row * a.nmem;
-    float* line_t = a.line_t + (int64_t)row * a.nmem;
-    int* line_k = a.line_k + (int64_t)row * a.nmem;
+    int* perm = a.perm + (int64_t)jrow * a.nmem;
+    int* perm_tmp = a.perm_tmp + (int64_t)jrow * a.nmem;
+    float* nrm = a.nrm + (int64_t)jrow * a.nmem;
+    float* line_t = a.line_t + (int64_t)jrow * a.nmem;
+    int* line_k = a.line_k + (int64_t)jrow * a.nmem;
     int *cst, *csz, *cflag, *hval;
     float* hkey;
     if (a.lds_clusters) {
@@ -662,15 +677,15 @@ __global__ __launch_bounds__(kRefBlock) void k_refine_slices(RefineArgs a) {
         hkey = reinterpret_cast<float*>(cflag + a.maxC);
         hval = reinterpret_cast<int*>(hkey + a.maxC);
     } else {
-        cst = a.cst + (int64_t)row * a.maxC;
-        csz = a.csz + (int64_t)row * a.maxC;
-        cflag = a.cflag + (int64_t)row * a.maxC;
-        hkey = a.hkey + (int64_t)row * a.maxC;
-        hval = a.hval + (int64_t)row * a.maxC;
+        cst = a.cst + (int64_t)jrow * a.maxC;
+        csz = a.csz + (int64_t)jrow * a.maxC;
+        cflag = a.cflag + (int64_t)jrow * a.maxC;
+        hkey = a.hkey + (int64_t)jrow * a.maxC;
+        hval = a.hval + (int64_t)jrow * a.maxC;
     }
-    int* rep = a.rep + (int64_t)s * a.maxC;
-    float* w = a.w + (int64_t)s * a.maxC;
-    const int64_t draw0 = a.draw_off[s];
+    int* rep = a.rep + (int64_t)orow * a.maxC;
+    float* w = a.w + (int64_t)orow * a.maxC;
+    const int64_t draw0 = a.job_slice ? a.job_off[j] : a.draw_off[s];
 
     if (t < a.nnb) sh.sNb[t] = a.nb[(int64_t)s * a.nnb + t];
     for (int e = t; e < a.nmem; e += kRefBlock) perm[e] = e;
@@ -727,10 +742,70 @@ __global__ __launch_bounds__(kRefBlock) void k_refine_slices(RefineArgs a) {
         ++cnt;
     }
     if (t == 0) {
-        a.count[s] = cnt;
-        a.used[s] = drawn;
+        a.count[orow] = cnt;
+        a.used[orow] = drawn;
     }
 }
+
+// the chosen jobs' results into their slices' rows: pick[k] = (slice, job)
+__global__ void k_take_jobs(const int2* __restrict__ pick, int maxC, const int* __restrict__ jrep,
+                            const float* __restrict__ jw, const int* __restrict__ jcount, int* __restrict__ rep,
+                            float* __restrict__ w, int* __restrict__ count) {
+    const int2 pk = pick[blockIdx.x];
+    for (int c = threadIdx.x; c < maxC; c += blockDim.x) {
+        rep[(int64_t)pk.x * maxC + c] = jrep[(int64_t)pk.y * maxC + c];
+        w[(int64_t)pk.x * maxC + c] = jw[(int64_t)pk.y * maxC + c];
+    }
+    if (threadIdx.x == 0) count[pk.x] = jcount[pk.y];
+}
+
+// The chain of draw offsets: slice s starts where slice s - 1 stopped.  A
+// speculative pass runs jobs (slice, offset): the first unresolved slice at
+// its exact offset, and each later slice at every offset within z standard
+// deviations of its estimate (the exact offset plus the mean draws each slice
+// in between took on its runs so far; the error's deviation grows with the
+// square root of the distance).  Counts are even (2 or 4 numbers a split), so
+// offsets step by 2.  The host then follows the chain through the jobs whose
+// offset turned out right.
+struct RefineChain {
+    int64_t per;
+    std::vector<double> sum, sq;
+    std::vector<int> runs;
+    explicit RefineChain(int S, int64_t per_) : per(per_), sum(S, 0.0), sq(S, 0.0), runs(S, 0) {}
+    void observe(int s, int used) {
+        sum[s] += used;
+        sq[s] += (double)used * used;
+        ++runs[s];
+    }
+    double mean(int s) const {
+        if (runs[s] > 0) return sum[s] / runs[s];
+        double a = 0;
+        int n = 0;
+        for (size_t k = 0; k < runs.size(); ++k) { a += sum[k]; n += runs[k]; }
+        return n > 0 ? a / n : (double)per;
+    }
+    // deviation of one slice's count about its mean: pooled over slices run
+    // more than once, else the spread between slices
+    double sigma() const {
+        double within = 0, all = 0, all2 = 0;
+        int dof = 0, n = 0;
+        for (size_t k = 0; k < runs.size(); ++k) {
+            if (runs[k] == 0) continue;
+            const double m = sum[k] / runs[k];
+            within += sq[k] - runs[k] * m * m;
+            dof += runs[k] - 1;
+            all += sum[k];
+            all2 += sq[k];
+            n += runs[k];
+        }
+        if (dof >= 16) return std::sqrt(std::max(0.0, within / dof));
+        if (n < 2) return 0.0;
+        const double m = all / n;
+        return std::sqrt(std::max(0.0, all2 / n - m * m));
+    }
+};
+
+constexpr double kRefSpecZ = 2.0;
 
 }  // namespace rzk
 
@@ -742,8 +817,8 @@ int64_t refine_draws_needed(int S, int ncl, int maxC) { return 4 * (int64_t)(max
 // representatives/weights d_grep/d_gw) into at most maxC clusters per slice.
 // h_draws: the MT numbers that follow the clustering's, at least
 // refine_draws_needed(S, ncl, maxC) of them; outputs d_rep/d_w (S x maxC) and
-// d_count (S).  Returns the number of passes over the slices (1 when every
-// slice took 4 numbers per split).
+// d_count (S).  Returns the number of passes (1 when every slice took 4
+// numbers per split).
 int launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb, int nnb, int ncl, const int* d_off,
                           const int* d_mem, int nmem, const int* d_grep, const float* d_gw, const float* d_mags,
                           const float* d_norms_R, int maxC, const std::vector<float>& h_draws_in, int* d_rep,
@@ -767,8 +842,15 @@ int launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb,
     const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)S, budget / row_bytes));
     const bool resident = batch >= S;                      // every slice's rows at once: copies built once
     // 20 regions below, each rounded up to 256 bytes
+    // speculative jobs per pass (resident only): one block per compute unit, at
+    // least one per slice; their results, slices, offsets and the picks
+    const int B = resident ? std::max(S, std::max(ctx->num_cus, 1)) : 0;
+    const size_t job_bytes = (size_t)B * ((size_t)maxC * 8 + 4 + 4 + 4 + 8) + sizeof(int2) * (size_t)S + 7 * 256;
+    // scratch rows: a batch of slices, or the jobs of a pass
+    const int rows = resident ? B : batch;
+    const size_t extra_rows = (size_t)(rows - batch) * ((size_t)nmem * 20 + (size_t)maxC * 20 + (size_t)ncl * 12 + 20 * 256);
     const size_t shared_bytes = sizeof(float) * h_draws.size() + sizeof(int64_t) * S + sizeof(int) * S +
-                                sizeof(float) * 3 * (size_t)nmem + 8 * 256;
+                                sizeof(float) * 3 * (size_t)nmem + 8 * 256 + job_bytes + extra_rows;
     void* raw = nullptr;
     RZK_HIP(hipMalloc(&raw, shared_bytes + row_bytes * batch));
     std::unique_ptr<void, void (*)(void*)> hold(raw, [](void* q) { (void)hipFree(q); });
@@ -787,19 +869,26 @@ int launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb,
     a.mags = d_mm; a.norms_R = d_nn;
     float* d_SM = (float*)take(sizeof(float) * (size_t)nmem * M * batch);
     a.SM = d_SM;
-    a.perm = (int*)take(sizeof(int) * (size_t)nmem * batch);
-    a.perm_tmp = (int*)take(sizeof(int) * (size_t)nmem * batch);
-    a.nrm = (float*)take(sizeof(float) * (size_t)nmem * batch);
-    a.line_t = (float*)take(sizeof(float) * (size_t)nmem * batch);
-    a.line_k = (int*)take(sizeof(int) * (size_t)nmem * batch);
-    a.cst = (int*)take(sizeof(int) * (size_t)maxC * batch);
-    a.csz = (int*)take(sizeof(int) * (size_t)maxC * batch);
-    a.cflag = (int*)take(sizeof(int) * (size_t)maxC * batch);
-    a.hkey = (float*)take(sizeof(float) * (size_t)maxC * batch);
-    a.hval = (int*)take(sizeof(int) * (size_t)maxC * batch);
-    float* d_cost0 = (float*)take(sizeof(float) * (size_t)ncl * batch);
-    float* d_hkey0 = (float*)take(sizeof(float) * (size_t)ncl * batch);
-    int* d_hval0 = (int*)take(sizeof(int) * (size_t)ncl * batch);
+    a.perm = (int*)take(sizeof(int) * (size_t)nmem * rows);
+    a.perm_tmp = (int*)take(sizeof(int) * (size_t)nmem * rows);
+    a.nrm = (float*)take(sizeof(float) * (size_t)nmem * rows);
+    a.line_t = (float*)take(sizeof(float) * (size_t)nmem * rows);
+    a.line_k = (int*)take(sizeof(int) * (size_t)nmem * rows);
+    a.cst = (int*)take(sizeof(int) * (size_t)maxC * rows);
+    a.csz = (int*)take(sizeof(int) * (size_t)maxC * rows);
+    a.cflag = (int*)take(sizeof(int) * (size_t)maxC * rows);
+    a.hkey = (float*)take(sizeof(float) * (size_t)maxC * rows);
+    a.hval = (int*)take(sizeof(int) * (size_t)maxC * rows);
+    float* d_cost0 = (float*)take(sizeof(float) * (size_t)ncl * rows);
+    float* d_hkey0 = (float*)take(sizeof(float) * (size_t)ncl * rows);
+    int* d_hval0 = (int*)take(sizeof(int) * (size_t)ncl * rows);
+    int* d_jrep = B ? (int*)take(sizeof(int) * (size_t)maxC * B) : nullptr;
+    float* d_jw = B ? (float*)take(sizeof(float) * (size_t)maxC * B) : nullptr;
+    int* d_jcount = B ? (int*)take(sizeof(int) * (size_t)B) : nullptr;
+    int* d_jused = B ? (int*)take(sizeof(int) * (size_t)B) : nullptr;
+    int* d_jslice = B ? (int*)take(sizeof(int) * (size_t)B) : nullptr;
+    int64_t* d_joff = B ? (int64_t*)take(sizeof(int64_t) * (size_t)B) : nullptr;
+    int2* d_pick = B ? (int2*)take(sizeof(int2) * (size_t)S) : nullptr;
     a.iota = d_iota; a.cost0 = d_cost0; a.hkey0 = d_hkey0; a.hval0 = d_hval0;
     RZK_CHECK((size_t)(p - static_cast<char*>(raw)) <= shared_bytes + row_bytes * batch, RZK_E_STATE,
               "refinement scratch layout overflow");
@@ -841,6 +930,83 @@ int launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb,
         RZK_HIP(hipGetLastError());
     };
     if (resident) prepare_rows(0, S, 0);
+    if (resident) {
+        // speculative passes over (slice, offset) jobs
+        RefineChain chain(S, per);
+        RefineArgs b = a;
+        b.s0 = 0;
+        b.batch0 = 0;
+        b.job_slice = d_jslice;
+        b.job_off = d_joff;
+        b.rep = d_jrep;
+        b.w = d_jw;
+        b.count = d_jcount;
+        b.used = d_jused;
+        std::vector<int> h_js, h_jused;
+        std::vector<int64_t> h_jo;
+        std::vector<int2> h_pick;
+        int first = 0;
+        int64_t o_first = 0;
+        for (int pass = 0; first < S; ++pass) {
+            RZK_CHECK(pass <= S, RZK_E_STATE, "refinement draw offsets did not settle");
+            h_js.clear();
+            h_jo.clear();
+            const double sig = chain.sigma();
+            double center = (double)o_first;
+            for (int i = 0; first + i < S && (int)h_js.size() < B; ++i) {
+                const int sl = first + i;
+                if (i > 0) center += chain.mean(sl - 1);
+                // offsets lie in [o_first, per * sl] and differ from o_first by an even count
+                const int64_t lo = o_first, hi = per * sl;
+                int64_t c = o_first + 2 * (int64_t)std::llround((center - (double)o_first) / 2.0);
+                c = std::min(std::max(c, lo), hi - ((hi - lo) & 1));
+                const int64_t h = i == 0 ? 0 : (int64_t)std::ceil(kRefSpecZ * sig * std::sqrt((double)i) / 2.0);
+                // c, c - 2, c + 2, c - 4, ... while they fit
+                for (int64_t d = 0; d <= h && (int)h_js.size() < B; ++d) {
+                    const int64_t cand[2] = {c - 2 * d, c + 2 * d};
+                    for (int q = 0; q < (d == 0 ? 1 : 2) && (int)h_js.size() < B; ++q) {
+                        if (cand[q] < lo || cand[q] > hi) continue;
+                        h_js.push_back(sl);
+                        h_jo.push_back(cand[q]);
+                    }
+                }
+            }
+            const int nj = (int)h_js.size();
+            RZK_HIP(hipMemcpyAsync(d_jslice, h_js.data(), sizeof(int) * nj, hipMemcpyHostToDevice, st));
+            RZK_HIP(hipMemcpyAsync(d_joff, h_jo.data(), sizeof(int64_t) * nj, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_refine_slices, dim3((unsigned)nj), dim3(kRefBlock), lds, st, b);
+            RZK_HIP(hipGetLastError());
+            h_jused.resize(nj);
+            RZK_HIP(hipMemcpyAsync(h_jused.data(), d_jused, sizeof(int) * nj, hipMemcpyDeviceToHost, st));
+            RZK_HIP(hipStreamSynchronize(st));
+            // follow the chain from the exact slice through the jobs at the right offsets
+            h_pick.clear();
+            int k = 0;
+            int64_t o = o_first;
+            for (int sl = first; sl < S; ++sl) {
+                while (k < nj && h_js[k] < sl) ++k;
+                int hit = -1;
+                for (int q = k; q < nj && h_js[q] == sl; ++q)
+                    if (h_jo[q] == o) { hit = q; break; }
+                if (hit < 0) break;
+                h_pick.push_back(make_int2(sl, hit));
+                o += h_jused[hit];
+            }
+            for (int q = 0; q < nj; ++q) chain.observe(h_js[q], h_jused[q]);
+            RZK_CHECK(!h_pick.empty(), RZK_E_STATE, "refinement pass resolved no slice");
+            const int np_ = (int)h_pick.size();
+            RZK_HIP(hipMemcpyAsync(d_pick, h_pick.data(), sizeof(int2) * np_, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_take_jobs, dim3((unsigned)np_), dim3(256), 0, st, d_pick, maxC, d_jrep, d_jw, d_jcount,
+                               d_rep, d_w, d_count);
+            RZK_HIP(hipGetLastError());
+            // the picks are read before the host vectors change again
+            RZK_HIP(hipStreamSynchronize(st));
+            first += np_;
+            o_first = o;
+            if (first == S) return pass + 1;
+        }
+        return 0;
+    }
     std::vector<int> h_used(S);
     int first = 0;                                         // slices before `first` are final
     for (int pass = 0; first < S; ++pass) {

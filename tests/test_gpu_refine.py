@@ -5,7 +5,8 @@ slice's cluster count, representatives and weights bit for bit, on inputs
 full of equal projections (columns that are zero over a slice's rows, where
 std::sort's tie order decides), on degenerate ones (one- and two-member
 clusters, early stops: slices that take fewer draws than the rule, so later
-slices are re-run from corrected stream offsets), on lines longer than the
+slices run again at guessed offsets until the chain of exact offsets reaches
+them), on lines longer than the
 LDS sort buffer, and through the whole LightSlice pipeline (per-slice pdfs,
 maps and the final gather over each slice's own representatives)."""
 import numpy as np
@@ -50,6 +51,7 @@ def _check(ctx, R, nb, o, draws, maxc):
     (300, 8, 3, 10, 24, 2, 0.0),         # continuous values
     (60, 12, 4, 20, 60, 3, 0.5),         # singletons and pairs: early stops, short draw counts
     (2000, 6, 2, 4, 9, 4, 0.3),
+    (240, 320, 4, 10, 24, 5, 0.5),       # more slices than compute units: long chains of guessed offsets
 ])
 def test_refine_slices_matches_oracle(ctx, V, S, nnb, ncl, maxc, seed, zero_frac):
     R, nb, o, draws = _case(V, S, nnb, ncl, maxc, seed, zero_frac)
