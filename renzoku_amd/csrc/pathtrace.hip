@@ -141,7 +141,7 @@ __device__ __forceinline__ int first_above(const float* cdf, int n, float u) {
 // LightSlice::sample -> multi_slice_sample: sample_map of the slice (receiver
 // frame, same_local_wi) or the uniform-hemisphere fallback; pdf re-evaluated
 // by multi_slice_pdf; returns the receiver BRDF for wi
-__device__ v3 guide_sample(const GuideView& gv, int slice, int res, const float* mat, const Onb& b, v3 sn, v3 wo,
+__device__ __forceinline__ v3 guide_sample(const GuideView& gv, int slice, int res, const float* mat, const Onb& b, v3 sn, v3 wo,
                            const uint32_t* u, v3& wi, float& pdf) {
     if (slice < 0) {
         float phi = kTwoPi * u24(u[0]);
@@ -264,14 +264,367 @@ __device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& 
     return trace_from(sv, pp, gv, ray, x, y, slice, smp, astk, st, nclosest, nshadow);
 }
 
+// ---------------------------------------------------------------------------
+// The same path samples as a per-lane state machine with one traversal site.
+// trace_path runs its four traversals (camera or continuation ray, shadow
+// segment, next-event BSDF ray) at four places in the code, so the lanes of a
+// wave trace together only while they are at the same step of the same
+// bounce, and a lane whose path has ended waits for the longest path of the
+// wave.  Here each lane runs its samples as stages between traversal queries:
+// every loop turn the lane consumes its last query's result and runs on to
+// its next query (starting its next sample when a path ends), and then every
+// lane of the wave traverses its own query at the same place.  The arithmetic
+// of each stage is trace_path's, in the same order, so every sample gives the
+// same bits.  Shadow segments use closest_hit4's any-hit mode (any_hit4's
+// answer).  State that outlives a query but is only read while shading (the
+// receiver record, the next event's light sample and partial sum, the
+// continuation's direction) lives in LDS, one column per lane; the rest stays
+// in registers.
+
+enum : int {
+    kPsSample = 0,      // start the next sample (no query pending)
+    kPsPrimary,         // result: the sample's first closest hit
+    kPsShadow,          // result: the next event's shadow segment
+    kPsNeeHit,          // result: the next event's BSDF ray
+    kPsContHit,         // result: the continuation ray
+};
+
+// LDS columns of one lane (a float each, kPtCold of them)
+enum : int {
+    kPcRc = 0,                          // receiver record, RZK_RECV_FLOATS
+    kPcLo = kPcRc + RZK_RECV_FLOATS,    // next event: partial sum (3)
+    kPcQ = kPcLo + 3,                   //   light sample point (3), normal (3), area pdf
+    kPcQn = kPcQ + 3,
+    kPcLpdf = kPcQn + 3,
+    kPcWi2 = kPcLpdf + 1,               //   BSDF direction (3), pdf
+    kPcDpdf2 = kPcWi2 + 3,
+    kPcWi = kPcLo,                      // continuation (after the next event): direction (3),
+    kPcBrdf = kPcWi + 3,                //   BRDF (3), pdf
+    kPcPdf = kPcBrdf + 3,
+    kPtCold = kPcDpdf2 + 1,
+};
+
+struct PtCold {
+    float* col;                          // this lane's first column entry
+    int pitch;                           // floats between a lane's columns (the block size)
+    __device__ float get(int f) const { return col[f * pitch]; }
+    __device__ void set(int f, float v) const { col[f * pitch] = v; }
+    __device__ v3 get3(int f) const { return mk(get(f), get(f + 1), get(f + 2)); }
+    __device__ void set3(int f, v3 v) const { set(f, v.x); set(f + 1, v.y); set(f + 2, v.z); }
+    __device__ Onb onb() const {
+        Onb b; b.u = get3(kPcRc + 12); b.v = get3(kPcRc + 15); b.n = get3(kPcRc + 18);
+        return b;
+    }
+};
+
+// cold columns after the traversal stack's kLdsStack ints per lane
+__device__ __forceinline__ PtCold pt_cold(int* s_stack) {
+    PtCold c;
+    c.pitch = (int)blockDim.x;
+    c.col = reinterpret_cast<float*>(s_stack + kLdsStack * blockDim.x) + threadIdx.x;
+    return c;
+}
+
+struct PtLane {
+    int stage;
+    int smp, bounce;
+    int m;                               // the receiver's material
+    int k;                               // the next event's light
+    bool brdf_sampling;
+    v3 L, T;
+    int qtri;                            // last query's result
+    float qt, qb, qg;
+};
+
+struct PtQuery {
+    Ray r;
+    float tmax;
+    bool any;
+};
+
+// numbers 4j .. 4j + 3 of path_rng(seed, pixel, sample, bounce)
+__device__ __forceinline__ void pt_rng_block(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce,
+                                             uint32_t j, uint32_t* u) {
+    uint32_t c[4] = {sample, bounce, j, 0x52a7u};
+    philox4x32_10(c, seed ^ 0x9a3c5e17u, pixel);
+    u[0] = c[0]; u[1] = c[1]; u[2] = c[2]; u[3] = c[3];
+}
+
+// where a lane's finished samples go: the pixel's running sum (out == NULL),
+// or sample smp's value to out + (smp - smp0) * stride
+struct PtSink {
+    float* out;
+    int64_t stride;
+    int smp0;
+    v3 acc;
+};
+
+// a path sample has ended: its value to the sink, the next sample next
+__device__ __forceinline__ void pt_sample_done(PtLane& s, PtSink& k) {
+    if (k.out) {
+        float* d = k.out + (int64_t)(s.smp - k.smp0) * k.stride;
+        d[0] = s.L.x;
+        d[1] = s.L.y;
+        d[2] = s.L.z;
+    } else {
+        k.acc = k.acc + s.L;
+    }
+    ++s.smp;
+    s.stage = kPsSample;
+}
+
+// next_event's end: L += T * (Lo / the light's selection pdf)
+__device__ __forceinline__ void pt_nee_end(const SceneView& sv, const PtCold& c, PtLane& s) {
+    const float sp = sv.lights[s.k].sel_pdf;
+    const v3 Lo = c.get3(kPcLo);
+    s.L = s.L + s.T * mk(Lo.x / sp, Lo.y / sp, Lo.z / sp);
+}
+
+// next_event's BSDF sample (numbers 5..7): a query when its pdf is positive
+__device__ __forceinline__ bool pt_nee_bsdf(const SceneView& sv, const PtParams& pp, const PtCold& c, uint32_t pixel,
+                                            PtLane& s, PtQuery& q, unsigned long long& nclosest) {
+    const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
+    const Onb b = c.onb();
+    const v3 p = c.get3(kPcRc), wo = c.get3(kPcRc + 9);
+    uint32_t u[4];
+    pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 1u, u);
+    v3 wi2;
+    float dpdf2;
+    bsdf_sample(mat, b, wo, &u[1], wi2, dpdf2);
+    if (!(dpdf2 > 0.0f)) return false;
+    c.set3(kPcWi2, wi2);
+    c.set(kPcDpdf2, dpdf2);
+    nclosest++;
+    q.r = ray_ctor(p, wi2);
+    q.tmax = 1e6f;
+    q.any = false;
+    s.stage = kPsNeeHit;
+    return true;
+}
+
+// next_event's start (numbers 1..4): the light, its sample and the shadow
+// query; false when the next event ends without a query (no light chosen, or
+// neither sample has a positive pdf)
+__device__ __forceinline__ bool pt_nee_begin(const SceneView& sv, const PtParams& pp, const PtCold& c, uint32_t pixel,
+                                             const uint32_t* u0, PtLane& s, PtQuery& q, unsigned long long& nshadow,
+                                             unsigned long long& nclosest) {
+    float ul = u24(u0[1]);
+    int k = -1;
+    for (int i = 0; i < sv.nlights; ++i)
+        if (ul < sv.lights[i].sel_cdf) { k = i; break; }
+    if (k < 0 || sv.lights[k].count <= 0) {
+        s.L = s.L + s.T * mk(0.f, 0.f, 0.f);
+        return false;
+    }
+    s.k = k;
+    const LightGpu& Lg = sv.lights[k];
+    const v3 p = c.get3(kPcRc);
+    c.set3(kPcLo, mk(0.f, 0.f, 0.f));
+    uint32_t u1[4];
+    pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 1u, u1);
+    float ut = u24(u0[2]);
+    int ti = Lg.first + Lg.count - 1;
+    for (int qq = 0; qq < Lg.count; ++qq)
+        if (ut <= sv.ltris[Lg.first + qq].cum) { ti = Lg.first + qq; break; }
+    LightTri T = sv.ltris[ti];
+    float tmp = sqrtf(1.0f - u24(u0[3]));
+    float bb = 1.0f - tmp;
+    float cc = u24(u1[0]) * tmp;
+    float aa = 1.0f - bb - cc;
+    v3 ql = (aa * mk(T.p0.x, T.p0.y, T.p0.z) + bb * mk(T.p1.x, T.p1.y, T.p1.z)) + cc * mk(T.p2.x, T.p2.y, T.p2.z);
+    float lpdf = T.ratio * (1.0f / T.n.w);
+    if (lpdf > 0.0f) {
+        c.set3(kPcQ, ql);
+        c.set3(kPcQn, mk(T.n.x, T.n.y, T.n.z));
+        c.set(kPcLpdf, lpdf);
+        nshadow++;
+        // segment_blocked(p, q)
+        q.r = ray_ctor(p, ql - p);
+        q.tmax = length(ql - p) - pp.tmin;
+        q.any = true;
+        s.stage = kPsShadow;
+        return true;
+    }
+    if (pt_nee_bsdf(sv, pp, c, pixel, s, q, nclosest)) return true;
+    pt_nee_end(sv, c, s);
+    return false;
+}
+
+// Runs lane s from its last query's result to its next query (true, query in
+// q) or past its last sample (false).  A finished sample's value goes to the
+// sink (a running sum in sample order, or one slot per sample).  Samples
+// s.smp .. smp_end - 1 of pixel (x, y); cam: the camera (NULL: the lane's one
+// sample starts from q.r as given).
+__device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam, const PtParams& pp, const GuideView& gv,
+                           const PtCold& c, int x, int y, int slice, int strata, int smp_end, PtLane& s, PtQuery& q,
+                           PtSink& acc, unsigned long long& nclosest, unsigned long long& nshadow) {
+    const uint32_t pixel = (uint32_t)(y * pp.width + x);
+    for (;;) {
+        bool at_hit = false;              // a new closest hit of the path's ray q.r
+        switch (s.stage) {
+        case kPsSample: {
+            if (s.smp >= smp_end) return false;
+            if (cam) {
+                PathRng r0 = path_rng(pp.seed, pixel, (uint32_t)s.smp, 0u);
+                float jx = u24(r0.u[0]), jy = u24(r0.u[1]);
+                if (strata > 0) {
+                    jx = ((float)(s.smp % strata) + jx) / (float)strata;
+                    jy = ((float)(s.smp / strata) + jy) / (float)strata;
+                }
+                q.r = camera_sample_ray(cam, (float)x + jx, (float)y + jy);
+            }
+            s.L = mk(0.f, 0.f, 0.f);
+            s.T = mk(1.f, 1.f, 1.f);
+            nclosest++;
+            q.tmax = 1e6f;
+            q.any = false;
+            s.stage = kPsPrimary;
+            return true;
+        }
+        case kPsPrimary:
+            s.bounce = 1;
+            at_hit = true;
+            break;
+        case kPsShadow: {
+            if (s.qtri < 0) {
+                const v3 p = c.get3(kPcRc), pn = c.get3(kPcRc + 6), wo = c.get3(kPcRc + 9);
+                const Onb b = c.onb();
+                const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
+                const v3 ql = c.get3(kPcQ), qn = c.get3(kPcQn);
+                const float lpdf = c.get(kPcLpdf);
+                v3 wi = unit(ql - p);
+                v3 Le = light_query(sv.lights[s.k], qn, -wi);
+                float G = form_factor_abs(p, pn, ql, qn);
+                if (G > 0.0f) {
+                    v3 brdf = bsdf_eval(mat, b, wo, wi);
+                    float dpdf = bsdf_pdf(mat, b, wo, wi);
+                    v3 pq = ql - p;
+                    float w1 = balance(lpdf * dot(pq, pq) / fabsf(dot(unit(p - ql), qn)), dpdf);
+                    c.set3(kPcLo, c.get3(kPcLo) + (((w1 * G) / lpdf) * Le) * brdf);
+                }
+            }
+            if (pt_nee_bsdf(sv, pp, c, pixel, s, q, nclosest)) return true;
+            pt_nee_end(sv, c, s);
+            break;
+        }
+        case kPsNeeHit: {
+            const int tri = s.qtri;
+            if (tri >= 0) {
+                float4 nf = sv.tri_n[tri];
+                int4 sf = sv.surf[__float_as_int(nf.w)];
+                if (sf.y == s.k) {
+                    const LightGpu& Lg = sv.lights[s.k];
+                    const v3 p = c.get3(kPcRc), pn = c.get3(kPcRc + 6), wo = c.get3(kPcRc + 9);
+                    const Onb b = c.onb();
+                    const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
+                    const v3 wi = c.get3(kPcWi2);
+                    const float dpdf = c.get(kPcDpdf2);
+                    v3 qh = p + s.qt * wi;
+                    v3 n = mk(nf.x, nf.y, nf.z);
+                    v3 qhn = unit((n + (n - n) * s.qb) + (n - n) * s.qg);
+                    if (sf.z & RZK_SURF_FLIPPED) qhn = -qhn;
+                    v3 Le = light_query(Lg, qhn, -wi);
+                    float pa = 0.f;
+                    for (int qq = 0; qq < Lg.count; ++qq) {
+                        const LightTri& lt = sv.ltris[Lg.first + qq];
+                        if (lt.tri == tri) { pa = lt.ratio * (1.0f / lt.n.w); break; }
+                    }
+                    v3 pq = qh - p;
+                    float lpdf = pa * dot(pq, pq) / fabsf(dot(unit(p - qh), qhn));
+                    float w2 = balance(dpdf, lpdf);
+                    v3 brdf = bsdf_eval(mat, b, wo, wi);
+                    float cosv = fabsf(dot(wi, pn));
+                    c.set3(kPcLo, c.get3(kPcLo) + (((w2 * cosv) / dpdf) * Le) * brdf);
+                }
+            }
+            pt_nee_end(sv, c, s);
+            break;
+        }
+        case kPsContHit: {
+            const int tri = s.qtri;
+            bool ended = tri < 0;
+            if (!ended) {
+                int4 sf = sv.surf[__float_as_int(sv.tri_n[tri].w)];
+                ended = sf.y >= 0;                         // light: counted by the light sample
+            }
+            if (ended) {
+                pt_sample_done(s, acc);
+                continue;
+            }
+            const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
+            const Onb b = c.onb();
+            const v3 sn = c.get3(kPcRc + 6), wo = c.get3(kPcRc + 9);
+            const v3 wi = c.get3(kPcWi);
+            const float pdf = c.get(kPcPdf);
+            const int gslice = s.bounce == 1 ? slice : -1;
+            float pdf_other = s.brdf_sampling ? guide_pdf(gv, gslice, pp.res, b, sn, wi) : bsdf_pdf(mat, b, wo, wi);
+            float weight = pdf / (pdf + pdf_other);
+            float cosv = fabsf(dot(wi, sn));
+            float sc = ((weight * cosv) / pdf) * 2.0f;
+            s.T = (s.T * c.get3(kPcBrdf)) * sc;
+            ++s.bounce;
+            at_hit = true;
+            break;
+        }
+        }
+        if (at_hit) {
+            // the bounce loop's head with the hit of q.r
+            if (!(s.qtri >= 0 && s.bounce <= pp.max_bounce)) { pt_sample_done(s, acc); continue; }
+            float rc[RZK_RECV_FLOATS];
+            int lidx;
+            fill_receiver(sv, q.r, s.qt, s.qb, s.qg, s.qtri, rc, s.m, lidx);
+            if (lidx >= 0) {
+                if (s.bounce == 1 && pp.direct)
+                    s.L = s.L + s.T * light_query(sv.lights[lidx], ld3(rc + 6), ld3(rc + 9));
+                pt_sample_done(s, acc);
+                continue;
+            }
+            if (s.m < 0) { pt_sample_done(s, acc); continue; }
+#pragma unroll
+            for (int i = 0; i < RZK_RECV_FLOATS; ++i) c.set(kPcRc + i, rc[i]);
+            uint32_t u0[4];
+            pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 0u, u0);
+            s.brdf_sampling = u24(u0[0]) < 0.5f;
+            if (s.bounce > 1 || pp.direct) {
+                if (pt_nee_begin(sv, pp, c, pixel, u0, s, q, nshadow, nclosest)) return true;
+            }
+        }
+        // the continuation (numbers 8..11), after the next event
+        if (s.bounce >= pp.max_bounce) { pt_sample_done(s, acc); continue; }
+        {
+            const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
+            const Onb b = c.onb();
+            const v3 sn = c.get3(kPcRc + 6), wo = c.get3(kPcRc + 9), p = c.get3(kPcRc);
+            const int gslice = s.bounce == 1 ? slice : -1;
+            uint32_t u2[4];
+            pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 2u, u2);
+            v3 wi, brdf;
+            float pdf;
+            if (s.brdf_sampling) brdf = bsdf_sample(mat, b, wo, u2, wi, pdf);
+            else brdf = guide_sample(gv, gslice, pp.res, mat, b, sn, wo, u2, wi, pdf);
+            if (!(pdf > 0.0f)) { pt_sample_done(s, acc); continue; }
+            c.set3(kPcWi, wi);
+            c.set3(kPcBrdf, brdf);
+            c.set(kPcPdf, pdf);
+            nclosest++;
+            q.r = ray_ctor(p, wi);
+            q.tmax = 1e6f;
+            q.any = false;
+            s.stage = kPsContHit;
+            return true;
+        }
+    }
+}
+
 __device__ __forceinline__ void add_ray_counts(unsigned long long* work, unsigned long long a, unsigned long long c) {
     for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off); c += __shfl_xor(c, off); }
     if ((threadIdx.x & 63) == 0) { atomicAdd(work, a); atomicAdd(work + 1, c); }
 }
 
-// One lane per pixel of the window, all its samples in order, summed in
-// sample order (MonteCarloIntegrator::integrate, monte_carlo_integrator.cc:45-70)
-__global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, PtParams pp, GuideView gv,
+// One lane per pixel of the window, all its samples in order (the state
+// machine above), summed in sample order (MonteCarloIntegrator::integrate,
+// monte_carlo_integrator.cc:45-70)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_guided_pt(SceneView sv, CamView cam, PtParams pp, GuideView gv,
                                                    float* __restrict__ out, unsigned long long* __restrict__ work) {
     extern __shared__ int s_stack[];
     AnyStack astk = any_stack(sv, s_stack);
@@ -283,10 +636,15 @@ __global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, Pt
         int x = pp.x0 + (int)(t % pp.w), y = pp.y0 + (int)(t / pp.w);
         int slice = pixel_slice(pp, gv, x, y);
         int strata = spp_strata(pp.spp);
-        v3 acc = mk(0.f, 0.f, 0.f);
-        for (int smp = 0; smp < pp.spp; ++smp)
-            acc = acc + trace_path(sv, cam.v, pp, gv, x, y, slice, strata, smp, astk, st, nclosest,
-                                   nshadow);
+        PtSink sink{nullptr, 0, 0, mk(0.f, 0.f, 0.f)};
+        const PtCold cold = pt_cold(s_stack);
+        PtLane ps;
+        ps.stage = kPsSample;
+        ps.smp = 0;
+        PtQuery q;
+        while (pt_advance(sv, cam.v, pp, gv, cold, x, y, slice, strata, pp.spp, ps, q, sink, nclosest, nshadow))
+            ps.qtri = closest_hit4(sv, q.r, pp.tmin, q.tmax, astk, ps.qt, ps.qb, ps.qg, st, q.any);
+        const v3 acc = sink.acc;
         int64_t o = (int64_t)y * pp.width + x;
         out[3 * o] = acc.x / (float)pp.spp;
         out[3 * o + 1] = acc.y / (float)pp.spp;
@@ -301,7 +659,7 @@ __global__ __launch_bounds__(128) void k_guided_pt(SceneView sv, CamView cam, Pt
 // One lane per given primary ray: ray i (a ray record) is sample smp[i] of pixel
 // (pix[2i], pix[2i+1]); out[3i..] is its radiance, unaveraged
 // (MonteCarloIntegrator::radiance(PrimaryRay&), monte_carlo_integrator.cc:40-43).
-__global__ __launch_bounds__(128) void k_guided_pt_rays(SceneView sv, PtParams pp, GuideView gv,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_guided_pt_rays(SceneView sv, PtParams pp, GuideView gv,
                                                         const float* __restrict__ rays, const int* __restrict__ pix,
                                                         const int* __restrict__ smp, int64_t n,
                                                         float* __restrict__ out) {
@@ -312,12 +670,18 @@ __global__ __launch_bounds__(128) void k_guided_pt_rays(SceneView sv, PtParams p
     unsigned long long nclosest = 0, nshadow = 0;
     if (i < n) {
         const float* rp = rays + 6 * i;
-        Ray r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
         int x = pix[2 * i], y = pix[2 * i + 1];
-        v3 L = trace_from(sv, pp, gv, r, x, y, pixel_slice(pp, gv, x, y), smp[i], astk, st, nclosest, nshadow);
-        out[3 * i] = L.x;
-        out[3 * i + 1] = L.y;
-        out[3 * i + 2] = L.z;
+        const int sm = smp[i];
+        PtSink sink{out + 3 * i, 3, sm, mk(0.f, 0.f, 0.f)};
+        const PtCold cold = pt_cold(s_stack);
+        PtLane ps;
+        ps.stage = kPsSample;
+        ps.smp = sm;
+        PtQuery q;
+        q.r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
+        while (pt_advance(sv, nullptr, pp, gv, cold, x, y, pixel_slice(pp, gv, x, y), 0, sm + 1, ps, q, sink, nclosest,
+                          nshadow))
+            ps.qtri = closest_hit4(sv, q.r, pp.tmin, q.tmax, astk, ps.qt, ps.qb, ps.qg, st, q.any);
     }
 }
 
@@ -867,7 +1231,7 @@ void launch_guided_pt_rays(rzk_ctx* ctx, const SceneView& sv_in, const PtParams&
     const int block = 128;
     unsigned grid = (unsigned)((n + block - 1) / block);
     bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4);
-    size_t lds = sizeof(int) * block * (size_t)kLdsStack;
+    size_t lds = sizeof(int) * block * ((size_t)kLdsStack + kPtCold);
     hipLaunchKernelGGL(k_guided_pt_rays, dim3(grid), dim3(block), lds, ctx->stream, sv, pp, gv, d_rays, d_pix, d_smp,
                        n, d_out);
     RZK_HIP(hipGetLastError());
@@ -883,7 +1247,7 @@ void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv_in, const float* cam, co
     bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4);
     CamView cv;
     for (int i = 0; i < RZK_CAM_FLOATS; ++i) cv.v[i] = cam[i];
-    size_t lds = sizeof(int) * block * (size_t)kLdsStack;
+    size_t lds = sizeof(int) * block * ((size_t)kLdsStack + kPtCold);
     hipLaunchKernelGGL(k_guided_pt, dim3(grid), dim3(block), lds, ctx->stream, sv, cv, pp, gv, d_out, d_work);
     RZK_HIP(hipGetLastError());
 }

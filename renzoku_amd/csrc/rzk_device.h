@@ -177,10 +177,13 @@ __device__ __forceinline__ bool any_hit4(const SceneView& sv, const Ray& r, floa
 // reference-order traversal whenever the closest triangle is unique (the
 // triangle test is the same arithmetic and every triangle whose box chain
 // passes at the final tmax is tested); on exact ties in t the winner may
-// differ.  The stack holds up to 2 * sv.stack4 ints.
+// differ.  The stack holds up to 2 * sv.stack4 ints.  any: return at the
+// first triangle hit (its index; t, beta, gamma of that hit).  tmax then never
+// shrinks, so every node whose box passes is visited until a hit: the hit /
+// no-hit answer is any_hit4's.
 __device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, float tmin, float tmax,
                                             AnyStack& stk, float& t_out, float& b_out, float& g_out,
-                                            TravStats& st) {
+                                            TravStats& st, bool any = false) {
     if (!box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]), mk(sv.rbmax[0], sv.rbmax[1], sv.rbmax[2]),
                  r, tmin, tmax))
         return -1;
@@ -223,6 +226,7 @@ __device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, f
                             g)) {
                     tmax = t; t_out = t; b_out = b; g_out = g;
                     best = __float_as_int(p0.w);
+                    if (any) return best;
                 }
             } else {
                 cd[c] = te;
