@@ -705,52 +705,82 @@ __global__ void k_batch_reset(const BatchHeader* __restrict__ hdr) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_batch_pt(SceneView sv, const BatchHeader* __restrict__ hdrp) {
+// item `item` for this lane: its job, pixel and samples; false when the
+// item's pixel block has no pixel for the lane
+__device__ __forceinline__ bool batch_item(const BatchHeader& hdr, unsigned long long item, int lane,
+                                           const JobGpu*& J, int& x, int& y, int& slice, int& s1, PtLane& ps,
+                                           PtSink& sink) {
+    int lo = 0, hi = (int)hdr.n_jobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if ((unsigned long long)hdr.jobs[mid].item_first <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    J = hdr.jobs + lo;
+    const int64_t local = (int64_t)(item - (unsigned long long)J->item_first);
+    const int nsb = J->samp_blocks;
+    const int64_t pb = local / nsb;
+    const int sb = (int)(local - pb * nsb);
+    const int64_t p = pb * 64 + lane;
+    const int64_t npx = (int64_t)J->pp.w * J->pp.h;
+    if (p >= npx) return false;
+    x = J->pp.x0 + (int)(p % J->pp.w);
+    y = J->pp.y0 + (int)(p / J->pp.w);
+    slice = pixel_slice(J->pp, J->gv, x, y);
+    const int s0 = sb * J->samples_per_item;
+    s1 = min(J->pp.spp, s0 + J->samples_per_item);
+    ps.stage = kPsSample;
+    ps.smp = s0;
+    sink.out = hdr.samples + J->sample_off + p * 3;
+    sink.stride = npx * 3;
+    sink.smp0 = 0;
+    return true;
+}
+
+// A wave takes items_per_dequeue items per atomic; each lane then runs its
+// pixel of those items one after the other through the path state machine,
+// so a lane whose paths end early goes on to its next item while the others
+// are still tracing.  Every sample's radiance goes to its slot of the job's
+// per-sample buffer.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_pt(
+    SceneView sv, const BatchHeader* __restrict__ hdrp) {
     extern __shared__ int s_stack[];
     const BatchHeader hdr = *hdrp;
     AnyStack astk = any_stack(sv, s_stack);
+    const PtCold cold = pt_cold(s_stack);
     const int lane = threadIdx.x & 63;
     TravStats st;
     unsigned long long nclosest = 0, nshadow = 0;
     const unsigned long long n_items = (unsigned long long)hdr.n_items;
-    const unsigned long long q = (unsigned long long)hdr.items_per_dequeue;
-    const int last_job = (int)hdr.n_jobs - 1;
+    const unsigned long long qd = (unsigned long long)hdr.items_per_dequeue;
     for (;;) {
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(hdr.counter, q);
+        if (lane == 0) base = atomicAdd(hdr.counter, qd);
         base = wave_bcast64(base);
         if (base >= n_items) break;
-        unsigned long long end = base + q < n_items ? base + q : n_items;
-        for (unsigned long long item = base; item < end; ++item) {
-            int lo = 0, hi = last_job;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if ((unsigned long long)hdr.jobs[mid].item_first <= item) lo = mid;
-                else hi = mid - 1;
-            }
-            const JobGpu* J = hdr.jobs + lo;
-            int64_t local = (int64_t)(item - (unsigned long long)J->item_first);
-            int nsb = J->samp_blocks;
-            int64_t pb = local / nsb;
-            int sb = (int)(local - pb * nsb);
-            int64_t p = pb * 64 + lane;
-            const PtParams pp = J->pp;
-            if (p < (int64_t)pp.w * pp.h) {
-                int x = pp.x0 + (int)(p % pp.w), y = pp.y0 + (int)(p / pp.w);
-                const GuideView gv = J->gv;
-                int slice = pixel_slice(pp, gv, x, y);
-                int s0 = sb * J->samples_per_item;
-                int s1 = min(pp.spp, s0 + J->samples_per_item);
-                const int64_t npx = (int64_t)pp.w * pp.h;
-                for (int smp = s0; smp < s1; ++smp) {
-                    v3 L = trace_path(sv, J->cam.v, pp, gv, x, y, slice, J->strata, smp, astk, st,
-                                      nclosest, nshadow);
-                    float* dst = hdr.samples + J->sample_off + ((int64_t)smp * npx + p) * 3;
-                    dst[0] = L.x;
-                    dst[1] = L.y;
-                    dst[2] = L.z;
+        const unsigned long long end = base + qd < n_items ? base + qd : n_items;
+        unsigned long long item = base;
+        const JobGpu* J = nullptr;
+        int x = 0, y = 0, slice = -1, s1 = 0;
+        bool have = false;
+        PtLane ps;
+        PtQuery q;
+        PtSink sink{nullptr, 0, 0, mk(0.f, 0.f, 0.f)};
+        for (;;) {
+            bool need = false;
+            while (item < end) {
+                if (!have) {
+                    have = batch_item(hdr, item, lane, J, x, y, slice, s1, ps, sink);
+                    if (!have) { ++item; continue; }
                 }
+                need = pt_advance(sv, J->cam.v, J->pp, J->gv, cold, x, y, slice, J->strata, s1, ps, q, sink,
+                                  nclosest, nshadow);
+                if (need) break;
+                have = false;
+                ++item;
             }
+            if (!need) break;
+            ps.qtri = closest_hit4(sv, q.r, J->pp.tmin, q.tmax, astk, ps.qt, ps.qb, ps.qg, st, q.any);
         }
     }
     if (hdr.work) add_ray_counts(hdr.work, nclosest, nshadow);
@@ -1219,7 +1249,7 @@ void launch_batch_reduce(hipStream_t stream, const BatchHeader* d_hdr, int grid)
 void launch_batch(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
                   int reduce_grid) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, d_hdr);
-    size_t lds = sizeof(int) * (size_t)block * (size_t)kLdsStack;
+    size_t lds = sizeof(int) * (size_t)block * ((size_t)kLdsStack + kPtCold);
     hipLaunchKernelGGL(k_batch_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
     hipLaunchKernelGGL(k_batch_reduce, dim3(reduce_grid), dim3(256), 0, stream, d_hdr);
 }
