@@ -789,7 +789,12 @@ class Clusters:
 
 
 class Server:
-    """rzk_server: continuous-batching request loop over a resident scene and clusters"""
+    """rzk_server: continuous-batching request loop over a resident scene and clusters.
+    launch: the engine and its launch shape; by default the persistent path kernel
+    (engine 0, 128-lane blocks, 16 per compute unit, one 64-pixel item per wave
+    dequeue); engine=1 selects the wavefront engine and its path slots."""
+
+    LAUNCH = dict(BatchConfig.DEFAULTS, engine=0, block=128, blocks_per_cu=16, items_per_dequeue=1)
 
     def __init__(self, device, scene, clusters, memory_budget=8 << 30, step_iterations=2, launch=None, guide=None):
         self.lib = load()
@@ -797,7 +802,7 @@ class Server:
         p = ServerParams()
         p.memory_budget = int(memory_budget)
         p.step_iterations = int(step_iterations)
-        p.launch = BatchConfig(**dict(BatchConfig.DEFAULTS, **(launch or {})))
+        p.launch = BatchConfig(**dict(self.LAUNCH, **(launch or {})))
         p.use_cluster_params = 0 if guide else 1
         if guide:
             p.guide = GuideParams(**guide)

@@ -219,6 +219,9 @@ static void batch_render(rzk_batch* b, const rzk_render_job* jobs, int32_t n, fl
     h.work = out_rays ? b->counters.p + 1 : nullptr;
     h.items_per_dequeue = b->cfg.items_per_dequeue;
     h.pad = 0;
+    h.job_lo = 0;
+    h.job_mask = -1;
+    h.item_lo = 0;
     hipStream_t st = ctx->stream;
     RZK_HIP(hipMemcpyAsync(b->jobs.p, b->h_jobs.p, sizeof(JobGpu) * n, hipMemcpyHostToDevice, st));
     RZK_HIP(hipMemcpyAsync(b->hdr.p, b->h_hdr.p, sizeof(BatchHeader), hipMemcpyHostToDevice, st));

@@ -43,13 +43,16 @@ struct JobGpu {
 // captured graph reads sizes and buffers from here, so one graph serves any
 // batch that fits the buffers.
 struct BatchHeader {
-    const JobGpu* jobs;
+    const JobGpu* jobs;            // job j at jobs[j & job_mask] (a ring when job_mask < n_jobs)
     int64_t n_jobs, n_items, n_pixels;
     float* samples;                // per-sample radiance
     float* out;                    // per-pixel mean, jobs concatenated, [h][w][3] each
     unsigned long long* counter;   // work-queue head
     unsigned long long* work;      // {closest rays, shadow rays} or null
     int items_per_dequeue, pad;
+    int64_t job_lo;                // first job whose items may be in the launch
+    int64_t job_mask;
+    int64_t item_lo;               // the launch runs items item_lo .. n_items - 1
 };
 
 // Wavefront engine: path state in structure-of-arrays slots ([3][P] planar
@@ -149,6 +152,8 @@ void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv, const float* cam, const
 // the three kernels of a batch on `stream` (captured into a graph by the caller)
 void launch_batch(hipStream_t stream, const SceneView& sv, const rzk::BatchHeader* d_hdr, int grid, int block,
                   int reduce_grid);
+// the queue reset and the persistent kernel alone (items item_lo .. n_items - 1)
+void launch_batch_items(hipStream_t stream, const SceneView& sv, const rzk::BatchHeader* d_hdr, int grid, int block);
 // one iteration of the wavefront engine (queues of parity `in` consumed,
 // parity in ^ 1 produced): extend + next-event rays, shadow rays, next-event
 // resolution, bounce shading, path regeneration

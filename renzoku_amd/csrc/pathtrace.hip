@@ -700,7 +700,7 @@ __device__ __forceinline__ unsigned long long wave_bcast64(unsigned long long v)
 
 __global__ void k_batch_reset(const BatchHeader* __restrict__ hdr) {
     if (threadIdx.x == 0) {
-        *hdr->counter = 0ull;
+        *hdr->counter = (unsigned long long)hdr->item_lo;
         if (hdr->work) { hdr->work[0] = 0ull; hdr->work[1] = 0ull; }
     }
 }
@@ -710,13 +710,13 @@ __global__ void k_batch_reset(const BatchHeader* __restrict__ hdr) {
 __device__ __forceinline__ bool batch_item(const BatchHeader& hdr, unsigned long long item, int lane,
                                            const JobGpu*& J, int& x, int& y, int& slice, int& s1, PtLane& ps,
                                            PtSink& sink) {
-    int lo = 0, hi = (int)hdr.n_jobs - 1;
+    int64_t lo = hdr.job_lo, hi = hdr.n_jobs - 1;
     while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if ((unsigned long long)hdr.jobs[mid].item_first <= item) lo = mid;
+        int64_t mid = (lo + hi + 1) >> 1;
+        if ((unsigned long long)hdr.jobs[mid & hdr.job_mask].item_first <= item) lo = mid;
         else hi = mid - 1;
     }
-    J = hdr.jobs + lo;
+    J = hdr.jobs + (lo & hdr.job_mask);
     const int64_t local = (int64_t)(item - (unsigned long long)J->item_first);
     const int nsb = J->samp_blocks;
     const int64_t pb = local / nsb;
@@ -1246,11 +1246,15 @@ void launch_batch_reduce(hipStream_t stream, const BatchHeader* d_hdr, int grid)
 }
 
 
-void launch_batch(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
-                  int reduce_grid) {
+void launch_batch_items(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, d_hdr);
     size_t lds = sizeof(int) * (size_t)block * ((size_t)kLdsStack + kPtCold);
     hipLaunchKernelGGL(k_batch_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
+}
+
+void launch_batch(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
+                  int reduce_grid) {
+    launch_batch_items(stream, sv, d_hdr, grid, block);
     hipLaunchKernelGGL(k_batch_reduce, dim3(reduce_grid), dim3(256), 0, stream, d_hdr);
 }
 

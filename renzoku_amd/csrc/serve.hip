@@ -5,13 +5,21 @@
 // thread admits them oldest first while the budget allows (guide tables +
 // per-sample radiance + image), builds each admitted request's guide from the
 // resident clusters, and appends the request as a job to the engine's job
-// ring.  The engine never stops between requests: its free path slots take
-// the next paths of whichever jobs are open, so requests that arrive while
-// others are on the GPU start as soon as slots free up (continuous batching
-// at path granularity).  When a job's paths are all finished its pixels are
-// summed in sample order, the image is copied out asynchronously, and its
-// memory returns to the budget.  A request's image is therefore the same,
-// bit for bit, alone or in any mix, and equal to the plain per-request render.
+// ring.  Two engines (launch.engine):
+//  * 0, the persistent path kernel: a job's work items (64-pixel blocks x
+//    samples) are numbered after the previous job's; each engine step runs
+//    the next range of items, up to step_iterations items per wave of the
+//    grid, across whichever jobs they belong to, so requests that arrive
+//    while others are on the GPU start at the next step.  A job is finished
+//    when the step holding its last item has finished.
+//  * 1, the wavefront engine: free path slots take the next paths of
+//    whichever jobs are open (continuous batching at path granularity); a
+//    job is finished when its finished-path count, read after each step,
+//    reaches its paths.
+// A finished job's pixels are summed in sample order, the image is copied out
+// asynchronously, and its memory returns to the budget.  A request's image is
+// therefore the same, bit for bit, alone or in any mix, and equal to the plain
+// per-request render.
 //
 // Replaces the reference's per-frame loop (Scene::start -> IntegratorScheduler,
 // scene.cc:441, integrator_scheduler.cc:143-229), which renders one image at
@@ -94,6 +102,7 @@ struct Request {
     size_t off_tab = 0, off_smp = 0, off_out = 0;
     rzk_guide guide;
     int64_t job = -1, paths = 0;
+    int64_t item_end = 0;            // persistent engine: one past the job's last work item
     hipEvent_t done_ev = nullptr;
     float* host_img = nullptr;       // pinned, filled by the final copy
     size_t host_bytes = 0;
@@ -119,6 +128,17 @@ struct rzk_server {
     rzk_guide_params gp{};
     Arena arena;
     WfEngine* wf = nullptr;
+    // persistent engine: launch shape, traversal spill, step headers, queue head, item counts
+    bool persistent = false;
+    int grid = 0, block = 0;
+    int64_t step_items = 0;
+    SceneView sv{};
+    int* d_spill = nullptr;
+    BatchHeader* d_hdr = nullptr;          // one per step slot
+    BatchHeader* h_hdr = nullptr;          // pinned staging, one per step slot
+    unsigned long long* d_head = nullptr;
+    int64_t total_items = 0, next_item = 0;
+    int64_t step_end[2] = {0, 0};
     JobGpu* d_jobs = nullptr;              // job ring
     JobGpu* h_jobs = nullptr;              // pinned staging, one entry per ring slot
     unsigned int* d_done = nullptr;        // finished paths per ring slot
@@ -204,11 +224,21 @@ struct rzk_server {
                               q->r.guide_mode, q->r.seed, q->r.tmin};
             J.pp = job_params(q->guide, rj);
             J.gv = q->guide.gv;
-            J.item_first = total_paths;
             J.sample_off = (int64_t)(o_smp / sizeof(float));
             int strata = 1;
             while ((strata + 1) * (strata + 1) <= q->r.spp) ++strata;
             J.strata = strata * strata == q->r.spp ? strata : 0;
+            const int64_t npx = (int64_t)q->W * q->H;
+            J.pix_blocks = (int)((npx + 63) / 64);
+            J.samples_per_item = 1;
+            J.samp_blocks = q->r.spp;
+            if (persistent) {
+                J.item_first = total_items;
+                total_items += (int64_t)J.pix_blocks * J.samp_blocks;
+                q->item_end = total_items;
+            } else {
+                J.item_first = total_paths;
+            }
             RZK_HIP(hipMemcpyAsync(d_jobs + (j & (kJobRing - 1)), &J, sizeof(JobGpu), hipMemcpyHostToDevice, st));
             RZK_HIP(hipMemsetAsync(d_done + (j & (kJobRing - 1)), 0, sizeof(unsigned int), st));
             q->job = j;
@@ -239,8 +269,12 @@ struct rzk_server {
         int64_t finished_paths = 0;
         for (auto it = open.begin(); it != open.end();) {
             Request* q = *it;
-            // a job admitted after the snapshot was queued has no count in it yet
-            if (q->job >= step_jobs[k] || (int64_t)done[q->job & (kJobRing - 1)] < q->paths) {
+            // a job admitted after the snapshot was queued has no count in it yet; with the
+            // persistent engine, every item before step k's end is done
+            const bool finished = persistent ? q->item_end <= step_end[k]
+                                             : q->job < step_jobs[k] &&
+                                                   (int64_t)done[q->job & (kJobRing - 1)] >= q->paths;
+            if (!finished) {
                 q->steps++;
                 ++it;
                 continue;
@@ -307,8 +341,9 @@ struct rzk_server {
                 }
                 continue;
             }
-            publish();
-            // one engine step: a few iterations, then a snapshot of the per-job finished counts
+            if (!persistent) publish();
+            // one engine step: a few iterations (or a range of work items), then a
+            // snapshot of the per-job finished counts
             int k = step_slot;
             step_slot ^= 1;
             if (step_busy[k]) {
@@ -317,14 +352,41 @@ struct rzk_server {
                 finish_done(k);
             }
             hipStream_t st = ctx->stream;
-            RZK_HIP(hipEventRecord(ev_t0[k], st));
-            wf_iterate(wf, iters);
-            RZK_HIP(hipMemcpyAsync(h_done + k * kJobRing, d_done, sizeof(unsigned int) * kJobRing,
-                                   hipMemcpyDeviceToHost, st));
-            RZK_HIP(hipEventRecord(ev_step[k], st));
-            step_busy[k] = true;
-            step_jobs[k] = n_jobs;
-            {
+            bool launched = true;
+            if (persistent) {
+                // the next range of work items
+                launched = next_item < total_items;
+                if (launched) {
+                    const int64_t a = next_item, e = std::min(total_items, a + step_items);
+                    BatchHeader& h = h_hdr[k];
+                    std::memset(&h, 0, sizeof(h));
+                    h.jobs = d_jobs;
+                    h.n_jobs = n_jobs;
+                    h.n_items = e;
+                    h.samples = (float*)arena.base;
+                    h.counter = d_head;
+                    h.items_per_dequeue = std::max(1, prm.launch.items_per_dequeue);
+                    h.job_lo = oldest_open_job();
+                    h.job_mask = kJobRing - 1;
+                    h.item_lo = a;
+                    RZK_HIP(hipMemcpyAsync(d_hdr + k, &h, sizeof(BatchHeader), hipMemcpyHostToDevice, st));
+                    RZK_HIP(hipEventRecord(ev_t0[k], st));
+                    launch_batch_items(st, sv, d_hdr + k, grid, block);
+                    RZK_HIP(hipGetLastError());
+                    RZK_HIP(hipEventRecord(ev_step[k], st));
+                    step_end[k] = e;
+                    next_item = e;
+                }
+            } else {
+                RZK_HIP(hipEventRecord(ev_t0[k], st));
+                wf_iterate(wf, iters);
+                RZK_HIP(hipMemcpyAsync(h_done + k * kJobRing, d_done, sizeof(unsigned int) * kJobRing,
+                                       hipMemcpyDeviceToHost, st));
+                RZK_HIP(hipEventRecord(ev_step[k], st));
+            }
+            if (launched) {
+                step_busy[k] = true;
+                step_jobs[k] = n_jobs;
                 std::lock_guard<std::mutex> lk(mu);
                 stats.rounds++;
             }
@@ -363,6 +425,10 @@ struct rzk_server {
         if (worker.joinable()) worker.join();
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         if (wf) wf_destroy(wf);
+        if (d_spill) (void)hipFree(d_spill);
+        if (d_hdr) (void)hipFree(d_hdr);
+        if (d_head) (void)hipFree(d_head);
+        if (h_hdr) (void)hipHostFree(h_hdr);
         for (int k = 0; k < 2; ++k) {
             if (ev_step[k]) (void)hipEventDestroy(ev_step[k]);
             if (ev_t0[k]) (void)hipEventDestroy(ev_t0[k]);
@@ -415,7 +481,8 @@ int rzk_server_create(int device, const rzk_scene* scene, const rzk_clusters* cl
         RZK_CHECK(prm->memory_budget > 0, RZK_E_INVALID, "bad server parameters");
         const rzk_batch_config& c = prm->launch;
         RZK_CHECK((c.block == 64 || c.block == 128 || c.block == 192 || c.block == 256) && c.blocks_per_cu >= 1 &&
-                      c.blocks_per_cu <= 64 && c.path_slots >= 4096 && c.path_slots <= (1 << 26),
+                      c.blocks_per_cu <= 64 && c.path_slots >= 4096 && c.path_slots <= (1 << 26) &&
+                      (c.engine == 0 || c.engine == 1) && c.items_per_dequeue >= 1 && c.items_per_dequeue <= 64,
                   RZK_E_INVALID, "bad launch shape");
         s->scene = scene;
         s->cl = cl;
@@ -434,9 +501,30 @@ int rzk_server_create(int device, const rzk_scene* scene, const rzk_clusters* cl
             RZK_HIP(hipEventCreate(&s->ev_step[k]));
             RZK_HIP(hipEventCreate(&s->ev_t0[k]));
         }
-        s->wf = wf_create(s->ctx, scene, c.path_slots, c.block, c.blocks_per_cu);
-        wf_reset(s->wf);
-        s->publish();
+        s->persistent = c.engine == 0;
+        if (s->persistent) {
+            s->block = c.block;
+            s->grid = std::max(1, s->ctx->num_cus) * c.blocks_per_cu;
+            const int64_t threads = (int64_t)s->grid * s->block;
+            s->step_items = (int64_t)std::max(1, std::min(8, prm->step_iterations)) * (threads / 64);
+            s->sv = scene_view(scene);
+            const int64_t extra = 2 * (int64_t)s->sv.stack4 - kLdsStack;   // closest_hit4 pairs beyond LDS
+            if (extra > 0) {
+                RZK_HIP(hipMalloc((void**)&s->d_spill, sizeof(int) * (size_t)(extra * threads)));
+                s->sv.spill = s->d_spill;
+                s->sv.spill_pitch = threads;
+            } else {
+                s->sv.spill = nullptr;
+                s->sv.spill_pitch = 0;
+            }
+            RZK_HIP(hipMalloc((void**)&s->d_hdr, 2 * sizeof(BatchHeader)));
+            RZK_HIP(hipMalloc((void**)&s->d_head, sizeof(unsigned long long)));
+            RZK_HIP(hipHostMalloc((void**)&s->h_hdr, 2 * sizeof(BatchHeader), hipHostMallocDefault));
+        } else {
+            s->wf = wf_create(s->ctx, scene, c.path_slots, c.block, c.blocks_per_cu);
+            wf_reset(s->wf);
+            s->publish();
+        }
         RZK_HIP(hipStreamSynchronize(s->ctx->stream));
         rzk_server* raw = s.get();
         s->worker = std::thread([raw] { raw->run(); });

@@ -6,7 +6,8 @@ request loop (rzk_guide_create, rzk_server_*).
   representative) transport values, same map build).
 * A request's image is the same, bit for bit, alone and in a mix of
   concurrent requests of other sizes, spp and cameras, and equals the plain
-  per-request drop-in (rzk_render_guided over rzk_guide_create).
+  per-request drop-in (rzk_render_guided over rzk_guide_create), with either
+  engine.
 * Admission never holds more than the memory budget."""
 import math
 import threading
@@ -63,7 +64,9 @@ def _plain(ctx, gs, cl, cam, kw):
     return img
 
 
-def test_served_alone_equals_mix_equals_plain(ctx, resident):
+@pytest.mark.parametrize("engine", [0, 1])
+def test_served_alone_equals_mix_equals_plain(ctx, resident, engine):
+    """engine 0: the persistent path kernel over ranges of work items; 1: the wavefront engine"""
     import renzoku_amd as rz
     sc, gs, vp, ls, cl = resident
     rng = np.random.default_rng(5)
@@ -73,7 +76,7 @@ def test_served_alone_equals_mix_equals_plain(ctx, resident):
         reqs.append((random_camera(rng, w, h), dict(spp=int(rng.integers(1, 10)), max_bounce=int(rng.integers(2, 6)),
                                                     direct_lighting=int(rng.integers(0, 2)), guide_mode=1,
                                                     seed=int(rng.integers(0, 2 ** 31)))))
-    srv = rz.Server(0, gs, cl, memory_budget=1 << 30, launch=dict(path_slots=8192))
+    srv = rz.Server(0, gs, cl, memory_budget=1 << 30, launch=dict(engine=engine, path_slots=8192))
     cam0, kw0 = reqs[0]
     alone, st_alone = srv.wait(srv.submit(cam0, **kw0))
     tickets = [srv.submit(cam, **kw) for cam, kw in reqs]
