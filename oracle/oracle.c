@@ -28,9 +28,11 @@
  * of the source: glibc powf/expf inside the Phong/Ward evaluations (ULP level,
  * "parity unpinned" beyond the tolerance stated in the tests).  The
  * transcendentals that pick a sampled direction (sin, cos, tan, atan, atan2,
- * acos, log, pow) are evaluated in double and rounded once, as the kernels do:
- * the correctly rounded float, which glibc's float functions return outside
- * hard-to-round cases.
+ * acos, log, pow) are restated in fp32 basic operations (xsincos .. xpow below),
+ * as the kernels restate them: within 3.5 ulp of the exact functions (pinned
+ * against float64 numpy in tests/test_oracle_xmath.py), where glibc's float
+ * functions are within 1 ulp, so a sampled direction may differ from the
+ * reference binary's by a few ulp.
  */
 #include "oracle.h"
 
@@ -984,19 +986,171 @@ int orc_cluster_by_sampling(int rows, int cols, const float *R, int num_clusters
 /* ======================================================================= */
 /* Concentric map and per-slice guiding distributions                       */
 /* ======================================================================= */
-/* Transcendentals that pick a ray direction: evaluated in double and rounded once
- * to float, as the kernels do (the same float from host and device double
- * functions, except within a double ulp of a float rounding boundary); the
- * correctly rounded value, which glibc's float functions also return outside
- * hard-to-round cases */
-static inline float sin_rd(float x) { return (float)sin((double)x); }
-static inline float cos_rd(float x) { return (float)cos((double)x); }
-static inline float tan_rd(float x) { return (float)tan((double)x); }
-static inline float atan_rd(float x) { return (float)atan((double)x); }
-static inline float atan2_rd(float y, float x) { return (float)atan2((double)y, (double)x); }
-static inline float acos_rd(float x) { return (float)acos((double)x); }
-static inline float log_rd(float x) { return (float)log((double)x); }
-static inline float pow_rd(float x, float y) { return (float)pow((double)x, (double)y); }
+/* Transcendentals that pick a ray direction, restated from the mathematics in
+ * fp32 add, multiply, divide and square root only (each correctly rounded, no
+ * contraction), in a fixed order, so the kernels' restatement of the same steps
+ * (rzk_math.h) gives the same bits as these for every argument.  Accuracy
+ * against the exact functions (tests/test_oracle_xmath.py): a few ulp on the
+ * angles and arguments the samplers use.
+ *
+ *   xsincos  |x| <= 1e4: k = nearest integer of x * 2/pi, r = x - k pi/2 with
+ *            pi/2 in three parts of which k times each is exact (Cody-Waite),
+ *            Taylor polynomials of degree 9 (sin) and 10 (cos) on |r| <= pi/4
+ *            (truncation below 2e-9), the quadrant k mod 4 picking and signing
+ *            the pair
+ *   xatan    |x| > 1 through pi/2 - atan(1/|x|), |x| > tan(pi/8) through
+ *            pi/4 + atan((|x| - 1) / (|x| + 1)), Taylor series to degree 21
+ *   xatan2   the quadrant from the signs, atan of the ratio of the smaller to
+ *            the larger magnitude (the C library's signed-zero cases)
+ *   xacos    |x| <= 1/2: pi/2 - asin(x); else 2 asin(sqrt((1 - |x|) / 2)),
+ *            reflected for x < 0; asin's Taylor series to degree 21 on [0, 1/2]
+ *   xlog     x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
+ *            s = (m - 1) / (m + 1), series to s^9; e ln 2 with ln 2 in two parts
+ *   xexp     k = nearest integer of t / ln 2, r = t - k ln 2 (two parts),
+ *            Taylor to r^8, times 2^k from the exponent bits
+ *   xpow     x >= 0: exp(y log x) */
+#define XM_PIO2_1   0x1.921ep+0f          /* pi/2 = PIO2_1 + PIO2_2 + PIO2_3 (+ 2^-58) */
+#define XM_PIO2_2   0x1.b544p-16f
+#define XM_PIO2_3   0x1.0b46p-34f
+#define XM_PIO2_LO  0x1.b54442p-16f      /* pi/2 - PIO2_1 */
+#define XM_PIO4_HI  0x1.921ep-1f
+#define XM_PIO4_LO  0x1.b54442p-17f
+#define XM_PI_HI    0x1.921ep+1f
+#define XM_PI_LO    0x1.b54442p-15f
+#define XM_LN2_HI   0x1.62e4p-1f
+#define XM_LN2_LO   0x1.7f7d1cp-20f
+
+static inline int xm_round(float v) { return (int)(v + (v >= 0.0f ? 0.5f : -0.5f)); }
+
+static void xsincos(float x, float *s, float *c) {
+    if (!(fabsf(x) <= 1e4f)) { *s = *c = (x - x) / (x - x); return; }
+    float kf = x * 0x1.45f306p-1f;
+    int k = xm_round(kf);
+    float fk = (float)k;
+    float r = ((x - fk * XM_PIO2_1) - fk * XM_PIO2_2) - fk * XM_PIO2_3;
+    float r2 = r * r;
+    float sr = r + r * r2 * (-0x1.555556p-3f + r2 * (0x1.111112p-7f + r2 * (-0x1.a01a02p-13f + r2 * 0x1.71de3ap-19f)));
+    float cr = 1.0f + r2 * (-0.5f + r2 * (0x1.555556p-5f + r2 * (-0x1.6c16c2p-10f + r2 * (0x1.a01a02p-16f +
+                                                                                       r2 * -0x1.27e4fcp-22f))));
+    switch (k & 3) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+    }
+}
+static inline float xsin(float x) { float s, c; xsincos(x, &s, &c); return s; }
+static inline float xcos(float x) { float s, c; xsincos(x, &s, &c); return c; }
+static inline float xtan(float x) { float s, c; xsincos(x, &s, &c); return s / c; }
+
+static float xatan(float x) {
+    if (x != x) return x;
+    float a = fabsf(x);
+    int inv = a > 1.0f, quarter;
+    if (inv) a = 1.0f / a;
+    quarter = a > 0x1.a8279ap-2f;
+    if (quarter) a = (a - 1.0f) / (a + 1.0f);
+    float z2 = a * a;
+    float p = a + a * z2 * (-0x1.555556p-2f + z2 * (0x1.99999ap-3f + z2 * (-0x1.24924ap-3f + z2 * (0x1.c71c72p-4f +
+              z2 * (-0x1.745d18p-4f + z2 * (0x1.3b13b2p-4f + z2 * (-0x1.111112p-4f + z2 * (0x1.e1e1e2p-5f +
+              z2 * (-0x1.af286cp-5f + z2 * 0x1.861862p-5f)))))))));
+    if (quarter) p = XM_PIO4_HI + (p + XM_PIO4_LO);
+    if (inv) p = (XM_PIO2_1 - p) + XM_PIO2_LO;
+    return copysignf(p, x);
+}
+
+static float xatan2(float y, float x) {
+    if (x != x || y != y) return x + y;
+    float ax = fabsf(x), ay = fabsf(y), t;
+    if (ay == 0.0f && ax == 0.0f) t = signbit(x) ? XM_PI_HI + XM_PI_LO : 0.0f;
+    else if (ay <= ax) {
+        t = xatan(ay / ax);
+        if (signbit(x)) t = (XM_PI_HI - t) + XM_PI_LO;
+    } else {
+        float u = xatan(ax / ay);
+        t = signbit(x) ? (XM_PIO2_1 + u) + XM_PIO2_LO : (XM_PIO2_1 - u) + XM_PIO2_LO;
+    }
+    return copysignf(t, y);
+}
+
+static inline float xasin_series(float z) {     /* |z| <= 1/2 */
+    float z2 = z * z;
+    return z + z * z2 * (0x1.555556p-3f + z2 * (0x1.333334p-4f + z2 * (0x1.6db6dcp-5f + z2 * (0x1.f1c71cp-6f +
+           z2 * (0x1.6e8ba2p-6f + z2 * (0x1.1c4ec4p-6f + z2 * (0x1.c9999ap-7f + z2 * (0x1.7a8788p-7f +
+           z2 * (0x1.3fde50p-7f + z2 * 0x1.12ef3cp-7f)))))))));
+}
+
+static float xacos(float x) {
+    float a = fabsf(x);
+    if (!(a <= 1.0f)) return (x - x) / (x - x);
+    if (a <= 0.5f) return (XM_PIO2_1 - xasin_series(x)) + XM_PIO2_LO;
+    float t = 2.0f * xasin_series(sqrtf((1.0f - a) * 0.5f));
+    return x > 0.0f ? t : (XM_PI_HI - t) + XM_PI_LO;
+}
+
+static float xlog(float x) {
+    if (x != x || x < 0.0f) return (x - x) / (x - x);
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    uint32_t ix;
+    memcpy(&ix, &x, 4);
+    int e = (int)(ix >> 23) - 127;
+    if (e == -127) {                                   /* subnormal */
+        float xs = x * 0x1p23f;
+        memcpy(&ix, &xs, 4);
+        e = (int)(ix >> 23) - 127 - 23;
+    }
+    ix = (ix & 0x007fffffu) | 0x3f800000u;
+    float m;
+    memcpy(&m, &ix, 4);
+    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e += 1; }
+    float sv = (m - 1.0f) / (m + 1.0f);
+    float s2 = sv * sv;
+    float lm = 2.0f * sv + sv * s2 * (0x1.555556p-1f + s2 * (0x1.99999ap-2f + s2 * (0x1.24924ap-2f + s2 * 0x1.c71c72p-3f)));
+    float fe = (float)e;
+    return fe * XM_LN2_HI + (lm + fe * XM_LN2_LO);
+}
+
+static float xexp(float t) {
+    if (t != t) return t;
+    if (t > 88.7f) return INFINITY;
+    if (t < -104.0f) return 0.0f;
+    int k = xm_round(t * 0x1.715476p+0f);
+    float fk = (float)k;
+    float r = (t - fk * XM_LN2_HI) - fk * XM_LN2_LO;
+    float p = 1.0f + (r + r * r * (0.5f + r * (0x1.555556p-3f + r * (0x1.555556p-5f + r * (0x1.111112p-7f +
+              r * (0x1.6c16c2p-10f + r * (0x1.a01a02p-13f + r * 0x1.a01a02p-16f)))))));
+    if (k < -126) { p = p * 0x1p-64f; k += 64; }
+    uint32_t b = (uint32_t)(k + 127) << 23;
+    float sc;
+    memcpy(&sc, &b, 4);
+    return p * sc;
+}
+
+static float xpow(float x, float y) {              /* x >= 0 */
+    if (x == 1.0f || y == 0.0f) return 1.0f;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : INFINITY;
+    return xexp(y * xlog(x));
+}
+
+/* test hook: fn 0 sin, 1 cos, 2 tan, 3 atan, 4 atan2(x, y), 5 acos, 6 log, 7 exp, 8 pow(x, y) */
+void orc_xmath(int fn, int n, const float *x, const float *y, float *out) {
+    for (int i = 0; i < n; ++i) {
+        float a = x[i], b = y ? y[i] : 0.0f, v = 0.0f;
+        switch (fn) {
+        case 0: v = xsin(a); break;
+        case 1: v = xcos(a); break;
+        case 2: v = xtan(a); break;
+        case 3: v = xatan(a); break;
+        case 4: v = xatan2(a, b); break;
+        case 5: v = xacos(a); break;
+        case 6: v = xlog(a); break;
+        case 7: v = xexp(a); break;
+        default: v = xpow(a, b); break;
+        }
+        out[i] = v;
+    }
+}
 
 /* concentric_mapping.h:16-47 */
 static void square_to_disk(float sx, float sy, float *u, float *v) {
@@ -1009,14 +1163,14 @@ static void square_to_disk(float sx, float sy, float *u, float *v) {
         if (a < b) { r = -a; phi = (A_PI / 4) * (4 + (b / a)); }
         else { r = -b; if (b != 0.0f) phi = (A_PI / 4) * (6 - (a / b)); else phi = 0.0f; }
     }
-    *u = r * cos_rd(phi);
-    *v = r * sin_rd(phi);
+    *u = r * xcos(phi);
+    *v = r * xsin(phi);
 }
 
 /* concentric_mapping.h:49-74 */
 static void disk_to_square(float dx, float dy, float *x, float *y) {
     float r = sqrtf(dx * dx + dy * dy);
-    float phi = atan2_rd(dy, dx);
+    float phi = xatan2(dy, dx);
     if (phi < -A_PI / 4) phi += TWO_PI;
     float a, b;
     if (phi < A_PI / 4) { a = r; b = phi * a / (A_PI / 4); }
@@ -1212,7 +1366,7 @@ static v3 l2w(v3 u, v3 v, v3 w, v3 l) { return add(add(scl(u, l.x), scl(v, l.y))
 static v3 w2l(v3 u, v3 v, v3 w, v3 x) { return mk(dot(u, x), dot(v, x), dot(w, x)); }
 
 static v3 sph2w(v3 u, v3 v, v3 w, float theta, float phi) {       /* onb.cc:155-163 */
-    float st = sin_rd(theta), ct = cos_rd(theta), sp = sin_rd(phi), cp = cos_rd(phi);
+    float st = xsin(theta), ct = xcos(theta), sp = xsin(phi), cp = xcos(phi);
     return l2w(u, v, w, mk(st * cp, st * sp, ct));
 }
 
@@ -1263,7 +1417,7 @@ static v3 bsdf_sample(const float *m, v3 u, v3 v, v3 w, v3 wo, const uint32_t *r
     if (type == ORC_LAMBERT) {
         float phi = TWO_PI * u24(r[1]);
         float st = sqrtf(u24(r[2]));
-        float x = st * cos_rd(phi), y = st * sin_rd(phi);
+        float x = st * xcos(phi), y = st * xsin(phi);
         float z = sqrtf(1.0f - st * st);
         *wi = unit(l2w(u, v, w, mk(x, y, z)));
         *pdf = z * INV_PI;
@@ -1274,7 +1428,7 @@ static v3 bsdf_sample(const float *m, v3 u, v3 v, v3 w, v3 wo, const uint32_t *r
     if (u24(r[0]) < rd) {
         float phi = TWO_PI * u24(r[1]);
         float st = sqrtf(u24(r[2]));
-        float x = st * cos_rd(phi), y = st * sin_rd(phi);
+        float x = st * xcos(phi), y = st * xsin(phi);
         float z = sqrtf(1.0f - st * st);
         *wi = unit(l2w(u, v, w, mk(x, y, z)));
     } else if (type == ORC_PHONG) {
@@ -1282,7 +1436,7 @@ static v3 bsdf_sample(const float *m, v3 u, v3 v, v3 w, v3 wo, const uint32_t *r
         v3 ou, ov, ow;
         onb_from_nu(wo, w, &ou, &ov, &ow);
         float phi = TWO_PI * u24(r[1]);
-        float alpha = acos_rd(pow_rd(u24(r[2]), 1.0f / (ne + 1.0f)));
+        float alpha = xacos(xpow(u24(r[2]), 1.0f / (ne + 1.0f)));
         v3 wr = sph2w(ou, ov, ow, alpha, phi);
         float d = dot(wr, w);
         if (d < 0.0f) return mk(0.f, 0.f, 0.f);
@@ -1293,11 +1447,11 @@ static v3 bsdf_sample(const float *m, v3 u, v3 v, v3 w, v3 wo, const uint32_t *r
         float s0 = u24o(r[1]), s1 = u24o(r[2]);
         v3 lwo = unit(w2l(u, v, w, wo));
         float angle = TWO_PI * s0;
-        float phi = atan_rd(ay_ax * tan_rd(angle));
-        if (sin_rd(phi) * sin_rd(angle) < 0 || cos_rd(phi) * cos_rd(angle) < 0) phi += A_PI;
-        float cp = cos_rd(phi), sp = sin_rd(phi);
-        float theta = atan_rd(sqrtf(-log_rd(s1) / (cp * cp / ax2 + sp * sp / ay2)));
-        float st = sin_rd(theta), ct = cos_rd(theta);
+        float phi = xatan(ay_ax * xtan(angle));
+        if (xsin(phi) * xsin(angle) < 0 || xcos(phi) * xcos(angle) < 0) phi += A_PI;
+        float cp = xcos(phi), sp = xsin(phi);
+        float theta = xatan(sqrtf(-xlog(s1) / (cp * cp / ax2 + sp * sp / ay2)));
+        float st = xsin(theta), ct = xcos(theta);
         v3 wh = mk(st * cp, st * sp, ct);
         v3 lwi = sub(scl(wh, 2.0f * dot(lwo, wh)), lwo);
         *wi = l2w(u, v, w, lwi);
@@ -1413,7 +1567,7 @@ static v3 guide_sample(const pt_args *a, int slice, const float *mat, v3 u, v3 v
                        const uint32_t *r, v3 *wi, float *pdf) {
     if (slice < 0) {
         float phi = TWO_PI * u24(r[0]);
-        float theta = acos_rd(u24(r[1]));
+        float theta = xacos(u24(r[1]));
         *wi = unit(sph2w(u, v, w, theta, phi));
     } else {
         int res = a->res;

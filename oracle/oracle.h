@@ -141,6 +141,9 @@ void orc_kd_find(const int *axis, const float *plane, const int *cl, int heap_us
 void orc_kd_nearest(const int *axis, const float *plane, const int *cl, int heap_used,
                     const float *boxes, int nslices, const float *q, int m, int k, int *out);
 
+/* ---- fp32 transcendentals of the direction samplers (test hook) ---- */
+void orc_xmath(int fn, int n, const float *x, const float *y, float *out);
+
 /* ---- guiding distributions ---- */
 void orc_direction_to_square(const float wi[3], float out[2]);
 void orc_square_to_direction(const float s[2], float out[3]);

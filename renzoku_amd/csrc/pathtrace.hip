@@ -145,7 +145,7 @@ __device__ __forceinline__ v3 guide_sample(const GuideView& gv, int slice, int r
                            const uint32_t* u, v3& wi, float& pdf) {
     if (slice < 0) {
         float phi = kTwoPi * u24(u[0]);
-        float theta = acos_rd(u24(u[1]));
+        float theta = xacos(u24(u[1]));
         wi = unit(spherical_to_world(b, theta, phi));
     } else {
         // DiscretePdf2D::sample (pdf.cc:171-186): row by the row cdf, column by that row's cdf

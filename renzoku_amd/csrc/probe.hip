@@ -1,4 +1,5 @@
-// probe.hip — libprobe.so: a vector-issue probe for the benchmark's rooflines
+// probe.hip — libprobe.so: a vector-issue probe for the benchmark's rooflines,
+// and the device evaluation of the samplers' fp32 transcendentals for the tests
 // (not part of the engine).  k_valu_probe keeps sixteen independent FMA chains
 // per lane, enough waves on every SIMD to hide the FMA latency, so it issues one
 // VALU instruction per SIMD slot; its SQ_INSTS_VALU x 64 / launch time is the
@@ -6,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+
+#include "rzk_math.h"
 
 __global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int iters, float m, float c) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -71,5 +74,47 @@ extern "C" int rzk_probe_valu(int kind, int blocks, int iters, float* out_ms) {
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);
     if (d) (void)hipFree(d);
+    return (int)err;
+}
+
+// fn 0 sin, 1 cos, 2 tan, 3 atan, 4 atan2(x, y), 5 acos, 6 log, 7 exp, 8 pow(x, y) of rzk_math.h
+__global__ void k_xmath(int fn, int n, const float* __restrict__ x, const float* __restrict__ y,
+                        float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float a = x[i], b = y ? y[i] : 0.0f;
+    float v;
+    switch (fn) {
+    case 0: v = rzk::xsin(a); break;
+    case 1: v = rzk::xcos(a); break;
+    case 2: v = rzk::xtan(a); break;
+    case 3: v = rzk::xatan(a); break;
+    case 4: v = rzk::xatan2(a, b); break;
+    case 5: v = rzk::xacos(a); break;
+    case 6: v = rzk::xlog(a); break;
+    case 7: v = rzk::xexp(a); break;
+    default: v = rzk::xpow(a, b); break;
+    }
+    out[i] = v;
+}
+
+// host arrays in and out (y may be NULL)
+extern "C" int rzk_probe_xmath(int fn, int n, const float* x, const float* y, float* out) {
+    if (n <= 0 || !x || !out || fn < 0 || fn > 8) return (int)hipErrorInvalidValue;
+    float *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    const size_t bytes = sizeof(float) * (size_t)n;
+    hipError_t err = hipMalloc((void**)&dx, bytes);
+    if (err == hipSuccess) err = hipMalloc((void**)&dout, bytes);
+    if (err == hipSuccess && y) err = hipMalloc((void**)&dy, bytes);
+    if (err == hipSuccess) err = hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice);
+    if (err == hipSuccess && y) err = hipMemcpy(dy, y, bytes, hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        hipLaunchKernelGGL(k_xmath, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, fn, n, dx, dy, dout);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost);
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (dout) (void)hipFree(dout);
     return (int)err;
 }

@@ -341,25 +341,148 @@ RZK_HD void lobe_weights(const float* m, float& rho_d, float& rho_s) {
     if (total > 0.0f) { rho_d /= total; rho_s /= total; }
 }
 
-// Transcendentals that pick a ray direction, evaluated in double and rounded
-// once to float.  Host (C library) and device double functions are both within
-// one double ulp, so these give the same float on both sides except where the
-// double result lies within a double ulp of a float rounding boundary (about
-// 2^-29 of arguments): the host restatement traces the same rays as the kernels.
-// The rounded value is the correctly rounded float result, which the C
-// library's float functions also return outside hard-to-round cases.
-RZK_HD float sin_rd(float x) { return (float)sin((double)x); }
-RZK_HD float cos_rd(float x) { return (float)cos((double)x); }
-RZK_HD float tan_rd(float x) { return (float)tan((double)x); }
-RZK_HD float atan_rd(float x) { return (float)atan((double)x); }
-RZK_HD float atan2_rd(float y, float x) { return (float)atan2((double)y, (double)x); }
-RZK_HD float acos_rd(float x) { return (float)acos((double)x); }
-RZK_HD float log_rd(float x) { return (float)log((double)x); }
-RZK_HD float pow_rd(float x, float y) { return (float)pow((double)x, (double)y); }
+// Transcendentals that pick a ray direction, from the mathematics in fp32
+// add, multiply, divide and square root only (each correctly rounded, no
+// contraction), in a fixed order, so that the host restatement of the same
+// steps gives the same bits for every argument and traces the same rays.
+// Within 3.5 ulp of the exact functions on the samplers' arguments.
+//   xsincos  |x| <= 1e4: k = nearest integer of x * 2/pi, r = x - k pi/2 with
+//            pi/2 in three parts of which k times each is exact (Cody-Waite),
+//            Taylor polynomials of degree 9 (sin) and 10 (cos) on |r| <= pi/4,
+//            the quadrant k mod 4 picking and signing the pair
+//   xatan    |x| > 1 through pi/2 - atan(1/|x|), |x| > tan(pi/8) through
+//            pi/4 + atan((|x| - 1) / (|x| + 1)), Taylor series to degree 21
+//   xatan2   the quadrant from the signs, atan of the smaller over the larger
+//            magnitude (the C library's signed-zero cases)
+//   xacos    |x| <= 1/2: pi/2 - asin(x); else 2 asin(sqrt((1 - |x|) / 2)),
+//            reflected for x < 0; asin's Taylor series to degree 21
+//   xlog     x = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh((m - 1) / (m + 1))
+//            to s^9, e ln 2 in two parts
+//   xexp     k = nearest integer of t / ln 2, r = t - k ln 2 (two parts), Taylor
+//            to r^8, times 2^k from the exponent bits
+//   xpow     x >= 0: exp(y log x)
+constexpr float kXmPio2_1 = 0x1.921ep+0f;      // pi/2 = kXmPio2_1 + kXmPio2_2 + kXmPio2_3 (+ 2^-58)
+constexpr float kXmPio2_2 = 0x1.b544p-16f;
+constexpr float kXmPio2_3 = 0x1.0b46p-34f;
+constexpr float kXmPio2Lo = 0x1.b54442p-16f;   // pi/2 - kXmPio2_1
+constexpr float kXmPio4Hi = 0x1.921ep-1f;
+constexpr float kXmPio4Lo = 0x1.b54442p-17f;
+constexpr float kXmPiHi = 0x1.921ep+1f;
+constexpr float kXmPiLo = 0x1.b54442p-15f;
+constexpr float kXmLn2Hi = 0x1.62e4p-1f;
+constexpr float kXmLn2Lo = 0x1.7f7d1cp-20f;
+
+RZK_HD int xm_round(float v) { return (int)(v + (v >= 0.0f ? 0.5f : -0.5f)); }
+
+RZK_HD void xsincos(float x, float& s, float& c) {
+    if (!(fabsf(x) <= 1e4f)) { s = c = (x - x) / (x - x); return; }
+    const float kf = x * 0x1.45f306p-1f;
+    const int k = xm_round(kf);
+    const float fk = (float)k;
+    const float r = ((x - fk * kXmPio2_1) - fk * kXmPio2_2) - fk * kXmPio2_3;
+    const float r2 = r * r;
+    const float sr = r + r * r2 * (-0x1.555556p-3f + r2 * (0x1.111112p-7f + r2 * (-0x1.a01a02p-13f + r2 * 0x1.71de3ap-19f)));
+    const float cr = 1.0f + r2 * (-0.5f + r2 * (0x1.555556p-5f + r2 * (-0x1.6c16c2p-10f + r2 * (0x1.a01a02p-16f +
+                                                                                            r2 * -0x1.27e4fcp-22f))));
+    switch (k & 3) {
+    case 0: s = sr; c = cr; break;
+    case 1: s = cr; c = -sr; break;
+    case 2: s = -sr; c = -cr; break;
+    default: s = -cr; c = sr; break;
+    }
+}
+RZK_HD float xsin(float x) { float s, c; xsincos(x, s, c); return s; }
+RZK_HD float xcos(float x) { float s, c; xsincos(x, s, c); return c; }
+RZK_HD float xtan(float x) { float s, c; xsincos(x, s, c); return s / c; }
+
+RZK_HD float xatan(float x) {
+    if (x != x) return x;
+    float a = fabsf(x);
+    const bool inv = a > 1.0f;
+    if (inv) a = 1.0f / a;
+    const bool quarter = a > 0x1.a8279ap-2f;
+    if (quarter) a = (a - 1.0f) / (a + 1.0f);
+    const float z2 = a * a;
+    float p = a + a * z2 * (-0x1.555556p-2f + z2 * (0x1.99999ap-3f + z2 * (-0x1.24924ap-3f + z2 * (0x1.c71c72p-4f +
+              z2 * (-0x1.745d18p-4f + z2 * (0x1.3b13b2p-4f + z2 * (-0x1.111112p-4f + z2 * (0x1.e1e1e2p-5f +
+              z2 * (-0x1.af286cp-5f + z2 * 0x1.861862p-5f)))))))));
+    if (quarter) p = kXmPio4Hi + (p + kXmPio4Lo);
+    if (inv) p = (kXmPio2_1 - p) + kXmPio2Lo;
+    return copysignf(p, x);
+}
+
+RZK_HD float xatan2(float y, float x) {
+    if (x != x || y != y) return x + y;
+    const float ax = fabsf(x), ay = fabsf(y);
+    float t;
+    if (ay == 0.0f && ax == 0.0f) {
+        t = signbit(x) ? kXmPiHi + kXmPiLo : 0.0f;
+    } else if (ay <= ax) {
+        t = xatan(ay / ax);
+        if (signbit(x)) t = (kXmPiHi - t) + kXmPiLo;
+    } else {
+        const float u = xatan(ax / ay);
+        t = signbit(x) ? (kXmPio2_1 + u) + kXmPio2Lo : (kXmPio2_1 - u) + kXmPio2Lo;
+    }
+    return copysignf(t, y);
+}
+
+RZK_HD float xasin_series(float z) {     // |z| <= 1/2
+    const float z2 = z * z;
+    return z + z * z2 * (0x1.555556p-3f + z2 * (0x1.333334p-4f + z2 * (0x1.6db6dcp-5f + z2 * (0x1.f1c71cp-6f +
+           z2 * (0x1.6e8ba2p-6f + z2 * (0x1.1c4ec4p-6f + z2 * (0x1.c9999ap-7f + z2 * (0x1.7a8788p-7f +
+           z2 * (0x1.3fde50p-7f + z2 * 0x1.12ef3cp-7f)))))))));
+}
+
+RZK_HD float xacos(float x) {
+    const float a = fabsf(x);
+    if (!(a <= 1.0f)) return (x - x) / (x - x);
+    if (a <= 0.5f) return (kXmPio2_1 - xasin_series(x)) + kXmPio2Lo;
+    const float t = 2.0f * xasin_series(sqrtf((1.0f - a) * 0.5f));
+    return x > 0.0f ? t : (kXmPiHi - t) + kXmPiLo;
+}
+
+RZK_HD float xlog(float x) {
+    if (x != x || x < 0.0f) return (x - x) / (x - x);
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    uint32_t ix = __builtin_bit_cast(uint32_t, x);
+    int e = (int)(ix >> 23) - 127;
+    if (e == -127) {                                   // subnormal
+        ix = __builtin_bit_cast(uint32_t, x * 0x1p23f);
+        e = (int)(ix >> 23) - 127 - 23;
+    }
+    float m = __builtin_bit_cast(float, (ix & 0x007fffffu) | 0x3f800000u);
+    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e += 1; }
+    const float sv = (m - 1.0f) / (m + 1.0f);
+    const float s2 = sv * sv;
+    const float lm = 2.0f * sv + sv * s2 * (0x1.555556p-1f + s2 * (0x1.99999ap-2f + s2 * (0x1.24924ap-2f + s2 * 0x1.c71c72p-3f)));
+    const float fe = (float)e;
+    return fe * kXmLn2Hi + (lm + fe * kXmLn2Lo);
+}
+
+RZK_HD float xexp(float t) {
+    if (t != t) return t;
+    if (t > 88.7f) return INFINITY;
+    if (t < -104.0f) return 0.0f;
+    int k = xm_round(t * 0x1.715476p+0f);
+    const float fk = (float)k;
+    const float r = (t - fk * kXmLn2Hi) - fk * kXmLn2Lo;
+    float p = 1.0f + (r + r * r * (0.5f + r * (0x1.555556p-3f + r * (0x1.555556p-5f + r * (0x1.111112p-7f +
+              r * (0x1.6c16c2p-10f + r * (0x1.a01a02p-13f + r * 0x1.a01a02p-16f)))))));
+    if (k < -126) { p = p * 0x1p-64f; k += 64; }
+    return p * __builtin_bit_cast(float, (uint32_t)(k + 127) << 23);
+}
+
+RZK_HD float xpow(float x, float y) {     // x >= 0
+    if (x == 1.0f || y == 0.0f) return 1.0f;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : INFINITY;
+    return xexp(y * xlog(x));
+}
 
 // Onb::spherical_to_world (onb.cc:155-163)
 RZK_HD v3 spherical_to_world(const Onb& b, float theta, float phi) {
-    float st = sin_rd(theta), ct = cos_rd(theta), sp = sin_rd(phi), cp = cos_rd(phi);
+    float st = xsin(theta), ct = xcos(theta), sp = xsin(phi), cp = xcos(phi);
     return local_to_world(b, mk(st * cp, st * sp, ct));
 }
 
@@ -395,8 +518,8 @@ RZK_HD float bsdf_pdf(const float* m, const Onb& b, v3 wo, v3 wi) {
 RZK_HD v3 cosine_dir(const Onb& b, float u0, float u1) {
     float phi = kTwoPi * u0;
     float sin_theta = sqrtf(u1);
-    float x = sin_theta * cos_rd(phi);
-    float y = sin_theta * sin_rd(phi);
+    float x = sin_theta * xcos(phi);
+    float y = sin_theta * xsin(phi);
     float z = sqrtf(1.0f - sin_theta * sin_theta);
     return unit(local_to_world(b, mk(x, y, z)));
 }
@@ -411,7 +534,7 @@ RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3
     if (type == kLambert) {
         float phi = kTwoPi * u24(u[1]);
         float sin_theta = sqrtf(u24(u[2]));
-        float x = sin_theta * cos_rd(phi), y = sin_theta * sin_rd(phi);
+        float x = sin_theta * xcos(phi), y = sin_theta * xsin(phi);
         float z = sqrtf(1.0f - sin_theta * sin_theta);
         wi = unit(local_to_world(b, mk(x, y, z)));
         pdf = z * kInvPi;
@@ -425,7 +548,7 @@ RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3
         float ne = m[7];
         Onb o = onb_from_nu(wo, b.n);
         float phi = kTwoPi * u24(u[1]);
-        float alpha = acos_rd(pow_rd(u24(u[2]), 1.0f / (ne + 1.0f)));
+        float alpha = xacos(xpow(u24(u[2]), 1.0f / (ne + 1.0f)));
         v3 wr = spherical_to_world(o, alpha, phi);
         float d = dot(wr, b.n);
         if (d < 0.0f) return mk(0.f, 0.f, 0.f);
@@ -437,11 +560,11 @@ RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3
         float s0 = u24_open(u[1]), s1 = u24_open(u[2]);
         v3 lwo = unit(world_to_local(b, wo));
         float angle = kTwoPi * s0;
-        float phi = atan_rd(ay_ax * tan_rd(angle));
-        if (sin_rd(phi) * sin_rd(angle) < 0 || cos_rd(phi) * cos_rd(angle) < 0) phi += kPi;
-        float cp = cos_rd(phi), sp = sin_rd(phi);
-        float theta = atan_rd(sqrtf(-log_rd(s1) / (cp * cp / ax2 + sp * sp / ay2)));
-        float st = sin_rd(theta), ct = cos_rd(theta);
+        float phi = xatan(ay_ax * xtan(angle));
+        if (xsin(phi) * xsin(angle) < 0 || xcos(phi) * xcos(angle) < 0) phi += kPi;
+        float cp = xcos(phi), sp = xsin(phi);
+        float theta = xatan(sqrtf(-xlog(s1) / (cp * cp / ax2 + sp * sp / ay2)));
+        float st = xsin(theta), ct = xcos(theta);
         v3 wh = mk(st * cp, st * sp, ct);
         v3 lwi = wh * (2.0f * dot(lwo, wh)) - lwo;
         wi = local_to_world(b, lwi);
@@ -455,7 +578,7 @@ RZK_HD void direction_to_square(v3 wi, float& sx, float& sy) {
     float r2 = 1.0f - wi.z;
     float dx = wi.x / sqrtf(2.0f - r2), dy = wi.y / sqrtf(2.0f - r2);
     float r = sqrtf(dx * dx + dy * dy);
-    float phi = atan2_rd(dy, dx);
+    float phi = xatan2(dy, dx);
     if (phi < -kPi / 4) phi += kTwoPi;
     float a, bb;
     if (phi < kPi / 4) { a = r; bb = phi * a / (kPi / 4); }
@@ -476,7 +599,7 @@ RZK_HD v3 square_to_direction(float sx, float sy) {
         if (a < bb) { r = -a; phi = (kPi / 4) * (4 + (bb / a)); }
         else { r = -bb; if (bb != 0.0f) phi = (kPi / 4) * (6 - (a / bb)); else phi = 0.0f; }
     }
-    float dx = r * cos_rd(phi), dy = r * sin_rd(phi);
+    float dx = r * xcos(phi), dy = r * xsin(phi);
     float r2 = dx * dx + dy * dy;
     return mk(dx * sqrtf(2.0f - r2), dy * sqrtf(2.0f - r2), 1.0f - r2);
 }
