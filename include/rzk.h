@@ -250,6 +250,24 @@ int rzk_render_vpl(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls,
 int rzk_reduced_matrix(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls,
                        const float *recv, const int32_t *recv_mat, int32_t rows, int clamp_mode,
                        float clamp, float tmin, float *out_R, float *out_V);
+/* The VPL evaluator over receiver x VPL pairs (IVirtualPointLightEvaluator,
+ * vpl_interface.h:126-139, as VirtualPointLight implements it):
+ *  term RZK_TERM_RADIANCE: radiance(scene, r, vpl, visible), vpl.cc:405-409 ->
+ *    gather_point :661-688: black when VPL k is blocked from receiver r, else the
+ *    clamped outgoing radiance (power x receiver BRDF x light BRDF x G) toward the
+ *    receiver's wo: the entries LightTransportMatrix::fill stores
+ *    (light_transport_matrix.cc:16-50);
+ *  term RZK_TERM_INCIDENT: incident_radiance(scene, r, vpl), vpl.cc:411-440: the
+ *    clamped incident radiance (geometric normal, white receiver BRDF), no
+ *    visibility.
+ * M(r,k) column-major Rgb [(k*rows+r)*3], V(r,k) = visibility [k*rows+r]; rows
+ * whose recv_mat is negative (no material, or rows the caller skips, such as
+ * receivers on a light) stay zero. */
+#define RZK_TERM_RADIANCE 0
+#define RZK_TERM_INCIDENT 1
+int rzk_transport_matrix(rzk_ctx *ctx, const rzk_scene *scene, const rzk_vpls *vpls,
+                         const float *recv, const int32_t *recv_mat, int32_t rows, int term, int clamp_mode,
+                         float clamp, float tmin, float *out_M, float *out_V);
 /* generate_alphas_for_clustering, light_transport_matrix.cc:72-106 (sequential
  * summation orders kept, bit-exact); mags = cluster_builder.cc:14-19 magnitude
  * of each flattened column (optional). */

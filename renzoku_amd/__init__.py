@@ -8,8 +8,8 @@ generator that feeds the gather (`vplgen`).
 from . import _lib, scenes, vplgen  # noqa: F401
 from ._lib import (Batch, Clusters, Ctx, Guide, KdSlices, LightSlice, RzkError, Scene, Server, Vpls,  # noqa: F401
                    alphas, cluster_assign, gather, generate_vpls, map_vec6, reduced_matrix, refine_slices,
-                   render_vpl, visibility)
+                   render_vpl, transport_matrix, visibility)
 
 __all__ = ["Ctx", "Scene", "Vpls", "LightSlice", "Guide", "Batch", "Clusters", "Server", "RzkError", "gather", "render_vpl", "visibility",
-           "reduced_matrix", "alphas", "cluster_assign", "refine_slices", "generate_vpls", "KdSlices", "map_vec6",
+           "reduced_matrix", "transport_matrix", "alphas", "cluster_assign", "refine_slices", "generate_vpls", "KdSlices", "map_vec6",
            "scenes", "vplgen"]

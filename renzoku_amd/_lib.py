@@ -29,7 +29,7 @@ EXPORTS = [
     "rzk_fill_receivers", "rzk_camera_primary",
     "rzk_vpls_create", "rzk_vpls_destroy", "rzk_generate_vpls",
     "rzk_gather", "rzk_gather_dev_noalloc", "rzk_visibility", "rzk_render_vpl",
-    "rzk_reduced_matrix", "rzk_alphas", "rzk_cluster_assign", "rzk_refine_slices",
+    "rzk_reduced_matrix", "rzk_transport_matrix", "rzk_alphas", "rzk_cluster_assign", "rzk_refine_slices",
     "rzk_lightslice_create", "rzk_lightslice_destroy", "rzk_lightslice_initialize",
     "rzk_lightslice_gather", "rzk_lightslice_get_info", "rzk_lightslice_count_work", "rzk_lightslice_get",
     "rzk_lightslice_render_guided",
@@ -187,6 +187,7 @@ def load() -> C.CDLL:
         "rzk_visibility": [P, P, P, P, P, I64, F, P],
         "rzk_render_vpl": [P, P, P, P, C.c_int, F, F, F, P],
         "rzk_reduced_matrix": [P, P, P, P, P, I32, C.c_int, F, F, P, P],
+        "rzk_transport_matrix": [P, P, P, P, P, I32, C.c_int, C.c_int, F, F, P, P],
         "rzk_alphas": [P, I32, I64, P, P, P],
         "rzk_cluster_assign": [P, I32, I64, P, I32, P, P, P],
         "rzk_refine_slices": [P, I32, I64, P, I32, P, I32, P, P, P, P, I32, P, I64, P, P, P],
@@ -523,6 +524,21 @@ def reduced_matrix(ctx, scene, vpls, recv, rmat, clamp_mode, clamp, tmin):
     ctx.check(ctx.lib.rzk_reduced_matrix(ctx.h, scene.h, vpls.h, ptr(recv), ptr(rmat), rows, clamp_mode, clamp, tmin,
                                          ptr(R), ptr(V)), "rzk_reduced_matrix")
     return R, V
+
+
+TERM_RADIANCE, TERM_INCIDENT = 0, 1
+
+
+def transport_matrix(ctx, scene, vpls, recv, rmat, clamp_mode, clamp, tmin, term=TERM_RADIANCE):
+    """rzk_transport_matrix: (M (V, rows, 3) per pair: outgoing radiance with
+    visibility, or incident radiance without; Vis (V, rows))"""
+    recv, rmat = f32(recv).reshape(-1, RECV_FLOATS), i32(rmat)
+    rows = len(rmat)
+    M = np.zeros((vpls.n, rows, 3), np.float32)
+    V = np.zeros((vpls.n, rows), np.float32)
+    ctx.check(ctx.lib.rzk_transport_matrix(ctx.h, scene.h, vpls.h, ptr(recv), ptr(rmat), rows, int(term), clamp_mode,
+                                           clamp, tmin, ptr(M), ptr(V)), "rzk_transport_matrix")
+    return M, V
 
 
 def alphas(ctx, R):

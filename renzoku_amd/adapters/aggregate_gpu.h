@@ -13,6 +13,7 @@
 // (aggregate.cc:138-152).
 #pragma once
 
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -41,6 +42,8 @@ public:
             throw std::runtime_error("AggregateGpu: one surface binding per surface");
         tri_surface_.assign(desc.tri_surface, desc.tri_surface + desc.num_triangles);
         tangents_.assign(desc.tangents, desc.tangents + 3 * (size_t)desc.num_triangles);
+        for (size_t i = 0; i < bindings_.size(); ++i)
+            if (bindings_[i].material) material_ids_[bindings_[i].material] = desc.surf_material[i];
         check(rzk_scene_create(ctx_, &desc, &scene_), "rzk_scene_create");
     }
     ~AggregateGpu() override { rzk_scene_destroy(scene_); }
@@ -128,6 +131,14 @@ public:
               "rzk_trace_segments");
     }
 
+    // the engine's material index of a bound surface's material (-1: none)
+    int material_id(const Material* m) const {
+        if (!m) return -1;
+        auto it = material_ids_.find(m);
+        if (it == material_ids_.end()) throw std::runtime_error("AggregateGpu: material of no bound surface");
+        return it->second;
+    }
+
     rzk_ctx* context() const { return ctx_; }
     rzk_scene* handle() const { return scene_; }
 
@@ -147,6 +158,7 @@ private:
     std::vector<SurfaceBinding> bindings_;
     std::vector<int32_t> tri_surface_;
     std::vector<float> tangents_;
+    std::map<const Material*, int> material_ids_;
 };
 
 }  // namespace Renzoku

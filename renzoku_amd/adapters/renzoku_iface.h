@@ -3,9 +3,12 @@
 // fields so the adapters compile and run here: Aggregate (aggregate.h:15-26),
 // Ray (ray.h:14-47), HitRecord (hitrecord.h:24-60), Integrator /
 // MonteCarloIntegrator (integrator.h:54-89, monte_carlo_integrator.h:28-54),
-// the sensor samples and primary-ray bundles (sensor.h:9-15, :124-149) and
-// the frame buffer's current image (frame.h:55-68).  Scene objects the
-// adapters only pass around (Material, AreaLight, Surface, Shape) stay opaque.
+// the sensor samples and primary-ray bundles (sensor.h:9-15, :124-149), the
+// frame buffer's current image (frame.h:55-68), Receiver / LocalGeometry
+// (local_geometry.h:11-60), BrdfPointLight (brdf_point_light.h:30-290), the
+// VPL evaluator (vpl_interface.h:126-139) and the transport matrices
+// (light_transport_matrix.h:16-80, matrix_c.h).  Scene objects the adapters
+// only pass around (Material, AreaLight, Surface, Shape) stay opaque.
 // In the reference tree the adapters include the reference's own headers
 // instead and nothing else changes.
 #pragma once
@@ -113,6 +116,120 @@ public:
     virtual bool hit(const Ray& r, Float tmin, Float tmax, Float time) const = 0;
     // visibility between p and q: true = blocked (aggregate.cc:84-90)
     virtual bool hit(const Vec3& p, const Vec3& q, Float tmin, Float time) const = 0;
+};
+
+// local_geometry.h:11-33
+struct Receiver {
+    Vec3 p;
+    Vec3 n;                  // geometric normal
+    Vec3 wo;
+    Material* m;
+    AreaLight* light;
+    Vec2 uv;
+    Vec3 tangent;
+    Vec3 shading_n;
+    Onb shading_uvn;
+    Receiver() : m(nullptr), light(nullptr) {}
+    Receiver(const Ray& r, const HitRecord& h)
+        : p(Vec3(r.org().x() + h.t * r.dir().x(), r.org().y() + h.t * r.dir().y(), r.org().z() + h.t * r.dir().z())),
+          n(h.normal), wo(Vec3(-r.dir().x(), -r.dir().y(), -r.dir().z())), m(h.material), light(h.light), uv(h.uv),
+          tangent(h.tangent), shading_n(h.shading_normal), shading_uvn(h.uvn) {}
+};
+
+// local_geometry.h:38-60
+struct LocalGeometry {
+    Vec3 p;
+    Vec3 n;                  // shading normal
+    Vec3 ng;                 // geometric normal
+    Vec3 tangent;
+    Vec2 uv;
+    Onb uvn;
+    LocalGeometry() {}
+    LocalGeometry(const Vec3& p, const Vec3& n, const Vec3& ng, const Onb& uvn, const Vec3& tangent, const Vec2 uv)
+        : p(p), n(n), ng(ng), tangent(tangent), uv(uv), uvn(uvn) {}
+    explicit LocalGeometry(const Receiver& r)
+        : p(r.p), n(r.shading_n), ng(r.n), tangent(r.tangent), uv(r.uv), uvn(r.shading_uvn) {}
+};
+
+// brdf_point_light.h: the parts a transport matrix reads.  A VPL on a light
+// (no material) has a white BRDF (brdf_point_light.h:235-240).
+class BrdfPointLight {
+public:
+    BrdfPointLight() : m(nullptr) {}
+    BrdfPointLight(const LocalGeometry& dg, Rgb power, Material* m, const Vec3& wi)
+        : dg(dg), contrib(power), m(m), wi(wi) {}
+    Vec3 org() const { return dg.p; }
+    Vec3 normal() const { return dg.n; }
+    LocalGeometry get_dg() const { return dg; }
+    Rgb power() const { return contrib; }
+    Rgb contribution() const { return contrib; }
+    Vec3 get_wi() const { return wi; }
+    Material* get_material() const { return m; }
+    bool is_on_light() const { return m == nullptr; }
+
+private:
+    LocalGeometry dg;
+    Rgb contrib;
+    Material* m;
+    Vec3 wi;
+};
+typedef std::vector<BrdfPointLight> BrdfPointLights;   // common.h:123
+typedef std::vector<Vec2> Pixels;                       // image_sampler.h:8
+
+class Scene;
+
+// vpl_interface.h:126-139
+class IVirtualPointLightEvaluator {
+public:
+    virtual ~IVirtualPointLightEvaluator() {}
+    virtual Rgb radiance(Scene* scene, const Receiver& r, const BrdfPointLight& light) {
+        bool visible;
+        return radiance(scene, r, light, visible);
+    }
+    virtual Rgb radiance(Scene* scene, const Receiver& r, const BrdfPointLight& light, bool& visible) = 0;
+    virtual Rgb incident_radiance(Scene* scene, const Receiver& r, const BrdfPointLight& light) = 0;
+};
+
+// matrix_c.h: a column-major rows x cols matrix, element (i, j) = row i, column j
+template <typename T>
+class MatrixC {
+public:
+    MatrixC(int rows = 0, int cols = 0) { resize(rows, cols); }
+    void resize(int rows, int cols) {
+        this->rows = rows;
+        this->cols = cols;
+        data.assign((size_t)rows * cols, T());
+    }
+    T& operator()(int i, int j) { return data[(size_t)j * rows + i]; }
+    const T& operator()(int i, int j) const { return data[(size_t)j * rows + i]; }
+    int get_rows() const { return rows; }
+    int get_cols() const { return cols; }
+    T* get_data() { return data.data(); }
+
+private:
+    int rows = 0, cols = 0;
+    std::vector<T> data;
+};
+
+// light_transport_matrix.h:16-55
+class TransportMatrix {
+public:
+    virtual ~TransportMatrix() {}
+    virtual void fill(Scene* scene, IVirtualPointLightEvaluator* renderer, BrdfPointLights& vpls, Pixels& pixels) = 0;
+    virtual int get_cols() const = 0;
+    virtual int get_rows() const = 0;
+};
+
+// light_transport_matrix.h:57-80: rows = pixel samples, columns = VPLs
+class LightTransportMatrix : public TransportMatrix {
+public:
+    LightTransportMatrix(int rows = 0, int cols = 0) : m(rows, cols) {}
+    int get_rows() const override { return m.get_rows(); }
+    int get_cols() const override { return m.get_cols(); }
+    virtual MatrixC<Rgb>& get_matrix() { return m; }
+
+protected:
+    MatrixC<Rgb> m;
 };
 
 // sensor.h:9-15

@@ -605,6 +605,29 @@ int rzk_reduced_matrix(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, cons
     });
 }
 
+int rzk_transport_matrix(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, const float* recv,
+                         const int32_t* recv_mat, int32_t rows, int term, int clamp_mode, float clamp, float tmin,
+                         float* out_M, float* out_V) {
+    return guarded(ctx, [&] {
+        RZK_CHECK(s && v && rows >= 0 && (rows == 0 || (recv && recv_mat && out_M && out_V)), RZK_E_INVALID,
+                  "bad transport matrix arguments");
+        RZK_CHECK(clamp_mode >= RZK_CLAMP_NONE && clamp_mode <= RZK_CLAMP_RADIANCE, RZK_E_INVALID, "bad clamp mode");
+        RZK_CHECK(term == RZK_TERM_RADIANCE || term == RZK_TERM_INCIDENT, RZK_E_INVALID, "bad transport term");
+        int64_t n = (int64_t)rows * v->n;
+        if (n == 0) return;
+        RZK_CHECK(s->bvh.ntri > 0, RZK_E_INVALID, "empty scene");
+        DevBuf<float> dv(RZK_RECV_FLOATS * rows), dM(3 * n), dV(n);
+        DevBuf<int> dm(rows);
+        dv.upload(ctx, recv, RZK_RECV_FLOATS * rows);
+        dm.upload(ctx, recv_mat, rows);
+        launch_reduced_matrix(ctx, scene_view(s), dv.p, dm.p, rows, v->d_order, v->d_vpl, v->d_mat, v->n, clamp_mode,
+                              clamp, tmin, dM.p, dV.p, nullptr, term == RZK_TERM_RADIANCE ? 1 : 2);
+        dM.download(ctx, out_M, 3 * n);
+        dV.download(ctx, out_V, n);
+        sync(ctx);
+    });
+}
+
 int rzk_alphas(rzk_ctx* ctx, int32_t rows, int64_t cols, const float* R, float* out_alphas, float* out_mags) {
     return guarded(ctx, [&] {
         RZK_CHECK(rows > 0 && cols >= 0 && (cols == 0 || (R && out_alphas)), RZK_E_INVALID, "bad alpha arguments");

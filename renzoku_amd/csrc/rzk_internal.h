@@ -180,7 +180,7 @@ void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv, const float* d_rec
                            const int* d_rmat, int rows, const int* d_vorder, const float* d_vpl,
                            const int* d_vmat,
                            int64_t nvpl, int clamp_mode, float clamp, float tmin, float* d_R,
-                           float* d_V, int* d_stats);
+                           float* d_V, int* d_stats, int term = 0);
 void launch_alphas(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_alphas,
                    float* d_mags);
 void launch_cluster_assign(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl,

@@ -141,7 +141,14 @@ constexpr int kRmXcd = 8;                      // XCDs; workgroup b runs on XCD 
 constexpr int kRmPair = 2;                     // blocks per row group, kRmXcd apart
 constexpr int kRmRows = kRmPair * (kRmBlock / 64);   // rows sharing a VPL chunk
 
-template <bool kCount>
+// kTerm 0: vis x incident radiance (the reduced matrix); 1: the light transport
+// matrix's entries (LightTransportMatrix::fill, light_transport_matrix.cc:16-50,
+// with VirtualPointLight::radiance as the evaluator, vpl.cc:405-409 ->
+// gather_point :661-688): black when blocked, else the clamped outgoing
+// radiance toward the receiver's wo; 2: the incident radiance without
+// visibility (VirtualPointLight::incident_radiance, vpl.cc:411-440).  V is the
+// visibility in every case.
+template <bool kCount, int kTerm = 0>
 __global__ __launch_bounds__(kRmBlock) void k_reduced_matrix(SceneView sv, const float* __restrict__ recv,
                                                                const int* __restrict__ rmat, int rows,
                                                                const int* __restrict__ vorder,
@@ -189,9 +196,19 @@ __global__ __launch_bounds__(kRmBlock) void k_reduced_matrix(SceneView sv, const
             float rc[RZK_RECV_FLOATS], vp[RZK_VPL_FLOATS];
             load_record(rc, recv + RZK_RECV_FLOATS * r, RZK_RECV_FLOATS);
             load_record(vp, vpl + RZK_VPL_FLOATS * k, RZK_VPL_FLOATS);
-            v3 inc = incident_radiance(sv.mat, rc, vp, vmat[k], mode, thr);
             vis = (float)(!blocked);
-            rv = inc.x * vis; g = inc.y * vis; b = inc.z * vis;
+            if (kTerm == 1) {
+                if (!blocked) {
+                    const v3 lo = gather_shade(sv.mat, rc, rmat[r], vp, vmat[k], mode, thr);
+                    rv = lo.x; g = lo.y; b = lo.z;
+                }
+            } else if (kTerm == 2) {
+                const v3 inc = incident_radiance(sv.mat, rc, vp, vmat[k], mode, thr);
+                rv = inc.x; g = inc.y; b = inc.z;
+            } else {
+                v3 inc = incident_radiance(sv.mat, rc, vp, vmat[k], mode, thr);
+                rv = inc.x * vis; g = inc.y * vis; b = inc.z * vis;
+            }
         }
         R[3 * o] = rv; R[3 * o + 1] = g; R[3 * o + 2] = b;
         V[o] = vis;
@@ -369,7 +386,7 @@ void launch_gather_shade(rzk_ctx* ctx, const SceneView& sv, const float* d_recv,
 void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv_in, const float* d_recv, const int* d_rmat,
                            int rows, const int* d_vorder, const float* d_vpl, const int* d_vmat,
                            int64_t nvpl, int clamp_mode, float clamp, float tmin, float* d_R, float* d_V,
-                           int* d_stats) {
+                           int* d_stats, int term) {
     if ((int64_t)rows * nvpl <= 0) return;
     SceneView sv = sv_in;
     // (row group, chunk) pairs, each kRmPair blocks kRmXcd apart (see the kernel);
@@ -378,7 +395,13 @@ void launch_reduced_matrix(rzk_ctx* ctx, const SceneView& sv_in, const float* d_
     const int64_t padded = (pairs + kRmXcd - 1) / kRmXcd * kRmXcd;
     unsigned grid = (unsigned)(padded * kRmPair);
     size_t lds = sizeof(int) * (kRmBlock / 64) * (size_t)sv.stack4;
-    if (d_stats)
+    if (term == 1)
+        hipLaunchKernelGGL((k_reduced_matrix<false, 1>), dim3(grid), dim3(kRmBlock), lds, ctx->stream, sv, d_recv,
+                           d_rmat, rows, d_vorder, d_vpl, d_vmat, nvpl, clamp_mode, clamp, tmin, d_R, d_V, nullptr);
+    else if (term == 2)
+        hipLaunchKernelGGL((k_reduced_matrix<false, 2>), dim3(grid), dim3(kRmBlock), lds, ctx->stream, sv, d_recv,
+                           d_rmat, rows, d_vorder, d_vpl, d_vmat, nvpl, clamp_mode, clamp, tmin, d_R, d_V, nullptr);
+    else if (d_stats)
         hipLaunchKernelGGL(k_reduced_matrix<true>, dim3(grid), dim3(kRmBlock), lds, ctx->stream, sv, d_recv,
                            d_rmat, rows, d_vorder, d_vpl, d_vmat, nvpl, clamp_mode, clamp, tmin, d_R, d_V, d_stats);
     else
