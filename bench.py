@@ -308,8 +308,7 @@ def run_guided(args):
     ctx = rz.Ctx(0)
     scene = rz.Scene(ctx, sc)
     log(f"[bench] scene: {sc.ntri} triangles, BVH depth {scene.info().depth} ({time.perf_counter() - t0:.1f} s)")
-    tracer, filler = vplgen.gpu_tracer(scene)
-    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 100_000, seed=2013, batch=65536)
+    vpl, vmat, n_emitted = vplgen.device_vpls(scene, args.vpls or 100_000, seed=2013)
     vpls = rz.Vpls(ctx, vpl, vmat)
     log(f"[bench] VPLs: {len(vmat)} from {n_emitted:.0f} light samples")
     ls = rz.LightSlice(ctx, scene, vpls, width=W, height=H, tile=16, num_clusters=1000, num_neighbor_slices=16,
@@ -385,8 +384,7 @@ def run_refine(args):
     sc = scenes.room(W, H, seed=2013, target_tris=args.tris or 200_000)
     ctx = rz.Ctx(0)
     scene = rz.Scene(ctx, sc)
-    tracer, filler = vplgen.gpu_tracer(scene)
-    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 65_536, seed=2013, batch=65536)
+    vpl, vmat, n_emitted = vplgen.device_vpls(scene, args.vpls or 65_536, seed=2013)
     vpls = rz.Vpls(ctx, vpl, vmat)
     params = dict(width=W, height=H, tile=32, num_clusters=300, num_neighbor_slices=16, map_resolution=32,
                   n_emitted=n_emitted, seed=2013)
@@ -573,10 +571,9 @@ def guides_for(ctx, sc, scene, levels):
     1,000 clusters each)"""
     import renzoku_amd as rz
     from renzoku_amd import vplgen
-    tracer, filler = vplgen.gpu_tracer(scene)
     guides = []
     for nv in levels:
-        vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, nv, seed=2013, batch=65536)
+        vpl, vmat, n_emitted = vplgen.device_vpls(scene, nv, seed=2013)
         vp = rz.Vpls(ctx, vpl, vmat)
         ls = rz.LightSlice(ctx, scene, vp, width=1280, height=720, tile=16, num_clusters=min(1000, nv // 4),
                            num_neighbor_slices=16, map_resolution=32, n_emitted=n_emitted, seed=2013)
@@ -689,8 +686,7 @@ def setup_c5(args, ctx):
     sc = scenes.instanced_meshes(1280, 720, seed=2013, target_tris=args.tris5 or 2_000_000)
     scene = rz.Scene(ctx, sc)
     log(f"[bench] C5 scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
-    tracer, filler = vplgen.gpu_tracer(scene)
-    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls5 or 200_000, seed=2013, batch=65536)
+    vpl, vmat, n_emitted = vplgen.device_vpls(scene, args.vpls5 or 200_000, seed=2013)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, scene, vp, width=1280, height=720, tile=32, num_clusters=1000, num_neighbor_slices=16,
                        n_emitted=n_emitted, seed=2013)
@@ -837,8 +833,7 @@ def setup_c2(args):
     scene = rz.Scene(ctx, sc)
     log(f"[bench] C2 scene: {sc.ntri} triangles, {scene.info().num_interior_nodes} interior nodes "
         f"({time.perf_counter() - t0:.1f} s)")
-    tracer, filler = vplgen.gpu_tracer(scene)
-    vpl, vmat, n_emitted = vplgen.generate_vpls(sc, tracer, filler, args.vpls or 65_536, seed=2013, batch=65536)
+    vpl, vmat, n_emitted = vplgen.device_vpls(scene, args.vpls or 65_536, seed=2013)
     params["n_emitted"] = n_emitted
     vpls = rz.Vpls(ctx, vpl, vmat)
     log(f"[bench] C2 VPLs: {len(vmat)} from {n_emitted:.0f} light samples")
@@ -968,8 +963,7 @@ def setup_c1(ctx):
     from renzoku_amd import scenes, vplgen
     sc = scenes.cornell_box(256, 256)
     scene = rz.Scene(ctx, sc)
-    tracer, filler = vplgen.gpu_tracer(scene)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 1024, seed=2013, batch=4096)
+    vpl, vmat, ne = vplgen.device_vpls(scene, 1024, seed=2013)
     vpls = rz.Vpls(ctx, vpl, vmat)
     px, py = scenes.pixel_grid(256, 256)
     _, _, recv, m, _ = scene.camera_primary(sc.camera, px, py)
@@ -1058,8 +1052,7 @@ def setup_c3(args, ctx):
     sc = scenes.instanced_meshes(W, H, seed=2013, target_tris=args.tris3 or 1_000_000)
     scene = rz.Scene(ctx, sc)
     log(f"[bench] C3 scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
-    tracer, filler = vplgen.gpu_tracer(scene)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 100_000, seed=2013, batch=65536)
+    vpl, vmat, ne = vplgen.device_vpls(scene, 100_000, seed=2013)
     vpls = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, scene, vpls, width=W, height=H, tile=16, num_clusters=1000, num_neighbor_slices=16,
                        map_resolution=32, n_emitted=ne, seed=2013)
@@ -1067,7 +1060,7 @@ def setup_c3(args, ctx):
     ls.initialize(sc.camera)
     ctx.synchronize()
     init_ms = (time.perf_counter() - t1) * 1e3
-    return dict(sc=sc, scene=scene, vpls=vpls, ls=ls, init_ms=init_ms, tracer=tracer, filler=filler)
+    return dict(sc=sc, scene=scene, vpls=vpls, ls=ls, init_ms=init_ms)
 
 
 C3_KW = dict(spp=16, max_bounce=5, direct_lighting=0, guide_mode=1, seed=2013, tmin=1e-3)

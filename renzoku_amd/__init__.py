@@ -2,8 +2,9 @@
 
 The compute lives in librzk.so (gfx950 HIP kernels behind the C ABI of
 include/rzk.h).  This package holds its ctypes binding (`_lib`), the seeded
-synthetic scenes the workloads are defined on (`scenes`) and the VPL set
-generator that feeds the gather (`vplgen`).
+synthetic scenes the workloads are defined on (`scenes`), the VPL set of a
+resident scene through the engine's generator and the pixel-tile helpers
+(`vplgen`), scene / frame files and the render-configuration importer.
 """
 from . import _lib, scenes, vplgen  # noqa: F401
 from ._lib import (Batch, Clusters, Ctx, Guide, KdSlices, LightSlice, RzkError, Scene, Server, Vpls,  # noqa: F401

@@ -32,8 +32,7 @@ def cornell_setup(ctx, cornell):
     from renzoku_amd import scenes, vplgen
     gs = rz.Scene(ctx, cornell)
     osc = oracle.OrcScene(cornell)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(cornell, tracer, filler, 1024, seed=2013, batch=4096)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 1024, seed=2013)
     px, py = scenes.pixel_grid(256, 256)
     sel = np.arange(0, 256 * 256, 5)
     rays = oracle.camera_rays(cornell.camera, px[sel], py[sel])
@@ -82,8 +81,7 @@ def room_setup(ctx, small_room):
     from renzoku_amd import vplgen
     gs = rz.Scene(ctx, small_room)
     osc = oracle.OrcScene(small_room)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(small_room, tracer, filler, 2048, seed=5, batch=8192)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 2048, seed=5)
     return gs, osc, vpl, vmat, ne, rz.Vpls(ctx, vpl, vmat)
 
 

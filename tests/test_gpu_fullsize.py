@@ -27,8 +27,7 @@ def test_c2_headline_configuration(ctx):
     W = H = 512
     sc = scenes.room(W, H, seed=2013, target_tris=200_000)
     gs = rz.Scene(ctx, sc)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 65_536, seed=2013, batch=65536)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 65_536, seed=2013)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, gs, vp, width=W, height=H, tile=32, num_clusters=300, num_neighbor_slices=16,
                        map_resolution=32, map_epsilon=1e-3, clamp_mode=1, clamp=0.01, incident_clamp=0.01,
@@ -140,8 +139,7 @@ def test_c5_served_full_scene(ctx):
     from renzoku_amd import scenes, vplgen
     sc = scenes.instanced_meshes(1280, 720, seed=2013, target_tris=2_000_000)
     gs = rz.Scene(ctx, sc)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 200_000, seed=2013, batch=65536)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 200_000, seed=2013)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, gs, vp, width=1280, height=720, tile=32, num_clusters=1000, num_neighbor_slices=16,
                        n_emitted=ne, seed=2013)

@@ -149,8 +149,7 @@ def test_lightslice_with_kd_slices_matches_oracle(ctx, pivots):
     W = H = 128
     sc = scenes.room(W, H, seed=11, target_tris=20_000)
     gs = rz.Scene(ctx, sc)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 2048, seed=5, batch=8192)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 2048, seed=5)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, gs, vp, width=W, height=H, num_clusters=32, num_neighbor_slices=4, n_emitted=ne,
                        seed=2013, max_slice_clusters=64, kd_max_slice_size=128, kd_pivots=int(pivots == "mt"))
@@ -215,8 +214,7 @@ def test_lightslice_kd_slices_without_refinement(ctx):
     W, H = 96, 80
     sc = scenes.cornell_box(W, H)
     gs = rz.Scene(ctx, sc)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 1024, seed=3, batch=4096)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 1024, seed=3)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, gs, vp, width=W, height=H, num_clusters=24, num_neighbor_slices=8, n_emitted=ne,
                        seed=99, kd_max_slice_size=64)

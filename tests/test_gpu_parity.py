@@ -102,10 +102,9 @@ def cornell_vpls(ctx, scenes_gpu):
     import renzoku_amd as rz
     from renzoku_amd import vplgen
     sc, gs, os_ = scenes_gpu["cornell"]
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 1024, seed=2013, batch=4096)
-    # the same VPL set from the oracle's tracing
-    ovpl, ovmat, one = vplgen.generate_vpls(sc, os_.trace_closest, os_.fill_receivers, 1024, seed=2013, batch=4096)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 1024, seed=2013)
+    # the same VPL set from the oracle's restatement of the generator
+    ovpl, ovmat, one = os_.generate_vpls(1024, seed=2013)
     assert np.array_equal(vpl.view(np.uint32), ovpl.view(np.uint32)) and ne == one
     return vpl, vmat, ne, rz.Vpls(ctx, vpl, vmat)
 
@@ -178,8 +177,7 @@ def room_setup(ctx, scenes_gpu):
     import renzoku_amd as rz
     from renzoku_amd import vplgen
     sc, gs, os_ = scenes_gpu["room"]
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 2048, seed=5, batch=8192)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 2048, seed=5)
     return sc, gs, os_, vpl, vmat, ne, rz.Vpls(ctx, vpl, vmat)
 
 
@@ -298,8 +296,7 @@ def test_render_vpl_chunked_gather_is_one_launch(ctx, scenes_gpu):
     import renzoku_amd as rz
     from renzoku_amd import scenes, vplgen
     sc, gs, os_ = scenes_gpu["cornell"]
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 2560, seed=8, batch=8192)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 2560, seed=8)
     vp = rz.Vpls(ctx, vpl, vmat)
     cam = np.array(scenes.cornell_box(1024, 1024).camera, np.float32)   # same box, 1024x1024 camera
     img = rz.render_vpl(ctx, gs, vp, cam, 1, 0.01, 1e-3, ne)

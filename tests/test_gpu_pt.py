@@ -22,8 +22,7 @@ def _setup(ctx, sc, nvpl, tile, clusters, seed=5):
     import renzoku_amd as rz
     from renzoku_amd import vplgen
     gs = rz.Scene(ctx, sc)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, nvpl, seed=seed, batch=8192)
+    vpl, vmat, ne = vplgen.device_vpls(gs, nvpl, seed=seed)
     vp = rz.Vpls(ctx, vpl, vmat)
     W, H = int(sc.camera[17]), int(sc.camera[18])
     ls = rz.LightSlice(ctx, gs, vp, width=W, height=H, tile=tile, num_clusters=clusters, num_neighbor_slices=4,

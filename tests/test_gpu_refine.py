@@ -101,8 +101,7 @@ def room_refined(ctx):
     from renzoku_amd import scenes, vplgen
     sc = scenes.room(128, 128, seed=11, target_tris=20_000)
     gs = rz.Scene(ctx, sc)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 2048, seed=5, batch=8192)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 2048, seed=5)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, gs, vp, width=128, height=128, tile=16, num_clusters=32, num_neighbor_slices=4,
                        n_emitted=ne, seed=2013, max_slice_clusters=64)

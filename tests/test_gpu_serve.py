@@ -24,8 +24,7 @@ def resident(ctx):
     from renzoku_amd import scenes, vplgen
     sc = scenes.room(96, 80, seed=11, target_tris=20_000)
     gs = rz.Scene(ctx, sc)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(sc, tracer, filler, 3072, seed=4, batch=8192)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 3072, seed=4)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, gs, vp, width=96, height=80, tile=16, num_clusters=48, num_neighbor_slices=4,
                        n_emitted=ne, seed=2013)

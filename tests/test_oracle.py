@@ -424,7 +424,7 @@ def test_gather_point_matches_composition(cornell):
     rays = oracle.camera_rays(cornell.camera, px[sel], py[sel])
     t, b, g, tri = os_.trace_closest(rays)
     recv, m, _ = os_.fill_receivers(rays, t, b, g, tri)
-    vpl, vmat, ne = vplgen.generate_vpls(cornell, os_.trace_closest, os_.fill_receivers, 64, seed=9, batch=256)
+    vpl, vmat, ne = os_.generate_vpls(64, seed=9)
     for i in range(len(sel)):
         if m[i] < 0:
             continue
@@ -458,7 +458,7 @@ def test_gather_threads_deterministic(cornell):
     rays = oracle.camera_rays(cornell.camera, px[sel], py[sel])
     t, b, g, tri = os_.trace_closest(rays)
     recv, m, _ = os_.fill_receivers(rays, t, b, g, tri)
-    vpl, vmat, ne = vplgen.generate_vpls(cornell, os_.trace_closest, os_.fill_receivers, 128, seed=2, batch=512)
+    vpl, vmat, ne = os_.generate_vpls(128, seed=2)
     a = os_.gather(recv, m, vpl, vmat, 1, 0.01, 1e-3, ne, nthreads=1)
     b2 = os_.gather(recv, m, vpl, vmat, 1, 0.01, 1e-3, ne, nthreads=6)
     assert np.array_equal(a, b2)

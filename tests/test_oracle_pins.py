@@ -247,7 +247,7 @@ def test_reduced_matrix_composition(cornell):
             n_s = _unit(recv[i, 3:6] + np.array([0.3, -0.2, 0.25]))
             recv[i] = _receiver(recv[i, :3], recv[i, 3:6], n_s, recv[i, 9:12])
     assert (m < 0).any() and (m >= 0).sum() > 10
-    vpl, vmat, _ = vplgen.generate_vpls(cornell, osc.trace_closest, osc.fill_receivers, 40, seed=5, batch=256)
+    vpl, vmat, _ = osc.generate_vpls(40, seed=5)
     for mode, thr in ((1, 0.01), (3, 5.0)):
         R, V = osc.reduced_matrix(recv, m, vpl, vmat, mode, thr, 1e-3)
         assert R.shape == (40, len(sel), 3) and V.shape == (40, len(sel))
@@ -406,7 +406,7 @@ def test_gather_clusters_weights_after_the_clamp(cornell):
     rays = oracle.camera_rays(cornell.camera, px[sel], py[sel])
     t, b, g, tri = osc.trace_closest(rays)
     recv, m, _ = osc.fill_receivers(rays, t, b, g, tri)
-    vpl, vmat, ne = vplgen.generate_vpls(cornell, osc.trace_closest, osc.fill_receivers, 48, seed=3, batch=512)
+    vpl, vmat, ne = osc.generate_vpls(48, seed=3)
     w = np.random.default_rng(2).uniform(0.5, 30.0, 48).astype(np.float32)
     for mode, thr in ((1, 3e-6), (3, 0.04)):
         ones = osc.gather_clusters(recv, m, vpl, vmat, np.ones(48, np.float32), mode, thr, 1e-3, ne)

@@ -40,8 +40,7 @@ def test_noalloc_gather_matches_gather(ctx, cornell):
     import renzoku_amd as rz
     from renzoku_amd import scenes, vplgen
     gs = rz.Scene(ctx, cornell)
-    tracer, filler = vplgen.gpu_tracer(gs)
-    vpl, vmat, ne = vplgen.generate_vpls(cornell, tracer, filler, 512, seed=2013, batch=4096)
+    vpl, vmat, ne = vplgen.device_vpls(gs, 512, seed=2013)
     big = rz.Vpls(ctx, vpl, vmat)
     small = rz.Vpls(ctx, vpl[:77], vmat[:77])
     px, py = scenes.pixel_grid(256, 256)
