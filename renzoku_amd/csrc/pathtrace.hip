@@ -621,6 +621,28 @@ __device__ __forceinline__ void add_ray_counts(unsigned long long* work, unsigne
     if ((threadIdx.x & 63) == 0) { atomicAdd(work, a); atomicAdd(work + 1, c); }
 }
 
+// Lanes still walking the tree at or below which a wave stops walking and lets
+// its waiting lanes shade and start their next query; the walkers resume from
+// where they stopped on the next pass.
+#ifndef RZK_TRAV_REGEN
+#define RZK_TRAV_REGEN 8
+#endif
+constexpr int kTravRegen = RZK_TRAV_REGEN;
+
+// The traversal phase of the path kernels: walk the wave's open queries one
+// node at a time until none is open, or until at most kTravRegen are and some
+// lane is waiting to shade.  Each query's visits are closest_hit4's, so the
+// answers do not depend on when the phase stops.
+__device__ __forceinline__ void trav_phase(const SceneView& sv, const Ray& r, float tmin, Trav4& tv,
+                                           AnyStack& astk, PtLane& ps, TravStats& st, bool busy, bool& trav) {
+    for (;;) {
+        const unsigned long long tm = __ballot(trav);
+        if (tm == 0ull) break;
+        if (kTravRegen > 0 && __popcll(tm) <= kTravRegen && __ballot(busy && !trav) != 0ull) break;
+        if (trav) trav = trav4_step(sv, r, tmin, tv, astk, ps.qt, ps.qb, ps.qg, st);
+    }
+}
+
 // One lane per pixel of the window, all its samples in order (the state
 // machine above), summed in sample order (MonteCarloIntegrator::integrate,
 // monte_carlo_integrator.cc:45-70)
@@ -631,19 +653,30 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     TravStats st;
     unsigned long long nclosest = 0, nshadow = 0;
-    int64_t npx = (int64_t)pp.w * pp.h;
-    if (t < npx) {
-        int x = pp.x0 + (int)(t % pp.w), y = pp.y0 + (int)(t / pp.w);
-        int slice = pixel_slice(pp, gv, x, y);
-        int strata = spp_strata(pp.spp);
-        PtSink sink{nullptr, 0, 0, mk(0.f, 0.f, 0.f)};
-        const PtCold cold = pt_cold(s_stack);
-        PtLane ps;
-        ps.stage = kPsSample;
-        ps.smp = 0;
-        PtQuery q;
-        while (pt_advance(sv, cam.v, pp, gv, cold, x, y, slice, strata, pp.spp, ps, q, sink, nclosest, nshadow))
-            ps.qtri = closest_hit4(sv, q.r, pp.tmin, q.tmax, astk, ps.qt, ps.qb, ps.qg, st, q.any);
+    const int64_t npx = (int64_t)pp.w * pp.h;
+    const bool own = t < npx;
+    const int x = own ? pp.x0 + (int)(t % pp.w) : 0, y = own ? pp.y0 + (int)(t / pp.w) : 0;
+    const int slice = own ? pixel_slice(pp, gv, x, y) : 0;
+    const int strata = spp_strata(pp.spp);
+    PtSink sink{nullptr, 0, 0, mk(0.f, 0.f, 0.f)};
+    const PtCold cold = pt_cold(s_stack);
+    PtLane ps;
+    ps.stage = kPsSample;
+    ps.smp = 0;
+    PtQuery q;
+    Trav4 tv;
+    tv.best = -1;
+    bool busy = own, trav = false;
+    for (;;) {
+        if (busy && !trav) {
+            ps.qtri = tv.best;
+            busy = pt_advance(sv, cam.v, pp, gv, cold, x, y, slice, strata, pp.spp, ps, q, sink, nclosest, nshadow);
+            if (busy) trav = trav4_begin(sv, q.r, pp.tmin, q.tmax, q.any, tv, ps.qt, ps.qb, ps.qg, st);
+        }
+        if (__ballot(busy) == 0ull) break;
+        trav_phase(sv, q.r, pp.tmin, tv, astk, ps, st, busy, trav);
+    }
+    if (own) {
         const v3 acc = sink.acc;
         int64_t o = (int64_t)y * pp.width + x;
         out[3 * o] = acc.x / (float)pp.spp;

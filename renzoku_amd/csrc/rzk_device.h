@@ -99,9 +99,38 @@ struct AnyStack {
         else spill[(int64_t)(sp - kLdsStack) * spitch + gid] = v;
         ++sp;
     }
+    // the two reads name their address spaces, so they cannot be merged into
+    // one generic-address read
     __device__ __forceinline__ int pop(int& sp) {
         --sp;
-        return sp < kLdsStack ? lds[sp * pitch] : spill[(int64_t)(sp - kLdsStack) * spitch + gid];
+        int v;
+        if (sp < kLdsStack) v = ((__attribute__((address_space(3))) const int*)lds)[sp * pitch];
+        else v = ((__attribute__((address_space(1))) const int*)spill)[(int64_t)(sp - kLdsStack) * spitch + gid];
+        return v;
+    }
+    // Two ints as one 8-byte entry, for walks that push only pairs (sp stays
+    // even): pair k of a lane sits at 8-byte word k * pitch of its LDS column
+    // pair (k < kLdsStack / 2), deeper ones at word (k - kLdsStack / 2) *
+    // spitch + gid of the spill area; the same footprint as 2k single entries.
+    __device__ __forceinline__ void push2(int& sp, int a, int b) {
+        const unsigned long long v = (unsigned long long)(unsigned)a | ((unsigned long long)(unsigned)b << 32);
+        if (sp < kLdsStack) lds2()[(sp >> 1) * pitch] = v;
+        else reinterpret_cast<unsigned long long*>(spill)[(int64_t)((sp - kLdsStack) >> 1) * spitch + gid] = v;
+        sp += 2;
+    }
+    __device__ __forceinline__ int2 pop2(int& sp) {
+        sp -= 2;
+        unsigned long long v;
+        if (sp < kLdsStack) {
+            v = ((__attribute__((address_space(3))) const unsigned long long*)lds2())[(sp >> 1) * pitch];
+        } else {
+            v = ((__attribute__((address_space(1))) const unsigned long long*)spill)[(int64_t)((sp - kLdsStack) >> 1) *
+                                                                                    spitch + gid];
+        }
+        return make_int2((int)(unsigned)v, (int)(unsigned)(v >> 32));
+    }
+    __device__ __forceinline__ unsigned long long* lds2() const {
+        return reinterpret_cast<unsigned long long*>(lds - threadIdx.x) + threadIdx.x;
     }
 };
 
@@ -169,94 +198,138 @@ __device__ __forceinline__ bool any_hit4(const SceneView& sv, const Ray& r, floa
     }
 }
 
-// Closest hit over the 4-wide collapse, nearest child first.  Leaf children
-// are tested as soon as their box passes (shrinking tmax); interior children
-// that pass are visited in order of entry distance, the others pushed as
-// (node, entry) pairs and dropped at pop time when their entry lies beyond
-// the current tmax.  The hit is the same triangle, t, beta and gamma as the
-// reference-order traversal whenever the closest triangle is unique (the
-// triangle test is the same arithmetic and every triangle whose box chain
-// passes at the final tmax is tested); on exact ties in t the winner may
-// differ.  The stack holds up to 2 * sv.stack4 ints.  any: return at the
-// first triangle hit (its index; t, beta, gamma of that hit).  tmax then never
-// shrinks, so every node whose box passes is visited until a hit: the hit /
-// no-hit answer is any_hit4's.
-__device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, float tmin, float tmax,
-                                            AnyStack& stk, float& t_out, float& b_out, float& g_out,
-                                            TravStats& st, bool any = false) {
+// Closest hit over the 4-wide collapse, nearest child first, one node per
+// step so that a lane can stop between nodes and resume later (the path
+// kernels interleave the walks of a wave's lanes with shading this way).
+// A step slab-tests the node's children at the tmax it starts with, then
+// tests the triangle children that passed, in slot order, shrinking tmax at
+// each hit; interior children that passed are visited in order of entry
+// distance, the others pushed as (node, entry) pairs, a child only when its
+// entry lies within the shrunk tmax, and dropped at pop time when their entry
+// lies beyond the tmax of that time.  The hit is the same triangle, t, beta
+// and gamma as the reference-order traversal whenever the closest triangle is
+// unique (the triangle test is the same arithmetic and every triangle whose
+// box chain passes at the final tmax is tested); on exact ties in t the winner
+// may differ.  Testing a node's triangles after all of its boxes lets a wave
+// run one triangle test per passing leaf of its busiest lane rather than one
+// per child slot that any lane needs.  The stack holds up to 2 * sv.stack4
+// ints.  any: stop at the first triangle hit (its index; t, beta, gamma of
+// that hit).  tmax then never shrinks, so every node whose box passes is
+// visited until a hit: the hit / no-hit answer is any_hit4's.
+//
+// trav4_begin does the root's part (false when the query is already answered);
+// trav4_step visits one node and returns false once the walk is over.  The
+// answer is then in best (-1: none) and, after a hit, in t_out, b_out, g_out
+// (left alone without one).
+struct Trav4 {
+    int node, sp, best;
+    float tmax;
+    bool any;
+};
+
+__device__ __forceinline__ bool trav4_begin(const SceneView& sv, const Ray& r, float tmin, float tmax, bool any,
+                                            Trav4& T, float& t_out, float& b_out, float& g_out, TravStats& st) {
+    T.best = -1;
+    T.sp = 0;
+    T.tmax = tmax;
+    T.any = any;
     if (!box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]), mk(sv.rbmax[0], sv.rbmax[1], sv.rbmax[2]),
                  r, tmin, tmax))
-        return -1;
-    int best = -1;
-    int node = sv.root4_ref;
-    float t, b, g;
-    if (node < 0) {
-        const TriGpu* tp = sv.tris4 + (~node);
+        return false;
+    T.node = sv.root4_ref;
+    if (T.node < 0) {
+        const TriGpu* tp = sv.tris4 + (~T.node);
         float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
+        float t, b, g;
         st.tris++;
         if (tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g)) {
             t_out = t; b_out = b; g_out = g;
-            best = __float_as_int(p0.w);
+            T.best = __float_as_int(p0.w);
         }
-        return best;
+        return false;
     }
-    int sp = 0;
-    for (;;) {
-        const Bvh4NodeGpu* np = sv.nodes4 + node;
-        float4 lx = np->lox, ly = np->loy, lz = np->loz, hx = np->hix, hy = np->hiy, hz = np->hiz;
-        int4 rf = np->ref;
-        st.nodes++;
-        const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w};
-        const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-        const float hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-        const int rfa[4] = {rf.x, rf.y, rf.z, rf.w};
-        float cd[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-        int cr[4] = {-1, -1, -1, -1};
+    return true;
+}
+
+__device__ __forceinline__ bool trav4_step(const SceneView& sv, const Ray& r, float tmin, Trav4& T, AnyStack& stk,
+                                           float& t_out, float& b_out, float& g_out, TravStats& st) {
+    const Bvh4NodeGpu* np = sv.nodes4 + T.node;
+    float4 lx = np->lox, ly = np->loy, lz = np->loz, hx = np->hix, hy = np->hiy, hz = np->hiz;
+    int4 rf = np->ref;
+    st.nodes++;
+    const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w};
+    const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+    const int rfa[4] = {rf.x, rf.y, rf.z, rf.w};
+    // every child slot is tested (an empty slot's box values are never used),
+    // so the node's loads all issue at once
+    float cd[4];
+    int cr[4];
+    unsigned leaves = 0;                   // slots whose triangle child passed
+    float tmax = T.tmax;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            int ref = rfa[c];
-            if (ref == kEmptyRef) continue;
-            float te;
-            if (!box_enter(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax, te)) continue;
-            if (ref < 0) {
-                const TriGpu* tp = sv.tris4 + (~ref);
-                float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
-                st.tris++;
-                if (tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b,
-                            g)) {
-                    tmax = t; t_out = t; b_out = b; g_out = g;
-                    best = __float_as_int(p0.w);
-                    if (any) return best;
-                }
-            } else {
-                cd[c] = te;
-                cr[c] = ref;
-            }
+    for (int c = 0; c < 4; ++c) {
+        const int ref = rfa[c];
+        float te;
+        const bool pass = box_enter(mk(lxa[c], lya[c], lza[c]), mk(hxa[c], hya[c], hza[c]), r, tmin, tmax, te) &&
+                          ref != kEmptyRef;
+        const bool inner = pass && ref >= 0;
+        leaves |= (pass && ref < 0) ? 1u << c : 0u;
+        cd[c] = inner ? te : INFINITY;
+        cr[c] = inner ? ref : -1;
+    }
+    while (leaves) {
+        const int c = __builtin_ctz(leaves);
+        leaves &= leaves - 1u;
+        int ref = rfa[0];
+        ref = c == 1 ? rfa[1] : ref;
+        ref = c == 2 ? rfa[2] : ref;
+        ref = c == 3 ? rfa[3] : ref;
+        const TriGpu* tp = sv.tris4 + (~ref);
+        float4 p0 = tp->p0, e1 = tp->e1, e2 = tp->e2;
+        float t, b, g;
+        st.tris++;
+        if (tri_hit(mk(p0.x, p0.y, p0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), r, tmin, tmax, t, b, g)) {
+            tmax = t; t_out = t; b_out = b; g_out = g;
+            T.best = __float_as_int(p0.w);
+            if (T.any) { T.tmax = tmax; return false; }
         }
-        // ascending entry distance (5-exchange network); empty slots carry +inf
+    }
+    T.tmax = tmax;
+    // ascending entry distance (5-exchange network); empty slots carry +inf
 #define RZK_CX(i, j)                                                              \
-        if (cd[j] < cd[i]) {                                                      \
-            float td = cd[i]; cd[i] = cd[j]; cd[j] = td;                          \
-            int tr = cr[i]; cr[i] = cr[j]; cr[j] = tr;                            \
-        }
-        RZK_CX(0, 1) RZK_CX(2, 3) RZK_CX(0, 2) RZK_CX(1, 3) RZK_CX(1, 2)
-#undef RZK_CX
-        node = -1;
-#pragma unroll
-        for (int c = 3; c >= 1; --c) {
-            if (cr[c] >= 0 && cd[c] <= tmax) {
-                stk.push(sp, cr[c]);
-                stk.push(sp, __float_as_int(cd[c]));
-            }
-        }
-        if (cr[0] >= 0 && cd[0] <= tmax) node = cr[0];
-        while (node < 0) {
-            if (sp == 0) return best;
-            float d = __int_as_float(stk.pop(sp));
-            int ref = stk.pop(sp);
-            if (d <= tmax) node = ref;
-        }
+    {                                                                             \
+        const bool sw = cd[j] < cd[i];                                            \
+        const float di = cd[i], dj = cd[j];                                       \
+        const int ri = cr[i], rj = cr[j];                                         \
+        cd[i] = sw ? dj : di; cd[j] = sw ? di : dj;                               \
+        cr[i] = sw ? rj : ri; cr[j] = sw ? ri : rj;                               \
     }
+    RZK_CX(0, 1) RZK_CX(2, 3) RZK_CX(0, 2) RZK_CX(1, 3) RZK_CX(1, 2)
+#undef RZK_CX
+    int node = -1;
+#pragma unroll
+    for (int c = 3; c >= 1; --c)
+        if (cr[c] >= 0 && cd[c] <= tmax) stk.push2(T.sp, cr[c], __float_as_int(cd[c]));
+    if (cr[0] >= 0 && cd[0] <= tmax) node = cr[0];
+    while (node < 0) {
+        if (T.sp == 0) return false;
+        const int2 e = stk.pop2(T.sp);
+        if (__int_as_float(e.y) <= tmax) node = e.x;
+    }
+    T.node = node;
+    return true;
+}
+
+// the whole walk at once
+__device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, float tmin, float tmax,
+                                            AnyStack& stk, float& t_out, float& b_out, float& g_out,
+                                            TravStats& st, bool any = false) {
+    Trav4 T;
+    if (trav4_begin(sv, r, tmin, tmax, any, T, t_out, b_out, g_out, st))
+        while (trav4_step(sv, r, tmin, T, stk, t_out, b_out, g_out, st)) {
+        }
+    return T.best;
 }
 
 // Wave-coherent any hit: the 64 lanes walk the 4-wide tree together.  The
