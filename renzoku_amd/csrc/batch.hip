@@ -125,7 +125,7 @@ struct rzk_batch {
         }
         hipStream_t st = ctx->stream;
         RZK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        launch_batch(st, sv, hdr.p, grid(), cfg.block, std::max(1, ctx->num_cus) * 8);
+        launch_batch(st, sv, hdr.p, grid(), cfg.block, cfg.items_per_dequeue, std::max(1, ctx->num_cus) * 8);
         hipError_t le = hipGetLastError();
         hipGraph_t g = nullptr;
         hipError_t ee = hipStreamEndCapture(st, &g);

@@ -150,10 +150,12 @@ void launch_guided_pt_rays(rzk_ctx* ctx, const SceneView& sv, const rzk::PtParam
 void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv, const float* cam, const rzk::PtParams& pp,
                       const rzk::GuideView& gv, float* d_out, unsigned long long* d_work);
 // the three kernels of a batch on `stream` (captured into a graph by the caller)
+// (items_per_dequeue: the header's; it picks the persistent kernel)
 void launch_batch(hipStream_t stream, const SceneView& sv, const rzk::BatchHeader* d_hdr, int grid, int block,
-                  int reduce_grid);
+                  int items_per_dequeue, int reduce_grid);
 // the queue reset and the persistent kernel alone (items item_lo .. n_items - 1)
-void launch_batch_items(hipStream_t stream, const SceneView& sv, const rzk::BatchHeader* d_hdr, int grid, int block);
+void launch_batch_items(hipStream_t stream, const SceneView& sv, const rzk::BatchHeader* d_hdr, int grid, int block,
+                        int items_per_dequeue);
 // one iteration of the wavefront engine (queues of parity `in` consumed,
 // parity in ^ 1 produced): extend + next-event rays, shadow rays, next-event
 // resolution, bounce shading, path regeneration

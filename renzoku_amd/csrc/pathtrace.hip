@@ -701,20 +701,30 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     TravStats st;
     unsigned long long nclosest = 0, nshadow = 0;
-    if (i < n) {
-        const float* rp = rays + 6 * i;
-        int x = pix[2 * i], y = pix[2 * i + 1];
-        const int sm = smp[i];
-        PtSink sink{out + 3 * i, 3, sm, mk(0.f, 0.f, 0.f)};
-        const PtCold cold = pt_cold(s_stack);
-        PtLane ps;
-        ps.stage = kPsSample;
-        ps.smp = sm;
-        PtQuery q;
-        q.r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
-        while (pt_advance(sv, nullptr, pp, gv, cold, x, y, pixel_slice(pp, gv, x, y), 0, sm + 1, ps, q, sink, nclosest,
-                          nshadow))
-            ps.qtri = closest_hit4(sv, q.r, pp.tmin, q.tmax, astk, ps.qt, ps.qb, ps.qg, st, q.any);
+    const bool own = i < n;
+    const int64_t ii = own ? i : 0;
+    const float* rp = rays + 6 * ii;
+    const int x = own ? pix[2 * ii] : 0, y = own ? pix[2 * ii + 1] : 0;
+    const int sm = own ? smp[ii] : 0;
+    const int slice = own ? pixel_slice(pp, gv, x, y) : 0;
+    PtSink sink{out + 3 * ii, 3, sm, mk(0.f, 0.f, 0.f)};
+    const PtCold cold = pt_cold(s_stack);
+    PtLane ps;
+    ps.stage = kPsSample;
+    ps.smp = sm;
+    PtQuery q;
+    if (own) q.r = ray_from_od(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]));
+    Trav4 tv;
+    tv.best = -1;
+    bool busy = own, trav = false;
+    for (;;) {
+        if (busy && !trav) {
+            ps.qtri = tv.best;
+            busy = pt_advance(sv, nullptr, pp, gv, cold, x, y, slice, 0, sm + 1, ps, q, sink, nclosest, nshadow);
+            if (busy) trav = trav4_begin(sv, q.r, pp.tmin, q.tmax, q.any, tv, ps.qt, ps.qb, ps.qg, st);
+        }
+        if (__ballot(busy) == 0ull) break;
+        trav_phase(sv, q.r, pp.tmin, tv, astk, ps, st, busy, trav);
     }
 }
 
@@ -738,36 +748,106 @@ __global__ void k_batch_reset(const BatchHeader* __restrict__ hdr) {
     }
 }
 
-// item `item` for this lane: its job, pixel and samples; false when the
-// item's pixel block has no pixel for the lane
-__device__ __forceinline__ bool batch_item(const BatchHeader& hdr, unsigned long long item, int lane,
-                                           const JobGpu*& J, int& x, int& y, int& slice, int& s1, PtLane& ps,
-                                           PtSink& sink) {
+// the job of work item `item`: the last job (of the ring window job_lo ..
+// n_jobs - 1) whose first item is at most `item`
+__device__ __forceinline__ int64_t batch_job(const BatchHeader& hdr, unsigned long long item) {
     int64_t lo = hdr.job_lo, hi = hdr.n_jobs - 1;
     while (lo < hi) {
         int64_t mid = (lo + hi + 1) >> 1;
         if ((unsigned long long)hdr.jobs[mid & hdr.job_mask].item_first <= item) lo = mid;
         else hi = mid - 1;
     }
-    J = hdr.jobs + (lo & hdr.job_mask);
+    return lo & hdr.job_mask;
+}
+
+// the pixel and samples of item `item` (of job J, parameters pp and gv) for
+// this lane; false when the item's pixel block has no pixel for the lane
+__device__ __forceinline__ bool batch_lane(const BatchHeader& hdr, const JobGpu* J, const PtParams& pp,
+                                           const GuideView& gv, unsigned long long item, int lane, int& x, int& y,
+                                           int& slice, int& s1, PtLane& ps, PtSink& sink) {
     const int64_t local = (int64_t)(item - (unsigned long long)J->item_first);
     const int nsb = J->samp_blocks;
     const int64_t pb = local / nsb;
     const int sb = (int)(local - pb * nsb);
     const int64_t p = pb * 64 + lane;
-    const int64_t npx = (int64_t)J->pp.w * J->pp.h;
+    const int64_t npx = (int64_t)pp.w * pp.h;
     if (p >= npx) return false;
-    x = J->pp.x0 + (int)(p % J->pp.w);
-    y = J->pp.y0 + (int)(p / J->pp.w);
-    slice = pixel_slice(J->pp, J->gv, x, y);
+    x = pp.x0 + (int)(p % pp.w);
+    y = pp.y0 + (int)(p / pp.w);
+    slice = pixel_slice(pp, gv, x, y);
     const int s0 = sb * J->samples_per_item;
-    s1 = min(J->pp.spp, s0 + J->samples_per_item);
+    s1 = min(pp.spp, s0 + J->samples_per_item);
     ps.stage = kPsSample;
     ps.smp = s0;
     sink.out = hdr.samples + J->sample_off + p * 3;
     sink.stride = npx * 3;
     sink.smp0 = 0;
     return true;
+}
+
+// item `item` for this lane: its job, pixel and samples; false when the
+// item's pixel block has no pixel for the lane
+__device__ __forceinline__ bool batch_item(const BatchHeader& hdr, unsigned long long item, int lane,
+                                           const JobGpu*& J, int& x, int& y, int& slice, int& s1, PtLane& ps,
+                                           PtSink& sink) {
+    J = hdr.jobs + batch_job(hdr, item);
+    return batch_lane(hdr, J, J->pp, J->gv, item, lane, x, y, slice, s1, ps, sink);
+}
+
+// a wave-uniform value held in scalar registers: each 32-bit word of src
+// read from the wave's first active lane (every lane holds the same value)
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T& src) {
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    T out;
+    const int* s = reinterpret_cast<const int*>(&src);
+    int* d = reinterpret_cast<int*>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = __builtin_amdgcn_readfirstlane(s[i]);
+    return out;
+}
+
+// One item per dequeue: the wave's lanes run the 64 pixels of one item, so
+// the item's job is the same for every lane and its parameters are held in
+// scalar registers.  Lanes walk and shade in phases as in k_guided_pt.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_pt_one(
+    SceneView sv, const BatchHeader* __restrict__ hdrp) {
+    extern __shared__ int s_stack[];
+    const BatchHeader hdr = *hdrp;
+    AnyStack astk = any_stack(sv, s_stack);
+    const PtCold cold = pt_cold(s_stack);
+    const int lane = threadIdx.x & 63;
+    TravStats st;
+    unsigned long long nclosest = 0, nshadow = 0;
+    const unsigned long long n_items = (unsigned long long)hdr.n_items;
+    for (;;) {
+        unsigned long long item = 0;
+        if (lane == 0) item = atomicAdd(hdr.counter, 1ull);
+        item = wave_bcast64(item);
+        if (item >= n_items) break;
+        const JobGpu* J = hdr.jobs + wave_uniform(batch_job(hdr, item));
+        const PtParams pp = wave_uniform(J->pp);
+        const GuideView gv = wave_uniform(J->gv);
+        const CamView cam = wave_uniform(J->cam);
+        const int strata = wave_uniform(J->strata);
+        int x = 0, y = 0, slice = -1, s1 = 0;
+        PtLane ps;
+        PtQuery q;
+        PtSink sink{nullptr, 0, 0, mk(0.f, 0.f, 0.f)};
+        bool busy = batch_lane(hdr, J, pp, gv, item, lane, x, y, slice, s1, ps, sink), trav = false;
+        Trav4 tv;
+        tv.best = -1;
+        for (;;) {
+            if (busy && !trav) {
+                ps.qtri = tv.best;
+                busy = pt_advance(sv, cam.v, pp, gv, cold, x, y, slice, strata, s1, ps, q, sink, nclosest, nshadow);
+                if (busy) trav = trav4_begin(sv, q.r, pp.tmin, q.tmax, q.any, tv, ps.qt, ps.qb, ps.qg, st);
+            }
+            if (__ballot(busy) == 0ull) break;
+            trav_phase(sv, q.r, pp.tmin, tv, astk, ps, st, busy, trav);
+        }
+    }
+    if (hdr.work) add_ray_counts(hdr.work, nclosest, nshadow);
 }
 
 // A wave takes items_per_dequeue items per atomic; each lane then runs its
@@ -799,21 +879,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         PtLane ps;
         PtQuery q;
         PtSink sink{nullptr, 0, 0, mk(0.f, 0.f, 0.f)};
+        Trav4 tv;
+        tv.best = -1;
+        float tmin = 0.f;
+        bool busy = true, trav = false;
         for (;;) {
-            bool need = false;
-            while (item < end) {
-                if (!have) {
-                    have = batch_item(hdr, item, lane, J, x, y, slice, s1, ps, sink);
-                    if (!have) { ++item; continue; }
+            if (busy && !trav) {
+                ps.qtri = tv.best;
+                busy = false;
+                while (item < end) {
+                    if (!have) {
+                        have = batch_item(hdr, item, lane, J, x, y, slice, s1, ps, sink);
+                        if (!have) { ++item; continue; }
+                    }
+                    busy = pt_advance(sv, J->cam.v, J->pp, J->gv, cold, x, y, slice, J->strata, s1, ps, q, sink,
+                                      nclosest, nshadow);
+                    if (busy) break;
+                    have = false;
+                    ++item;
                 }
-                need = pt_advance(sv, J->cam.v, J->pp, J->gv, cold, x, y, slice, J->strata, s1, ps, q, sink,
-                                  nclosest, nshadow);
-                if (need) break;
-                have = false;
-                ++item;
+                if (busy) {
+                    tmin = J->pp.tmin;
+                    trav = trav4_begin(sv, q.r, tmin, q.tmax, q.any, tv, ps.qt, ps.qb, ps.qg, st);
+                }
             }
-            if (!need) break;
-            ps.qtri = closest_hit4(sv, q.r, J->pp.tmin, q.tmax, astk, ps.qt, ps.qb, ps.qg, st, q.any);
+            if (__ballot(busy) == 0ull) break;
+            trav_phase(sv, q.r, tmin, tv, astk, ps, st, busy, trav);
         }
     }
     if (hdr.work) add_ray_counts(hdr.work, nclosest, nshadow);
@@ -1279,15 +1370,17 @@ void launch_batch_reduce(hipStream_t stream, const BatchHeader* d_hdr, int grid)
 }
 
 
-void launch_batch_items(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block) {
+void launch_batch_items(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
+                        int items_per_dequeue) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, d_hdr);
     size_t lds = sizeof(int) * (size_t)block * ((size_t)kLdsStack + kPtCold);
-    hipLaunchKernelGGL(k_batch_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
+    if (items_per_dequeue == 1) hipLaunchKernelGGL(k_batch_pt_one, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
+    else hipLaunchKernelGGL(k_batch_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
 }
 
 void launch_batch(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
-                  int reduce_grid) {
-    launch_batch_items(stream, sv, d_hdr, grid, block);
+                  int items_per_dequeue, int reduce_grid) {
+    launch_batch_items(stream, sv, d_hdr, grid, block, items_per_dequeue);
     hipLaunchKernelGGL(k_batch_reduce, dim3(reduce_grid), dim3(256), 0, stream, d_hdr);
 }
 

@@ -371,7 +371,7 @@ struct rzk_server {
                     h.item_lo = a;
                     RZK_HIP(hipMemcpyAsync(d_hdr + k, &h, sizeof(BatchHeader), hipMemcpyHostToDevice, st));
                     RZK_HIP(hipEventRecord(ev_t0[k], st));
-                    launch_batch_items(st, sv, d_hdr + k, grid, block);
+                    launch_batch_items(st, sv, d_hdr + k, grid, block, h.items_per_dequeue);
                     RZK_HIP(hipGetLastError());
                     RZK_HIP(hipEventRecord(ev_step[k], st));
                     step_end[k] = e;
