@@ -287,6 +287,8 @@ enum : int {
     kPsShadow,          // result: the next event's shadow segment
     kPsNeeHit,          // result: the next event's BSDF ray
     kPsContHit,         // result: the continuation ray
+    kPsNeeSample,       // next: the next event's BSDF sample (no query pending)
+    kPsContinue,        // next: the continuation's direction sample (no query pending)
 };
 
 // LDS columns of one lane (a float each, kPtCold of them)
@@ -380,41 +382,19 @@ __device__ __forceinline__ void pt_nee_end(const SceneView& sv, const PtCold& c,
     s.L = s.L + s.T * mk(Lo.x / sp, Lo.y / sp, Lo.z / sp);
 }
 
-// next_event's BSDF sample (numbers 5..7): a query when its pdf is positive
-__device__ __forceinline__ bool pt_nee_bsdf(const SceneView& sv, const PtParams& pp, const PtCold& c, uint32_t pixel,
-                                            PtLane& s, PtQuery& q, unsigned long long& nclosest) {
-    const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
-    const Onb b = c.onb();
-    const v3 p = c.get3(kPcRc), wo = c.get3(kPcRc + 9);
-    uint32_t u[4];
-    pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 1u, u);
-    v3 wi2;
-    float dpdf2;
-    bsdf_sample(mat, b, wo, &u[1], wi2, dpdf2);
-    if (!(dpdf2 > 0.0f)) return false;
-    c.set3(kPcWi2, wi2);
-    c.set(kPcDpdf2, dpdf2);
-    nclosest++;
-    q.r = ray_ctor(p, wi2);
-    q.tmax = 1e6f;
-    q.any = false;
-    s.stage = kPsNeeHit;
-    return true;
-}
-
 // next_event's start (numbers 1..4): the light, its sample and the shadow
-// query; false when the next event ends without a query (no light chosen, or
-// neither sample has a positive pdf)
-__device__ __forceinline__ bool pt_nee_begin(const SceneView& sv, const PtParams& pp, const PtCold& c, uint32_t pixel,
-                                             const uint32_t* u0, PtLane& s, PtQuery& q, unsigned long long& nshadow,
-                                             unsigned long long& nclosest) {
+// query.  Returns 1 with the shadow query in q, 2 when the light sample has
+// no positive pdf (the BSDF sample comes next), 0 when no light is chosen
+// (the next event adds nothing and ends).
+__device__ __forceinline__ int pt_nee_begin(const SceneView& sv, const PtParams& pp, const PtCold& c, uint32_t pixel,
+                                            const uint32_t* u0, PtLane& s, PtQuery& q, unsigned long long& nshadow) {
     float ul = u24(u0[1]);
     int k = -1;
     for (int i = 0; i < sv.nlights; ++i)
         if (ul < sv.lights[i].sel_cdf) { k = i; break; }
     if (k < 0 || sv.lights[k].count <= 0) {
         s.L = s.L + s.T * mk(0.f, 0.f, 0.f);
-        return false;
+        return 0;
     }
     s.k = k;
     const LightGpu& Lg = sv.lights[k];
@@ -443,11 +423,9 @@ __device__ __forceinline__ bool pt_nee_begin(const SceneView& sv, const PtParams
         q.tmax = length(ql - p) - pp.tmin;
         q.any = true;
         s.stage = kPsShadow;
-        return true;
+        return 1;
     }
-    if (pt_nee_bsdf(sv, pp, c, pixel, s, q, nclosest)) return true;
-    pt_nee_end(sv, c, s);
-    return false;
+    return 2;
 }
 
 // Runs lane s from its last query's result to its next query (true, query in
@@ -503,8 +481,7 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
                     c.set3(kPcLo, c.get3(kPcLo) + (((w1 * G) / lpdf) * Le) * brdf);
                 }
             }
-            if (pt_nee_bsdf(sv, pp, c, pixel, s, q, nclosest)) return true;
-            pt_nee_end(sv, c, s);
+            s.stage = kPsNeeSample;
             break;
         }
         case kPsNeeHit: {
@@ -538,6 +515,7 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
                 }
             }
             pt_nee_end(sv, c, s);
+            s.stage = kPsContinue;
             break;
         }
         case kPsContHit: {
@@ -566,6 +544,9 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             at_hit = true;
             break;
         }
+        case kPsNeeSample:
+        case kPsContinue:
+            break;
         }
         if (at_hit) {
             // the bounce loop's head with the hit of q.r
@@ -585,32 +566,53 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             uint32_t u0[4];
             pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 0u, u0);
             s.brdf_sampling = u24(u0[0]) < 0.5f;
+            s.stage = kPsContinue;
             if (s.bounce > 1 || pp.direct) {
-                if (pt_nee_begin(sv, pp, c, pixel, u0, s, q, nshadow, nclosest)) return true;
+                const int nee = pt_nee_begin(sv, pp, c, pixel, u0, s, q, nshadow);
+                if (nee == 1) return true;
+                if (nee == 2) s.stage = kPsNeeSample;
             }
         }
-        // the continuation (numbers 8..11), after the next event
-        if (s.bounce >= pp.max_bounce) { pt_sample_done(s, acc); continue; }
+        // One sampling site for the next event's BSDF sample (numbers 5..7 of
+        // block 1, next_event's) and the continuation (numbers 8..11, after
+        // the next event), so a wave whose lanes want either runs it once.
+        if (s.stage == kPsContinue && s.bounce >= pp.max_bounce) { pt_sample_done(s, acc); continue; }
         {
+            const bool nee = s.stage == kPsNeeSample;
             const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
             const Onb b = c.onb();
             const v3 sn = c.get3(kPcRc + 6), wo = c.get3(kPcRc + 9), p = c.get3(kPcRc);
             const int gslice = s.bounce == 1 ? slice : -1;
-            uint32_t u2[4];
-            pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 2u, u2);
+            uint32_t u[4];
+            pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, nee ? 1u : 2u, u);
+            const uint32_t ub[3] = {nee ? u[1] : u[0], nee ? u[2] : u[1], nee ? u[3] : u[2]};
             v3 wi, brdf;
             float pdf;
-            if (s.brdf_sampling) brdf = bsdf_sample(mat, b, wo, u2, wi, pdf);
-            else brdf = guide_sample(gv, gslice, pp.res, mat, b, sn, wo, u2, wi, pdf);
-            if (!(pdf > 0.0f)) { pt_sample_done(s, acc); continue; }
-            c.set3(kPcWi, wi);
-            c.set3(kPcBrdf, brdf);
-            c.set(kPcPdf, pdf);
+            if (nee || s.brdf_sampling) brdf = bsdf_sample(mat, b, wo, ub, wi, pdf);
+            else brdf = guide_sample(gv, gslice, pp.res, mat, b, sn, wo, u, wi, pdf);
+            if (!(pdf > 0.0f)) {
+                if (nee) {
+                    pt_nee_end(sv, c, s);
+                    s.stage = kPsContinue;
+                } else {
+                    pt_sample_done(s, acc);
+                }
+                continue;
+            }
+            if (nee) {
+                c.set3(kPcWi2, wi);
+                c.set(kPcDpdf2, pdf);
+                s.stage = kPsNeeHit;
+            } else {
+                c.set3(kPcWi, wi);
+                c.set3(kPcBrdf, brdf);
+                c.set(kPcPdf, pdf);
+                s.stage = kPsContHit;
+            }
             nclosest++;
             q.r = ray_ctor(p, wi);
             q.tmax = 1e6f;
             q.any = false;
-            s.stage = kPsContHit;
             return true;
         }
     }
