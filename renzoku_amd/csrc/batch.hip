@@ -73,9 +73,11 @@ bool valid_config(const rzk_batch_config& c) {
            c.path_slots >= 4096 && c.path_slots <= (1 << 26);
 }
 
+// whether two configurations capture the same graph: the same launch shape
+// and the same persistent kernel (one item per dequeue has its own)
 bool same_launch(const rzk_batch_config& a, const rzk_batch_config& b) {
     return a.block == b.block && a.blocks_per_cu == b.blocks_per_cu && a.engine == b.engine &&
-           a.path_slots == b.path_slots;
+           a.path_slots == b.path_slots && (a.items_per_dequeue == 1) == (b.items_per_dequeue == 1);
 }
 
 }  // namespace
