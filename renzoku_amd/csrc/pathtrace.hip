@@ -127,40 +127,61 @@ __device__ __forceinline__ float guide_pdf(const GuideView& gv, int slice, int r
     return map_pdf(gv, slice, res, world_to_local(b, wi));
 }
 
-// first i in [0, n) with u < cdf[i], -1 when none (cdf non-decreasing, or all NaN)
+#ifndef RZK_SCAN_CHUNK
+#define RZK_SCAN_CHUNK 16
+#endif
+
+// first i in [0, n) with u < cdf[i], -1 when none (cdf non-decreasing, or all
+// NaN).  The predicate !(u < cdf[i]) holds on a prefix, so the answer is its
+// length: entries are read kScanChunk at a time, all loads of a chunk in
+// flight together, instead of one dependent load per halving step.
+constexpr int kScanChunk = RZK_SCAN_CHUNK;
 __device__ __forceinline__ int first_above(const float* cdf, int n, float u) {
-    int lo = 0, hi = n;                 // answer in [lo, hi]; hi == n means none
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (u < cdf[mid]) hi = mid;
-        else lo = mid + 1;
+    int cnt = 0;
+    for (int base = 0; base < n; base += kScanChunk) {
+        float v[kScanChunk];
+#pragma unroll
+        for (int i = 0; i < kScanChunk; ++i) v[i] = base + i < n ? cdf[base + i] : INFINITY;
+        int c = 0;
+#pragma unroll
+        for (int i = 0; i < kScanChunk; ++i) c += (u < v[i]) ? 0 : 1;
+        cnt += c;
+        if (c < kScanChunk) break;
     }
-    return lo < n ? lo : -1;
+    return cnt < n ? cnt : -1;
 }
 
-// LightSlice::sample -> multi_slice_sample: sample_map of the slice (receiver
-// frame, same_local_wi) or the uniform-hemisphere fallback; pdf re-evaluated
-// by multi_slice_pdf; returns the receiver BRDF for wi
-__device__ __forceinline__ v3 guide_sample(const GuideView& gv, int slice, int res, const float* mat, const Onb& b, v3 sn, v3 wo,
-                           const uint32_t* u, v3& wi, float& pdf) {
+// The direction step of LightSlice::sample -> multi_slice_sample: sample_map
+// of the slice (receiver frame, same_local_wi) or the uniform-hemisphere
+// fallback.  False (wi = 0) when the slice's map has no cell to pick.
+__device__ __forceinline__ bool guide_sample_dir(const GuideView& gv, int slice, int res, const Onb& b,
+                                                 const uint32_t* u, v3& wi) {
     if (slice < 0) {
         float phi = kTwoPi * u24(u[0]);
         float theta = xacos(u24(u[1]));
         wi = unit(spherical_to_world(b, theta, phi));
-    } else {
-        // DiscretePdf2D::sample (pdf.cc:171-186): row by the row cdf, column by that row's cdf
-        // the first index with u < cdf[i]; the predicate is monotone in i
-        // (cumulative sums, the last row entry forced to 1 > u), so a binary
-        // search finds the index the reference's linear scan finds
-        float ur = u24(u[0]), uc = u24(u[1]);
-        const float* rcdf = gv.row_cdf + (int64_t)slice * res;
-        int row = first_above(rcdf, res, ur), col = -1;
-        if (row >= 0) col = first_above(gv.col_cdf + ((int64_t)slice * res + row) * res, res, uc);
-        if (row < 0 || col < 0) { wi = mk(0.f, 0.f, 0.f); pdf = 0.f; return mk(0.f, 0.f, 0.f); }
-        float sx = ((float)col + u24(u[2])) / (float)res;
-        float sy = ((float)row + u24(u[3])) / (float)res;
-        wi = local_to_world(b, square_to_direction(sx, sy));
+        return true;
     }
+    // DiscretePdf2D::sample (pdf.cc:171-186): row by the row cdf, column by that row's cdf
+    // the first index with u < cdf[i]; the predicate is monotone in i
+    // (cumulative sums, the last row entry forced to 1 > u), so first_above
+    // finds the index the reference's linear scan finds
+    float ur = u24(u[0]), uc = u24(u[1]);
+    const float* rcdf = gv.row_cdf + (int64_t)slice * res;
+    int row = first_above(rcdf, res, ur), col = -1;
+    if (row >= 0) col = first_above(gv.col_cdf + ((int64_t)slice * res + row) * res, res, uc);
+    if (row < 0 || col < 0) { wi = mk(0.f, 0.f, 0.f); return false; }
+    float sx = ((float)col + u24(u[2])) / (float)res;
+    float sy = ((float)row + u24(u[3])) / (float)res;
+    wi = local_to_world(b, square_to_direction(sx, sy));
+    return true;
+}
+
+// LightSlice::sample -> multi_slice_sample: the direction, its pdf
+// re-evaluated by multi_slice_pdf; returns the receiver BRDF for wi
+__device__ __forceinline__ v3 guide_sample(const GuideView& gv, int slice, int res, const float* mat, const Onb& b, v3 sn, v3 wo,
+                           const uint32_t* u, v3& wi, float& pdf) {
+    if (!guide_sample_dir(gv, slice, res, b, u, wi)) { pdf = 0.f; return mk(0.f, 0.f, 0.f); }
     pdf = guide_pdf(gv, slice, res, b, sn, wi);
     return bsdf_eval(mat, b, wo, wi);
 }
@@ -300,9 +321,11 @@ enum : int {
     kPcLpdf = kPcQn + 3,
     kPcWi2 = kPcLpdf + 1,               //   BSDF direction (3), pdf
     kPcDpdf2 = kPcWi2 + 3,
+    kPcBrdf2 = kPcQ,                    //   BSDF sample's BRDF (3; the light sample is read by then)
     kPcWi = kPcLo,                      // continuation (after the next event): direction (3),
-    kPcBrdf = kPcWi + 3,                //   BRDF (3), pdf
+    kPcBrdf = kPcWi + 3,                //   BRDF (3), pdf, the other technique's pdf
     kPcPdf = kPcBrdf + 3,
+    kPcPdfO = kPcPdf + 1,
     kPtCold = kPcDpdf2 + 1,
 };
 
@@ -491,9 +514,7 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
                 int4 sf = sv.surf[__float_as_int(nf.w)];
                 if (sf.y == s.k) {
                     const LightGpu& Lg = sv.lights[s.k];
-                    const v3 p = c.get3(kPcRc), pn = c.get3(kPcRc + 6), wo = c.get3(kPcRc + 9);
-                    const Onb b = c.onb();
-                    const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
+                    const v3 p = c.get3(kPcRc), pn = c.get3(kPcRc + 6);
                     const v3 wi = c.get3(kPcWi2);
                     const float dpdf = c.get(kPcDpdf2);
                     v3 qh = p + s.qt * wi;
@@ -509,7 +530,7 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
                     v3 pq = qh - p;
                     float lpdf = pa * dot(pq, pq) / fabsf(dot(unit(p - qh), qhn));
                     float w2 = balance(dpdf, lpdf);
-                    v3 brdf = bsdf_eval(mat, b, wo, wi);
+                    v3 brdf = c.get3(kPcBrdf2);
                     float cosv = fabsf(dot(wi, pn));
                     c.set3(kPcLo, c.get3(kPcLo) + (((w2 * cosv) / dpdf) * Le) * brdf);
                 }
@@ -529,13 +550,10 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
                 pt_sample_done(s, acc);
                 continue;
             }
-            const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
-            const Onb b = c.onb();
-            const v3 sn = c.get3(kPcRc + 6), wo = c.get3(kPcRc + 9);
+            const v3 sn = c.get3(kPcRc + 6);
             const v3 wi = c.get3(kPcWi);
             const float pdf = c.get(kPcPdf);
-            const int gslice = s.bounce == 1 ? slice : -1;
-            float pdf_other = s.brdf_sampling ? guide_pdf(gv, gslice, pp.res, b, sn, wi) : bsdf_pdf(mat, b, wo, wi);
+            const float pdf_other = c.get(kPcPdfO);
             float weight = pdf / (pdf + pdf_other);
             float cosv = fabsf(dot(wi, sn));
             float sc = ((weight * cosv) / pdf) * 2.0f;
@@ -586,10 +604,25 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             uint32_t u[4];
             pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, nee ? 1u : 2u, u);
             const uint32_t ub[3] = {nee ? u[1] : u[0], nee ? u[2] : u[1], nee ? u[3] : u[2]};
-            v3 wi, brdf;
-            float pdf;
-            if (nee || s.brdf_sampling) brdf = bsdf_sample(mat, b, wo, ub, wi, pdf);
-            else brdf = guide_sample(gv, gslice, pp.res, mat, b, sn, wo, u, wi, pdf);
+            // the direction by the lane's technique, then, on one path for
+            // every lane, the values bsdf_sample / guide_sample and the
+            // continuation's hit would compute for it: the BSDF pdf, the
+            // guide pdf (continuations only) and the BRDF
+            const bool by_bsdf = nee || s.brdf_sampling;
+            v3 wi;
+            float lam_pdf = 0.0f;
+            int kind;
+            if (by_bsdf) kind = bsdf_sample_dir(mat, b, wo, ub, wi, lam_pdf);
+            else kind = guide_sample_dir(gv, gslice, pp.res, b, u, wi) ? 1 : 0;
+            float pdf = 0.0f, pdf_other = 0.0f;
+            v3 brdf = mk(0.f, 0.f, 0.f);
+            if (kind != 0) {
+                const float pdf_b = bsdf_pdf(mat, b, wo, wi);
+                const float pdf_g = nee ? 0.0f : guide_pdf(gv, gslice, pp.res, b, sn, wi);
+                brdf = bsdf_eval(mat, b, wo, wi);
+                pdf = !by_bsdf ? pdf_g : kind == 2 ? lam_pdf : pdf_b;
+                pdf_other = s.brdf_sampling ? pdf_g : pdf_b;
+            }
             if (!(pdf > 0.0f)) {
                 if (nee) {
                     pt_nee_end(sv, c, s);
@@ -602,11 +635,13 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             if (nee) {
                 c.set3(kPcWi2, wi);
                 c.set(kPcDpdf2, pdf);
+                c.set3(kPcBrdf2, brdf);
                 s.stage = kPsNeeHit;
             } else {
                 c.set3(kPcWi, wi);
-                c.set3(kPcBrdf, brdf);
+                c.set3(kPcBrdf, kind == 2 ? mk(mat[1], mat[2], mat[3]) * kInvPi : brdf);
                 c.set(kPcPdf, pdf);
+                c.set(kPcPdfO, pdf_other);
                 s.stage = kPsContHit;
             }
             nclosest++;

@@ -524,12 +524,15 @@ RZK_HD v3 cosine_dir(const Onb& b, float u0, float u1) {
     return unit(local_to_world(b, mk(x, y, z)));
 }
 
-// BSDF sampling (lambertian.cc:5-27, phong.cc:21-72, ward.cc:24-101): u[0] picks the
-// lobe, u[1], u[2] the direction.  Returns the BRDF value; pdf = 0 on rejection.
-RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3& wi, float& pdf) {
+// The direction step of BSDF sampling (lambertian.cc:5-27, phong.cc:21-72,
+// ward.cc:24-101): u[0] picks the lobe, u[1], u[2] the direction.  Returns 0
+// on rejection (wi = 0), 2 for a Lambertian material, whose pdf (lam_pdf, from
+// the local z) and BRDF (kd / pi) the sampler gives directly, else 1: the pdf
+// and BRDF are bsdf_pdf and bsdf_eval of wi.
+RZK_HD int bsdf_sample_dir(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3& wi, float& lam_pdf) {
     wi = mk(0.f, 0.f, 0.f);
-    pdf = 0.0f;
-    if (dot(wo, b.n) <= kZeroEps) return mk(0.f, 0.f, 0.f);
+    lam_pdf = 0.0f;
+    if (dot(wo, b.n) <= kZeroEps) return 0;
     int type = (int)m[0];
     if (type == kLambert) {
         float phi = kTwoPi * u24(u[1]);
@@ -537,8 +540,8 @@ RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3
         float x = sin_theta * xcos(phi), y = sin_theta * xsin(phi);
         float z = sqrtf(1.0f - sin_theta * sin_theta);
         wi = unit(local_to_world(b, mk(x, y, z)));
-        pdf = z * kInvPi;
-        return mk(m[1], m[2], m[3]) * kInvPi;
+        lam_pdf = z * kInvPi;
+        return 2;
     }
     float rho_d, rho_s;
     lobe_weights(m, rho_d, rho_s);
@@ -551,7 +554,7 @@ RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3
         float alpha = xacos(xpow(u24(u[2]), 1.0f / (ne + 1.0f)));
         v3 wr = spherical_to_world(o, alpha, phi);
         float d = dot(wr, b.n);
-        if (d < 0.0f) return mk(0.f, 0.f, 0.f);
+        if (d < 0.0f) return 0;
         wi = b.n * (2.0f * d) - wr;
     } else {
         float ax = m[7], ay = m[8];
@@ -568,6 +571,20 @@ RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3
         v3 wh = mk(st * cp, st * sp, ct);
         v3 lwi = wh * (2.0f * dot(lwo, wh)) - lwo;
         wi = local_to_world(b, lwi);
+    }
+    return 1;
+}
+
+// BSDF sampling: the direction, then its pdf and BRDF.  Returns the BRDF
+// value; pdf = 0 on rejection.
+RZK_HD v3 bsdf_sample(const float* m, const Onb& b, v3 wo, const uint32_t* u, v3& wi, float& pdf) {
+    float lam_pdf;
+    const int kind = bsdf_sample_dir(m, b, wo, u, wi, lam_pdf);
+    pdf = 0.0f;
+    if (kind == 0) return mk(0.f, 0.f, 0.f);
+    if (kind == 2) {
+        pdf = lam_pdf;
+        return mk(m[1], m[2], m[3]) * kInvPi;
     }
     pdf = bsdf_pdf(m, b, wo, wi);
     return bsdf_eval(m, b, wo, wi);
