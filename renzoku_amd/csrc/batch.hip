@@ -116,7 +116,7 @@ struct rzk_batch {
         drop_graph();
         sv = scene_view(scene);
         int64_t threads = (int64_t)grid() * cfg.block;
-        int64_t extra = 2 * (int64_t)sv.stack4 - kLdsStack;   // closest_hit4 pairs
+        int64_t extra = 2 * (int64_t)sv.stack4 - kPtLdsStack;   // closest_hit4 pairs
         if (extra > 0) {
             spill.reserve((size_t)(extra * threads));
             sv.spill = spill.p;

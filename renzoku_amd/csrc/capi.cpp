@@ -115,8 +115,8 @@ void ctx_release(rzk_ctx* ctx) {
     delete ctx;
 }
 
-void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries) {
-    int64_t extra = (entries > 0 ? entries : sv.stack4) - kLdsStack;
+void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries, int lds_entries) {
+    int64_t extra = (entries > 0 ? entries : sv.stack4) - lds_entries;
     if (extra <= 0 || threads <= 0) { sv.spill = nullptr; sv.spill_pitch = 0; return; }
     sv.spill = static_cast<int*>(ctx_scratch(ctx, sizeof(int) * (size_t)extra * (size_t)threads));
     sv.spill_pitch = threads;

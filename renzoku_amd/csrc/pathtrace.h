@@ -9,6 +9,14 @@
 
 namespace rzk {
 
+// Traversal stack entries per lane kept in LDS by the path kernels (deeper
+// ones spill to HBM); with the LDS state columns it sets how many waves fit
+// on a compute unit.
+#ifndef RZK_PT_LDS_STACK
+#define RZK_PT_LDS_STACK 8
+#endif
+constexpr int kPtLdsStack = RZK_PT_LDS_STACK;
+
 struct PtParams {
     int width, height, spp, max_bounce, direct, guide_mode, tile, res, tiles_x, tiles_y;
     int x0, y0, w, h;              // rendered window of the image

@@ -18,6 +18,16 @@
 
 namespace rzk {
 
+using PtStack = AnyStackT<kPtLdsStack>;
+
+// Waves per SIMD the path kernels' registers are budgeted for (512 / waves
+// VGPRs each); the LDS columns (kPtLdsStack + kPtCold ints per lane) fit
+// that many.
+#ifndef RZK_PT_WAVES
+#define RZK_PT_WAVES 4
+#endif
+constexpr int kPtWaves = RZK_PT_WAVES;
+
 // AreaLight::query (area_light.cc:19-25 with AreaLight::pdf, area_light.h:120-131)
 __device__ __forceinline__ v3 light_query(const LightGpu& L, v3 n, v3 wo) {
     if (dot(wo, n) < kZeroEps) return mk(0.f, 0.f, 0.f);
@@ -30,7 +40,7 @@ __device__ __forceinline__ float balance(float a, float b) { return a / (a + b);
 // Scene::sample_light (DiscretePdf, strict <), then AreaLight::first_bounce:
 // light sampling + BSDF sampling combined with the balance heuristic.
 __device__ v3 next_event(const SceneView& sv, const float* rc, int m, const PathRng& rng, float tmin,
-                         AnyStack& astk, TravStats& st, unsigned long long& nclosest, unsigned long long& nshadow) {
+                         PtStack& astk, TravStats& st, unsigned long long& nclosest, unsigned long long& nshadow) {
     float ul = u24(rng.u[1]);
     int k = -1;
     for (int i = 0; i < sv.nlights; ++i)
@@ -224,7 +234,7 @@ __device__ __forceinline__ int spp_strata(int spp) {
 // unrolled into a throughput product (vpl.cc:934-1041).  The numbers of bounce
 // b come from Philox(seed, pixel, sample, b), so the value depends on nothing else.
 __device__ v3 trace_from(const SceneView& sv, const PtParams& pp, const GuideView& gv, Ray ray, int x, int y,
-                         int slice, int smp, AnyStack& astk, TravStats& st, unsigned long long& nclosest,
+                         int slice, int smp, PtStack& astk, TravStats& st, unsigned long long& nclosest,
                          unsigned long long& nshadow) {
     uint32_t pixel = (uint32_t)(y * pp.width + x);
     v3 L = mk(0.f, 0.f, 0.f), T = mk(1.f, 1.f, 1.f);
@@ -272,7 +282,7 @@ __device__ v3 trace_from(const SceneView& sv, const PtParams& pp, const GuideVie
 // One path sample of pixel (x, y): the camera ray through the sample's jittered
 // position (2-D strata when spp is a square), then trace_from.
 __device__ v3 trace_path(const SceneView& sv, const float* cam, const PtParams& pp, const GuideView& gv, int x,
-                         int y, int slice, int strata, int smp, AnyStack& astk, TravStats& st,
+                         int y, int slice, int strata, int smp, PtStack& astk, TravStats& st,
                          unsigned long long& nclosest, unsigned long long& nshadow) {
     uint32_t pixel = (uint32_t)(y * pp.width + x);
     PathRng r0 = path_rng(pp.seed, pixel, (uint32_t)smp, 0u);
@@ -314,8 +324,11 @@ enum : int {
 
 // LDS columns of one lane (a float each, kPtCold of them)
 enum : int {
-    kPcRc = 0,                          // receiver record, RZK_RECV_FLOATS
-    kPcLo = kPcRc + RZK_RECV_FLOATS,    // next event: partial sum (3)
+    kPcP = 0,                           // receiver: point (3), shading normal (3), wo (3),
+    kPcSn = kPcP + 3,                   //   frame u, v, n (9); the geometric normal is
+    kPcWo = kPcSn + 3,                  //   not read after the hit
+    kPcOnb = kPcWo + 3,
+    kPcLo = kPcOnb + 9,                 // next event: partial sum (3)
     kPcQ = kPcLo + 3,                   //   light sample point (3), normal (3), area pdf
     kPcQn = kPcQ + 3,
     kPcLpdf = kPcQn + 3,
@@ -337,16 +350,16 @@ struct PtCold {
     __device__ v3 get3(int f) const { return mk(get(f), get(f + 1), get(f + 2)); }
     __device__ void set3(int f, v3 v) const { set(f, v.x); set(f + 1, v.y); set(f + 2, v.z); }
     __device__ Onb onb() const {
-        Onb b; b.u = get3(kPcRc + 12); b.v = get3(kPcRc + 15); b.n = get3(kPcRc + 18);
+        Onb b; b.u = get3(kPcOnb); b.v = get3(kPcOnb + 3); b.n = get3(kPcOnb + 6);
         return b;
     }
 };
 
-// cold columns after the traversal stack's kLdsStack ints per lane
+// cold columns after the traversal stack's kPtLdsStack ints per lane
 __device__ __forceinline__ PtCold pt_cold(int* s_stack) {
     PtCold c;
     c.pitch = (int)blockDim.x;
-    c.col = reinterpret_cast<float*>(s_stack + kLdsStack * blockDim.x) + threadIdx.x;
+    c.col = reinterpret_cast<float*>(s_stack + kPtLdsStack * blockDim.x) + threadIdx.x;
     return c;
 }
 
@@ -421,7 +434,7 @@ __device__ __forceinline__ int pt_nee_begin(const SceneView& sv, const PtParams&
     }
     s.k = k;
     const LightGpu& Lg = sv.lights[k];
-    const v3 p = c.get3(kPcRc);
+    const v3 p = c.get3(kPcP);
     c.set3(kPcLo, mk(0.f, 0.f, 0.f));
     uint32_t u1[4];
     pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 1u, u1);
@@ -488,7 +501,7 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             break;
         case kPsShadow: {
             if (s.qtri < 0) {
-                const v3 p = c.get3(kPcRc), pn = c.get3(kPcRc + 6), wo = c.get3(kPcRc + 9);
+                const v3 p = c.get3(kPcP), pn = c.get3(kPcSn), wo = c.get3(kPcWo);
                 const Onb b = c.onb();
                 const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
                 const v3 ql = c.get3(kPcQ), qn = c.get3(kPcQn);
@@ -514,7 +527,7 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
                 int4 sf = sv.surf[__float_as_int(nf.w)];
                 if (sf.y == s.k) {
                     const LightGpu& Lg = sv.lights[s.k];
-                    const v3 p = c.get3(kPcRc), pn = c.get3(kPcRc + 6);
+                    const v3 p = c.get3(kPcP), pn = c.get3(kPcSn);
                     const v3 wi = c.get3(kPcWi2);
                     const float dpdf = c.get(kPcDpdf2);
                     v3 qh = p + s.qt * wi;
@@ -550,7 +563,7 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
                 pt_sample_done(s, acc);
                 continue;
             }
-            const v3 sn = c.get3(kPcRc + 6);
+            const v3 sn = c.get3(kPcSn);
             const v3 wi = c.get3(kPcWi);
             const float pdf = c.get(kPcPdf);
             const float pdf_other = c.get(kPcPdfO);
@@ -579,8 +592,11 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
                 continue;
             }
             if (s.m < 0) { pt_sample_done(s, acc); continue; }
+            c.set3(kPcP, ld3(rc));
+            c.set3(kPcSn, ld3(rc + 6));
+            c.set3(kPcWo, ld3(rc + 9));
 #pragma unroll
-            for (int i = 0; i < RZK_RECV_FLOATS; ++i) c.set(kPcRc + i, rc[i]);
+            for (int i = 0; i < 9; ++i) c.set(kPcOnb + i, rc[12 + i]);
             uint32_t u0[4];
             pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 0u, u0);
             s.brdf_sampling = u24(u0[0]) < 0.5f;
@@ -599,7 +615,7 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             const bool nee = s.stage == kPsNeeSample;
             const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
             const Onb b = c.onb();
-            const v3 sn = c.get3(kPcRc + 6), wo = c.get3(kPcRc + 9), p = c.get3(kPcRc);
+            const v3 sn = c.get3(kPcSn), wo = c.get3(kPcWo), p = c.get3(kPcP);
             const int gslice = s.bounce == 1 ? slice : -1;
             uint32_t u[4];
             pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, nee ? 1u : 2u, u);
@@ -671,7 +687,7 @@ constexpr int kTravRegen = RZK_TRAV_REGEN;
 // lane is waiting to shade.  Each query's visits are closest_hit4's, so the
 // answers do not depend on when the phase stops.
 __device__ __forceinline__ void trav_phase(const SceneView& sv, const Ray& r, float tmin, Trav4& tv,
-                                           AnyStack& astk, PtLane& ps, TravStats& st, bool busy, bool& trav) {
+                                           PtStack& astk, PtLane& ps, TravStats& st, bool busy, bool& trav) {
     for (;;) {
         const unsigned long long tm = __ballot(trav);
         if (tm == 0ull) break;
@@ -683,10 +699,10 @@ __device__ __forceinline__ void trav_phase(const SceneView& sv, const Ray& r, fl
 // One lane per pixel of the window, all its samples in order (the state
 // machine above), summed in sample order (MonteCarloIntegrator::integrate,
 // monte_carlo_integrator.cc:45-70)
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_guided_pt(SceneView sv, CamView cam, PtParams pp, GuideView gv,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kPtWaves))) void k_guided_pt(SceneView sv, CamView cam, PtParams pp, GuideView gv,
                                                    float* __restrict__ out, unsigned long long* __restrict__ work) {
     extern __shared__ int s_stack[];
-    AnyStack astk = any_stack(sv, s_stack);
+    PtStack astk = any_stack<kPtLdsStack>(sv, s_stack);
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     TravStats st;
     unsigned long long nclosest = 0, nshadow = 0;
@@ -729,12 +745,12 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k
 // One lane per given primary ray: ray i (a ray record) is sample smp[i] of pixel
 // (pix[2i], pix[2i+1]); out[3i..] is its radiance, unaveraged
 // (MonteCarloIntegrator::radiance(PrimaryRay&), monte_carlo_integrator.cc:40-43).
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_guided_pt_rays(SceneView sv, PtParams pp, GuideView gv,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kPtWaves))) void k_guided_pt_rays(SceneView sv, PtParams pp, GuideView gv,
                                                         const float* __restrict__ rays, const int* __restrict__ pix,
                                                         const int* __restrict__ smp, int64_t n,
                                                         float* __restrict__ out) {
     extern __shared__ int s_stack[];
-    AnyStack astk = any_stack(sv, s_stack);
+    PtStack astk = any_stack<kPtLdsStack>(sv, s_stack);
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     TravStats st;
     unsigned long long nclosest = 0, nshadow = 0;
@@ -847,11 +863,11 @@ __device__ __forceinline__ T wave_uniform(const T& src) {
 // One item per dequeue: the wave's lanes run the 64 pixels of one item, so
 // the item's job is the same for every lane and its parameters are held in
 // scalar registers.  Lanes walk and shade in phases as in k_guided_pt.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_pt_one(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPtWaves))) void k_batch_pt_one(
     SceneView sv, const BatchHeader* __restrict__ hdrp) {
     extern __shared__ int s_stack[];
     const BatchHeader hdr = *hdrp;
-    AnyStack astk = any_stack(sv, s_stack);
+    PtStack astk = any_stack<kPtLdsStack>(sv, s_stack);
     const PtCold cold = pt_cold(s_stack);
     const int lane = threadIdx.x & 63;
     TravStats st;
@@ -892,11 +908,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // so a lane whose paths end early goes on to its next item while the others
 // are still tracing.  Every sample's radiance goes to its slot of the job's
 // per-sample buffer.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_pt(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPtWaves))) void k_batch_pt(
     SceneView sv, const BatchHeader* __restrict__ hdrp) {
     extern __shared__ int s_stack[];
     const BatchHeader hdr = *hdrp;
-    AnyStack astk = any_stack(sv, s_stack);
+    PtStack astk = any_stack<kPtLdsStack>(sv, s_stack);
     const PtCold cold = pt_cold(s_stack);
     const int lane = threadIdx.x & 63;
     TravStats st;
@@ -1410,7 +1426,7 @@ void launch_batch_reduce(hipStream_t stream, const BatchHeader* d_hdr, int grid)
 void launch_batch_items(hipStream_t stream, const SceneView& sv, const BatchHeader* d_hdr, int grid, int block,
                         int items_per_dequeue) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, d_hdr);
-    size_t lds = sizeof(int) * (size_t)block * ((size_t)kLdsStack + kPtCold);
+    size_t lds = sizeof(int) * (size_t)block * ((size_t)kPtLdsStack + kPtCold);
     if (items_per_dequeue == 1) hipLaunchKernelGGL(k_batch_pt_one, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
     else hipLaunchKernelGGL(k_batch_pt, dim3(grid), dim3(block), lds, stream, sv, d_hdr);
 }
@@ -1427,8 +1443,8 @@ void launch_guided_pt_rays(rzk_ctx* ctx, const SceneView& sv_in, const PtParams&
     SceneView sv = sv_in;
     const int block = 128;
     unsigned grid = (unsigned)((n + block - 1) / block);
-    bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4);
-    size_t lds = sizeof(int) * block * ((size_t)kLdsStack + kPtCold);
+    bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4, kPtLdsStack);
+    size_t lds = sizeof(int) * block * ((size_t)kPtLdsStack + kPtCold);
     hipLaunchKernelGGL(k_guided_pt_rays, dim3(grid), dim3(block), lds, ctx->stream, sv, pp, gv, d_rays, d_pix, d_smp,
                        n, d_out);
     RZK_HIP(hipGetLastError());
@@ -1441,10 +1457,10 @@ void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv_in, const float* cam, co
     SceneView sv = sv_in;
     const int block = 128;
     unsigned grid = (unsigned)((n + block - 1) / block);
-    bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4);
+    bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4, kPtLdsStack);
     CamView cv;
     for (int i = 0; i < RZK_CAM_FLOATS; ++i) cv.v[i] = cam[i];
-    size_t lds = sizeof(int) * block * ((size_t)kLdsStack + kPtCold);
+    size_t lds = sizeof(int) * block * ((size_t)kPtLdsStack + kPtCold);
     hipLaunchKernelGGL(k_guided_pt, dim3(grid), dim3(block), lds, ctx->stream, sv, cv, pp, gv, d_out, d_work);
     RZK_HIP(hipGetLastError());
 }

@@ -84,19 +84,21 @@ __device__ __forceinline__ void add_work_totals(unsigned long long* tot, const T
     }
 }
 
-// Any-hit traversal stack: the top kLdsStack entries of a lane live in LDS
-// (one column per lane, `pitch` = block size), deeper ones in HBM at
-// spill[(entry - kLdsStack) * spill_pitch + gid], so a launch's LDS footprint
-// stays small whatever the tree depth.
-struct AnyStack {
+// Any-hit traversal stack: the top kL entries of a lane live in LDS (one
+// column per lane, `pitch` = block size), deeper ones in HBM at
+// spill[(entry - kL) * spill_pitch + gid], so a launch's LDS footprint stays
+// small whatever the tree depth.  kL is even.
+template <int kL>
+struct AnyStackT {
+    static_assert(kL >= 0 && kL % 2 == 0, "LDS stack depth");
     int* lds;
     int pitch;
     int* spill;
     int64_t spitch;
     int64_t gid;
     __device__ __forceinline__ void push(int& sp, int v) {
-        if (sp < kLdsStack) lds[sp * pitch] = v;
-        else spill[(int64_t)(sp - kLdsStack) * spitch + gid] = v;
+        if (sp < kL) lds[sp * pitch] = v;
+        else spill[(int64_t)(sp - kL) * spitch + gid] = v;
         ++sp;
     }
     // the two reads name their address spaces, so they cannot be merged into
@@ -104,27 +106,27 @@ struct AnyStack {
     __device__ __forceinline__ int pop(int& sp) {
         --sp;
         int v;
-        if (sp < kLdsStack) v = ((__attribute__((address_space(3))) const int*)lds)[sp * pitch];
-        else v = ((__attribute__((address_space(1))) const int*)spill)[(int64_t)(sp - kLdsStack) * spitch + gid];
+        if (sp < kL) v = ((__attribute__((address_space(3))) const int*)lds)[sp * pitch];
+        else v = ((__attribute__((address_space(1))) const int*)spill)[(int64_t)(sp - kL) * spitch + gid];
         return v;
     }
     // Two ints as one 8-byte entry, for walks that push only pairs (sp stays
     // even): pair k of a lane sits at 8-byte word k * pitch of its LDS column
-    // pair (k < kLdsStack / 2), deeper ones at word (k - kLdsStack / 2) *
+    // pair (k < kL / 2), deeper ones at word (k - kL / 2) *
     // spitch + gid of the spill area; the same footprint as 2k single entries.
     __device__ __forceinline__ void push2(int& sp, int a, int b) {
         const unsigned long long v = (unsigned long long)(unsigned)a | ((unsigned long long)(unsigned)b << 32);
-        if (sp < kLdsStack) lds2()[(sp >> 1) * pitch] = v;
-        else reinterpret_cast<unsigned long long*>(spill)[(int64_t)((sp - kLdsStack) >> 1) * spitch + gid] = v;
+        if (sp < kL) lds2()[(sp >> 1) * pitch] = v;
+        else reinterpret_cast<unsigned long long*>(spill)[(int64_t)((sp - kL) >> 1) * spitch + gid] = v;
         sp += 2;
     }
     __device__ __forceinline__ int2 pop2(int& sp) {
         sp -= 2;
         unsigned long long v;
-        if (sp < kLdsStack) {
+        if (sp < kL) {
             v = ((__attribute__((address_space(3))) const unsigned long long*)lds2())[(sp >> 1) * pitch];
         } else {
-            v = ((__attribute__((address_space(1))) const unsigned long long*)spill)[(int64_t)((sp - kLdsStack) >> 1) *
+            v = ((__attribute__((address_space(1))) const unsigned long long*)spill)[(int64_t)((sp - kL) >> 1) *
                                                                                     spitch + gid];
         }
         return make_int2((int)(unsigned)v, (int)(unsigned)(v >> 32));
@@ -134,8 +136,11 @@ struct AnyStack {
     }
 };
 
-__device__ __forceinline__ AnyStack any_stack(const SceneView& sv, int* lds_base) {
-    AnyStack s;
+using AnyStack = AnyStackT<kLdsStack>;
+
+template <int kL = kLdsStack>
+__device__ __forceinline__ AnyStackT<kL> any_stack(const SceneView& sv, int* lds_base) {
+    AnyStackT<kL> s;
     s.lds = lds_base + threadIdx.x;
     s.pitch = blockDim.x;
     s.spill = sv.spill;
@@ -148,8 +153,9 @@ __device__ __forceinline__ AnyStack any_stack(const SceneView& sv, int* lds_base
 // the binary traversal's (aggregate_bvh.cc:452-478) for every ray: a triangle
 // is tested iff its own box and all enclosing boxes pass the slab test, and
 // the enclosing ones pass whenever the inner one does.  Visit order is free.
+template <class Stk>
 __device__ __forceinline__ bool any_hit4(const SceneView& sv, const Ray& r, float tmin, float tmax,
-                                         AnyStack& stk, TravStats& st) {
+                                         Stk& stk, TravStats& st) {
     if (!box_hit(mk(sv.rbmin[0], sv.rbmin[1], sv.rbmin[2]), mk(sv.rbmax[0], sv.rbmax[1], sv.rbmax[2]),
                  r, tmin, tmax))
         return false;
@@ -251,7 +257,8 @@ __device__ __forceinline__ bool trav4_begin(const SceneView& sv, const Ray& r, f
     return true;
 }
 
-__device__ __forceinline__ bool trav4_step(const SceneView& sv, const Ray& r, float tmin, Trav4& T, AnyStack& stk,
+template <class Stk>
+__device__ __forceinline__ bool trav4_step(const SceneView& sv, const Ray& r, float tmin, Trav4& T, Stk& stk,
                                            float& t_out, float& b_out, float& g_out, TravStats& st) {
     const Bvh4NodeGpu* np = sv.nodes4 + T.node;
     float4 lx = np->lox, ly = np->loy, lz = np->loz, hx = np->hix, hy = np->hiy, hz = np->hiz;
@@ -322,8 +329,9 @@ __device__ __forceinline__ bool trav4_step(const SceneView& sv, const Ray& r, fl
 }
 
 // the whole walk at once
+template <class Stk>
 __device__ __forceinline__ int closest_hit4(const SceneView& sv, const Ray& r, float tmin, float tmax,
-                                            AnyStack& stk, float& t_out, float& b_out, float& g_out,
+                                            Stk& stk, float& t_out, float& b_out, float& g_out,
                                             TravStats& st, bool any = false) {
     Trav4 T;
     if (trav4_begin(sv, r, tmin, tmax, any, T, t_out, b_out, g_out, st))
@@ -466,8 +474,9 @@ __device__ __forceinline__ bool any_hit4_wave(const SceneView& sv, const Ray& r,
 }
 
 // Segment visibility p -> q (aggregate.cc:84-90): true when blocked.
+template <class Stk>
 __device__ __forceinline__ bool segment_blocked(const SceneView& sv, v3 p, v3 q, float tmin,
-                                                AnyStack& stk, TravStats& st) {
+                                                Stk& stk, TravStats& st) {
     Ray r = ray_ctor(p, q - p);
     float tmax = length(q - p) - tmin;
     return any_hit4(sv, r, tmin, tmax, stk, st);

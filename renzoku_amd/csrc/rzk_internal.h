@@ -148,7 +148,7 @@ void* ctx_scratch(rzk_ctx* ctx, size_t bytes);
 void gather_noalloc_setup(int device);
 // size sv.spill for a launch of `threads` lanes whose traversal stack holds up
 // to `entries` ints (0: sv.stack4, the any-hit bound; closest_hit4 needs 2 * stack4)
-void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries = 0);
+void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries = 0, int lds_entries = kLdsStack);
 
 // kernel launchers (traverse.hip, shade.hip, cluster.hip)
 void launch_trace_closest(rzk_ctx* ctx, const SceneView& sv, const float* d_rays, int64_t n,

@@ -508,7 +508,7 @@ int rzk_server_create(int device, const rzk_scene* scene, const rzk_clusters* cl
             const int64_t threads = (int64_t)s->grid * s->block;
             s->step_items = (int64_t)std::max(1, std::min(8, prm->step_iterations)) * (threads / 64);
             s->sv = scene_view(scene);
-            const int64_t extra = 2 * (int64_t)s->sv.stack4 - kLdsStack;   // closest_hit4 pairs beyond LDS
+            const int64_t extra = 2 * (int64_t)s->sv.stack4 - kPtLdsStack;   // closest_hit4 pairs beyond LDS
             if (extra > 0) {
                 RZK_HIP(hipMalloc((void**)&s->d_spill, sizeof(int) * (size_t)(extra * threads)));
                 s->sv.spill = s->d_spill;
