@@ -13,7 +13,7 @@ namespace rzk {
 // ones spill to HBM); with the LDS state columns it sets how many waves fit
 // on a compute unit.
 #ifndef RZK_PT_LDS_STACK
-#define RZK_PT_LDS_STACK 8
+#define RZK_PT_LDS_STACK 12
 #endif
 constexpr int kPtLdsStack = RZK_PT_LDS_STACK;
 

@@ -329,12 +329,10 @@ enum : int {
     kPcWo = kPcSn + 3,                  //   not read after the hit
     kPcOnb = kPcWo + 3,
     kPcLo = kPcOnb + 9,                 // next event: partial sum (3)
-    kPcQ = kPcLo + 3,                   //   light sample point (3), normal (3), area pdf
-    kPcQn = kPcQ + 3,
-    kPcLpdf = kPcQn + 3,
-    kPcWi2 = kPcLpdf + 1,               //   BSDF direction (3), pdf
+    kPcLterm = kPcLo + 3,               //   the light sample's term (3), added if its segment is clear
+    kPcWi2 = kPcLterm + 3,              //   BSDF direction (3), pdf
     kPcDpdf2 = kPcWi2 + 3,
-    kPcBrdf2 = kPcQ,                    //   BSDF sample's BRDF (3; the light sample is read by then)
+    kPcBrdf2 = kPcLterm,                //   BSDF sample's BRDF (3; the light term is added by then)
     kPcWi = kPcLo,                      // continuation (after the next event): direction (3),
     kPcBrdf = kPcWi + 3,                //   BRDF (3), pdf, the other technique's pdf
     kPcPdf = kPcBrdf + 3,
@@ -450,9 +448,24 @@ __device__ __forceinline__ int pt_nee_begin(const SceneView& sv, const PtParams&
     v3 ql = (aa * mk(T.p0.x, T.p0.y, T.p0.z) + bb * mk(T.p1.x, T.p1.y, T.p1.z)) + cc * mk(T.p2.x, T.p2.y, T.p2.z);
     float lpdf = T.ratio * (1.0f / T.n.w);
     if (lpdf > 0.0f) {
-        c.set3(kPcQ, ql);
-        c.set3(kPcQn, mk(T.n.x, T.n.y, T.n.z));
-        c.set(kPcLpdf, lpdf);
+        // the light sample's term (AreaLight::first_bounce's light-sampling
+        // half), evaluated now and added once the segment turns out clear
+        const v3 qn = mk(T.n.x, T.n.y, T.n.z);
+        const v3 pn = c.get3(kPcSn), wo = c.get3(kPcWo);
+        const Onb b = c.onb();
+        const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
+        const v3 wi = unit(ql - p);
+        const v3 Le = light_query(Lg, qn, -wi);
+        const float G = form_factor_abs(p, pn, ql, qn);
+        v3 term = mk(0.f, 0.f, 0.f);
+        if (G > 0.0f) {
+            v3 brdf = bsdf_eval(mat, b, wo, wi);
+            float dpdf = bsdf_pdf(mat, b, wo, wi);
+            v3 pq = ql - p;
+            float w1 = balance(lpdf * dot(pq, pq) / fabsf(dot(unit(p - ql), qn)), dpdf);
+            term = (((w1 * G) / lpdf) * Le) * brdf;
+        }
+        c.set3(kPcLterm, term);
         nshadow++;
         // segment_blocked(p, q)
         q.r = ray_ctor(p, ql - p);
@@ -500,23 +513,9 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             at_hit = true;
             break;
         case kPsShadow: {
-            if (s.qtri < 0) {
-                const v3 p = c.get3(kPcP), pn = c.get3(kPcSn), wo = c.get3(kPcWo);
-                const Onb b = c.onb();
-                const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
-                const v3 ql = c.get3(kPcQ), qn = c.get3(kPcQn);
-                const float lpdf = c.get(kPcLpdf);
-                v3 wi = unit(ql - p);
-                v3 Le = light_query(sv.lights[s.k], qn, -wi);
-                float G = form_factor_abs(p, pn, ql, qn);
-                if (G > 0.0f) {
-                    v3 brdf = bsdf_eval(mat, b, wo, wi);
-                    float dpdf = bsdf_pdf(mat, b, wo, wi);
-                    v3 pq = ql - p;
-                    float w1 = balance(lpdf * dot(pq, pq) / fabsf(dot(unit(p - ql), qn)), dpdf);
-                    c.set3(kPcLo, c.get3(kPcLo) + (((w1 * G) / lpdf) * Le) * brdf);
-                }
-            }
+            // a clear segment adds the light term (zero when G <= 0, which
+            // leaves the zero partial sum as it is)
+            if (s.qtri < 0) c.set3(kPcLo, c.get3(kPcLo) + c.get3(kPcLterm));
             s.stage = kPsNeeSample;
             break;
         }
