@@ -320,6 +320,7 @@ enum : int {
     kPsContHit,         // result: the continuation ray
     kPsNeeSample,       // next: the next event's BSDF sample (no query pending)
     kPsContinue,        // next: the continuation's direction sample (no query pending)
+    kPsLightTerm,       // next: the light sample's term, then its shadow segment
 };
 
 // LDS columns of one lane (a float each, kPtCold of them)
@@ -333,6 +334,9 @@ enum : int {
     kPcWi2 = kPcLterm + 3,              //   BSDF direction (3), pdf
     kPcDpdf2 = kPcWi2 + 3,
     kPcBrdf2 = kPcLterm,                //   BSDF sample's BRDF (3; the light term is added by then)
+    kPcQl = kPcLterm,                   //   light sample point (3), normal (3), area pdf, from the
+    kPcQn = kPcWi2,                     //   sample to its term
+    kPcLpdf = kPcDpdf2,
     kPcWi = kPcLo,                      // continuation (after the next event): direction (3),
     kPcBrdf = kPcWi + 3,                //   BRDF (3), pdf, the other technique's pdf
     kPcPdf = kPcBrdf + 3,
@@ -416,12 +420,13 @@ __device__ __forceinline__ void pt_nee_end(const SceneView& sv, const PtCold& c,
     s.L = s.L + s.T * mk(Lo.x / sp, Lo.y / sp, Lo.z / sp);
 }
 
-// next_event's start (numbers 1..4): the light, its sample and the shadow
-// query.  Returns 1 with the shadow query in q, 2 when the light sample has
-// no positive pdf (the BSDF sample comes next), 0 when no light is chosen
-// (the next event adds nothing and ends).
+// next_event's start (numbers 1..4): the light and its sample.  Returns 1
+// with the sample staged in the lane's columns (its term and shadow segment
+// come next), 2 when the light sample has no positive pdf (the BSDF sample
+// comes next), 0 when no light is chosen (the next event adds nothing and
+// ends).
 __device__ __forceinline__ int pt_nee_begin(const SceneView& sv, const PtParams& pp, const PtCold& c, uint32_t pixel,
-                                            const uint32_t* u0, PtLane& s, PtQuery& q, unsigned long long& nshadow) {
+                                            const uint32_t* u0, PtLane& s) {
     float ul = u24(u0[1]);
     int k = -1;
     for (int i = 0; i < sv.nlights; ++i)
@@ -432,7 +437,6 @@ __device__ __forceinline__ int pt_nee_begin(const SceneView& sv, const PtParams&
     }
     s.k = k;
     const LightGpu& Lg = sv.lights[k];
-    const v3 p = c.get3(kPcP);
     c.set3(kPcLo, mk(0.f, 0.f, 0.f));
     uint32_t u1[4];
     pt_rng_block(pp.seed, pixel, (uint32_t)s.smp, (uint32_t)s.bounce, 1u, u1);
@@ -448,30 +452,10 @@ __device__ __forceinline__ int pt_nee_begin(const SceneView& sv, const PtParams&
     v3 ql = (aa * mk(T.p0.x, T.p0.y, T.p0.z) + bb * mk(T.p1.x, T.p1.y, T.p1.z)) + cc * mk(T.p2.x, T.p2.y, T.p2.z);
     float lpdf = T.ratio * (1.0f / T.n.w);
     if (lpdf > 0.0f) {
-        // the light sample's term (AreaLight::first_bounce's light-sampling
-        // half), evaluated now and added once the segment turns out clear
-        const v3 qn = mk(T.n.x, T.n.y, T.n.z);
-        const v3 pn = c.get3(kPcSn), wo = c.get3(kPcWo);
-        const Onb b = c.onb();
-        const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
-        const v3 wi = unit(ql - p);
-        const v3 Le = light_query(Lg, qn, -wi);
-        const float G = form_factor_abs(p, pn, ql, qn);
-        v3 term = mk(0.f, 0.f, 0.f);
-        if (G > 0.0f) {
-            v3 brdf = bsdf_eval(mat, b, wo, wi);
-            float dpdf = bsdf_pdf(mat, b, wo, wi);
-            v3 pq = ql - p;
-            float w1 = balance(lpdf * dot(pq, pq) / fabsf(dot(unit(p - ql), qn)), dpdf);
-            term = (((w1 * G) / lpdf) * Le) * brdf;
-        }
-        c.set3(kPcLterm, term);
-        nshadow++;
-        // segment_blocked(p, q)
-        q.r = ray_ctor(p, ql - p);
-        q.tmax = length(ql - p) - pp.tmin;
-        q.any = true;
-        s.stage = kPsShadow;
+        c.set3(kPcQl, ql);
+        c.set3(kPcQn, mk(T.n.x, T.n.y, T.n.z));
+        c.set(kPcLpdf, lpdf);
+        s.stage = kPsLightTerm;
         return 1;
     }
     return 2;
@@ -601,16 +585,18 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             s.brdf_sampling = u24(u0[0]) < 0.5f;
             s.stage = kPsContinue;
             if (s.bounce > 1 || pp.direct) {
-                const int nee = pt_nee_begin(sv, pp, c, pixel, u0, s, q, nshadow);
-                if (nee == 1) return true;
+                const int nee = pt_nee_begin(sv, pp, c, pixel, u0, s);
                 if (nee == 2) s.stage = kPsNeeSample;
             }
         }
-        // One sampling site for the next event's BSDF sample (numbers 5..7 of
+        // One site for the light sample's term (AreaLight::first_bounce's
+        // light-sampling half), the next event's BSDF sample (numbers 5..7 of
         // block 1, next_event's) and the continuation (numbers 8..11, after
-        // the next event), so a wave whose lanes want either runs it once.
+        // the next event): every lane's BSDF pdf and BRDF are evaluated on one
+        // path, so a wave whose lanes want any of them runs it once.
         if (s.stage == kPsContinue && s.bounce >= pp.max_bounce) { pt_sample_done(s, acc); continue; }
         {
+            const bool light = s.stage == kPsLightTerm;
             const bool nee = s.stage == kPsNeeSample;
             const float* mat = sv.mat + RZK_MAT_FLOATS * s.m;
             const Onb b = c.onb();
@@ -627,16 +613,43 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             v3 wi;
             float lam_pdf = 0.0f;
             int kind;
-            if (by_bsdf) kind = bsdf_sample_dir(mat, b, wo, ub, wi, lam_pdf);
-            else kind = guide_sample_dir(gv, gslice, pp.res, b, u, wi) ? 1 : 0;
-            float pdf = 0.0f, pdf_other = 0.0f;
+            if (light) {
+                wi = unit(c.get3(kPcQl) - p);
+                kind = 1;
+            } else if (by_bsdf) {
+                kind = bsdf_sample_dir(mat, b, wo, ub, wi, lam_pdf);
+            } else {
+                kind = guide_sample_dir(gv, gslice, pp.res, b, u, wi) ? 1 : 0;
+            }
+            float pdf = 0.0f, pdf_other = 0.0f, pdf_b = 0.0f;
             v3 brdf = mk(0.f, 0.f, 0.f);
             if (kind != 0) {
-                const float pdf_b = bsdf_pdf(mat, b, wo, wi);
-                const float pdf_g = nee ? 0.0f : guide_pdf(gv, gslice, pp.res, b, sn, wi);
+                pdf_b = bsdf_pdf(mat, b, wo, wi);
+                const float pdf_g = (nee || light) ? 0.0f : guide_pdf(gv, gslice, pp.res, b, sn, wi);
                 brdf = bsdf_eval(mat, b, wo, wi);
                 pdf = !by_bsdf ? pdf_g : kind == 2 ? lam_pdf : pdf_b;
                 pdf_other = s.brdf_sampling ? pdf_g : pdf_b;
+            }
+            if (light) {
+                // the term, added once the segment p -> ql turns out clear
+                // (segment_blocked(p, q), the shadow query)
+                const v3 ql = c.get3(kPcQl), qn = c.get3(kPcQn);
+                const float lpdf = c.get(kPcLpdf);
+                const v3 Le = light_query(sv.lights[s.k], qn, -wi);
+                const float G = form_factor_abs(p, sn, ql, qn);
+                v3 term = mk(0.f, 0.f, 0.f);
+                if (G > 0.0f) {
+                    v3 pq = ql - p;
+                    float w1 = balance(lpdf * dot(pq, pq) / fabsf(dot(unit(p - ql), qn)), pdf_b);
+                    term = (((w1 * G) / lpdf) * Le) * brdf;
+                }
+                c.set3(kPcLterm, term);
+                nshadow++;
+                q.r = ray_ctor(p, ql - p);
+                q.tmax = length(ql - p) - pp.tmin;
+                q.any = true;
+                s.stage = kPsShadow;
+                return true;
             }
             if (!(pdf > 0.0f)) {
                 if (nee) {
