@@ -536,16 +536,10 @@ __device__ __forceinline__ bool pt_advance(const SceneView& sv, const float* cam
             break;
         }
         case kPsContHit: {
-            const int tri = s.qtri;
-            bool ended = tri < 0;
-            if (!ended) {
-                int4 sf = sv.surf[__float_as_int(sv.tri_n[tri].w)];
-                ended = sf.y >= 0;                         // light: counted by the light sample
-            }
-            if (ended) {
-                pt_sample_done(s, acc);
-                continue;
-            }
+            // A miss or a light ends the sample (a light is counted by the
+            // light sample): the bounce loop's head below ends it on the
+            // receiver it finds, after a throughput update that then goes
+            // unused, so the hit's surface is read once, there.
             const v3 sn = c.get3(kPcSn);
             const v3 wi = c.get3(kPcWi);
             const float pdf = c.get(kPcPdf);
