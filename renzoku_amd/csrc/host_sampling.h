@@ -54,21 +54,24 @@ private:
 class DiscretePdf {
 public:
     bool build(const float* arr, size_t n) {
-        pdf_.assign(n, 0.f);
-        cdf_.assign(n, 0.f);
-        runmax_.assign(n, 0.f);
+        // every entry is written below whenever the pdf is valid
+        pdf_.resize(n);
+        cdf_.resize(n);
+        runmax_.resize(n);
         valid_ = false;
         if (n == 0) return false;
         float sum = 0.f;
         for (size_t i = 0; i < n; ++i) sum += arr[i];
         if (sum == 0.0f) return false;
         for (size_t i = 0; i < n; ++i) pdf_[i] = arr[i] / sum;
-        cdf_[0] = pdf_[0];
-        for (size_t i = 1; i < n; ++i) cdf_[i] = cdf_[i - 1] + pdf_[i];
-        cdf_[n - 1] = 1.0f;
-        float m = -INFINITY;
+        // cdf[i] = cdf[i - 1] + pdf[i] (the last one then forced to 1) and its
+        // running maximum, in one pass
+        float acc = 0.f, m = -INFINITY;
         for (size_t i = 0; i < n; ++i) {
-            if (cdf_[i] > m) m = cdf_[i];    // NaN entries never satisfy u < cdf
+            acc = i == 0 ? pdf_[0] : acc + pdf_[i];
+            const float ci = i + 1 == n ? 1.0f : acc;
+            cdf_[i] = ci;
+            if (ci > m) m = ci;              // NaN entries never satisfy u < cdf
             runmax_[i] = m;
         }
         valid_ = true;
@@ -96,5 +99,26 @@ private:
     std::vector<float> pdf_, cdf_, runmax_;
     bool valid_ = false;
 };
+
+// DiscretePdf over w[idx[0]], ..., w[idx[n - 1]] built and sampled once with u:
+// the same sums, quotients and strict-< search, walked only up to the index
+// sampled.  Returns it (pdf_k its probability) or -1 (pdf_k = 0) when the
+// weights sum to zero or there are none.
+inline int sample_members(const float* w, const int* idx, int n, float u, float& pdf_k) {
+    pdf_k = 0.f;
+    if (n <= 0) return -1;
+    float sum = 0.f;
+    for (int i = 0; i < n; ++i) sum += w[idx[i]];
+    if (sum == 0.0f) return -1;
+    float acc = 0.f, m = -INFINITY;
+    for (int i = 0; i < n; ++i) {
+        const float p = w[idx[i]] / sum;
+        acc = i == 0 ? p : acc + p;
+        const float ci = i + 1 == n ? 1.0f : acc;
+        if (ci > m) m = ci;
+        if (u < m) { pdf_k = p; return i; }
+    }
+    return -1;
+}
 
 }  // namespace rzk

@@ -298,6 +298,8 @@ struct rzk_lightslice {
     int64_t h_pt_work[4] = {0, 0, 0, 0};   // last counted render_guided: closest, shadow rays, nodes, tris
     std::vector<int> h_centers, h_assign, h_reps;
     std::vector<float> h_center_w, h_rep_w, h_alphas, h_mags;
+    std::vector<int> h_cnt, h_off, h_mem, h_pos;   // cluster members (reused between initializations)
+    DiscretePdf h_apdf;                            // the alphas' pdf
     // per-slice clusters (refinement): S x ld representatives / weights and counts
     DBuf<int> sreps, scount, cl_off, cl_mem;
     DBuf<float> srep_w, norms_R;
@@ -893,7 +895,7 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         RZK_HIP(hipMemcpyAsync(ls->h_alphas.data(), ls->alphas.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
         ls_sync(ctx);
         // centre sampling (cluster_builder.cc:114-156) on the host MT stream
-        DiscretePdf apdf;
+        DiscretePdf& apdf = ls->h_apdf;
         apdf.build(ls->h_alphas.data(), (size_t)V);
         int C = p.num_clusters;
         ls->h_centers.clear();
@@ -924,31 +926,33 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         RZK_HIP(hipMemcpyAsync(ls->h_mags.data(), ls->mags.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
         ls_sync(ctx);
         // members in increasing VPL order, representatives (cluster_builder.cc:230-272)
-        std::vector<int> cnt(ncl + 1, 0), off(ncl + 1, 0), mem(V);
+        std::vector<int>& cnt = ls->h_cnt;
+        std::vector<int>& off = ls->h_off;
+        std::vector<int>& mem = ls->h_mem;
+        cnt.assign(ncl + 1, 0);
+        off.assign(ncl + 1, 0);
+        mem.resize(V);
         for (int64_t i = 0; i < V; ++i)
             if (ls->h_assign[i] >= 0) cnt[ls->h_assign[i]]++;
         for (int c = 0; c < ncl; ++c) off[c + 1] = off[c] + cnt[c];
-        std::vector<int> fill(ncl, 0);
+        std::vector<int>& pos = ls->h_pos;
+        pos.assign(off.begin(), off.end());
         for (int64_t i = 0; i < V; ++i) {
             int c = ls->h_assign[i];
-            if (c >= 0) mem[off[c] + fill[c]++] = (int)i;
+            if (c >= 0) mem[pos[c]++] = (int)i;
         }
         ls->h_reps.clear();
         ls->h_rep_w.clear();
-        std::vector<float> mm;
         for (int c = 0; c < ncl; ++c) {
-            int m = cnt[c];
-            mm.resize(m);
-            for (int q = 0; q < m; ++q) mm[q] = ls->h_mags[mem[off[c] + q]];
-            DiscretePdf cp;
-            cp.build(mm.data(), (size_t)m);
+            const int m = off[c + 1] - off[c];
+            const int* mc = mem.data() + off[c];
             float u = rd.next_float();
             float pj = 0.f;
-            int j = cp.sample(u, pj);
+            const int j = sample_members(ls->h_mags.data(), mc, m, u, pj);
             int rep;
             float w;
-            if (j < 0) { rep = m > 0 ? mem[off[c] + m / 2] : -1; w = 0.0f; }
-            else { rep = mem[off[c] + j]; w = 1.0f / pj; }
+            if (j < 0) { rep = m > 0 ? mc[m / 2] : -1; w = 0.0f; }
+            else { rep = mc[j]; w = 1.0f / pj; }
             if (rep >= 0) { ls->h_reps.push_back(rep); ls->h_rep_w.push_back(w); }
         }
         int nreps = (int)ls->h_reps.size();
