@@ -33,21 +33,6 @@ struct DevArray {
     ~DevArray() { if (p) (void)hipFree(p); }
 };
 
-template <typename T>
-struct PinnedArray {
-    T* p = nullptr;
-    size_t n = 0;
-    void reserve(size_t count) {
-        if (count <= n && p) return;
-        if (p) RZK_HIP(hipHostFree(p));
-        p = nullptr;
-        n = 0;
-        RZK_HIP(hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(count, 1), hipHostMallocDefault));
-        n = std::max<size_t>(count, 1);
-    }
-    ~PinnedArray() { if (p) (void)hipHostFree(p); }
-};
-
 template <typename F>
 int batch_guard(rzk_ctx* ctx, F&& f) {
     try {

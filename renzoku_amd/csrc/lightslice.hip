@@ -296,8 +296,10 @@ struct rzk_lightslice {
     DBuf<float> pt_image;
     DBuf<unsigned long long> pt_work;
     int64_t h_pt_work[4] = {0, 0, 0, 0};   // last counted render_guided: closest, shadow rays, nodes, tris
-    std::vector<int> h_centers, h_assign, h_reps;
-    std::vector<float> h_center_w, h_rep_w, h_alphas, h_mags;
+    std::vector<int> h_centers, h_reps;
+    std::vector<float> h_center_w, h_rep_w;
+    PinnedArray<float> h_alphas, h_mags;           // device-to-host staging of the clustering inputs
+    PinnedArray<int> h_assign;
     std::vector<int> h_cnt, h_off, h_mem, h_pos;   // cluster members (reused between initializations)
     DiscretePdf h_apdf;                            // the alphas' pdf
     // per-slice clusters (refinement): S x ld representatives / weights and counts
@@ -891,12 +893,12 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         // alphas (light_transport_matrix.cc:72-106)
         launch_alphas_full(ctx, S, V, ls->R.p, ls->alphas.p, ls->mags.p, ls->norms.p, ls->u.p, ls->v.p, ls->c0.p,
                            ls->mT.p);
-        ls->h_alphas.resize(V);
-        RZK_HIP(hipMemcpyAsync(ls->h_alphas.data(), ls->alphas.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
+        ls->h_alphas.reserve(V);
+        RZK_HIP(hipMemcpyAsync(ls->h_alphas.p, ls->alphas.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
         ls_sync(ctx);
         // centre sampling (cluster_builder.cc:114-156) on the host MT stream
         DiscretePdf& apdf = ls->h_apdf;
-        apdf.build(ls->h_alphas.data(), (size_t)V);
+        apdf.build(ls->h_alphas.p, (size_t)V);
         int C = p.num_clusters;
         ls->h_centers.clear();
         ls->h_center_w.clear();
@@ -920,10 +922,10 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         RZK_HIP(hipMemcpyAsync(ls->center_w.p, ls->h_center_w.data(), sizeof(float) * ncl, hipMemcpyHostToDevice, st));
         launch_assign_full(ctx, S, V, ls->R.p, ncl, ls->centers.p, ls->center_w.p, ls->mags.p, ls->K.p, ls->Knorm.p,
                            ls->assign.p);
-        ls->h_assign.resize(V);
-        ls->h_mags.resize(V);
-        RZK_HIP(hipMemcpyAsync(ls->h_assign.data(), ls->assign.p, sizeof(int) * V, hipMemcpyDeviceToHost, st));
-        RZK_HIP(hipMemcpyAsync(ls->h_mags.data(), ls->mags.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
+        ls->h_assign.reserve(V);
+        ls->h_mags.reserve(V);
+        RZK_HIP(hipMemcpyAsync(ls->h_assign.p, ls->assign.p, sizeof(int) * V, hipMemcpyDeviceToHost, st));
+        RZK_HIP(hipMemcpyAsync(ls->h_mags.p, ls->mags.p, sizeof(float) * V, hipMemcpyDeviceToHost, st));
         ls_sync(ctx);
         // members in increasing VPL order, representatives (cluster_builder.cc:230-272)
         std::vector<int>& cnt = ls->h_cnt;
@@ -933,12 +935,12 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         off.assign(ncl + 1, 0);
         mem.resize(V);
         for (int64_t i = 0; i < V; ++i)
-            if (ls->h_assign[i] >= 0) cnt[ls->h_assign[i]]++;
+            if (ls->h_assign.p[i] >= 0) cnt[ls->h_assign.p[i]]++;
         for (int c = 0; c < ncl; ++c) off[c + 1] = off[c] + cnt[c];
         std::vector<int>& pos = ls->h_pos;
         pos.assign(off.begin(), off.end());
         for (int64_t i = 0; i < V; ++i) {
-            int c = ls->h_assign[i];
+            int c = ls->h_assign.p[i];
             if (c >= 0) mem[pos[c]++] = (int)i;
         }
         ls->h_reps.clear();
@@ -948,7 +950,7 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
             const int* mc = mem.data() + off[c];
             float u = rd.next_float();
             float pj = 0.f;
-            const int j = sample_members(ls->h_mags.data(), mc, m, u, pj);
+            const int j = sample_members(ls->h_mags.p, mc, m, u, pj);
             int rep;
             float w;
             if (j < 0) { rep = m > 0 ? mc[m / 2] : -1; w = 0.0f; }
@@ -1165,11 +1167,11 @@ int rzk_lightslice_get(const rzk_lightslice* ls, const char* name, void* dst, in
         bool host = false;
         if (nm == "R") { src = ls->R.p; need = 12 * (int64_t)S * V; }
         else if (nm == "V") { src = ls->Vm.p; need = 4 * (int64_t)S * V; }
-        else if (nm == "alphas") { src = ls->h_alphas.data(); need = 4 * V; host = true; }
-        else if (nm == "mags") { src = ls->h_mags.data(); need = 4 * V; host = true; }
+        else if (nm == "alphas") { src = ls->h_alphas.p; need = 4 * V; host = true; }
+        else if (nm == "mags") { src = ls->h_mags.p; need = 4 * V; host = true; }
         else if (nm == "centers") { src = ls->h_centers.data(); need = 4 * (int64_t)ls->h_centers.size(); host = true; }
         else if (nm == "center_w") { src = ls->h_center_w.data(); need = 4 * (int64_t)ls->h_center_w.size(); host = true; }
-        else if (nm == "assign") { src = ls->h_assign.data(); need = 4 * V; host = true; }
+        else if (nm == "assign") { src = ls->h_assign.p; need = 4 * V; host = true; }
         else if (nm == "reps") { src = ls->h_reps.data(); need = 4 * (int64_t)nreps; host = true; }
         else if (nm == "rep_w") { src = ls->h_rep_w.data(); need = 4 * (int64_t)nreps; host = true; }
         else if (nm == "rows_recv") { src = ls->rows_recv.p; need = 4 * (int64_t)RZK_RECV_FLOATS * S; }
@@ -1216,6 +1218,7 @@ int rzk_lightslice_get(const rzk_lightslice* ls, const char* name, void* dst, in
         else throw RzkError{RZK_E_INVALID, "unknown array name " + nm};
         RZK_CHECK(bytes == need, RZK_E_INVALID, "size mismatch for " + nm + ": need " + std::to_string(need));
         if (need == 0) return;
+        RZK_CHECK(src != nullptr, RZK_E_STATE, nm + " is not available before initialize");
         if (host) std::memcpy(dst, src, (size_t)need);
         else {
             RZK_HIP(hipMemcpyAsync(dst, src, (size_t)need, hipMemcpyDeviceToHost, ctx->stream));

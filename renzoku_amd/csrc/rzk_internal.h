@@ -148,6 +148,26 @@ void* ctx_scratch(rzk_ctx* ctx, size_t bytes);
 void gather_noalloc_setup(int device);
 // size sv.spill for a launch of `threads` lanes whose traversal stack holds up
 // to `entries` ints (0: sv.stack4, the any-hit bound; closest_hit4 needs 2 * stack4)
+// Page-locked host array (grown on demand, never shrunk): device-to-host
+// copies into it run as direct DMA.
+template <typename T>
+struct PinnedArray {
+    T* p = nullptr;
+    size_t n = 0;
+    PinnedArray() = default;
+    PinnedArray(const PinnedArray&) = delete;
+    PinnedArray& operator=(const PinnedArray&) = delete;
+    void reserve(size_t count) {
+        if (count <= n && p) return;
+        if (p) RZK_HIP(hipHostFree(p));
+        p = nullptr;
+        n = 0;
+        RZK_HIP(hipHostMalloc((void**)&p, sizeof(T) * (count > 0 ? count : 1), hipHostMallocDefault));
+        n = count > 0 ? count : 1;
+    }
+    ~PinnedArray() { if (p) (void)hipHostFree(p); }
+};
+
 void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries = 0, int lds_entries = kLdsStack);
 
 // kernel launchers (traverse.hip, shade.hip, cluster.hip)
