@@ -42,9 +42,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# MI355X vector issue: 256 CUs x 4 SIMDs, each retiring one wave64 VALU instruction per 4 cycles, at
-# 2.4 GHz: 256 x 64 lane-operations per cycle
-VALU_LANE_OPS_PEAK = 256 * 64 * 2.4e9
+# MI355X fp32 vector peak as published: 157.3 TFLOP/s, i.e. 78.65 T FMA lane-operations per second
+# (256 CUs x 4 SIMDs x 32 lanes x 2.4 GHz).  The probes below measure what instruction classes reach:
+# with every operand in a vector register about 0.92 of it (multiply + add), with a scalar-register
+# operand (a wave-uniform value, as the wave-coherent walks' node planes) about half of it.
+VALU_LANE_OPS_PEAK = 157.3e12 / 2
 # counter passes of the live PMC leg (one rocprofv3 run each; at most 8 SQ / 4 TCC counters per pass)
 PMC_PASSES = [["SQ_INSTS_VALU", "SQ_WAVES"], ["VALUUtilization", "VALUBusy"], ["FETCH_SIZE"], ["WRITE_SIZE"]]
 # kernels whose per-launch counters the PMC child reports (substring of the kernel name)
@@ -55,8 +57,11 @@ PMC_KERNELS = {"slice_visibility": ("k_slice_visibility", "first"), "slice_shade
                "reduced_matrix": ("k_reduced_matrix", "first"), "visibility": ("k_visibility", "first"),
                "gather_shade": ("k_gather_shade", "first"), "guided_pt": ("k_guided_pt", "first"),
                "copy": ("copyBuffer", "last"), "probe_fma": ("k_valu_probe(", "last"),
-               "probe_slab": ("k_valu_probe_slab(", "last")}
-PROBES = {"probe_fma": 0, "probe_slab": 1}
+               "probe_slab": ("k_valu_probe_slab(", "last"), "probe_muladd_vv": ("k_valu_probe_vv(", "last"),
+               "probe_slab_vv": ("k_valu_probe_slab_vv(", "last"), "probe_fma_vv": ("k_valu_probe_fma_vv(", "last")}
+# vector-issue probes (renzoku_amd/libprobe.so kind): FMA and the slab-test mix with scalar-register
+# operands, then multiply + add, the slab mix and FMA with vector-register operands only
+PROBES = {"probe_fma": 0, "probe_slab": 1, "probe_muladd_vv": 4, "probe_slab_vv": 5, "probe_fma_vv": 6}
 # the vector-issue probe launch (renzoku_amd/libprobe.so): 4,096 blocks of 256 lanes (16 waves per SIMD),
 # 16 independent FMA chains per lane, about 8 ms
 PROBE_BLOCKS, PROBE_ITERS = 4096, 8192
@@ -232,25 +237,26 @@ def probe_ms():
 
 
 def valu_peak(pmc, probe_launch_ms):
-    """(sustained VALU issue rate in lane-ops/s, how it was obtained): the best probe's SQ_INSTS_VALU x 64
-    over its launch time when the probes ran in the counter pass, else the 2.4 GHz clock model"""
+    """(VALU peak in lane-ops/s, how it was obtained, {probe: measured lane-ops/s}): the published fp32
+    vector peak, with each probe's SQ_INSTS_VALU x 64 over its launch time when the probes ran in the
+    counter pass"""
     rates, desc = {}, []
     for key in PROBES:
         insts, ms = pmc.get(key, {}).get("SQ_INSTS_VALU"), probe_launch_ms.get(key)
         if insts and ms:
             rates[key] = insts * 64.0 / (ms / 1e3)
             desc.append(f"{key}: {int(insts)} wave instructions in {ms:.4f} ms = "
-                        f"{rates[key] / VALU_LANE_OPS_PEAK:.4f} x the clock model")
-    if rates:
-        best = max(rates, key=rates.get)
-        return rates[best], f"measured ({best}); " + "; ".join(desc)
-    return VALU_LANE_OPS_PEAK, "model: 256 CUs x 64 lanes x 2.4 GHz (no probe counts)"
+                        f"{rates[key] / 1e12:.2f} T ({rates[key] / VALU_LANE_OPS_PEAK:.3f} of the peak)")
+    src = "published: MI355X fp32 vector 157.3 TFLOP/s = 78.65 T FMA lane-ops/s"
+    if desc:
+        src += "; measured probes: " + "; ".join(desc)
+    return VALU_LANE_OPS_PEAK, src, rates
 
 
 def valu_roofline(pmc, keys, launch_ms, kernel_name, byte_model=None, peak=None):
     """roofline of a vector-issue-bound launch: SQ_INSTS_VALU x 64 lanes per launch / launch time against
     the sustained issue peak (valu_peak); measured HBM traffic beside it"""
-    peak_rate, peak_src = peak if peak is not None else (VALU_LANE_OPS_PEAK, "model: 256 CUs x 64 lanes x 2.4 GHz")
+    peak_rate, peak_src, probe_rates = peak if peak is not None else (VALU_LANE_OPS_PEAK, "published", {})
     insts = [pmc.get(k, {}).get("SQ_INSTS_VALU") for k in keys]
     traffic, scales = pmc_traffic(pmc, keys)
     r = {"bound": "valu", "unit": "T VALU lane-ops/s", "peak": round(peak_rate / 1e12, 3),
@@ -261,6 +267,11 @@ def valu_roofline(pmc, keys, launch_ms, kernel_name, byte_model=None, peak=None)
         r["achieved"] = round(ops / (launch_ms / 1e3) / 1e12, 3)
         r["frac"] = round(ops / (launch_ms / 1e3) / peak_rate, 4)
         r["sq_insts_valu"] = {k: int(pmc[k]["SQ_INSTS_VALU"]) for k in keys}
+        if "probe_slab" in probe_rates:
+            # the issue rate of the class these kernels run (box tests whose node planes are
+            # wave-uniform scalar-register operands), as measured by the slab-mix probe
+            r["scalar_operand_issue"] = round(probe_rates["probe_slab"] / 1e12, 3)
+            r["frac_of_scalar_operand_issue"] = round(ops / (launch_ms / 1e3) / probe_rates["probe_slab"], 4)
     for k in keys:
         for c in ("VALUUtilization", "VALUBusy"):
             if c in pmc.get(k, {}):
@@ -288,6 +299,7 @@ def hbm_roofline(pmc, keys, launch_ms, algo_bytes, kernel_name, note, peak=None)
     insts = [pmc.get(k, {}).get("SQ_INSTS_VALU") for k in keys]
     if all(v is not None for v in insts):
         r["valu_frac"] = round(sum(insts) * 64.0 / (launch_ms / 1e3) / (peak[0] if peak else VALU_LANE_OPS_PEAK), 4)
+        r["valu_peak"] = round((peak[0] if peak else VALU_LANE_OPS_PEAK) / 1e12, 3)
     for k in keys:
         if "VALUUtilization" in pmc.get(k, {}):
             r.setdefault("valuutilization", {})[k] = round(pmc[k]["VALUUtilization"], 2)
@@ -913,9 +925,6 @@ def measure_c2(args, c2):
         ls.gather(to_host=False)
         ctx.synchronize()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    work = ls.count_work()
-    lsinf = ls.info()
-    log(f"[bench] work per step: {work}")
     ctx.synchronize()
     gather_ms = []
     t_start = time.perf_counter()
@@ -930,6 +939,10 @@ def measure_c2(args, c2):
     ms_per_step = total_s * 1e3 / args.steps
     g_ms = float(np.mean(gather_ms))
     log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, gather kernel {g_ms:.3f} ms")
+    # the step's work, counted after the timed steps (the LightSlice is initialized by then)
+    work = ls.count_work()
+    lsinf = ls.info()
+    log(f"[bench] work per step: {work}")
     shadow_rays = work["reduced_pairs"] + work["gather_pairs"]
     value = shadow_rays / (ms_per_step / 1e3) / 1e6
     out = {

@@ -19,7 +19,7 @@ void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, 
                         float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0, float* d_mT);
 void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl, const int* d_centers,
                         const float* d_weights, const float* d_mags, float* d_K, float* d_Knorm,
-                        int* d_assign);
+                        unsigned long long* d_key, int* d_assign);
 
 // ---------------------------------------------------------------------------
 // small RAII device buffer
@@ -654,11 +654,12 @@ int rzk_cluster_assign(rzk_ctx* ctx, int32_t rows, int64_t cols, const float* R,
         DevBuf<float> dR(n3 * cols), dw(ncl), dK(n3 * ncl), dKn(ncl), dn(cols), dm(cols), da(cols), du(rows),
             dvv(cols), dc(1), dmT((size_t)rows * cols);
         DevBuf<int> dc_(ncl), das(cols);
+        DevBuf<unsigned long long> dkey(cols);
         dR.upload(ctx, R, n3 * cols);
         dw.upload(ctx, weights, ncl);
         dc_.upload(ctx, centers, ncl);
         launch_alphas_full(ctx, rows, cols, dR.p, da.p, dm.p, dn.p, du.p, dvv.p, dc.p, dmT.p);
-        launch_assign_full(ctx, rows, cols, dR.p, ncl, dc_.p, dw.p, dm.p, dK.p, dKn.p, das.p);
+        launch_assign_full(ctx, rows, cols, dR.p, ncl, dc_.p, dw.p, dm.p, dK.p, dKn.p, dkey.p, das.p);
         das.download(ctx, out_assign, cols);
         sync(ctx);
     });

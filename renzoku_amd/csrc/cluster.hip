@@ -183,90 +183,139 @@ __global__ void k_build_K(int rows, int ncl, const float* __restrict__ R, const 
     }
 }
 
-// Assignment: block = 64 columns x all centres in tiles of 64.  Each of the 256
-// lanes accumulates a 4x4 block of (column, centre) dot products over the
-// 3*rows flattened entries in order, staged through LDS in chunks of 32.
-constexpr int kAsCols = 64, kAsCen = 64, kAsChunk = 32;
+// Assignment: block = 64 columns x one tile of 64 centres (grid.y = the tile).
+// Each of the 128 lanes accumulates an 8 x 4 block of (column, centre) dot
+// products over the 3*rows flattened entries in order, staged through LDS in
+// chunks of 32 with 16-byte aligned rows, so a step of the chain reads its 8
+// column and 4 centre values with three 16-byte LDS loads; a chunk's global
+// loads are issued together into registers while the chunk before it is being
+// accumulated.  A column's minimum over the tile goes to key[column] by a
+// 64-bit atomic min of (distance bits << 32 | centre): distances are finite
+// and non-negative, so the smallest key is the smallest distance and, among
+// equal ones, the smallest centre, which is what the sequential scan with
+// strict < keeps.
+constexpr int kAsCols = 64, kAsCen = 64, kAsChunk = 32, kAsPitch = 68, kAsBlock = 128;
+constexpr int kAsCw = 8, kAsKw = 4;                       // columns, centres per lane
+constexpr int kAsGroups = kAsCen / kAsKw;                 // centre groups (16)
+constexpr int kAsPer = kAsChunk * kAsCols / kAsBlock;     // staged values per lane and array (16)
+constexpr unsigned long long kAsNone = ~0ull;
 
-__global__ __launch_bounds__(256) void k_assign(int rows, int64_t cols, int ncl, const float* __restrict__ R,
-                                                const float* __restrict__ K, const float* __restrict__ Knorm,
-                                                const float* __restrict__ mags, int* __restrict__ assign) {
-    __shared__ float s_m[kAsChunk][kAsCols + 1];
-    __shared__ float s_k[kAsChunk][kAsCen + 1];
-    __shared__ float s_bd[16][kAsCols];
-    __shared__ int s_bj[16][kAsCols];
+__global__ void k_assign_init(int64_t cols, unsigned long long* __restrict__ key) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < cols) key[c] = kAsNone;
+}
+
+__global__ void k_assign_final(int64_t cols, const unsigned long long* __restrict__ key, int* __restrict__ assign) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < cols) assign[c] = key[c] == kAsNone ? -1 : (int)(unsigned)(key[c] & 0xffffffffull);
+}
+
+__global__ __launch_bounds__(kAsBlock) void k_assign(int rows, int64_t cols, int ncl, const float* __restrict__ R,
+                                                     const float* __restrict__ K, const float* __restrict__ Knorm,
+                                                     const float* __restrict__ mags,
+                                                     unsigned long long* __restrict__ key) {
+    // the staged chunks, then (after the last chunk) the groups' minima
+    __shared__ __align__(16) float s_lds[2 * kAsChunk * kAsPitch];
+    static_assert(2 * kAsGroups * kAsCols <= 2 * kAsChunk * kAsPitch, "minima fit the staging area");
+    float (*s_m)[kAsPitch] = reinterpret_cast<float (*)[kAsPitch]>(s_lds);
+    float (*s_k)[kAsPitch] = reinterpret_cast<float (*)[kAsPitch]>(s_lds + kAsChunk * kAsPitch);
+    float (*s_bd)[kAsCols] = reinterpret_cast<float (*)[kAsCols]>(s_lds);
+    int (*s_bj)[kAsCols] = reinterpret_cast<int (*)[kAsCols]>(s_lds + kAsGroups * kAsCols);
     const int n3 = 3 * rows;
     const int64_t col0 = (int64_t)blockIdx.x * kAsCols;
-    const int tc = threadIdx.x & 15;          // column group: columns tc*4 .. tc*4+3
-    const int tk = threadIdx.x >> 4;          // centre group: centres tk*4 .. tk*4+3
-    float best_d[4];
-    int best_j[4];
+    const int j0 = blockIdx.y * kAsCen;
+    const int tc = threadIdx.x & 7;           // column group: columns tc*8 .. tc*8+7
+    const int tk = threadIdx.x >> 3;          // centre group: centres tk*4 .. tk*4+3
+    // staged element k of this lane: column / centre cc0 + 4k of the tile,
+    // entry tt0 of the chunk (e = threadIdx.x + k * kAsBlock, cc = e / kAsChunk)
+    static_assert(kAsBlock % kAsChunk == 0, "a lane keeps one entry of the chunk");
+    constexpr int kAsStep = kAsBlock / kAsChunk;           // columns between a lane's staged values (4)
+    const int cc0 = threadIdx.x / kAsChunk, tt0 = threadIdx.x % kAsChunk;
+    const int64_t cmax = cols - (col0 + cc0);             // staged columns k with k * kAsStep < cmax exist
+    const int jmax = ncl - (j0 + cc0);
+    float vm[kAsPer], vk[kAsPer];
+    auto load_chunk = [&](int t0) {
+        const int t = t0 + tt0;
+        const bool tok = t < n3;
+        const float* rb = R + (col0 + cc0) * n3 + (tok ? t : 0);
+        const float* kb = K + (int64_t)(j0 + cc0) * n3 + (tok ? t : 0);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) { best_d[a] = FLT_MAX; best_j[a] = -1; }
-    float nm[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        int64_t c = col0 + tc * 4 + a;
-        nm[a] = c < cols ? mags[c] : 0.f;
-    }
-    for (int j0 = 0; j0 < ncl; j0 += kAsCen) {
-        float acc[4][4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
-        for (int t0 = 0; t0 < n3; t0 += kAsChunk) {
-            __syncthreads();
-            for (int e = threadIdx.x; e < kAsChunk * kAsCols; e += 256) {
-                int cc = e / kAsChunk, tt = e - cc * kAsChunk;
-                int64_t c = col0 + cc;
-                s_m[tt][cc] = (c < cols && t0 + tt < n3) ? R[c * n3 + t0 + tt] : 0.f;
-                int jj = j0 + cc;
-                s_k[tt][cc] = (jj < ncl && t0 + tt < n3) ? K[(int64_t)jj * n3 + t0 + tt] : 0.f;
-            }
-            __syncthreads();
-            int tmax = (n3 - t0) < kAsChunk ? (n3 - t0) : kAsChunk;
-            for (int tt = 0; tt < tmax; ++tt) {
-                float mv[4], kv[4];
-#pragma unroll
-                for (int a = 0; a < 4; ++a) mv[a] = s_m[tt][tc * 4 + a];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) kv[b] = s_k[tt][tk * 4 + b];
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] += mv[a] * kv[b];
-            }
+        for (int k = 0; k < kAsPer; ++k) {
+            const bool okm = tok && (int64_t)k * kAsStep < cmax, okk = tok && k * kAsStep < jmax;
+            const float xm = okm ? rb[(int64_t)k * kAsStep * n3] : 0.f;
+            const float xk = okk ? kb[(int64_t)k * kAsStep * n3] : 0.f;
+            vm[k] = xm;
+            vk[k] = xk;
         }
-        // dist = |Knorm_j * norm_mi - dot|; strict < keeps the first minimum
+    };
+    float acc[kAsCw][kAsKw];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            int j = j0 + tk * 4 + b;
-            if (j >= ncl) continue;
-            float kn = Knorm[j];
+    for (int a = 0; a < kAsCw; ++a)
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                float dist = fabsf(kn * nm[a] - acc[a][b]);
-                if (dist < best_d[a]) { best_d[a] = dist; best_j[a] = j; }
-            }
+        for (int b = 0; b < kAsKw; ++b) acc[a][b] = 0.f;
+    load_chunk(0);
+    for (int t0 = 0; t0 < n3; t0 += kAsChunk) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kAsPer; ++k) {
+            s_m[tt0][cc0 + k * kAsStep] = vm[k];
+            s_k[tt0][cc0 + k * kAsStep] = vk[k];
+        }
+        __syncthreads();
+        if (t0 + kAsChunk < n3) load_chunk(t0 + kAsChunk);
+        const int tmax = (n3 - t0) < kAsChunk ? (n3 - t0) : kAsChunk;
+#pragma unroll 2
+        for (int tt = 0; tt < tmax; ++tt) {
+            const float4 m0 = *reinterpret_cast<const float4*>(&s_m[tt][tc * kAsCw]);
+            const float4 m1 = *reinterpret_cast<const float4*>(&s_m[tt][tc * kAsCw + 4]);
+            const float4 kq = *reinterpret_cast<const float4*>(&s_k[tt][tk * kAsKw]);
+            const float mv[kAsCw] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+            const float kv[kAsKw] = {kq.x, kq.y, kq.z, kq.w};
+#pragma unroll
+            for (int a = 0; a < kAsCw; ++a)
+#pragma unroll
+                for (int b = 0; b < kAsKw; ++b) acc[a][b] += mv[a] * kv[b];
         }
     }
-    // combine the 16 centre groups: smallest distance, ties -> smallest centre index
+    // dist = |Knorm_j * norm_mi - dot|; strict < keeps the first minimum
+    float best_d[kAsCw];
+    int best_j[kAsCw];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) { s_bd[tk][tc * 4 + a] = best_d[a]; s_bj[tk][tc * 4 + a] = best_j[a]; }
+    for (int a = 0; a < kAsCw; ++a) {
+        best_d[a] = FLT_MAX;
+        best_j[a] = -1;
+    }
+#pragma unroll
+    for (int b = 0; b < kAsKw; ++b) {
+        int j = j0 + tk * kAsKw + b;
+        if (j >= ncl) continue;
+        float kn = Knorm[j];
+#pragma unroll
+        for (int a = 0; a < kAsCw; ++a) {
+            int64_t c = col0 + tc * kAsCw + a;
+            float nm = c < cols ? mags[c] : 0.f;
+            float dist = fabsf(kn * nm - acc[a][b]);
+            if (dist < best_d[a]) { best_d[a] = dist; best_j[a] = j; }
+        }
+    }
+    // combine the centre groups: smallest distance, ties -> smallest centre index
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < kAsCw; ++a) { s_bd[tk][tc * kAsCw + a] = best_d[a]; s_bj[tk][tc * kAsCw + a] = best_j[a]; }
     __syncthreads();
     if (threadIdx.x < kAsCols) {
         int cc = threadIdx.x;
         float bd = FLT_MAX;
         int bj = -1;
-        for (int g = 0; g < 16; ++g) {
+        for (int g = 0; g < kAsGroups; ++g) {
             float d = s_bd[g][cc];
             int j = s_bj[g][cc];
             if (j < 0) continue;
             if (d < bd || (d == bd && (bj < 0 || j < bj))) { bd = d; bj = j; }
         }
         int64_t c = col0 + cc;
-        if (c < cols) assign[c] = bj;
+        if (c < cols && bj >= 0)
+            atomicMin(key + c, ((unsigned long long)__float_as_uint(bd) << 32) | (unsigned long long)(unsigned)bj);
     }
 }
 
@@ -300,13 +349,16 @@ void launch_column_norms_R(rzk_ctx* ctx, int rows, int64_t cols, const float* d_
     RZK_HIP(hipGetLastError());
 }
 
+// d_key: cols 64-bit words of scratch
 void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl, const int* d_centers,
                         const float* d_weights, const float* d_mags, float* d_K, float* d_Knorm,
-                        int* d_assign) {
+                        unsigned long long* d_key, int* d_assign) {
     if (cols <= 0 || ncl <= 0) return;
     hipLaunchKernelGGL(k_build_K, dim3(ncl), dim3(256), 0, ctx->stream, rows, ncl, d_R, d_centers, d_weights,
                        d_K, d_Knorm);
-    hipLaunchKernelGGL(k_assign, dim3(nblk(cols, kAsCols)), dim3(256), 0, ctx->stream, rows, cols, ncl, d_R, d_K,
-                       d_Knorm, d_mags, d_assign);
+    hipLaunchKernelGGL(k_assign_init, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, cols, d_key);
+    hipLaunchKernelGGL(k_assign, dim3(nblk(cols, kAsCols), (unsigned)((ncl + kAsCen - 1) / kAsCen)), dim3(kAsBlock), 0,
+                       ctx->stream, rows, cols, ncl, d_R, d_K, d_Knorm, d_mags, d_key);
+    hipLaunchKernelGGL(k_assign_final, dim3(nblk(cols, 256)), dim3(256), 0, ctx->stream, cols, d_key, d_assign);
     RZK_HIP(hipGetLastError());
 }

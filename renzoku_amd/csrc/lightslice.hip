@@ -20,7 +20,7 @@ void launch_alphas_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, 
                         float* d_mags, float* d_norms, float* d_u, float* d_v, float* d_c0, float* d_mT);
 void launch_assign_full(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, int ncl, const int* d_centers,
                         const float* d_weights, const float* d_mags, float* d_K, float* d_Knorm,
-                        int* d_assign);
+                        unsigned long long* d_key, int* d_assign);
 void launch_column_norms_R(rzk_ctx* ctx, int rows, int64_t cols, const float* d_R, float* d_out);
 int64_t refine_draws_needed(int S, int ncl, int maxC);
 int launch_refine_slices(rzk_ctx* ctx, const float* d_R, int S, const int* d_nb, int nnb, int ncl, const int* d_off,
@@ -289,6 +289,7 @@ struct rzk_lightslice {
     DBuf<float> rows_recv, R, Vm;
     // clustering
     DBuf<float> alphas, mags, norms, u, v, c0, center_w, K, Knorm, rep_w, mT;
+    DBuf<unsigned long long> akey;                 // assignment keys (one per VPL)
     DBuf<int> centers, assign, reps;
     // slices
     DBuf<int> nb, upper;
@@ -836,7 +837,7 @@ int rzk_lightslice_create(rzk_ctx* ctx, const rzk_scene* s, const rzk_vpls* v, c
         ls->vis_bits.alloc((size_t)CW * ((ls->W + 7) / 8) * ((ls->H + 7) / 8));
         ls->alphas.alloc(V); ls->mags.alloc(V); ls->norms.alloc(V); ls->v.alloc(V); ls->c0.alloc(1);
         ls->centers.alloc(C); ls->center_w.alloc(C); ls->Knorm.alloc(C);
-        ls->assign.alloc(V); ls->reps.alloc(C); ls->rep_w.alloc(C);
+        ls->assign.alloc(V); ls->reps.alloc(C); ls->rep_w.alloc(C); ls->akey.alloc(V);
         if (p->max_slice_clusters > 0) { ls->norms_R.alloc(V); ls->cl_off.alloc(C + 1); ls->cl_mem.alloc(V); }
         std::memset(&ls->info, 0, sizeof(ls->info));
         if (!ls->kd) {
@@ -921,7 +922,7 @@ int rzk_lightslice_initialize(rzk_lightslice* ls, const float* camera) {
         RZK_HIP(hipMemcpyAsync(ls->centers.p, ls->h_centers.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, st));
         RZK_HIP(hipMemcpyAsync(ls->center_w.p, ls->h_center_w.data(), sizeof(float) * ncl, hipMemcpyHostToDevice, st));
         launch_assign_full(ctx, S, V, ls->R.p, ncl, ls->centers.p, ls->center_w.p, ls->mags.p, ls->K.p, ls->Knorm.p,
-                           ls->assign.p);
+                           ls->akey.p, ls->assign.p);
         ls->h_assign.reserve(V);
         ls->h_mags.reserve(V);
         RZK_HIP(hipMemcpyAsync(ls->h_assign.p, ls->assign.p, sizeof(int) * V, hipMemcpyDeviceToHost, st));

@@ -47,13 +47,122 @@ __global__ __launch_bounds__(256) void k_valu_probe_slab(float* __restrict__ out
     out[gid] = s;
 }
 
-// One launch of probe `kind` (0 FMA chains, 1 slab mix) over `blocks` x 256
+// Separate multiply and add (the contraction-free dot products of the
+// clustering), sixteen independent chains per lane.
+__global__ __launch_bounds__(256) void k_valu_probe_muladd(float* __restrict__ out, int iters, float m, float c) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = gid * 1e-7f + (float)k;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) a[k] = a[k] * m + c;
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += a[k];
+    out[gid] = s;
+}
+
+// Minimum / maximum only, sixteen independent chains per lane.
+__global__ __launch_bounds__(256) void k_valu_probe_minmax(float* __restrict__ out, int iters, float m, float c) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = gid * 1e-7f + (float)k;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) a[k] = fmaxf(fminf(a[k], m), c);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += a[k];
+    out[gid] = s;
+}
+
+// Multiply and add with every operand in a vector register (as in the
+// clustering's dot products), thirty-two independent chains per lane.
+__global__ __launch_bounds__(256) void k_valu_probe_vv(float* __restrict__ out, int iters, float m, float c) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float acc[8][4], mv[8], kv[4];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) mv[a] = gid * 1e-9f + c * (float)a;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) kv[b] = m - gid * 1e-9f * (float)b;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (float)(a + b);
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = acc[a][b] * kv[b] + mv[a];
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) s += acc[a][b];
+    out[gid] = s;
+}
+
+// The slab-test mix of k_valu_probe_slab with every operand in a vector
+// register.
+__global__ __launch_bounds__(256) void k_valu_probe_slab_vv(float* __restrict__ out, int iters, float m, float c) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = gid * 1e-7f + 0.01f * (float)k;
+    const float e = gid * 1e-12f;              // per-lane, so the constants below live in vector registers
+    const float lo = -4.f + e, hi = 4.f - e, y = 2.5f + e, mv = m + e, cv = c - e;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            float t = (a[k] - cv) * mv;
+            t = fmaxf(fminf(t, hi), lo);
+            a[k] = t < y ? t : -t;
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += a[k];
+    out[gid] = s;
+}
+
+// FMA chains with every operand in a vector register.
+__global__ __launch_bounds__(256) void k_valu_probe_fma_vv(float* __restrict__ out, int iters, float m, float c) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = gid * 1e-7f + (float)k;
+    const float e = gid * 1e-12f;
+    const float mv = m + e, cv = c - e;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) a[k] = __builtin_fmaf(a[k], mv, cv);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += a[k];
+    out[gid] = s;
+}
+
+// One launch of probe `kind` (0 FMA chains, 1 slab mix, 2 multiply + add,
+// 3 min / max, 4 multiply + add on vector registers only, 5 the slab mix and
+// 6 FMA chains with vector-register operands) over `blocks` x 256
 // threads running `iters` rounds; the
 // launch is timed with HIP events (after one untimed warm launch).  Returns 0,
 // or the HIP error code.
 extern "C" int rzk_probe_valu(int kind, int blocks, int iters, float* out_ms) {
-    if (blocks <= 0 || iters <= 0 || !out_ms || kind < 0 || kind > 1) return (int)hipErrorInvalidValue;
-    auto kern = kind == 0 ? k_valu_probe : k_valu_probe_slab;
+    if (blocks <= 0 || iters <= 0 || !out_ms || kind < 0 || kind > 6) return (int)hipErrorInvalidValue;
+    auto kern = kind == 0   ? k_valu_probe
+                : kind == 1 ? k_valu_probe_slab
+                : kind == 2 ? k_valu_probe_muladd
+                : kind == 3 ? k_valu_probe_minmax
+                : kind == 4 ? k_valu_probe_vv
+                : kind == 5 ? k_valu_probe_slab_vv
+                            : k_valu_probe_fma_vv;
     float* d = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
