@@ -130,6 +130,74 @@ __global__ __launch_bounds__(256) void k_valu_probe_slab_vv(float* __restrict__ 
     out[gid] = s;
 }
 
+// The slab mix with one scalar-register operand: the multiplier (a wave-uniform
+// plane, as in the wave-coherent walks); the other operands in vector registers.
+__global__ __launch_bounds__(256) void k_valu_probe_slab_sv(float* __restrict__ out, int iters, float m, float c) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = gid * 1e-7f + 0.01f * (float)k;
+    const float e = gid * 1e-12f;
+    const float lo = -4.f + e, hi = 4.f - e, y = 2.5f + e, cv = c - e;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            float t = (a[k] - cv) * m;
+            t = fmaxf(fminf(t, hi), lo);
+            a[k] = t < y ? t : -t;
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += a[k];
+    out[gid] = s;
+}
+
+// The slab mix with scalar operands in the subtract and the multiply only.
+__global__ __launch_bounds__(256) void k_valu_probe_slab_ss(float* __restrict__ out, int iters, float m, float c) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = gid * 1e-7f + 0.01f * (float)k;
+    const float e = gid * 1e-12f;
+    const float lo = -4.f + e, hi = 4.f - e, y = 2.5f + e;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            float t = (a[k] - c) * m;
+            t = fmaxf(fminf(t, hi), lo);
+            a[k] = t < y ? t : -t;
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += a[k];
+    out[gid] = s;
+}
+
+// The slab mix with constants (inline and literal) in min, max and compare,
+// the subtract and multiply on vector registers.
+__global__ __launch_bounds__(256) void k_valu_probe_slab_kc(float* __restrict__ out, int iters, float m, float c) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = gid * 1e-7f + 0.01f * (float)k;
+    const float e = gid * 1e-12f;
+    const float mv = m + e, cv = c - e;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            float t = (a[k] - cv) * mv;
+            t = fmaxf(fminf(t, 4.f), -4.f);
+            a[k] = t < 2.5f ? t : -t;
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += a[k];
+    out[gid] = s;
+}
+
 // FMA chains with every operand in a vector register.
 __global__ __launch_bounds__(256) void k_valu_probe_fma_vv(float* __restrict__ out, int iters, float m, float c) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,19 +218,24 @@ __global__ __launch_bounds__(256) void k_valu_probe_fma_vv(float* __restrict__ o
 
 // One launch of probe `kind` (0 FMA chains, 1 slab mix, 2 multiply + add,
 // 3 min / max, 4 multiply + add on vector registers only, 5 the slab mix and
-// 6 FMA chains with vector-register operands) over `blocks` x 256
+// 6 FMA chains with vector-register operands, 7 the slab mix with a scalar
+// multiplier, 8 with a scalar subtrahend and multiplier, 9 with constant
+// min / max / compare operands) over `blocks` x 256
 // threads running `iters` rounds; the
 // launch is timed with HIP events (after one untimed warm launch).  Returns 0,
 // or the HIP error code.
 extern "C" int rzk_probe_valu(int kind, int blocks, int iters, float* out_ms) {
-    if (blocks <= 0 || iters <= 0 || !out_ms || kind < 0 || kind > 6) return (int)hipErrorInvalidValue;
+    if (blocks <= 0 || iters <= 0 || !out_ms || kind < 0 || kind > 9) return (int)hipErrorInvalidValue;
     auto kern = kind == 0   ? k_valu_probe
                 : kind == 1 ? k_valu_probe_slab
                 : kind == 2 ? k_valu_probe_muladd
                 : kind == 3 ? k_valu_probe_minmax
                 : kind == 4 ? k_valu_probe_vv
                 : kind == 5 ? k_valu_probe_slab_vv
-                            : k_valu_probe_fma_vv;
+                : kind == 6 ? k_valu_probe_fma_vv
+                : kind == 7 ? k_valu_probe_slab_sv
+                : kind == 8 ? k_valu_probe_slab_ss
+                            : k_valu_probe_slab_kc;
     float* d = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
