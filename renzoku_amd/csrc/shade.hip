@@ -101,14 +101,20 @@ __global__ __launch_bounds__(kShadeBlock) void k_gather_shade(const float* __res
     }
     int64_t g = t / 64;
     int lane = (int)(t & 63);
+    static_assert(kVplTile == 64, "one visibility word of the tile per lane");
     for (int64_t k0 = 0; k0 < nvpl; k0 += kVplTile) {
         int cnt = (int)((nvpl - k0) < kVplTile ? (nvpl - k0) : kVplTile);
+        // this wave's visibility words of the tile's VPLs, word kk in lane kk,
+        // all in one load; the loop below takes them with readlane
+        const uint64_t wv = (lane < cnt && g < ngroups) ? bits[(k0 + lane) * ngroups + g] : ~0ull;
+        const uint32_t wlo = (uint32_t)wv, whi = (uint32_t)(wv >> 32);
         __syncthreads();
         stage_vpls(s_vpl, vpl, vmat, k0, cnt);
         __syncthreads();
         if (i >= 0 && m >= 0) {
             for (int kk = 0; kk < cnt; ++kk) {
-                uint64_t w = bits[(k0 + kk) * ngroups + g];
+                const uint64_t w = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)wlo, kk) |
+                                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)whi, kk) << 32);
                 if ((w >> lane) & 1ull) continue;
                 float vp[kVplRec];
                 load_record(vp, s_vpl + kk * kVplRec, kVplRec);
