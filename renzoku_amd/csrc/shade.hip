@@ -318,6 +318,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_shade(const float* __rest
         const int nreps = sr.nreps;
         for (int c0 = 0; c0 < nreps; c0 += kVplTile) {
             int cnt = (nreps - c0) < kVplTile ? (nreps - c0) : kVplTile;
+            // this wave's visibility words of the tile's representatives, word
+            // cc in lane cc, all in one load; the loop below takes them with readlane
+            const uint64_t wv = (lane < cnt && tile < ntiles) ? bits[(int64_t)(c0 + lane) * ntiles + tile] : ~0ull;
+            const uint32_t wlo = (uint32_t)wv, whi = (uint32_t)(wv >> 32);
             __syncthreads();
             for (int e = threadIdx.x; e < cnt * (kVplRec + 1); e += blockDim.x) {
                 int c = e / (kVplRec + 1), f = e - c * (kVplRec + 1);
@@ -331,7 +335,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_slice_shade(const float* __rest
             __syncthreads();
             if (m >= 0) {
                 for (int cc = 0; cc < cnt; ++cc) {
-                    uint64_t w = bits[(int64_t)(c0 + cc) * ntiles + tile];
+                    const uint64_t w = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)wlo, cc) |
+                                       ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)whi, cc) << 32);
                     if ((w >> lane) & 1ull) continue;
                     float vp[kVplRec + 1];
                     load_record(vp, s_vpl + cc * (kVplRec + 1), kVplRec + 1);
