@@ -27,6 +27,9 @@ using PtStack = AnyStackT<kPtLdsStack>;
 #define RZK_PT_WAVES 4
 #endif
 constexpr int kPtWaves = RZK_PT_WAVES;
+// Lanes per block of the one-lane-per-pixel path kernels: one wave, so the
+// uneven lengths of the waves' work are balanced at wave granularity.
+constexpr int kPtBlock = 64;
 
 // AreaLight::query (area_light.cc:19-25 with AreaLight::pdf, area_light.h:120-131)
 __device__ __forceinline__ v3 light_query(const LightGpu& L, v3 n, v3 wo) {
@@ -1447,7 +1450,7 @@ void launch_guided_pt_rays(rzk_ctx* ctx, const SceneView& sv_in, const PtParams&
                            const float* d_rays, const int* d_pix, const int* d_smp, int64_t n, float* d_out) {
     if (n <= 0) return;
     SceneView sv = sv_in;
-    const int block = 128;
+    const int block = kPtBlock;
     unsigned grid = (unsigned)((n + block - 1) / block);
     bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4, kPtLdsStack);
     size_t lds = sizeof(int) * block * ((size_t)kPtLdsStack + kPtCold);
@@ -1461,7 +1464,7 @@ void launch_guided_pt(rzk_ctx* ctx, const SceneView& sv_in, const float* cam, co
     int64_t n = (int64_t)pp.w * pp.h;
     if (n <= 0) return;
     SceneView sv = sv_in;
-    const int block = 128;
+    const int block = kPtBlock;
     unsigned grid = (unsigned)((n + block - 1) / block);
     bind_spill(ctx, sv, (int64_t)grid * block, 2 * sv.stack4, kPtLdsStack);
     CamView cv;
