@@ -1270,7 +1270,7 @@ def main():
     ap.add_argument("--requests", type=int, default=32, help="served workload / C5 leg: number of requests")
     ap.add_argument("--rate", type=float, default=200.0, help="C5 leg: Poisson arrival rate (1/s)")
     ap.add_argument("--budget-gb", type=int, default=16, help="served workload: memory budget (GiB)")
-    ap.add_argument("--step-iterations", type=int, default=2,
+    ap.add_argument("--step-iterations", type=int, default=8,
                     help="served workload: engine iterations between looks at finished requests")
     ap.add_argument("--seed", type=int, default=2013, help="served workload: request stream seed")
     ap.add_argument("--gpus", type=int, default=1)

@@ -812,7 +812,7 @@ class Server:
 
     LAUNCH = dict(BatchConfig.DEFAULTS, engine=0, block=128, blocks_per_cu=16, items_per_dequeue=1)
 
-    def __init__(self, device, scene, clusters, memory_budget=8 << 30, step_iterations=2, launch=None, guide=None):
+    def __init__(self, device, scene, clusters, memory_budget=8 << 30, step_iterations=8, launch=None, guide=None):
         self.lib = load()
         self.scene, self.clusters = scene, clusters
         p = ServerParams()
