@@ -377,6 +377,77 @@ def run_guided(args):
     print(json.dumps(out), flush=True)
 
 
+def run_build(args):
+    """Scene creation with the trees built on the host (rzk_scene_create: Morton binary tree and binned-SAH
+    4-wide tree) against the device build (rzk_scene_create_ex, RZK_SCENE_BUILD_DEVICE: the same binary
+    tree, the 4-wide tree collapsed from it) on the 1M- and 2M-triangle instanced scenes; then the guided
+    path-tracing frame of the 1M scene (1280x720, 16 spp, depth 5, the same 100k VPLs) on either 4-wide tree."""
+    import renzoku_amd as rz
+    from renzoku_amd import scenes, vplgen
+    ctx = rz.Ctx(0)
+    warm = scenes.cornell_box(64, 64)
+    rz.Scene(ctx, warm, build="device")
+    res = {}
+    for ntri in (1_000_000, 2_000_000):
+        sc = scenes.instanced_meshes(1280, 720, seed=2013, target_tris=ntri)
+        t0 = time.perf_counter()
+        hs = rz.Scene(ctx, sc)
+        ctx.synchronize()
+        host_ms = (time.perf_counter() - t0) * 1e3
+        dev_ms = []
+        for i in range(max(args.steps, 1)):
+            t0 = time.perf_counter()
+            ds = rz.Scene(ctx, sc, build="device")
+            ctx.synchronize()
+            dev_ms.append((time.perf_counter() - t0) * 1e3)
+            if i + 1 < args.steps:
+                del ds
+        hi, di = hs.info(), ds.info()
+        same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                   for a, b in zip(hs.export_trees()[:2], ds.export_trees()[:2]))
+        res[int(sc.ntri)] = dict(host_ms=round(host_ms, 1), device_ms=round(float(np.median(dev_ms)), 1),
+                                 speedup=round(host_ms / float(np.median(dev_ms)), 2),
+                                 binary_tree_bit_equal=bool(same), depth=int(hi.depth),
+                                 device_bytes_host_build=int(hi.device_bytes),
+                                 device_bytes_device_build=int(di.device_bytes))
+        log(f"[bench] {sc.ntri} triangles: host build {host_ms:.0f} ms, device build {np.median(dev_ms):.1f} ms "
+            f"(binary tree bit-equal: {same})")
+        if ntri == 1_000_000:
+            sc1, hs1, ds1 = sc, hs, ds
+        else:
+            del hs, ds
+    W, H = 1280, 720
+    vpl, vmat, ne = vplgen.device_vpls(hs1, 100_000, seed=2013)
+    vpls = rz.Vpls(ctx, vpl, vmat)
+    kw = dict(spp=16, max_bounce=5, direct_lighting=0, guide_mode=1, seed=2013, tmin=1e-3)
+    frames, imgs = {}, {}
+    ev = HipEvents()
+    e0, e1 = ev.create(), ev.create()
+    for name, scene in (("host", hs1), ("device", ds1)):
+        ls = rz.LightSlice(ctx, scene, vpls, width=W, height=H, tile=16, num_clusters=1000, num_neighbor_slices=16,
+                           map_resolution=32, n_emitted=ne, seed=2013)
+        ls.initialize(sc1.camera)
+        imgs[name], _ = ls.render_guided(sc1.camera, count_rays=False, to_host=True, **kw)
+        ms = []
+        for i in range(args.steps):
+            ev.record(e0, ctx.stream)
+            ls.render_guided(sc1.camera, count_rays=False, to_host=False, **kw)
+            ev.record(e1, ctx.stream)
+            ms.append(ev.elapsed_ms(e0, e1))
+        frames[name] = round(float(np.mean(ms)), 2)
+        log(f"[bench] guided frame on the {name}-built 4-wide tree: {frames[name]:.2f} ms")
+        del ls
+    a, b = imgs["host"], imgs["device"]
+    same_px = float(np.mean(np.all(a.view(np.uint32) == b.view(np.uint32), axis=-1)))
+    out = {"metric": "scene build ms (host vs device)", "value": res[max(res)]["device_ms"], "unit": "ms",
+           "n_gpus": 1, "steps": args.steps, "warmup": 1, "higher_is_better": False, "dtype": "f32",
+           "data": "synthetic (seeded instanced-mesh scenes)",
+           "config": {"workload": "scene creation, 1M and 2M triangles; guided frame 1280x720 16 spp depth 5",
+                      "parallelism": "single GPU"},
+           "builds": res, "guided_frame_ms": frames, "guided_image_equal_pixels": round(same_px, 6)}
+    print(json.dumps(out), flush=True)
+
+
 def run_refine(args):
     """LightSlice per-slice refinement on the 200k-triangle LightSlice workload
     (512x512, 65,536 VPLs, 32x32-pixel slices, 16 neighbour slices, 300 kept
@@ -1263,7 +1334,7 @@ def run_default(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="lightslice", choices=["lightslice", "guided", "batched", "served", "refine", "vplgen",
-                                                                 "kdslice"],
+                                                                 "kdslice", "build"],
                     help="lightslice: the 200k-triangle LightSlice gather step (the headline line); "
                          "guided: guided path tracing of the 1M-triangle scene; batched: 256 uneven jobs")
     ap.add_argument("--jobs", type=int, default=256, help="batched workload: number of jobs")
@@ -1310,6 +1381,8 @@ def main():
         return run_vplgen(args)
     if args.workload == "kdslice":
         return run_kdslice(args)
+    if args.workload == "build":
+        return run_build(args)
 
     return run_default(args)
 

@@ -101,12 +101,30 @@ int rzk_copy_d2h(rzk_ctx *ctx, void *dst, const void *src, int64_t bytes);
  * triangle per leaf, identical topology to the reference; uploaded as a
  * compacted BVH2 (64 B interior nodes, 48 B triangle records). */
 int rzk_scene_create(rzk_ctx *ctx, const rzk_scene_desc *desc, rzk_scene **out);
+/* The same scene with its trees built on the device (the reference builds on
+ * the host: AggregateBvh::build, aggregate_bvh.cc:48-240).  With
+ * RZK_SCENE_BUILD_DEVICE the Morton binary tree is the one rzk_scene_create
+ * makes, node for node and bit for bit (Morton codes, a stable radix sort,
+ * top-down splits and bottom-up boxes, one depth per launch); the 4-wide tree
+ * the any-hit, gather and path-tracing walks use is that binary tree collapsed
+ * four-wide instead of the host's binned-SAH tree.  Any-hit and segment answers
+ * are the same; closest hits are the same wherever the nearest triangle is
+ * unique.  flags = 0 is rzk_scene_create.  Scenes of fewer than two triangles
+ * are built on the host. */
+#define RZK_SCENE_BUILD_DEVICE 1
+int rzk_scene_create_ex(rzk_ctx *ctx, const rzk_scene_desc *desc, int flags, rzk_scene **out);
 int rzk_scene_destroy(rzk_scene *scene);
 int rzk_scene_get_info(const rzk_scene *scene, rzk_scene_info *out);
 /* the reference's BFS node array (aggregate_bvh.cc:118-207) for parity checks */
 int rzk_scene_export_bvh(const rzk_scene *scene, int32_t *sorted_tri, uint32_t *sorted_morton,
                          int32_t *node_child, int32_t *node_range, float *node_box,
-                         int32_t *num_nodes);
+                         int32_t *num_nodes);  /* RZK_E_STATE for node arrays of a device-built scene */
+/* the device trees as the walks read them: binary nodes (64 B each, num_interior_nodes),
+ * Morton-order triangles (48 B each), 4-wide nodes (128 B each, *num_nodes4) and the
+ * 4-wide tree's triangles; any pointer may be NULL.  No reference counterpart: parity
+ * checks of the two builds. */
+int rzk_scene_export_trees(const rzk_scene *scene, void *nodes, void *tris, void *nodes4, void *tris4,
+                           int32_t *num_nodes4);
 
 /* Aggregate::hit(const Ray&, tmin, tmax, time, HitRecord&) — closest hit,
  * aggregate_bvh.cc:402-448.  Batched, host buffers. */

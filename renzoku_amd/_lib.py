@@ -23,7 +23,8 @@ SURF_FLIPPED, SURF_TWO_SIDED = 1, 2
 EXPORTS = [
     "rzk_ctx_create", "rzk_ctx_destroy", "rzk_ctx_last_error", "rzk_ctx_stream", "rzk_ctx_synchronize",
     "rzk_device_alloc", "rzk_device_free", "rzk_copy_h2d", "rzk_copy_d2h",
-    "rzk_scene_create", "rzk_scene_destroy", "rzk_scene_get_info", "rzk_scene_export_bvh",
+    "rzk_scene_create", "rzk_scene_create_ex", "rzk_scene_destroy", "rzk_scene_get_info", "rzk_scene_export_bvh",
+    "rzk_scene_export_trees",
     "rzk_trace_closest", "rzk_trace_any", "rzk_trace_segments",
     "rzk_trace_closest_dev", "rzk_trace_any_dev", "rzk_trace_segments_dev", "rzk_trace_stats",
     "rzk_fill_receivers", "rzk_camera_primary",
@@ -161,6 +162,8 @@ def load() -> C.CDLL:
         "rzk_scene_destroy": [P],
         "rzk_scene_get_info": [P, C.POINTER(SceneInfo)],
         "rzk_scene_export_bvh": [P, P, P, P, P, P, C.POINTER(I32)],
+        "rzk_scene_create_ex": [P, C.POINTER(SceneDesc), I32, C.POINTER(P)],
+        "rzk_scene_export_trees": [P, P, P, P, P, C.POINTER(I32)],
         "rzk_trace_closest": [P, P, P, I64, F, F, P],
         "rzk_trace_any": [P, P, P, I64, F, F, P],
         "rzk_trace_segments": [P, P, P, P, I64, F, P],
@@ -280,10 +283,16 @@ class Ctx:
             pass
 
 
-class Scene:
-    """The Aggregate: scene + Morton LBVH on the device."""
+SCENE_BUILD_DEVICE = 1
 
-    def __init__(self, ctx: Ctx, sc):
+
+class Scene:
+    """The Aggregate: scene + Morton LBVH on the device.  build="device" builds
+    both trees on the GPU (rzk_scene_create_ex with RZK_SCENE_BUILD_DEVICE)."""
+
+    def __init__(self, ctx: Ctx, sc, build="host"):
+        if build not in ("host", "device"):
+            raise ValueError(f"build must be 'host' or 'device', not {build!r}")
         self.ctx, self.lib = ctx, ctx.lib
         self._keep = [f32(sc.positions), f32(sc.tangents), i32(sc.tri_surface), i32(sc.surf_material),
                       i32(sc.surf_light), i32(sc.surf_flags), f32(sc.materials), f32(sc.light_radiance)]
@@ -291,7 +300,8 @@ class Scene:
         d = SceneDesc(len(ts), ptr(pos), ptr(tan), ptr(ts), len(sm), ptr(sm), ptr(sl), ptr(sf),
                       mat.shape[0], ptr(mat), lr.shape[0], ptr(lr))
         h = C.c_void_p()
-        ctx.check(self.lib.rzk_scene_create(ctx.h, C.byref(d), C.byref(h)), "rzk_scene_create")
+        flags = SCENE_BUILD_DEVICE if build == "device" else 0
+        ctx.check(self.lib.rzk_scene_create_ex(ctx.h, C.byref(d), flags, C.byref(h)), "rzk_scene_create_ex")
         self.h = h
         self.ntri = len(ts)
 
@@ -306,8 +316,32 @@ class Scene:
         st, sm = np.zeros(n, np.int32), np.zeros(n, np.uint32)
         ch, rg, bx = np.zeros(2 * nn, np.int32), np.zeros(2 * nn, np.int32), np.zeros(6 * nn, np.float32)
         cnt = C.c_int32()
-        self.lib.rzk_scene_export_bvh(self.h, ptr(st), ptr(sm), ptr(ch), ptr(rg), ptr(bx), C.byref(cnt))
+        self.ctx.check(self.lib.rzk_scene_export_bvh(self.h, ptr(st), ptr(sm), ptr(ch), ptr(rg), ptr(bx),
+                                                     C.byref(cnt)), "rzk_scene_export_bvh")
         return st, sm, ch.reshape(-1, 2), rg.reshape(-1, 2), bx.reshape(-1, 6)
+
+    def export_bvh_sorted(self):
+        """(sorted position -> triangle, sorted Morton codes); also for a device-built scene"""
+        st, sm = np.zeros(self.ntri, np.int32), np.zeros(self.ntri, np.uint32)
+        cnt = C.c_int32()
+        self.ctx.check(self.lib.rzk_scene_export_bvh(self.h, ptr(st), ptr(sm), None, None, None, C.byref(cnt)),
+                       "rzk_scene_export_bvh")
+        return st, sm
+
+    def export_trees(self):
+        """(binary nodes [num_interior, 16] f32, Morton-order triangles [ntri, 12] f32,
+        4-wide nodes [n4, 32] f32, 4-wide triangles [ntri, 12] f32) as the walks read them"""
+        inf = self.info()
+        n4 = C.c_int32()
+        self.ctx.check(self.lib.rzk_scene_export_trees(self.h, None, None, None, None, C.byref(n4)),
+                       "rzk_scene_export_trees")
+        nodes = np.zeros((inf.num_interior_nodes, 16), np.float32)
+        tris = np.zeros((self.ntri, 12), np.float32)
+        nodes4 = np.zeros((n4.value, 32), np.float32)
+        tris4 = np.zeros((self.ntri, 12), np.float32)
+        self.ctx.check(self.lib.rzk_scene_export_trees(self.h, ptr(nodes), ptr(tris), ptr(nodes4), ptr(tris4),
+                                                       C.byref(n4)), "rzk_scene_export_trees")
+        return nodes, tris, nodes4, tris4
 
     def trace_closest(self, rays, tmin=1e-3, tmax=1e6):
         rays = f32(rays).reshape(-1, 6)

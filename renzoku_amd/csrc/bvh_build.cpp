@@ -240,6 +240,7 @@ void build_sah4(int ntri, const float* pos, const std::vector<v3>& bmin, const s
     out.root4_ref = 0;
     out.depth4 = depth4;
     out.stack4 = 3 * depth4 + 1;
+    out.num_nodes4 = (int)out.nodes4.size();
 }
 
 void bvh_build_host(int ntri, const float* pos, BvhHost& out) {
@@ -336,6 +337,7 @@ void bvh_build_host(int ntri, const float* pos, BvhHost& out) {
         out.root4_ref = ~0;
         out.depth4 = 0;
         out.stack4 = 1;
+        out.num_nodes4 = 0;
         out.tris4 = out.tris;
         return;
     }

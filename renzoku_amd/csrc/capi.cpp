@@ -188,7 +188,12 @@ int rzk_copy_d2h(rzk_ctx* ctx, void* dst, const void* src, int64_t bytes) {
 
 // ---------------------------------------------------------------------------
 int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
+    return rzk_scene_create_ex(ctx, d, 0, out);
+}
+
+int rzk_scene_create_ex(rzk_ctx* ctx, const rzk_scene_desc* d, int flags, rzk_scene** out) {
     return guarded(ctx, [&] {
+        RZK_CHECK((flags & ~RZK_SCENE_BUILD_DEVICE) == 0, RZK_E_INVALID, "unknown scene flags");
         RZK_CHECK(ctx && d && out, RZK_E_INVALID, "null argument");
         *out = nullptr;
         RZK_CHECK(d->num_triangles >= 0 && d->num_surfaces >= 0 && d->num_materials >= 0, RZK_E_INVALID,
@@ -206,7 +211,10 @@ int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
         std::unique_ptr<rzk_scene> s(new rzk_scene());
         s->ctx = ctx;
         int n = d->num_triangles;
-        bvh_build_host(n, d->positions, s->bvh);
+        if ((flags & RZK_SCENE_BUILD_DEVICE) && n >= 2)
+            bvh_build_device(ctx, n, d->positions, s.get());
+        else
+            bvh_build_host(n, d->positions, s->bvh);
         s->nsurf = d->num_surfaces;
         s->nmat = d->num_materials;
         s->nlight = d->num_lights;
@@ -237,10 +245,12 @@ int rzk_scene_create(rzk_ctx* ctx, const rzk_scene_desc* d, rzk_scene** out) {
             RZK_HIP(hipMalloc(dst, bytes));
             if (src) RZK_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
         };
-        up((void**)&s->d_nodes, s->bvh.nodes.data(), sizeof(BvhNodeGpu) * s->bvh.nodes.size());
-        up((void**)&s->d_nodes4, s->bvh.nodes4.data(), sizeof(Bvh4NodeGpu) * s->bvh.nodes4.size());
-        up((void**)&s->d_tris, s->bvh.tris.data(), sizeof(TriGpu) * s->bvh.tris.size());
-        up((void**)&s->d_tris4, s->bvh.tris4.data(), sizeof(TriGpu) * s->bvh.tris4.size());
+        if (!s->bvh.device_built) {
+            up((void**)&s->d_nodes, s->bvh.nodes.data(), sizeof(BvhNodeGpu) * s->bvh.nodes.size());
+            up((void**)&s->d_nodes4, s->bvh.nodes4.data(), sizeof(Bvh4NodeGpu) * s->bvh.nodes4.size());
+            up((void**)&s->d_tris, s->bvh.tris.data(), sizeof(TriGpu) * s->bvh.tris.size());
+            up((void**)&s->d_tris4, s->bvh.tris4.data(), sizeof(TriGpu) * s->bvh.tris4.size());
+        }
         up((void**)&s->d_tri_n, tn.data(), sizeof(float4) * n);
         up((void**)&s->d_tri_tan, tt.data(), sizeof(float4) * n);
         up((void**)&s->d_surf, surf.data(), sizeof(int4) * s->nsurf);
@@ -326,7 +336,7 @@ int rzk_scene_get_info(const rzk_scene* s, rzk_scene_info* out) {
     out->depth = s->bvh.depth;
     for (int i = 0; i < 6; ++i) out->box[i] = s->bvh.box[i];
     out->device_bytes = (int64_t)sizeof(BvhNodeGpu) * s->bvh.num_interior +
-                        (int64_t)sizeof(Bvh4NodeGpu) * (int64_t)s->bvh.nodes4.size() +
+                        (int64_t)sizeof(Bvh4NodeGpu) * (int64_t)s->bvh.num_nodes4 +
                         (int64_t)2 * sizeof(TriGpu) * s->bvh.ntri + (int64_t)2 * sizeof(float4) * s->bvh.ntri;
     return RZK_OK;
 }
@@ -334,6 +344,7 @@ int rzk_scene_get_info(const rzk_scene* s, rzk_scene_info* out) {
 int rzk_scene_export_bvh(const rzk_scene* s, int32_t* sorted_tri, uint32_t* sorted_morton, int32_t* node_child,
                          int32_t* node_range, float* node_box, int32_t* num_nodes) {
     if (!s) return RZK_E_INVALID;
+    if (s->bvh.device_built && (node_child || node_range || node_box)) return RZK_E_STATE;
     int nn = s->bvh.ntri > 0 ? 2 * s->bvh.ntri - 1 : 0;
     if (num_nodes) *num_nodes = nn;
     for (int i = 0; i < s->bvh.ntri; ++i) {
@@ -346,6 +357,23 @@ int rzk_scene_export_bvh(const rzk_scene* s, int32_t* sorted_tri, uint32_t* sort
         if (node_box) std::memcpy(node_box + 6 * j, &s->bvh.bfs_box[6 * j], 6 * sizeof(float));
     }
     return RZK_OK;
+}
+
+int rzk_scene_export_trees(const rzk_scene* s, void* nodes, void* tris, void* nodes4, void* tris4,
+                           int32_t* num_nodes4) {
+    if (!s) return RZK_E_INVALID;
+    return guarded(s->ctx, [&] {
+        if (num_nodes4) *num_nodes4 = s->bvh.num_nodes4;
+        if (s->bvh.ntri == 0) return;
+        auto get = [&](void* dst, const void* src, size_t bytes) {
+            if (dst && bytes) RZK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->ctx->stream));
+        };
+        get(nodes, s->d_nodes, sizeof(BvhNodeGpu) * (size_t)s->bvh.num_interior);
+        get(tris, s->d_tris, sizeof(TriGpu) * (size_t)s->bvh.ntri);
+        get(nodes4, s->d_nodes4, sizeof(Bvh4NodeGpu) * (size_t)s->bvh.num_nodes4);
+        get(tris4, s->d_tris4, sizeof(TriGpu) * (size_t)s->bvh.ntri);
+        RZK_HIP(hipStreamSynchronize(s->ctx->stream));
+    });
 }
 
 // ---------------------------------------------------------------------------

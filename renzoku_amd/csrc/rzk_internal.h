@@ -45,6 +45,8 @@ struct BvhHost {
     int root4_ref = 0;
     int depth4 = 0;           // 4-wide nodes on the deepest path
     int stack4 = 0;           // bound on any-hit stack entries (3 per 4-wide level)
+    int num_nodes4 = 0;       // 4-wide nodes (nodes4 holds them after a host build)
+    bool device_built = false;           // trees built on the device: nodes/tris vectors and the BFS view stay empty
     std::vector<TriGpu> tris;            // reference-tree leaf (Morton) order
     std::vector<TriGpu> tris4;           // 4-wide tree leaf order
     std::vector<int> sorted_tri;         // sorted position -> original triangle
@@ -97,6 +99,12 @@ struct rzk_scene {
     std::vector<float> h_light_radiance;
 };
 
+// Builds the scene's two trees on the device from host positions (bvh_device.hip):
+// the Morton binary tree node-for-node as bvh_build_host lays it out, and a
+// 4-wide tree collapsed from it.  Fills s->bvh's scalars and sorted arrays and
+// s->d_nodes, d_nodes4, d_tris, d_tris4.  Needs ntri >= 2.
+void bvh_build_device(rzk_ctx* ctx, int ntri, const float* pos, rzk_scene* s);
+
 struct rzk_vpls {
     rzk_ctx* ctx = nullptr;
     int64_t n = 0;
@@ -146,8 +154,6 @@ void* ctx_scratch(rzk_ctx* ctx, size_t bytes);
 // resolves the device address of the no-allocation gather's static scratch on
 // `device` (current), once per device (gather_noalloc.hip)
 void gather_noalloc_setup(int device);
-// size sv.spill for a launch of `threads` lanes whose traversal stack holds up
-// to `entries` ints (0: sv.stack4, the any-hit bound; closest_hit4 needs 2 * stack4)
 // Page-locked host array (grown on demand, never shrunk): device-to-host
 // copies into it run as direct DMA.
 template <typename T>
@@ -168,6 +174,8 @@ struct PinnedArray {
     ~PinnedArray() { if (p) (void)hipHostFree(p); }
 };
 
+// size sv.spill for a launch of `threads` lanes whose traversal stack holds up
+// to `entries` ints (0: sv.stack4, the any-hit bound; closest_hit4 needs 2 * stack4)
 void bind_spill(rzk_ctx* ctx, SceneView& sv, int64_t threads, int entries = 0, int lds_entries = kLdsStack);
 
 // kernel launchers (traverse.hip, shade.hip, cluster.hip)
