@@ -35,8 +35,10 @@ struct SurfaceBinding {
 class AggregateGpu : public Aggregate {
 public:
     // triangles flattened in surface order (aggregate_bvh.cc:60-75), as rzk_scene_desc describes;
-    // bindings: one per surface (desc.num_surfaces), or empty (pointers stay null)
-    AggregateGpu(rzk_ctx* ctx, const rzk_scene_desc& desc, std::vector<SurfaceBinding> bindings = {})
+    // bindings: one per surface (desc.num_surfaces), or empty (pointers stay null);
+    // scene_flags: RZK_SCENE_BUILD_DEVICE builds the trees on the GPU (rzk_scene_create_ex)
+    AggregateGpu(rzk_ctx* ctx, const rzk_scene_desc& desc, std::vector<SurfaceBinding> bindings = {},
+                 int scene_flags = 0)
         : ctx_(ctx), bindings_(std::move(bindings)) {
         if (!bindings_.empty() && (int)bindings_.size() != desc.num_surfaces)
             throw std::runtime_error("AggregateGpu: one surface binding per surface");
@@ -44,7 +46,7 @@ public:
         tangents_.assign(desc.tangents, desc.tangents + 3 * (size_t)desc.num_triangles);
         for (size_t i = 0; i < bindings_.size(); ++i)
             if (bindings_[i].material) material_ids_[bindings_[i].material] = desc.surf_material[i];
-        check(rzk_scene_create(ctx_, &desc, &scene_), "rzk_scene_create");
+        check(rzk_scene_create_ex(ctx_, &desc, scene_flags, &scene_), "rzk_scene_create_ex");
     }
     ~AggregateGpu() override { rzk_scene_destroy(scene_); }
     AggregateGpu(const AggregateGpu&) = delete;
