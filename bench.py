@@ -767,8 +767,23 @@ def setup_c5(args, ctx):
     from renzoku_amd import scenes, vplgen
     t0 = time.perf_counter()
     sc = scenes.instanced_meshes(1280, 720, seed=2013, target_tris=args.tris5 or 2_000_000)
+    t1 = time.perf_counter()
     scene = rz.Scene(ctx, sc)
+    host_ms = (time.perf_counter() - t1) * 1e3
     log(f"[bench] C5 scene: {sc.ntri} triangles ({time.perf_counter() - t0:.1f} s)")
+    # the same scene with its trees built on the device (second build timed: the first loads the kernels)
+    dev_ms = []
+    for _ in range(2):
+        t1 = time.perf_counter()
+        ds = rz.Scene(ctx, sc, build="device")
+        ctx.synchronize()
+        dev_ms.append((time.perf_counter() - t1) * 1e3)
+    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+               for a, b in zip(scene.export_trees()[:2], ds.export_trees()[:2]))
+    del ds
+    build = {"host_ms": round(host_ms, 1), "device_ms": round(dev_ms[-1], 1),
+             "speedup": round(host_ms / dev_ms[-1], 1), "binary_tree_bit_equal": bool(same)}
+    log(f"[bench] C5 scene build: host {host_ms:.0f} ms, device {dev_ms[-1]:.1f} ms, bit-equal {same}")
     vpl, vmat, n_emitted = vplgen.device_vpls(scene, args.vpls5 or 200_000, seed=2013)
     vp = rz.Vpls(ctx, vpl, vmat)
     ls = rz.LightSlice(ctx, scene, vp, width=1280, height=720, tile=32, num_clusters=1000, num_neighbor_slices=16,
@@ -776,7 +791,7 @@ def setup_c5(args, ctx):
     ls.initialize(sc.camera)
     clusters = ls.clusters()
     log(f"[bench] C5: {len(vmat)} VPLs, {ls.info().num_clusters} resident clusters")
-    return dict(sc=sc, scene=scene, vp=vp, ls=ls, clusters=clusters, nvpl=len(vmat))
+    return dict(sc=sc, scene=scene, vp=vp, ls=ls, clusters=clusters, nvpl=len(vmat), build=build)
 
 
 def measure_served(args, ctx, c5, nreq, rate, seed):
@@ -1295,6 +1310,7 @@ def run_default(args):
                 leg = {k: r[k] for k in ("value", "unit", "latency_ms", "plain_requests_per_s", "plain_latency_ms",
                                          "speedup_vs_plain", "paths_per_s", "accuracy")}
                 leg["workload"] = r["config"]["workload"]
+                leg["scene_build"] = c5["build"]
                 if c3 is not None and "cpu_paths_per_s" in c3:
                     leg["cpu_baseline"] = {
                         "value": round(args.requests / (r["_paths"] / c3["cpu_paths_per_s"]), 4),
