@@ -519,7 +519,7 @@ int rzk_render_guided_rays(rzk_ctx *ctx, const rzk_guide *guide, const rzk_pt_pa
 typedef struct rzk_server_params {
     int64_t memory_budget;          /* bytes of device memory for admitted requests (guide tables,
                                        per-sample radiance, image) */
-    int32_t step_iterations;        /* engine step between looks at finished requests (1..8): wavefront
+    int32_t step_iterations;        /* engine step between looks at finished requests (1..32): wavefront
                                        iterations, or work items per wave of the grid for the
                                        persistent kernel */
     rzk_batch_config launch;        /* engine (0: the persistent path kernel, items of 64 pixels x one

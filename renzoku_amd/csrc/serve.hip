@@ -44,6 +44,7 @@ using namespace rzk;
 namespace {
 
 constexpr int64_t kJobRing = 1 << 14;      // jobs open at once (a power of two)
+constexpr int kMaxStepIterations = 32;    // work items per wave (or wavefront iterations) per step
 
 // first-fit allocator over one device block: the memory budget
 struct Arena {
@@ -316,7 +317,7 @@ struct rzk_server {
     }
 
     void loop() {
-        const int iters = std::max(1, std::min(8, prm.step_iterations));
+        const int iters = std::max(1, std::min(kMaxStepIterations, prm.step_iterations));
         while (true) {
             {
                 std::unique_lock<std::mutex> lk(mu);
@@ -506,7 +507,7 @@ int rzk_server_create(int device, const rzk_scene* scene, const rzk_clusters* cl
             s->block = c.block;
             s->grid = std::max(1, s->ctx->num_cus) * c.blocks_per_cu;
             const int64_t threads = (int64_t)s->grid * s->block;
-            s->step_items = (int64_t)std::max(1, std::min(8, prm->step_iterations)) * (threads / 64);
+            s->step_items = (int64_t)std::max(1, std::min(kMaxStepIterations, prm->step_iterations)) * (threads / 64);
             s->sv = scene_view(scene);
             const int64_t extra = 2 * (int64_t)s->sv.stack4 - kPtLdsStack;   // closest_hit4 pairs beyond LDS
             if (extra > 0) {
