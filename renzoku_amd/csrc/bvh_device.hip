@@ -134,6 +134,21 @@ __global__ void __launch_bounds__(kBB) k_sorted_tris(const float* __restrict__ p
     tris[i] = t;
 }
 
+// per original triangle: geometric normal (triangle.cc:17-18) with the surface id's
+// bits, and the tangent (the host build's arithmetic, so the same bits)
+__global__ void __launch_bounds__(kBB) k_tri_frames(const float* __restrict__ pos, const float* __restrict__ tan,
+                                                    const int* __restrict__ tri_surface, int n,
+                                                    float4* __restrict__ tn, float4* __restrict__ tt) {
+    int k = blockIdx.x * kBB + threadIdx.x;
+    if (k >= n) return;
+    const float* p = pos + 9 * (int64_t)k;
+    v3 p0 = ld3(p), p1 = ld3(p + 3), p2 = ld3(p + 6);
+    v3 nn = unit(cross(p0 - p1, p0 - p2));
+    tn[k] = make_float4(nn.x, nn.y, nn.z, __int_as_float(tri_surface[k]));
+    const float* t = tan + 3 * (int64_t)k;
+    tt[k] = make_float4(t[0], t[1], t[2], 0.f);
+}
+
 __device__ int clz32(uint32_t v) { return v ? __clz((int)v) : 32; }
 
 // last index i in [first, last) with clz(code[first] ^ code[i]) above the
@@ -291,7 +306,8 @@ __global__ void __launch_bounds__(kBB) k_collapse_level(Item4* items, int off, i
 
 using namespace rzk;
 
-void bvh_build_device(rzk_ctx* ctx, int n, const float* pos, rzk_scene* s) {
+void bvh_build_device(rzk_ctx* ctx, int n, const float* pos, const float* tangents, const int* tri_surface,
+                      rzk_scene* s) {
     RZK_CHECK(n >= 2, RZK_E_INVALID, "device build needs at least two triangles");
     hipStream_t st = ctx->stream;
     BvhHost& out = s->bvh;
@@ -308,6 +324,18 @@ void bvh_build_device(rzk_ctx* ctx, int n, const float* pos, rzk_scene* s) {
     k_tri_boxes<<<nb, kBB, 0, st>>>(dpos.p, n, tbox.p, part.p);
     k_scene_box<<<1, kBB, 0, st>>>(part.p, (int)nb, sbox.p);
     RZK_HIP(hipGetLastError());
+    {
+        Dev<float> dtan(3 * (size_t)n);
+        Dev<int> dts(n);
+        Dev<float4> tn(n), tt(n);
+        RZK_HIP(hipMemcpyAsync(dtan.p, tangents, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, st));
+        RZK_HIP(hipMemcpyAsync(dts.p, tri_surface, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, st));
+        k_tri_frames<<<nb, kBB, 0, st>>>(dpos.p, dtan.p, dts.p, n, tn.p, tt.p);
+        RZK_HIP(hipGetLastError());
+        RZK_HIP(hipStreamSynchronize(st));   // the host arrays may go before the copies ran
+        s->d_tri_n = tn.release();
+        s->d_tri_tan = tt.release();
+    }
 
     // K1-K2: Morton codes, stable sort (equal codes keep triangle order)
     Dev<uint32_t> code(n), code_s(n);

@@ -212,7 +212,7 @@ int rzk_scene_create_ex(rzk_ctx* ctx, const rzk_scene_desc* d, int flags, rzk_sc
         s->ctx = ctx;
         int n = d->num_triangles;
         if ((flags & RZK_SCENE_BUILD_DEVICE) && n >= 2)
-            bvh_build_device(ctx, n, d->positions, s.get());
+            bvh_build_device(ctx, n, d->positions, d->tangents, d->tri_surface, s.get());
         else
             bvh_build_host(n, d->positions, s->bvh);
         s->nsurf = d->num_surfaces;
@@ -226,8 +226,10 @@ int rzk_scene_create_ex(rzk_ctx* ctx, const rzk_scene_desc* d, int flags, rzk_sc
         if (d->num_lights > 0 && d->light_radiance)
             s->h_light_radiance.assign(d->light_radiance, d->light_radiance + 3 * d->num_lights);
         // per original triangle: geometric normal (triangle.cc:17-18) + surface id, tangent
-        std::vector<float4> tn(n > 0 ? n : 1), tt(n > 0 ? n : 1);
-        for (int k = 0; k < n; ++k) {
+        // (a device build computes them on the device, with the same arithmetic)
+        const bool frames_on_host = !s->bvh.device_built;
+        std::vector<float4> tn(frames_on_host && n > 0 ? n : 1), tt(frames_on_host && n > 0 ? n : 1);
+        for (int k = 0; frames_on_host && k < n; ++k) {
             const float* p = d->positions + 9 * k;
             v3 p0 = ld3(p), p1 = ld3(p + 3), p2 = ld3(p + 6);
             v3 nn = unit(cross(p0 - p1, p0 - p2));
@@ -251,8 +253,10 @@ int rzk_scene_create_ex(rzk_ctx* ctx, const rzk_scene_desc* d, int flags, rzk_sc
             up((void**)&s->d_tris, s->bvh.tris.data(), sizeof(TriGpu) * s->bvh.tris.size());
             up((void**)&s->d_tris4, s->bvh.tris4.data(), sizeof(TriGpu) * s->bvh.tris4.size());
         }
-        up((void**)&s->d_tri_n, tn.data(), sizeof(float4) * n);
-        up((void**)&s->d_tri_tan, tt.data(), sizeof(float4) * n);
+        if (frames_on_host) {
+            up((void**)&s->d_tri_n, tn.data(), sizeof(float4) * n);
+            up((void**)&s->d_tri_tan, tt.data(), sizeof(float4) * n);
+        }
         up((void**)&s->d_surf, surf.data(), sizeof(int4) * s->nsurf);
         up((void**)&s->d_mat, s->h_mat.empty() ? nullptr : s->h_mat.data(), sizeof(float) * s->h_mat.size());
         // light tables: emitting triangles grouped per light in surface order;

@@ -102,8 +102,10 @@ struct rzk_scene {
 // Builds the scene's two trees on the device from host positions (bvh_device.hip):
 // the Morton binary tree node-for-node as bvh_build_host lays it out, and a
 // 4-wide tree collapsed from it.  Fills s->bvh's scalars and sorted arrays and
-// s->d_nodes, d_nodes4, d_tris, d_tris4.  Needs ntri >= 2.
-void bvh_build_device(rzk_ctx* ctx, int ntri, const float* pos, rzk_scene* s);
+// s->d_nodes, d_nodes4, d_tris, d_tris4, and the per-triangle frames d_tri_n,
+// d_tri_tan.  Needs ntri >= 2.
+void bvh_build_device(rzk_ctx* ctx, int ntri, const float* pos, const float* tangents, const int* tri_surface,
+                      rzk_scene* s);
 
 struct rzk_vpls {
     rzk_ctx* ctx = nullptr;

@@ -6,7 +6,8 @@ Bars: the binary Morton tree (sorted order, codes, node records with their
 child boxes and refs, triangle records, depth, scene box) bit-equal to the host
 build's; the 4-wide tree a valid collapse of it (every triangle once, every
 slot box the exact union of what lies below, preorder layout); every walk on
-the binary tree (closest hits and their visit counts) and every
+the binary tree (closest hits, their visit counts, the receivers built from
+their triangles' frames) and every
 any-hit, segment and visibility answer bit-equal between the two builds, so
 the VPL render's image is too.  Scenes: the Cornell box, the room, a soup whose
 second half repeats the first (equal Morton codes: midpoint splits, exact
@@ -128,7 +129,11 @@ def random_rays(sc, n, seed):
 def test_walks_bit_equal(built, name):
     sc, hs, ds = built[name]
     rays = random_rays(sc, 8192, 21)
-    for a, b in zip(hs.trace_closest(rays), ds.trace_closest(rays)):
+    hits = hs.trace_closest(rays)
+    for a, b in zip(hits, ds.trace_closest(rays)):
+        assert np.array_equal(bits(a), bits(b))
+    # receivers read the per-triangle frames (normal, surface, tangent) the device build computes itself
+    for a, b in zip(hs.fill_receivers(rays, *hits), ds.fill_receivers(rays, *hits)):
         assert np.array_equal(bits(a), bits(b))
     for kind in (0, 1):
         assert np.array_equal(hs.trace_stats(rays, kind), ds.trace_stats(rays, kind))
